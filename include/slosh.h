@@ -1,0 +1,235 @@
+/*
+ * slosh.h -- C ABI of the MI355X per-step liquid solver (FLIP/APIC on a sparse 8^3-brick MAC grid).
+ *
+ * This header is the drop-in boundary that replaces the reference's render-thread proxy
+ * (SloshDemo/Plugins/Slosh/Source/SloshCore/Public/SloshProxy.h:23-147) and its plain-data contract
+ * (SloshDemo/Plugins/Slosh/Source/SloshCore/Public/SloshTypes.h:10-231). A UE-side shim re-implements
+ * FSloshProxy over these calls; the solver kernels it drives are hand-written gfx950 HIP.
+ *
+ * ABI rules
+ *   - Plain C: POD structs with fixed sizes (checked by static asserts in the library), plain pointers and
+ *     byte counts, no exceptions across the boundary; every call returns a slosh_status.
+ *   - One host thread per context. All device work of a context is ordered on one internal HIP stream;
+ *     every call that returns host data is synchronous.
+ *   - Caller-owned host buffers. The library owns all device memory of a context.
+ *   - Settings outside their documented range are rejected with SLOSH_EINVAL (the reference clamped them
+ *     silently, SloshProxy.cpp:344-347 and :411-427).
+ *
+ * Sim space: metres, origin at the domain's min corner. Cell (i,j,k) spans [i h,(i+1) h] x ... ; the outer cell
+ * layer is solid wall. Any axis may be "up": gravity is a free vector passed to every step.
+ */
+#ifndef SLOSH_H_
+#define SLOSH_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLOSH_ABI_VERSION 1u
+
+typedef struct slosh_ctx slosh_ctx; /* opaque; owns all device memory of one simulation domain */
+
+typedef enum slosh_status {
+    SLOSH_OK = 0,
+    SLOSH_EINVAL = 1,    /* bad argument or setting out of range */
+    SLOSH_ENOMEM = 2,    /* host or device allocation failed */
+    SLOSH_EDEVICE = 3,   /* HIP runtime error, or no usable gfx950 device */
+    SLOSH_ECAPACITY = 4, /* a request exceeds a fixed capacity (particle capacity, collider count, buffer size) */
+    SLOSH_ESTATE = 5     /* call not valid in the current state */
+} slosh_status;
+
+/* Pressure solvers. Both solve the same operator (diag_c + 1e-4) p_c - sum_{fluid n} p_n = -div_c. */
+enum {
+    SLOSH_SOLVER_MGPCG = 0, /* multigrid-preconditioned CG, FP64 dots, to ||r||/||b|| <= tolerance */
+    SLOSH_SOLVER_RBSOR = 1, /* red-black SOR with sor_weight, to tolerance or max_iterations */
+    SLOSH_SOLVER_NONE = 2   /* keep the (warm-started) pressure as is; used by parity tests of the other kernels */
+};
+
+/* Solver settings: FSloshSolverSettings (SloshTypes.h:113-160), restricted to what the per-step path reads. */
+typedef struct slosh_settings {
+    int32_t grid_dim[3];          /* cells per axis, 4..4096, each a multiple of nothing in particular */
+    float cell_size;              /* h in metres, > 0 */
+    uint32_t particle_capacity;   /* maximum particle slots */
+    float sparse_storage_budget;  /* brick pool capacity as a fraction of all 8^3 bricks, (0, 1] */
+    int32_t pressure_solver;      /* SLOSH_SOLVER_* */
+    float tolerance;              /* relative residual target ||r||_2 / ||b||_2 (e.g. 1e-6) */
+    int32_t max_iterations;       /* MGPCG iteration cap / RB-SOR iteration cap */
+    float sor_weight;             /* RB-SOR relaxation, [0.5, 1.95] */
+    int32_t ghost_fluid;          /* 1: free-surface p = 0 at the 0.5 density iso level (theta weights) */
+    int32_t extrapolation_layers; /* velocity extrapolation layers, 0..8 */
+    float boundary_friction;      /* 0 free slip .. 1 no slip */
+    float max_speed;              /* m/s; speeds above are counted (reference stats 9..13), not clamped */
+    float particle_radius_cells;  /* collider contact radius in cells */
+    float density_correction;     /* velocity-based density correction strength (0 in the parity profile) */
+    float density_deadband;
+    uint32_t flags;               /* SLOSH_FLAG_* */
+    int32_t reserved[6];          /* must be zero */
+} slosh_settings;
+
+#define SLOSH_FLAG_NONE 0u
+
+/* Emission request: FSloshEmitRequest (SloshTypes.h:169-181). shape 0 = FillCells (8 jittered particles per cell). */
+enum { SLOSH_EMIT_FILL_CELLS = 0, SLOSH_EMIT_SPHERE = 1, SLOSH_EMIT_BOX = 2 };
+typedef struct slosh_emit_request {
+    uint32_t shape;
+    uint32_t seed;         /* EmitSeed of the jitter hash; the reference increments its own from 1 (SloshProxy.cpp:529) */
+    float center[3];
+    float extent[3];
+    float velocity[3];
+    float rotation[4];     /* quaternion x,y,z,w */
+    float velocity_spread; /* radians */
+    float speed_jitter;
+    int32_t cell_min[3];   /* FillCells: first cell; clamped by the caller as QueueFillLocalBox does */
+    int32_t cell_dims[3];  /* FillCells: cells per axis */
+    uint32_t count;        /* sphere/box: particles */
+    uint32_t reserved;
+} slosh_emit_request;
+
+/* Analytic collider: FSloshColliderGPU (SloshTypes.h:32-44), 80 bytes. type 0 sphere, 1 box, 2 capsule. */
+typedef struct slosh_collider {
+    float center[3];
+    uint32_t type;
+    float rotation[4];
+    float extent[3];
+    uint32_t flags;
+    float linear_velocity[3];
+    float pad0;
+    float angular_velocity[3];
+    float pad1;
+} slosh_collider;
+#define SLOSH_MAX_COLLIDERS 64u
+
+/* Particle records returned by slosh_read_particles. */
+enum { SLOSH_LAYOUT_F32_64 = 0, SLOSH_LAYOUT_REF_48 = 1 };
+typedef struct slosh_particle { /* 64 B, FP32 APIC matrix */
+    float pos[3];
+    float age; /* < 0 marks a dead slot */
+    float vel[3];
+    float c[9]; /* rows Cx, Cy, Cz */
+} slosh_particle;
+typedef struct slosh_particle_ref48 { /* FSloshParticleGPU (SloshTypes.h:84-91): C as 9 halves in 5 uints */
+    float pos[3];
+    float age;
+    float vel[3];
+    uint32_t c[5];
+} slosh_particle_ref48;
+
+/* Grid fields for slosh_read_grid / slosh_write_grid. */
+enum {
+    SLOSH_FIELD_CELL_FLAGS = 0,  /* uint32: bits 0-1 type (0 air, 1 fluid, 2 solid), bit 2 dense, bits 8-15 collider+1 */
+    SLOSH_FIELD_PRESSURE = 1,    /* float, scaled pressure p_hat = p dt / (rho h) */
+    SLOSH_FIELD_DIVERGENCE = 2,  /* float, sum of face velocity differences (not divided by h) */
+    SLOSH_FIELD_CELL_DIAG = 3,   /* float, ghost-fluid operator diagonal (without the 1e-4) */
+    SLOSH_FIELD_VEL = 4,         /* float[3 planes]: face velocities U, V, W (each cell owns its min faces) */
+    SLOSH_FIELD_FACE_FLAGS = 5,  /* uint32[3 planes] */
+    SLOSH_FIELD_FACE_WEIGHT = 6, /* float[3 planes] */
+    SLOSH_FIELD_ACCUM = 7,       /* int32[6 planes]: P2G momentum U,V,W then weight U,V,W (fixed point 2^20); defined
+                                    between the P2G and normalise stages of a step */
+    SLOSH_FIELD_BRICK_TABLE = 9, /* uint32 per 8^3 brick: pool slot, 0 = none (order argument ignored) */
+    SLOSH_FIELD_PRESSURE_F64 = 10 /* double: the solver's FP64 pressure */
+};
+enum { SLOSH_ORDER_DENSE = 0 /* x fastest over the whole grid, unallocated cells read 0 */ };
+
+/* Diagnostics of the last step: FSloshResults (SloshTypes.h:208-231) plus solver convergence data. */
+typedef struct slosh_results {
+    uint64_t sim_frame;
+    uint32_t live_particles;
+    uint32_t fluid_cells;
+    uint32_t active_bricks;        /* allocated bricks after this step's allocation */
+    uint32_t pool_overflow_bricks; /* wanted bricks that found the pool full */
+    float max_particle_speed;
+    float max_divergence;          /* 1/s, post projection, over fluid cells */
+    float mean_divergence;         /* 1/s */
+    uint32_t clamped_particles;    /* particles above max_speed */
+    int32_t solver_iterations;
+    int32_t converged;             /* 1 when ||r||/||b|| <= tolerance */
+    double residual_rel;           /* final ||r||_2 / ||b||_2 of the solver's recurrence residual */
+    double rhs_norm;               /* ||b||_2 */
+    float step_ms;                 /* device time of the step (HIP events) */
+    float solve_ms;                /* device time of the pressure solve */
+    int32_t fallback_used;         /* reserved for the flexible-CG preconditioner fallback; 0 */
+    int32_t reserved[5];
+} slosh_results;
+
+/* ---- lifecycle --------------------------------------------------------------------------------------------- */
+/* Fills *s with the parity-profile defaults (APIC, ghost fluid on, density correction 0, MGPCG 1e-6). */
+void slosh_default_settings(slosh_settings *s);
+/* = FSloshProxy::Allocate (SloshProxy.cpp:185-306). Creates all device state on the current HIP device. */
+slosh_status slosh_create(const slosh_settings *s, slosh_ctx **out);
+void slosh_destroy(slosh_ctx *ctx);
+/* = FSloshFrameInput::bReset: drops every particle and clears all grid state. */
+slosh_status slosh_reset(slosh_ctx *ctx);
+
+/* ---- per-frame inputs -------------------------------------------------------------------------------------- */
+/* = the emit passes of Step_RenderThread (SloshProxy.cpp:502-540; Slosh.usf:272-358), run at once.
+ * Returns SLOSH_ECAPACITY (emitting nothing) if the request does not fit particle_capacity. */
+slosh_status slosh_emit(slosh_ctx *ctx, const slosh_emit_request *r);
+/* = FSloshFrameInput::Colliders (SloshTypes.h:187); n <= SLOSH_MAX_COLLIDERS; kept until replaced. */
+slosh_status slosh_set_colliders(slosh_ctx *ctx, const slosh_collider *c, uint32_t n);
+
+/* ---- the hot path -------------------------------------------------------------------------------------------- */
+/* One substep of Step_RenderThread (SloshProxy.cpp:564-713) with a fixed dt and the given gravity (m/s^2):
+ * sort -> brick pool -> classify -> P2G -> normalise -> divergence -> pressure -> project -> extrapolate -> G2P. */
+slosh_status slosh_step(slosh_ctx *ctx, float dt, const float gravity[3]);
+
+/* ---- readback ------------------------------------------------------------------------------------------------ */
+/* Copies the live particles (sorted order of the last step) into dst; *count receives the number written. */
+slosh_status slosh_read_particles(slosh_ctx *ctx, void *dst, uint64_t dst_bytes, uint32_t *count, int32_t layout);
+/* Copies a grid field in the given order. Per-face fields hold 3 planes (U, V, W) of grid_dim cells each. */
+slosh_status slosh_read_grid(slosh_ctx *ctx, int32_t field, void *dst, uint64_t dst_bytes, int32_t order);
+/* = FSloshProxy::GetResults (SloshProxy.cpp:177-183), synchronous. */
+slosh_status slosh_get_results(slosh_ctx *ctx, slosh_results *out);
+/* Zero-copy access for a renderer: packs the particles into a device buffer of the given layout and returns it
+ * (valid until the next call that changes particles). = GetParticleBuffer_RenderThread (SloshProxy.h:41). */
+slosh_status slosh_device_particles(slosh_ctx *ctx, int32_t layout, void **dev_ptr, uint32_t *count);
+
+/* ---- state injection (checkpoint / resume, tests) ------------------------------------------------------------ */
+/* Replaces all particles with n records of layout SLOSH_LAYOUT_F32_64 (no reference counterpart: the reference
+ * only emits; this is the resume half of read_particles). The brick touch set is rebuilt from the positions. */
+slosh_status slosh_write_particles(slosh_ctx *ctx, const void *src, uint32_t n);
+/* Writes a per-cell float field (SLOSH_FIELD_PRESSURE only) in dense order; cells of unallocated bricks are
+ * ignored. Used to resume a warm start. */
+slosh_status slosh_write_grid(slosh_ctx *ctx, int32_t field, const void *src, uint64_t src_bytes, int32_t order);
+
+/* ---- test hook ------------------------------------------------------------------------------------------------ */
+/* Runs slosh_step but stops after the given stage so intermediate fields can be read back. */
+enum {
+    SLOSH_STAGE_SORT = 1,
+    SLOSH_STAGE_POOL = 2,
+    SLOSH_STAGE_P2G = 3,        /* classification + P2G + cell marking */
+    SLOSH_STAGE_NORMALIZE = 4,
+    SLOSH_STAGE_DIVERGENCE = 5,
+    SLOSH_STAGE_SOLVE = 6,
+    SLOSH_STAGE_PROJECT = 7,
+    SLOSH_STAGE_EXTRAPOLATE = 8,
+    SLOSH_STAGE_ALL = 9
+};
+slosh_status slosh_step_until(slosh_ctx *ctx, float dt, const float gravity[3], int32_t stop_after);
+
+/* ---- profiling (no reference counterpart; the reference's equivalent is its RDG event scopes and GPU stat,
+ *      SloshProxy.cpp:12, :342, :566) ---------------------------------------------------------------------------- */
+typedef struct slosh_kernel_time {
+    char name[32];      /* kernel name as a kernel trace shows it (k_g2p, k_apply_a, ...) */
+    uint32_t launches;  /* timed launches since the last reset */
+    uint32_t pad;
+    double total_ms;    /* summed HIP-event time of those launches */
+} slosh_kernel_time;
+/* Selects the kernels to time with HIP events on the context's stream: bit i = kernel i of slosh_kernel_times'
+ * order, ~0 = all, 0 = none (the default). */
+slosh_status slosh_set_kernel_timing(slosh_ctx *ctx, uint64_t mask);
+/* Writes up to cap entries (one per kernel, fixed order) and sets *n to the kernel count; out == NULL resets
+ * the totals. */
+slosh_status slosh_kernel_times(slosh_ctx *ctx, slosh_kernel_time *out, uint32_t cap, uint32_t *n);
+
+/* Library identification: ABI version and the offload target the kernels were built for. */
+uint32_t slosh_abi_version(void);
+const char *slosh_build_info(void);
+const char *slosh_status_string(slosh_status s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SLOSH_H_ */

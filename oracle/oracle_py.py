@@ -1,0 +1,170 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so). TEST INFRASTRUCTURE ONLY.
+
+The oracle is the checker of the HIP solver: the tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` load it; the product library and its Python wrapper never do.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from slosh_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+CELL_COUNT_FIELD = 8  # oracle-only field: particles per cell after P2G
+
+
+def build():
+    """Compiles the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    lib.osim_create.restype = vp
+    lib.osim_create.argtypes = [C.POINTER(abi.Settings)]
+    lib.osim_destroy.argtypes = [vp]
+    lib.osim_destroy.restype = None
+    lib.osim_emit.argtypes = [vp, C.POINTER(abi.EmitRequest)]
+    lib.osim_set_colliders.argtypes = [vp, C.POINTER(abi.Collider), C.c_uint32]
+    lib.osim_write_particles.argtypes = [vp, vp, C.c_uint32]
+    lib.osim_write_pressure.argtypes = [vp, vp]
+    lib.osim_step_until.argtypes = [vp, C.c_float, C.POINTER(C.c_float), C.c_int]
+    lib.osim_num_particles.argtypes = [vp]
+    lib.osim_num_particles.restype = C.c_uint32
+    lib.osim_read_particles.argtypes = [vp, vp, C.c_uint32]
+    lib.osim_read_grid.argtypes = [vp, C.c_int, vp, C.c_uint64]
+    lib.osim_get_results.argtypes = [vp, C.POINTER(abi.Results)]
+    lib.osim_solve_fp64.argtypes = [vp, C.c_double, C.c_int, vp, C.POINTER(C.c_double)]
+    lib.osim_true_residual.argtypes = [vp, vp]
+    lib.osim_true_residual.restype = C.c_double
+    lib.osim_true_residual_d.argtypes = [vp, vp]
+    lib.osim_true_residual_d.restype = C.c_double
+    lib.oracle_pcg_hash.argtypes = [C.c_uint32]
+    lib.oracle_pcg_hash.restype = C.c_uint32
+    lib.oracle_hash_to_unit.argtypes = [C.c_uint32]
+    lib.oracle_hash_to_unit.restype = C.c_float
+    lib.oracle_collider_sdf.argtypes = [C.POINTER(abi.Collider), C.POINTER(C.c_float)]
+    lib.oracle_collider_sdf.restype = C.c_float
+    lib.oracle_ghost_theta.argtypes = [C.c_float, C.c_float]
+    lib.oracle_ghost_theta.restype = C.c_float
+    _lib = lib
+    return lib
+
+
+def _check(status, what):
+    if status != 0:
+        raise RuntimeError(f"oracle {what} failed: {abi.STATUS_NAMES.get(status, status)}")
+
+
+class OracleSim:
+    """CPU restatement of one solver domain, with the same call surface as slosh_amd.Sim."""
+
+    def __init__(self, settings):
+        self.lib = load()
+        self.settings = settings
+        self.dims = tuple(int(settings.grid_dim[a]) for a in range(3))
+        self.h = C.c_void_p(self.lib.osim_create(C.byref(settings)))
+        if not self.h.value:
+            raise RuntimeError("osim_create failed")
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.lib.osim_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def emit(self, req):
+        _check(self.lib.osim_emit(self.h, C.byref(req)), "emit")
+
+    def set_colliders(self, colliders):
+        arr = (abi.Collider * max(len(colliders), 1))(*colliders)
+        _check(self.lib.osim_set_colliders(self.h, arr, len(colliders)), "set_colliders")
+
+    def write_particles(self, parts):
+        parts = np.ascontiguousarray(parts, dtype=abi.PARTICLE_DTYPE)
+        _check(self.lib.osim_write_particles(self.h, parts.ctypes.data, len(parts)), "write_particles")
+
+    def write_pressure(self, dense):
+        dense = np.ascontiguousarray(dense, dtype=np.float32)
+        _check(self.lib.osim_write_pressure(self.h, dense.ctypes.data), "write_pressure")
+
+    def step(self, dt, gravity, stop_after=abi.STAGE_ALL):
+        g = (C.c_float * 3)(*[float(x) for x in gravity])
+        _check(self.lib.osim_step_until(self.h, C.c_float(dt), g, int(stop_after)), "step")
+
+    def num_particles(self):
+        return int(self.lib.osim_num_particles(self.h))
+
+    def read_particles(self):
+        n = self.num_particles()
+        out = np.zeros(n, dtype=abi.PARTICLE_DTYPE)
+        self.lib.osim_read_particles(self.h, out.ctypes.data, n)
+        return out
+
+    def read_grid(self, field):
+        nx, ny, nz = self.dims
+        if field == abi.FIELD_BRICK_TABLE:
+            nb = ((nx + 7) // 8) * ((ny + 7) // 8) * ((nz + 7) // 8)
+            out = np.zeros(nb, dtype=np.uint32)
+            _check(self.lib.osim_read_grid(self.h, field, out.ctypes.data, out.nbytes), "read_grid")
+            return out
+        if field == CELL_COUNT_FIELD:
+            out = np.zeros((nz, ny, nx), dtype=np.int32)
+            _check(self.lib.osim_read_grid(self.h, field, out.ctypes.data, out.nbytes), "read_grid")
+            return out
+        dt, planes = abi.FIELD_INFO[field]
+        out = np.zeros((planes, nz, ny, nx), dtype=dt)
+        _check(self.lib.osim_read_grid(self.h, field, out.ctypes.data, out.nbytes), "read_grid")
+        return out if planes > 1 else out[0]
+
+    def results(self):
+        r = abi.Results()
+        self.lib.osim_get_results(self.h, C.byref(r))
+        return r
+
+    def solve_fp64(self, rel_tol=1e-13, max_iter=500):
+        nx, ny, nz = self.dims
+        out = np.zeros((nz, ny, nx), dtype=np.float64)
+        rel = C.c_double(0.0)
+        iters = self.lib.osim_solve_fp64(self.h, rel_tol, max_iter, out.ctypes.data, C.byref(rel))
+        return out, iters, rel.value
+
+    def true_residual(self, dense_p=None):
+        """FP64 relative residual of dense_p (float32 dense field), or of the solver's own FP64 pressure."""
+        if dense_p is None:
+            return float(self.lib.osim_true_residual(self.h, None))
+        if np.asarray(dense_p).dtype == np.float64:
+            dense_p = np.ascontiguousarray(dense_p)
+            return float(self.lib.osim_true_residual_d(self.h, dense_p.ctypes.data))
+        dense_p = np.ascontiguousarray(dense_p, dtype=np.float32)
+        return float(self.lib.osim_true_residual(self.h, dense_p.ctypes.data))
+
+
+def pcg_hash(v):
+    return int(load().oracle_pcg_hash(int(v) & 0xFFFFFFFF))
+
+
+def ghost_theta(rf, ra):
+    return float(load().oracle_ghost_theta(rf, ra))
+
+
+def collider_sdf(col, p):
+    pp = (C.c_float * 3)(*[float(x) for x in p])
+    return float(load().oracle_collider_sdf(C.byref(col), pp))

@@ -1,0 +1,186 @@
+"""slosh_amd -- MI355X-native per-step liquid solver (FLIP/APIC on sparse 8^3 bricks) behind a C ABI.
+
+The solver is ``libslosh.so`` (hand-written gfx950 HIP kernels + a C++ host library, ``include/slosh.h``). This
+module is a thin ctypes binding of that ABI for tests and benchmarks; it never falls back to any CPU code: if the
+library is missing or the device is not a gfx950, ``Sim`` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libslosh.so")
+
+_lib = None
+
+
+class SloshError(RuntimeError):
+    def __init__(self, what, status):
+        super().__init__(f"{what}: {abi.STATUS_NAMES.get(status, status)}")
+        self.status = status
+
+
+def load_library(path=LIB_PATH):
+    """Loads libslosh.so and declares its C signatures. Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    st = C.c_int
+    lib.slosh_abi_version.restype = C.c_uint32
+    lib.slosh_build_info.restype = C.c_char_p
+    lib.slosh_status_string.restype = C.c_char_p
+    lib.slosh_status_string.argtypes = [st]
+    lib.slosh_default_settings.argtypes = [C.POINTER(abi.Settings)]
+    lib.slosh_default_settings.restype = None
+    lib.slosh_create.argtypes = [C.POINTER(abi.Settings), C.POINTER(vp)]
+    lib.slosh_create.restype = st
+    lib.slosh_destroy.argtypes = [vp]
+    lib.slosh_destroy.restype = None
+    lib.slosh_reset.argtypes = [vp]
+    lib.slosh_reset.restype = st
+    lib.slosh_emit.argtypes = [vp, C.POINTER(abi.EmitRequest)]
+    lib.slosh_emit.restype = st
+    lib.slosh_set_colliders.argtypes = [vp, C.POINTER(abi.Collider), C.c_uint32]
+    lib.slosh_set_colliders.restype = st
+    lib.slosh_step.argtypes = [vp, C.c_float, C.POINTER(C.c_float)]
+    lib.slosh_step.restype = st
+    lib.slosh_step_until.argtypes = [vp, C.c_float, C.POINTER(C.c_float), C.c_int32]
+    lib.slosh_step_until.restype = st
+    lib.slosh_read_particles.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint32), C.c_int32]
+    lib.slosh_read_particles.restype = st
+    lib.slosh_read_grid.argtypes = [vp, C.c_int32, vp, C.c_uint64, C.c_int32]
+    lib.slosh_read_grid.restype = st
+    lib.slosh_get_results.argtypes = [vp, C.POINTER(abi.Results)]
+    lib.slosh_get_results.restype = st
+    lib.slosh_device_particles.argtypes = [vp, C.c_int32, C.POINTER(vp), C.POINTER(C.c_uint32)]
+    lib.slosh_device_particles.restype = st
+    lib.slosh_write_particles.argtypes = [vp, vp, C.c_uint32]
+    lib.slosh_write_particles.restype = st
+    lib.slosh_write_grid.argtypes = [vp, C.c_int32, vp, C.c_uint64, C.c_int32]
+    lib.slosh_write_grid.restype = st
+    lib.slosh_set_kernel_timing.argtypes = [vp, C.c_uint64]
+    lib.slosh_set_kernel_timing.restype = st
+    lib.slosh_kernel_times.argtypes = [vp, C.POINTER(abi.KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]
+    lib.slosh_kernel_times.restype = st
+    _lib = lib
+    return lib
+
+
+def _check(status, what):
+    if status != abi.SLOSH_OK:
+        raise SloshError(what, status)
+
+
+def default_settings():
+    s = abi.Settings()
+    load_library().slosh_default_settings(C.byref(s))
+    return s
+
+
+class Sim:
+    """One solver domain on the current HIP device (the C ABI's slosh_ctx)."""
+
+    def __init__(self, settings):
+        self.lib = load_library()
+        self.settings = settings
+        self.dims = tuple(int(settings.grid_dim[a]) for a in range(3))
+        h = C.c_void_p()
+        _check(self.lib.slosh_create(C.byref(settings), C.byref(h)), "slosh_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.slosh_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        _check(self.lib.slosh_reset(self.h), "slosh_reset")
+
+    def emit(self, req):
+        _check(self.lib.slosh_emit(self.h, C.byref(req)), "slosh_emit")
+
+    def set_colliders(self, colliders):
+        arr = (abi.Collider * max(len(colliders), 1))(*colliders)
+        _check(self.lib.slosh_set_colliders(self.h, arr, len(colliders)), "slosh_set_colliders")
+
+    def write_particles(self, parts):
+        parts = np.ascontiguousarray(parts, dtype=abi.PARTICLE_DTYPE)
+        _check(self.lib.slosh_write_particles(self.h, parts.ctypes.data, len(parts)), "slosh_write_particles")
+
+    def write_pressure(self, dense):
+        dense = np.ascontiguousarray(dense, dtype=np.float32)
+        _check(self.lib.slosh_write_grid(self.h, abi.FIELD_PRESSURE, dense.ctypes.data, dense.nbytes, abi.ORDER_DENSE),
+               "slosh_write_grid")
+
+    def step(self, dt, gravity, stop_after=abi.STAGE_ALL):
+        g = (C.c_float * 3)(*[float(x) for x in gravity])
+        if stop_after == abi.STAGE_ALL:
+            _check(self.lib.slosh_step(self.h, C.c_float(dt), g), "slosh_step")
+        else:
+            _check(self.lib.slosh_step_until(self.h, C.c_float(dt), g, int(stop_after)), "slosh_step_until")
+
+    def results(self):
+        r = abi.Results()
+        _check(self.lib.slosh_get_results(self.h, C.byref(r)), "slosh_get_results")
+        return r
+
+    def num_particles(self):
+        return int(self.results().live_particles)
+
+    def read_particles(self, layout=abi.LAYOUT_F32_64):
+        n = self.num_particles()
+        dt = abi.PARTICLE_DTYPE if layout == abi.LAYOUT_F32_64 else abi.PARTICLE_REF48_DTYPE
+        out = np.zeros(max(n, 1), dtype=dt)
+        cnt = C.c_uint32(0)
+        _check(self.lib.slosh_read_particles(self.h, out.ctypes.data, out.nbytes, C.byref(cnt), layout),
+               "slosh_read_particles")
+        return out[: cnt.value]
+
+    def read_grid(self, field):
+        nx, ny, nz = self.dims
+        if field == abi.FIELD_BRICK_TABLE:
+            nb = ((nx + 7) // 8) * ((ny + 7) // 8) * ((nz + 7) // 8)
+            out = np.zeros(nb, dtype=np.uint32)
+        else:
+            dt, planes = abi.FIELD_INFO[field]
+            out = np.zeros((planes, nz, ny, nx), dtype=dt)
+        _check(self.lib.slosh_read_grid(self.h, field, out.ctypes.data, out.nbytes, abi.ORDER_DENSE), "slosh_read_grid")
+        if field != abi.FIELD_BRICK_TABLE and out.shape[0] == 1:
+            return out[0]
+        return out
+
+    def set_kernel_timing(self, mask):
+        """Times the selected kernels (bit i = kernel i of kernel_times(); ~0 all, 0 none) with HIP events."""
+        _check(self.lib.slosh_set_kernel_timing(self.h, C.c_uint64(mask & 0xFFFFFFFFFFFFFFFF)), "slosh_set_kernel_timing")
+
+    def kernel_times(self, reset=False):
+        """{kernel name: (launches, total ms)} since the last reset; reset=True clears the totals after reading."""
+        n = C.c_uint32(0)
+        arr = (abi.KernelTime * 64)()
+        _check(self.lib.slosh_kernel_times(self.h, arr, 64, C.byref(n)), "slosh_kernel_times")
+        out = {arr[k].name.decode(): (int(arr[k].launches), float(arr[k].total_ms)) for k in range(n.value)}
+        if reset:
+            _check(self.lib.slosh_kernel_times(self.h, None, 0, C.byref(n)), "slosh_kernel_times")
+        return out
+
+    def kernel_index(self, name):
+        return list(self.kernel_times().keys()).index(name)
+
+    def device_particles(self, layout=abi.LAYOUT_F32_64):
+        ptr = C.c_void_p()
+        cnt = C.c_uint32(0)
+        _check(self.lib.slosh_device_particles(self.h, layout, C.byref(ptr), C.byref(cnt)), "slosh_device_particles")
+        return ptr.value, cnt.value

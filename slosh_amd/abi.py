@@ -1,0 +1,207 @@
+"""ctypes mirror of include/slosh.h (the C ABI of the solver library).
+
+The structures here must match the header byte for byte; ``tests/test_abi.py`` checks their sizes against the
+library's own ``static_assert``s by calling ``slosh_default_settings`` and comparing field values.
+"""
+import ctypes as C
+
+import numpy as np
+
+SLOSH_OK = 0
+SLOSH_EINVAL = 1
+SLOSH_ENOMEM = 2
+SLOSH_EDEVICE = 3
+SLOSH_ECAPACITY = 4
+SLOSH_ESTATE = 5
+
+STATUS_NAMES = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "ECAPACITY", 5: "ESTATE"}
+
+SOLVER_MGPCG = 0
+SOLVER_RBSOR = 1
+SOLVER_NONE = 2
+
+EMIT_FILL_CELLS = 0
+EMIT_SPHERE = 1
+EMIT_BOX = 2
+
+LAYOUT_F32_64 = 0
+LAYOUT_REF_48 = 1
+
+FIELD_CELL_FLAGS = 0
+FIELD_PRESSURE = 1
+FIELD_DIVERGENCE = 2
+FIELD_CELL_DIAG = 3
+FIELD_VEL = 4
+FIELD_FACE_FLAGS = 5
+FIELD_FACE_WEIGHT = 6
+FIELD_ACCUM = 7
+FIELD_BRICK_TABLE = 9
+FIELD_PRESSURE_F64 = 10
+
+# (numpy dtype, planes) of each dense grid field
+FIELD_INFO = {
+    FIELD_CELL_FLAGS: (np.uint32, 1),
+    FIELD_PRESSURE: (np.float32, 1),
+    FIELD_DIVERGENCE: (np.float32, 1),
+    FIELD_CELL_DIAG: (np.float32, 1),
+    FIELD_VEL: (np.float32, 3),
+    FIELD_FACE_FLAGS: (np.uint32, 3),
+    FIELD_FACE_WEIGHT: (np.float32, 3),
+    FIELD_ACCUM: (np.int32, 6),
+    FIELD_PRESSURE_F64: (np.float64, 1),
+}
+
+ORDER_DENSE = 0
+
+STAGE_SORT = 1
+STAGE_POOL = 2
+STAGE_P2G = 3
+STAGE_NORMALIZE = 4
+STAGE_DIVERGENCE = 5
+STAGE_SOLVE = 6
+STAGE_PROJECT = 7
+STAGE_EXTRAPOLATE = 8
+STAGE_ALL = 9
+
+MAX_COLLIDERS = 64
+
+CELL_AIR, CELL_FLUID, CELL_SOLID = 0, 1, 2
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("grid_dim", C.c_int32 * 3),
+        ("cell_size", C.c_float),
+        ("particle_capacity", C.c_uint32),
+        ("sparse_storage_budget", C.c_float),
+        ("pressure_solver", C.c_int32),
+        ("tolerance", C.c_float),
+        ("max_iterations", C.c_int32),
+        ("sor_weight", C.c_float),
+        ("ghost_fluid", C.c_int32),
+        ("extrapolation_layers", C.c_int32),
+        ("boundary_friction", C.c_float),
+        ("max_speed", C.c_float),
+        ("particle_radius_cells", C.c_float),
+        ("density_correction", C.c_float),
+        ("density_deadband", C.c_float),
+        ("flags", C.c_uint32),
+        ("reserved", C.c_int32 * 6),
+    ]
+
+
+class EmitRequest(C.Structure):
+    _fields_ = [
+        ("shape", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("center", C.c_float * 3),
+        ("extent", C.c_float * 3),
+        ("velocity", C.c_float * 3),
+        ("rotation", C.c_float * 4),
+        ("velocity_spread", C.c_float),
+        ("speed_jitter", C.c_float),
+        ("cell_min", C.c_int32 * 3),
+        ("cell_dims", C.c_int32 * 3),
+        ("count", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class Collider(C.Structure):
+    _fields_ = [
+        ("center", C.c_float * 3),
+        ("type", C.c_uint32),
+        ("rotation", C.c_float * 4),
+        ("extent", C.c_float * 3),
+        ("flags", C.c_uint32),
+        ("linear_velocity", C.c_float * 3),
+        ("pad0", C.c_float),
+        ("angular_velocity", C.c_float * 3),
+        ("pad1", C.c_float),
+    ]
+
+
+class Results(C.Structure):
+    _fields_ = [
+        ("sim_frame", C.c_uint64),
+        ("live_particles", C.c_uint32),
+        ("fluid_cells", C.c_uint32),
+        ("active_bricks", C.c_uint32),
+        ("pool_overflow_bricks", C.c_uint32),
+        ("max_particle_speed", C.c_float),
+        ("max_divergence", C.c_float),
+        ("mean_divergence", C.c_float),
+        ("clamped_particles", C.c_uint32),
+        ("solver_iterations", C.c_int32),
+        ("converged", C.c_int32),
+        ("residual_rel", C.c_double),
+        ("rhs_norm", C.c_double),
+        ("step_ms", C.c_float),
+        ("solve_ms", C.c_float),
+        ("fallback_used", C.c_int32),
+        ("reserved", C.c_int32 * 5),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint32), ("pad", C.c_uint32), ("total_ms", C.c_double)]
+
+
+PARTICLE_DTYPE = np.dtype([("pos", np.float32, 3), ("age", np.float32), ("vel", np.float32, 3), ("c", np.float32, 9)])
+PARTICLE_REF48_DTYPE = np.dtype([("pos", np.float32, 3), ("age", np.float32), ("vel", np.float32, 3), ("c", np.uint32, 5)])
+
+assert C.sizeof(Collider) == 80
+assert C.sizeof(Settings) == 96
+assert C.sizeof(EmitRequest) == 100
+assert C.sizeof(Results) == 96
+assert PARTICLE_DTYPE.itemsize == 64 and PARTICLE_REF48_DTYPE.itemsize == 48
+
+
+def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6, max_iterations=200, sor_weight=1.8,
+                  budget=1.0, extrapolation_layers=2, ghost_fluid=1, friction=0.0, max_speed=20.0):
+    """Parity-profile settings (APIC, ghost fluid, no density correction) with the given grid and solver."""
+    s = Settings()
+    for a in range(3):
+        s.grid_dim[a] = int(grid[a])
+    s.cell_size = float(cell_size)
+    s.particle_capacity = int(capacity)
+    s.sparse_storage_budget = float(budget)
+    s.pressure_solver = int(solver)
+    s.tolerance = float(tolerance)
+    s.max_iterations = int(max_iterations)
+    s.sor_weight = float(sor_weight)
+    s.ghost_fluid = int(ghost_fluid)
+    s.extrapolation_layers = int(extrapolation_layers)
+    s.boundary_friction = float(friction)
+    s.max_speed = float(max_speed)
+    s.particle_radius_cells = 0.25
+    s.density_correction = 0.0
+    s.density_deadband = 0.05
+    s.flags = 0
+    return s
+
+
+def fill_request(cell_min, cell_dims, seed, velocity=(0.0, 0.0, 0.0)):
+    r = EmitRequest()
+    r.shape = EMIT_FILL_CELLS
+    r.seed = int(seed) & 0xFFFFFFFF
+    for a in range(3):
+        r.cell_min[a] = int(cell_min[a])
+        r.cell_dims[a] = int(cell_dims[a])
+        r.velocity[a] = float(velocity[a])
+    r.rotation[3] = 1.0
+    return r
+
+
+def sphere_collider(center, radius, linear_velocity=(0.0, 0.0, 0.0)):
+    c = Collider()
+    for a in range(3):
+        c.center[a] = float(center[a])
+        c.linear_velocity[a] = float(linear_velocity[a])
+    c.type = 0
+    c.rotation[3] = 1.0
+    c.extent[0] = float(radius)
+    return c

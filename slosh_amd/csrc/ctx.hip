@@ -1,0 +1,718 @@
+// ctx.hip -- context lifecycle, step orchestration and the C ABI (include/slosh.h).
+//
+// Replaces FSloshProxy (SloshDemo/Plugins/Slosh/Source/SloshCore/Private/SloshProxy.cpp): Allocate (:185-306)
+// becomes slosh_create, the emit passes of Step_RenderThread (:502-540) become slosh_emit, and one substep of the
+// step loop (:564-713) becomes slosh_step. Device work is ordered on one HIP stream per context; the host reads
+// back the allocated-brick count once per step (HIP has no indirect dispatch) and the solver state once per
+// chunk of iterations.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "slosh_ctx.h"
+
+using namespace slosh;
+
+namespace slosh {
+const char *const kKernelNames[K_COUNT] = {
+    "k_emit_fill", "k_sort_tiles", "k_sort_runs_scatter", "k_sort_key_runs", "k_sort_scatter", "k_scan",
+    "k_pool", "k_p2g", "k_normalize", "k_divergence", "k_project", "k_div_stats", "k_extrapolate", "k_g2p",
+    "k_build_type", "k_coarse_factor", "k_pcg_init", "k_smooth0", "k_smooth", "k_restrict0", "k_restrict",
+    "k_prolong0", "k_prolong", "k_coarse_solve", "k_apply_a", "k_update_pr", "k_update_d", "k_sor_sweep",
+    "k_sor_residual", "k_fill"};
+} // namespace slosh
+
+static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");
+static_assert(sizeof(slosh_particle) == 64, "64-byte FP32 particle record");
+static_assert(sizeof(slosh_particle_ref48) == 48, "FSloshParticleGPU layout (SloshTypes.h:84-91)");
+static_assert(sizeof(slosh_settings) == 96, "settings ABI size");
+static_assert(sizeof(slosh_emit_request) == 100, "emit request ABI size");
+static_assert(sizeof(slosh_results) == 96, "results ABI size");
+
+namespace {
+
+template <typename T> void dalloc(T **p, size_t count)
+{
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    if (hipMalloc((void **)p, bytes) != hipSuccess) throw std::bad_alloc();
+    SLOSH_HIP(hipMemset(*p, 0, bytes));
+}
+
+template <typename T> void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+void free_all(slosh_ctx *c)
+{
+    for (int k = 0; k < 2; ++k) {
+        dfree(c->parts[k].pa);
+        for (int a = 0; a < 3; ++a) dfree(c->parts[k].v[a]);
+        for (int q = 0; q < 9; ++q) dfree(c->parts[k].c[q]);
+    }
+    dfree(c->packed); dfree(c->tile_nruns); dfree(c->run_key); dfree(c->run_len); dfree(c->run_base);
+    dfree(c->run_sorted); dfree(c->key_count); dfree(c->key_off); dfree(c->key_runs); dfree(c->run_off);
+    dfree(c->run_cursor); dfree(c->scan_blocks);
+    dfree(c->table); dfree(c->slot_brick); dfree(c->active); dfree(c->need); dfree(c->need_rank);
+    dfree(c->freeflag); dfree(c->free_rank); dfree(c->free_list); dfree(c->alloc_flag); dfree(c->alloc_rank);
+    dfree(c->new_slot); dfree(c->touch);
+    dfree(c->tiles); dfree(c->tile_valid);
+    for (int k = 0; k < 3; ++k) dfree(c->vel[k]);
+    dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag);
+    dfree(c->r); dfree(c->z); dfree(c->d); dfree(c->q);
+    for (int l = 1; l < kMaxLevels; ++l) {
+        dfree(c->lv[l].type); dfree(c->lv[l].x); dfree(c->lv[l].b);
+    }
+    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->partials); dfree(c->red_counter);
+    dfree(c->state); dfree(c->stats); dfree(c->d_col);
+    dfree(c->staging);
+    for (uint32_t k = 0; k < c->tev_cap; ++k) (void)hipEventDestroy(c->tev[k]);
+    delete[] c->tev;
+    delete[] c->tev_kernel;
+    c->tev = nullptr;
+    c->tev_kernel = nullptr;
+    c->tev_cap = 0;
+    if (c->h_stats) (void)hipHostFree(c->h_stats);
+    if (c->h_state) (void)hipHostFree(c->h_state);
+    c->h_stats = nullptr;
+    c->h_state = nullptr;
+    if (c->ev_step0) (void)hipEventDestroy(c->ev_step0);
+    if (c->ev_step1) (void)hipEventDestroy(c->ev_step1);
+    if (c->ev_solve0) (void)hipEventDestroy(c->ev_solve0);
+    if (c->ev_solve1) (void)hipEventDestroy(c->ev_solve1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+}
+
+bool settings_valid(const slosh_settings *s)
+{
+    for (int a = 0; a < 3; ++a)
+        if (s->grid_dim[a] < 4 || s->grid_dim[a] > 4096) return false;
+    if (!(s->cell_size > 0.0f) || !(s->sparse_storage_budget > 0.0f) || s->sparse_storage_budget > 1.0f) return false;
+    if (s->particle_capacity == 0u) return false;
+    if (s->pressure_solver < SLOSH_SOLVER_MGPCG || s->pressure_solver > SLOSH_SOLVER_NONE) return false;
+    if (!(s->tolerance > 0.0f) || s->max_iterations < 0) return false;
+    if (!(s->sor_weight >= 0.5f && s->sor_weight <= 1.95f)) return false;
+    if (s->extrapolation_layers < 0 || s->extrapolation_layers > 4) return false;
+    if (!(s->boundary_friction >= 0.0f && s->boundary_friction <= 1.0f)) return false;
+    for (int k = 0; k < 6; ++k)
+        if (s->reserved[k] != 0) return false;
+    return true;
+}
+
+void ensure_staging(slosh_ctx *c, size_t bytes)
+{
+    if (c->staging_bytes >= bytes) return;
+    dfree(c->staging);
+    c->staging_bytes = 0;
+    if (hipMalloc(&c->staging, bytes) != hipSuccess) throw std::bad_alloc();
+    c->staging_bytes = bytes;
+}
+
+void create_impl(slosh_ctx *c)
+{
+    const slosh_settings &s = c->s;
+    SLOSH_HIP(hipGetDevice(&c->device));
+    SLOSH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    SLOSH_HIP(hipEventCreate(&c->ev_step0));
+    SLOSH_HIP(hipEventCreate(&c->ev_step1));
+    SLOSH_HIP(hipEventCreate(&c->ev_solve0));
+    SLOSH_HIP(hipEventCreate(&c->ev_solve1));
+    Geo &g = c->geo;
+    for (int a = 0; a < 3; ++a) {
+        g.n[a] = s.grid_dim[a];
+        g.bd[a] = (s.grid_dim[a] + 7) / 8;
+    }
+    g.nbricks = (uint32_t)g.bd[0] * (uint32_t)g.bd[1] * (uint32_t)g.bd[2];
+    uint32_t slots = (uint32_t)ceil((double)s.sparse_storage_budget * (double)g.nbricks);
+    if (slots < 1u) slots = 1u;
+    if (slots > g.nbricks) slots = g.nbricks;
+    g.npool = slots + 1u;
+    const uint64_t pc64 = (uint64_t)g.npool * kBrickCells;
+    if (pc64 * 3u > 0xFFFFFFFFull || g.nbricks >= (1u << 23)) throw std::bad_alloc();
+    g.pool_cells = (uint32_t)pc64;
+    g.h = s.cell_size;
+    g.inv_h = 1.0f / s.cell_size;
+    c->nkeys = g.nbricks + 1u;
+    c->cap = s.particle_capacity;
+    c->ntiles_max = (c->cap + 255u) / 256u;
+    const size_t cap = c->cap, runs = (size_t)c->ntiles_max * 256u, pc = g.pool_cells;
+    for (int k = 0; k < 2; ++k) {
+        dalloc(&c->parts[k].pa, cap);
+        for (int a = 0; a < 3; ++a) dalloc(&c->parts[k].v[a], cap);
+        for (int q = 0; q < 9; ++q) dalloc(&c->parts[k].c[q], cap);
+    }
+    dalloc(&c->packed, cap);
+    dalloc(&c->tile_nruns, c->ntiles_max);
+    dalloc(&c->run_key, runs);
+    dalloc(&c->run_len, runs);
+    dalloc(&c->run_base, runs);
+    dalloc(&c->run_sorted, runs);
+    dalloc(&c->key_count, c->nkeys);
+    dalloc(&c->key_runs, c->nkeys);
+    dalloc(&c->run_cursor, c->nkeys);
+    dalloc(&c->key_off, (size_t)c->nkeys + 1u);
+    dalloc(&c->run_off, (size_t)c->nkeys + 1u);
+    uint32_t maxn = c->nkeys > g.npool ? c->nkeys : g.npool;
+    dalloc(&c->scan_blocks, (size_t)(maxn / 1024u + 2u));
+    dalloc(&c->table, g.nbricks);
+    dalloc(&c->slot_brick, g.npool);
+    dalloc(&c->active, g.nbricks);
+    dalloc(&c->need, g.nbricks);
+    dalloc(&c->need_rank, (size_t)g.nbricks + 1u);
+    dalloc(&c->freeflag, g.npool);
+    dalloc(&c->free_rank, (size_t)g.npool + 1u);
+    dalloc(&c->free_list, g.npool);
+    dalloc(&c->alloc_flag, g.nbricks);
+    dalloc(&c->alloc_rank, (size_t)g.nbricks + 1u);
+    dalloc(&c->new_slot, g.npool);
+    dalloc(&c->touch, g.nbricks);
+    dalloc(&c->tiles, (size_t)g.npool * kTileInts);
+    dalloc(&c->tile_valid, g.npool);
+    for (int k = 0; k < 3; ++k) dalloc(&c->vel[k], 3 * pc);
+    dalloc(&c->fflags, 3 * pc);
+    dalloc(&c->fweight, 3 * pc);
+    dalloc(&c->ctype, pc);
+    dalloc(&c->pres, pc);
+    dalloc(&c->div, pc);
+    dalloc(&c->diag, pc);
+    dalloc(&c->r, pc);
+    dalloc(&c->z, pc);
+    dalloc(&c->d, pc);
+    dalloc(&c->q, pc);
+    // MG hierarchy: halve (rounding up) until a level holds <= 64 cells or 8 levels exist.
+    c->nlevels = 1;
+    c->lv[0].dim[0] = g.n[0];
+    c->lv[0].dim[1] = g.n[1];
+    c->lv[0].dim[2] = g.n[2];
+    c->lv[0].cells = (uint32_t)g.n[0] * (uint32_t)g.n[1] * (uint32_t)g.n[2];
+    while (c->lv[c->nlevels - 1].cells > kCoarsestCells && c->nlevels < kMaxLevels) {
+        Level &p = c->lv[c->nlevels - 1];
+        Level &l = c->lv[c->nlevels];
+        for (int a = 0; a < 3; ++a) l.dim[a] = (p.dim[a] + 1) / 2;
+        l.cells = (uint32_t)l.dim[0] * (uint32_t)l.dim[1] * (uint32_t)l.dim[2];
+        dalloc(&l.type, l.cells);
+        dalloc(&l.x, l.cells);
+        dalloc(&l.b, l.cells);
+        c->nlevels++;
+    }
+    if (c->lv[c->nlevels - 1].cells > kCoarsestCells) throw std::bad_alloc();
+    dalloc(&c->coarse_idx, kCoarsestCells);
+    dalloc(&c->coarse_n, 1);
+    dalloc(&c->coarse_inv, (size_t)kCoarsestCells * kCoarsestCells);
+    dalloc(&c->partials, 2 * (size_t)g.npool + 2u);
+    dalloc(&c->red_counter, 1);
+    dalloc(&c->state, 1);
+    dalloc(&c->stats, 1);
+    dalloc(&c->d_col, SLOSH_MAX_COLLIDERS);
+    if (hipHostMalloc((void **)&c->h_stats, sizeof(StepStats), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+    if (hipHostMalloc((void **)&c->h_state, sizeof(SolverState), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+    memset(c->h_stats, 0, sizeof(StepStats));
+    memset(c->h_state, 0, sizeof(SolverState));
+    g.table = c->table;
+    g.active = c->active;
+    launch_fill_u32(c->stream, c->slot_brick, kNoBrick, g.npool);
+    SLOSH_HIP(hipStreamSynchronize(c->stream));
+}
+
+void reset_impl(slosh_ctx *c)
+{
+    const Geo &g = c->geo;
+    c->np = 0;
+    c->ndead = 0;
+    c->cur = 0;
+    c->vcur = 0;
+    c->nactive = 0;
+    c->frame = 0;
+    c->prev_dt = 0.0f;
+    memset(&c->res, 0, sizeof(c->res));
+    launch_fill_u32(c->stream, c->table, 0u, g.nbricks);
+    launch_fill_u32(c->stream, c->slot_brick, kNoBrick, g.npool);
+    launch_fill_u8(c->stream, c->touch, 0u, g.nbricks);
+    launch_fill_u8(c->stream, c->new_slot, 0u, g.npool);
+    launch_fill_f64(c->stream, c->pres, 0.0, g.pool_cells);
+    SLOSH_HIP(hipStreamSynchronize(c->stream));
+}
+
+// Folds the event pairs recorded since the last harvest into the per-kernel totals (stream must be idle).
+void harvest_timing(slosh_ctx *c)
+{
+    for (uint32_t k = 0; k + 1 < c->tev_used; k += 2) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->tev[k], c->tev[k + 1]) == hipSuccess) {
+            int id = c->tev_kernel[k / 2];
+            c->ktime_ms[id] += (double)ms;
+            c->klaunches[id] += 1u;
+        }
+    }
+    c->tev_used = 0;
+}
+
+void read_stats(slosh_ctx *c)
+{
+    SLOSH_HIP(hipMemcpyAsync(c->h_stats, c->stats, sizeof(StepStats), hipMemcpyDeviceToHost, c->stream));
+    SLOSH_HIP(hipStreamSynchronize(c->stream));
+    harvest_timing(c);
+}
+
+// One substep in the order of SloshProxy.cpp:544-713, stopping after `stop` (SLOSH_STAGE_*).
+void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
+{
+    c->frame++;
+    const float warm = c->prev_dt > 0.0f ? dt / c->prev_dt : 1.0f;
+    c->prev_dt = dt;
+    c->res.sim_frame = c->frame;
+    c->res.solver_iterations = 0;
+    c->res.residual_rel = 0.0;
+    c->res.converged = 1;
+    c->res.solve_ms = 0.0f;
+    launch_fill_u32(c->stream, (uint32_t *)c->stats, 0u, sizeof(StepStats) / 4u);
+    SLOSH_HIP(hipEventRecord(c->ev_step0, c->stream));
+    run_sort(c);
+    c->np -= c->ndead;
+    c->ndead = 0;
+    c->res.live_particles = c->np;
+    if (stop > SLOSH_STAGE_SORT) {
+        run_pool(c);
+        read_stats(c);
+        c->nactive = c->h_stats->nactive;
+        c->res.active_bricks = c->nactive;
+        c->res.pool_overflow_bricks = c->h_stats->overflow;
+    }
+    if (stop > SLOSH_STAGE_POOL) launch_p2g(c, warm);
+    if (stop > SLOSH_STAGE_P2G) launch_normalize(c, dt, gv);
+    if (stop > SLOSH_STAGE_NORMALIZE) launch_divergence(c, dt);
+    if (stop > SLOSH_STAGE_DIVERGENCE) {
+        SLOSH_HIP(hipEventRecord(c->ev_solve0, c->stream));
+        if (c->s.pressure_solver == SLOSH_SOLVER_MGPCG) run_mgpcg(c);
+        else if (c->s.pressure_solver == SLOSH_SOLVER_RBSOR) run_rbsor(c);
+        SLOSH_HIP(hipEventRecord(c->ev_solve1, c->stream));
+        if (c->s.pressure_solver != SLOSH_SOLVER_NONE) {
+            SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+            SLOSH_HIP(hipStreamSynchronize(c->stream));
+            harvest_timing(c);
+            if (c->nactive) {
+                c->res.solver_iterations = c->h_state->iters;
+                c->res.residual_rel = c->h_state->rel;
+                c->res.rhs_norm = sqrt(c->h_state->bb);
+                c->res.converged = c->h_state->converged;
+            }
+            float ms = 0.0f;
+            SLOSH_HIP(hipEventElapsedTime(&ms, c->ev_solve0, c->ev_solve1));
+            c->res.solve_ms = ms;
+        }
+    }
+    if (stop > SLOSH_STAGE_SOLVE) launch_project(c);
+    if (stop > SLOSH_STAGE_PROJECT)
+        for (int k = 0; k < c->s.extrapolation_layers; ++k) launch_extrapolate(c, (uint32_t)k);
+    if (stop > SLOSH_STAGE_EXTRAPOLATE) launch_g2p(c, dt, gv);
+    SLOSH_HIP(hipEventRecord(c->ev_step1, c->stream));
+    read_stats(c);
+    const StepStats &st = *c->h_stats;
+    c->res.fluid_cells = st.fluid_cells;
+    float f;
+    memcpy(&f, &st.max_speed_bits, 4);
+    c->res.max_particle_speed = f;
+    memcpy(&f, &st.max_div_bits, 4);
+    c->res.max_divergence = f;
+    c->res.mean_divergence = st.div_count ? (float)(st.div_sum / (double)st.div_count) : 0.0f;
+    c->res.clamped_particles = st.clamped;
+    float ms = 0.0f;
+    SLOSH_HIP(hipEventElapsedTime(&ms, c->ev_step0, c->ev_step1));
+    c->res.step_ms = ms;
+}
+
+// Pool -> dense conversion on the host (x fastest; unallocated cells read 0).
+template <typename Src, typename Dst>
+void pool_to_dense(const slosh_ctx *c, const std::vector<uint32_t> &table, const Src *pool, Dst *dense, int planes)
+{
+    const Geo &g = c->geo;
+    const size_t cells = (size_t)g.n[0] * g.n[1] * g.n[2];
+    for (int pl = 0; pl < planes; ++pl)
+        for (int z = 0; z < g.n[2]; ++z)
+            for (int y = 0; y < g.n[1]; ++y)
+                for (int x = 0; x < g.n[0]; ++x) {
+                    uint32_t b = (uint32_t)((x >> 3) + g.bd[0] * ((y >> 3) + g.bd[1] * (z >> 3)));
+                    uint32_t slot = table[b];
+                    size_t di = (size_t)pl * cells + (size_t)x + (size_t)g.n[0] * ((size_t)y + (size_t)g.n[1] * z);
+                    if (slot == 0u) {
+                        dense[di] = (Dst)0;
+                        continue;
+                    }
+                    uint32_t l = (uint32_t)((x & 7) + 8 * ((y & 7) + 8 * (z & 7)));
+                    dense[di] = (Dst)pool[(size_t)pl * g.pool_cells + (size_t)slot * kBrickCells + l];
+                }
+}
+
+template <typename T> std::vector<T> fetch(const slosh_ctx *c, const T *d, size_t n)
+{
+    std::vector<T> h(n);
+    SLOSH_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    SLOSH_HIP(hipStreamSynchronize(c->stream));
+    return h;
+}
+
+slosh_status read_grid_impl(slosh_ctx *c, int32_t field, void *dst, uint64_t bytes)
+{
+    const Geo &g = c->geo;
+    const size_t cells = (size_t)g.n[0] * g.n[1] * g.n[2];
+    std::vector<uint32_t> table = fetch(c, c->table, g.nbricks);
+    if (field == SLOSH_FIELD_BRICK_TABLE) {
+        if (bytes < (uint64_t)g.nbricks * 4u) return SLOSH_ECAPACITY;
+        memcpy(dst, table.data(), (size_t)g.nbricks * 4u);
+        return SLOSH_OK;
+    }
+    int planes = 1;
+    if (field == SLOSH_FIELD_VEL || field == SLOSH_FIELD_FACE_FLAGS || field == SLOSH_FIELD_FACE_WEIGHT) planes = 3;
+    if (field == SLOSH_FIELD_ACCUM) planes = 6;
+    if (field == SLOSH_FIELD_PRESSURE_F64) {
+        if (bytes < (uint64_t)cells * 8u) return SLOSH_ECAPACITY;
+        auto h = fetch(c, c->pres, g.pool_cells);
+        pool_to_dense(c, table, h.data(), (double *)dst, 1);
+        return SLOSH_OK;
+    }
+    if (field < 0 || field > SLOSH_FIELD_ACCUM) return SLOSH_EINVAL;
+    if (bytes < (uint64_t)cells * 4u * (uint64_t)planes) return SLOSH_ECAPACITY;
+    const size_t pc = g.pool_cells;
+    switch (field) {
+    case SLOSH_FIELD_CELL_FLAGS: {
+        auto h = fetch(c, c->ctype, pc);
+        pool_to_dense(c, table, h.data(), (uint32_t *)dst, 1);
+        break;
+    }
+    case SLOSH_FIELD_PRESSURE: {
+        auto h = fetch(c, c->pres, pc);
+        pool_to_dense(c, table, h.data(), (float *)dst, 1);
+        break;
+    }
+    case SLOSH_FIELD_DIVERGENCE: {
+        auto h = fetch(c, c->div, pc);
+        pool_to_dense(c, table, h.data(), (float *)dst, 1);
+        break;
+    }
+    case SLOSH_FIELD_CELL_DIAG: {
+        auto h = fetch(c, c->diag, pc);
+        pool_to_dense(c, table, h.data(), (float *)dst, 1);
+        break;
+    }
+    case SLOSH_FIELD_VEL: {
+        auto h = fetch(c, c->vel[c->vcur], 3 * pc);
+        pool_to_dense(c, table, h.data(), (float *)dst, 3);
+        break;
+    }
+    case SLOSH_FIELD_FACE_FLAGS: {
+        auto h = fetch(c, c->fflags, 3 * pc);
+        pool_to_dense(c, table, h.data(), (uint32_t *)dst, 3);
+        break;
+    }
+    case SLOSH_FIELD_FACE_WEIGHT: {
+        auto h = fetch(c, c->fweight, 3 * pc);
+        pool_to_dense(c, table, h.data(), (float *)dst, 3);
+        break;
+    }
+    case SLOSH_FIELD_ACCUM: {
+        // P2G sums: the overlapping brick tiles of each face, summed as the normalise pass does.
+        auto tiles = fetch(c, c->tiles, (size_t)g.npool * kTileInts);
+        auto valid = fetch(c, c->tile_valid, g.npool);
+        int32_t *out = (int32_t *)dst;
+        memset(out, 0, cells * 4u * 6u);
+        for (int z = 0; z < g.n[2]; ++z)
+            for (int y = 0; y < g.n[1]; ++y)
+                for (int x = 0; x < g.n[0]; ++x) {
+                    int p[3] = {x, y, z};
+                    uint32_t b = (uint32_t)((x >> 3) + g.bd[0] * ((y >> 3) + g.bd[1] * (z >> 3)));
+                    if (table[b] == 0u) continue;
+                    size_t di = (size_t)x + (size_t)g.n[0] * ((size_t)y + (size_t)g.n[1] * z);
+                    for (int dz = -1; dz <= 1; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                int nb[3] = {(x >> 3) + dx, (y >> 3) + dy, (z >> 3) + dz};
+                                bool ok = true;
+                                int tc[3];
+                                for (int a = 0; a < 3; ++a) {
+                                    if (nb[a] < 0 || nb[a] >= g.bd[a]) ok = false;
+                                    tc[a] = p[a] - (nb[a] * 8 - 1);
+                                    if (tc[a] < 0 || tc[a] >= kTileDim) ok = false;
+                                }
+                                if (!ok) continue;
+                                uint32_t ns = table[(uint32_t)(nb[0] + g.bd[0] * (nb[1] + g.bd[1] * nb[2]))];
+                                if (ns == 0u || !valid[ns]) continue;
+                                const int32_t *t = tiles.data() + (size_t)ns * kTileInts +
+                                                   (tc[0] + kTileDim * (tc[1] + kTileDim * tc[2]));
+                                for (int q = 0; q < 6; ++q) out[(size_t)q * cells + di] += t[q * kTileNodes];
+                            }
+                }
+        break;
+    }
+    default:
+        return SLOSH_EINVAL;
+    }
+    return SLOSH_OK;
+}
+
+thread_local char g_info[256];
+
+} // namespace
+
+#define ABI_GUARD(body)                                                                                          \
+    try {                                                                                                        \
+        body                                                                                                     \
+    } catch (const slosh::HipError &e) {                                                                        \
+        fprintf(stderr, "slosh: HIP error %d (%s) at %s:%d: %s\n", (int)e.err, hipGetErrorString(e.err), e.file, \
+                e.line, e.what);                                                                                 \
+        return SLOSH_EDEVICE;                                                                                    \
+    } catch (const std::bad_alloc &) {                                                                          \
+        return SLOSH_ENOMEM;                                                                                     \
+    }
+
+extern "C" {
+
+uint32_t slosh_abi_version(void) { return SLOSH_ABI_VERSION; }
+
+const char *slosh_build_info(void)
+{
+    snprintf(g_info, sizeof(g_info), "slosh_amd abi %u; offload gfx950; HIP %d.%d; -ffp-contract=off", SLOSH_ABI_VERSION,
+             HIP_VERSION_MAJOR, HIP_VERSION_MINOR);
+    return g_info;
+}
+
+const char *slosh_status_string(slosh_status s)
+{
+    switch (s) {
+    case SLOSH_OK: return "ok";
+    case SLOSH_EINVAL: return "invalid argument";
+    case SLOSH_ENOMEM: return "out of memory";
+    case SLOSH_EDEVICE: return "device error";
+    case SLOSH_ECAPACITY: return "capacity exceeded";
+    case SLOSH_ESTATE: return "invalid state";
+    }
+    return "unknown";
+}
+
+void slosh_default_settings(slosh_settings *s)
+{
+    if (!s) return;
+    memset(s, 0, sizeof(*s));
+    s->grid_dim[0] = 64;
+    s->grid_dim[1] = 64;
+    s->grid_dim[2] = 64;
+    s->cell_size = 1.0f / 64.0f;
+    s->particle_capacity = 1u << 20;
+    s->sparse_storage_budget = 1.0f;
+    s->pressure_solver = SLOSH_SOLVER_MGPCG;
+    s->tolerance = 1e-6f;
+    s->max_iterations = 200;
+    s->sor_weight = 1.8f;
+    s->ghost_fluid = 1;
+    s->extrapolation_layers = 2;
+    s->boundary_friction = 0.0f;
+    s->max_speed = 20.0f;
+    s->particle_radius_cells = 0.25f;
+    s->density_correction = 0.0f;
+    s->density_deadband = 0.05f;
+}
+
+slosh_status slosh_create(const slosh_settings *s, slosh_ctx **out)
+{
+    if (!s || !out) return SLOSH_EINVAL;
+    *out = nullptr;
+    if (!settings_valid(s)) return SLOSH_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SLOSH_EDEVICE;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return SLOSH_EDEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return SLOSH_EDEVICE;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        fprintf(stderr, "slosh: device %d is %s; the kernels are built for gfx950\n", dev, prop.gcnArchName);
+        return SLOSH_EDEVICE;
+    }
+    slosh_ctx *c = new (std::nothrow) slosh_ctx();
+    if (!c) return SLOSH_ENOMEM;
+    c->s = *s;
+    try {
+        create_impl(c);
+    } catch (const slosh::HipError &e) {
+        fprintf(stderr, "slosh: HIP error %d (%s) at %s:%d\n", (int)e.err, hipGetErrorString(e.err), e.file, e.line);
+        free_all(c);
+        delete c;
+        return SLOSH_EDEVICE;
+    } catch (const std::bad_alloc &) {
+        free_all(c);
+        delete c;
+        return SLOSH_ENOMEM;
+    }
+    *out = c;
+    return SLOSH_OK;
+}
+
+void slosh_destroy(slosh_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+slosh_status slosh_reset(slosh_ctx *c)
+{
+    if (!c) return SLOSH_EINVAL;
+    ABI_GUARD(reset_impl(c); return SLOSH_OK;)
+}
+
+slosh_status slosh_emit(slosh_ctx *c, const slosh_emit_request *r)
+{
+    if (!c || !r) return SLOSH_EINVAL;
+    if (r->shape != SLOSH_EMIT_FILL_CELLS) return SLOSH_EINVAL;
+    for (int a = 0; a < 3; ++a)
+        if (r->cell_dims[a] < 0) return SLOSH_EINVAL;
+    uint64_t count = (uint64_t)r->cell_dims[0] * (uint64_t)r->cell_dims[1] * (uint64_t)r->cell_dims[2] * 8u;
+    if (count == 0) return SLOSH_OK;
+    if ((uint64_t)c->np + count > (uint64_t)c->cap) return SLOSH_ECAPACITY;
+    ABI_GUARD(launch_emit_fill(c, r, (uint32_t)count); c->np += (uint32_t)count;
+              SLOSH_HIP(hipStreamSynchronize(c->stream)); return SLOSH_OK;)
+}
+
+slosh_status slosh_set_colliders(slosh_ctx *c, const slosh_collider *col, uint32_t n)
+{
+    if (!c || (n && !col)) return SLOSH_EINVAL;
+    if (n > SLOSH_MAX_COLLIDERS) return SLOSH_ECAPACITY;
+    ABI_GUARD(if (n) {
+        memcpy(c->h_col, col, n * sizeof(slosh_collider));
+        SLOSH_HIP(hipMemcpyAsync(c->d_col, c->h_col, n * sizeof(slosh_collider), hipMemcpyHostToDevice, c->stream));
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+    } c->ncol = n;
+              return SLOSH_OK;)
+}
+
+slosh_status slosh_step_until(slosh_ctx *c, float dt, const float gravity[3], int32_t stop_after)
+{
+    if (!c || !gravity || !(dt > 0.0f) || stop_after < SLOSH_STAGE_SORT || stop_after > SLOSH_STAGE_ALL)
+        return SLOSH_EINVAL;
+    ABI_GUARD(step_impl(c, dt, gravity, stop_after); return SLOSH_OK;)
+}
+
+slosh_status slosh_step(slosh_ctx *c, float dt, const float gravity[3])
+{
+    return slosh_step_until(c, dt, gravity, SLOSH_STAGE_ALL);
+}
+
+slosh_status slosh_read_particles(slosh_ctx *c, void *dst, uint64_t dst_bytes, uint32_t *count, int32_t layout)
+{
+    if (!c || !count || (layout != SLOSH_LAYOUT_F32_64 && layout != SLOSH_LAYOUT_REF_48)) return SLOSH_EINVAL;
+    const size_t rec = layout == SLOSH_LAYOUT_F32_64 ? 64u : 48u;
+    *count = 0;
+    if ((uint64_t)c->np * rec > dst_bytes) return SLOSH_ECAPACITY;
+    if (c->np == 0) return SLOSH_OK;
+    if (!dst) return SLOSH_EINVAL;
+    ABI_GUARD(ensure_staging(c, (size_t)c->np * rec); launch_pack_particles(c, c->staging, layout);
+              SLOSH_HIP(hipMemcpyAsync(dst, c->staging, (size_t)c->np * rec, hipMemcpyDeviceToHost, c->stream));
+              SLOSH_HIP(hipStreamSynchronize(c->stream)); *count = c->np; return SLOSH_OK;)
+}
+
+slosh_status slosh_device_particles(slosh_ctx *c, int32_t layout, void **dev_ptr, uint32_t *count)
+{
+    if (!c || !dev_ptr || !count || (layout != SLOSH_LAYOUT_F32_64 && layout != SLOSH_LAYOUT_REF_48)) return SLOSH_EINVAL;
+    const size_t rec = layout == SLOSH_LAYOUT_F32_64 ? 64u : 48u;
+    ABI_GUARD(ensure_staging(c, (size_t)(c->np ? c->np : 1u) * rec); launch_pack_particles(c, c->staging, layout);
+              SLOSH_HIP(hipStreamSynchronize(c->stream)); *dev_ptr = c->staging; *count = c->np; return SLOSH_OK;)
+}
+
+slosh_status slosh_write_particles(slosh_ctx *c, const void *src, uint32_t n)
+{
+    if (!c || (n && !src)) return SLOSH_EINVAL;
+    if (n > c->cap) return SLOSH_ECAPACITY;
+    ABI_GUARD(launch_fill_u8(c->stream, c->touch, 0u, c->geo.nbricks); c->np = 0; c->ndead = 0; if (n) {
+        ensure_staging(c, (size_t)n * sizeof(slosh_particle));
+        SLOSH_HIP(hipMemcpyAsync(c->staging, src, (size_t)n * sizeof(slosh_particle), hipMemcpyHostToDevice, c->stream));
+        launch_unpack_particles(c, (const slosh_particle *)c->staging, n);
+        c->np = n;
+        const slosh_particle *sp = (const slosh_particle *)src;
+        for (uint32_t i = 0; i < n; ++i) c->ndead += (sp[i].age >= 0.0f) ? 0u : 1u;
+    } SLOSH_HIP(hipStreamSynchronize(c->stream));
+              return SLOSH_OK;)
+}
+
+slosh_status slosh_read_grid(slosh_ctx *c, int32_t field, void *dst, uint64_t dst_bytes, int32_t order)
+{
+    if (!c || !dst || order != SLOSH_ORDER_DENSE) return SLOSH_EINVAL;
+    ABI_GUARD(return read_grid_impl(c, field, dst, dst_bytes);)
+}
+
+slosh_status slosh_write_grid(slosh_ctx *c, int32_t field, const void *src, uint64_t src_bytes, int32_t order)
+{
+    if (!c || !src || order != SLOSH_ORDER_DENSE || field != SLOSH_FIELD_PRESSURE) return SLOSH_EINVAL;
+    const Geo &g = c->geo;
+    const size_t cells = (size_t)g.n[0] * g.n[1] * g.n[2];
+    if (src_bytes < cells * 4u) return SLOSH_ECAPACITY;
+    ABI_GUARD({
+        std::vector<uint32_t> table = fetch(c, c->table, g.nbricks);
+        std::vector<double> pool = fetch(c, c->pres, g.pool_cells);
+        const float *in = (const float *)src;
+        for (int z = 0; z < g.n[2]; ++z)
+            for (int y = 0; y < g.n[1]; ++y)
+                for (int x = 0; x < g.n[0]; ++x) {
+                    uint32_t slot = table[(uint32_t)((x >> 3) + g.bd[0] * ((y >> 3) + g.bd[1] * (z >> 3)))];
+                    if (!slot) continue;
+                    pool[(size_t)slot * kBrickCells + (size_t)((x & 7) + 8 * ((y & 7) + 8 * (z & 7)))] =
+                        (double)in[(size_t)x + (size_t)g.n[0] * ((size_t)y + (size_t)g.n[1] * z)];
+                }
+        SLOSH_HIP(hipMemcpyAsync(c->pres, pool.data(), pool.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        return SLOSH_OK;
+    })
+}
+
+slosh_status slosh_set_kernel_timing(slosh_ctx *c, uint64_t mask)
+{
+    if (!c) return SLOSH_EINVAL;
+    ABI_GUARD({
+        if (mask && !c->tev) {
+            const uint32_t cap = 8192;
+            c->tev = new hipEvent_t[cap];
+            c->tev_kernel = new int[cap / 2];
+            for (uint32_t k = 0; k < cap; ++k) {
+                SLOSH_HIP(hipEventCreate(&c->tev[k]));
+                c->tev_cap = k + 1;
+            }
+        }
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        harvest_timing(c);
+        c->timing_mask = mask;
+        return SLOSH_OK;
+    })
+}
+
+slosh_status slosh_kernel_times(slosh_ctx *c, slosh_kernel_time *out, uint32_t cap, uint32_t *n)
+{
+    if (!c || !n) return SLOSH_EINVAL;
+    *n = K_COUNT;
+    if (!out) {
+        for (int k = 0; k < K_COUNT; ++k) {
+            c->ktime_ms[k] = 0.0;
+            c->klaunches[k] = 0u;
+        }
+        return SLOSH_OK;
+    }
+    for (uint32_t k = 0; k < cap && k < (uint32_t)K_COUNT; ++k) {
+        memset(&out[k], 0, sizeof(out[k]));
+        strncpy(out[k].name, kKernelNames[k], sizeof(out[k].name) - 1);
+        out[k].launches = c->klaunches[k];
+        out[k].total_ms = c->ktime_ms[k];
+    }
+    return SLOSH_OK;
+}
+
+slosh_status slosh_get_results(slosh_ctx *c, slosh_results *out)
+{
+    if (!c || !out) return SLOSH_EINVAL;
+    *out = c->res;
+    out->live_particles = c->np;
+    return SLOSH_OK;
+}
+
+} // extern "C"

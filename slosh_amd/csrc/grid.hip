@@ -1,0 +1,587 @@
+// grid.hip -- grid kernels of the solver step on gfx950: brick pool, normalise, divergence, projection,
+// extrapolation, post-projection divergence statistics, and the scan / fill utilities they use.
+//
+// Grid passes run one 512-thread workgroup (8 waves) per allocated brick: thread l owns cell l of the brick's
+// pool slot, so every per-cell array is read and written as contiguous 2 KB (FP32) runs.
+#include "slosh_ctx.h"
+
+namespace slosh {
+
+namespace {
+
+inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u8(uint8_t *p, uint8_t v, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(double *p, double v, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---- exclusive scan of u32: 1024 elements per workgroup, block sums scanned by one workgroup ----------------
+constexpr uint32_t kScanBlock = 1024;
+
+__device__ uint32_t block_scan_excl_256(uint32_t v, uint32_t *sh, uint32_t *total)
+{
+    const uint32_t t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        uint32_t add = t >= off ? sh[t - off] : 0u;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    uint32_t incl = sh[t];
+    *total = sh[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *block_sums)
+{
+    __shared__ uint32_t sh[256];
+    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < 4; ++k)
+        if (base + k < n) s += in[base + k];
+    uint32_t total;
+    block_scan_excl_256(s, sh, &total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_blocks(uint32_t *block_sums, uint32_t nb)
+{
+    __shared__ uint32_t sh[256];
+    const uint32_t per = (nb + 255u) / 256u;
+    const uint32_t beg = threadIdx.x * per;
+    uint32_t s = 0;
+    for (uint32_t k = beg; k < beg + per && k < nb; ++k) s += block_sums[k];
+    uint32_t total;
+    uint32_t run = block_scan_excl_256(s, sh, &total);
+    for (uint32_t k = beg; k < beg + per && k < nb; ++k) {
+        uint32_t v = block_sums[k];
+        block_sums[k] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(const uint32_t *in, uint32_t n, const uint32_t *block_sums,
+                                                   uint32_t *out)
+{
+    __shared__ uint32_t sh[256];
+    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+    uint32_t v[4], s = 0;
+    for (uint32_t k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_scan_excl_256(s, sh, &total) + block_sums[blockIdx.x];
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) out[n] = run;
+}
+
+// ---- brick pool (Slosh.usf:1719-1831), deterministic slot assignment -----------------------------------------
+// Want = any touched brick in the 3x3x3 brick neighbourhood; allocated bricks no longer wanted free their slot.
+__global__ void k_pool_want(Geo g, const uint8_t *touch, uint32_t *table, uint32_t *slot_brick, uint32_t *need)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= g.nbricks) return;
+    int c[3];
+    brick_coord(g, b, c);
+    bool want = false;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int x = c[0] + dx, y = c[1] + dy, z = c[2] + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= g.bd[0] || y >= g.bd[1] || z >= g.bd[2]) continue;
+                want = want || touch[(uint32_t)(x + g.bd[0] * (y + g.bd[1] * z))] != 0u;
+            }
+    uint32_t slot = table[b];
+    if (!want && slot != 0u) {
+        table[b] = 0u;
+        slot_brick[slot] = kNoBrick;
+        slot = 0u;
+    }
+    need[b] = (want && slot == 0u) ? 1u : 0u;
+}
+
+__global__ void k_pool_freeflags(const uint32_t *slot_brick, uint32_t npool, uint32_t *freeflag)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npool) return;
+    freeflag[s] = (s >= 1u && slot_brick[s] == kNoBrick) ? 1u : 0u;
+}
+
+__global__ void k_pool_free_list(const uint32_t *freeflag, const uint32_t *free_rank, uint32_t npool, uint32_t *free_list)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npool) return;
+    if (freeflag[s]) free_list[free_rank[s]] = s;
+}
+
+// Wanted bricks without storage take the lowest free slots in ascending brick order; the rest overflow.
+__global__ void k_pool_assign(Geo g, const uint32_t *need, const uint32_t *need_rank, const uint32_t *free_list,
+                              const uint32_t *free_rank, uint32_t npool, uint32_t *table, uint32_t *slot_brick,
+                              uint8_t *new_slot, uint8_t *touch, uint32_t *alloc_flag, StepStats *stats)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= g.nbricks) return;
+    touch[b] = 0u;
+    uint32_t slot = table[b];
+    if (need[b]) {
+        uint32_t j = need_rank[b];
+        if (j < free_rank[npool]) {
+            slot = free_list[j];
+            table[b] = slot;
+            slot_brick[slot] = b;
+            new_slot[slot] = 1u;
+        } else {
+            atomicAdd(&stats->overflow, 1u);
+        }
+    }
+    alloc_flag[b] = slot != 0u ? 1u : 0u;
+}
+
+__global__ void k_pool_active(Geo g, const uint32_t *alloc_flag, const uint32_t *alloc_rank, uint32_t *active,
+                              StepStats *stats)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) stats->nactive = alloc_rank[g.nbricks];
+    if (b >= g.nbricks) return;
+    if (alloc_flag[b]) active[alloc_rank[b]] = b;
+}
+
+// ---- per-brick helpers -------------------------------------------------------------------------------------
+struct CellPos {
+    int x, y, z;
+    uint32_t idx;
+    uint32_t slot;
+    int bc[3];
+};
+
+__device__ __forceinline__ bool cell_of_block(const Geo &g, CellPos &c)
+{
+    const uint32_t b = g.active[blockIdx.x];
+    const uint32_t l = threadIdx.x;
+    brick_coord(g, b, c.bc);
+    c.x = c.bc[0] * 8 + (int)(l & 7u);
+    c.y = c.bc[1] * 8 + (int)((l >> 3) & 7u);
+    c.z = c.bc[2] * 8 + (int)(l >> 6);
+    c.slot = g.table[b];
+    c.idx = c.slot * kBrickCells + l;
+    return in_grid(g, c.x, c.y, c.z);
+}
+
+__device__ __forceinline__ uint32_t type_at(const Geo &g, const uint32_t *ctype, int x, int y, int z)
+{
+    return in_grid(g, x, y, z) ? ctype[cell_index(g, x, y, z)] : kCellSolid;
+}
+
+__device__ __forceinline__ uint32_t face_index(const Geo &g, int axis, int x, int y, int z)
+{
+    return (uint32_t)axis * g.pool_cells + cell_index(g, x, y, z);
+}
+
+// Slosh.usf:232-241
+__device__ __forceinline__ float solid_face_velocity(const slosh_collider *col, uint32_t packed, int axis,
+                                                     const float fp[3])
+{
+    uint32_t id = (packed >> kColliderShift) & 0xFFu;
+    if (id == 0u) return 0.0f;
+    return collider_velocity_axis(col[id - 1u], fp, axis);
+}
+
+// Slosh.usf:604-651 (NormalizeFace, APIC) over the P2G tiles: each face sums the tiles of the (up to 8) bricks
+// whose node range covers it, then ratio, weight, gravity and solid boundary velocities as the reference.
+__global__ __launch_bounds__(512) void k_normalize(Geo g, const int32_t *tiles, const uint8_t *tile_valid,
+                                                   const uint32_t *ctype, float *vel, float *fweight, uint8_t *fflags,
+                                                   float gdt0, float gdt1, float gdt2, const slosh_collider *col)
+{
+    CellPos c;
+    if (!cell_of_block(g, c)) return;
+    const int lc[3] = {c.x & 7, c.y & 7, c.z & 7};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = lc[a] == 0 ? -1 : 0;
+        hi[a] = lc[a] == 7 ? 1 : 0;
+    }
+    int32_t mom[3] = {0, 0, 0}, wt[3] = {0, 0, 0};
+    for (int dz = lo[2]; dz <= hi[2]; ++dz)
+        for (int dy = lo[1]; dy <= hi[1]; ++dy)
+            for (int dx = lo[0]; dx <= hi[0]; ++dx) {
+                int nb[3] = {c.bc[0] + dx, c.bc[1] + dy, c.bc[2] + dz};
+                if (nb[0] < 0 || nb[1] < 0 || nb[2] < 0 || nb[0] >= g.bd[0] || nb[1] >= g.bd[1] || nb[2] >= g.bd[2])
+                    continue;
+                uint32_t ns = g.table[(uint32_t)(nb[0] + g.bd[0] * (nb[1] + g.bd[1] * nb[2]))];
+                if (ns == 0u || !tile_valid[ns]) continue;
+                int tx = c.x - (nb[0] * 8 - 1), ty = c.y - (nb[1] * 8 - 1), tz = c.z - (nb[2] * 8 - 1);
+                const int32_t *t = tiles + (size_t)ns * kTileInts + (tx + kTileDim * (ty + kTileDim * tz));
+                for (int a = 0; a < 3; ++a) {
+                    mom[a] += t[a * kTileNodes];
+                    wt[a] += t[(3 + a) * kTileNodes];
+                }
+            }
+    const float gdt[3] = {gdt0, gdt1, gdt2};
+    const uint32_t tb = ctype[c.idx];
+    for (int a = 0; a < 3; ++a) {
+        uint32_t f = (uint32_t)a * g.pool_cells + c.idx;
+        float u = wt[a] > 0 ? (float)mom[a] / (float)wt[a] : 0.0f;
+        fweight[f] = (float)wt[a] * kFixedInv;
+        uint32_t flags = wt[a] > 0 ? kFaceHasWeight : 0u;
+        int ca[3] = {c.x, c.y, c.z};
+        ca[a] -= 1;
+        uint32_t ta = type_at(g, ctype, ca[0], ca[1], ca[2]);
+        uint32_t sa = ta & kCellTypeMask, sb = tb & kCellTypeMask;
+        float fp[3] = {(float)c.x + 0.5f, (float)c.y + 0.5f, (float)c.z + 0.5f};
+        fp[a] = (float)(a == 0 ? c.x : (a == 1 ? c.y : c.z));
+        for (int k = 0; k < 3; ++k) fp[k] = fp[k] * g.h;
+        if (sa == kCellSolid && sb == kCellSolid) {
+            u = solid_face_velocity(col, tb, a, fp);
+        } else if (sa == kCellSolid || sb == kCellSolid) {
+            u = solid_face_velocity(col, sb == kCellSolid ? tb : ta, a, fp);
+            flags |= kFaceValid | kFaceSolid;
+        } else if (sa == kCellFluid || sb == kCellFluid) {
+            u += gdt[a];
+            flags |= kFaceValid;
+        } else {
+            u = 0.0f;
+        }
+        vel[f] = u;
+        fflags[f] = (uint8_t)flags;
+    }
+}
+
+// Slosh.usf:211-224
+__device__ __forceinline__ float cell_density(const Geo &g, const float *fw, int x, int y, int z)
+{
+    if (!in_grid(g, x, y, z)) return 0.0f;
+    float ws = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        int p[3] = {x, y, z};
+        p[a] += 1;
+        ws += fw[face_index(g, a, x, y, z)] + fw[face_index(g, a, p[0], p[1], p[2])];
+    }
+    return ws / (6.0f * kTargetPpc);
+}
+
+__device__ __forceinline__ float air_face_weight(const Geo &g, const float *fw, int ghost, float rho_fluid, int x,
+                                                 int y, int z)
+{
+    return ghost ? 1.0f / ghost_theta(rho_fluid, cell_density(g, fw, x, y, z)) : 1.0f;
+}
+
+// Slosh.usf:692-741 (DivergenceCS)
+__global__ __launch_bounds__(512) void k_divergence(Geo g, const float *vel, const float *fw, const uint32_t *ctype,
+                                                    float *div, float *diag, int ghost, float dc, float deadband,
+                                                    float h_over_dt)
+{
+    CellPos c;
+    if (!cell_of_block(g, c)) return;
+    if ((ctype[c.idx] & kCellTypeMask) != kCellFluid) {
+        div[c.idx] = 0.0f;
+        diag[c.idx] = 0.0f;
+        return;
+    }
+    float dv = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        int p[3] = {c.x, c.y, c.z};
+        p[a] += 1;
+        dv += vel[face_index(g, a, p[0], p[1], p[2])] - vel[(uint32_t)a * g.pool_cells + c.idx];
+    }
+    float ws = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        int p[3] = {c.x, c.y, c.z};
+        p[a] += 1;
+        ws += fw[(uint32_t)a * g.pool_cells + c.idx] + fw[face_index(g, a, p[0], p[1], p[2])];
+    }
+    float rho = ws / (6.0f * kTargetPpc);
+    float excess = fmaxf(rho - (1.0f + deadband), 0.0f);
+    float target = fminf(dc * excess, 0.5f) * h_over_dt;
+    div[c.idx] = dv - target;
+    float dg = 0.0f;
+    for (int nb = 0; nb < 6; ++nb) {
+        int d[3];
+        nb_offset(nb, d);
+        int nx = c.x + d[0], ny = c.y + d[1], nz = c.z + d[2];
+        uint32_t t = type_at(g, ctype, nx, ny, nz) & kCellTypeMask;
+        dg += t == kCellFluid ? 1.0f : (t == kCellAir ? air_face_weight(g, fw, ghost, rho, nx, ny, nz) : 0.0f);
+    }
+    diag[c.idx] = dg;
+}
+
+// Slosh.usf:995-1052 (ProjectFace) on the FP64 pressure, without collider-coupling accumulation.
+__global__ __launch_bounds__(512) void k_project(Geo g, float *vel, const float *fw, const uint32_t *ctype,
+                                                 const double *pres, uint8_t *fflags, int ghost)
+{
+    CellPos c;
+    if (!cell_of_block(g, c)) return;
+    const uint32_t tb = ctype[c.idx] & kCellTypeMask;
+    for (int a = 0; a < 3; ++a) {
+        uint32_t f = (uint32_t)a * g.pool_cells + c.idx;
+        int ca[3] = {c.x, c.y, c.z};
+        ca[a] -= 1;
+        uint32_t ta = type_at(g, ctype, ca[0], ca[1], ca[2]) & kCellTypeMask;
+        uint32_t flags = (uint32_t)fflags[f] & kFaceHasWeight;
+        if (ta == kCellSolid && tb == kCellSolid) {
+        } else if (ta == kCellSolid || tb == kCellSolid) {
+            flags |= kFaceValid | kFaceSolid;
+        } else if (ta == kCellFluid || tb == kCellFluid) {
+            double pa, pb;
+            if (ta == kCellFluid && tb == kCellFluid) {
+                pa = pres[cell_index(g, ca[0], ca[1], ca[2])];
+                pb = pres[c.idx];
+            } else if (ta == kCellFluid) {
+                pa = pres[cell_index(g, ca[0], ca[1], ca[2])];
+                pb = pa - pa * (double)air_face_weight(g, fw, ghost, cell_density(g, fw, ca[0], ca[1], ca[2]), c.x, c.y, c.z);
+            } else {
+                pb = pres[c.idx];
+                pa = pb - pb * (double)air_face_weight(g, fw, ghost, cell_density(g, fw, c.x, c.y, c.z), ca[0], ca[1], ca[2]);
+            }
+            vel[f] -= (float)(pb - pa);
+            flags |= kFaceValid;
+        }
+        fflags[f] = (uint8_t)flags;
+    }
+}
+
+// Slosh.usf:1222-1240 (ResidualCS): max and sum of |div|/h over fluid cells after projection. The sum is reduced
+// per workgroup into partials (FP64) and summed in a fixed order by the last workgroup.
+__global__ __launch_bounds__(512) void k_div_stats(Geo g, const float *vel, const uint32_t *ctype, double *partials,
+                                                   uint32_t *counter, StepStats *stats)
+{
+    __shared__ double ssum[512];
+    __shared__ uint32_t smax, scount;
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        smax = 0u;
+        scount = 0u;
+    }
+    __syncthreads();
+    CellPos c;
+    double rate_d = 0.0;
+    if (cell_of_block(g, c) && (ctype[c.idx] & kCellTypeMask) == kCellFluid) {
+        float dv = 0.0f;
+        for (int a = 0; a < 3; ++a) {
+            int p[3] = {c.x, c.y, c.z};
+            p[a] += 1;
+            dv += vel[face_index(g, a, p[0], p[1], p[2])] - vel[(uint32_t)a * g.pool_cells + c.idx];
+        }
+        float rate = fabsf(dv) * g.inv_h;
+        atomicMax(&smax, __float_as_uint(rate));
+        atomicAdd(&scount, 1u);
+        rate_d = (double)rate;
+    }
+    ssum[threadIdx.x] = rate_d;
+    __syncthreads();
+    for (uint32_t s = 256; s > 0; s >>= 1) {
+        if (threadIdx.x < s) ssum[threadIdx.x] += ssum[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = ssum[0];
+        if (smax) atomicMax(&stats->max_div_bits, smax);
+        if (scount) atomicAdd(&stats->div_count, scount);
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc = 0.0;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) acc += partials[b];
+    ssum[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t s = 256; s > 0; s >>= 1) {
+        if (threadIdx.x < s) ssum[threadIdx.x] += ssum[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        stats->div_sum = ssum[0];
+        *counter = 0u;
+    }
+}
+
+// Slosh.usf:1263-1310 (ExtrapolateFace) for layer `layer`: src -> dst, marking filled faces with bit 4+layer.
+__global__ __launch_bounds__(512) void k_extrapolate(Geo g, const float *src, float *dst, uint8_t *fflags, uint32_t layer)
+{
+    CellPos c;
+    if (!cell_of_block(g, c)) return;
+    uint32_t known = kFaceValid;
+    for (uint32_t k = 0; k < layer; ++k) known |= 1u << (kFaceExtrapShift + k);
+    for (int a = 0; a < 3; ++a) {
+        uint32_t f = (uint32_t)a * g.pool_cells + c.idx;
+        uint32_t flags = fflags[f];
+        float center = src[f];
+        if (flags & known) {
+            dst[f] = center;
+            continue;
+        }
+        float sum = 0.0f, cnt = 0.0f;
+        for (int nb = 0; nb < 6; ++nb) {
+            int d[3];
+            nb_offset(nb, d);
+            int nx = c.x + d[0], ny = c.y + d[1], nz = c.z + d[2];
+            if (!in_grid(g, nx, ny, nz)) continue;
+            uint32_t ni = face_index(g, a, nx, ny, nz);
+            uint32_t nf = fflags[ni];
+            if ((nf & known) != 0u && (nf & kFaceSolid) == 0u) {
+                sum += src[ni];
+                cnt += 1.0f;
+            }
+        }
+        if (cnt > 0.0f) {
+            dst[f] = sum / cnt;
+            fflags[f] = (uint8_t)(flags | (1u << (kFaceExtrapShift + layer)));
+        } else {
+            dst[f] = 0.0f;
+        }
+    }
+}
+
+} // namespace
+
+void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
+    SLOSH_HIP(hipGetLastError());
+}
+void launch_fill_u8(hipStream_t st, uint8_t *p, uint8_t v, size_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
+    SLOSH_HIP(hipGetLastError());
+}
+void launch_fill_f64(hipStream_t st, double *p, double v, size_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_f64, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
+    SLOSH_HIP(hipGetLastError());
+}
+
+// out[0..n] = exclusive prefix sums of in[0..n), out[n] = total. Supports n <= 256 * 1024 * 1024.
+void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n)
+{
+    const uint32_t nb = n ? blocks_for(n, kScanBlock) : 1u;
+    {
+        SLOSH_TIMED(c, K_SCAN);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, in, n, c->scan_blocks);
+    }
+    {
+        SLOSH_TIMED(c, K_SCAN);
+        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(256), 0, c->stream, c->scan_blocks, nb);
+    }
+    {
+        SLOSH_TIMED(c, K_SCAN);
+        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, c->stream, in, n, c->scan_blocks, out);
+    }
+    SLOSH_HIP(hipGetLastError());
+}
+
+void run_pool(slosh_ctx *c)
+{
+    const Geo &g = c->geo;
+    uint32_t *table = c->table;
+    {
+        SLOSH_TIMED(c, K_POOL);
+        hipLaunchKernelGGL(k_pool_want, dim3(blocks_for(g.nbricks, 256)), dim3(256), 0, c->stream, g, c->touch, table,
+                           c->slot_brick, c->need);
+    }
+    {
+        SLOSH_TIMED(c, K_POOL);
+        hipLaunchKernelGGL(k_pool_freeflags, dim3(blocks_for(g.npool, 256)), dim3(256), 0, c->stream, c->slot_brick, g.npool,
+                           c->freeflag);
+    }
+    SLOSH_HIP(hipGetLastError());
+    scan_exclusive(c, c->need, c->need_rank, g.nbricks);
+    scan_exclusive(c, c->freeflag, c->free_rank, g.npool);
+    {
+        SLOSH_TIMED(c, K_POOL);
+        hipLaunchKernelGGL(k_pool_free_list, dim3(blocks_for(g.npool, 256)), dim3(256), 0, c->stream, c->freeflag,
+                           c->free_rank, g.npool, c->free_list);
+    }
+    {
+        SLOSH_TIMED(c, K_POOL);
+        hipLaunchKernelGGL(k_pool_assign, dim3(blocks_for(g.nbricks, 256)), dim3(256), 0, c->stream, g, c->need,
+                           c->need_rank, c->free_list, c->free_rank, g.npool, table, c->slot_brick,
+                           c->new_slot, c->touch, c->alloc_flag,
+                           c->stats);
+    }
+    SLOSH_HIP(hipGetLastError());
+    scan_exclusive(c, c->alloc_flag, c->alloc_rank, g.nbricks);
+    {
+        SLOSH_TIMED(c, K_POOL);
+        hipLaunchKernelGGL(k_pool_active, dim3(blocks_for(g.nbricks, 256)), dim3(256), 0, c->stream, g, c->alloc_flag,
+                           c->alloc_rank, c->active, c->stats);
+    }
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_normalize(slosh_ctx *c, float dt, const float g[3])
+{
+    if (!c->nactive) return;
+    c->vcur = 0;
+    {
+        SLOSH_TIMED(c, K_NORMALIZE);
+        hipLaunchKernelGGL(k_normalize, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->tiles, c->tile_valid,
+                           c->ctype, c->vel[0], c->fweight, c->fflags, g[0] * dt, g[1] * dt, g[2] * dt, c->d_col);
+    }
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_divergence(slosh_ctx *c, float dt)
+{
+    if (!c->nactive) return;
+    {
+        SLOSH_TIMED(c, K_DIVERGENCE);
+        hipLaunchKernelGGL(k_divergence, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->fweight,
+                           c->ctype, c->div, c->diag, c->s.ghost_fluid, c->s.density_correction, c->s.density_deadband,
+                           c->geo.h / dt);
+    }
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_project(slosh_ctx *c)
+{
+    if (!c->nactive) return;
+    {
+        SLOSH_TIMED(c, K_PROJECT);
+        hipLaunchKernelGGL(k_project, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->fweight,
+                           c->ctype, c->pres, c->fflags, c->s.ghost_fluid);
+    }
+    {
+        SLOSH_TIMED(c, K_DIV_STATS);
+        hipLaunchKernelGGL(k_div_stats, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->ctype,
+                           c->partials, c->red_counter, c->stats);
+    }
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_extrapolate(slosh_ctx *c, uint32_t layer)
+{
+    if (!c->nactive) return;
+    int dst = (c->vcur + 1) % 3;
+    {
+        SLOSH_TIMED(c, K_EXTRAPOLATE);
+        hipLaunchKernelGGL(k_extrapolate, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->vel[dst],
+                           c->fflags, layer);
+    }
+    SLOSH_HIP(hipGetLastError());
+    c->vcur = dst;
+}
+
+} // namespace slosh

@@ -1,0 +1,647 @@
+// particles.hip -- particle kernels of the solver step on gfx950: emission, stable brick sort, P2G, G2P.
+//
+// Launch shapes are wave64-native: 256-thread workgroups (4 waves) for particle passes and one workgroup per
+// allocated 8^3 brick for the P2G gather, whose LDS tile holds every node the brick's particles touch.
+#include "slosh_ctx.h"
+
+namespace slosh {
+
+namespace {
+
+constexpr uint32_t kTile = 256; // particles per sort tile (one workgroup)
+
+__device__ __forceinline__ uint32_t key_of(const Geo &g, float4 p, uint32_t dead_key)
+{
+    if (!(p.w >= 0.0f)) return dead_key;
+    int c[3];
+    pos_cell(g, p.x, p.y, p.z, c);
+    return brick_of_cell(g, c[0], c[1], c[2]);
+}
+
+__device__ __forceinline__ void touch_brick(const Geo &g, uint8_t *touch, float x, float y, float z)
+{
+    int c[3];
+    pos_cell(g, x, y, z, c);
+    touch[brick_of_cell(g, c[0], c[1], c[2])] = 1u;
+}
+
+// Slosh.usf:280-358 (EmitParticlesCS), mode 0 FillCells.
+__global__ void k_emit_fill(Geo g, Parts p, uint32_t base, uint32_t count, slosh_emit_request r, uint8_t *touch)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t h0 = pcg_hash(i * 3u + r.seed * 7919u);
+    uint32_t h1 = pcg_hash(h0);
+    uint32_t h2 = pcg_hash(h1);
+    float rnd[3] = {hash_to_unit(h0), hash_to_unit(h1), hash_to_unit(h2)};
+    uint32_t dx = (uint32_t)r.cell_dims[0], dy = (uint32_t)r.cell_dims[1];
+    uint32_t cl = i / 8u, sub = i % 8u;
+    int local[3] = {(int)(cl % dx), (int)((cl / dx) % dy), (int)(cl / (dx * dy))};
+    float so[3] = {(float)(sub & 1u), (float)((sub >> 1u) & 1u), (float)((sub >> 2u) & 1u)};
+    float lo = g.h * 1.01f;
+    float pos[3];
+    for (int a = 0; a < 3; ++a) {
+        float hi = ((float)g.n[a] - 1.01f) * g.h;
+        float in_cell = ((so[a] + 0.5f) + 0.8f * (rnd[a] - 0.5f)) * 0.5f;
+        float v = ((float)(r.cell_min[a] + local[a]) + in_cell) * g.h;
+        pos[a] = clampf(v, lo, hi);
+    }
+    uint32_t j = base + i;
+    p.pa[j] = make_float4(pos[0], pos[1], pos[2], 0.0f);
+    for (int a = 0; a < 3; ++a) p.v[a][j] = r.velocity[a];
+    for (int k = 0; k < 9; ++k) p.c[k][j] = 0.0f;
+    touch_brick(g, touch, pos[0], pos[1], pos[2]);
+}
+
+// Resume from 64-byte records: positions are clamped into the particle band [1.01h, (N-1.01)h] of Slosh.usf:1459.
+__global__ void k_unpack(Geo g, Parts p, const slosh_particle *src, uint32_t n, uint8_t *touch)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    slosh_particle s = src[i];
+    float lo = g.h * 1.01f;
+    float pos[3];
+    for (int a = 0; a < 3; ++a) pos[a] = clampf(s.pos[a], lo, ((float)g.n[a] - 1.01f) * g.h);
+    p.pa[i] = make_float4(pos[0], pos[1], pos[2], s.age);
+    for (int a = 0; a < 3; ++a) p.v[a][i] = s.vel[a];
+    for (int k = 0; k < 9; ++k) p.c[k][i] = s.c[k];
+    if (s.age >= 0.0f) touch_brick(g, touch, pos[0], pos[1], pos[2]);
+}
+
+// SloshCommon.ush:71-88 (PackHalf2): clamp to +-65000 then round to nearest even half.
+__device__ __forceinline__ uint32_t pack_half2(float a, float b)
+{
+    _Float16 ha = (_Float16)clampf(a, -65000.0f, 65000.0f);
+    _Float16 hb = (_Float16)clampf(b, -65000.0f, 65000.0f);
+    return (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+}
+
+__global__ void k_pack(Parts p, uint32_t n, void *dst, int layout)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 pa = p.pa[i];
+    if (layout == SLOSH_LAYOUT_F32_64) {
+        slosh_particle o;
+        o.pos[0] = pa.x; o.pos[1] = pa.y; o.pos[2] = pa.z; o.age = pa.w;
+        for (int a = 0; a < 3; ++a) o.vel[a] = p.v[a][i];
+        for (int k = 0; k < 9; ++k) o.c[k] = p.c[k][i];
+        ((slosh_particle *)dst)[i] = o;
+    } else {
+        slosh_particle_ref48 o;
+        o.pos[0] = pa.x; o.pos[1] = pa.y; o.pos[2] = pa.z; o.age = pa.w;
+        for (int a = 0; a < 3; ++a) o.vel[a] = p.v[a][i];
+        float cm[9];
+        for (int k = 0; k < 9; ++k) cm[k] = p.c[k][i];
+        o.c[0] = pack_half2(cm[0], cm[1]);
+        o.c[1] = pack_half2(cm[2], cm[3]);
+        o.c[2] = pack_half2(cm[4], cm[5]);
+        o.c[3] = pack_half2(cm[6], cm[7]);
+        o.c[4] = pack_half2(cm[8], 0.0f);
+        ((slosh_particle_ref48 *)dst)[i] = o;
+    }
+}
+
+// Block-wide bitonic sort of `size` (power of two) u32 in LDS, ascending.
+__device__ void bitonic_lds(uint32_t *s, uint32_t size)
+{
+    for (uint32_t k = 2; k <= size; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < size; t += blockDim.x) {
+                uint32_t ixj = t ^ j;
+                if (ixj > t) {
+                    uint32_t a = s[t], b = s[ixj];
+                    bool up = (t & k) == 0u;
+                    if ((a > b) == up) {
+                        s[t] = b;
+                        s[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Sort pass 1. Per tile of 256 consecutive particles: key = 8^3 brick of the clamped cell (dead: last key), a
+// stable in-tile rank per key from an LDS bitonic sort of (key << 8 | lane), and the tile's runs of equal keys.
+// Reference counterpart: SortCountCS (SloshSort.usf:65-76), whose atomic rank is not stable.
+__global__ __launch_bounds__(256) void k_sort_tiles(Geo g, const float4 *pa, uint32_t np, uint32_t dead_key,
+                                                    uint32_t *packed, uint32_t *tile_nruns, uint32_t *run_key,
+                                                    uint32_t *run_len, uint32_t *key_count, uint32_t *key_runs)
+{
+    __shared__ uint32_t s[kTile];
+    __shared__ uint32_t sc[kTile];
+    __shared__ uint32_t start[kTile];
+    const uint32_t t = threadIdx.x;
+    const uint32_t i = blockIdx.x * kTile + t;
+    uint32_t v = 0xFFFFFFFFu;
+    if (i < np) v = (key_of(g, pa[i], dead_key) << 8) | t;
+    s[t] = v;
+    __syncthreads();
+    bitonic_lds(s, kTile);
+    const uint32_t e = s[t];
+    const bool valid = e != 0xFFFFFFFFu;
+    const uint32_t k = e >> 8;
+    const bool flag = valid && (t == 0u || (s[t - 1] >> 8) != k);
+    sc[t] = flag ? 1u : 0u;
+    __syncthreads();
+    for (uint32_t off = 1; off < kTile; off <<= 1) {
+        uint32_t add = t >= off ? sc[t - off] : 0u;
+        __syncthreads();
+        sc[t] += add;
+        __syncthreads();
+    }
+    const uint32_t nruns = sc[kTile - 1];
+    const uint32_t run = sc[t] - 1u;
+    if (flag) start[run] = t;
+    __syncthreads();
+    const uint32_t nvalid = np - blockIdx.x * kTile < kTile ? np - blockIdx.x * kTile : kTile;
+    if (valid) packed[blockIdx.x * kTile + (e & 255u)] = (run << 8) | (t - start[run]);
+    if (flag) {
+        uint32_t next = run + 1u < nruns ? start[run + 1u] : nvalid;
+        uint32_t len = next - t;
+        uint32_t gid = blockIdx.x * kTile + run;
+        run_key[gid] = k;
+        run_len[gid] = len;
+        atomicAdd(&key_count[k], len);
+        atomicAdd(&key_runs[k], 1u);
+    }
+    if (t == 0u) tile_nruns[blockIdx.x] = nruns;
+}
+
+// Sort pass 2: bucket each run under its key (arbitrary order within the key, fixed by pass 3).
+__global__ __launch_bounds__(256) void k_sort_runs_scatter(const uint32_t *tile_nruns, const uint32_t *run_key,
+                                                           const uint32_t *run_off, uint32_t *run_cursor,
+                                                           uint32_t *run_sorted)
+{
+    uint32_t r = threadIdx.x;
+    if (r >= tile_nruns[blockIdx.x]) return;
+    uint32_t gid = blockIdx.x * kTile + r;
+    uint32_t k = run_key[gid];
+    uint32_t pos = run_off[k] + atomicAdd(&run_cursor[k], 1u);
+    run_sorted[pos] = gid;
+}
+
+constexpr uint32_t kRunLds = 2048;
+
+// Sort pass 3, one workgroup per key: order the key's runs by tile (run ids grow with the tile) and give each run
+// its base = key offset + particles of this key in earlier tiles. Together with the in-tile rank this is the
+// stable order by (key, old index).
+__global__ __launch_bounds__(256) void k_sort_key_runs(const uint32_t *run_off, const uint32_t *key_off,
+                                                       const uint32_t *run_len, uint32_t *run_sorted, uint32_t *run_base)
+{
+    __shared__ uint32_t s[kRunLds];
+    __shared__ uint32_t part[256];
+    const uint32_t k = blockIdx.x;
+    const uint32_t b0 = run_off[k], m = run_off[k + 1] - b0;
+    if (m == 0u) return;
+    const uint32_t kb = key_off[k];
+    if (m == 1u) {
+        if (threadIdx.x == 0) run_base[run_sorted[b0]] = kb;
+        return;
+    }
+    if (m <= kRunLds) {
+        uint32_t size = 2;
+        while (size < m) size <<= 1;
+        for (uint32_t t = threadIdx.x; t < size; t += blockDim.x) s[t] = t < m ? run_sorted[b0 + t] : 0xFFFFFFFFu;
+        __syncthreads();
+        bitonic_lds(s, size);
+        // exclusive scan of run lengths in sorted order: contiguous chunk per thread, block scan of chunk sums
+        const uint32_t per = (m + blockDim.x - 1) / blockDim.x;
+        const uint32_t beg = threadIdx.x * per;
+        uint32_t sum = 0;
+        for (uint32_t j = beg; j < beg + per && j < m; ++j) sum += run_len[s[j]];
+        part[threadIdx.x] = sum;
+        __syncthreads();
+        for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+            uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        uint32_t run_acc = part[threadIdx.x] - sum;
+        for (uint32_t j = beg; j < beg + per && j < m; ++j) {
+            uint32_t gid = s[j];
+            run_base[gid] = kb + run_acc;
+            run_acc += run_len[gid];
+        }
+        return;
+    }
+    // Rare fallback for keys split over more than kRunLds tiles: base = sum of lengths of runs from earlier tiles.
+    for (uint32_t j = threadIdx.x; j < m; j += blockDim.x) {
+        uint32_t gid = run_sorted[b0 + j];
+        uint32_t acc = 0;
+        for (uint32_t q = 0; q < m; ++q) {
+            uint32_t o = run_sorted[b0 + q];
+            if (o < gid) acc += run_len[o];
+        }
+        run_base[gid] = kb + acc;
+    }
+}
+
+// Sort pass 4: move each live particle's 64-byte record to its stable position (SortScatterCS, SloshSort.usf:135).
+__global__ __launch_bounds__(256) void k_sort_scatter(Parts src, Parts dst, uint32_t np, const uint32_t *packed,
+                                                      const uint32_t *run_key, const uint32_t *run_base,
+                                                      uint32_t dead_key)
+{
+    uint32_t i = blockIdx.x * kTile + threadIdx.x;
+    if (i >= np) return;
+    uint32_t pk = packed[i];
+    uint32_t gid = blockIdx.x * kTile + (pk >> 8);
+    if (run_key[gid] == dead_key) return;
+    uint32_t j = run_base[gid] + (pk & 255u);
+    dst.pa[j] = src.pa[i];
+    for (int a = 0; a < 3; ++a) dst.v[a][j] = src.v[a][i];
+    for (int q = 0; q < 9; ++q) dst.c[q][j] = src.c[q][i];
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// P2G (Slosh.usf:448-563) as a per-brick gather: one workgroup per allocated brick reads the brick's sorted
+// particles, accumulates the int32 fixed-point momentum and weight of every node they touch into an LDS tile
+// (nodes [brick_min - 1, brick_min + 9) per axis), and stores the tile; the normalise pass sums the up to 8 tiles
+// that overlap each face. No global atomics; integer sums make the result independent of order. The same
+// workgroup counts particles per cell and classifies its 512 cells (BuildSolidCS :365-395, MarkCellsCS :569-598).
+__global__ __launch_bounds__(256) void k_p2g(Geo g, Parts p, const uint32_t *key_off, int32_t *tiles,
+                                             uint8_t *tile_valid, uint32_t *ctype, double *pres, uint8_t *new_slot,
+                                             float warm, const slosh_collider *col, uint32_t ncol, StepStats *stats)
+{
+    __shared__ int32_t tile[kTileInts];
+    __shared__ int32_t cnt[kBrickCells];
+    __shared__ uint32_t fluid_sum;
+    const uint32_t b = g.active[blockIdx.x];
+    const uint32_t slot = g.table[b];
+    const bool fresh = new_slot[slot] != 0u; // recycled slot: its old pressure is not a warm start (ClearNewSlotsCS)
+    int bc[3];
+    brick_coord(g, b, bc);
+    const int ox = bc[0] * 8 - 1, oy = bc[1] * 8 - 1, oz = bc[2] * 8 - 1;
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileInts; t += blockDim.x) tile[t] = 0;
+    for (uint32_t t = threadIdx.x; t < kBrickCells; t += blockDim.x) cnt[t] = 0;
+    if (threadIdx.x == 0) fluid_sum = 0u;
+    __syncthreads();
+    const uint32_t i0 = key_off[b], i1 = key_off[b + 1];
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        float4 pa = p.pa[i];
+        float xp[3] = {pa.x, pa.y, pa.z};
+        for (int axis = 0; axis < 3; ++axis) {
+            float up = p.v[axis][i];
+            float cp0 = p.c[3 * axis][i], cp1 = p.c[3 * axis + 1][i], cp2 = p.c[3 * axis + 2][i];
+            float off[3] = {0.5f, 0.5f, 0.5f};
+            off[axis] = 0.0f;
+            float gg[3], fr[3];
+            int base[3];
+            for (int a = 0; a < 3; ++a) {
+                gg[a] = xp[a] * g.inv_h - off[a];
+                base[a] = (int)floorf(gg[a]);
+                fr[a] = gg[a] - (float)base[a];
+            }
+            for (uint32_t oi = 0; oi < 8u; ++oi) {
+                int od[3] = {(int)(oi & 1u), (int)((oi >> 1) & 1u), (int)((oi >> 2) & 1u)};
+                int node[3] = {base[0] + od[0], base[1] + od[1], base[2] + od[2]};
+                if (!in_grid(g, node[0], node[1], node[2])) continue;
+                float w3[3];
+                for (int a = 0; a < 3; ++a) w3[a] = od[a] ? fr[a] : 1.0f - fr[a];
+                float w = w3[0] * w3[1] * w3[2];
+                if (w <= 0.0f) continue;
+                float dxn[3];
+                for (int a = 0; a < 3; ++a) dxn[a] = ((float)node[a] + off[a]) * g.h - xp[a];
+                float dot = cp0 * dxn[0] + cp1 * dxn[1] + cp2 * dxn[2];
+                int32_t mom = (int32_t)rintf(w * (up + dot) * kFixedScale);
+                int32_t wt = (int32_t)rintf(w * kFixedScale);
+                int tx = node[0] - ox, ty = node[1] - oy, tz = node[2] - oz;
+                uint32_t li = (uint32_t)(tx + kTileDim * (ty + kTileDim * tz));
+                atomicAdd(&tile[axis * kTileNodes + li], mom);
+                atomicAdd(&tile[(3 + axis) * kTileNodes + li], wt);
+            }
+        }
+        int c[3];
+        pos_cell(g, xp[0], xp[1], xp[2], c);
+        atomicAdd(&cnt[local_of_cell(c[0], c[1], c[2])], 1);
+    }
+    __syncthreads();
+    const bool any = i1 > i0;
+    if (threadIdx.x == 0) tile_valid[slot] = any ? 1u : 0u;
+    if (any) {
+        int32_t *dst = tiles + (size_t)slot * kTileInts;
+        for (uint32_t t = threadIdx.x; t < (uint32_t)kTileInts; t += blockDim.x) dst[t] = tile[t];
+    }
+    uint32_t fluid = 0;
+    for (uint32_t l = threadIdx.x; l < kBrickCells; l += blockDim.x) {
+        int x = bc[0] * 8 + (int)(l & 7u), y = bc[1] * 8 + (int)((l >> 3) & 7u), z = bc[2] * 8 + (int)(l >> 6);
+        if (!in_grid(g, x, y, z)) continue;
+        uint32_t packed = kCellAir;
+        if (x == 0 || y == 0 || z == 0 || x == g.n[0] - 1 || y == g.n[1] - 1 || z == g.n[2] - 1) {
+            packed = kCellSolid;
+        } else {
+            float pc[3] = {((float)x + 0.5f) * g.h, ((float)y + 0.5f) * g.h, ((float)z + 0.5f) * g.h};
+            float best = 0.0f;
+            for (uint32_t k = 0; k < ncol; ++k) {
+                float dd = collider_sdf(col[k], pc);
+                if (dd < best) {
+                    best = dd;
+                    packed = kCellSolid | ((k + 1u) << kColliderShift);
+                }
+            }
+        }
+        if ((packed & kCellTypeMask) != kCellSolid) {
+            int32_t count = cnt[l];
+            packed = count > 0 ? kCellFluid : kCellAir;
+            if ((float)count >= 0.5f * kTargetPpc) packed |= kCellDense;
+        }
+        uint32_t idx = slot * kBrickCells + l;
+        ctype[idx] = packed;
+        if ((packed & kCellTypeMask) == kCellFluid) {
+            pres[idx] = (fresh ? 0.0 : pres[idx]) * (double)warm;
+            fluid++;
+        } else {
+            pres[idx] = 0.0;
+        }
+    }
+    if (fluid) atomicAdd(&fluid_sum, fluid);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (fluid_sum) atomicAdd(&stats->fluid_cells, fluid_sum);
+        if (fresh) new_slot[slot] = 0u;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t face_index(const Geo &g, int axis, int x, int y, int z)
+{
+    return (uint32_t)axis * g.pool_cells + cell_index(g, x, y, z);
+}
+
+// Slosh.usf:1376-1410 (GatherComponent, APIC).
+__device__ __forceinline__ void gather_component(const Geo &g, const float *vel, int axis, const float xp[3],
+                                                 float &unew, float grad[3])
+{
+    float off[3] = {0.5f, 0.5f, 0.5f};
+    off[axis] = 0.0f;
+    float gg[3], fr[3];
+    int base[3];
+    for (int a = 0; a < 3; ++a) {
+        gg[a] = xp[a] * g.inv_h - off[a];
+        base[a] = (int)floorf(gg[a]);
+        fr[a] = gg[a] - (float)base[a];
+    }
+    float u_acc = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+    for (uint32_t oi = 0; oi < 8u; ++oi) {
+        int od[3] = {(int)(oi & 1u), (int)((oi >> 1) & 1u), (int)((oi >> 2) & 1u)};
+        int node[3];
+        float w3[3], sg[3];
+        for (int a = 0; a < 3; ++a) {
+            node[a] = clampi(base[a] + od[a], 0, g.n[a] - 1);
+            w3[a] = od[a] ? fr[a] : 1.0f - fr[a];
+            sg[a] = od[a] ? 1.0f : -1.0f;
+        }
+        float w = w3[0] * w3[1] * w3[2];
+        float gw0 = sg[0] * (w3[1] * w3[2]) * g.inv_h;
+        float gw1 = sg[1] * (w3[0] * w3[2]) * g.inv_h;
+        float gw2 = sg[2] * (w3[0] * w3[1]) * g.inv_h;
+        float u = vel[face_index(g, axis, node[0], node[1], node[2])];
+        u_acc += w * u;
+        g0 += gw0 * u;
+        g1 += gw1 * u;
+        g2 += gw2 * u;
+    }
+    unew = u_acc;
+    grad[0] = g0;
+    grad[1] = g1;
+    grad[2] = g2;
+}
+
+// Slosh.usf:1417-1443 (SampleVelocityComponent).
+__device__ __forceinline__ float sample_component(const Geo &g, const float *vel, const uint8_t *fflags, int axis,
+                                                  const float xp[3], float fallback, bool check)
+{
+    const uint32_t known = kFaceValid | (0xFFFFFFFFu << kFaceExtrapShift);
+    float off[3] = {0.5f, 0.5f, 0.5f};
+    off[axis] = 0.0f;
+    float gg[3], fr[3];
+    int base[3];
+    for (int a = 0; a < 3; ++a) {
+        gg[a] = xp[a] * g.inv_h - off[a];
+        base[a] = (int)floorf(gg[a]);
+        fr[a] = gg[a] - (float)base[a];
+    }
+    float u = 0.0f;
+    for (uint32_t oi = 0; oi < 8u; ++oi) {
+        int od[3] = {(int)(oi & 1u), (int)((oi >> 1) & 1u), (int)((oi >> 2) & 1u)};
+        int node[3];
+        float w3[3];
+        for (int a = 0; a < 3; ++a) {
+            node[a] = clampi(base[a] + od[a], 0, g.n[a] - 1);
+            w3[a] = od[a] ? fr[a] : 1.0f - fr[a];
+        }
+        uint32_t f = face_index(g, axis, node[0], node[1], node[2]);
+        float uf = vel[f];
+        if (check && ((uint32_t)fflags[f] & known) == 0u) uf = fallback;
+        u += w3[0] * w3[1] * w3[2] * uf;
+    }
+    return u;
+}
+
+// Slosh.usf:1451-1678 (AdvectParticle + G2PAdvectCS), APIC; manipulators, adhesion and drains are off.
+__global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, uint32_t np, const float *vel, const uint8_t *fflags,
+                                             float dt, float gx, float gy, float gz, float friction, float radius,
+                                             float max_speed, const slosh_collider *col, uint32_t ncol,
+                                             uint8_t *touch, StepStats *stats)
+{
+    __shared__ uint32_t tb[256];
+    __shared__ uint32_t smax, sclamp;
+    if (threadIdx.x == 0) {
+        smax = 0u;
+        sclamp = 0u;
+    }
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t brick = kNoBrick;
+    if (i < np) {
+        float4 pa = p.pa[i];
+        if (pa.w >= 0.0f) {
+            const float grav[3] = {gx, gy, gz};
+            float pos[3] = {pa.x, pa.y, pa.z};
+            float pic[3], cm[9];
+            gather_component(g, vel, 0, pos, pic[0], &cm[0]);
+            gather_component(g, vel, 1, pos, pic[1], &cm[3]);
+            gather_component(g, vel, 2, pos, pic[2], &cm[6]);
+            float nv[3] = {pic[0], pic[1], pic[2]};
+            float lo = g.h * 1.01f;
+            float hi[3];
+            for (int a = 0; a < 3; ++a) hi[a] = ((float)g.n[a] - 1.01f) * g.h;
+            float hdt = 0.5f * dt;
+            float mid[3];
+            for (int a = 0; a < 3; ++a) mid[a] = clampf(pos[a] + hdt * pic[a], lo, hi[a]);
+            float midcells = hdt * len3(pic[0], pic[1], pic[2]) * g.inv_h;
+            float v2[3] = {pic[0], pic[1], pic[2]};
+            if (midcells < 2.0f) {
+                bool check = midcells > 1.0f;
+                float t = clampf(midcells - 1.0f, 0.0f, 1.0f);
+                for (int a = 0; a < 3; ++a) {
+                    float sv = sample_component(g, vel, fflags, a, mid, pic[a], check);
+                    v2[a] = sv + t * (pic[a] - sv);
+                }
+            }
+            float x[3];
+            for (int a = 0; a < 3; ++a) x[a] = pos[a] + dt * v2[a];
+            int c[3];
+            pos_cell(g, pos[0], pos[1], pos[2], c);
+            if (cell_index(g, c[0], c[1], c[2]) < kBrickCells) {
+                for (int a = 0; a < 3; ++a) {
+                    nv[a] = p.v[a][i] + grav[a] * dt;
+                    x[a] = pos[a] + dt * nv[a];
+                }
+                for (int k = 0; k < 9; ++k) cm[k] = 0.0f;
+            }
+            for (uint32_t k = 0; k < ncol; ++k) {
+                const slosh_collider cc = col[k];
+                float dd = collider_sdf(cc, x);
+                if (dd < radius) {
+                    float nn[3];
+                    collider_normal(cc, x, 0.25f * g.h, nn);
+                    float push = radius - dd;
+                    for (int a = 0; a < 3; ++a) x[a] = x[a] + nn[a] * push;
+                    float vs[3] = {collider_velocity_axis(cc, x, 0), collider_velocity_axis(cc, x, 1),
+                                   collider_velocity_axis(cc, x, 2)};
+                    float rel[3] = {nv[0] - vs[0], nv[1] - vs[1], nv[2] - vs[2]};
+                    float vn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
+                    if (vn < 0.0f) {
+                        for (int a = 0; a < 3; ++a) rel[a] = rel[a] - vn * nn[a];
+                        float rn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
+                        float tg[3];
+                        for (int a = 0; a < 3; ++a) tg[a] = rel[a] - rn * nn[a];
+                        for (int a = 0; a < 3; ++a) rel[a] = rel[a] - friction * tg[a];
+                    }
+                    for (int a = 0; a < 3; ++a) nv[a] = vs[a] + rel[a];
+                }
+            }
+            uint32_t contact = 0;
+            for (int a = 0; a < 3; ++a) {
+                if (x[a] < lo) {
+                    x[a] = lo;
+                    nv[a] = fmaxf(nv[a], 0.0f);
+                    contact |= 1u << a;
+                } else if (x[a] > hi[a]) {
+                    x[a] = hi[a];
+                    nv[a] = fminf(nv[a], 0.0f);
+                    contact |= 1u << a;
+                }
+            }
+            if (contact != 0u && friction > 0.0f)
+                for (int a = 0; a < 3; ++a)
+                    if ((contact & (1u << a)) == 0u) nv[a] *= 1.0f - friction;
+            float speed = len3(nv[0], nv[1], nv[2]);
+            if (speed > max_speed) atomicAdd(&sclamp, 1u);
+            p.pa[i] = make_float4(x[0], x[1], x[2], pa.w + dt);
+            for (int a = 0; a < 3; ++a) p.v[a][i] = nv[a];
+            for (int k = 0; k < 9; ++k) p.c[k][i] = cm[k];
+            atomicMax(&smax, __float_as_uint(speed));
+            pos_cell(g, x[0], x[1], x[2], c);
+            brick = brick_of_cell(g, c[0], c[1], c[2]);
+        }
+    }
+    tb[threadIdx.x] = brick;
+    __syncthreads();
+    if (brick != kNoBrick && (threadIdx.x == 0 || tb[threadIdx.x - 1] != brick)) touch[brick] = 1u;
+    if (threadIdx.x == 0) {
+        if (smax) atomicMax(&stats->max_speed_bits, smax);
+        if (sclamp) atomicAdd(&stats->clamped, sclamp);
+    }
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+} // namespace
+
+void launch_emit_fill(slosh_ctx *c, const slosh_emit_request *r, uint32_t count)
+{
+    if (!count) return;
+    {
+        SLOSH_TIMED(c, K_EMIT);
+        hipLaunchKernelGGL(k_emit_fill, dim3(blocks_for(count, 256)), dim3(256), 0, c->stream, c->geo, c->parts[c->cur],
+                           c->np, count, *r, c->touch);
+    }
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_unpack_particles(slosh_ctx *c, const slosh_particle *d_src, uint32_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_unpack, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, c->geo, c->parts[c->cur], d_src, n,
+                       c->touch);
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_pack_particles(slosh_ctx *c, void *d_dst, int layout)
+{
+    if (!c->np) return;
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(c->np, 256)), dim3(256), 0, c->stream, c->parts[c->cur], c->np, d_dst,
+                       layout);
+    SLOSH_HIP(hipGetLastError());
+}
+
+void run_sort(slosh_ctx *c)
+{
+    const uint32_t np = c->np;
+    const uint32_t dead = c->nkeys - 1u;
+    launch_fill_u32(c->stream, c->key_count, 0u, c->nkeys);
+    launch_fill_u32(c->stream, c->key_runs, 0u, c->nkeys);
+    launch_fill_u32(c->stream, c->run_cursor, 0u, c->nkeys);
+    if (np == 0) {
+        scan_exclusive(c, c->key_count, c->key_off, c->nkeys);
+        return;
+    }
+    const uint32_t ntiles = blocks_for(np, kTile);
+    {
+        SLOSH_TIMED(c, K_SORT_TILES);
+        hipLaunchKernelGGL(k_sort_tiles, dim3(ntiles), dim3(kTile), 0, c->stream, c->geo, c->parts[c->cur].pa, np, dead,
+                           c->packed, c->tile_nruns, c->run_key, c->run_len, c->key_count, c->key_runs);
+    }
+    SLOSH_HIP(hipGetLastError());
+    scan_exclusive(c, c->key_count, c->key_off, c->nkeys);
+    scan_exclusive(c, c->key_runs, c->run_off, c->nkeys);
+    {
+        SLOSH_TIMED(c, K_SORT_RUNS);
+        hipLaunchKernelGGL(k_sort_runs_scatter, dim3(ntiles), dim3(kTile), 0, c->stream, c->tile_nruns, c->run_key,
+                           c->run_off, c->run_cursor, c->run_sorted);
+    }
+    SLOSH_HIP(hipGetLastError());
+    {
+        SLOSH_TIMED(c, K_SORT_KEY_RUNS);
+        hipLaunchKernelGGL(k_sort_key_runs, dim3(c->nkeys - 1u), dim3(256), 0, c->stream, c->run_off, c->key_off,
+                           c->run_len, c->run_sorted, c->run_base);
+    }
+    SLOSH_HIP(hipGetLastError());
+    {
+        SLOSH_TIMED(c, K_SORT_SCATTER);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kTile), 0, c->stream, c->parts[c->cur],
+                           c->parts[1 - c->cur], np, c->packed, c->run_key, c->run_base, dead);
+    }
+    SLOSH_HIP(hipGetLastError());
+    c->cur = 1 - c->cur;
+}
+
+void launch_p2g(slosh_ctx *c, float warm_scale)
+{
+    if (!c->nactive) return;
+    {
+        SLOSH_TIMED(c, K_P2G);
+        hipLaunchKernelGGL(k_p2g, dim3(c->nactive), dim3(256), 0, c->stream, c->geo, c->parts[c->cur], c->key_off, c->tiles,
+                           c->tile_valid, c->ctype, c->pres, c->new_slot, warm_scale, c->d_col, c->ncol, c->stats);
+    }
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_g2p(slosh_ctx *c, float dt, const float g[3])
+{
+    if (!c->np) return;
+    {
+        SLOSH_TIMED(c, K_G2P);
+        hipLaunchKernelGGL(k_g2p, dim3(blocks_for(c->np, 256)), dim3(256), 0, c->stream, c->geo, c->parts[c->cur], c->np,
+                           c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction,
+                           c->s.particle_radius_cells * c->geo.h, c->s.max_speed, c->d_col, c->ncol, c->touch, c->stats);
+    }
+    SLOSH_HIP(hipGetLastError());
+}
+
+} // namespace slosh

@@ -1,0 +1,201 @@
+// slosh_ctx.h -- host-side context of one simulation domain and the launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "slosh_device.h"
+
+namespace slosh {
+
+// Particles in structure-of-arrays form: position + age as one float4 (the G2P and sort-key passes read 16 B),
+// velocity and the APIC matrix as scalar planes.
+struct Parts {
+    float4 *pa;
+    float *v[3];
+    float *c[9];
+};
+
+// PCG / SOR scalars, kept on the device so no iteration needs a host round trip.
+struct SolverState {
+    double rho;    // r.z of the current preconditioned residual
+    double dq;     // d.Ad
+    double alpha;
+    double beta;
+    double rr;     // ||r||^2
+    double bb;     // ||b||^2
+    double rel;    // ||r|| / ||b||
+    double tol;
+    int32_t done;  // 1 once converged or out of iterations: every solver kernel then returns at once
+    int32_t iters;
+    int32_t maxit;
+    int32_t converged;
+};
+
+struct Level {
+    int dim[3];
+    uint32_t cells;
+    uint8_t *type; // levels >= 1 (dense, x fastest)
+    float *x, *b;
+};
+
+// Kernels that can be timed with HIP events on the context's stream (slosh_set_kernel_timing).
+enum KernelId {
+    K_EMIT = 0, K_SORT_TILES, K_SORT_RUNS, K_SORT_KEY_RUNS, K_SORT_SCATTER, K_SCAN, K_POOL, K_P2G, K_NORMALIZE,
+    K_DIVERGENCE, K_PROJECT, K_DIV_STATS, K_EXTRAPOLATE, K_G2P, K_MG_TYPES, K_COARSE_FACTOR, K_PCG_INIT, K_SMOOTH0,
+    K_SMOOTH, K_RESTRICT0, K_RESTRICT, K_PROLONG0, K_PROLONG, K_COARSE_SOLVE, K_APPLY_A, K_UPDATE_PR, K_UPDATE_D,
+    K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL, K_COUNT
+};
+// Names as the compiled kernels are called (what a kernel trace of the library shows).
+extern const char *const kKernelNames[K_COUNT];
+
+struct StepStats {
+    uint32_t fluid_cells;
+    uint32_t clamped;
+    uint32_t max_speed_bits;
+    uint32_t max_div_bits;
+    uint32_t overflow;
+    uint32_t nactive;
+    uint32_t nfree;
+    uint32_t pad;
+    double div_sum;
+    uint32_t div_count;
+    uint32_t pad2;
+};
+
+} // namespace slosh
+
+struct slosh_ctx {
+    slosh_settings s;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    slosh::Geo geo{};      // host copy; table/active point to device memory
+    uint32_t nkeys = 0;    // nbricks + 1 (dead key)
+    uint32_t cap = 0;      // particle capacity
+    uint32_t np = 0;       // particle slots in use (host-exact)
+    uint32_t ndead = 0;    // dead records among them, dropped by the next sort
+    uint32_t ntiles_max = 0;
+    uint32_t nactive = 0;  // allocated bricks after the last pool pass (read back once per step)
+
+    slosh::Parts parts[2]{};
+    int cur = 0; // which particle buffer holds the live, sorted particles
+
+    // sort scratch
+    uint32_t *packed = nullptr, *tile_nruns = nullptr, *run_key = nullptr, *run_len = nullptr, *run_base = nullptr,
+             *run_sorted = nullptr, *key_count = nullptr, *key_off = nullptr, *key_runs = nullptr, *run_off = nullptr,
+             *run_cursor = nullptr;
+    // scan scratch
+    uint32_t *scan_blocks = nullptr;
+    // brick pool
+    uint32_t *table = nullptr, *slot_brick = nullptr, *active = nullptr, *need = nullptr, *need_rank = nullptr,
+             *freeflag = nullptr, *free_rank = nullptr, *free_list = nullptr, *alloc_flag = nullptr,
+             *alloc_rank = nullptr;
+    uint8_t *new_slot = nullptr; // slots assigned this step: their warm-start pressure reads as 0
+    uint8_t *touch = nullptr;
+    // grid (pool layout)
+    int32_t *tiles = nullptr;
+    uint8_t *tile_valid = nullptr;
+    float *vel[3] = {nullptr, nullptr, nullptr};
+    int vcur = 0;
+    uint8_t *fflags = nullptr;
+    float *fweight = nullptr;
+    uint32_t *ctype = nullptr;
+    double *pres = nullptr;
+    float *div = nullptr, *diag = nullptr;
+    // solver
+    float *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr;
+    int nlevels = 1;
+    slosh::Level lv[slosh::kMaxLevels]{};
+    uint32_t *coarse_idx = nullptr; // [kCoarsestCells]
+    uint32_t *coarse_n = nullptr;
+    double *coarse_inv = nullptr;   // [kCoarsestCells^2]
+    double *partials = nullptr;
+    uint32_t *red_counter = nullptr;
+    slosh::SolverState *state = nullptr;
+    slosh::StepStats *stats = nullptr;
+    // colliders
+    slosh_collider *d_col = nullptr;
+    slosh_collider h_col[SLOSH_MAX_COLLIDERS];
+    uint32_t ncol = 0;
+    // host pinned mirrors
+    slosh::StepStats *h_stats = nullptr;
+    slosh::SolverState *h_state = nullptr;
+    // readback staging
+    void *staging = nullptr;
+    size_t staging_bytes = 0;
+    // results and timing
+    slosh_results res{};
+    uint64_t frame = 0;
+    float prev_dt = 0.0f;
+    hipEvent_t ev_step0 = nullptr, ev_step1 = nullptr, ev_solve0 = nullptr, ev_solve1 = nullptr;
+    // per-kernel timing (event pairs around selected launches, accumulated after each step's final sync)
+    uint64_t timing_mask = 0;
+    hipEvent_t *tev = nullptr;
+    uint32_t tev_cap = 0, tev_used = 0;
+    int *tev_kernel = nullptr;
+    double ktime_ms[slosh::K_COUNT] = {};
+    uint32_t klaunches[slosh::K_COUNT] = {};
+};
+
+namespace slosh {
+
+// ---- particles.hip ----
+void launch_emit_fill(slosh_ctx *c, const slosh_emit_request *r, uint32_t count);
+void launch_unpack_particles(slosh_ctx *c, const slosh_particle *d_src, uint32_t n);
+void launch_pack_particles(slosh_ctx *c, void *d_dst, int layout);
+void run_sort(slosh_ctx *c);
+void launch_p2g(slosh_ctx *c, float warm_scale);
+void launch_g2p(slosh_ctx *c, float dt, const float g[3]);
+
+// ---- grid.hip ----
+void run_pool(slosh_ctx *c);
+void launch_normalize(slosh_ctx *c, float dt, const float g[3]);
+void launch_divergence(slosh_ctx *c, float dt);
+void launch_project(slosh_ctx *c);
+void launch_extrapolate(slosh_ctx *c, uint32_t layer);
+void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n);
+void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
+void launch_fill_u8(hipStream_t st, uint8_t *p, uint8_t v, size_t n);
+void launch_fill_f64(hipStream_t st, double *p, double v, size_t n);
+
+// ---- solver.hip ----
+void run_mgpcg(slosh_ctx *c);
+void run_rbsor(slosh_ctx *c);
+
+} // namespace slosh
+
+#define SLOSH_HIP(call)                                                                                          \
+    do {                                                                                                         \
+        hipError_t e_ = (call);                                                                                  \
+        if (e_ != hipSuccess) throw slosh::HipError(e_, #call, __FILE__, __LINE__);                            \
+    } while (0)
+
+namespace slosh {
+struct HipError {
+    hipError_t err;
+    const char *what;
+    const char *file;
+    int line;
+    HipError(hipError_t e, const char *w, const char *f, int l) : err(e), what(w), file(f), line(l) {}
+};
+
+// Records an event pair around the launches in its scope when kernel `id` is selected for timing.
+struct KTimer {
+    slosh_ctx *c;
+    int slot;
+    KTimer(slosh_ctx *ctx, int id) : c(ctx), slot(-1)
+    {
+        if (!(c->timing_mask >> id & 1ull) || c->tev_used + 2 > c->tev_cap) return;
+        slot = (int)c->tev_used;
+        c->tev_used += 2;
+        c->tev_kernel[slot / 2] = id;
+        (void)hipEventRecord(c->tev[slot], c->stream);
+    }
+    ~KTimer()
+    {
+        if (slot >= 0) (void)hipEventRecord(c->tev[slot + 1], c->stream);
+    }
+};
+} // namespace slosh
+
+#define SLOSH_TIMED(c, id) slosh::KTimer slosh_ktimer_(c, slosh::id)
