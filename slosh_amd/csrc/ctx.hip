@@ -22,7 +22,7 @@ const char *const kKernelNames[K_COUNT] = {
     "k_pool", "k_p2g", "k_normalize", "k_divergence", "k_project", "k_div_stats", "k_extrapolate", "k_g2p",
     "k_build_type", "k_coarse_factor", "k_pcg_init", "k_smooth0", "k_smooth", "k_restrict0", "k_restrict",
     "k_prolong0", "k_prolong", "k_coarse_solve", "k_apply_a", "k_update_pr", "k_update_d", "k_sor_sweep",
-    "k_sor_residual", "k_fill"};
+    "k_sor_residual", "k_fill", "k_finalize", "k_coarse_tail", "k_step_stats"};
 } // namespace slosh
 
 static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");
@@ -62,12 +62,14 @@ void free_all(slosh_ctx *c)
     dfree(c->new_slot); dfree(c->touch);
     dfree(c->tiles); dfree(c->tile_valid);
     for (int k = 0; k < 3; ++k) dfree(c->vel[k]);
-    dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag);
+    dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag); dfree(c->fmask);
     dfree(c->r); dfree(c->z); dfree(c->d); dfree(c->q);
     for (int l = 1; l < kMaxLevels; ++l) {
         dfree(c->lv[l].type); dfree(c->lv[l].x); dfree(c->lv[l].b);
     }
-    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->partials); dfree(c->red_counter);
+    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->partials);
+    dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
+    dfree(c->bp_clamp);
     dfree(c->state); dfree(c->stats); dfree(c->d_col);
     dfree(c->staging);
     for (uint32_t k = 0; k < c->tev_cap; ++k) (void)hipEventDestroy(c->tev[k]);
@@ -180,6 +182,7 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->pres, pc);
     dalloc(&c->div, pc);
     dalloc(&c->diag, pc);
+    dalloc(&c->fmask, pc);
     dalloc(&c->r, pc);
     dalloc(&c->z, pc);
     dalloc(&c->d, pc);
@@ -205,7 +208,12 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->coarse_n, 1);
     dalloc(&c->coarse_inv, (size_t)kCoarsestCells * kCoarsestCells);
     dalloc(&c->partials, 2 * (size_t)g.npool + 2u);
-    dalloc(&c->red_counter, 1);
+    dalloc(&c->bp_fluid, g.npool);
+    dalloc(&c->bp_div_max, g.npool);
+    dalloc(&c->bp_div_cnt, g.npool);
+    dalloc(&c->bp_div_sum, g.npool);
+    dalloc(&c->bp_speed, c->ntiles_max);
+    dalloc(&c->bp_clamp, c->ntiles_max);
     dalloc(&c->state, 1);
     dalloc(&c->stats, 1);
     dalloc(&c->d_col, SLOSH_MAX_COLLIDERS);
@@ -271,6 +279,9 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     c->res.converged = 1;
     c->res.solve_ms = 0.0f;
     launch_fill_u32(c->stream, (uint32_t *)c->stats, 0u, sizeof(StepStats) / 4u);
+    c->n_p2g_blocks = 0;
+    c->n_div_blocks = 0;
+    c->n_g2p_blocks = 0;
     SLOSH_HIP(hipEventRecord(c->ev_step0, c->stream));
     run_sort(c);
     c->np -= c->ndead;
@@ -310,6 +321,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     if (stop > SLOSH_STAGE_PROJECT)
         for (int k = 0; k < c->s.extrapolation_layers; ++k) launch_extrapolate(c, (uint32_t)k);
     if (stop > SLOSH_STAGE_EXTRAPOLATE) launch_g2p(c, dt, gv);
+    launch_step_stats(c);
     SLOSH_HIP(hipEventRecord(c->ev_step1, c->stream));
     read_stats(c);
     const StepStats &st = *c->h_stats;

@@ -285,15 +285,18 @@ __device__ __forceinline__ float air_face_weight(const Geo &g, const float *fw, 
 }
 
 // Slosh.usf:692-741 (DivergenceCS)
+// Also writes the solver's per-cell stencil mask: bit n (0..5, order -x +x -y +y -z +z) = neighbour n is fluid,
+// bit 6 = the cell itself is fluid (0 for every other cell).
 __global__ __launch_bounds__(512) void k_divergence(Geo g, const float *vel, const float *fw, const uint32_t *ctype,
-                                                    float *div, float *diag, int ghost, float dc, float deadband,
-                                                    float h_over_dt)
+                                                    float *div, float *diag, uint8_t *fmask, int ghost, float dc,
+                                                    float deadband, float h, float dt)
 {
     CellPos c;
-    if (!cell_of_block(g, c)) return;
-    if ((ctype[c.idx] & kCellTypeMask) != kCellFluid) {
+    if (!cell_of_block(g, c) || (ctype[c.idx] & kCellTypeMask) != kCellFluid) {
+        // cells of a partial brick beyond the grid get a zero mask too (their slot may be recycled)
         div[c.idx] = 0.0f;
         diag[c.idx] = 0.0f;
+        fmask[c.idx] = 0u;
         return;
     }
     float dv = 0.0f;
@@ -310,17 +313,20 @@ __global__ __launch_bounds__(512) void k_divergence(Geo g, const float *vel, con
     }
     float rho = ws / (6.0f * kTargetPpc);
     float excess = fmaxf(rho - (1.0f + deadband), 0.0f);
-    float target = fminf(dc * excess, 0.5f) * h_over_dt;
+    float target = fminf(dc * excess, 0.5f) * h / dt;
     div[c.idx] = dv - target;
     float dg = 0.0f;
+    uint32_t mask = 1u << 6;
     for (int nb = 0; nb < 6; ++nb) {
         int d[3];
         nb_offset(nb, d);
         int nx = c.x + d[0], ny = c.y + d[1], nz = c.z + d[2];
         uint32_t t = type_at(g, ctype, nx, ny, nz) & kCellTypeMask;
+        if (t == kCellFluid) mask |= 1u << nb;
         dg += t == kCellFluid ? 1.0f : (t == kCellAir ? air_face_weight(g, fw, ghost, rho, nx, ny, nz) : 0.0f);
     }
     diag[c.idx] = dg;
+    fmask[c.idx] = (uint8_t)mask;
 }
 
 // Slosh.usf:995-1052 (ProjectFace) on the FP64 pressure, without collider-coupling accumulation.
@@ -358,14 +364,13 @@ __global__ __launch_bounds__(512) void k_project(Geo g, float *vel, const float 
     }
 }
 
-// Slosh.usf:1222-1240 (ResidualCS): max and sum of |div|/h over fluid cells after projection. The sum is reduced
-// per workgroup into partials (FP64) and summed in a fixed order by the last workgroup.
-__global__ __launch_bounds__(512) void k_div_stats(Geo g, const float *vel, const uint32_t *ctype, double *partials,
-                                                   uint32_t *counter, StepStats *stats)
+// Slosh.usf:1222-1240 (ResidualCS): max and sum of |div|/h over fluid cells after projection, one partial per
+// brick (reduced with the other step statistics by k_step_stats).
+__global__ __launch_bounds__(512) void k_div_stats(Geo g, const float *vel, const uint32_t *ctype, uint32_t *bp_max,
+                                                   uint32_t *bp_cnt, double *bp_sum)
 {
     __shared__ double ssum[512];
     __shared__ uint32_t smax, scount;
-    __shared__ bool last;
     if (threadIdx.x == 0) {
         smax = 0u;
         scount = 0u;
@@ -392,26 +397,59 @@ __global__ __launch_bounds__(512) void k_div_stats(Geo g, const float *vel, cons
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = ssum[0];
-        if (smax) atomicMax(&stats->max_div_bits, smax);
-        if (scount) atomicAdd(&stats->div_count, scount);
-        __threadfence();
-        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+        bp_sum[blockIdx.x] = ssum[0];
+        bp_max[blockIdx.x] = smax;
+        bp_cnt[blockIdx.x] = scount;
     }
+}
+
+// End of step: one workgroup reduces the per-workgroup statistics of P2G, G2P and the divergence pass (counts and
+// maxima in any order, the FP64 divergence sum in a fixed order).
+__global__ __launch_bounds__(256) void k_step_stats(const uint32_t *bp_fluid, uint32_t n_p2g, const uint32_t *bp_div_max,
+                                                    const uint32_t *bp_div_cnt, const double *bp_div_sum,
+                                                    uint32_t n_div, const uint32_t *bp_speed,
+                                                    const uint32_t *bp_clamp, uint32_t n_g2p, StepStats *stats)
+{
+    __shared__ uint32_t su[5][256];
+    __shared__ double sd[256];
+    const uint32_t t = threadIdx.x;
+    uint32_t fluid = 0, dmax = 0, dcnt = 0, smax = 0, clamp = 0;
+    double dsum = 0.0;
+    for (uint32_t b = t; b < n_p2g; b += 256) fluid += bp_fluid[b];
+    for (uint32_t b = t; b < n_div; b += 256) {
+        dmax = max(dmax, bp_div_max[b]);
+        dcnt += bp_div_cnt[b];
+        dsum += bp_div_sum[b];
+    }
+    for (uint32_t b = t; b < n_g2p; b += 256) {
+        smax = max(smax, bp_speed[b]);
+        clamp += bp_clamp[b];
+    }
+    su[0][t] = fluid;
+    su[1][t] = dmax;
+    su[2][t] = dcnt;
+    su[3][t] = smax;
+    su[4][t] = clamp;
+    sd[t] = dsum;
     __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double acc = 0.0;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) acc += partials[b];
-    ssum[threadIdx.x] = acc;
-    __syncthreads();
-    for (uint32_t s = 256; s > 0; s >>= 1) {
-        if (threadIdx.x < s) ssum[threadIdx.x] += ssum[threadIdx.x + s];
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            su[0][t] += su[0][t + s];
+            su[1][t] = max(su[1][t], su[1][t + s]);
+            su[2][t] += su[2][t + s];
+            su[3][t] = max(su[3][t], su[3][t + s]);
+            su[4][t] += su[4][t + s];
+            sd[t] += sd[t + s];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        stats->div_sum = ssum[0];
-        *counter = 0u;
+    if (t == 0) {
+        stats->fluid_cells = su[0][0];
+        stats->max_div_bits = su[1][0];
+        stats->div_count = su[2][0];
+        stats->max_speed_bits = su[3][0];
+        stats->clamped = su[4][0];
+        stats->div_sum = sd[0];
     }
 }
 
@@ -549,8 +587,8 @@ void launch_divergence(slosh_ctx *c, float dt)
     {
         SLOSH_TIMED(c, K_DIVERGENCE);
         hipLaunchKernelGGL(k_divergence, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->fweight,
-                           c->ctype, c->div, c->diag, c->s.ghost_fluid, c->s.density_correction, c->s.density_deadband,
-                           c->geo.h / dt);
+                           c->ctype, c->div, c->diag, c->fmask, c->s.ghost_fluid, c->s.density_correction,
+                           c->s.density_deadband, c->geo.h, dt);
     }
     SLOSH_HIP(hipGetLastError());
 }
@@ -566,8 +604,18 @@ void launch_project(slosh_ctx *c)
     {
         SLOSH_TIMED(c, K_DIV_STATS);
         hipLaunchKernelGGL(k_div_stats, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->ctype,
-                           c->partials, c->red_counter, c->stats);
+                           c->bp_div_max, c->bp_div_cnt, c->bp_div_sum);
     }
+    c->n_div_blocks = c->nactive;
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_step_stats(slosh_ctx *c)
+{
+    SLOSH_TIMED(c, K_STEP_STATS);
+    hipLaunchKernelGGL(k_step_stats, dim3(1), dim3(256), 0, c->stream, c->bp_fluid, c->n_p2g_blocks, c->bp_div_max,
+                       c->bp_div_cnt, c->bp_div_sum, c->n_div_blocks, c->bp_speed, c->bp_clamp, c->n_g2p_blocks,
+                       c->stats);
     SLOSH_HIP(hipGetLastError());
 }
 

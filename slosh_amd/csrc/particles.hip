@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(Parts src, Parts dst, uint
 // workgroup counts particles per cell and classifies its 512 cells (BuildSolidCS :365-395, MarkCellsCS :569-598).
 __global__ __launch_bounds__(256) void k_p2g(Geo g, Parts p, const uint32_t *key_off, int32_t *tiles,
                                              uint8_t *tile_valid, uint32_t *ctype, double *pres, uint8_t *new_slot,
-                                             float warm, const slosh_collider *col, uint32_t ncol, StepStats *stats)
+                                             float warm, const slosh_collider *col, uint32_t ncol, uint32_t *bp_fluid)
 {
     __shared__ int32_t tile[kTileInts];
     __shared__ int32_t cnt[kBrickCells];
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_p2g(Geo g, Parts p, const uint32_t *key
     if (fluid) atomicAdd(&fluid_sum, fluid);
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (fluid_sum) atomicAdd(&stats->fluid_cells, fluid_sum);
+        bp_fluid[blockIdx.x] = fluid_sum;
         if (fresh) new_slot[slot] = 0u;
     }
 }
@@ -445,7 +445,7 @@ __device__ __forceinline__ float sample_component(const Geo &g, const float *vel
 __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, uint32_t np, const float *vel, const uint8_t *fflags,
                                              float dt, float gx, float gy, float gz, float friction, float radius,
                                              float max_speed, const slosh_collider *col, uint32_t ncol,
-                                             uint8_t *touch, StepStats *stats)
+                                             uint8_t *touch, uint32_t *bp_speed, uint32_t *bp_clamp)
 {
     __shared__ uint32_t tb[256];
     __shared__ uint32_t smax, sclamp;
@@ -544,8 +544,8 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, uint32_t np, const 
     __syncthreads();
     if (brick != kNoBrick && (threadIdx.x == 0 || tb[threadIdx.x - 1] != brick)) touch[brick] = 1u;
     if (threadIdx.x == 0) {
-        if (smax) atomicMax(&stats->max_speed_bits, smax);
-        if (sclamp) atomicAdd(&stats->clamped, sclamp);
+        bp_speed[blockIdx.x] = smax;
+        bp_clamp[blockIdx.x] = sclamp;
     }
 }
 
@@ -627,7 +627,8 @@ void launch_p2g(slosh_ctx *c, float warm_scale)
     {
         SLOSH_TIMED(c, K_P2G);
         hipLaunchKernelGGL(k_p2g, dim3(c->nactive), dim3(256), 0, c->stream, c->geo, c->parts[c->cur], c->key_off, c->tiles,
-                           c->tile_valid, c->ctype, c->pres, c->new_slot, warm_scale, c->d_col, c->ncol, c->stats);
+                           c->tile_valid, c->ctype, c->pres, c->new_slot, warm_scale, c->d_col, c->ncol, c->bp_fluid);
+        c->n_p2g_blocks = c->nactive;
     }
     SLOSH_HIP(hipGetLastError());
 }
@@ -639,7 +640,9 @@ void launch_g2p(slosh_ctx *c, float dt, const float g[3])
         SLOSH_TIMED(c, K_G2P);
         hipLaunchKernelGGL(k_g2p, dim3(blocks_for(c->np, 256)), dim3(256), 0, c->stream, c->geo, c->parts[c->cur], c->np,
                            c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction,
-                           c->s.particle_radius_cells * c->geo.h, c->s.max_speed, c->d_col, c->ncol, c->touch, c->stats);
+                           c->s.particle_radius_cells * c->geo.h, c->s.max_speed, c->d_col, c->ncol, c->touch,
+                           c->bp_speed, c->bp_clamp);
+        c->n_g2p_blocks = blocks_for(c->np, 256);
     }
     SLOSH_HIP(hipGetLastError());
 }

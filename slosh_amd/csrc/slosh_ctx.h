@@ -44,7 +44,7 @@ enum KernelId {
     K_EMIT = 0, K_SORT_TILES, K_SORT_RUNS, K_SORT_KEY_RUNS, K_SORT_SCATTER, K_SCAN, K_POOL, K_P2G, K_NORMALIZE,
     K_DIVERGENCE, K_PROJECT, K_DIV_STATS, K_EXTRAPOLATE, K_G2P, K_MG_TYPES, K_COARSE_FACTOR, K_PCG_INIT, K_SMOOTH0,
     K_SMOOTH, K_RESTRICT0, K_RESTRICT, K_PROLONG0, K_PROLONG, K_COARSE_SOLVE, K_APPLY_A, K_UPDATE_PR, K_UPDATE_D,
-    K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL, K_COUNT
+    K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL, K_FINALIZE, K_COARSE_TAIL, K_STEP_STATS, K_COUNT
 };
 // Names as the compiled kernels are called (what a kernel trace of the library shows).
 extern const char *const kKernelNames[K_COUNT];
@@ -102,6 +102,7 @@ struct slosh_ctx {
     uint32_t *ctype = nullptr;
     double *pres = nullptr;
     float *div = nullptr, *diag = nullptr;
+    uint8_t *fmask = nullptr; // solver stencil mask per cell (fluid neighbours, self fluid)
     // solver
     float *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr;
     int nlevels = 1;
@@ -110,9 +111,16 @@ struct slosh_ctx {
     uint32_t *coarse_n = nullptr;
     double *coarse_inv = nullptr;   // [kCoarsestCells^2]
     double *partials = nullptr;
-    uint32_t *red_counter = nullptr;
     slosh::SolverState *state = nullptr;
     slosh::StepStats *stats = nullptr;
+    // per-workgroup statistics partials, reduced once at the end of a step (no cross-workgroup atomics)
+    uint32_t *bp_fluid = nullptr;   // [npool]   P2G: fluid cells per brick
+    uint32_t *bp_div_max = nullptr; // [npool]   |div|/h max bits per brick
+    uint32_t *bp_div_cnt = nullptr; // [npool]
+    double *bp_div_sum = nullptr;   // [npool]
+    uint32_t *bp_speed = nullptr;   // [ntiles]  G2P: max speed bits per 256 particles
+    uint32_t *bp_clamp = nullptr;   // [ntiles]
+    uint32_t n_p2g_blocks = 0, n_div_blocks = 0, n_g2p_blocks = 0;
     // colliders
     slosh_collider *d_col = nullptr;
     slosh_collider h_col[SLOSH_MAX_COLLIDERS];
@@ -153,6 +161,7 @@ void launch_normalize(slosh_ctx *c, float dt, const float g[3]);
 void launch_divergence(slosh_ctx *c, float dt);
 void launch_project(slosh_ctx *c);
 void launch_extrapolate(slosh_ctx *c, uint32_t layer);
+void launch_step_stats(slosh_ctx *c);
 void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
 void launch_fill_u8(hipStream_t st, uint8_t *p, uint8_t v, size_t n);
@@ -161,6 +170,18 @@ void launch_fill_f64(hipStream_t st, double *p, double v, size_t n);
 // ---- solver.hip ----
 void run_mgpcg(slosh_ctx *c);
 void run_rbsor(slosh_ctx *c);
+
+// ---- level0.hip: finest-level solver passes (partials of a reduction: one per workgroup, l0_groups of them) ----
+uint32_t l0_groups(const slosh_ctx *c);
+void l0_pcg_init(slosh_ctx *c);
+void l0_smooth(slosh_ctx *c, uint32_t colour, int want_dot);
+void l0_restrict(slosh_ctx *c);
+void l0_prolong(slosh_ctx *c);
+void l0_apply_a(slosh_ctx *c);
+void l0_update_pr(slosh_ctx *c);
+void l0_update_d(slosh_ctx *c, int first);
+void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w);
+void l0_sor_residual(slosh_ctx *c);
 
 } // namespace slosh
 
