@@ -187,20 +187,25 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->z, pc);
     dalloc(&c->d, pc);
     dalloc(&c->q, pc);
-    // MG hierarchy: halve (rounding up) until a level holds <= 64 cells or 8 levels exist.
+    // MG hierarchy: halve (rounding up) until a level holds <= 64 cells or 8 levels exist. Levels 1..3 live in the
+    // brick pool ((8 >> l)^3 cells per slot), coarser ones are dense. dalloc zeroes them: the null slot of a
+    // pooled level must read as AIR with x = b = 0.
     c->nlevels = 1;
     c->lv[0].dim[0] = g.n[0];
     c->lv[0].dim[1] = g.n[1];
     c->lv[0].dim[2] = g.n[2];
     c->lv[0].cells = (uint32_t)g.n[0] * (uint32_t)g.n[1] * (uint32_t)g.n[2];
     while (c->lv[c->nlevels - 1].cells > kCoarsestCells && c->nlevels < kMaxLevels) {
-        Level &p = c->lv[c->nlevels - 1];
-        Level &l = c->lv[c->nlevels];
+        const int li = c->nlevels;
+        Level &p = c->lv[li - 1];
+        Level &l = c->lv[li];
         for (int a = 0; a < 3; ++a) l.dim[a] = (p.dim[a] + 1) / 2;
         l.cells = (uint32_t)l.dim[0] * (uint32_t)l.dim[1] * (uint32_t)l.dim[2];
-        dalloc(&l.type, l.cells);
-        dalloc(&l.x, l.cells);
-        dalloc(&l.b, l.cells);
+        l.pooled = li <= 3;
+        const size_t n = l.pooled ? (size_t)g.npool * (size_t)(512u >> (3 * li)) : (size_t)l.cells;
+        dalloc(&l.type, n);
+        dalloc(&l.x, n);
+        dalloc(&l.b, n);
         c->nlevels++;
     }
     if (c->lv[c->nlevels - 1].cells > kCoarsestCells) throw std::bad_alloc();

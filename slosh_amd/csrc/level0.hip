@@ -145,11 +145,25 @@ __global__ __launch_bounds__(kThreads) void k_smooth0(Geo g, uint32_t nactive, c
     if (want_dot) block_partial<1>(v, partials);
 }
 
-// Restriction into dense level 1: the residuals of the brick's 512 cells in LDS, then b_c = 1/4 of the fluid
+// Type of level-1 cell (local lx, ly, lz of brick B, possibly one step outside the brick) in the pooled level.
+__device__ __forceinline__ uint32_t type1_at(const Brick &B, const PLevel &L1, int lx, int ly, int lz)
+{
+    const int x = B.bc[0] * 4 + lx, y = B.bc[1] * 4 + ly, z = B.bc[2] * 4 + lz;
+    if (x < 0 || y < 0 || z < 0 || x >= L1.dim[0] || y >= L1.dim[1] || z >= L1.dim[2]) return kCellSolid;
+    uint32_t base = B.base >> 3; // slot * 64
+    if (lx < 0) base = B.nb[0] >> 3, lx += 4;
+    else if (lx > 3) base = B.nb[1] >> 3, lx -= 4;
+    else if (ly < 0) base = B.nb[2] >> 3, ly += 4;
+    else if (ly > 3) base = B.nb[3] >> 3, ly -= 4;
+    else if (lz < 0) base = B.nb[4] >> 3, lz += 4;
+    else if (lz > 3) base = B.nb[5] >> 3, lz -= 4;
+    return L1.type[base + (uint32_t)(lx + 4 * ly + 16 * lz)];
+}
+
+// Restriction into pooled level 1: the residuals of the brick's 512 cells in LDS, then b_c = 1/4 of the fluid
 // children's sum for the brick's 4^3 level-1 cells, and x_c = b_c / diag_c on red cells (fused red sweep).
 __global__ __launch_bounds__(kThreads) void k_restrict0(Geo g, uint32_t nactive, const uint8_t *fmask,
-                                                        const float *diag, const float *xf, const float *bf, int c0,
-                                                        int c1, int c2, const uint8_t *ctype1, float *xc, float *bc,
+                                                        const float *diag, const float *xf, const float *bf, PLevel L1,
                                                         const SolverState *st)
 {
     if (st->done) return;
@@ -171,9 +185,10 @@ __global__ __launch_bounds__(kThreads) void k_restrict0(Geo g, uint32_t nactive,
             const uint32_t t = threadIdx.x;
             const int lx = (int)(t & 3u), ly = (int)((t >> 2) & 3u), lz = (int)(t >> 4);
             const int cx = B.bc[0] * 4 + lx, cy = B.bc[1] * 4 + ly, cz = B.bc[2] * 4 + lz;
-            if (cx < c0 && cy < c1 && cz < c2) {
-                const uint32_t ci = (uint32_t)(cx + c0 * (cy + c1 * cz));
-                if (ctype1[ci] != kCellFluid) {
+            if (cx < L1.dim[0] && cy < L1.dim[1] && cz < L1.dim[2]) {
+                const uint32_t ci = (B.base >> 3) + t;
+                float *xc = L1.x, *bc = L1.b;
+                if (L1.type[ci] != kCellFluid) {
                     bc[ci] = 0.0f;
                     xc[ci] = 0.0f;
                 } else {
@@ -189,10 +204,7 @@ __global__ __launch_bounds__(kThreads) void k_restrict0(Geo g, uint32_t nactive,
                     for (int nb = 0; nb < 6; ++nb) {
                         int d[3];
                         nb_offset(nb, d);
-                        int x = cx + d[0], y = cy + d[1], z = cz + d[2];
-                        uint32_t ty = (x < 0 || y < 0 || z < 0 || x >= c0 || y >= c1 || z >= c2)
-                                          ? kCellSolid
-                                          : ctype1[(uint32_t)(x + c0 * (y + c1 * z))];
+                        uint32_t ty = type1_at(B, L1, lx + d[0], ly + d[1], lz + d[2]);
                         cdg += ty != kCellSolid ? 1.0f : 0.0f;
                     }
                     xc[ci] = is_red(cx, cy, cz) ? bv / cdg : 0.0f;
@@ -203,10 +215,10 @@ __global__ __launch_bounds__(kThreads) void k_restrict0(Geo g, uint32_t nactive,
     }
 }
 
-// Prolongation from dense level 1: x_f += x_c(parent) on fluid cells with a fluid parent.
-__global__ __launch_bounds__(kThreads) void k_prolong0(Geo g, uint32_t nactive, const uint8_t *fmask, float *xf, int c0,
-                                                       int c1, const uint8_t *ctype1, const float *xc,
-                                                       const SolverState *st)
+// Prolongation from pooled level 1 (the parent lives in the same brick): x_f += x_c(parent) on fluid cells with
+// a fluid parent.
+__global__ __launch_bounds__(kThreads) void k_prolong0(Geo g, uint32_t nactive, const uint8_t *fmask, float *xf,
+                                                       PLevel L1, const SolverState *st)
 {
     if (st->done) return;
     for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
@@ -214,10 +226,8 @@ __global__ __launch_bounds__(kThreads) void k_prolong0(Geo g, uint32_t nactive, 
         for (uint32_t l = threadIdx.x; l < kBrickCells; l += kThreads) {
             const uint32_t i = B.base + l;
             if (!(fmask[i] & kSelfFluid)) continue;
-            const int x = B.bc[0] * 8 + (int)(l & 7u), y = B.bc[1] * 8 + (int)((l >> 3) & 7u),
-                      z = B.bc[2] * 8 + (int)(l >> 6);
-            const uint32_t pi = (uint32_t)((x >> 1) + c0 * ((y >> 1) + c1 * (z >> 1)));
-            if (ctype1[pi] == kCellFluid) xf[i] += xc[pi];
+            const uint32_t pi = (B.base >> 3) + ((l & 7u) >> 1) + 4u * (((l >> 3) & 7u) >> 1) + 16u * ((l >> 6) >> 1);
+            if (L1.type[pi] == kCellFluid) xf[i] += L1.x[pi];
         }
     }
 }
@@ -346,17 +356,9 @@ void l0_smooth(slosh_ctx *c, uint32_t colour, int want_dot)
     L0_LAUNCH(K_SMOOTH0, k_smooth0, c->diag, c->z, c->r, colour, want_dot, c->partials, c->state);
 }
 
-void l0_restrict(slosh_ctx *c)
-{
-    const Level &L = c->lv[1];
-    L0_LAUNCH(K_RESTRICT0, k_restrict0, c->diag, c->z, c->r, L.dim[0], L.dim[1], L.dim[2], L.type, L.x, L.b, c->state);
-}
+void l0_restrict(slosh_ctx *c) { L0_LAUNCH(K_RESTRICT0, k_restrict0, c->diag, c->z, c->r, plevel_of(c, 1), c->state); }
 
-void l0_prolong(slosh_ctx *c)
-{
-    const Level &L = c->lv[1];
-    L0_LAUNCH(K_PROLONG0, k_prolong0, c->z, L.dim[0], L.dim[1], L.type, L.x, c->state);
-}
+void l0_prolong(slosh_ctx *c) { L0_LAUNCH(K_PROLONG0, k_prolong0, c->z, plevel_of(c, 1), c->state); }
 
 void l0_apply_a(slosh_ctx *c) { L0_LAUNCH(K_APPLY_A, k_apply_a, c->diag, c->d, c->q, c->partials, c->state); }
 

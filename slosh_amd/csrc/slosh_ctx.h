@@ -32,12 +32,45 @@ struct SolverState {
     int32_t converged;
 };
 
+// Multigrid level >= 1. Levels 1..3 are pooled (stored per allocated brick, (8 >> l)^3 cells per slot); coarser
+// levels are dense (x fastest).
 struct Level {
     int dim[3];
     uint32_t cells;
-    uint8_t *type; // levels >= 1 (dense, x fastest)
+    bool pooled;
+    uint8_t *type;
     float *x, *b;
 };
+
+// Kernel-side views of a pooled and of a dense level.
+struct PLevel {
+    int dim[3];
+    int ls;        // log2 of cells per brick axis
+    int s;         // cells per brick axis (8 >> level)
+    uint32_t cpb;  // cells per brick
+    uint8_t *type;
+    float *x, *b;
+};
+struct DenseLevel {
+    int dim[3];
+    uint32_t cells;
+    uint8_t *type;
+    float *x, *b;
+};
+
+__device__ __forceinline__ bool pinside(const PLevel &L, int x, int y, int z)
+{
+    return x >= 0 && y >= 0 && z >= 0 && x < L.dim[0] && y < L.dim[1] && z < L.dim[2];
+}
+
+// Pooled index of level coordinates inside the level's extent (through the brick table).
+__device__ __forceinline__ uint32_t pindex(const Geo &g, const PLevel &L, int x, int y, int z)
+{
+    const int m = L.s - 1;
+    uint32_t b = (uint32_t)((x >> L.ls) + g.bd[0] * ((y >> L.ls) + g.bd[1] * (z >> L.ls)));
+    uint32_t local = (uint32_t)((x & m) + L.s * ((y & m) + L.s * (z & m)));
+    return g.table[b] * L.cpb + local;
+}
 
 // Kernels that can be timed with HIP events on the context's stream (slosh_set_kernel_timing).
 enum KernelId {
@@ -170,6 +203,17 @@ void launch_fill_f64(hipStream_t st, double *p, double v, size_t n);
 // ---- solver.hip ----
 void run_mgpcg(slosh_ctx *c);
 void run_rbsor(slosh_ctx *c);
+
+// ---- pooled.hip: MG levels 1..3 in the brick pool, and the pooled <-> dense transitions ----
+PLevel plevel_of(const slosh_ctx *c, int l);
+DenseLevel dlevel_of(const slosh_ctx *c, int l);
+void pooled_build_type(slosh_ctx *c, int l);
+void pooled_smooth(slosh_ctx *c, int l, uint32_t colour);
+void pooled_restrict(slosh_ctx *c, int l);           // pooled l -> pooled l + 1
+void pooled_prolong(slosh_ctx *c, int l);            // pooled l + 1 -> pooled l
+void dense_build_type_from_pooled(slosh_ctx *c, int l); // dense l from pooled l - 1
+void restrict_pooled_to_dense(slosh_ctx *c, int l);  // pooled l -> dense l + 1
+void prolong_dense_to_pooled(slosh_ctx *c, int l);   // dense l + 1 -> pooled l
 
 // ---- level0.hip: finest-level solver passes (partials of a reduction: one per workgroup, l0_groups of them) ----
 uint32_t l0_groups(const slosh_ctx *c);

@@ -4,8 +4,10 @@
 // MGPCG keeps FP32 vectors, an FP64 pressure (the alpha-update accumulator), FP64 dot products and alpha/beta, and
 // solves the coarsest level (<= 64 cells) exactly in FP64. All scalars live in a device SolverState; every solver
 // kernel returns at once when `done` is set, so the host enqueues iterations in chunks and reads the state back
-// once per chunk. The level-0 passes are in level0.hip; this file holds the dense coarse levels, the coarsest
-// solve and the iteration drivers.
+// once per chunk.
+//
+// Level layout: level 0 is the brick pool (level0.hip), levels 1..3 are pooled per brick (pooled.hip), levels 4
+// and coarser are dense and run inside one workgroup, in LDS (k_coarse_tail below).
 //
 // Reductions are deterministic and free of cross-workgroup synchronisation: each workgroup writes its FP64
 // partial sum, and a one-workgroup finalize kernel sums the partials in a fixed order and updates the state. (A
@@ -19,28 +21,26 @@ namespace {
 
 inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
-struct Dense {
-    int dim[3];
-    uint32_t cells;
-    const uint8_t *type;
-};
+constexpr int kFirstDense = 4;        // first dense MG level
+constexpr uint32_t kTailCells = 4800; // LDS capacity of the tail (16^3 + 8^3 + 4^3 + ...)
 
-__device__ __forceinline__ uint32_t dtype_at(const Dense &L, int x, int y, int z)
-{
-    if (x < 0 || y < 0 || z < 0 || x >= L.dim[0] || y >= L.dim[1] || z >= L.dim[2]) return kCellSolid;
-    return L.type[(uint32_t)(x + L.dim[0] * (y + L.dim[1] * z))];
-}
-
-__device__ __forceinline__ void dcoord(const Dense &L, uint32_t i, int &x, int &y, int &z)
+__device__ __forceinline__ void dcoord(const DenseLevel &L, uint32_t i, int &x, int &y, int &z)
 {
     x = (int)(i % (uint32_t)L.dim[0]);
     y = (int)((i / (uint32_t)L.dim[0]) % (uint32_t)L.dim[1]);
     z = (int)(i / (uint32_t)(L.dim[0] * L.dim[1]));
 }
 
+__device__ __forceinline__ uint32_t dtype_at(const DenseLevel &L, int x, int y, int z)
+{
+    if (x < 0 || y < 0 || z < 0 || x >= L.dim[0] || y >= L.dim[1] || z >= L.dim[2]) return kCellSolid;
+    return L.type[(uint32_t)(x + L.dim[0] * (y + L.dim[1] * z))];
+}
+
 // Stencil of a dense coarse level (SloshPressureMG.usf:142-157 without the ghost diagonal): diagonal = 1e-4 +
 // non-solid neighbours, sum over fluid neighbours, neighbours in the order -x +x -y +y -z +z.
-__device__ __forceinline__ void dstencil(const Dense &L, const float *buf, int x, int y, int z, float &dg, float &sum)
+__device__ __forceinline__ void dstencil(const DenseLevel &L, const float *buf, int x, int y, int z, float &dg,
+                                         float &sum)
 {
     dg = kDiagEps;
     sum = 0.0f;
@@ -123,26 +123,9 @@ __global__ __launch_bounds__(256) void k_finalize(int kind, const double *partia
     }
 }
 
-// ---- dense coarse levels ------------------------------------------------------------------------------------
+// ---- dense levels (>= 4) ----------------------------------------------------------------------------------
 // SloshPressureMG.usf:256-278 (MGBuildTypeCS): AIR if any child is air, else FLUID if any child is fluid.
-__global__ void k_build_type1(Geo g, const uint32_t *ctype, Dense C, uint8_t *type)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= C.cells) return;
-    int x, y, z;
-    dcoord(C, i, x, y, z);
-    bool air = false, fluid = false;
-    for (uint32_t oi = 0; oi < 8u; ++oi) {
-        int fx = 2 * x + (int)(oi & 1u), fy = 2 * y + (int)((oi >> 1) & 1u), fz = 2 * z + (int)(oi >> 2);
-        if (!in_grid(g, fx, fy, fz)) continue;
-        uint32_t t = ctype[cell_index(g, fx, fy, fz)] & kCellTypeMask;
-        air = air || t == kCellAir;
-        fluid = fluid || t == kCellFluid;
-    }
-    type[i] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
-}
-
-__global__ void k_build_type(Dense F, Dense C, uint8_t *type)
+__global__ void k_build_type(DenseLevel F, DenseLevel C)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C.cells) return;
@@ -156,31 +139,28 @@ __global__ void k_build_type(Dense F, Dense C, uint8_t *type)
         air = air || t == kCellAir;
         fluid = fluid || t == kCellFluid;
     }
-    type[i] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
+    C.type[i] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
 }
 
-// One dense-level cell of each V-cycle operation; the multi-workgroup kernels and the single-workgroup coarse
-// tail share them.
-__device__ __forceinline__ void smooth_cell(const Dense &L, float *x, const float *b, uint32_t colour, uint32_t i)
+__device__ __forceinline__ void smooth_cell(const DenseLevel &L, uint32_t colour, uint32_t i)
 {
     if (L.type[i] != kCellFluid) return;
     int cx, cy, cz;
     dcoord(L, i, cx, cy, cz);
     if ((is_red(cx, cy, cz) ? 0u : 1u) != colour) return;
     float dg, s;
-    dstencil(L, x, cx, cy, cz, dg, s);
-    x[i] = (b[i] + s) / dg;
+    dstencil(L, L.x, cx, cy, cz, dg, s);
+    L.x[i] = (L.b[i] + s) / dg;
 }
 
 // SloshPressureMG.usf:343-381 (MGRestrictCS)
-__device__ __forceinline__ void restrict_cell(const Dense &F, const float *xf, const float *bf, const Dense &C,
-                                              float *xc, float *bc, uint32_t ci)
+__device__ __forceinline__ void restrict_cell(const DenseLevel &F, const DenseLevel &C, uint32_t ci)
 {
     int cx, cy, cz;
     dcoord(C, ci, cx, cy, cz);
     if (C.type[ci] != kCellFluid) {
-        bc[ci] = 0.0f;
-        xc[ci] = 0.0f;
+        C.b[ci] = 0.0f;
+        C.x[ci] = 0.0f;
         return;
     }
     float sum = 0.0f;
@@ -189,56 +169,122 @@ __device__ __forceinline__ void restrict_cell(const Dense &F, const float *xf, c
         if (dtype_at(F, fx, fy, fz) != kCellFluid) continue;
         uint32_t fi = (uint32_t)(fx + F.dim[0] * (fy + F.dim[1] * fz));
         float dg, s;
-        dstencil(F, xf, fx, fy, fz, dg, s);
-        sum += bf[fi] - (dg * xf[fi] - s);
+        dstencil(F, F.x, fx, fy, fz, dg, s);
+        sum += F.b[fi] - (dg * F.x[fi] - s);
     }
     float bv = 0.25f * sum;
-    bc[ci] = bv;
+    C.b[ci] = bv;
     float cdg = kDiagEps;
     for (int nb = 0; nb < 6; ++nb) {
         int d[3];
         nb_offset(nb, d);
         cdg += dtype_at(C, cx + d[0], cy + d[1], cz + d[2]) != kCellSolid ? 1.0f : 0.0f;
     }
-    xc[ci] = is_red(cx, cy, cz) ? bv / cdg : 0.0f;
+    C.x[ci] = is_red(cx, cy, cz) ? bv / cdg : 0.0f;
 }
 
 // SloshPressureMG.usf:418-433 (MGProlongCS)
-__device__ __forceinline__ void prolong_cell(const Dense &F, float *xf, const Dense &C, const float *xc, uint32_t i)
+__device__ __forceinline__ void prolong_cell(const DenseLevel &F, const DenseLevel &C, uint32_t i)
 {
     if (F.type[i] != kCellFluid) return;
     int x, y, z;
     dcoord(F, i, x, y, z);
     uint32_t pi = (uint32_t)((x >> 1) + C.dim[0] * ((y >> 1) + C.dim[1] * (z >> 1)));
-    if (C.type[pi] == kCellFluid) xf[i] += xc[pi];
+    if (C.type[pi] == kCellFluid) F.x[i] += C.x[pi];
 }
 
-__global__ void k_smooth(Dense L, float *x, const float *b, uint32_t colour, const SolverState *st)
+__global__ void k_dsmooth(DenseLevel L, uint32_t colour, const SolverState *st)
 {
     if (st->done) return;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < L.cells) smooth_cell(L, x, b, colour, i);
+    if (i < L.cells) smooth_cell(L, colour, i);
 }
 
-__global__ void k_restrict(Dense F, const float *xf, const float *bf, Dense C, float *xc, float *bc,
-                           const SolverState *st)
-{
-    if (st->done) return;
-    uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci < C.cells) restrict_cell(F, xf, bf, C, xc, bc, ci);
-}
-
-__global__ void k_prolong(Dense F, float *xf, Dense C, const float *xc, const SolverState *st)
+__global__ void k_drestrict(DenseLevel F, DenseLevel C, const SolverState *st)
 {
     if (st->done) return;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < F.cells) prolong_cell(F, xf, C, xc, i);
+    if (i < C.cells) restrict_cell(F, C, i);
+}
+
+__global__ void k_dprolong(DenseLevel F, DenseLevel C, const SolverState *st)
+{
+    if (st->done) return;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < F.cells) prolong_cell(F, C, i);
+}
+
+struct TailLevels {
+    int n;                       // levels t .. m-1, the last one the coarsest
+    DenseLevel L[kMaxLevels];
+};
+
+// The dense levels of the V-cycle (SloshProxy.cpp:904-959) in one 1024-thread workgroup, entirely in LDS: black
+// pre-sweep of the first tail level, restrictions with fused red sweeps and black pre-sweeps down to the
+// coarsest level, the exact FP64 coarsest solve, then prolongation with black and red post-sweeps back up. Only
+// the first level's right-hand side and initial guess come from global memory, and only its result goes back.
+__global__ __launch_bounds__(1024) void k_coarse_tail(TailLevels T, const uint32_t *cidx, const uint32_t *cn,
+                                                      const double *inv, const SolverState *st)
+{
+    if (st->done) return;
+    constexpr int N = (int)kCoarsestCells;
+    __shared__ uint8_t s_type[kTailCells];
+    __shared__ float s_x[kTailCells], s_b[kTailCells];
+    __shared__ double bs[N];
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    DenseLevel L[kMaxLevels];
+    uint32_t off = 0;
+    for (int k = 0; k < T.n; ++k) {
+        L[k] = T.L[k];
+        L[k].type = s_type + off;
+        L[k].x = s_x + off;
+        L[k].b = s_b + off;
+        for (uint32_t i = tid; i < L[k].cells; i += nt) L[k].type[i] = T.L[k].type[i];
+        off += L[k].cells;
+    }
+    for (uint32_t i = tid; i < L[0].cells; i += nt) {
+        L[0].x[i] = T.L[0].x[i];
+        L[0].b[i] = T.L[0].b[i];
+    }
+    __syncthreads();
+    const int m = T.n;
+    if (m > 1) {
+        for (uint32_t i = tid; i < L[0].cells; i += nt) smooth_cell(L[0], 1u, i);
+        __syncthreads();
+    }
+    for (int l = 0; l < m - 1; ++l) {
+        for (uint32_t ci = tid; ci < L[l + 1].cells; ci += nt) restrict_cell(L[l], L[l + 1], ci);
+        __syncthreads();
+        if (l + 1 < m - 1) {
+            for (uint32_t i = tid; i < L[l + 1].cells; i += nt) smooth_cell(L[l + 1], 1u, i);
+            __syncthreads();
+        }
+    }
+    const int n = (int)*cn;
+    if ((int)tid < n) bs[tid] = (double)L[m - 1].b[cidx[tid]];
+    __syncthreads();
+    if ((int)tid < n) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += inv[(size_t)tid * N + j] * bs[j];
+        L[m - 1].x[cidx[tid]] = (float)s;
+    }
+    __syncthreads();
+    for (int l = m - 2; l >= 0; --l) {
+        for (uint32_t i = tid; i < L[l].cells; i += nt) prolong_cell(L[l], L[l + 1], i);
+        __syncthreads();
+        for (uint32_t i = tid; i < L[l].cells; i += nt) smooth_cell(L[l], 1u, i);
+        __syncthreads();
+        for (uint32_t i = tid; i < L[l].cells; i += nt) smooth_cell(L[l], 0u, i);
+        __syncthreads();
+    }
+    for (uint32_t i = tid; i < L[0].cells; i += nt) T.L[0].x[i] = L[0].x[i];
 }
 
 // ---- coarsest level: FP64 Cholesky of the <= 64 fluid unknowns and its explicit inverse (once per solve) ----
-// Level 0 is the coarsest only for grids of <= 64 cells (one brick); then the pool diagonal is used.
-__global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ctype, const float *diag, Dense C,
-                                                       int level0, uint32_t *cidx, uint32_t *cn, double *inv)
+// The coarsest level is level 0 (mode 0, grids of <= 64 cells), a pooled level (mode 1) or a dense level (mode 2).
+__global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ctype, const float *diag, int mode,
+                                                       PLevel P, DenseLevel D, uint32_t *cidx, uint32_t *cn,
+                                                       double *inv)
 {
     constexpr int N = (int)kCoarsestCells;
     __shared__ double a[N][N + 1];
@@ -246,12 +292,20 @@ __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ct
     __shared__ uint32_t pidx[N];
     __shared__ uint32_t n_s;
     const int t = threadIdx.x;
-    const int dims[3] = {level0 ? g.n[0] : C.dim[0], level0 ? g.n[1] : C.dim[1], level0 ? g.n[2] : C.dim[2]};
+    const int dims[3] = {mode == 0 ? g.n[0] : (mode == 1 ? P.dim[0] : D.dim[0]),
+                         mode == 0 ? g.n[1] : (mode == 1 ? P.dim[1] : D.dim[1]),
+                         mode == 0 ? g.n[2] : (mode == 1 ? P.dim[2] : D.dim[2])};
     const uint32_t cells = (uint32_t)(dims[0] * dims[1] * dims[2]);
+    auto index_of = [&](int x, int y, int z) -> uint32_t {
+        if (mode == 0) return cell_index(g, x, y, z);
+        if (mode == 1) return pindex(g, P, x, y, z);
+        return (uint32_t)(x + dims[0] * (y + dims[1] * z));
+    };
     auto type_of = [&](int x, int y, int z) -> uint32_t {
         if (x < 0 || y < 0 || z < 0 || x >= dims[0] || y >= dims[1] || z >= dims[2]) return kCellSolid;
-        if (level0) return ctype[cell_index(g, x, y, z)] & kCellTypeMask;
-        return C.type[(uint32_t)(x + dims[0] * (y + dims[1] * z))];
+        uint32_t i = index_of(x, y, z);
+        if (mode == 0) return ctype[i] & kCellTypeMask;
+        return mode == 1 ? P.type[i] : D.type[i];
     };
     if (t == 0) {
         uint32_t n = 0;
@@ -262,7 +316,7 @@ __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ct
             px[n] = x;
             py[n] = y;
             pz[n] = z;
-            pidx[n] = level0 ? cell_index(g, x, y, z) : i;
+            pidx[n] = index_of(x, y, z);
             n++;
         }
         n_s = n;
@@ -286,7 +340,7 @@ __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ct
                 for (int j = 0; j < n; ++j)
                     if (px[j] == nx && py[j] == ny && pz[j] == nz) a[t][j] = -1.0;
         }
-        if (level0) dg = diag[pidx[t]] + kDiagEps;
+        if (mode == 0) dg = diag[pidx[t]] + kDiagEps;
         a[t][t] = (double)dg;
     }
     __syncthreads();
@@ -322,8 +376,8 @@ __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ct
     }
 }
 
-// Exact coarsest solve x = A^-1 b in FP64 (replaces MGCoarseSolveCS, SloshPressureMG.usf:386-413), for the case
-// where level 0 itself is the coarsest level.
+// Exact coarsest solve x = A^-1 b in FP64 (replaces MGCoarseSolveCS, SloshPressureMG.usf:386-413) when the
+// coarsest level is level 0 or pooled; b and x are indexed by the factor's cell list.
 __global__ __launch_bounds__(64) void k_coarse_solve(const uint32_t *cidx, const uint32_t *cn, const double *inv,
                                                      const float *b, float *x, const SolverState *st)
 {
@@ -340,55 +394,6 @@ __global__ __launch_bounds__(64) void k_coarse_solve(const uint32_t *cidx, const
     x[cidx[t]] = (float)s;
 }
 
-struct Tail {
-    Dense L[kMaxLevels];
-    float *x[kMaxLevels];
-    float *b[kMaxLevels];
-};
-
-// The small levels t..m-1 of the V-cycle (SloshProxy.cpp:904-959) in one 1024-thread workgroup: black
-// pre-sweep of level t, restrictions with fused red sweeps and black pre-sweeps down to the coarsest level, the
-// exact FP64 coarsest solve, then prolongation with black and red post-sweeps back up to level t. One launch
-// replaces ~5 per level; the levels (<= 4096 cells) stay in L2.
-__global__ __launch_bounds__(1024) void k_coarse_tail(Tail T, int t, int m, const uint32_t *cidx, const uint32_t *cn,
-                                                      const double *inv, const SolverState *st)
-{
-    if (st->done) return;
-    constexpr int N = (int)kCoarsestCells;
-    __shared__ double bs[N];
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    if (t < m - 1) {
-        for (uint32_t i = tid; i < T.L[t].cells; i += nt) smooth_cell(T.L[t], T.x[t], T.b[t], 1u, i);
-        __syncthreads();
-    }
-    for (int l = t; l < m - 1; ++l) {
-        for (uint32_t ci = tid; ci < T.L[l + 1].cells; ci += nt)
-            restrict_cell(T.L[l], T.x[l], T.b[l], T.L[l + 1], T.x[l + 1], T.b[l + 1], ci);
-        __syncthreads();
-        if (l + 1 < m - 1) {
-            for (uint32_t i = tid; i < T.L[l + 1].cells; i += nt) smooth_cell(T.L[l + 1], T.x[l + 1], T.b[l + 1], 1u, i);
-            __syncthreads();
-        }
-    }
-    const int n = (int)*cn;
-    if ((int)tid < n) bs[tid] = (double)T.b[m - 1][cidx[tid]];
-    __syncthreads();
-    if ((int)tid < n) {
-        double s = 0.0;
-        for (int j = 0; j < n; ++j) s += inv[(size_t)tid * N + j] * bs[j];
-        T.x[m - 1][cidx[tid]] = (float)s;
-    }
-    __syncthreads();
-    for (int l = m - 2; l >= t; --l) {
-        for (uint32_t i = tid; i < T.L[l].cells; i += nt) prolong_cell(T.L[l], T.x[l], T.L[l + 1], T.x[l + 1], i);
-        __syncthreads();
-        for (uint32_t i = tid; i < T.L[l].cells; i += nt) smooth_cell(T.L[l], T.x[l], T.b[l], 1u, i);
-        __syncthreads();
-        for (uint32_t i = tid; i < T.L[l].cells; i += nt) smooth_cell(T.L[l], T.x[l], T.b[l], 0u, i);
-        __syncthreads();
-    }
-}
-
 __global__ void k_state_reset(SolverState *st, double tol, int maxit)
 {
     st->tol = tol;
@@ -400,17 +405,6 @@ __global__ void k_state_reset(SolverState *st, double tol, int maxit)
     st->alpha = 0.0;
     st->beta = 0.0;
     st->rel = 0.0;
-}
-
-Dense dense_of(const slosh_ctx *c, int l)
-{
-    Dense D;
-    D.dim[0] = c->lv[l].dim[0];
-    D.dim[1] = c->lv[l].dim[1];
-    D.dim[2] = c->lv[l].dim[2];
-    D.cells = c->lv[l].cells;
-    D.type = c->lv[l].type;
-    return D;
 }
 
 void finalize(slosh_ctx *c, int kind, int iters = 0)
@@ -425,19 +419,8 @@ void smooth_level(slosh_ctx *c, int l, uint32_t colour, int want_dot)
         l0_smooth(c, colour, want_dot);
         if (want_dot) finalize(c, kFinRho);
     } else {
-        Dense L = dense_of(c, l);
-        SLOSH_TIMED(c, K_SMOOTH);
-        hipLaunchKernelGGL(k_smooth, dim3(blocks_for(L.cells, 256)), dim3(256), 0, c->stream, L, c->lv[l].x,
-                           c->lv[l].b, colour, c->state);
+        pooled_smooth(c, l, colour);
     }
-}
-
-// First level handled by the single-workgroup tail: the first level >= 1 with at most 4096 cells.
-int tail_level(const slosh_ctx *c)
-{
-    for (int l = 1; l < c->nlevels; ++l)
-        if (c->lv[l].cells <= 4096u) return l;
-    return c->nlevels - 1;
 }
 
 // SloshProxy.cpp:904-959 (VCycle) with the exact coarsest solve. The level-0 red pre-sweep is already in z.
@@ -453,47 +436,61 @@ void vcycle(slosh_ctx *c)
         smooth_level(c, 0, 0u, 1);
         return;
     }
-    const int t = tail_level(c);
+    // multi-workgroup levels: 0 .. top, where top is the coarsest pooled level
+    const int top = m - 1 < kFirstDense ? m - 1 : kFirstDense - 1;
     smooth_level(c, 0, 1u, 0);
-    for (int l = 0; l < t; ++l) {
-        if (l == 0) {
-            l0_restrict(c);
-        } else {
-            Dense C = dense_of(c, l + 1);
-            Dense F = dense_of(c, l);
-            SLOSH_TIMED(c, K_RESTRICT);
-            hipLaunchKernelGGL(k_restrict, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, F, c->lv[l].x,
-                               c->lv[l].b, C, c->lv[l + 1].x, c->lv[l + 1].b, c->state);
-        }
-        if (l + 1 < t) smooth_level(c, l + 1, 1u, 0);
+    for (int l = 0; l < top; ++l) {
+        if (l == 0) l0_restrict(c);
+        else pooled_restrict(c, l);
+        if (l + 1 < m - 1) smooth_level(c, l + 1, 1u, 0);
     }
-    {
-        Tail T;
-        for (int l = 0; l < kMaxLevels; ++l) {
-            if (l >= 1 && l < m) {
-                T.L[l] = dense_of(c, l);
-                T.x[l] = c->lv[l].x;
-                T.b[l] = c->lv[l].b;
-            } else {
-                T.L[l] = Dense{{0, 0, 0}, 0u, nullptr};
-                T.x[l] = nullptr;
-                T.b[l] = nullptr;
+    if (m - 1 == top) {
+        // the coarsest level is pooled (or level 1..3): exact solve on it
+        SLOSH_TIMED(c, K_COARSE_SOLVE);
+        hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(64), 0, c->stream, c->coarse_idx, c->coarse_n, c->coarse_inv,
+                           c->lv[top].b, c->lv[top].x, c->state);
+    } else {
+        // dense levels kFirstDense .. t-1 as grid-wide passes, levels t .. m-1 in the LDS tail
+        int t = m - 1;
+        uint32_t tail_cells = c->lv[m - 1].cells;
+        while (t - 1 >= kFirstDense && tail_cells + c->lv[t - 1].cells <= kTailCells) tail_cells += c->lv[--t].cells;
+        restrict_pooled_to_dense(c, top);
+        for (int l = kFirstDense; l < t; ++l) {
+            DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
+            {
+                SLOSH_TIMED(c, K_SMOOTH);
+                hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, 1u, c->state);
+            }
+            SLOSH_TIMED(c, K_RESTRICT);
+            hipLaunchKernelGGL(k_drestrict, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
+        }
+        TailLevels T;
+        T.n = m - t;
+        for (int k = 0; k < kMaxLevels; ++k) T.L[k] = k < T.n ? dlevel_of(c, t + k) : DenseLevel{};
+        {
+            SLOSH_TIMED(c, K_COARSE_TAIL);
+            hipLaunchKernelGGL(k_coarse_tail, dim3(1), dim3(1024), 0, c->stream, T, c->coarse_idx, c->coarse_n,
+                               c->coarse_inv, c->state);
+        }
+        for (int l = t - 1; l >= kFirstDense; --l) {
+            DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
+            {
+                SLOSH_TIMED(c, K_PROLONG);
+                hipLaunchKernelGGL(k_dprolong, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
+            }
+            for (uint32_t colour : {1u, 0u}) {
+                SLOSH_TIMED(c, K_SMOOTH);
+                hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, colour,
+                                   c->state);
             }
         }
-        SLOSH_TIMED(c, K_COARSE_TAIL);
-        hipLaunchKernelGGL(k_coarse_tail, dim3(1), dim3(1024), 0, c->stream, T, t, m, c->coarse_idx, c->coarse_n,
-                           c->coarse_inv, c->state);
+        prolong_dense_to_pooled(c, top);
+        smooth_level(c, top, 1u, 0);
+        smooth_level(c, top, 0u, 0);
     }
-    for (int l = t - 1; l >= 0; --l) {
-        if (l == 0) {
-            l0_prolong(c);
-        } else {
-            Dense C = dense_of(c, l + 1);
-            Dense F = dense_of(c, l);
-            SLOSH_TIMED(c, K_PROLONG);
-            hipLaunchKernelGGL(k_prolong, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, c->lv[l].x, C,
-                               c->lv[l + 1].x, c->state);
-        }
+    for (int l = top - 1; l >= 0; --l) {
+        if (l == 0) l0_prolong(c);
+        else pooled_prolong(c, l);
         smooth_level(c, l, 1u, 0);
         smooth_level(c, l, 0u, l == 0 ? 1 : 0);
     }
@@ -509,22 +506,24 @@ void run_mgpcg(slosh_ctx *c)
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
     const int m = c->nlevels;
     for (int l = 1; l < m; ++l) {
-        Dense C = dense_of(c, l);
-        SLOSH_TIMED(c, K_MG_TYPES);
-        if (l == 1) {
-            hipLaunchKernelGGL(k_build_type1, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, c->geo, c->ctype,
-                               C, c->lv[1].type);
+        if (c->lv[l].pooled) {
+            pooled_build_type(c, l);
+        } else if (c->lv[l - 1].pooled) {
+            dense_build_type_from_pooled(c, l);
         } else {
-            Dense F = dense_of(c, l - 1);
-            hipLaunchKernelGGL(k_build_type, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, F, C,
-                               c->lv[l].type);
+            DenseLevel C = dlevel_of(c, l);
+            SLOSH_TIMED(c, K_MG_TYPES);
+            hipLaunchKernelGGL(k_build_type, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, dlevel_of(c, l - 1),
+                               C);
         }
     }
     {
-        Dense C = dense_of(c, m - 1);
+        const int mode = m == 1 ? 0 : (c->lv[m - 1].pooled ? 1 : 2);
+        PLevel P = mode == 1 ? plevel_of(c, m - 1) : PLevel{};
+        DenseLevel D = mode == 2 ? dlevel_of(c, m - 1) : DenseLevel{};
         SLOSH_TIMED(c, K_COARSE_FACTOR);
-        hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(256), 0, c->stream, c->geo, c->ctype, c->diag, C,
-                           m == 1 ? 1 : 0, c->coarse_idx, c->coarse_n, c->coarse_inv);
+        hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(256), 0, c->stream, c->geo, c->ctype, c->diag, mode, P, D,
+                           c->coarse_idx, c->coarse_n, c->coarse_inv);
     }
     l0_pcg_init(c);
     finalize(c, kFinInit);
