@@ -20,8 +20,8 @@ namespace slosh {
 const char *const kKernelNames[K_COUNT] = {
     "k_emit_fill", "k_sort_tiles", "k_sort_runs_scatter", "k_sort_key_runs", "k_sort_scatter", "k_scan",
     "k_pool", "k_p2g", "k_normalize", "k_divergence", "k_project", "k_div_stats", "k_extrapolate", "k_g2p",
-    "k_build_type", "k_coarse_factor", "k_pcg_init", "k_smooth0", "k_smooth", "k_restrict0", "k_restrict",
-    "k_prolong0", "k_prolong", "k_coarse_solve", "k_apply_a", "k_update_pr", "k_update_d", "k_sor_sweep",
+    "k_mg_setup", "k_coarse_factor", "k_pcg_init", "k_red_dot", "k_dsmooth", "k_restrict0", "k_pdown",
+    "k_prolong_black", "k_pup", "k_coarse_solve", "k_direction", "k_update_black", "k_update_d", "k_sor_sweep",
     "k_sor_residual", "k_fill", "k_finalize", "k_coarse_tail", "k_step_stats"};
 } // namespace slosh
 
@@ -63,9 +63,11 @@ void free_all(slosh_ctx *c)
     dfree(c->tiles); dfree(c->tile_valid);
     for (int k = 0; k < 3; ++k) dfree(c->vel[k]);
     dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag); dfree(c->fmask);
-    dfree(c->r); dfree(c->z); dfree(c->d); dfree(c->q);
+    dfree(c->r); dfree(c->r2); dfree(c->z); dfree(c->z1); dfree(c->d); dfree(c->d2); dfree(c->q);
+    dfree(c->binfo); dfree(c->res3);
     for (int l = 1; l < kMaxLevels; ++l) {
-        dfree(c->lv[l].type); dfree(c->lv[l].x); dfree(c->lv[l].b);
+        dfree(c->lv[l].type); dfree(c->lv[l].info); dfree(c->lv[l].coord);
+        dfree(c->lv[l].x); dfree(c->lv[l].x1); dfree(c->lv[l].b);
     }
     dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->partials);
     dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
@@ -184,9 +186,14 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->diag, pc);
     dalloc(&c->fmask, pc);
     dalloc(&c->r, pc);
+    dalloc(&c->r2, pc);
     dalloc(&c->z, pc);
+    dalloc(&c->z1, pc);
     dalloc(&c->d, pc);
+    dalloc(&c->d2, pc);
     dalloc(&c->q, pc);
+    dalloc(&c->binfo, (size_t)g.npool * kBinfoWords);
+    dalloc(&c->res3, g.nbricks);
     // MG hierarchy: halve (rounding up) until a level holds <= 64 cells or 8 levels exist. Levels 1..3 live in the
     // brick pool ((8 >> l)^3 cells per slot), coarser ones are dense. dalloc zeroes them: the null slot of a
     // pooled level must read as AIR with x = b = 0.
@@ -201,11 +208,24 @@ void create_impl(slosh_ctx *c)
         Level &l = c->lv[li];
         for (int a = 0; a < 3; ++a) l.dim[a] = (p.dim[a] + 1) / 2;
         l.cells = (uint32_t)l.dim[0] * (uint32_t)l.dim[1] * (uint32_t)l.dim[2];
-        l.pooled = li <= 3;
+        l.pooled = li <= 2;
         const size_t n = l.pooled ? (size_t)g.npool * (size_t)(512u >> (3 * li)) : (size_t)l.cells;
         dalloc(&l.type, n);
+        dalloc(&l.info, n);
         dalloc(&l.x, n);
         dalloc(&l.b, n);
+        dalloc(&l.x1, n);
+        if (!l.pooled) {
+            if (l.dim[0] > 1024 || l.dim[1] > 1024 || l.dim[2] > 1024) throw std::bad_alloc();
+            std::vector<uint32_t> coord(n);
+            for (uint32_t i = 0; i < l.cells; ++i) {
+                const uint32_t x = i % (uint32_t)l.dim[0], y = (i / (uint32_t)l.dim[0]) % (uint32_t)l.dim[1],
+                               z = i / (uint32_t)(l.dim[0] * l.dim[1]);
+                coord[i] = x | y << 10 | z << 20;
+            }
+            dalloc(&l.coord, n);
+            SLOSH_HIP(hipMemcpy(l.coord, coord.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
         c->nlevels++;
     }
     if (c->lv[c->nlevels - 1].cells > kCoarsestCells) throw std::bad_alloc();

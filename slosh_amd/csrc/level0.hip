@@ -3,12 +3,17 @@
 // SloshPressureMG.usf's level-0 kernels (PCGInitCS :284-305, MGSmoothCS :311-338, MGRestrictCS :343-381,
 // MGProlongCS :418-433, PCGApplyACS :438-454, PCGUpdatePRCS :495-524, PCGUpdateDCS :527-543) and
 // Slosh.usf:856-892 (PressureRBGSCS), restated for the MI355X:
-//   - a capped grid of 256-thread workgroups strides over the allocated bricks, so a reduction leaves at most
-//     kMaxGroups partials for its finalize kernel;
+//   - one workgroup per allocated brick, 256 threads, thread t owning the red/black pair of cells
+//     (colour_cell(t, 0), colour_cell(t, 1)); the brick's slot and its six face neighbours' slots come from a
+//     per-solve table of every allocated brick's 3x3x3 neighbourhood (scalar loads), so no pass walks the brick
+//     table;
 //   - the stencil reads a per-cell mask of fluid neighbours (written by the divergence pass) instead of six
-//     neighbour cell types, and neighbours in another brick are addressed through the six face-neighbour slots of
-//     the brick, which are wave-uniform (scalar loads), instead of one brick-table lookup per neighbour;
-//   - a red-black sweep maps the 256 threads onto the 256 cells of one colour, so no lane idles.
+//     neighbour cell types;
+//   - passes that the reference runs back to back are fused where the second needs only values the first can
+//     recompute for a neighbour: the residual update with the red initialisation and the black sweep, the
+//     prolongation with the black sweep, and the direction update with the operator application. The residual,
+//     the iterate and the direction ping-pong between buffers so that no workgroup reads a value another one is
+//     overwriting in the same pass. Every value is computed with the reference's expression and operand order.
 #include "slosh_ctx.h"
 
 namespace slosh {
@@ -16,46 +21,44 @@ namespace slosh {
 namespace {
 
 constexpr uint32_t kThreads = 256;
-constexpr uint32_t kMaxGroups = 2048;
 constexpr uint32_t kSelfFluid = 1u << 6;
 
 // Wave-uniform description of one allocated brick: its slot base and the slot bases of its six face neighbours
 // (0 = null brick, which only masked-out neighbours can point into).
 struct Brick {
-    int bc[3];
     uint32_t base;
     uint32_t nb[6];
 };
 
-__device__ __forceinline__ Brick brick_at(const Geo &g, uint32_t e)
+// Entries of the 27-neighbourhood brick table (index (dx+1) + 3(dy+1) + 9(dz+1)) of the brick itself and of its
+// face neighbours in the order -x +x -y +y -z +z.
+__device__ __forceinline__ Brick load_brick(const uint32_t *binfo, uint32_t e)
 {
+    constexpr int kFace[6] = {12, 14, 10, 16, 4, 22};
     Brick B;
-    const uint32_t b = g.active[e];
-    brick_coord(g, b, B.bc);
-    B.base = g.table[b] * kBrickCells;
-    for (int n = 0; n < 6; ++n) {
-        int d[3];
-        nb_offset(n, d);
-        int x = B.bc[0] + d[0], y = B.bc[1] + d[1], z = B.bc[2] + d[2];
-        uint32_t s = 0u;
-        if (x >= 0 && y >= 0 && z >= 0 && x < g.bd[0] && y < g.bd[1] && z < g.bd[2])
-            s = g.table[(uint32_t)(x + g.bd[0] * (y + g.bd[1] * z))];
-        B.nb[n] = s * kBrickCells;
-    }
+    const uint32_t *p = binfo + (size_t)e * kBinfoWords;
+    B.base = p[13] * kBrickCells;
+    for (int n = 0; n < 6; ++n) B.nb[n] = p[kFace[n]] * kBrickCells;
     return B;
 }
 
-// Pool index of neighbour n (order -x +x -y +y -z +z) of local cell l.
-__device__ __forceinline__ uint32_t nbr(const Brick &B, uint32_t l, int n)
+// Pool index of neighbour n (order -x +x -y +y -z +z) of local cell l, and its local index in its own brick.
+__device__ __forceinline__ uint32_t nbr(const Brick &B, uint32_t l, int n, uint32_t &nl)
 {
     switch (n) {
-    case 0: return (l & 7u) ? B.base + l - 1u : B.nb[0] + l + 7u;
-    case 1: return ((l & 7u) != 7u) ? B.base + l + 1u : B.nb[1] + l - 7u;
-    case 2: return (l & 56u) ? B.base + l - 8u : B.nb[2] + l + 56u;
-    case 3: return ((l & 56u) != 56u) ? B.base + l + 8u : B.nb[3] + l - 56u;
-    case 4: return (l & 448u) ? B.base + l - 64u : B.nb[4] + l + 448u;
-    default: return ((l & 448u) != 448u) ? B.base + l + 64u : B.nb[5] + l - 448u;
+    case 0: nl = (l & 7u) ? l - 1u : l + 7u; return ((l & 7u) ? B.base : B.nb[0]) + nl;
+    case 1: nl = ((l & 7u) != 7u) ? l + 1u : l - 7u; return (((l & 7u) != 7u) ? B.base : B.nb[1]) + nl;
+    case 2: nl = (l & 56u) ? l - 8u : l + 56u; return ((l & 56u) ? B.base : B.nb[2]) + nl;
+    case 3: nl = ((l & 56u) != 56u) ? l + 8u : l - 56u; return (((l & 56u) != 56u) ? B.base : B.nb[3]) + nl;
+    case 4: nl = (l & 448u) ? l - 64u : l + 448u; return ((l & 448u) ? B.base : B.nb[4]) + nl;
+    default: nl = ((l & 448u) != 448u) ? l + 64u : l - 448u; return (((l & 448u) != 448u) ? B.base : B.nb[5]) + nl;
     }
+}
+
+__device__ __forceinline__ uint32_t nbr(const Brick &B, uint32_t l, int n)
+{
+    uint32_t nl;
+    return nbr(B, l, n, nl);
 }
 
 template <typename T, typename S> __device__ __forceinline__ T nbsum(const Brick &B, uint32_t l, uint32_t m, const S *v)
@@ -74,7 +77,11 @@ __device__ __forceinline__ uint32_t colour_cell(uint32_t t, uint32_t colour)
     return x + 8u * y + 64u * z;
 }
 
-__device__ __forceinline__ bool red_local(uint32_t l) { return (((l & 7u) + ((l >> 3) & 7u) + (l >> 6)) & 1u) == 0u; }
+// Level-1 parent (local index within the brick's 4^3 block) of level-0 local cell l.
+__device__ __forceinline__ uint32_t parent1(uint32_t l)
+{
+    return ((l & 7u) >> 1) + 4u * (((l >> 3) & 7u) >> 1) + 16u * ((l >> 6) >> 1);
+}
 
 template <int NV> __device__ void block_partial(const double (&v)[NV], double *partials)
 {
@@ -90,284 +97,298 @@ template <int NV> __device__ void block_partial(const double (&v)[NV], double *p
         for (int k = 0; k < NV; ++k) partials[(size_t)blockIdx.x * NV + k] = sh[k][0];
 }
 
-// r = b - A p (FP64 p), z_red = r / diag, z_black = 0; partials ||b||^2, ||r||^2.
-__global__ __launch_bounds__(kThreads) void k_pcg_init(Geo g, uint32_t nactive, const uint8_t *fmask, const float *div,
-                                                       const float *diag, const double *pres, float *r, float *z,
-                                                       float *d, double *partials)
+// Slots of the 3x3x3 brick neighbourhood (0 outside the grid or unallocated) and the brick index of every
+// allocated brick, once per solve.
+__global__ void k_brick_info(Geo g, uint32_t nactive, uint32_t *binfo)
 {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nactive) return;
+    const uint32_t b = g.active[e];
+    int bc[3];
+    brick_coord(g, b, bc);
+    uint32_t *p = binfo + (size_t)e * kBinfoWords;
+    for (int k = 0; k < 27; ++k) {
+        const int x = bc[0] + k % 3 - 1, y = bc[1] + (k / 3) % 3 - 1, z = bc[2] + k / 9 - 1;
+        uint32_t s = 0u;
+        if (x >= 0 && y >= 0 && z >= 0 && x < g.bd[0] && y < g.bd[1] && z < g.bd[2])
+            s = g.table[(uint32_t)(x + g.bd[0] * (y + g.bd[1] * z))];
+        p[k] = s;
+    }
+    p[27] = b;
+}
+
+// r = b - A p (FP64 p); partials ||b||^2, ||r||^2.
+__global__ __launch_bounds__(kThreads) void k_pcg_init(const uint32_t *binfo, const uint8_t *fmask, const float *div,
+                                                       const float *diag, const double *pres, float *r,
+                                                       double *partials)
+{
+    const Brick B = load_brick(binfo, blockIdx.x);
     double v[2] = {0.0, 0.0};
-    for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
-        const Brick B = brick_at(g, e);
-        for (uint32_t l = threadIdx.x; l < kBrickCells; l += kThreads) {
-            const uint32_t i = B.base + l, m = fmask[i];
-            if (!(m & kSelfFluid)) {
-                r[i] = 0.0f;
-                z[i] = 0.0f;
-                d[i] = 0.0f;
-                continue;
-            }
-            double s = nbsum<double>(B, l, m, pres);
-            float dg = diag[i] + kDiagEps;
-            float ri = (float)(-(double)div[i] - ((double)dg * pres[i] - s));
-            r[i] = ri;
-            z[i] = red_local(l) ? ri / dg : 0.0f;
-            v[0] += (double)div[i] * (double)div[i];
-            v[1] += (double)ri * (double)ri;
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
+        if (!(m & kSelfFluid)) {
+            r[i] = 0.0f;
+            continue;
         }
+        double s = nbsum<double>(B, l, m, pres);
+        float dg = diag[i] + kDiagEps;
+        float ri = (float)(-(double)div[i] - ((double)dg * pres[i] - s));
+        r[i] = ri;
+        v[0] += (double)div[i] * (double)div[i];
+        v[1] += (double)ri * (double)ri;
     }
     block_partial<2>(v, partials);
 }
 
-// One Gauss-Seidel colour sweep x = (b + sum x_n) / diag (x = z, b = r); with want_dot also the r.z partials.
-__global__ __launch_bounds__(kThreads) void k_smooth0(Geo g, uint32_t nactive, const uint8_t *fmask,
-                                                      const float *diag, float *x, const float *bvec, uint32_t colour,
-                                                      int want_dot, double *partials, const SolverState *st)
+// The residual update of the iteration fused with the V-cycle's first level-0 sweeps (PCGUpdatePRCS with its
+// fused red initialisation, then MGSmoothCS black): red z = r / diag, black z = (r + sum_n z_n) / diag where
+// the red neighbours' z is recomputed from r_in and q.
+__global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo, const uint8_t *fmask,
+                                                           const float *diag, double *pres, const float *r_in,
+                                                           float *r_out, float *z, const float *d, const float *q,
+                                                           int first, double *partials, const SolverState *st)
 {
     if (st->done) return;
+    const double alpha = first ? 0.0 : st->alpha;
+    const Brick B = load_brick(binfo, blockIdx.x);
     double v[1] = {0.0};
-    const uint32_t l = colour_cell(threadIdx.x, colour);
-    for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
-        const Brick B = brick_at(g, e);
-        const uint32_t i = B.base + l, m = fmask[i];
-        if (m & kSelfFluid) {
-            float s = nbsum<float>(B, l, m, x);
-            float dg = diag[i] + kDiagEps;
-            float b = bvec[i];
-            float xv = (b + s) / dg;
-            x[i] = xv;
-            v[0] += (double)b * (double)xv;
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
+        if (!(m & kSelfFluid)) continue;
+        float ri;
+        if (first) {
+            ri = r_in[i];
+        } else {
+            pres[i] = pres[i] + alpha * (double)d[i];
+            ri = (float)((double)r_in[i] - alpha * (double)q[i]);
         }
-        if (want_dot) {
-            const uint32_t j = B.base + (l ^ 1u);
-            if (fmask[j] & kSelfFluid) v[0] += (double)bvec[j] * (double)x[j];
+        r_out[i] = ri;
+        v[0] += (double)ri * (double)ri;
+        float dg = diag[i] + kDiagEps;
+        if (c == 0u) {
+            z[i] = ri / dg;
+        } else {
+            float s = 0.0f;
+            for (int n = 0; n < 6; ++n) {
+                if (!(m >> n & 1u)) continue;
+                const uint32_t j = nbr(B, l, n);
+                float rj = first ? r_in[j] : (float)((double)r_in[j] - alpha * (double)q[j]);
+                s += rj / (diag[j] + kDiagEps);
+            }
+            z[i] = (ri + s) / dg;
         }
     }
-    if (want_dot) block_partial<1>(v, partials);
-}
-
-// Type of level-1 cell (local lx, ly, lz of brick B, possibly one step outside the brick) in the pooled level.
-__device__ __forceinline__ uint32_t type1_at(const Brick &B, const PLevel &L1, int lx, int ly, int lz)
-{
-    const int x = B.bc[0] * 4 + lx, y = B.bc[1] * 4 + ly, z = B.bc[2] * 4 + lz;
-    if (x < 0 || y < 0 || z < 0 || x >= L1.dim[0] || y >= L1.dim[1] || z >= L1.dim[2]) return kCellSolid;
-    uint32_t base = B.base >> 3; // slot * 64
-    if (lx < 0) base = B.nb[0] >> 3, lx += 4;
-    else if (lx > 3) base = B.nb[1] >> 3, lx -= 4;
-    else if (ly < 0) base = B.nb[2] >> 3, ly += 4;
-    else if (ly > 3) base = B.nb[3] >> 3, ly -= 4;
-    else if (lz < 0) base = B.nb[4] >> 3, lz += 4;
-    else if (lz > 3) base = B.nb[5] >> 3, lz -= 4;
-    return L1.type[base + (uint32_t)(lx + 4 * ly + 16 * lz)];
+    if (!first) block_partial<1>(v, partials);
 }
 
 // Restriction into pooled level 1: the residuals of the brick's 512 cells in LDS, then b_c = 1/4 of the fluid
 // children's sum for the brick's 4^3 level-1 cells, and x_c = b_c / diag_c on red cells (fused red sweep).
-__global__ __launch_bounds__(kThreads) void k_restrict0(Geo g, uint32_t nactive, const uint8_t *fmask,
-                                                        const float *diag, const float *xf, const float *bf, PLevel L1,
-                                                        const SolverState *st)
+__global__ __launch_bounds__(kThreads) void k_restrict0(const uint32_t *binfo, const uint8_t *fmask,
+                                                        const float *diag, const float *xf, const float *bf,
+                                                        PLevel L1, const SolverState *st)
 {
     if (st->done) return;
     __shared__ float res[kBrickCells];
     __shared__ uint8_t fl[kBrickCells];
-    for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
-        const Brick B = brick_at(g, e);
-        for (uint32_t l = threadIdx.x; l < kBrickCells; l += kThreads) {
-            const uint32_t i = B.base + l, m = fmask[i];
-            fl[l] = (uint8_t)((m & kSelfFluid) ? 1u : 0u);
-            if (m & kSelfFluid) {
-                float s = nbsum<float>(B, l, m, xf);
-                float dg = diag[i] + kDiagEps;
-                res[l] = bf[i] - (dg * xf[i] - s);
-            }
+    const Brick B = load_brick(binfo, blockIdx.x);
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
+        fl[l] = (uint8_t)((m & kSelfFluid) ? 1u : 0u);
+        if (m & kSelfFluid) {
+            float s = nbsum<float>(B, l, m, xf);
+            float dg = diag[i] + kDiagEps;
+            res[l] = bf[i] - (dg * xf[i] - s);
         }
-        __syncthreads();
-        if (threadIdx.x < 64u) {
-            const uint32_t t = threadIdx.x;
-            const int lx = (int)(t & 3u), ly = (int)((t >> 2) & 3u), lz = (int)(t >> 4);
-            const int cx = B.bc[0] * 4 + lx, cy = B.bc[1] * 4 + ly, cz = B.bc[2] * 4 + lz;
-            if (cx < L1.dim[0] && cy < L1.dim[1] && cz < L1.dim[2]) {
-                const uint32_t ci = (B.base >> 3) + t;
-                float *xc = L1.x, *bc = L1.b;
-                if (L1.type[ci] != kCellFluid) {
-                    bc[ci] = 0.0f;
-                    xc[ci] = 0.0f;
-                } else {
-                    float sum = 0.0f;
-                    for (uint32_t oi = 0; oi < 8u; ++oi) {
-                        uint32_t fx = 2u * lx + (oi & 1u), fy = 2u * ly + ((oi >> 1) & 1u), fz = 2u * lz + (oi >> 2);
-                        uint32_t fl_i = fx + 8u * fy + 64u * fz;
-                        if (fl[fl_i]) sum += res[fl_i];
-                    }
-                    float bv = 0.25f * sum;
-                    bc[ci] = bv;
-                    float cdg = kDiagEps;
-                    for (int nb = 0; nb < 6; ++nb) {
-                        int d[3];
-                        nb_offset(nb, d);
-                        uint32_t ty = type1_at(B, L1, lx + d[0], ly + d[1], lz + d[2]);
-                        cdg += ty != kCellSolid ? 1.0f : 0.0f;
-                    }
-                    xc[ci] = is_red(cx, cy, cz) ? bv / cdg : 0.0f;
-                }
-            }
-        }
-        __syncthreads();
     }
+    __syncthreads();
+    if (threadIdx.x >= 64u) return;
+    const uint32_t t = threadIdx.x;
+    const uint32_t lx = t & 3u, ly = (t >> 2) & 3u, lz = t >> 4;
+    const uint32_t ci = (B.base >> 3) + t;
+    const uint32_t info = L1.info[ci];
+    if (!(info & kInfoSelf)) {
+        L1.b[ci] = 0.0f;
+        L1.x[ci] = 0.0f;
+        return;
+    }
+    float sum = 0.0f;
+    for (uint32_t oi = 0; oi < 8u; ++oi) {
+        uint32_t fx = 2u * lx + (oi & 1u), fy = 2u * ly + ((oi >> 1) & 1u), fz = 2u * lz + (oi >> 2);
+        uint32_t f = fx + 8u * fy + 64u * fz;
+        if (fl[f]) sum += res[f];
+    }
+    float bv = 0.25f * sum;
+    L1.b[ci] = bv;
+    L1.x[ci] = (info & kInfoRed) ? bv / info_diag(info) : 0.0f;
 }
 
-// Prolongation from pooled level 1 (the parent lives in the same brick): x_f += x_c(parent) on fluid cells with
-// a fluid parent.
-__global__ __launch_bounds__(kThreads) void k_prolong0(Geo g, uint32_t nactive, const uint8_t *fmask, float *xf,
-                                                       PLevel L1, const SolverState *st)
+// Prolongation from pooled level 1 (x1) fused with the black sweep: red z1 = z + x1(parent), black
+// z1 = (r + sum_n (z_n + x1(parent_n))) / diag; a parent contributes only when it is fluid.
+__global__ __launch_bounds__(kThreads) void k_prolong_black(const uint32_t *binfo, const uint8_t *fmask,
+                                                            const float *diag, const float *r, const float *z,
+                                                            float *z1, PLevel L1, const SolverState *st)
 {
     if (st->done) return;
-    for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
-        const Brick B = brick_at(g, e);
-        for (uint32_t l = threadIdx.x; l < kBrickCells; l += kThreads) {
-            const uint32_t i = B.base + l;
-            if (!(fmask[i] & kSelfFluid)) continue;
-            const uint32_t pi = (B.base >> 3) + ((l & 7u) >> 1) + 4u * (((l >> 3) & 7u) >> 1) + 16u * ((l >> 6) >> 1);
-            if (L1.type[pi] == kCellFluid) xf[i] += L1.x[pi];
+    const Brick B = load_brick(binfo, blockIdx.x);
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
+        if (!(m & kSelfFluid)) continue;
+        if (c == 0u) {
+            float v = z[i];
+            const uint32_t pi = (B.base >> 3) + parent1(l);
+            if (L1.info[pi] & kInfoSelf) v += L1.x1[pi];
+            z1[i] = v;
+        } else {
+            float s = 0.0f;
+            for (int n = 0; n < 6; ++n) {
+                if (!(m >> n & 1u)) continue;
+                uint32_t nl;
+                const uint32_t j = nbr(B, l, n, nl);
+                float v = z[j];
+                const uint32_t pj = ((j - nl) >> 3) + parent1(nl);
+                if (L1.info[pj] & kInfoSelf) v += L1.x1[pj];
+                s += v;
+            }
+            z1[i] = (r[i] + s) / (diag[i] + kDiagEps);
         }
     }
 }
 
-// q = A d with an FP64 stencil sum (for smooth d the diagonal and neighbour terms cancel, and an FP32 sum would
-// put a large relative error into q that the recurrence residual never sees); partials d.q.
-__global__ __launch_bounds__(kThreads) void k_apply_a(Geo g, uint32_t nactive, const uint8_t *fmask, const float *diag,
-                                                      const float *d, float *q, double *partials,
+// Red sweep z1 = (r + sum_n z1_n) / diag and the partials of r.z1 over all fluid cells.
+__global__ __launch_bounds__(kThreads) void k_red_dot(const uint32_t *binfo, const uint8_t *fmask, const float *diag,
+                                                      const float *r, float *z1, double *partials,
                                                       const SolverState *st)
 {
     if (st->done) return;
+    const Brick B = load_brick(binfo, blockIdx.x);
     double v[1] = {0.0};
-    for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
-        const Brick B = brick_at(g, e);
-        for (uint32_t l = threadIdx.x; l < kBrickCells; l += kThreads) {
-            const uint32_t i = B.base + l, m = fmask[i];
-            if (!(m & kSelfFluid)) continue;
-            double s = nbsum<double>(B, l, m, d);
-            float dg = diag[i] + kDiagEps;
-            float dv = d[i];
-            double qd = (double)dg * (double)dv - s;
-            q[i] = (float)qd;
-            v[0] += (double)dv * qd;
+    {
+        const uint32_t l = colour_cell(threadIdx.x, 0u), i = B.base + l, m = fmask[i];
+        if (m & kSelfFluid) {
+            float s = nbsum<float>(B, l, m, z1);
+            float b = r[i];
+            float xv = (b + s) / (diag[i] + kDiagEps);
+            z1[i] = xv;
+            v[0] += (double)b * (double)xv;
         }
+    }
+    {
+        const uint32_t i = B.base + colour_cell(threadIdx.x, 1u);
+        if (fmask[i] & kSelfFluid) v[0] += (double)r[i] * (double)z1[i];
     }
     block_partial<1>(v, partials);
 }
 
-// p += alpha d (FP64), r -= alpha q, z_red = r / diag (the next V-cycle's fused red sweep); partials ||r||^2.
-__global__ __launch_bounds__(kThreads) void k_update_pr(Geo g, uint32_t nactive, const uint8_t *fmask,
-                                                        const float *diag, double *pres, float *r, float *z,
-                                                        const float *d, const float *q, double *partials,
+// d_out = z1 + beta d_in (beta = 0 for the first direction) fused with q = A d_out: the neighbours' new
+// direction is recomputed from z1 and d_in. The stencil sum is FP64 (for smooth d the diagonal and neighbour
+// terms cancel, and an FP32 sum would put a large relative error into q that the recurrence residual never
+// sees); partials d.q.
+__global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, const uint8_t *fmask,
+                                                        const float *diag, const float *z1, const float *d_in,
+                                                        float *d_out, float *q, int first, double *partials,
                                                         const SolverState *st)
 {
     if (st->done) return;
-    const double alpha = st->alpha;
+    const double beta = first ? 0.0 : st->beta;
+    const Brick B = load_brick(binfo, blockIdx.x);
     double v[1] = {0.0};
-    for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
-        const Brick B = brick_at(g, e);
-        for (uint32_t l = threadIdx.x; l < kBrickCells; l += kThreads) {
-            const uint32_t i = B.base + l;
-            if (!(fmask[i] & kSelfFluid)) continue;
-            pres[i] = pres[i] + alpha * (double)d[i];
-            float ri = (float)((double)r[i] - alpha * (double)q[i]);
-            r[i] = ri;
-            float dg = diag[i] + kDiagEps;
-            z[i] = red_local(l) ? ri / dg : 0.0f;
-            v[0] += (double)ri * (double)ri;
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
+        if (!(m & kSelfFluid)) continue;
+        const float dv = first ? z1[i] : (float)((double)z1[i] + beta * (double)d_in[i]);
+        d_out[i] = dv;
+        double s = 0.0;
+        for (int n = 0; n < 6; ++n) {
+            if (!(m >> n & 1u)) continue;
+            const uint32_t j = nbr(B, l, n);
+            const float dj = first ? z1[j] : (float)((double)z1[j] + beta * (double)d_in[j]);
+            s += (double)dj;
         }
+        const float dg = diag[i] + kDiagEps;
+        const double qd = (double)dg * (double)dv - s;
+        q[i] = (float)qd;
+        v[0] += (double)dv * qd;
     }
     block_partial<1>(v, partials);
 }
 
-// d = z + beta d (beta = 0 for the first direction).
-__global__ __launch_bounds__(kThreads) void k_update_d(Geo g, uint32_t nactive, const uint8_t *fmask, const float *z,
-                                                       float *d, int first, const SolverState *st)
-{
-    if (st->done) return;
-    const double beta = first ? 0.0 : st->beta;
-    for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
-        const Brick B = brick_at(g, e);
-        for (uint32_t l = threadIdx.x; l < kBrickCells; l += kThreads) {
-            const uint32_t i = B.base + l;
-            if (!(fmask[i] & kSelfFluid)) continue;
-            d[i] = (float)((double)z[i] + beta * (double)d[i]);
-        }
-    }
-}
-
 // Red-black SOR sweep on the FP64 pressure: p += w ((sum p_n - div) / diag - p).
-__global__ __launch_bounds__(kThreads) void k_sor_sweep(Geo g, uint32_t nactive, const uint8_t *fmask,
-                                                        const float *div, const float *diag, double *pres,
-                                                        uint32_t colour, double w, const SolverState *st)
+__global__ __launch_bounds__(kThreads) void k_sor_sweep(const uint32_t *binfo, const uint8_t *fmask, const float *div,
+                                                        const float *diag, double *pres, uint32_t colour, double w,
+                                                        const SolverState *st)
 {
     if (st->done) return;
-    const uint32_t l = colour_cell(threadIdx.x, colour);
-    for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
-        const Brick B = brick_at(g, e);
-        const uint32_t i = B.base + l, m = fmask[i];
-        if (!(m & kSelfFluid)) continue;
-        double s = nbsum<double>(B, l, m, pres);
-        double dg = (double)(diag[i] + kDiagEps);
-        double gs = (s - (double)div[i]) / dg;
-        double old = pres[i];
-        pres[i] = old + w * (gs - old);
-    }
+    const Brick B = load_brick(binfo, blockIdx.x);
+    const uint32_t l = colour_cell(threadIdx.x, colour), i = B.base + l, m = fmask[i];
+    if (!(m & kSelfFluid)) return;
+    double s = nbsum<double>(B, l, m, pres);
+    double dg = (double)(diag[i] + kDiagEps);
+    double gs = (s - (double)div[i]) / dg;
+    double old = pres[i];
+    pres[i] = old + w * (gs - old);
 }
 
 // Partials ||b||^2 and ||b - A p||^2 for the SOR convergence check.
-__global__ __launch_bounds__(kThreads) void k_sor_residual(Geo g, uint32_t nactive, const uint8_t *fmask,
+__global__ __launch_bounds__(kThreads) void k_sor_residual(const uint32_t *binfo, const uint8_t *fmask,
                                                            const float *div, const float *diag, const double *pres,
                                                            double *partials, const SolverState *st)
 {
     if (st->done) return;
+    const Brick B = load_brick(binfo, blockIdx.x);
     double v[2] = {0.0, 0.0};
-    for (uint32_t e = blockIdx.x; e < nactive; e += gridDim.x) {
-        const Brick B = brick_at(g, e);
-        for (uint32_t l = threadIdx.x; l < kBrickCells; l += kThreads) {
-            const uint32_t i = B.base + l, m = fmask[i];
-            if (!(m & kSelfFluid)) continue;
-            double s = nbsum<double>(B, l, m, pres);
-            double dg = (double)(diag[i] + kDiagEps);
-            double ri = -(double)div[i] - (dg * pres[i] - s);
-            v[0] += (double)div[i] * (double)div[i];
-            v[1] += ri * ri;
-        }
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
+        if (!(m & kSelfFluid)) continue;
+        double s = nbsum<double>(B, l, m, pres);
+        double dg = (double)(diag[i] + kDiagEps);
+        double ri = -(double)div[i] - (dg * pres[i] - s);
+        v[0] += (double)div[i] * (double)div[i];
+        v[1] += ri * ri;
     }
     block_partial<2>(v, partials);
 }
 
 } // namespace
 
-uint32_t l0_groups(const slosh_ctx *c) { return c->nactive < kMaxGroups ? c->nactive : kMaxGroups; }
-
 #define L0_LAUNCH(kid, kernel, ...)                                                                              \
     do {                                                                                                         \
         SLOSH_TIMED(c, kid);                                                                                     \
-        hipLaunchKernelGGL(kernel, dim3(l0_groups(c)), dim3(kThreads), 0, c->stream, c->geo, c->nactive, c->fmask, \
-                           __VA_ARGS__);                                                                         \
+        hipLaunchKernelGGL(kernel, dim3(c->nactive), dim3(kThreads), 0, c->stream, c->binfo, c->fmask, __VA_ARGS__); \
     } while (0)
 
-void l0_pcg_init(slosh_ctx *c) { L0_LAUNCH(K_PCG_INIT, k_pcg_init, c->div, c->diag, c->pres, c->r, c->z, c->d, c->partials); }
-
-void l0_smooth(slosh_ctx *c, uint32_t colour, int want_dot)
+void l0_brick_info(slosh_ctx *c)
 {
-    L0_LAUNCH(K_SMOOTH0, k_smooth0, c->diag, c->z, c->r, colour, want_dot, c->partials, c->state);
+    SLOSH_TIMED(c, K_MG_TYPES);
+    hipLaunchKernelGGL(k_brick_info, dim3((c->nactive + 255u) / 256u), dim3(256), 0, c->stream, c->geo, c->nactive,
+                       c->binfo);
 }
 
-void l0_restrict(slosh_ctx *c) { L0_LAUNCH(K_RESTRICT0, k_restrict0, c->diag, c->z, c->r, plevel_of(c, 1), c->state); }
-
-void l0_prolong(slosh_ctx *c) { L0_LAUNCH(K_PROLONG0, k_prolong0, c->z, plevel_of(c, 1), c->state); }
-
-void l0_apply_a(slosh_ctx *c) { L0_LAUNCH(K_APPLY_A, k_apply_a, c->diag, c->d, c->q, c->partials, c->state); }
-
-void l0_update_pr(slosh_ctx *c)
+void l0_pcg_init(slosh_ctx *c, float *r_out)
 {
-    L0_LAUNCH(K_UPDATE_PR, k_update_pr, c->diag, c->pres, c->r, c->z, c->d, c->q, c->partials, c->state);
+    L0_LAUNCH(K_PCG_INIT, k_pcg_init, c->div, c->diag, c->pres, r_out, c->partials);
 }
 
-void l0_update_d(slosh_ctx *c, int first) { L0_LAUNCH(K_UPDATE_D, k_update_d, c->z, c->d, first, c->state); }
+void l0_update_black(slosh_ctx *c, const float *r_in, float *r_out, const float *d, int first)
+{
+    L0_LAUNCH(K_UPDATE_PR, k_update_black, c->diag, c->pres, r_in, r_out, c->z, d, c->q, first, c->partials, c->state);
+}
+
+void l0_restrict(slosh_ctx *c, const float *r)
+{
+    L0_LAUNCH(K_RESTRICT0, k_restrict0, c->diag, c->z, r, plevel_of(c, 1), c->state);
+}
+
+void l0_prolong_black(slosh_ctx *c, const float *r)
+{
+    L0_LAUNCH(K_PROLONG0, k_prolong_black, c->diag, r, c->z, c->z1, plevel_of(c, 1), c->state);
+}
+
+void l0_red_dot(slosh_ctx *c, const float *r) { L0_LAUNCH(K_SMOOTH0, k_red_dot, c->diag, r, c->z1, c->partials, c->state); }
+
+void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, int first)
+{
+    L0_LAUNCH(K_APPLY_A, k_direction, c->diag, c->z1, d_in, d_out, c->q, first, c->partials, c->state);
+}
 
 void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w)
 {

@@ -1,12 +1,17 @@
-// pooled.hip -- multigrid levels 1..3 stored in the brick pool, on gfx950.
+// pooled.hip -- multigrid levels 1 and 2 stored in the brick pool, on gfx950.
 //
 // The reference keeps every coarse level as a dense grid (SloshProxy.cpp:269-285), so level 1 of a 256^3 domain is
-// a 2M-cell array swept in full although the liquid occupies a thin layer. Here a level-l cell (l = 1, 2, 3) lives
+// a 2M-cell array swept in full although the liquid occupies a thin layer. Here a level-l cell (l = 1, 2) lives
 // with the 8^3 brick that contains its children: (8 >> l)^3 cells per allocated brick, at slot * cells_per_brick +
-// local. Unallocated bricks read as the zero null slot, i.e. AIR, which is what the reference's coarsening rule
+// local. Level 3 has one cell per brick and is stored densely by brick index. Unallocated bricks read as the zero null slot, i.e. AIR, which is what the reference's coarsening rule
 // gives for an all-air block. Cells beyond the level's extent (partial bricks) are typed SOLID and never touched.
+//
 // Operations are those of SloshPressureMG.usf (MGBuildTypeCS :256-278, MGSmoothCS :311-338, MGRestrictCS
-// :343-381, MGProlongCS :418-433) on the counted stencil with the 1e-4 diagonal term.
+// :343-381, MGProlongCS :418-433) on the counted stencil with the 1e-4 diagonal term. A level's V-cycle work is
+// two passes: down = black sweep + residual + restriction, up = prolongation + black sweep + red sweep. A pass
+// needs the other colour's new value at a neighbour that another workgroup is producing, so it recomputes that
+// value from inputs no workgroup writes in the pass (the down-sweep iterate x and the up-sweep result x1 are
+// separate buffers); each value is the reference's expression in the reference's operand order.
 #include "slosh_ctx.h"
 
 namespace slosh {
@@ -43,34 +48,6 @@ __device__ __forceinline__ bool pcell(const Geo &g, const PLevel &L, uint32_t na
     c.idx = c.slot * L.cpb + local;
     c.inside = c.x < L.dim[0] && c.y < L.dim[1] && c.z < L.dim[2];
     return true;
-}
-
-__device__ __forceinline__ uint32_t ptype_at(const Geo &g, const PLevel &L, int x, int y, int z)
-{
-    return pinside(L, x, y, z) ? L.type[pindex(g, L, x, y, z)] : kCellSolid;
-}
-
-// Counted stencil at (x, y, z): diagonal 1e-4 + non-solid neighbours, sum of v over fluid neighbours.
-__device__ __forceinline__ void pstencil(const Geo &g, const PLevel &L, const float *v, const PCell &c, float &dg,
-                                         float &sum)
-{
-    dg = kDiagEps;
-    sum = 0.0f;
-    for (int nb = 0; nb < 6; ++nb) {
-        int d[3];
-        nb_offset(nb, d);
-        int nx = c.x + d[0], ny = c.y + d[1], nz = c.z + d[2];
-        if (!pinside(L, nx, ny, nz)) continue;
-        int lx = c.lx + d[0], ly = c.ly + d[1], lz = c.lz + d[2];
-        uint32_t ni;
-        if (lx >= 0 && ly >= 0 && lz >= 0 && lx < L.s && ly < L.s && lz < L.s)
-            ni = c.slot * L.cpb + (uint32_t)(lx + L.s * (ly + L.s * lz));
-        else
-            ni = pindex(g, L, nx, ny, nz);
-        uint32_t t = L.type[ni];
-        if (t != kCellSolid) dg += 1.0f;
-        if (t == kCellFluid) sum += v[ni];
-    }
 }
 
 // Coarse types of pooled level 1 from level 0 (children in the same brick).
@@ -114,77 +91,36 @@ __global__ void k_ptype(Geo g, uint32_t nactive, PLevel F, PLevel C)
     C.type[c.idx] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
 }
 
-__global__ void k_psmooth(Geo g, uint32_t nactive, PLevel L, uint32_t colour, const SolverState *st)
+// Stencil word of every allocated cell of pooled level L (after its types).
+__global__ void k_pinfo(Geo g, uint32_t nactive, PLevel L)
 {
-    if (st->done) return;
     PCell c;
     if (!pcell(g, L, nactive, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
-    if (!c.inside || L.type[c.idx] != kCellFluid || (is_red(c.x, c.y, c.z) ? 0u : 1u) != colour) return;
-    float dg, s;
-    pstencil(g, L, L.x, c, dg, s);
-    L.x[c.idx] = (L.b[c.idx] + s) / dg;
+    uint32_t info = 0u;
+    if (c.inside && L.type[c.idx] == kCellFluid) {
+        info = kInfoSelf | (is_red(c.x, c.y, c.z) ? kInfoRed : 0u);
+        uint32_t cnt = 0u;
+        for (int n = 0; n < 6; ++n) {
+            int d[3];
+            nb_offset(n, d);
+            const int x = c.x + d[0], y = c.y + d[1], z = c.z + d[2];
+            if (!pinside(L, x, y, z)) continue;
+            const uint32_t t = L.type[pindex(g, L, x, y, z)];
+            if (t != kCellSolid) cnt++;
+            if (t == kCellFluid) info |= 1u << n;
+        }
+        info |= cnt << 8;
+    }
+    L.info[c.idx] = (uint16_t)info;
 }
 
-// Restriction pooled F -> pooled C (children in the same brick), with the fused red sweep x_c = b_c / diag_c.
-__global__ void k_prestrict(Geo g, uint32_t nactive, PLevel F, PLevel C, const SolverState *st)
-{
-    if (st->done) return;
-    PCell c;
-    if (!pcell(g, C, nactive, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
-    if (!c.inside) return;
-    if (C.type[c.idx] != kCellFluid) {
-        C.b[c.idx] = 0.0f;
-        C.x[c.idx] = 0.0f;
-        return;
-    }
-    float sum = 0.0f;
-    for (uint32_t oi = 0; oi < 8u; ++oi) {
-        int ox = (int)(oi & 1u), oy = (int)((oi >> 1) & 1u), oz = (int)(oi >> 2);
-        PCell f;
-        f.x = 2 * c.x + ox;
-        f.y = 2 * c.y + oy;
-        f.z = 2 * c.z + oz;
-        if (!pinside(F, f.x, f.y, f.z)) continue;
-        f.lx = 2 * c.lx + ox;
-        f.ly = 2 * c.ly + oy;
-        f.lz = 2 * c.lz + oz;
-        f.slot = c.slot;
-        f.idx = c.slot * F.cpb + (uint32_t)(f.lx + F.s * (f.ly + F.s * f.lz));
-        if (F.type[f.idx] != kCellFluid) continue;
-        float dg, s;
-        pstencil(g, F, F.x, f, dg, s);
-        sum += F.b[f.idx] - (dg * F.x[f.idx] - s);
-    }
-    float bv = 0.25f * sum;
-    C.b[c.idx] = bv;
-    float cdg = kDiagEps;
-    for (int nb = 0; nb < 6; ++nb) {
-        int d[3];
-        nb_offset(nb, d);
-        cdg += ptype_at(g, C, c.x + d[0], c.y + d[1], c.z + d[2]) != kCellSolid ? 1.0f : 0.0f;
-    }
-    C.x[c.idx] = is_red(c.x, c.y, c.z) ? bv / cdg : 0.0f;
-}
-
-// Prolongation pooled C -> pooled F: x_f += x_c(parent) (parent in the same brick).
-__global__ void k_pprolong(Geo g, uint32_t nactive, PLevel F, PLevel C, const SolverState *st)
-{
-    if (st->done) return;
-    PCell f;
-    if (!pcell(g, F, nactive, blockIdx.x * blockDim.x + threadIdx.x, f)) return;
-    if (!f.inside || F.type[f.idx] != kCellFluid) return;
-    uint32_t pl = (uint32_t)((f.lx >> 1) + C.s * ((f.ly >> 1) + C.s * (f.lz >> 1)));
-    uint32_t pi = f.slot * C.cpb + pl;
-    if (C.type[pi] == kCellFluid) F.x[f.idx] += C.x[pi];
-}
-
-// Dense level C (16^3 and coarser) from pooled level F: coarse types, restriction and prolongation.
+// Dense level C (level 3, one cell per brick, and coarser) types from pooled level F.
 __global__ void k_dtype_from_pooled(Geo g, PLevel F, DenseLevel C)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C.cells) return;
-    int x = (int)(i % (uint32_t)C.dim[0]), y = (int)((i / (uint32_t)C.dim[0]) % (uint32_t)C.dim[1]),
-        z = (int)(i / (uint32_t)(C.dim[0] * C.dim[1]));
+    const uint32_t pc = C.coord[i];
+    const int x = (int)(pc & 1023u), y = (int)((pc >> 10) & 1023u), z = (int)(pc >> 20);
     bool air = false, fluid = false;
     for (uint32_t oi = 0; oi < 8u; ++oi) {
         int fx = 2 * x + (int)(oi & 1u), fy = 2 * y + (int)((oi >> 1) & 1u), fz = 2 * z + (int)(oi >> 2);
@@ -196,59 +132,159 @@ __global__ void k_dtype_from_pooled(Geo g, PLevel F, DenseLevel C)
     C.type[i] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
 }
 
-__global__ void k_restrict_pd(Geo g, PLevel F, DenseLevel C, const SolverState *st)
+// ---- levels 1 and 2: one workgroup per brick, the brick's block and a two-cell halo staged in LDS ----------
+// The box spans block-relative coordinates [-2, S + 2)^3 (S = cells per brick axis); its cells come from the
+// brick's 3x3x3 neighbourhood. A black value needed on the one-cell halo is computed there from the staged red
+// values, so the block's residual (down) or red sweep (up) needs nothing another workgroup writes.
+template <int S> struct Box {
+    static constexpr int BD = S + 4;
+    static constexpr int NB = BD * BD * BD;
+    static constexpr int CPB = S * S * S;
+    static constexpr int IN = S + 2; // the block plus a one-cell halo
+    __device__ static int off(int n)
+    {
+        const int s = n < 2 ? 1 : (n < 4 ? BD : BD * BD);
+        return (n & 1) ? s : -s;
+    }
+    __device__ static uint32_t index(const uint32_t *nb27, int u, int v, int w)
+    {
+        const int ox = u < 0 ? -1 : (u >= S ? 1 : 0), oy = v < 0 ? -1 : (v >= S ? 1 : 0),
+                  oz = w < 0 ? -1 : (w >= S ? 1 : 0);
+        const uint32_t slot = nb27[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        return slot * (uint32_t)CPB + (uint32_t)((u - ox * S) + S * ((v - oy * S) + S * (w - oz * S)));
+    }
+    __device__ static uint32_t box_of(int u, int v, int w) { return (uint32_t)((u + 2) + BD * ((v + 2) + BD * (w + 2))); }
+};
+
+template <int S> __device__ __forceinline__ float box_sum(const float *v, uint32_t k, uint32_t info)
 {
-    if (st->done) return;
-    uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= C.cells) return;
-    int cx = (int)(ci % (uint32_t)C.dim[0]), cy = (int)((ci / (uint32_t)C.dim[0]) % (uint32_t)C.dim[1]),
-        cz = (int)(ci / (uint32_t)(C.dim[0] * C.dim[1]));
-    if (C.type[ci] != kCellFluid) {
-        C.b[ci] = 0.0f;
-        C.x[ci] = 0.0f;
-        return;
-    }
-    float sum = 0.0f;
-    for (uint32_t oi = 0; oi < 8u; ++oi) {
-        PCell f;
-        f.x = 2 * cx + (int)(oi & 1u);
-        f.y = 2 * cy + (int)((oi >> 1) & 1u);
-        f.z = 2 * cz + (int)(oi >> 2);
-        if (!pinside(F, f.x, f.y, f.z)) continue;
-        const int m = F.s - 1;
-        f.lx = f.x & m;
-        f.ly = f.y & m;
-        f.lz = f.z & m;
-        f.idx = pindex(g, F, f.x, f.y, f.z);
-        f.slot = f.idx / F.cpb;
-        if (F.type[f.idx] != kCellFluid) continue;
-        float dg, s;
-        pstencil(g, F, F.x, f, dg, s);
-        sum += F.b[f.idx] - (dg * F.x[f.idx] - s);
-    }
-    float bv = 0.25f * sum;
-    C.b[ci] = bv;
-    float cdg = kDiagEps;
-    for (int nb = 0; nb < 6; ++nb) {
-        int d[3];
-        nb_offset(nb, d);
-        int x = cx + d[0], y = cy + d[1], z = cz + d[2];
-        uint32_t t = (x < 0 || y < 0 || z < 0 || x >= C.dim[0] || y >= C.dim[1] || z >= C.dim[2])
-                         ? kCellSolid
-                         : C.type[(uint32_t)(x + C.dim[0] * (y + C.dim[1] * z))];
-        cdg += t != kCellSolid ? 1.0f : 0.0f;
-    }
-    C.x[ci] = is_red(cx, cy, cz) ? bv / cdg : 0.0f;
+    float s = 0.0f;
+    for (int n = 0; n < 6; ++n)
+        if (info >> n & 1u) s += v[(int)k + Box<S>::off(n)];
+    return s;
 }
 
-__global__ void k_prolong_dp(Geo g, uint32_t nactive, PLevel F, DenseLevel C, const SolverState *st)
+// Down pass of pooled level L (S = 8 >> l): black sweep, residual, restriction into pooled level C.
+template <int S>
+__global__ __launch_bounds__(256) void k_pdown_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+                                                   const SolverState *st)
 {
     if (st->done) return;
-    PCell f;
-    if (!pcell(g, F, nactive, blockIdx.x * blockDim.x + threadIdx.x, f)) return;
-    if (!f.inside || F.type[f.idx] != kCellFluid) return;
-    uint32_t pi = (uint32_t)((f.x >> 1) + C.dim[0] * ((f.y >> 1) + C.dim[1] * (f.z >> 1)));
-    if (C.type[pi] == kCellFluid) F.x[f.idx] += C.x[pi];
+    using Bx = Box<S>;
+    constexpr int S2 = S / 2, CPB2 = S2 * S2 * S2;
+    __shared__ uint32_t s_nb[28];
+    __shared__ float s_x[Bx::NB], s_b[Bx::NB];
+    __shared__ uint16_t s_info[Bx::NB];
+    __shared__ float s_res[Bx::CPB];
+    const uint32_t t = threadIdx.x;
+    if (t < 28u) s_nb[t] = binfo[(size_t)blockIdx.x * kBinfoWords + t];
+    __syncthreads();
+    for (uint32_t k = t; k < (uint32_t)Bx::NB; k += 256u) {
+        const int u = (int)(k % Bx::BD) - 2, v = (int)((k / Bx::BD) % Bx::BD) - 2, w = (int)(k / (Bx::BD * Bx::BD)) - 2;
+        const uint32_t gi = Bx::index(s_nb, u, v, w);
+        const uint32_t info = L.info[gi];
+        const bool f = (info & kInfoSelf) != 0u;
+        s_info[k] = (uint16_t)info;
+        s_x[k] = f ? L.x[gi] : 0.0f;
+        s_b[k] = f ? L.b[gi] : 0.0f;
+    }
+    __syncthreads();
+    for (uint32_t k = t; k < (uint32_t)(Bx::IN * Bx::IN * Bx::IN); k += 256u) {
+        const int u = (int)(k % Bx::IN) - 1, v = (int)((k / Bx::IN) % Bx::IN) - 1, w = (int)(k / (Bx::IN * Bx::IN)) - 1;
+        const uint32_t kb = Bx::box_of(u, v, w), info = s_info[kb];
+        if (!(info & kInfoSelf) || (info & kInfoRed)) continue;
+        s_x[kb] = (s_b[kb] + box_sum<S>(s_x, kb, info)) / info_diag(info);
+    }
+    __syncthreads();
+    const uint32_t slot = s_nb[13];
+    for (uint32_t k = t; k < (uint32_t)Bx::CPB; k += 256u) {
+        const int u = (int)(k % S), v = (int)((k / S) % S), w = (int)(k / (S * S));
+        const uint32_t kb = Bx::box_of(u, v, w), info = s_info[kb];
+        float r = 0.0f;
+        if (info & kInfoSelf) {
+            if (!(info & kInfoRed)) L.x[slot * Bx::CPB + k] = s_x[kb];
+            r = s_b[kb] - (info_diag(info) * s_x[kb] - box_sum<S>(s_x, kb, info));
+        }
+        s_res[k] = r;
+    }
+    __syncthreads();
+    for (uint32_t k = t; k < (uint32_t)CPB2; k += 256u) {
+        const int cx = (int)(k % S2), cy = (int)((k / S2) % S2), cz = (int)(k / (S2 * S2));
+        // level 2 restricts into level 3, stored densely by brick index
+        const uint32_t ci = S == 2 ? s_nb[27] : slot * CPB2 + k;
+        const uint32_t cinfo = S == 2 ? D.info[ci] : C.info[ci];
+        float *cb = S == 2 ? D.b : C.b, *cxv = S == 2 ? D.x : C.x;
+        if (!(cinfo & kInfoSelf)) {
+            cb[ci] = 0.0f;
+            cxv[ci] = 0.0f;
+            continue;
+        }
+        float sum = 0.0f;
+        for (uint32_t oi = 0; oi < 8u; ++oi) {
+            const int fx = 2 * cx + (int)(oi & 1u), fy = 2 * cy + (int)((oi >> 1) & 1u), fz = 2 * cz + (int)(oi >> 2);
+            if (s_info[Bx::box_of(fx, fy, fz)] & kInfoSelf) sum += s_res[fx + S * (fy + S * fz)];
+        }
+        const float bv = 0.25f * sum;
+        cb[ci] = bv;
+        cxv[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
+    }
+}
+
+// Up pass of pooled level L (S = 8 >> l) from pooled parent level C: prolongation, black sweep, red sweep.
+template <int S>
+__global__ __launch_bounds__(256) void k_pup_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+                                                 const SolverState *st)
+{
+    if (st->done) return;
+    using Bx = Box<S>;
+    constexpr int S2 = S / 2, CPB2 = S2 * S2 * S2;
+    __shared__ uint32_t s_nb[28];
+    __shared__ float s_v[Bx::NB], s_b[Bx::NB];
+    __shared__ uint16_t s_info[Bx::NB];
+    const uint32_t t = threadIdx.x;
+    if (t < 28u) s_nb[t] = binfo[(size_t)blockIdx.x * kBinfoWords + t];
+    __syncthreads();
+    for (uint32_t k = t; k < (uint32_t)Bx::NB; k += 256u) {
+        const int u = (int)(k % Bx::BD) - 2, v = (int)((k / Bx::BD) % Bx::BD) - 2, w = (int)(k / (Bx::BD * Bx::BD)) - 2;
+        const int ox = u < 0 ? -1 : (u >= S ? 1 : 0), oy = v < 0 ? -1 : (v >= S ? 1 : 0),
+                  oz = w < 0 ? -1 : (w >= S ? 1 : 0);
+        const int lu = u - ox * S, lv = v - oy * S, lw = w - oz * S;
+        const uint32_t nslot = s_nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        const uint32_t gi = nslot * (uint32_t)Bx::CPB + (uint32_t)(lu + S * (lv + S * lw));
+        const uint32_t info = L.info[gi];
+        float val = 0.0f, bv = 0.0f;
+        if (info & kInfoSelf) {
+            val = L.x[gi];
+            if (S == 2) {
+                // parent: the level-3 cell of the (allocated, in-grid) brick holding this cell
+                const uint32_t pi = (uint32_t)((int)s_nb[27] + ox + D.dim[0] * (oy + D.dim[1] * oz));
+                if (D.info[pi] & kInfoSelf) val += D.x1[pi];
+            } else {
+                const uint32_t pi = nslot * (uint32_t)CPB2 + (uint32_t)((lu >> 1) + S2 * ((lv >> 1) + S2 * (lw >> 1)));
+                if (C.info[pi] & kInfoSelf) val += C.x1[pi];
+            }
+            bv = L.b[gi];
+        }
+        s_info[k] = (uint16_t)info;
+        s_v[k] = val;
+        s_b[k] = bv;
+    }
+    __syncthreads();
+    for (uint32_t k = t; k < (uint32_t)(Bx::IN * Bx::IN * Bx::IN); k += 256u) {
+        const int u = (int)(k % Bx::IN) - 1, v = (int)((k / Bx::IN) % Bx::IN) - 1, w = (int)(k / (Bx::IN * Bx::IN)) - 1;
+        const uint32_t kb = Bx::box_of(u, v, w), info = s_info[kb];
+        if (!(info & kInfoSelf) || (info & kInfoRed)) continue;
+        s_v[kb] = (s_b[kb] + box_sum<S>(s_v, kb, info)) / info_diag(info);
+    }
+    __syncthreads();
+    const uint32_t slot = s_nb[13];
+    for (uint32_t k = t; k < (uint32_t)Bx::CPB; k += 256u) {
+        const int u = (int)(k % S), v = (int)((k / S) % S), w = (int)(k / (S * S));
+        const uint32_t kb = Bx::box_of(u, v, w), info = s_info[kb];
+        if (!(info & kInfoSelf)) continue;
+        L.x1[slot * Bx::CPB + k] =
+            (info & kInfoRed) ? (s_b[kb] + box_sum<S>(s_v, kb, info)) / info_diag(info) : s_v[kb];
+    }
 }
 
 uint32_t pass_blocks(const slosh_ctx *c, const PLevel &L) { return blocks_for((uint64_t)c->nactive * L.cpb, 256); }
@@ -265,7 +301,9 @@ PLevel plevel_of(const slosh_ctx *c, int l)
     P.s = 1 << P.ls;
     P.cpb = (uint32_t)(P.s * P.s * P.s);
     P.type = c->lv[l].type;
+    P.info = c->lv[l].info;
     P.x = c->lv[l].x;
+    P.x1 = c->lv[l].x1;
     P.b = c->lv[l].b;
     return P;
 }
@@ -278,7 +316,10 @@ DenseLevel dlevel_of(const slosh_ctx *c, int l)
     D.dim[2] = c->lv[l].dim[2];
     D.cells = c->lv[l].cells;
     D.type = c->lv[l].type;
+    D.info = c->lv[l].info;
+    D.coord = c->lv[l].coord;
     D.x = c->lv[l].x;
+    D.x1 = c->lv[l].x1;
     D.b = c->lv[l].b;
     return D;
 }
@@ -294,28 +335,11 @@ void pooled_build_type(slosh_ctx *c, int l)
                            plevel_of(c, l - 1), L);
 }
 
-void pooled_smooth(slosh_ctx *c, int l, uint32_t colour)
+void pooled_build_info(slosh_ctx *c, int l)
 {
     PLevel L = plevel_of(c, l);
-    SLOSH_TIMED(c, K_SMOOTH);
-    hipLaunchKernelGGL(k_psmooth, dim3(pass_blocks(c, L)), dim3(256), 0, c->stream, c->geo, c->nactive, L, colour,
-                       c->state);
-}
-
-void pooled_restrict(slosh_ctx *c, int l)
-{
-    PLevel C = plevel_of(c, l + 1);
-    SLOSH_TIMED(c, K_RESTRICT);
-    hipLaunchKernelGGL(k_prestrict, dim3(pass_blocks(c, C)), dim3(256), 0, c->stream, c->geo, c->nactive,
-                       plevel_of(c, l), C, c->state);
-}
-
-void pooled_prolong(slosh_ctx *c, int l)
-{
-    PLevel F = plevel_of(c, l);
-    SLOSH_TIMED(c, K_PROLONG);
-    hipLaunchKernelGGL(k_pprolong, dim3(pass_blocks(c, F)), dim3(256), 0, c->stream, c->geo, c->nactive, F,
-                       plevel_of(c, l + 1), c->state);
+    SLOSH_TIMED(c, K_MG_TYPES);
+    hipLaunchKernelGGL(k_pinfo, dim3(pass_blocks(c, L)), dim3(256), 0, c->stream, c->geo, c->nactive, L);
 }
 
 void dense_build_type_from_pooled(slosh_ctx *c, int l)
@@ -326,20 +350,28 @@ void dense_build_type_from_pooled(slosh_ctx *c, int l)
                        plevel_of(c, l - 1), C);
 }
 
-void restrict_pooled_to_dense(slosh_ctx *c, int l)
+void pooled_down(slosh_ctx *c, int l)
 {
-    DenseLevel C = dlevel_of(c, l + 1);
+    PLevel L = plevel_of(c, l);
+    PLevel C = l == 1 ? plevel_of(c, 2) : PLevel{};
+    DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
     SLOSH_TIMED(c, K_RESTRICT);
-    hipLaunchKernelGGL(k_restrict_pd, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, c->geo,
-                       plevel_of(c, l), C, c->state);
+    if (l == 1)
+        hipLaunchKernelGGL(k_pdown_lds<4>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
+    else
+        hipLaunchKernelGGL(k_pdown_lds<2>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
 }
 
-void prolong_dense_to_pooled(slosh_ctx *c, int l)
+void pooled_up(slosh_ctx *c, int l)
 {
-    PLevel F = plevel_of(c, l);
+    PLevel L = plevel_of(c, l);
+    PLevel C = l == 1 ? plevel_of(c, 2) : PLevel{};
+    DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
     SLOSH_TIMED(c, K_PROLONG);
-    hipLaunchKernelGGL(k_prolong_dp, dim3(pass_blocks(c, F)), dim3(256), 0, c->stream, c->geo, c->nactive, F,
-                       dlevel_of(c, l + 1), c->state);
+    if (l == 1)
+        hipLaunchKernelGGL(k_pup_lds<4>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
+    else
+        hipLaunchKernelGGL(k_pup_lds<2>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
 }
 
 } // namespace slosh

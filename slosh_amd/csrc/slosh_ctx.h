@@ -32,15 +32,35 @@ struct SolverState {
     int32_t converged;
 };
 
-// Multigrid level >= 1. Levels 1..3 are pooled (stored per allocated brick, (8 >> l)^3 cells per slot); coarser
-// levels are dense (x fastest).
+// Multigrid level >= 1. Levels 1 and 2 are pooled (stored per allocated brick, (8 >> l)^3 cells per slot); level 3
+// (one cell per brick) and coarser levels are dense (x fastest; level 3 is indexed like the brick grid). Every level keeps a per-cell stencil word (see kInfo*), the down-sweep iterate x,
+// the up-sweep result x1 and the right-hand side b.
 struct Level {
     int dim[3];
     uint32_t cells;
     bool pooled;
     uint8_t *type;
-    float *x, *b;
+    uint16_t *info;
+    uint32_t *coord; // dense levels: packed cell coordinates x | y << 10 | z << 20
+    float *x, *x1, *b;
 };
+
+// Words per allocated brick in the per-solve brick table: slots of the 3x3x3 neighbourhood (entry (dx+1) +
+// 3(dy+1) + 9(dz+1), 0 = none), then the brick index.
+constexpr uint32_t kBinfoWords = 32u;
+
+// Stencil word of a coarse cell: bits 0-5 fluid neighbours (-x +x -y +y -z +z), bit 6 the cell is fluid, bit 7 red
+// ((x + y + z) even), bits 8-11 the number of non-solid neighbours (the coarse diagonal without the 1e-4).
+constexpr uint32_t kInfoSelf = 1u << 6;
+constexpr uint32_t kInfoRed = 1u << 7;
+
+// Diagonal of a coarse cell, 1e-4 plus one per non-solid neighbour, summed in the same order as the reference.
+__device__ __forceinline__ float info_diag(uint32_t info)
+{
+    float d = kDiagEps;
+    for (uint32_t k = 0; k < (info >> 8); ++k) d += 1.0f;
+    return d;
+}
 
 // Kernel-side views of a pooled and of a dense level.
 struct PLevel {
@@ -49,13 +69,16 @@ struct PLevel {
     int s;         // cells per brick axis (8 >> level)
     uint32_t cpb;  // cells per brick
     uint8_t *type;
-    float *x, *b;
+    uint16_t *info;
+    float *x, *x1, *b;
 };
 struct DenseLevel {
     int dim[3];
     uint32_t cells;
     uint8_t *type;
-    float *x, *b;
+    uint16_t *info;
+    const uint32_t *coord;
+    float *x, *x1, *b;
 };
 
 __device__ __forceinline__ bool pinside(const PLevel &L, int x, int y, int z)
@@ -136,8 +159,11 @@ struct slosh_ctx {
     double *pres = nullptr;
     float *div = nullptr, *diag = nullptr;
     uint8_t *fmask = nullptr; // solver stencil mask per cell (fluid neighbours, self fluid)
-    // solver
-    float *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr;
+    // solver: residual and search direction ping-pong between iterations, z is the down-sweep iterate and z1 the
+    // V-cycle result; binfo holds kBinfoWords per allocated brick (its 3x3x3 neighbourhood slots, brick index)
+    float *r = nullptr, *r2 = nullptr, *z = nullptr, *z1 = nullptr, *d = nullptr, *d2 = nullptr, *q = nullptr;
+    uint32_t *binfo = nullptr;
+    float *res3 = nullptr; // level-3 residual (dense, by brick index), restricted into level 4
     int nlevels = 1;
     slosh::Level lv[slosh::kMaxLevels]{};
     uint32_t *coarse_idx = nullptr; // [kCoarsestCells]
@@ -204,26 +230,31 @@ void launch_fill_f64(hipStream_t st, double *p, double v, size_t n);
 void run_mgpcg(slosh_ctx *c);
 void run_rbsor(slosh_ctx *c);
 
-// ---- pooled.hip: MG levels 1..3 in the brick pool, and the pooled <-> dense transitions ----
+// ---- pooled.hip: MG levels 1 and 2 in the brick pool ----
 PLevel plevel_of(const slosh_ctx *c, int l);
 DenseLevel dlevel_of(const slosh_ctx *c, int l);
 void pooled_build_type(slosh_ctx *c, int l);
-void pooled_smooth(slosh_ctx *c, int l, uint32_t colour);
-void pooled_restrict(slosh_ctx *c, int l);           // pooled l -> pooled l + 1
-void pooled_prolong(slosh_ctx *c, int l);            // pooled l + 1 -> pooled l
-void dense_build_type_from_pooled(slosh_ctx *c, int l); // dense l from pooled l - 1
-void restrict_pooled_to_dense(slosh_ctx *c, int l);  // pooled l -> dense l + 1
-void prolong_dense_to_pooled(slosh_ctx *c, int l);   // dense l + 1 -> pooled l
+void pooled_build_info(slosh_ctx *c, int l);
+void dense_build_type_from_pooled(slosh_ctx *c, int l); // dense level l from pooled level l - 1
+// Black sweep of pooled level l (1 or 2), then its residual restricted into level l + 1 (b, and x = b / diag on
+// red cells).
+void pooled_down(slosh_ctx *c, int l);
+// Prolongation of the parent level's result into pooled level l, then a black and a red sweep: x1 of level l.
+void pooled_up(slosh_ctx *c, int l);
 
-// ---- level0.hip: finest-level solver passes (partials of a reduction: one per workgroup, l0_groups of them) ----
-uint32_t l0_groups(const slosh_ctx *c);
-void l0_pcg_init(slosh_ctx *c);
-void l0_smooth(slosh_ctx *c, uint32_t colour, int want_dot);
-void l0_restrict(slosh_ctx *c);
-void l0_prolong(slosh_ctx *c);
-void l0_apply_a(slosh_ctx *c);
-void l0_update_pr(slosh_ctx *c);
-void l0_update_d(slosh_ctx *c, int first);
+// ---- level0.hip: finest-level solver passes (one brick per workgroup; one reduction partial per brick) ----
+void l0_brick_info(slosh_ctx *c);
+void l0_pcg_init(slosh_ctx *c, float *r_out);
+// p += alpha d, r_out = r_in - alpha q (first: r_out = r_in), z = r_out / diag on red cells, then the black sweep;
+// partials ||r||^2 unless first.
+void l0_update_black(slosh_ctx *c, const float *r_in, float *r_out, const float *d, int first);
+void l0_restrict(slosh_ctx *c, const float *r);
+// z1 = z + prolongation of level 1's x1, then the black sweep into z1.
+void l0_prolong_black(slosh_ctx *c, const float *r);
+// Red sweep on z1 and the partials r.z1.
+void l0_red_dot(slosh_ctx *c, const float *r);
+// d_out = z1 + beta d_in (first: d_out = z1), q = A d_out; partials d.q.
+void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, int first);
 void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w);
 void l0_sor_residual(slosh_ctx *c);
 

@@ -6,8 +6,9 @@
 // kernel returns at once when `done` is set, so the host enqueues iterations in chunks and reads the state back
 // once per chunk.
 //
-// Level layout: level 0 is the brick pool (level0.hip), levels 1..3 are pooled per brick (pooled.hip), levels 4
-// and coarser are dense and run inside one workgroup, in LDS (k_coarse_tail below).
+// Level layout: level 0 is the brick pool (level0.hip), levels 1 and 2 are pooled per brick (pooled.hip), level 3
+// (one cell per brick) is dense by brick index, and levels 4 and coarser are dense and run inside one workgroup, in
+// LDS (k_coarse_tail below).
 //
 // Reductions are deterministic and free of cross-workgroup synchronisation: each workgroup writes its FP64
 // partial sum, and a one-workgroup finalize kernel sums the partials in a fixed order and updates the state. (A
@@ -26,62 +27,70 @@ constexpr uint32_t kTailCells = 4800; // LDS capacity of the tail (16^3 + 8^3 + 
 
 __device__ __forceinline__ void dcoord(const DenseLevel &L, uint32_t i, int &x, int &y, int &z)
 {
-    x = (int)(i % (uint32_t)L.dim[0]);
-    y = (int)((i / (uint32_t)L.dim[0]) % (uint32_t)L.dim[1]);
-    z = (int)(i / (uint32_t)(L.dim[0] * L.dim[1]));
+    const uint32_t pc = L.coord[i];
+    x = (int)(pc & 1023u);
+    y = (int)((pc >> 10) & 1023u);
+    z = (int)(pc >> 20);
 }
 
-__device__ __forceinline__ uint32_t dtype_at(const DenseLevel &L, int x, int y, int z)
+// Index offset of neighbour n (order -x +x -y +y -z +z) in a dense level.
+__device__ __forceinline__ int doff(const DenseLevel &L, int n)
 {
-    if (x < 0 || y < 0 || z < 0 || x >= L.dim[0] || y >= L.dim[1] || z >= L.dim[2]) return kCellSolid;
-    return L.type[(uint32_t)(x + L.dim[0] * (y + L.dim[1] * z))];
+    const int s = n < 2 ? 1 : (n < 4 ? L.dim[0] : L.dim[0] * L.dim[1]);
+    return (n & 1) ? s : -s;
 }
 
-// Stencil of a dense coarse level (SloshPressureMG.usf:142-157 without the ghost diagonal): diagonal = 1e-4 +
-// non-solid neighbours, sum over fluid neighbours, neighbours in the order -x +x -y +y -z +z.
-__device__ __forceinline__ void dstencil(const DenseLevel &L, const float *buf, int x, int y, int z, float &dg,
-                                         float &sum)
+__device__ __forceinline__ float dsum(const DenseLevel &L, const float *v, uint32_t i, uint32_t info)
 {
-    dg = kDiagEps;
-    sum = 0.0f;
-    for (int nb = 0; nb < 6; ++nb) {
-        int d[3];
-        nb_offset(nb, d);
-        int nx = x + d[0], ny = y + d[1], nz = z + d[2];
-        if (nx < 0 || ny < 0 || nz < 0 || nx >= L.dim[0] || ny >= L.dim[1] || nz >= L.dim[2]) continue;
-        uint32_t li = (uint32_t)(nx + L.dim[0] * (ny + L.dim[1] * nz));
-        uint32_t t = L.type[li];
-        if (t != kCellSolid) dg += 1.0f;
-        if (t == kCellFluid) sum += buf[li];
-    }
+    float s = 0.0f;
+    for (int n = 0; n < 6; ++n)
+        if (info >> n & 1u) s += v[(int)i + doff(L, n)];
+    return s;
 }
 
 enum FinKind { kFinInit = 0, kFinRho = 1, kFinAlpha = 2, kFinRr = 3, kFinSor = 4 };
 
-// Sums the partials of the preceding reduction in a fixed order and applies them to the solver state.
-__global__ __launch_bounds__(256) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
-                                                  SolverState *st)
+// Sums the partials of the preceding reduction in a fixed order (per thread: a strided run loaded four at a time;
+// then a wavefront butterfly and the 16 wavefront sums in order) and applies them to the solver state.
+__global__ __launch_bounds__(1024) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
+                                                   SolverState *st)
 {
     if (kind != kFinInit && st->done) return;
     const int nv = (kind == kFinInit || kind == kFinSor) ? 2 : 1;
-    __shared__ double sh[2][256];
+    __shared__ double sh[2][16];
+    const uint32_t t = threadIdx.x;
     double a0 = 0.0, a1 = 0.0;
-    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
+    uint32_t b = t;
+    for (; b + 3072u < nblocks; b += 4096u) {
+        double v0[4], v1[4];
+        for (int k = 0; k < 4; ++k) {
+            v0[k] = partials[(size_t)(b + 1024u * k) * nv];
+            v1[k] = nv == 2 ? partials[(size_t)(b + 1024u * k) * nv + 1] : 0.0;
+        }
+        for (int k = 0; k < 4; ++k) {
+            a0 += v0[k];
+            a1 += v1[k];
+        }
+    }
+    for (; b < nblocks; b += 1024u) {
         a0 += partials[(size_t)b * nv];
         if (nv == 2) a1 += partials[(size_t)b * nv + 1];
     }
-    sh[0][threadIdx.x] = a0;
-    sh[1][threadIdx.x] = a1;
-    __syncthreads();
-    for (uint32_t s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
-        }
-        __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+        a0 += __shfl_xor(a0, w);
+        a1 += __shfl_xor(a1, w);
     }
-    if (threadIdx.x != 0) return;
-    const double t0 = sh[0][0], t1 = sh[1][0];
+    if ((t & 63u) == 0u) {
+        sh[0][t >> 6] = a0;
+        sh[1][t >> 6] = a1;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    double t0 = 0.0, t1 = 0.0;
+    for (int w = 0; w < 16; ++w) {
+        t0 += sh[0][w];
+        t1 += sh[1][w];
+    }
     if (kind == kFinInit || kind == kFinSor) {
         // t0 = ||b||^2, t1 = ||r||^2
         st->bb = t0;
@@ -142,55 +151,96 @@ __global__ void k_build_type(DenseLevel F, DenseLevel C)
     C.type[i] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
 }
 
-__device__ __forceinline__ void smooth_cell(const DenseLevel &L, uint32_t colour, uint32_t i)
+// Stencil word of a dense level's cells (after its types).
+__global__ void k_dinfo(DenseLevel L)
 {
-    if (L.type[i] != kCellFluid) return;
-    int cx, cy, cz;
-    dcoord(L, i, cx, cy, cz);
-    if ((is_red(cx, cy, cz) ? 0u : 1u) != colour) return;
-    float dg, s;
-    dstencil(L, L.x, cx, cy, cz, dg, s);
-    L.x[i] = (L.b[i] + s) / dg;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.cells) return;
+    uint32_t info = 0u;
+    if (L.type[i] == kCellFluid) {
+        int x, y, z;
+        dcoord(L, i, x, y, z);
+        info = kInfoSelf | (is_red(x, y, z) ? kInfoRed : 0u);
+        uint32_t cnt = 0u;
+        for (int n = 0; n < 6; ++n) {
+            int d[3];
+            nb_offset(n, d);
+            const int nx = x + d[0], ny = y + d[1], nz = z + d[2];
+            if (nx < 0 || ny < 0 || nz < 0 || nx >= L.dim[0] || ny >= L.dim[1] || nz >= L.dim[2]) continue;
+            const uint32_t t = L.type[(uint32_t)(nx + L.dim[0] * (ny + L.dim[1] * nz))];
+            if (t != kCellSolid) cnt++;
+            if (t == kCellFluid) info |= 1u << n;
+        }
+        info |= cnt << 8;
+    }
+    L.info[i] = (uint16_t)info;
 }
 
-// SloshPressureMG.usf:343-381 (MGRestrictCS)
+// SloshPressureMG.usf:311-338 (MGSmoothCS), one colour (0 red, 1 black).
+__device__ __forceinline__ void smooth_cell(const DenseLevel &L, uint32_t colour, uint32_t i)
+{
+    const uint32_t info = L.info[i];
+    if (!(info & kInfoSelf) || ((info & kInfoRed) ? 0u : 1u) != colour) return;
+    L.x[i] = (L.b[i] + dsum(L, L.x, i, info)) / info_diag(info);
+}
+
+// SloshPressureMG.usf:343-381 (MGRestrictCS) with the fused red sweep x_c = b_c / diag_c.
 __device__ __forceinline__ void restrict_cell(const DenseLevel &F, const DenseLevel &C, uint32_t ci)
 {
-    int cx, cy, cz;
-    dcoord(C, ci, cx, cy, cz);
-    if (C.type[ci] != kCellFluid) {
+    const uint32_t cinfo = C.info[ci];
+    if (!(cinfo & kInfoSelf)) {
         C.b[ci] = 0.0f;
         C.x[ci] = 0.0f;
         return;
     }
+    int cx, cy, cz;
+    dcoord(C, ci, cx, cy, cz);
     float sum = 0.0f;
     for (uint32_t oi = 0; oi < 8u; ++oi) {
         int fx = 2 * cx + (int)(oi & 1u), fy = 2 * cy + (int)((oi >> 1) & 1u), fz = 2 * cz + (int)(oi >> 2);
-        if (dtype_at(F, fx, fy, fz) != kCellFluid) continue;
-        uint32_t fi = (uint32_t)(fx + F.dim[0] * (fy + F.dim[1] * fz));
-        float dg, s;
-        dstencil(F, F.x, fx, fy, fz, dg, s);
-        sum += F.b[fi] - (dg * F.x[fi] - s);
+        if (fx >= F.dim[0] || fy >= F.dim[1] || fz >= F.dim[2]) continue;
+        const uint32_t fi = (uint32_t)(fx + F.dim[0] * (fy + F.dim[1] * fz));
+        const uint32_t finfo = F.info[fi];
+        if (!(finfo & kInfoSelf)) continue;
+        sum += F.b[fi] - (info_diag(finfo) * F.x[fi] - dsum(F, F.x, fi, finfo));
     }
-    float bv = 0.25f * sum;
+    const float bv = 0.25f * sum;
     C.b[ci] = bv;
-    float cdg = kDiagEps;
-    for (int nb = 0; nb < 6; ++nb) {
-        int d[3];
-        nb_offset(nb, d);
-        cdg += dtype_at(C, cx + d[0], cy + d[1], cz + d[2]) != kCellSolid ? 1.0f : 0.0f;
+    C.x[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
+}
+
+// The same restriction into level 4 from level 3's residuals (dense, of dimensions fdim, zero on every cell that is
+// not fluid: adding +0 for a skipped child leaves the sum unchanged, since the sum starts at +0 and never becomes
+// -0).
+__device__ __forceinline__ void restrict_res_cell(const int fdim[3], const float *res, const DenseLevel &C, uint32_t ci)
+{
+    const uint32_t cinfo = C.info[ci];
+    if (!(cinfo & kInfoSelf)) {
+        C.b[ci] = 0.0f;
+        C.x[ci] = 0.0f;
+        return;
     }
-    C.x[ci] = is_red(cx, cy, cz) ? bv / cdg : 0.0f;
+    int cx, cy, cz;
+    dcoord(C, ci, cx, cy, cz);
+    float sum = 0.0f;
+    for (uint32_t oi = 0; oi < 8u; ++oi) {
+        int fx = 2 * cx + (int)(oi & 1u), fy = 2 * cy + (int)((oi >> 1) & 1u), fz = 2 * cz + (int)(oi >> 2);
+        if (fx >= fdim[0] || fy >= fdim[1] || fz >= fdim[2]) continue;
+        sum += res[fx + fdim[0] * (fy + fdim[1] * fz)];
+    }
+    const float bv = 0.25f * sum;
+    C.b[ci] = bv;
+    C.x[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
 }
 
 // SloshPressureMG.usf:418-433 (MGProlongCS)
 __device__ __forceinline__ void prolong_cell(const DenseLevel &F, const DenseLevel &C, uint32_t i)
 {
-    if (F.type[i] != kCellFluid) return;
+    if (!(F.info[i] & kInfoSelf)) return;
     int x, y, z;
     dcoord(F, i, x, y, z);
     uint32_t pi = (uint32_t)((x >> 1) + C.dim[0] * ((y >> 1) + C.dim[1] * (z >> 1)));
-    if (C.type[pi] == kCellFluid) F.x[i] += C.x[pi];
+    if (C.info[pi] & kInfoSelf) F.x[i] += C.x[pi];
 }
 
 __global__ void k_dsmooth(DenseLevel L, uint32_t colour, const SolverState *st)
@@ -207,6 +257,87 @@ __global__ void k_drestrict(DenseLevel F, DenseLevel C, const SolverState *st)
     if (i < C.cells) restrict_cell(F, C, i);
 }
 
+__global__ void k_drestrict_res(DenseLevel F, const float *res, DenseLevel C, const SolverState *st)
+{
+    if (st->done) return;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < C.cells) restrict_res_cell(F.dim, res, C, i);
+}
+
+// Level 3 (one cell per brick), down pass: black sweep, then the residual of every cell (0 where not fluid). The
+// residual needs the black value of a neighbour that another workgroup is producing: it is recomputed from the
+// red values, which this pass does not write.
+__global__ void k_d3down(DenseLevel L, float *res, const SolverState *st)
+{
+    if (st->done) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.cells) return;
+    const uint32_t info = L.info[i];
+    if (!(info & kInfoSelf)) {
+        res[i] = 0.0f;
+        return;
+    }
+    float xc;
+    if (!(info & kInfoRed)) {
+        xc = (L.b[i] + dsum(L, L.x, i, info)) / info_diag(info);
+        L.x[i] = xc;
+    } else {
+        xc = L.x[i];
+    }
+    float s = 0.0f;
+    for (int n = 0; n < 6; ++n) {
+        if (!(info >> n & 1u)) continue;
+        const uint32_t j = (uint32_t)((int)i + doff(L, n));
+        if (info & kInfoRed) {
+            const uint32_t jinfo = L.info[j];
+            s += (L.b[j] + dsum(L, L.x, j, jinfo)) / info_diag(jinfo);
+        } else {
+            s += L.x[j];
+        }
+    }
+    res[i] = L.b[i] - (info_diag(info) * xc - s);
+}
+
+// Down-sweep iterate of level-3 cell j prolonged from level 4 (a parent contributes only when it is fluid).
+__device__ __forceinline__ float d3_prolonged(const DenseLevel &L, const DenseLevel &P, uint32_t j)
+{
+    float v = L.x[j];
+    int x, y, z;
+    dcoord(L, j, x, y, z);
+    const uint32_t pi = (uint32_t)((x >> 1) + P.dim[0] * ((y >> 1) + P.dim[1] * (z >> 1)));
+    if (P.info[pi] & kInfoSelf) v += P.x[pi];
+    return v;
+}
+
+__device__ __forceinline__ float d3_black_up(const DenseLevel &L, const DenseLevel &P, uint32_t j, uint32_t info)
+{
+    float s = 0.0f;
+    for (int n = 0; n < 6; ++n)
+        if (info >> n & 1u) s += d3_prolonged(L, P, (uint32_t)((int)j + doff(L, n)));
+    return (L.b[j] + s) / info_diag(info);
+}
+
+// Level 3, up pass from level 4: prolongation, black sweep, red sweep; result in x1.
+__global__ void k_d3up(DenseLevel L, DenseLevel P, const SolverState *st)
+{
+    if (st->done) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.cells) return;
+    const uint32_t info = L.info[i];
+    if (!(info & kInfoSelf)) return;
+    if (!(info & kInfoRed)) {
+        L.x1[i] = d3_black_up(L, P, i, info);
+        return;
+    }
+    float s = 0.0f;
+    for (int n = 0; n < 6; ++n) {
+        if (!(info >> n & 1u)) continue;
+        const uint32_t j = (uint32_t)((int)i + doff(L, n));
+        s += d3_black_up(L, P, j, L.info[j]);
+    }
+    L.x1[i] = (L.b[i] + s) / info_diag(info);
+}
+
 __global__ void k_dprolong(DenseLevel F, DenseLevel C, const SolverState *st)
 {
     if (st->done) return;
@@ -214,70 +345,95 @@ __global__ void k_dprolong(DenseLevel F, DenseLevel C, const SolverState *st)
     if (i < F.cells) prolong_cell(F, C, i);
 }
 
-struct TailLevels {
-    int n;                       // levels t .. m-1, the last one the coarsest
+struct TailArgs {
+    int n;          // dense levels in the tail, the last one the coarsest
+    int from_res;   // the first tail level is restricted from level 3's residuals, else its b and x are in memory
+    int fdim[3];    // level 3's dimensions
+    uint32_t off[kMaxLevels];
     DenseLevel L[kMaxLevels];
 };
 
-// The dense levels of the V-cycle (SloshProxy.cpp:904-959) in one 1024-thread workgroup, entirely in LDS: black
-// pre-sweep of the first tail level, restrictions with fused red sweeps and black pre-sweeps down to the
-// coarsest level, the exact FP64 coarsest solve, then prolongation with black and red post-sweeps back up. Only
-// the first level's right-hand side and initial guess come from global memory, and only its result goes back.
-__global__ __launch_bounds__(1024) void k_coarse_tail(TailLevels T, const uint32_t *cidx, const uint32_t *cn,
-                                                      const double *inv, const SolverState *st)
+__device__ __forceinline__ DenseLevel tview(const TailArgs &T, int k, uint16_t *si, uint32_t *sc, float *sx, float *sb)
+{
+    DenseLevel V = T.L[k];
+    V.info = si + T.off[k];
+    V.coord = sc + T.off[k];
+    V.x = sx + T.off[k];
+    V.b = sb + T.off[k];
+    return V;
+}
+
+// The dense levels of the V-cycle (SloshProxy.cpp:904-959) in one 1024-thread workgroup, entirely in LDS (stencil
+// words, cell coordinates and both vectors): the restriction into the first tail level, its black pre-sweep,
+// restrictions with fused red sweeps and black pre-sweeps down to the coarsest level, the exact FP64 coarsest
+// solve, then prolongation with black and red post-sweeps back up. Only the first level's result goes back to
+// memory.
+__global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const float *res3, const uint32_t *cidx,
+                                                      const uint32_t *cn, const double *inv, const SolverState *st)
 {
     if (st->done) return;
     constexpr int N = (int)kCoarsestCells;
-    __shared__ uint8_t s_type[kTailCells];
+    __shared__ uint16_t s_info[kTailCells];
+    __shared__ uint32_t s_coord[kTailCells];
     __shared__ float s_x[kTailCells], s_b[kTailCells];
     __shared__ double bs[N];
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    DenseLevel L[kMaxLevels];
-    uint32_t off = 0;
-    for (int k = 0; k < T.n; ++k) {
-        L[k] = T.L[k];
-        L[k].type = s_type + off;
-        L[k].x = s_x + off;
-        L[k].b = s_b + off;
-        for (uint32_t i = tid; i < L[k].cells; i += nt) L[k].type[i] = T.L[k].type[i];
-        off += L[k].cells;
-    }
-    for (uint32_t i = tid; i < L[0].cells; i += nt) {
-        L[0].x[i] = T.L[0].x[i];
-        L[0].b[i] = T.L[0].b[i];
-    }
-    __syncthreads();
     const int m = T.n;
-    if (m > 1) {
-        for (uint32_t i = tid; i < L[0].cells; i += nt) smooth_cell(L[0], 1u, i);
+    for (int k = 0; k < m; ++k)
+        for (uint32_t i = tid; i < T.L[k].cells; i += nt) {
+            s_info[T.off[k] + i] = T.L[k].info[i];
+            s_coord[T.off[k] + i] = T.L[k].coord[i];
+        }
+    __syncthreads();
+    {
+        const DenseLevel V0 = tview(T, 0, s_info, s_coord, s_x, s_b);
+        if (T.from_res) {
+            for (uint32_t i = tid; i < V0.cells; i += nt) restrict_res_cell(T.fdim, res3, V0, i);
+        } else {
+            for (uint32_t i = tid; i < V0.cells; i += nt) {
+                V0.x[i] = T.L[0].x[i];
+                V0.b[i] = T.L[0].b[i];
+            }
+        }
         __syncthreads();
-    }
-    for (int l = 0; l < m - 1; ++l) {
-        for (uint32_t ci = tid; ci < L[l + 1].cells; ci += nt) restrict_cell(L[l], L[l + 1], ci);
-        __syncthreads();
-        if (l + 1 < m - 1) {
-            for (uint32_t i = tid; i < L[l + 1].cells; i += nt) smooth_cell(L[l + 1], 1u, i);
+        if (m > 1) {
+            for (uint32_t i = tid; i < V0.cells; i += nt) smooth_cell(V0, 1u, i);
             __syncthreads();
         }
     }
-    const int n = (int)*cn;
-    if ((int)tid < n) bs[tid] = (double)L[m - 1].b[cidx[tid]];
-    __syncthreads();
-    if ((int)tid < n) {
+    for (int l = 0; l < m - 1; ++l) {
+        const DenseLevel F = tview(T, l, s_info, s_coord, s_x, s_b), C = tview(T, l + 1, s_info, s_coord, s_x, s_b);
+        for (uint32_t ci = tid; ci < C.cells; ci += nt) restrict_cell(F, C, ci);
+        __syncthreads();
+        if (l + 1 < m - 1) {
+            for (uint32_t i = tid; i < C.cells; i += nt) smooth_cell(C, 1u, i);
+            __syncthreads();
+        }
+    }
+    {
+        // x = A^-1 b on the coarsest level: 16 lanes per row, partial sums combined by a fixed butterfly
+        const DenseLevel Vc = tview(T, m - 1, s_info, s_coord, s_x, s_b);
+        const int n = (int)*cn;
+        if ((int)tid < n) bs[tid] = (double)Vc.b[cidx[tid]];
+        __syncthreads();
+        const int row = (int)(tid >> 4), part = (int)(tid & 15u);
         double s = 0.0;
-        for (int j = 0; j < n; ++j) s += inv[(size_t)tid * N + j] * bs[j];
-        L[m - 1].x[cidx[tid]] = (float)s;
+        if (row < n)
+            for (int j = part; j < n; j += 16) s += inv[(size_t)row * N + j] * bs[j];
+        for (int w = 8; w > 0; w >>= 1) s += __shfl_xor(s, w, 16);
+        if (row < n && part == 0) Vc.x[cidx[row]] = (float)s;
+        __syncthreads();
     }
-    __syncthreads();
     for (int l = m - 2; l >= 0; --l) {
-        for (uint32_t i = tid; i < L[l].cells; i += nt) prolong_cell(L[l], L[l + 1], i);
+        const DenseLevel F = tview(T, l, s_info, s_coord, s_x, s_b), C = tview(T, l + 1, s_info, s_coord, s_x, s_b);
+        for (uint32_t i = tid; i < F.cells; i += nt) prolong_cell(F, C, i);
         __syncthreads();
-        for (uint32_t i = tid; i < L[l].cells; i += nt) smooth_cell(L[l], 1u, i);
+        for (uint32_t i = tid; i < F.cells; i += nt) smooth_cell(F, 1u, i);
         __syncthreads();
-        for (uint32_t i = tid; i < L[l].cells; i += nt) smooth_cell(L[l], 0u, i);
+        for (uint32_t i = tid; i < F.cells; i += nt) smooth_cell(F, 0u, i);
         __syncthreads();
     }
-    for (uint32_t i = tid; i < L[0].cells; i += nt) T.L[0].x[i] = L[0].x[i];
+    for (uint32_t i = tid; i < T.L[0].cells; i += nt) T.L[0].x[i] = s_x[T.off[0] + i];
 }
 
 // ---- coarsest level: FP64 Cholesky of the <= 64 fluid unknowns and its explicit inverse (once per solve) ----
@@ -410,100 +566,101 @@ __global__ void k_state_reset(SolverState *st, double tol, int maxit)
 void finalize(slosh_ctx *c, int kind, int iters = 0)
 {
     SLOSH_TIMED(c, K_FINALIZE);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, kind, c->partials, l0_groups(c), iters, c->state);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, kind, c->partials, c->nactive, iters, c->state);
 }
 
-void smooth_level(slosh_ctx *c, int l, uint32_t colour, int want_dot)
+void coarse_solve(slosh_ctx *c, const float *b, float *x)
 {
-    if (l == 0) {
-        l0_smooth(c, colour, want_dot);
-        if (want_dot) finalize(c, kFinRho);
-    } else {
-        pooled_smooth(c, l, colour);
-    }
+    SLOSH_TIMED(c, K_COARSE_SOLVE);
+    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(64), 0, c->stream, c->coarse_idx, c->coarse_n, c->coarse_inv, b,
+                       x, c->state);
 }
 
-// SloshProxy.cpp:904-959 (VCycle) with the exact coarsest solve. The level-0 red pre-sweep is already in z.
-void vcycle(slosh_ctx *c)
+// Levels 3 .. m-1 (m >= 5): level 3's down pass, grid-wide passes for dense levels kFirstDense .. t-1 (only when
+// the dense levels do not all fit the tail's LDS), the tail on levels t .. m-1, the grid-wide up passes, level 3's
+// up pass.
+void dense_part(slosh_ctx *c)
+{
+    const int m = c->nlevels;
+    const DenseLevel L3 = dlevel_of(c, 3);
+    {
+        SLOSH_TIMED(c, K_RESTRICT);
+        hipLaunchKernelGGL(k_d3down, dim3(blocks_for(L3.cells, 256)), dim3(256), 0, c->stream, L3, c->res3, c->state);
+    }
+    int t = m - 1;
+    uint32_t tail_cells = c->lv[m - 1].cells;
+    while (t - 1 >= kFirstDense && tail_cells + c->lv[t - 1].cells <= kTailCells) tail_cells += c->lv[--t].cells;
+    if (t > kFirstDense) {
+        DenseLevel C = dlevel_of(c, kFirstDense);
+        SLOSH_TIMED(c, K_RESTRICT);
+        hipLaunchKernelGGL(k_drestrict_res, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, L3, c->res3, C,
+                           c->state);
+    }
+    for (int l = kFirstDense; l < t; ++l) {
+        DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
+        {
+            SLOSH_TIMED(c, K_SMOOTH);
+            hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, 1u, c->state);
+        }
+        SLOSH_TIMED(c, K_RESTRICT);
+        hipLaunchKernelGGL(k_drestrict, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
+    }
+    TailArgs T{};
+    T.n = m - t;
+    T.from_res = t == kFirstDense ? 1 : 0;
+    for (int a = 0; a < 3; ++a) T.fdim[a] = L3.dim[a];
+    uint32_t off = 0;
+    for (int k = 0; k < T.n; ++k) {
+        T.L[k] = dlevel_of(c, t + k);
+        T.off[k] = off;
+        off += T.L[k].cells;
+    }
+    {
+        SLOSH_TIMED(c, K_COARSE_TAIL);
+        hipLaunchKernelGGL(k_coarse_tail, dim3(1), dim3(1024), 0, c->stream, T, c->res3, c->coarse_idx, c->coarse_n,
+                           c->coarse_inv, c->state);
+    }
+    for (int l = t - 1; l >= kFirstDense; --l) {
+        DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
+        {
+            SLOSH_TIMED(c, K_PROLONG);
+            hipLaunchKernelGGL(k_dprolong, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
+        }
+        for (uint32_t colour : {1u, 0u}) {
+            SLOSH_TIMED(c, K_SMOOTH);
+            hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, colour,
+                               c->state);
+        }
+    }
+    SLOSH_TIMED(c, K_PROLONG);
+    hipLaunchKernelGGL(k_d3up, dim3(blocks_for(L3.cells, 256)), dim3(256), 0, c->stream, L3, dlevel_of(c, kFirstDense),
+                       c->state);
+}
+
+// SloshProxy.cpp:904-959 (VCycle) with the exact coarsest solve, from the level-0 down-sweep iterate (red
+// initialisation and black sweep already in z) to the red sweep of the result z1 and the r.z1 reduction.
+void vcycle(slosh_ctx *c, const float *r)
 {
     const int m = c->nlevels;
     if (m == 1) {
-        {
-            SLOSH_TIMED(c, K_COARSE_SOLVE);
-            hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(64), 0, c->stream, c->coarse_idx, c->coarse_n,
-                               c->coarse_inv, c->r, c->z, c->state);
-        }
-        smooth_level(c, 0, 0u, 1);
-        return;
-    }
-    // multi-workgroup levels: 0 .. top, where top is the coarsest pooled level
-    const int top = m - 1 < kFirstDense ? m - 1 : kFirstDense - 1;
-    smooth_level(c, 0, 1u, 0);
-    for (int l = 0; l < top; ++l) {
-        if (l == 0) l0_restrict(c);
-        else pooled_restrict(c, l);
-        if (l + 1 < m - 1) smooth_level(c, l + 1, 1u, 0);
-    }
-    if (m - 1 == top) {
-        // the coarsest level is pooled (or level 1..3): exact solve on it
-        SLOSH_TIMED(c, K_COARSE_SOLVE);
-        hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(64), 0, c->stream, c->coarse_idx, c->coarse_n, c->coarse_inv,
-                           c->lv[top].b, c->lv[top].x, c->state);
+        coarse_solve(c, r, c->z1);
     } else {
-        // dense levels kFirstDense .. t-1 as grid-wide passes, levels t .. m-1 in the LDS tail
-        int t = m - 1;
-        uint32_t tail_cells = c->lv[m - 1].cells;
-        while (t - 1 >= kFirstDense && tail_cells + c->lv[t - 1].cells <= kTailCells) tail_cells += c->lv[--t].cells;
-        restrict_pooled_to_dense(c, top);
-        for (int l = kFirstDense; l < t; ++l) {
-            DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
-            {
-                SLOSH_TIMED(c, K_SMOOTH);
-                hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, 1u, c->state);
-            }
-            SLOSH_TIMED(c, K_RESTRICT);
-            hipLaunchKernelGGL(k_drestrict, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
-        }
-        TailLevels T;
-        T.n = m - t;
-        for (int k = 0; k < kMaxLevels; ++k) T.L[k] = k < T.n ? dlevel_of(c, t + k) : DenseLevel{};
-        {
-            SLOSH_TIMED(c, K_COARSE_TAIL);
-            hipLaunchKernelGGL(k_coarse_tail, dim3(1), dim3(1024), 0, c->stream, T, c->coarse_idx, c->coarse_n,
-                               c->coarse_inv, c->state);
-        }
-        for (int l = t - 1; l >= kFirstDense; --l) {
-            DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
-            {
-                SLOSH_TIMED(c, K_PROLONG);
-                hipLaunchKernelGGL(k_dprolong, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
-            }
-            for (uint32_t colour : {1u, 0u}) {
-                SLOSH_TIMED(c, K_SMOOTH);
-                hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, colour,
-                                   c->state);
-            }
-        }
-        prolong_dense_to_pooled(c, top);
-        smooth_level(c, top, 1u, 0);
-        smooth_level(c, top, 0u, 0);
+        l0_restrict(c, r);
+        const int last_pooled = m - 2 < 2 ? m - 2 : 2; // pooled levels with a down pass (1, 2)
+        for (int l = 1; l <= last_pooled; ++l) pooled_down(c, l);
+        if (m - 1 <= 3) coarse_solve(c, c->lv[m - 1].b, c->lv[m - 1].x1);
+        else dense_part(c);
+        for (int l = last_pooled; l >= 1; --l) pooled_up(c, l);
+        l0_prolong_black(c, r);
     }
-    for (int l = top - 1; l >= 0; --l) {
-        if (l == 0) l0_prolong(c);
-        else pooled_prolong(c, l);
-        smooth_level(c, l, 1u, 0);
-        smooth_level(c, l, 0u, l == 0 ? 1 : 0);
-    }
+    l0_red_dot(c, r);
+    finalize(c, kFinRho);
 }
 
-} // namespace
-
-// SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG), to ||r|| <= tol ||b||.
-void run_mgpcg(slosh_ctx *c)
+// Per-solve set-up: brick table of the allocated bricks, coarse types and stencil words, coarsest factor.
+void mg_setup(slosh_ctx *c)
 {
-    if (!c->nactive) return;
-    const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 200;
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
+    l0_brick_info(c);
     const int m = c->nlevels;
     for (int l = 1; l < m; ++l) {
         if (c->lv[l].pooled) {
@@ -517,28 +674,51 @@ void run_mgpcg(slosh_ctx *c)
                                C);
         }
     }
-    {
-        const int mode = m == 1 ? 0 : (c->lv[m - 1].pooled ? 1 : 2);
-        PLevel P = mode == 1 ? plevel_of(c, m - 1) : PLevel{};
-        DenseLevel D = mode == 2 ? dlevel_of(c, m - 1) : DenseLevel{};
-        SLOSH_TIMED(c, K_COARSE_FACTOR);
-        hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(256), 0, c->stream, c->geo, c->ctype, c->diag, mode, P, D,
-                           c->coarse_idx, c->coarse_n, c->coarse_inv);
+    for (int l = 1; l < m; ++l) {
+        if (c->lv[l].pooled) {
+            pooled_build_info(c, l);
+        } else {
+            DenseLevel D = dlevel_of(c, l);
+            SLOSH_TIMED(c, K_MG_TYPES);
+            hipLaunchKernelGGL(k_dinfo, dim3(blocks_for(D.cells, 256)), dim3(256), 0, c->stream, D);
+        }
     }
-    l0_pcg_init(c);
+    const int mode = m == 1 ? 0 : (c->lv[m - 1].pooled ? 1 : 2);
+    PLevel P = mode == 1 ? plevel_of(c, m - 1) : PLevel{};
+    DenseLevel D = mode == 2 ? dlevel_of(c, m - 1) : DenseLevel{};
+    SLOSH_TIMED(c, K_COARSE_FACTOR);
+    hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(256), 0, c->stream, c->geo, c->ctype, c->diag, mode, P, D,
+                       c->coarse_idx, c->coarse_n, c->coarse_inv);
+}
+
+} // namespace
+
+// SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG), to ||r|| <= tol ||b||.
+void run_mgpcg(slosh_ctx *c)
+{
+    if (!c->nactive) return;
+    const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 200;
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
+    mg_setup(c);
+    float *R[2] = {c->r, c->r2};
+    float *D[2] = {c->d, c->d2};
+    l0_pcg_init(c, R[0]);
     finalize(c, kFinInit);
-    vcycle(c);
-    l0_update_d(c, 1);
+    l0_update_black(c, R[0], R[1], D[0], 1);
+    int rc = 1, dc = 0;
+    vcycle(c, R[rc]);
+    l0_direction(c, D[1], D[0], 1);
     SLOSH_HIP(hipGetLastError());
     const int chunk = 4;
     for (int it = 0; it <= maxit; it += chunk) {
         for (int k = 0; k < chunk; ++k) {
-            l0_apply_a(c);
             finalize(c, kFinAlpha);
-            l0_update_pr(c);
+            l0_update_black(c, R[rc], R[rc ^ 1], D[dc], 0);
+            rc ^= 1;
             finalize(c, kFinRr);
-            vcycle(c);
-            l0_update_d(c, 0);
+            vcycle(c, R[rc]);
+            l0_direction(c, D[dc], D[dc ^ 1], 0);
+            dc ^= 1;
         }
         SLOSH_HIP(hipGetLastError());
         SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
@@ -553,6 +733,7 @@ void run_rbsor(slosh_ctx *c)
     if (!c->nactive) return;
     const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 1000;
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
+    l0_brick_info(c);
     const double w = (double)c->s.sor_weight;
     int it = 0;
     auto residual = [&](int iters) {
