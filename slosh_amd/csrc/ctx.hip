@@ -237,8 +237,8 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->bp_div_max, g.npool);
     dalloc(&c->bp_div_cnt, g.npool);
     dalloc(&c->bp_div_sum, g.npool);
-    dalloc(&c->bp_speed, c->ntiles_max);
-    dalloc(&c->bp_clamp, c->ntiles_max);
+    dalloc(&c->bp_speed, g.nbricks);
+    dalloc(&c->bp_clamp, g.nbricks);
     dalloc(&c->state, 1);
     dalloc(&c->stats, 1);
     dalloc(&c->d_col, SLOSH_MAX_COLLIDERS);

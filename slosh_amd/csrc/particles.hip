@@ -371,8 +371,84 @@ __device__ __forceinline__ uint32_t face_index(const Geo &g, int axis, int x, in
     return (uint32_t)axis * g.pool_cells + cell_index(g, x, y, z);
 }
 
+// Face velocities for G2P: the 12^3 nodes [brick_min - 2, brick_min + 10) of each component staged in LDS by the
+// brick's workgroup, with a global read (through the brick table) for a node outside that box.
+constexpr int kG2PBox = 12;
+constexpr int kG2PBoxNodes = kG2PBox * kG2PBox * kG2PBox;
+
+struct FaceSource {
+    const float *vel;
+    const float *box; // [3][kG2PBoxNodes], the workgroup's LDS box
+    bool staged;      // the box holds this brick's neighbourhood
+    int org[3];       // global node of box entry 0
+    // True when the 2x2x2 nodes from `base` (before clamping to the grid) all lie in the box; clamping only moves a
+    // node toward the grid interior, which the box of an in-grid brick covers.
+    __device__ bool covers(const int base[3]) const
+    {
+        bool in = staged;
+        for (int a = 0; a < 3; ++a) in = in && base[a] >= org[a] && base[a] + 1 < org[a] + kG2PBox;
+        return in;
+    }
+    __device__ float lds(int axis, int x, int y, int z) const
+    {
+        return box[axis * kG2PBoxNodes + (x - org[0]) + kG2PBox * ((y - org[1]) + kG2PBox * (z - org[2]))];
+    }
+};
+
+// Staged fast path of the two samplers below, for a stencil whose nodes need no clamping and lie in the box. The
+// 2x2x2 weights are formed from the per-axis factors with the reference's association, w = (wx wy) wz and
+// grad_a = (sign_a (w_b w_c)) / h, so every product and sum rounds exactly as in the general form.
+__device__ __forceinline__ bool stencil_base(const Geo &g, const FaceSource &src, int axis, const float xp[3],
+                                             int base[3], float fr[3], uint32_t &lds0)
+{
+    bool fast = src.staged;
+    for (int a = 0; a < 3; ++a) {
+        const float gg = xp[a] * g.inv_h - (a == axis ? 0.0f : 0.5f);
+        base[a] = (int)floorf(gg);
+        fr[a] = gg - (float)base[a];
+        fast = fast && base[a] >= 0 && base[a] <= g.n[a] - 2 && base[a] >= src.org[a] &&
+               base[a] + 1 < src.org[a] + kG2PBox;
+    }
+    lds0 = (uint32_t)(axis * kG2PBoxNodes + (base[0] - src.org[0]) +
+                      kG2PBox * ((base[1] - src.org[1]) + kG2PBox * (base[2] - src.org[2])));
+    return fast;
+}
+
+constexpr uint32_t kBoxCorner[8] = {0u, 1u, (uint32_t)kG2PBox, (uint32_t)kG2PBox + 1u, (uint32_t)(kG2PBox * kG2PBox),
+                                    (uint32_t)(kG2PBox * kG2PBox) + 1u, (uint32_t)(kG2PBox * kG2PBox + kG2PBox),
+                                    (uint32_t)(kG2PBox * kG2PBox + kG2PBox) + 1u};
+
+__device__ __forceinline__ void gather_lds(const float *box, uint32_t lds0, const float fr[3], float inv_h,
+                                           float &unew, float grad[3])
+{
+    const float wx[2] = {1.0f - fr[0], fr[0]}, wy[2] = {1.0f - fr[1], fr[1]}, wz[2] = {1.0f - fr[2], fr[2]};
+    float u_acc = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+    for (uint32_t oi = 0; oi < 8u; ++oi) {
+        const uint32_t ox = oi & 1u, oy = (oi >> 1) & 1u, oz = oi >> 2;
+        const float u = box[lds0 + kBoxCorner[oi]];
+        const float w = wx[ox] * wy[oy] * wz[oz];
+        const float a0 = (wy[oy] * wz[oz]) * inv_h, a1 = (wx[ox] * wz[oz]) * inv_h, a2 = (wx[ox] * wy[oy]) * inv_h;
+        u_acc += w * u;
+        g0 += (ox ? a0 : -a0) * u;
+        g1 += (oy ? a1 : -a1) * u;
+        g2 += (oz ? a2 : -a2) * u;
+    }
+    unew = u_acc;
+    grad[0] = g0;
+    grad[1] = g1;
+    grad[2] = g2;
+}
+
+__device__ __forceinline__ float sample_lds(const float *box, uint32_t lds0, const float fr[3])
+{
+    const float wx[2] = {1.0f - fr[0], fr[0]}, wy[2] = {1.0f - fr[1], fr[1]}, wz[2] = {1.0f - fr[2], fr[2]};
+    float u = 0.0f;
+    for (uint32_t oi = 0; oi < 8u; ++oi) u += wx[oi & 1u] * wy[(oi >> 1) & 1u] * wz[oi >> 2] * box[lds0 + kBoxCorner[oi]];
+    return u;
+}
+
 // Slosh.usf:1376-1410 (GatherComponent, APIC).
-__device__ __forceinline__ void gather_component(const Geo &g, const float *vel, int axis, const float xp[3],
+__device__ __forceinline__ void gather_component(const Geo &g, const FaceSource &src, int axis, const float xp[3],
                                                  float &unew, float grad[3])
 {
     float off[3] = {0.5f, 0.5f, 0.5f};
@@ -384,13 +460,23 @@ __device__ __forceinline__ void gather_component(const Geo &g, const float *vel,
         base[a] = (int)floorf(gg[a]);
         fr[a] = gg[a] - (float)base[a];
     }
+    float uv[8];
+    if (src.covers(base)) {
+        for (uint32_t oi = 0; oi < 8u; ++oi)
+            uv[oi] = src.lds(axis, clampi(base[0] + (int)(oi & 1u), 0, g.n[0] - 1),
+                             clampi(base[1] + (int)((oi >> 1) & 1u), 0, g.n[1] - 1),
+                             clampi(base[2] + (int)((oi >> 2) & 1u), 0, g.n[2] - 1));
+    } else {
+        for (uint32_t oi = 0; oi < 8u; ++oi)
+            uv[oi] = src.vel[face_index(g, axis, clampi(base[0] + (int)(oi & 1u), 0, g.n[0] - 1),
+                                        clampi(base[1] + (int)((oi >> 1) & 1u), 0, g.n[1] - 1),
+                                        clampi(base[2] + (int)((oi >> 2) & 1u), 0, g.n[2] - 1))];
+    }
     float u_acc = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
     for (uint32_t oi = 0; oi < 8u; ++oi) {
         int od[3] = {(int)(oi & 1u), (int)((oi >> 1) & 1u), (int)((oi >> 2) & 1u)};
-        int node[3];
         float w3[3], sg[3];
         for (int a = 0; a < 3; ++a) {
-            node[a] = clampi(base[a] + od[a], 0, g.n[a] - 1);
             w3[a] = od[a] ? fr[a] : 1.0f - fr[a];
             sg[a] = od[a] ? 1.0f : -1.0f;
         }
@@ -398,7 +484,7 @@ __device__ __forceinline__ void gather_component(const Geo &g, const float *vel,
         float gw0 = sg[0] * (w3[1] * w3[2]) * g.inv_h;
         float gw1 = sg[1] * (w3[0] * w3[2]) * g.inv_h;
         float gw2 = sg[2] * (w3[0] * w3[1]) * g.inv_h;
-        float u = vel[face_index(g, axis, node[0], node[1], node[2])];
+        float u = uv[oi];
         u_acc += w * u;
         g0 += gw0 * u;
         g1 += gw1 * u;
@@ -411,7 +497,7 @@ __device__ __forceinline__ void gather_component(const Geo &g, const float *vel,
 }
 
 // Slosh.usf:1417-1443 (SampleVelocityComponent).
-__device__ __forceinline__ float sample_component(const Geo &g, const float *vel, const uint8_t *fflags, int axis,
+__device__ __forceinline__ float sample_component(const Geo &g, const FaceSource &src, const uint8_t *fflags, int axis,
                                                   const float xp[3], float fallback, bool check)
 {
     const uint32_t known = kFaceValid | (0xFFFFFFFFu << kFaceExtrapShift);
@@ -424,128 +510,187 @@ __device__ __forceinline__ float sample_component(const Geo &g, const float *vel
         base[a] = (int)floorf(gg[a]);
         fr[a] = gg[a] - (float)base[a];
     }
+    float uv[8];
+    if (src.covers(base) && !check) {
+        for (uint32_t oi = 0; oi < 8u; ++oi)
+            uv[oi] = src.lds(axis, clampi(base[0] + (int)(oi & 1u), 0, g.n[0] - 1),
+                             clampi(base[1] + (int)((oi >> 1) & 1u), 0, g.n[1] - 1),
+                             clampi(base[2] + (int)((oi >> 2) & 1u), 0, g.n[2] - 1));
+    } else {
+        for (uint32_t oi = 0; oi < 8u; ++oi) {
+            const uint32_t f = face_index(g, axis, clampi(base[0] + (int)(oi & 1u), 0, g.n[0] - 1),
+                                          clampi(base[1] + (int)((oi >> 1) & 1u), 0, g.n[1] - 1),
+                                          clampi(base[2] + (int)((oi >> 2) & 1u), 0, g.n[2] - 1));
+            float uf = src.vel[f];
+            if (check && ((uint32_t)fflags[f] & known) == 0u) uf = fallback;
+            uv[oi] = uf;
+        }
+    }
     float u = 0.0f;
     for (uint32_t oi = 0; oi < 8u; ++oi) {
         int od[3] = {(int)(oi & 1u), (int)((oi >> 1) & 1u), (int)((oi >> 2) & 1u)};
-        int node[3];
         float w3[3];
-        for (int a = 0; a < 3; ++a) {
-            node[a] = clampi(base[a] + od[a], 0, g.n[a] - 1);
-            w3[a] = od[a] ? fr[a] : 1.0f - fr[a];
-        }
-        uint32_t f = face_index(g, axis, node[0], node[1], node[2]);
-        float uf = vel[f];
-        if (check && ((uint32_t)fflags[f] & known) == 0u) uf = fallback;
-        u += w3[0] * w3[1] * w3[2] * uf;
+        for (int a = 0; a < 3; ++a) w3[a] = od[a] ? fr[a] : 1.0f - fr[a];
+        u += w3[0] * w3[1] * w3[2] * uv[oi];
     }
     return u;
 }
 
-// Slosh.usf:1451-1678 (AdvectParticle + G2PAdvectCS), APIC; manipulators, adhesion and drains are off.
-__global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, uint32_t np, const float *vel, const uint8_t *fflags,
-                                             float dt, float gx, float gy, float gz, float friction, float radius,
-                                             float max_speed, const slosh_collider *col, uint32_t ncol,
-                                             uint8_t *touch, uint32_t *bp_speed, uint32_t *bp_clamp)
+// Slosh.usf:1451-1678 (AdvectParticle + G2PAdvectCS), APIC; manipulators, adhesion and drains are off. One
+// workgroup per brick key of the sorted particle array: the brick's particles all lie inside it, so their gathers
+// read the staged box (an unallocated brick, whose particles are ballistic, reads through the table).
+__global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key_off, const float *vel,
+                                             const uint8_t *fflags, float dt, float gx, float gy, float gz,
+                                             float friction, float radius, float max_speed, const slosh_collider *col,
+                                             uint32_t ncol, uint8_t *touch, uint32_t *bp_speed, uint32_t *bp_clamp)
 {
+    __shared__ float s_box[3 * kG2PBoxNodes];
     __shared__ uint32_t tb[256];
     __shared__ uint32_t smax, sclamp;
+    const uint32_t key = blockIdx.x;
+    const uint32_t beg = key_off[key], end = key_off[key + 1];
     if (threadIdx.x == 0) {
         smax = 0u;
         sclamp = 0u;
     }
-    __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t brick = kNoBrick;
-    if (i < np) {
-        float4 pa = p.pa[i];
-        if (pa.w >= 0.0f) {
-            const float grav[3] = {gx, gy, gz};
-            float pos[3] = {pa.x, pa.y, pa.z};
-            float pic[3], cm[9];
-            gather_component(g, vel, 0, pos, pic[0], &cm[0]);
-            gather_component(g, vel, 1, pos, pic[1], &cm[3]);
-            gather_component(g, vel, 2, pos, pic[2], &cm[6]);
-            float nv[3] = {pic[0], pic[1], pic[2]};
-            float lo = g.h * 1.01f;
-            float hi[3];
-            for (int a = 0; a < 3; ++a) hi[a] = ((float)g.n[a] - 1.01f) * g.h;
-            float hdt = 0.5f * dt;
-            float mid[3];
-            for (int a = 0; a < 3; ++a) mid[a] = clampf(pos[a] + hdt * pic[a], lo, hi[a]);
-            float midcells = hdt * len3(pic[0], pic[1], pic[2]) * g.inv_h;
-            float v2[3] = {pic[0], pic[1], pic[2]};
-            if (midcells < 2.0f) {
-                bool check = midcells > 1.0f;
-                float t = clampf(midcells - 1.0f, 0.0f, 1.0f);
-                for (int a = 0; a < 3; ++a) {
-                    float sv = sample_component(g, vel, fflags, a, mid, pic[a], check);
-                    v2[a] = sv + t * (pic[a] - sv);
-                }
-            }
-            float x[3];
-            for (int a = 0; a < 3; ++a) x[a] = pos[a] + dt * v2[a];
-            int c[3];
-            pos_cell(g, pos[0], pos[1], pos[2], c);
-            if (cell_index(g, c[0], c[1], c[2]) < kBrickCells) {
-                for (int a = 0; a < 3; ++a) {
-                    nv[a] = p.v[a][i] + grav[a] * dt;
-                    x[a] = pos[a] + dt * nv[a];
-                }
-                for (int k = 0; k < 9; ++k) cm[k] = 0.0f;
-            }
-            for (uint32_t k = 0; k < ncol; ++k) {
-                const slosh_collider cc = col[k];
-                float dd = collider_sdf(cc, x);
-                if (dd < radius) {
-                    float nn[3];
-                    collider_normal(cc, x, 0.25f * g.h, nn);
-                    float push = radius - dd;
-                    for (int a = 0; a < 3; ++a) x[a] = x[a] + nn[a] * push;
-                    float vs[3] = {collider_velocity_axis(cc, x, 0), collider_velocity_axis(cc, x, 1),
-                                   collider_velocity_axis(cc, x, 2)};
-                    float rel[3] = {nv[0] - vs[0], nv[1] - vs[1], nv[2] - vs[2]};
-                    float vn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
-                    if (vn < 0.0f) {
-                        for (int a = 0; a < 3; ++a) rel[a] = rel[a] - vn * nn[a];
-                        float rn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
-                        float tg[3];
-                        for (int a = 0; a < 3; ++a) tg[a] = rel[a] - rn * nn[a];
-                        for (int a = 0; a < 3; ++a) rel[a] = rel[a] - friction * tg[a];
-                    }
-                    for (int a = 0; a < 3; ++a) nv[a] = vs[a] + rel[a];
-                }
-            }
-            uint32_t contact = 0;
-            for (int a = 0; a < 3; ++a) {
-                if (x[a] < lo) {
-                    x[a] = lo;
-                    nv[a] = fmaxf(nv[a], 0.0f);
-                    contact |= 1u << a;
-                } else if (x[a] > hi[a]) {
-                    x[a] = hi[a];
-                    nv[a] = fminf(nv[a], 0.0f);
-                    contact |= 1u << a;
-                }
-            }
-            if (contact != 0u && friction > 0.0f)
-                for (int a = 0; a < 3; ++a)
-                    if ((contact & (1u << a)) == 0u) nv[a] *= 1.0f - friction;
-            float speed = len3(nv[0], nv[1], nv[2]);
-            if (speed > max_speed) atomicAdd(&sclamp, 1u);
-            p.pa[i] = make_float4(x[0], x[1], x[2], pa.w + dt);
-            for (int a = 0; a < 3; ++a) p.v[a][i] = nv[a];
-            for (int k = 0; k < 9; ++k) p.c[k][i] = cm[k];
-            atomicMax(&smax, __float_as_uint(speed));
-            pos_cell(g, x[0], x[1], x[2], c);
-            brick = brick_of_cell(g, c[0], c[1], c[2]);
+    if (beg == end) {
+        if (threadIdx.x == 0) {
+            bp_speed[key] = 0u;
+            bp_clamp[key] = 0u;
+        }
+        return;
+    }
+    int bc[3];
+    brick_coord(g, key, bc);
+    const bool staged = g.table[key] != 0u;
+    FaceSource src;
+    src.vel = vel;
+    src.box = s_box;
+    src.staged = staged;
+    for (int a = 0; a < 3; ++a) src.org[a] = bc[a] * 8 - 2;
+    if (staged) {
+        for (uint32_t k = threadIdx.x; k < (uint32_t)(3 * kG2PBoxNodes); k += 256u) {
+            const int axis = (int)(k / kG2PBoxNodes), q = (int)(k % kG2PBoxNodes);
+            const int x = src.org[0] + q % kG2PBox, y = src.org[1] + (q / kG2PBox) % kG2PBox,
+                      z = src.org[2] + q / (kG2PBox * kG2PBox);
+            s_box[k] = in_grid(g, x, y, z) ? vel[face_index(g, axis, x, y, z)] : 0.0f;
         }
     }
-    tb[threadIdx.x] = brick;
     __syncthreads();
-    if (brick != kNoBrick && (threadIdx.x == 0 || tb[threadIdx.x - 1] != brick)) touch[brick] = 1u;
+    uint32_t vmax = 0u, nclamp = 0u; // per thread; speeds are non-negative, so their bits order like the values
+    for (uint32_t base_i = beg; base_i < end; base_i += 256u) {
+        const uint32_t i = base_i + threadIdx.x;
+        uint32_t brick = kNoBrick;
+        if (i < end) {
+            float4 pa = p.pa[i];
+            if (pa.w >= 0.0f) {
+                const float grav[3] = {gx, gy, gz};
+                float pos[3] = {pa.x, pa.y, pa.z};
+                float pic[3], cm[9];
+                for (int a = 0; a < 3; ++a) {
+                    int bs[3];
+                    float fr[3];
+                    uint32_t l0;
+                    if (stencil_base(g, src, a, pos, bs, fr, l0)) gather_lds(s_box, l0, fr, g.inv_h, pic[a], &cm[3 * a]);
+                    else gather_component(g, src, a, pos, pic[a], &cm[3 * a]);
+                }
+                float nv[3] = {pic[0], pic[1], pic[2]};
+                float lo = g.h * 1.01f;
+                float hi[3];
+                for (int a = 0; a < 3; ++a) hi[a] = ((float)g.n[a] - 1.01f) * g.h;
+                float hdt = 0.5f * dt;
+                float mid[3];
+                for (int a = 0; a < 3; ++a) mid[a] = clampf(pos[a] + hdt * pic[a], lo, hi[a]);
+                float midcells = hdt * len3(pic[0], pic[1], pic[2]) * g.inv_h;
+                float v2[3] = {pic[0], pic[1], pic[2]};
+                if (midcells < 2.0f) {
+                    bool check = midcells > 1.0f;
+                    float t = clampf(midcells - 1.0f, 0.0f, 1.0f);
+                    for (int a = 0; a < 3; ++a) {
+                        int bs[3];
+                        float fr[3];
+                        uint32_t l0;
+                        float sv;
+                        if (!check && stencil_base(g, src, a, mid, bs, fr, l0)) sv = sample_lds(s_box, l0, fr);
+                        else sv = sample_component(g, src, fflags, a, mid, pic[a], check);
+                        v2[a] = sv + t * (pic[a] - sv);
+                    }
+                }
+                float x[3];
+                for (int a = 0; a < 3; ++a) x[a] = pos[a] + dt * v2[a];
+                int c[3];
+                pos_cell(g, pos[0], pos[1], pos[2], c);
+                if (cell_index(g, c[0], c[1], c[2]) < kBrickCells) {
+                    for (int a = 0; a < 3; ++a) {
+                        nv[a] = p.v[a][i] + grav[a] * dt;
+                        x[a] = pos[a] + dt * nv[a];
+                    }
+                    for (int k = 0; k < 9; ++k) cm[k] = 0.0f;
+                }
+                for (uint32_t k = 0; k < ncol; ++k) {
+                    const slosh_collider cc = col[k];
+                    float dd = collider_sdf(cc, x);
+                    if (dd < radius) {
+                        float nn[3];
+                        collider_normal(cc, x, 0.25f * g.h, nn);
+                        float push = radius - dd;
+                        for (int a = 0; a < 3; ++a) x[a] = x[a] + nn[a] * push;
+                        float vs[3] = {collider_velocity_axis(cc, x, 0), collider_velocity_axis(cc, x, 1),
+                                       collider_velocity_axis(cc, x, 2)};
+                        float rel[3] = {nv[0] - vs[0], nv[1] - vs[1], nv[2] - vs[2]};
+                        float vn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
+                        if (vn < 0.0f) {
+                            for (int a = 0; a < 3; ++a) rel[a] = rel[a] - vn * nn[a];
+                            float rn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
+                            float tg[3];
+                            for (int a = 0; a < 3; ++a) tg[a] = rel[a] - rn * nn[a];
+                            for (int a = 0; a < 3; ++a) rel[a] = rel[a] - friction * tg[a];
+                        }
+                        for (int a = 0; a < 3; ++a) nv[a] = vs[a] + rel[a];
+                    }
+                }
+                uint32_t contact = 0;
+                for (int a = 0; a < 3; ++a) {
+                    if (x[a] < lo) {
+                        x[a] = lo;
+                        nv[a] = fmaxf(nv[a], 0.0f);
+                        contact |= 1u << a;
+                    } else if (x[a] > hi[a]) {
+                        x[a] = hi[a];
+                        nv[a] = fminf(nv[a], 0.0f);
+                        contact |= 1u << a;
+                    }
+                }
+                if (contact != 0u && friction > 0.0f)
+                    for (int a = 0; a < 3; ++a)
+                        if ((contact & (1u << a)) == 0u) nv[a] *= 1.0f - friction;
+                float speed = len3(nv[0], nv[1], nv[2]);
+                if (speed > max_speed) nclamp++;
+                p.pa[i] = make_float4(x[0], x[1], x[2], pa.w + dt);
+                for (int a = 0; a < 3; ++a) p.v[a][i] = nv[a];
+                for (int k = 0; k < 9; ++k) p.c[k][i] = cm[k];
+                vmax = max(vmax, __float_as_uint(speed));
+                pos_cell(g, x[0], x[1], x[2], c);
+                brick = brick_of_cell(g, c[0], c[1], c[2]);
+            }
+        }
+        tb[threadIdx.x] = brick;
+        __syncthreads();
+        if (brick != kNoBrick && (threadIdx.x == 0 || tb[threadIdx.x - 1] != brick)) touch[brick] = 1u;
+        __syncthreads();
+    }
+    for (int w = 32; w > 0; w >>= 1) {
+        vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, w));
+        nclamp += (uint32_t)__shfl_xor((int)nclamp, w);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicMax(&smax, vmax);
+        atomicAdd(&sclamp, nclamp);
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        bp_speed[blockIdx.x] = smax;
-        bp_clamp[blockIdx.x] = sclamp;
+        bp_speed[key] = smax;
+        bp_clamp[key] = sclamp;
     }
 }
 
@@ -638,11 +783,11 @@ void launch_g2p(slosh_ctx *c, float dt, const float g[3])
     if (!c->np) return;
     {
         SLOSH_TIMED(c, K_G2P);
-        hipLaunchKernelGGL(k_g2p, dim3(blocks_for(c->np, 256)), dim3(256), 0, c->stream, c->geo, c->parts[c->cur], c->np,
+        hipLaunchKernelGGL(k_g2p, dim3(c->geo.nbricks), dim3(256), 0, c->stream, c->geo, c->parts[c->cur], c->key_off,
                            c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction,
                            c->s.particle_radius_cells * c->geo.h, c->s.max_speed, c->d_col, c->ncol, c->touch,
                            c->bp_speed, c->bp_clamp);
-        c->n_g2p_blocks = blocks_for(c->np, 256);
+        c->n_g2p_blocks = c->geo.nbricks;
     }
     SLOSH_HIP(hipGetLastError());
 }

@@ -177,8 +177,8 @@ struct slosh_ctx {
     uint32_t *bp_div_max = nullptr; // [npool]   |div|/h max bits per brick
     uint32_t *bp_div_cnt = nullptr; // [npool]
     double *bp_div_sum = nullptr;   // [npool]
-    uint32_t *bp_speed = nullptr;   // [ntiles]  G2P: max speed bits per 256 particles
-    uint32_t *bp_clamp = nullptr;   // [ntiles]
+    uint32_t *bp_speed = nullptr;   // [nbricks] G2P: max speed bits per brick of the sorted particles
+    uint32_t *bp_clamp = nullptr;   // [nbricks]
     uint32_t n_p2g_blocks = 0, n_div_blocks = 0, n_g2p_blocks = 0;
     // colliders
     slosh_collider *d_col = nullptr;
