@@ -46,31 +46,57 @@ def algorithmic_bytes(kernel, np_, nc, nf, nb):
     """Bytes a kernel must move per launch: every array element it reads or writes, once (neighbour reuse
     perfect). np_ particles, nc allocated cells, nf fluid cells, nb allocated bricks."""
     table = {
-        # pos+age 16 B read; pos+age, vel, C written (64 B); face velocities + flags gathered once per cell
-        "k_g2p": 80 * np_ + 15 * nc,
+        # pos+age 16 B read; pos+age, vel, C written (64 B); staged face velocities once per cell
+        "k_g2p": 80 * np_ + 12 * nc,
         # 64-byte record read and written, 4 B in-tile rank read
         "k_sort_scatter": 132 * np_,
         # pos+age 16 B read, 4 B rank written
         "k_sort_tiles": 20 * np_,
-        # pos, vel, C read (64 B); brick tile of 6000 ints written; cell flags written; FP64 pressure read+written
+        # pos, vel, C read (64 B); brick tile of 6000 ints written; flags written; FP64 pressure read+written
         "k_p2g": 64 * np_ + 24000 * nb + 4 * nc + 16 * nc,
         # tiles read; flags read; velocity, weight (FP32 x3) and face flags (1 B x3) written
         "k_normalize": 24000 * nb + 4 * nc + 27 * nc,
-        # velocity + weight read, flags read, divergence + diagonal written
         "k_divergence": 24 * nc + 4 * nc + 8 * nc,
         "k_project": 12 * nc + 12 * nc + 4 * nc + 8 * nc + 3 * nc + 15 * nc,
         "k_extrapolate": 12 * nc + 3 * nc + 12 * nc + 3 * nc,
-        # level-0 MGPCG passes: FP32 vectors on fluid cells, flags on allocated cells
-        "k_apply_a": 12 * nf + 4 * nf + 4 * nc,
-        "k_update_pr": 32 * nf + 12 * nf + 4 * nc,
-        "k_smooth0": 12 * nf + 2 * nf + 4 * nc,
-        "k_update_d": 12 * nf + 4 * nc,
+        # level-0 MGPCG passes: vectors on fluid cells, the one-byte stencil mask on every allocated cell
+        # p (8+8), d, r_in, q, diag read; r_out, z written
+        "k_update_black": 40 * nf + nc,
+        # z1, d_in, diag read; d_out, q written
+        "k_direction": 20 * nf + nc,
+        # diag, z, r read; level-1 b and x written (one level-1 cell per 8 cells)
+        "k_restrict0": 12 * nf + 2 * nc,
+        # diag, r, z read, z1 written; level-1 stencil word and x1
+        "k_prolong_black": 16 * nf + nc + 6 * nc // 8,
+        # diag, r, z1 read; z1 written on red cells
+        "k_red_dot": 14 * nf + nc,
         "k_pcg_init": 8 * nf + 16 * nf + 12 * nc + 4 * nc,
-        "k_restrict0": 12 * nf + 4 * nc,
-        "k_prolong0": 8 * nf + nf // 8 * 4 + 4 * nc,
-        "k_sor_sweep": 16 * nf + 8 * nf // 2 + 4 * nc,
+        "k_sor_sweep": 16 * nf + 8 * nf // 2 + nc,
     }
     return table.get(kernel)
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 7 and not line.startswith("#") and f[0] == workload and f[1] == kernel:
+            return int(float(f[6]))
+    return None
+
+
+def copy_calibration(mib=512, reps=3):
+    """Device-to-device copies of a known size, so a PMC pass over this run can scale FETCH_SIZE/WRITE_SIZE."""
+    import torch
+
+    src = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def cpu_baseline(workload, seconds_budget=60.0):
@@ -103,6 +129,8 @@ def main():
     ap.add_argument("--solver", default="mgpcg", choices=["mgpcg", "rbsor"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-report", action="store_true", help="print per-kernel times to stderr")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run 512 MiB device copies first (scale reference for rocprofv3 --pmc FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver is a single-GPU library (one context per device); "
@@ -111,6 +139,8 @@ def main():
 
     from slosh_amd import Sim, abi
 
+    if args.copy_calibration:
+        copy_calibration()
     grid, h, reqs, dt, grav = build_scene(args.workload)
     n = sum(int(r.cell_dims[0]) * int(r.cell_dims[1]) * int(r.cell_dims[2]) * 8 for r in reqs)
     solver = abi.SOLVER_MGPCG if args.solver == "mgpcg" else abi.SOLVER_RBSOR
@@ -153,7 +183,8 @@ def main():
     nf = res.fluid_cells
     abytes = algorithmic_bytes(dominant, res.live_particles, nc, nf, nb)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": None, "traffic": None, "launches": launches, "avg_us": avg_s * 1e6}
+                "frac": None, "traffic": measured_traffic(args.workload, dominant), "launches": launches,
+                "avg_us": avg_s * 1e6}
     if abytes:
         ach = abytes / avg_s / 1e9
         roofline.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),

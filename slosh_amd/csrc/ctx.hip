@@ -20,9 +20,10 @@ namespace slosh {
 const char *const kKernelNames[K_COUNT] = {
     "k_emit_fill", "k_sort_tiles", "k_sort_runs_scatter", "k_sort_key_runs", "k_sort_scatter", "k_scan",
     "k_pool", "k_p2g", "k_normalize", "k_divergence", "k_project", "k_div_stats", "k_extrapolate", "k_g2p",
-    "k_mg_setup", "k_coarse_factor", "k_pcg_init", "k_red_dot", "k_dsmooth", "k_restrict0", "k_pdown",
-    "k_prolong_black", "k_pup", "k_coarse_solve", "k_direction", "k_update_black", "k_update_d", "k_sor_sweep",
-    "k_sor_residual", "k_fill", "k_finalize", "k_coarse_tail", "k_step_stats"};
+    "k_mg_setup", "k_coarse_factor", "k_pcg_init", "k_red_dot", "k_restrict0", "k_prolong_black", "k_direction",
+    "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d3down", "k_d3up",
+    "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
+    "k_fill", "k_finalize", "k_step_stats"};
 } // namespace slosh
 
 static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");

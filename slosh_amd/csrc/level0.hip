@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(const uint32_t *binfo, co
                                                        const float *diag, const double *pres, float *r,
                                                        double *partials)
 {
-    const Brick B = load_brick(binfo, blockIdx.x);
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     double v[2] = {0.0, 0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
         const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo
 {
     if (st->done) return;
     const double alpha = first ? 0.0 : st->alpha;
-    const Brick B = load_brick(binfo, blockIdx.x);
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     double v[1] = {0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
         const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void k_restrict0(const uint32_t *binfo, c
     if (st->done) return;
     __shared__ float res[kBrickCells];
     __shared__ uint8_t fl[kBrickCells];
-    const Brick B = load_brick(binfo, blockIdx.x);
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     for (uint32_t c = 0; c < 2u; ++c) {
         const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
         fl[l] = (uint8_t)((m & kSelfFluid) ? 1u : 0u);
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kThreads) void k_prolong_black(const uint32_t *binf
                                                             float *z1, PLevel L1, const SolverState *st)
 {
     if (st->done) return;
-    const Brick B = load_brick(binfo, blockIdx.x);
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     for (uint32_t c = 0; c < 2u; ++c) {
         const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
         if (!(m & kSelfFluid)) continue;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kThreads) void k_red_dot(const uint32_t *binfo, con
                                                       const SolverState *st)
 {
     if (st->done) return;
-    const Brick B = load_brick(binfo, blockIdx.x);
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     double v[1] = {0.0};
     {
         const uint32_t l = colour_cell(threadIdx.x, 0u), i = B.base + l, m = fmask[i];
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, c
 {
     if (st->done) return;
     const double beta = first ? 0.0 : st->beta;
-    const Brick B = load_brick(binfo, blockIdx.x);
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     double v[1] = {0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
         const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kThreads) void k_sor_sweep(const uint32_t *binfo, c
                                                         const SolverState *st)
 {
     if (st->done) return;
-    const Brick B = load_brick(binfo, blockIdx.x);
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     const uint32_t l = colour_cell(threadIdx.x, colour), i = B.base + l, m = fmask[i];
     if (!(m & kSelfFluid)) return;
     double s = nbsum<double>(B, l, m, pres);
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kThreads) void k_sor_residual(const uint32_t *binfo
                                                            double *partials, const SolverState *st)
 {
     if (st->done) return;
-    const Brick B = load_brick(binfo, blockIdx.x);
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     double v[2] = {0.0, 0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
         const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kThreads) void k_sor_residual(const uint32_t *binfo
 
 void l0_brick_info(slosh_ctx *c)
 {
-    SLOSH_TIMED(c, K_MG_TYPES);
+    SLOSH_TIMED(c, K_MG_SETUP);
     hipLaunchKernelGGL(k_brick_info, dim3((c->nactive + 255u) / 256u), dim3(256), 0, c->stream, c->geo, c->nactive,
                        c->binfo);
 }
@@ -370,7 +370,7 @@ void l0_pcg_init(slosh_ctx *c, float *r_out)
 
 void l0_update_black(slosh_ctx *c, const float *r_in, float *r_out, const float *d, int first)
 {
-    L0_LAUNCH(K_UPDATE_PR, k_update_black, c->diag, c->pres, r_in, r_out, c->z, d, c->q, first, c->partials, c->state);
+    L0_LAUNCH(K_UPDATE_BLACK, k_update_black, c->diag, c->pres, r_in, r_out, c->z, d, c->q, first, c->partials, c->state);
 }
 
 void l0_restrict(slosh_ctx *c, const float *r)
@@ -380,14 +380,14 @@ void l0_restrict(slosh_ctx *c, const float *r)
 
 void l0_prolong_black(slosh_ctx *c, const float *r)
 {
-    L0_LAUNCH(K_PROLONG0, k_prolong_black, c->diag, r, c->z, c->z1, plevel_of(c, 1), c->state);
+    L0_LAUNCH(K_PROLONG_BLACK, k_prolong_black, c->diag, r, c->z, c->z1, plevel_of(c, 1), c->state);
 }
 
-void l0_red_dot(slosh_ctx *c, const float *r) { L0_LAUNCH(K_SMOOTH0, k_red_dot, c->diag, r, c->z1, c->partials, c->state); }
+void l0_red_dot(slosh_ctx *c, const float *r) { L0_LAUNCH(K_RED_DOT, k_red_dot, c->diag, r, c->z1, c->partials, c->state); }
 
 void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, int first)
 {
-    L0_LAUNCH(K_APPLY_A, k_direction, c->diag, c->z1, d_in, d_out, c->q, first, c->partials, c->state);
+    L0_LAUNCH(K_DIRECTION, k_direction, c->diag, c->z1, d_in, d_out, c->q, first, c->partials, c->state);
 }
 
 void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w)

@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_pdown_lds(const uint32_t *binfo, PLevel
     __shared__ uint16_t s_info[Bx::NB];
     __shared__ float s_res[Bx::CPB];
     const uint32_t t = threadIdx.x;
-    if (t < 28u) s_nb[t] = binfo[(size_t)blockIdx.x * kBinfoWords + t];
+    if (t < 28u) s_nb[t] = binfo[(size_t)xcd_remap(blockIdx.x, gridDim.x) * kBinfoWords + t];
     __syncthreads();
     for (uint32_t k = t; k < (uint32_t)Bx::NB; k += 256u) {
         const int u = (int)(k % Bx::BD) - 2, v = (int)((k / Bx::BD) % Bx::BD) - 2, w = (int)(k / (Bx::BD * Bx::BD)) - 2;
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(256) void k_pup_lds(const uint32_t *binfo, PLevel L
     __shared__ float s_v[Bx::NB], s_b[Bx::NB];
     __shared__ uint16_t s_info[Bx::NB];
     const uint32_t t = threadIdx.x;
-    if (t < 28u) s_nb[t] = binfo[(size_t)blockIdx.x * kBinfoWords + t];
+    if (t < 28u) s_nb[t] = binfo[(size_t)xcd_remap(blockIdx.x, gridDim.x) * kBinfoWords + t];
     __syncthreads();
     for (uint32_t k = t; k < (uint32_t)Bx::NB; k += 256u) {
         const int u = (int)(k % Bx::BD) - 2, v = (int)((k / Bx::BD) % Bx::BD) - 2, w = (int)(k / (Bx::BD * Bx::BD)) - 2;
@@ -327,7 +327,7 @@ DenseLevel dlevel_of(const slosh_ctx *c, int l)
 void pooled_build_type(slosh_ctx *c, int l)
 {
     PLevel L = plevel_of(c, l);
-    SLOSH_TIMED(c, K_MG_TYPES);
+    SLOSH_TIMED(c, K_MG_SETUP);
     if (l == 1)
         hipLaunchKernelGGL(k_ptype1, dim3(pass_blocks(c, L)), dim3(256), 0, c->stream, c->geo, c->nactive, c->ctype, L);
     else
@@ -338,14 +338,14 @@ void pooled_build_type(slosh_ctx *c, int l)
 void pooled_build_info(slosh_ctx *c, int l)
 {
     PLevel L = plevel_of(c, l);
-    SLOSH_TIMED(c, K_MG_TYPES);
+    SLOSH_TIMED(c, K_MG_SETUP);
     hipLaunchKernelGGL(k_pinfo, dim3(pass_blocks(c, L)), dim3(256), 0, c->stream, c->geo, c->nactive, L);
 }
 
 void dense_build_type_from_pooled(slosh_ctx *c, int l)
 {
     DenseLevel C = dlevel_of(c, l);
-    SLOSH_TIMED(c, K_MG_TYPES);
+    SLOSH_TIMED(c, K_MG_SETUP);
     hipLaunchKernelGGL(k_dtype_from_pooled, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, c->geo,
                        plevel_of(c, l - 1), C);
 }
@@ -355,7 +355,7 @@ void pooled_down(slosh_ctx *c, int l)
     PLevel L = plevel_of(c, l);
     PLevel C = l == 1 ? plevel_of(c, 2) : PLevel{};
     DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
-    SLOSH_TIMED(c, K_RESTRICT);
+    KTimer timer(c, l == 1 ? K_PDOWN1 : K_PDOWN2);
     if (l == 1)
         hipLaunchKernelGGL(k_pdown_lds<4>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
     else
@@ -367,7 +367,7 @@ void pooled_up(slosh_ctx *c, int l)
     PLevel L = plevel_of(c, l);
     PLevel C = l == 1 ? plevel_of(c, 2) : PLevel{};
     DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
-    SLOSH_TIMED(c, K_PROLONG);
+    KTimer timer(c, l == 1 ? K_PUP1 : K_PUP2);
     if (l == 1)
         hipLaunchKernelGGL(k_pup_lds<4>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
     else

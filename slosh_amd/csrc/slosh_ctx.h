@@ -49,6 +49,15 @@ struct Level {
 // 3(dy+1) + 9(dz+1), 0 = none), then the brick index.
 constexpr uint32_t kBinfoWords = 32u;
 
+// Brick handled by workgroup w of a one-brick-per-workgroup pass over n bricks. Workgroups are dispatched
+// round-robin over the MI355X's 8 XCDs, each with its own L2; giving XCD k a contiguous range of the ascending
+// brick list keeps most of a brick's neighbours (and their halo reads) in the same L2. A bijection on [0, n).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t w, uint32_t n)
+{
+    const uint32_t q = n >> 3, r = n & 7u, k = w & 7u, j = w >> 3;
+    return k * q + (k < r ? k : r) + j;
+}
+
 // Stencil word of a coarse cell: bits 0-5 fluid neighbours (-x +x -y +y -z +z), bit 6 the cell is fluid, bit 7 red
 // ((x + y + z) even), bits 8-11 the number of non-solid neighbours (the coarse diagonal without the 1e-4).
 constexpr uint32_t kInfoSelf = 1u << 6;
@@ -98,9 +107,10 @@ __device__ __forceinline__ uint32_t pindex(const Geo &g, const PLevel &L, int x,
 // Kernels that can be timed with HIP events on the context's stream (slosh_set_kernel_timing).
 enum KernelId {
     K_EMIT = 0, K_SORT_TILES, K_SORT_RUNS, K_SORT_KEY_RUNS, K_SORT_SCATTER, K_SCAN, K_POOL, K_P2G, K_NORMALIZE,
-    K_DIVERGENCE, K_PROJECT, K_DIV_STATS, K_EXTRAPOLATE, K_G2P, K_MG_TYPES, K_COARSE_FACTOR, K_PCG_INIT, K_SMOOTH0,
-    K_SMOOTH, K_RESTRICT0, K_RESTRICT, K_PROLONG0, K_PROLONG, K_COARSE_SOLVE, K_APPLY_A, K_UPDATE_PR, K_UPDATE_D,
-    K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL, K_FINALIZE, K_COARSE_TAIL, K_STEP_STATS, K_COUNT
+    K_DIVERGENCE, K_PROJECT, K_DIV_STATS, K_EXTRAPOLATE, K_G2P, K_MG_SETUP, K_COARSE_FACTOR, K_PCG_INIT, K_RED_DOT,
+    K_RESTRICT0, K_PROLONG_BLACK, K_DIRECTION, K_UPDATE_BLACK, K_PDOWN1, K_PDOWN2, K_PUP1, K_PUP2, K_D3DOWN, K_D3UP,
+    K_DSMOOTH, K_DRESTRICT, K_DPROLONG, K_COARSE_SOLVE, K_COARSE_TAIL, K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL,
+    K_FINALIZE, K_STEP_STATS, K_COUNT
 };
 // Names as the compiled kernels are called (what a kernel trace of the library shows).
 extern const char *const kKernelNames[K_COUNT];

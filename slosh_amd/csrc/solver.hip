@@ -373,22 +373,77 @@ __global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const float *r
 {
     if (st->done) return;
     constexpr int N = (int)kCoarsestCells;
+    constexpr int kPer = (int)((kTailCells + 1023u) / 1024u); // cells per thread of any tail pass
     __shared__ uint16_t s_info[kTailCells];
     __shared__ uint32_t s_coord[kTailCells];
     __shared__ float s_x[kTailCells], s_b[kTailCells];
     __shared__ double bs[N];
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const int m = T.n;
-    for (int k = 0; k < m; ++k)
-        for (uint32_t i = tid; i < T.L[k].cells; i += nt) {
-            s_info[T.off[k] + i] = T.L[k].info[i];
-            s_coord[T.off[k] + i] = T.L[k].coord[i];
+    // The global reads are issued up front and all at once (one memory latency each, not one per loop trip): the
+    // coarsest level's inverse rows and cell list, then the levels' stencil words and coordinates.
+    const int row = (int)(tid >> 4), part = (int)(tid & 15u);
+    const int n = (int)*cn;
+    double invr[N / 16];
+    for (int k = 0; k < N / 16; ++k) invr[k] = inv[(size_t)row * N + part + 16 * k];
+    const uint32_t c_row = cidx[row], c_tid = cidx[tid & (uint32_t)(N - 1)];
+    const uint32_t total = T.off[m - 1] + T.L[m - 1].cells;
+    {
+        uint16_t vi[kPer];
+        uint32_t vc[kPer];
+        for (int u = 0; u < kPer; ++u) {
+            const uint32_t f = tid + nt * (uint32_t)u;
+            vi[u] = 0;
+            vc[u] = 0;
+            if (f < total) {
+                int k = 0;
+                while (k + 1 < m && f >= T.off[k + 1]) ++k;
+                vi[u] = T.L[k].info[f - T.off[k]];
+                vc[u] = T.L[k].coord[f - T.off[k]];
+            }
         }
+        for (int u = 0; u < kPer; ++u) {
+            const uint32_t f = tid + nt * (uint32_t)u;
+            if (f < total) {
+                s_info[f] = vi[u];
+                s_coord[f] = vc[u];
+            }
+        }
+    }
     __syncthreads();
     {
         const DenseLevel V0 = tview(T, 0, s_info, s_coord, s_x, s_b);
         if (T.from_res) {
-            for (uint32_t i = tid; i < V0.cells; i += nt) restrict_res_cell(T.fdim, res3, V0, i);
+            // restriction from level 3's residuals (see restrict_res_cell): the 8 children of each of this
+            // thread's cells are read first, then summed in child order
+            float rv[kPer][8];
+            for (int u = 0; u < kPer; ++u) {
+                const uint32_t ci = tid + nt * (uint32_t)u;
+                for (int oi = 0; oi < 8; ++oi) rv[u][oi] = 0.0f;
+                if (ci >= V0.cells || !(V0.info[ci] & kInfoSelf)) continue;
+                int cx, cy, cz;
+                dcoord(V0, ci, cx, cy, cz);
+                for (int oi = 0; oi < 8; ++oi) {
+                    const int fx = 2 * cx + (oi & 1), fy = 2 * cy + ((oi >> 1) & 1), fz = 2 * cz + (oi >> 2);
+                    if (fx < T.fdim[0] && fy < T.fdim[1] && fz < T.fdim[2])
+                        rv[u][oi] = res3[fx + T.fdim[0] * (fy + T.fdim[1] * fz)];
+                }
+            }
+            for (int u = 0; u < kPer; ++u) {
+                const uint32_t ci = tid + nt * (uint32_t)u;
+                if (ci >= V0.cells) continue;
+                const uint32_t cinfo = V0.info[ci];
+                if (!(cinfo & kInfoSelf)) {
+                    V0.b[ci] = 0.0f;
+                    V0.x[ci] = 0.0f;
+                    continue;
+                }
+                float sum = 0.0f;
+                for (int oi = 0; oi < 8; ++oi) sum += rv[u][oi];
+                const float bv = 0.25f * sum;
+                V0.b[ci] = bv;
+                V0.x[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
+            }
         } else {
             for (uint32_t i = tid; i < V0.cells; i += nt) {
                 V0.x[i] = T.L[0].x[i];
@@ -413,15 +468,13 @@ __global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const float *r
     {
         // x = A^-1 b on the coarsest level: 16 lanes per row, partial sums combined by a fixed butterfly
         const DenseLevel Vc = tview(T, m - 1, s_info, s_coord, s_x, s_b);
-        const int n = (int)*cn;
-        if ((int)tid < n) bs[tid] = (double)Vc.b[cidx[tid]];
+        if ((int)tid < n) bs[tid] = (double)Vc.b[c_tid];
         __syncthreads();
-        const int row = (int)(tid >> 4), part = (int)(tid & 15u);
         double s = 0.0;
-        if (row < n)
-            for (int j = part; j < n; j += 16) s += inv[(size_t)row * N + j] * bs[j];
+        for (int k = 0; k < N / 16; ++k)
+            if (row < n && part + 16 * k < n) s += invr[k] * bs[part + 16 * k];
         for (int w = 8; w > 0; w >>= 1) s += __shfl_xor(s, w, 16);
-        if (row < n && part == 0) Vc.x[cidx[row]] = (float)s;
+        if (row < n && part == 0) Vc.x[c_row] = (float)s;
         __syncthreads();
     }
     for (int l = m - 2; l >= 0; --l) {
@@ -584,7 +637,7 @@ void dense_part(slosh_ctx *c)
     const int m = c->nlevels;
     const DenseLevel L3 = dlevel_of(c, 3);
     {
-        SLOSH_TIMED(c, K_RESTRICT);
+        SLOSH_TIMED(c, K_D3DOWN);
         hipLaunchKernelGGL(k_d3down, dim3(blocks_for(L3.cells, 256)), dim3(256), 0, c->stream, L3, c->res3, c->state);
     }
     int t = m - 1;
@@ -592,17 +645,17 @@ void dense_part(slosh_ctx *c)
     while (t - 1 >= kFirstDense && tail_cells + c->lv[t - 1].cells <= kTailCells) tail_cells += c->lv[--t].cells;
     if (t > kFirstDense) {
         DenseLevel C = dlevel_of(c, kFirstDense);
-        SLOSH_TIMED(c, K_RESTRICT);
+        SLOSH_TIMED(c, K_DRESTRICT);
         hipLaunchKernelGGL(k_drestrict_res, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, L3, c->res3, C,
                            c->state);
     }
     for (int l = kFirstDense; l < t; ++l) {
         DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
         {
-            SLOSH_TIMED(c, K_SMOOTH);
+            SLOSH_TIMED(c, K_DSMOOTH);
             hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, 1u, c->state);
         }
-        SLOSH_TIMED(c, K_RESTRICT);
+        SLOSH_TIMED(c, K_DRESTRICT);
         hipLaunchKernelGGL(k_drestrict, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
     }
     TailArgs T{};
@@ -623,16 +676,16 @@ void dense_part(slosh_ctx *c)
     for (int l = t - 1; l >= kFirstDense; --l) {
         DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
         {
-            SLOSH_TIMED(c, K_PROLONG);
+            SLOSH_TIMED(c, K_DPROLONG);
             hipLaunchKernelGGL(k_dprolong, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
         }
         for (uint32_t colour : {1u, 0u}) {
-            SLOSH_TIMED(c, K_SMOOTH);
+            SLOSH_TIMED(c, K_DSMOOTH);
             hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, colour,
                                c->state);
         }
     }
-    SLOSH_TIMED(c, K_PROLONG);
+    SLOSH_TIMED(c, K_D3UP);
     hipLaunchKernelGGL(k_d3up, dim3(blocks_for(L3.cells, 256)), dim3(256), 0, c->stream, L3, dlevel_of(c, kFirstDense),
                        c->state);
 }
@@ -669,7 +722,7 @@ void mg_setup(slosh_ctx *c)
             dense_build_type_from_pooled(c, l);
         } else {
             DenseLevel C = dlevel_of(c, l);
-            SLOSH_TIMED(c, K_MG_TYPES);
+            SLOSH_TIMED(c, K_MG_SETUP);
             hipLaunchKernelGGL(k_build_type, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, dlevel_of(c, l - 1),
                                C);
         }
@@ -679,7 +732,7 @@ void mg_setup(slosh_ctx *c)
             pooled_build_info(c, l);
         } else {
             DenseLevel D = dlevel_of(c, l);
-            SLOSH_TIMED(c, K_MG_TYPES);
+            SLOSH_TIMED(c, K_MG_SETUP);
             hipLaunchKernelGGL(k_dinfo, dim3(blocks_for(D.cells, 256)), dim3(256), 0, c->stream, D);
         }
     }
