@@ -45,7 +45,9 @@ typedef enum slosh_status {
 enum {
     SLOSH_SOLVER_MGPCG = 0, /* multigrid-preconditioned CG, FP64 dots, to ||r||/||b|| <= tolerance */
     SLOSH_SOLVER_RBSOR = 1, /* red-black SOR with sor_weight, to tolerance or max_iterations */
-    SLOSH_SOLVER_NONE = 2   /* keep the (warm-started) pressure as is; used by parity tests of the other kernels */
+    SLOSH_SOLVER_NONE = 2,  /* keep the (warm-started) pressure as is; used by parity tests of the other kernels */
+    SLOSH_SOLVER_FCG_UNET = 3 /* flexible CG preconditioned by the learned U-Net (slosh_unet_*), with MGPCG as the
+                                 fallback when it stalls and as the judge of its result; grid dims multiples of 8 */
 };
 
 /* Solver settings: FSloshSolverSettings (SloshTypes.h:113-160), restricted to what the per-step path reads. */
@@ -70,6 +72,7 @@ typedef struct slosh_settings {
 } slosh_settings;
 
 #define SLOSH_FLAG_NONE 0u
+#define SLOSH_FLAG_CAPTURE_RESIDUALS 1u /* keep MGPCG residuals of each solve as training data for slosh_unet_train */
 
 /* Emission request: FSloshEmitRequest (SloshTypes.h:169-181). shape 0 = FillCells (8 jittered particles per cell). */
 enum { SLOSH_EMIT_FILL_CELLS = 0, SLOSH_EMIT_SPHERE = 1, SLOSH_EMIT_BOX = 2 };
@@ -150,7 +153,7 @@ typedef struct slosh_results {
     double rhs_norm;               /* ||b||_2 */
     float step_ms;                 /* device time of the step (HIP events) */
     float solve_ms;                /* device time of the pressure solve */
-    int32_t fallback_used;         /* reserved for the flexible-CG preconditioner fallback; 0 */
+    int32_t fallback_used;         /* flexible CG: 1 when MGPCG finished the solve, 2 when no network was set */
     int32_t reserved[5];
 } slosh_results;
 
@@ -223,6 +226,37 @@ slosh_status slosh_set_kernel_timing(slosh_ctx *ctx, uint64_t mask);
 /* Writes up to cap entries (one per kernel, fixed order) and sets *n to the kernel count; out == NULL resets
  * the totals. */
 slosh_status slosh_kernel_times(slosh_ctx *ctx, slosh_kernel_time *out, uint32_t cap, uint32_t *n);
+
+/* ---- learned preconditioner (no reference counterpart: the reference preconditions CG with its multigrid
+ *      V-cycle only; this is the library's own component) --------------------------------------------------------
+ * A 3-level 3-D U-Net (16/32/64 channels, 3x3x3 convolutions, BF16 weights on MFMA, FP32 accumulate) maps the
+ * residual and the cell types to the preconditioned residual. Parameters: FP32, per layer W[27][cout][cin] then
+ * bias[cout], layers E0a 4->16, E0b 16->16, E1a 16->32, E1b 32->32, B2a 32->64, B2b 64->64, D1p 64->32,
+ * D1b 32->32, D0p 32->16, D0b 16->16, head 1x1 16->1. */
+uint64_t slosh_unet_param_count(void);
+slosh_status slosh_unet_set_weights(slosh_ctx *ctx, const float *w, uint64_t n);
+slosh_status slosh_unet_get_weights(slosh_ctx *ctx, float *w, uint64_t n);
+/* Applies the network to a residual given in dense order (x fastest) on the system of the last step; writes the
+ * preconditioned residual (0 off fluid cells). Diagnostic and test entry point. */
+slosh_status slosh_unet_apply(slosh_ctx *ctx, const float *r_dense, float *z_dense);
+/* Training loss sum_fluid (A zhat - r/s)^2 / sum_fluid (r/s)^2 for the residual r (dense order) on the system of
+ * the last step, and its gradient w.r.t. the parameters (n = slosh_unet_param_count floats). */
+slosh_status slosh_unet_loss_grad(slosh_ctx *ctx, const float *r_dense, double *loss, float *grad);
+typedef struct slosh_unet_train_config {
+    uint32_t iterations;  /* Adam steps */
+    uint32_t batch;       /* captured residuals per step */
+    float learning_rate;
+    uint32_t offset;      /* first captured residual used */
+} slosh_unet_train_config;
+typedef struct slosh_unet_train_report {
+    double first_loss;    /* mean loss of the first step's batch */
+    double last_loss;     /* mean loss of the last step's batch */
+    uint32_t systems;     /* captured residuals available */
+    uint32_t adam_steps;  /* total Adam steps since the weights were set */
+} slosh_unet_train_report;
+/* Trains on the residuals captured by the last solve (settings flag SLOSH_FLAG_CAPTURE_RESIDUALS) of the current
+ * system, with the library's forward/backward HIP kernels and Adam. */
+slosh_status slosh_unet_train(slosh_ctx *ctx, const slosh_unet_train_config *cfg, slosh_unet_train_report *out);
 
 /* Library identification: ABI version and the offload target the kernels were built for. */
 uint32_t slosh_abi_version(void);

@@ -69,6 +69,18 @@ def load_library(path=LIB_PATH):
     lib.slosh_set_kernel_timing.restype = st
     lib.slosh_kernel_times.argtypes = [vp, C.POINTER(abi.KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]
     lib.slosh_kernel_times.restype = st
+    lib.slosh_unet_param_count.argtypes = []
+    lib.slosh_unet_param_count.restype = C.c_uint64
+    lib.slosh_unet_set_weights.argtypes = [vp, vp, C.c_uint64]
+    lib.slosh_unet_set_weights.restype = st
+    lib.slosh_unet_get_weights.argtypes = [vp, vp, C.c_uint64]
+    lib.slosh_unet_get_weights.restype = st
+    lib.slosh_unet_apply.argtypes = [vp, vp, vp]
+    lib.slosh_unet_apply.restype = st
+    lib.slosh_unet_loss_grad.argtypes = [vp, vp, C.POINTER(C.c_double), vp]
+    lib.slosh_unet_loss_grad.restype = st
+    lib.slosh_unet_train.argtypes = [vp, C.POINTER(abi.UNetTrainConfig), C.POINTER(abi.UNetTrainReport)]
+    lib.slosh_unet_train.restype = st
     _lib = lib
     return lib
 
@@ -178,6 +190,40 @@ class Sim:
 
     def kernel_index(self, name):
         return list(self.kernel_times().keys()).index(name)
+
+    # ---- learned preconditioner ----
+    def unet_param_count(self):
+        return int(self.lib.slosh_unet_param_count())
+
+    def unet_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(self.lib.slosh_unet_set_weights(self.h, w.ctypes.data, w.size), "slosh_unet_set_weights")
+
+    def unet_get_weights(self):
+        w = np.zeros(self.unet_param_count(), np.float32)
+        _check(self.lib.slosh_unet_get_weights(self.h, w.ctypes.data, w.size), "slosh_unet_get_weights")
+        return w
+
+    def unet_apply(self, r_dense):
+        """Preconditioned residual z = M(r) for a dense residual (z, y, x) on the system of the last step."""
+        r = np.ascontiguousarray(r_dense, dtype=np.float32)
+        z = np.zeros_like(r)
+        _check(self.lib.slosh_unet_apply(self.h, r.ctypes.data, z.ctypes.data), "slosh_unet_apply")
+        return z
+
+    def unet_loss_grad(self, r_dense):
+        r = np.ascontiguousarray(r_dense, dtype=np.float32)
+        g = np.zeros(self.unet_param_count(), np.float32)
+        loss = C.c_double(0.0)
+        _check(self.lib.slosh_unet_loss_grad(self.h, r.ctypes.data, C.byref(loss), g.ctypes.data), "slosh_unet_loss_grad")
+        return loss.value, g
+
+    def unet_train(self, iterations, batch=4, learning_rate=1e-3, offset=0):
+        cfg = abi.UNetTrainConfig(int(iterations), int(batch), float(learning_rate), int(offset))
+        rep = abi.UNetTrainReport()
+        _check(self.lib.slosh_unet_train(self.h, C.byref(cfg), C.byref(rep)), "slosh_unet_train")
+        return {"first_loss": rep.first_loss, "last_loss": rep.last_loss, "systems": rep.systems,
+                "adam_steps": rep.adam_steps}
 
     def device_particles(self, layout=abi.LAYOUT_F32_64):
         ptr = C.c_void_p()

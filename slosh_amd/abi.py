@@ -19,6 +19,9 @@ STATUS_NAMES = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "ECAPACITY",
 SOLVER_MGPCG = 0
 SOLVER_RBSOR = 1
 SOLVER_NONE = 2
+SOLVER_FCG_UNET = 3
+
+FLAG_CAPTURE_RESIDUALS = 1
 
 EMIT_FILL_CELLS = 0
 EMIT_SPHERE = 1
@@ -150,6 +153,15 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint32), ("pad", C.c_uint32), ("total_ms", C.c_double)]
 
 
+class UNetTrainConfig(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("batch", C.c_uint32), ("learning_rate", C.c_float), ("offset", C.c_uint32)]
+
+
+class UNetTrainReport(C.Structure):
+    _fields_ = [("first_loss", C.c_double), ("last_loss", C.c_double), ("systems", C.c_uint32),
+                ("adam_steps", C.c_uint32)]
+
+
 PARTICLE_DTYPE = np.dtype([("pos", np.float32, 3), ("age", np.float32), ("vel", np.float32, 3), ("c", np.float32, 9)])
 PARTICLE_REF48_DTYPE = np.dtype([("pos", np.float32, 3), ("age", np.float32), ("vel", np.float32, 3), ("c", np.uint32, 5)])
 
@@ -161,7 +173,7 @@ assert PARTICLE_DTYPE.itemsize == 64 and PARTICLE_REF48_DTYPE.itemsize == 48
 
 
 def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6, max_iterations=200, sor_weight=1.8,
-                  budget=1.0, extrapolation_layers=2, ghost_fluid=1, friction=0.0, max_speed=20.0):
+                  budget=1.0, extrapolation_layers=2, ghost_fluid=1, friction=0.0, max_speed=20.0, flags=0):
     """Parity-profile settings (APIC, ghost fluid, no density correction) with the given grid and solver."""
     s = Settings()
     for a in range(3):
@@ -180,7 +192,7 @@ def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6
     s.particle_radius_cells = 0.25
     s.density_correction = 0.0
     s.density_deadband = 0.05
-    s.flags = 0
+    s.flags = int(flags)
     return s
 
 

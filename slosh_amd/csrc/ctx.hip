@@ -23,7 +23,7 @@ const char *const kKernelNames[K_COUNT] = {
     "k_mg_setup", "k_coarse_factor", "k_pcg_init", "k_red_dot", "k_restrict0", "k_prolong_black", "k_direction",
     "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d3down", "k_d3up",
     "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
-    "k_fill", "k_finalize", "k_step_stats"};
+    "k_fill", "k_finalize", "k_step_stats", "k_unet_conv", "k_unet_io", "k_unet_train", "k_fcg"};
 } // namespace slosh
 
 static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");
@@ -65,7 +65,8 @@ void free_all(slosh_ctx *c)
     for (int k = 0; k < 3; ++k) dfree(c->vel[k]);
     dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag); dfree(c->fmask);
     dfree(c->r); dfree(c->r2); dfree(c->z); dfree(c->z1); dfree(c->d); dfree(c->d2); dfree(c->q);
-    dfree(c->binfo); dfree(c->res3);
+    dfree(c->binfo); dfree(c->res3); dfree(c->fcg_z); dfree(c->capture);
+    unet_free(c);
     for (int l = 1; l < kMaxLevels; ++l) {
         dfree(c->lv[l].type); dfree(c->lv[l].info); dfree(c->lv[l].coord);
         dfree(c->lv[l].x); dfree(c->lv[l].x1); dfree(c->lv[l].b);
@@ -99,7 +100,11 @@ bool settings_valid(const slosh_settings *s)
         if (s->grid_dim[a] < 4 || s->grid_dim[a] > 4096) return false;
     if (!(s->cell_size > 0.0f) || !(s->sparse_storage_budget > 0.0f) || s->sparse_storage_budget > 1.0f) return false;
     if (s->particle_capacity == 0u) return false;
-    if (s->pressure_solver < SLOSH_SOLVER_MGPCG || s->pressure_solver > SLOSH_SOLVER_NONE) return false;
+    if (s->pressure_solver < SLOSH_SOLVER_MGPCG || s->pressure_solver > SLOSH_SOLVER_FCG_UNET) return false;
+    if (s->pressure_solver == SLOSH_SOLVER_FCG_UNET)
+        for (int a = 0; a < 3; ++a)
+            if (s->grid_dim[a] % 8 != 0) return false; // the network's levels need whole bricks
+    if (s->flags & ~(uint32_t)SLOSH_FLAG_CAPTURE_RESIDUALS) return false;
     if (!(s->tolerance > 0.0f) || s->max_iterations < 0) return false;
     if (!(s->sor_weight >= 0.5f && s->sor_weight <= 1.95f)) return false;
     if (s->extrapolation_layers < 0 || s->extrapolation_layers > 4) return false;
@@ -194,6 +199,11 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->d2, pc);
     dalloc(&c->q, pc);
     dalloc(&c->binfo, (size_t)g.npool * kBinfoWords);
+    if (s.pressure_solver == SLOSH_SOLVER_FCG_UNET) dalloc(&c->fcg_z, pc);
+    if (s.flags & SLOSH_FLAG_CAPTURE_RESIDUALS) {
+        c->capture_cap = 12;
+        dalloc(&c->capture, (size_t)c->capture_cap * pc);
+    }
     dalloc(&c->res3, g.nbricks);
     // MG hierarchy: halve (rounding up) until a level holds <= 64 cells or 8 levels exist. Levels 1..3 live in the
     // brick pool ((8 >> l)^3 cells per slot), coarser ones are dense. dalloc zeroes them: the null slot of a
@@ -325,15 +335,22 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     if (stop > SLOSH_STAGE_NORMALIZE) launch_divergence(c, dt);
     if (stop > SLOSH_STAGE_DIVERGENCE) {
         SLOSH_HIP(hipEventRecord(c->ev_solve0, c->stream));
+        c->fcg_iters = 0;
+        c->fcg_fallback = 0;
         if (c->s.pressure_solver == SLOSH_SOLVER_MGPCG) run_mgpcg(c);
         else if (c->s.pressure_solver == SLOSH_SOLVER_RBSOR) run_rbsor(c);
+        else if (c->s.pressure_solver == SLOSH_SOLVER_FCG_UNET) run_fcg(c);
         SLOSH_HIP(hipEventRecord(c->ev_solve1, c->stream));
+        c->res.fallback_used = c->fcg_fallback;
         if (c->s.pressure_solver != SLOSH_SOLVER_NONE) {
             SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
             SLOSH_HIP(hipStreamSynchronize(c->stream));
             harvest_timing(c);
             if (c->nactive) {
-                c->res.solver_iterations = c->h_state->iters;
+                // flexible CG: its own iterations, plus those of the MGPCG fallback when it ran
+                const bool fcg = c->s.pressure_solver == SLOSH_SOLVER_FCG_UNET && c->fcg_fallback != 2;
+                c->res.solver_iterations = fcg ? (c->fcg_fallback ? c->fcg_iters + c->h_state->iters : c->fcg_iters)
+                                               : c->h_state->iters;
                 c->res.residual_rel = c->h_state->rel;
                 c->res.rhs_norm = sqrt(c->h_state->bb);
                 c->res.converged = c->h_state->converged;
@@ -496,9 +513,9 @@ thread_local char g_info[256];
 
 } // namespace
 
-#define ABI_GUARD(body)                                                                                          \
+#define ABI_GUARD(...)                                                                                           \
     try {                                                                                                        \
-        body                                                                                                     \
+        __VA_ARGS__                                                                                              \
     } catch (const slosh::HipError &e) {                                                                        \
         fprintf(stderr, "slosh: HIP error %d (%s) at %s:%d: %s\n", (int)e.err, hipGetErrorString(e.err), e.file, \
                 e.line, e.what);                                                                                 \
@@ -743,6 +760,143 @@ slosh_status slosh_kernel_times(slosh_ctx *c, slosh_kernel_time *out, uint32_t c
         out[k].total_ms = c->ktime_ms[k];
     }
     return SLOSH_OK;
+}
+
+uint64_t slosh_unet_param_count(void) { return (uint64_t)slosh::unet_param_count(); }
+
+slosh_status slosh_unet_set_weights(slosh_ctx *c, const float *w, uint64_t n)
+{
+    if (!c || !w || n != (uint64_t)unet_param_count()) return SLOSH_EINVAL;
+    ABI_GUARD({
+        unet_init(c);
+        SLOSH_HIP(hipMemcpyAsync(c->unet.w32, w, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        unet_upload_bf16(c);
+        if (c->unet.m) {
+            SLOSH_HIP(hipMemsetAsync(c->unet.m, 0, n * sizeof(float), c->stream));
+            SLOSH_HIP(hipMemsetAsync(c->unet.v, 0, n * sizeof(float), c->stream));
+            SLOSH_HIP(hipMemsetAsync(c->unet.grad, 0, n * sizeof(float), c->stream));
+        }
+        c->unet.adam_t = 0;
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        c->unet.loaded = true;
+        return SLOSH_OK;
+    })
+}
+
+slosh_status slosh_unet_get_weights(slosh_ctx *c, float *w, uint64_t n)
+{
+    if (!c || !w || n != (uint64_t)unet_param_count()) return SLOSH_EINVAL;
+    if (!c->unet.loaded) return SLOSH_ESTATE;
+    ABI_GUARD({
+        SLOSH_HIP(hipMemcpyAsync(w, c->unet.w32, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        return SLOSH_OK;
+    })
+}
+
+namespace {
+// Dense (x fastest) float field -> pool vector of the current allocation (unallocated cells dropped), on device.
+void dense_to_pool_device(slosh_ctx *c, const float *dense, float *pool_dev)
+{
+    const Geo &g = c->geo;
+    std::vector<uint32_t> table = fetch(c, c->table, g.nbricks);
+    std::vector<float> pool((size_t)g.pool_cells, 0.0f);
+    for (int z = 0; z < g.n[2]; ++z)
+        for (int y = 0; y < g.n[1]; ++y)
+            for (int x = 0; x < g.n[0]; ++x) {
+                uint32_t slot = table[(uint32_t)((x >> 3) + g.bd[0] * ((y >> 3) + g.bd[1] * (z >> 3)))];
+                if (!slot) continue;
+                pool[(size_t)slot * kBrickCells + (size_t)((x & 7) + 8 * ((y & 7) + 8 * (z & 7)))] =
+                    dense[(size_t)x + (size_t)g.n[0] * ((size_t)y + (size_t)g.n[1] * z)];
+            }
+    SLOSH_HIP(hipMemcpyAsync(pool_dev, pool.data(), pool.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SLOSH_HIP(hipStreamSynchronize(c->stream));
+}
+
+bool unet_ready(const slosh_ctx *c)
+{
+    const Geo &g = c->geo;
+    return c->unet.loaded && c->nactive > 0 && g.n[0] % 8 == 0 && g.n[1] % 8 == 0 && g.n[2] % 8 == 0;
+}
+} // namespace
+
+slosh_status slosh_unet_apply(slosh_ctx *c, const float *r_dense, float *z_dense)
+{
+    if (!c || !r_dense || !z_dense) return SLOSH_EINVAL;
+    if (!unet_ready(c)) return SLOSH_ESTATE;
+    ABI_GUARD({
+        const Geo &g = c->geo;
+        float *rv = nullptr, *zv = nullptr;
+        SLOSH_HIP(hipMalloc((void **)&rv, (size_t)g.pool_cells * 4u));
+        SLOSH_HIP(hipMalloc((void **)&zv, (size_t)g.pool_cells * 4u));
+        SLOSH_HIP(hipMemsetAsync(zv, 0, (size_t)g.pool_cells * 4u, c->stream));
+        dense_to_pool_device(c, r_dense, rv);
+        l0_brick_info(c);
+        unet_apply(c, rv, zv);
+        std::vector<uint32_t> table = fetch(c, c->table, g.nbricks);
+        std::vector<float> h = fetch(c, zv, g.pool_cells);
+        pool_to_dense(c, table, h.data(), z_dense, 1);
+        (void)hipFree(rv);
+        (void)hipFree(zv);
+        return SLOSH_OK;
+    })
+}
+
+slosh_status slosh_unet_loss_grad(slosh_ctx *c, const float *r_dense, double *loss, float *grad)
+{
+    if (!c || !r_dense || !loss || !grad) return SLOSH_EINVAL;
+    if (!unet_ready(c)) return SLOSH_ESTATE;
+    ABI_GUARD({
+        const Geo &g = c->geo;
+        UNet &u = c->unet;
+        float *rv = nullptr;
+        SLOSH_HIP(hipMalloc((void **)&rv, (size_t)g.pool_cells * 4u));
+        dense_to_pool_device(c, r_dense, rv);
+        l0_brick_info(c);
+        unet_backward(c, rv);
+        UNetScalars sc;
+        SLOSH_HIP(hipMemcpyAsync(&sc, u.sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+        SLOSH_HIP(hipMemcpyAsync(grad, u.grad, u.nparams * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        SLOSH_HIP(hipMemsetAsync(u.grad, 0, u.nparams * sizeof(float), c->stream));
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        *loss = sc.rhat2 > 0.0 ? sc.err2 / sc.rhat2 : 0.0;
+        (void)hipFree(rv);
+        return SLOSH_OK;
+    })
+}
+
+slosh_status slosh_unet_train(slosh_ctx *c, const slosh_unet_train_config *cfg, slosh_unet_train_report *out)
+{
+    if (!c || !cfg || cfg->iterations == 0 || cfg->batch == 0 || !(cfg->learning_rate > 0.0f)) return SLOSH_EINVAL;
+    if (!unet_ready(c) || !c->capture || c->capture_n == 0) return SLOSH_ESTATE;
+    ABI_GUARD({
+        UNet &u = c->unet;
+        l0_brick_info(c);
+        double first = 0.0, last = 0.0;
+        uint32_t k = cfg->offset;
+        for (uint32_t it = 0; it < cfg->iterations; ++it) {
+            SLOSH_HIP(hipMemsetAsync(&u.sc->loss_sum, 0, 2 * sizeof(double), c->stream));
+            for (uint32_t b = 0; b < cfg->batch; ++b, ++k)
+                unet_backward(c, c->capture + (size_t)(k % c->capture_n) * c->geo.pool_cells);
+            unet_adam(c, cfg->learning_rate, 1.0f / (float)cfg->batch);
+            if (it == 0 || it + 1 == cfg->iterations) {
+                UNetScalars sc;
+                SLOSH_HIP(hipMemcpyAsync(&sc, u.sc, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+                SLOSH_HIP(hipStreamSynchronize(c->stream));
+                const double mean = sc.loss_count > 0.0 ? sc.loss_sum / sc.loss_count : 0.0;
+                if (it == 0) first = mean;
+                last = mean;
+            }
+        }
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        if (out) {
+            out->first_loss = first;
+            out->last_loss = last;
+            out->systems = c->capture_n;
+            out->adam_steps = (uint32_t)u.adam_t;
+        }
+        return SLOSH_OK;
+    })
 }
 
 slosh_status slosh_get_results(slosh_ctx *c, slosh_results *out)

@@ -348,6 +348,71 @@ __global__ __launch_bounds__(kThreads) void k_sor_residual(const uint32_t *binfo
     block_partial<2>(v, partials);
 }
 
+// ---- flexible CG (Notay's FCG(1)) with a nonlinear preconditioner z = M(r) ----
+// Partials of z . q_prev (the orthogonalisation coefficient of the next direction).
+__global__ __launch_bounds__(kThreads) void k_fcg_zq(const uint32_t *binfo, const uint8_t *fmask, const float *z,
+                                                     const float *q, double *partials, const SolverState *st)
+{
+    if (st->done) return;
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    double v[1] = {0.0};
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t i = B.base + colour_cell(threadIdx.x, c);
+        if (fmask[i] & kSelfFluid) v[0] += (double)z[i] * (double)q[i];
+    }
+    block_partial<1>(v, partials);
+}
+
+// d_out = z + beta d_in (first: z), beta = -(z . q_prev) / (d_prev . q_prev), fused with q = A d_out (FP64 stencil
+// sum, neighbours' directions recomputed); partials d . q and d . r.
+__global__ __launch_bounds__(kThreads) void k_fcg_direction(const uint32_t *binfo, const uint8_t *fmask,
+                                                            const float *diag, const float *z, const float *d_in,
+                                                            float *d_out, float *q, const float *r, int first,
+                                                            double *partials, const SolverState *st)
+{
+    if (st->done) return;
+    const double beta = first ? 0.0 : st->beta;
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    double v[2] = {0.0, 0.0};
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
+        if (!(m & kSelfFluid)) continue;
+        const float dv = first ? z[i] : (float)((double)z[i] + beta * (double)d_in[i]);
+        d_out[i] = dv;
+        double s = 0.0;
+        for (int n = 0; n < 6; ++n) {
+            if (!(m >> n & 1u)) continue;
+            const uint32_t j = nbr(B, l, n);
+            s += (double)(first ? z[j] : (float)((double)z[j] + beta * (double)d_in[j]));
+        }
+        const double qd = (double)(diag[i] + kDiagEps) * (double)dv - s;
+        q[i] = (float)qd;
+        v[0] += (double)dv * qd;
+        v[1] += (double)dv * (double)r[i];
+    }
+    block_partial<2>(v, partials);
+}
+
+// p += alpha d (FP64), r -= alpha q; partials ||r||^2.
+__global__ __launch_bounds__(kThreads) void k_fcg_update(const uint32_t *binfo, const uint8_t *fmask, double *pres,
+                                                         float *r, const float *d, const float *q, double *partials,
+                                                         const SolverState *st)
+{
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    double v[1] = {0.0};
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t i = B.base + colour_cell(threadIdx.x, c);
+        if (!(fmask[i] & kSelfFluid)) continue;
+        pres[i] = pres[i] + alpha * (double)d[i];
+        const float ri = (float)((double)r[i] - alpha * (double)q[i]);
+        r[i] = ri;
+        v[0] += (double)ri * (double)ri;
+    }
+    block_partial<1>(v, partials);
+}
+
 } // namespace
 
 #define L0_LAUNCH(kid, kernel, ...)                                                                              \
@@ -388,6 +453,18 @@ void l0_red_dot(slosh_ctx *c, const float *r) { L0_LAUNCH(K_RED_DOT, k_red_dot, 
 void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, int first)
 {
     L0_LAUNCH(K_DIRECTION, k_direction, c->diag, c->z1, d_in, d_out, c->q, first, c->partials, c->state);
+}
+
+void l0_fcg_zq(slosh_ctx *c, const float *z) { L0_LAUNCH(K_FCG, k_fcg_zq, z, c->q, c->partials, c->state); }
+
+void l0_fcg_direction(slosh_ctx *c, const float *z, const float *d_in, float *d_out, const float *r, int first)
+{
+    L0_LAUNCH(K_FCG, k_fcg_direction, c->diag, z, d_in, d_out, c->q, r, first, c->partials, c->state);
+}
+
+void l0_fcg_update(slosh_ctx *c, float *r, const float *d)
+{
+    L0_LAUNCH(K_FCG, k_fcg_update, c->pres, r, d, c->q, c->partials, c->state);
 }
 
 void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w)

@@ -26,6 +26,7 @@ struct SolverState {
     double bb;     // ||b||^2
     double rel;    // ||r|| / ||b||
     double tol;
+    double rel_true; // ||b - A p|| / ||b|| of the last true-residual check
     int32_t done;  // 1 once converged or out of iterations: every solver kernel then returns at once
     int32_t iters;
     int32_t maxit;
@@ -110,10 +111,43 @@ enum KernelId {
     K_DIVERGENCE, K_PROJECT, K_DIV_STATS, K_EXTRAPOLATE, K_G2P, K_MG_SETUP, K_COARSE_FACTOR, K_PCG_INIT, K_RED_DOT,
     K_RESTRICT0, K_PROLONG_BLACK, K_DIRECTION, K_UPDATE_BLACK, K_PDOWN1, K_PDOWN2, K_PUP1, K_PUP2, K_D3DOWN, K_D3UP,
     K_DSMOOTH, K_DRESTRICT, K_DPROLONG, K_COARSE_SOLVE, K_COARSE_TAIL, K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL,
-    K_FINALIZE, K_STEP_STATS, K_COUNT
+    K_FINALIZE, K_STEP_STATS, K_UNET_CONV, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_COUNT
 };
 // Names as the compiled kernels are called (what a kernel trace of the library shows).
 extern const char *const kKernelNames[K_COUNT];
+
+// ---- learned preconditioner (unet.hip) ----
+struct UNetScalars {
+    float s, s_inv;      // RMS of the input residual over fluid cells, and its inverse
+    double err2, rhat2;  // loss terms of the last backward pass: sum (A zhat - rhat)^2, sum rhat^2
+    double loss_sum;     // sum of err2 / rhat2 over backward passes since the last reset
+    double loss_count;
+};
+struct UNetLayerDesc {
+    int cin, cout, level, taps;
+};
+constexpr int kUNetLayerCount = 11;
+extern const UNetLayerDesc kUNetLayers[kUNetLayerCount];
+struct UNet {
+    size_t nparams = 0;
+    size_t w_off[kUNetLayerCount]{}, b_off[kUNetLayerCount]{}, wf_off[kUNetLayerCount]{}, wt_off[kUNetLayerCount]{};
+    float *w32 = nullptr;   // FP32 master parameters (flat)
+    uint16_t *wbf = nullptr; // BF16 weights per layer: forward [tap][cout][cin (padded to 16)], mirrored-transposed
+    UNetScalars *sc = nullptr;
+    // activations, channel-last BF16 in the pool layout of their level
+    uint16_t *x0 = nullptr, *a0 = nullptr, *s0 = nullptr, *t0 = nullptr, *d0 = nullptr;                  // level 0
+    uint16_t *p0 = nullptr, *a1 = nullptr, *s1 = nullptr, *t1 = nullptr, *d1 = nullptr, *c1 = nullptr;   // level 1
+    uint16_t *p1 = nullptr, *a2 = nullptr, *b2 = nullptr, *c2 = nullptr;                                 // level 2
+    float *zhat = nullptr;
+    double *partials = nullptr;
+    // training state (allocated by the first backward pass)
+    float *grad = nullptr, *m = nullptr, *v = nullptr, *e = nullptr;
+    uint16_t *g16a = nullptr, *g16b = nullptr, *g16c = nullptr, *g16_1a = nullptr, *g16_1b = nullptr;
+    uint16_t *g32a = nullptr, *g32b = nullptr, *g32c = nullptr, *g32_2a = nullptr, *g32_2b = nullptr;
+    uint16_t *g64a = nullptr, *g64b = nullptr;
+    int adam_t = 0;
+    bool loaded = false; // weights were set
+};
 
 struct StepStats {
     uint32_t fluid_cells;
@@ -190,6 +224,13 @@ struct slosh_ctx {
     uint32_t *bp_speed = nullptr;   // [nbricks] G2P: max speed bits per brick of the sorted particles
     uint32_t *bp_clamp = nullptr;   // [nbricks]
     uint32_t n_p2g_blocks = 0, n_div_blocks = 0, n_g2p_blocks = 0;
+    // learned preconditioner and the residuals captured for its training
+    slosh::UNet unet;
+    float *fcg_z = nullptr;           // [pool_cells] preconditioned residual of the flexible CG
+    float *capture = nullptr;         // [capture_cap][pool_cells]
+    uint32_t capture_cap = 0, capture_n = 0;
+    int capture_iter[32] = {};        // MGPCG iteration of each captured residual
+    int fcg_fallback = 0, fcg_iters = 0;
     // colliders
     slosh_collider *d_col = nullptr;
     slosh_collider h_col[SLOSH_MAX_COLLIDERS];
@@ -266,7 +307,23 @@ void l0_red_dot(slosh_ctx *c, const float *r);
 // d_out = z1 + beta d_in (first: d_out = z1), q = A d_out; partials d.q.
 void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, int first);
 void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w);
+// Partials ||b||^2 and ||b - A p||^2 of the FP64 pressure (the true residual).
 void l0_sor_residual(slosh_ctx *c);
+void l0_fcg_zq(slosh_ctx *c, const float *z);
+void l0_fcg_direction(slosh_ctx *c, const float *z, const float *d_in, float *d_out, const float *r, int first);
+void l0_fcg_update(slosh_ctx *c, float *r, const float *d);
+
+// ---- unet.hip ----
+size_t unet_param_count();
+void unet_init(slosh_ctx *c);
+void unet_free(slosh_ctx *c);
+void unet_upload_bf16(slosh_ctx *c);
+void unet_apply(slosh_ctx *c, const float *r, float *z);
+void unet_backward(slosh_ctx *c, const float *r);
+void unet_adam(slosh_ctx *c, float lr, float scale);
+
+// ---- fcg.hip: flexible CG preconditioned by the network, MGPCG as fallback ----
+void run_fcg(slosh_ctx *c);
 
 } // namespace slosh
 

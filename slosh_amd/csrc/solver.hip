@@ -14,6 +14,8 @@
 // partial sum, and a one-workgroup finalize kernel sums the partials in a fixed order and updates the state. (A
 // grid-wide "last workgroup" ticket needs a device-scope fence per workgroup, which on gfx950's separate per-XCD
 // L2 caches costs more than the whole pass.)
+#include <cmath>
+
 #include "slosh_ctx.h"
 
 namespace slosh {
@@ -48,15 +50,21 @@ __device__ __forceinline__ float dsum(const DenseLevel &L, const float *v, uint3
     return s;
 }
 
-enum FinKind { kFinInit = 0, kFinRho = 1, kFinAlpha = 2, kFinRr = 3, kFinSor = 4 };
+enum FinKind { kFinInit = 0, kFinRho = 1, kFinAlpha = 2, kFinRr = 3, kFinSor = 4, kFinBetaF = 5, kFinAlphaF = 6,
+               kFinCheck = 7 };
+
+__host__ __device__ constexpr int fin_values(int kind)
+{
+    return (kind == kFinInit || kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck) ? 2 : 1;
+}
 
 // Sums the partials of the preceding reduction in a fixed order (per thread: a strided run loaded four at a time;
 // then a wavefront butterfly and the 16 wavefront sums in order) and applies them to the solver state.
 __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
                                                    SolverState *st)
 {
-    if (kind != kFinInit && st->done) return;
-    const int nv = (kind == kFinInit || kind == kFinSor) ? 2 : 1;
+    if (kind != kFinInit && kind != kFinCheck && st->done) return;
+    const int nv = fin_values(kind);
     __shared__ double sh[2][16];
     const uint32_t t = threadIdx.x;
     double a0 = 0.0, a1 = 0.0;
@@ -119,6 +127,15 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
     } else if (kind == kFinAlpha) {
         st->dq = t0;
         st->alpha = fabs(t0) > 1e-30 ? st->rho / t0 : 0.0;
+    } else if (kind == kFinBetaF) {
+        // flexible CG: beta = -(z . q_prev) / (d_prev . q_prev)
+        st->beta = fabs(st->dq) > 1e-30 ? -t0 / st->dq : 0.0;
+    } else if (kind == kFinAlphaF) {
+        // flexible CG: alpha = (d . r) / (d . q)
+        st->dq = t0;
+        st->alpha = fabs(t0) > 1e-30 ? t1 / t0 : 0.0;
+    } else if (kind == kFinCheck) {
+        st->rel_true = t0 > 0.0 ? sqrt(t1) / sqrt(t0) : 0.0;
     } else {
         st->rr = t0;
         st->rel = sqrt(t0) / sqrt(st->bb);
@@ -620,6 +637,7 @@ void finalize(slosh_ctx *c, int kind, int iters = 0)
 {
     SLOSH_TIMED(c, K_FINALIZE);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, kind, c->partials, c->nactive, iters, c->state);
+    (void)fin_values;
 }
 
 void coarse_solve(slosh_ctx *c, const float *b, float *x)
@@ -757,26 +775,119 @@ void run_mgpcg(slosh_ctx *c)
     float *D[2] = {c->d, c->d2};
     l0_pcg_init(c, R[0]);
     finalize(c, kFinInit);
+    // residuals kept for training the learned preconditioner: the initial one and those after iterations 1, 2,
+    // 3, 4, 6, 8, 11, 15, 20, ... (slots past the solve's last iteration are dropped afterwards)
+    uint32_t ncap = 0;
+    int next_cap = 0, cap_gap = 1;
+    auto capture = [&](const float *r, int iteration) {
+        if (!c->capture || ncap >= c->capture_cap || iteration != next_cap) return;
+        SLOSH_HIP(hipMemcpyAsync(c->capture + (size_t)ncap * c->geo.pool_cells, r, (size_t)c->geo.pool_cells * 4u,
+                                 hipMemcpyDeviceToDevice, c->stream));
+        c->capture_iter[ncap++] = iteration;
+        next_cap += cap_gap;
+        if (iteration >= 4) cap_gap++;
+    };
+    capture(R[0], 0);
     l0_update_black(c, R[0], R[1], D[0], 1);
     int rc = 1, dc = 0;
     vcycle(c, R[rc]);
     l0_direction(c, D[1], D[0], 1);
     SLOSH_HIP(hipGetLastError());
     const int chunk = 4;
+    int done_it = 0;
     for (int it = 0; it <= maxit; it += chunk) {
         for (int k = 0; k < chunk; ++k) {
             finalize(c, kFinAlpha);
             l0_update_black(c, R[rc], R[rc ^ 1], D[dc], 0);
             rc ^= 1;
             finalize(c, kFinRr);
+            capture(R[rc], it + k + 1);
             vcycle(c, R[rc]);
             l0_direction(c, D[dc], D[dc ^ 1], 0);
             dc ^= 1;
         }
+        done_it = it + chunk;
         SLOSH_HIP(hipGetLastError());
         SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
         if (c->h_state->done) break;
+    }
+    (void)done_it;
+    if (c->capture) {
+        uint32_t valid = 0;
+        while (valid < ncap && c->capture_iter[valid] <= c->h_state->iters) valid++;
+        c->capture_n = valid;
+    }
+}
+
+// Flexible CG (Notay 2000, FCG(1)) preconditioned by the network: z = M(r), d = z - (z.q_prev / d_prev.q_prev) d_prev,
+// alpha = d.r / d.q. MGPCG is the fallback and the judge: when the relative residual has not halved over the last
+// four iterations, or an iteration limit is reached, or the recurrence converges but the true residual
+// ||b - A p|| / ||b|| of the FP64 pressure exceeds twice the tolerance, MGPCG continues from the current pressure
+// to the tolerance. Every accepted solve therefore meets the MGPCG stopping rule.
+void run_fcg(slosh_ctx *c)
+{
+    c->fcg_iters = 0;
+    c->fcg_fallback = 0;
+    if (!c->nactive) return;
+    if (!c->unet.loaded) {
+        c->fcg_fallback = 2; // no network weights: the solve is plain MGPCG
+        run_mgpcg(c);
+        return;
+    }
+    const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 200;
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
+    l0_brick_info(c);
+    float *D[2] = {c->d, c->d2};
+    float *r = c->r;
+    l0_pcg_init(c, r);
+    finalize(c, kFinInit);
+    int dc = 0;
+    double hist[4] = {0, 0, 0, 0};
+    bool fallback = false;
+    for (int it = 0; it < maxit; ++it) {
+        unet_apply(c, r, c->fcg_z);
+        if (it > 0) {
+            l0_fcg_zq(c, c->fcg_z);
+            finalize(c, kFinBetaF);
+        }
+        l0_fcg_direction(c, c->fcg_z, D[dc], D[dc ^ 1], r, it == 0);
+        dc ^= 1;
+        finalize(c, kFinAlphaF);
+        l0_fcg_update(c, r, D[dc]);
+        finalize(c, kFinRr);
+        SLOSH_HIP(hipGetLastError());
+        SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        const SolverState &h = *c->h_state;
+        c->fcg_iters = h.iters;
+        if (!std::isfinite(h.rel)) {
+            launch_fill_f64(c->stream, c->pres, 0.0, c->geo.pool_cells);
+            fallback = true;
+            break;
+        }
+        if (h.done) {
+            fallback = !h.converged;
+            break;
+        }
+        if (it >= 4 && h.rel > 0.5 * hist[it % 4]) {
+            fallback = true;
+            break;
+        }
+        hist[it % 4] = h.rel;
+        if (it + 1 == maxit) fallback = true;
+    }
+    if (!fallback) {
+        // MGPCG as the judge: the true residual of the accepted pressure
+        l0_sor_residual(c);
+        finalize(c, kFinCheck);
+        SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        if (!(c->h_state->rel_true <= 2.0 * (double)c->s.tolerance)) fallback = true;
+    }
+    if (fallback) {
+        c->fcg_fallback = 1;
+        run_mgpcg(c);
     }
 }
 

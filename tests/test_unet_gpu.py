@@ -1,0 +1,145 @@
+"""The learned preconditioner on the GPU, through the C ABI: the network's forward pass and its loss gradient
+against the oracle (oracle/unet_oracle.py) on the pressure system of a real step, training with the library's own
+kernels, and the flexible-CG solver with its MGPCG fallback and judge."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import unet_oracle as U
+from slosh_amd import abi, scenes, unet
+
+pytestmark = pytest.mark.gpu
+
+
+def make_sim(n=32, solver=abi.SOLVER_MGPCG, flags=0, steps=1, seed=0):
+    from slosh_amd import Sim
+
+    grid, h, reqs = scenes.dam_break(n, seed=seed)
+    s = abi.make_settings(grid, h, 8 * n ** 3, solver=solver, tolerance=1e-6, flags=flags)
+    sim = Sim(s)
+    for r in reqs:
+        sim.emit(r)
+    for _ in range(steps):
+        sim.step(1.0 / 120.0, (0.0, -9.8, 0.0))
+    return sim
+
+
+def oracle_domain(sim):
+    ctype = sim.read_grid(abi.FIELD_CELL_FLAGS)
+    diag = sim.read_grid(abi.FIELD_CELL_DIAG)
+    table = sim.read_grid(abi.FIELD_BRICK_TABLE)
+    nz, ny, nx = ctype.shape
+    alloc = (table.reshape(nz // 8, ny // 8, nx // 8) != 0).repeat(8, 0).repeat(8, 1).repeat(8, 2)
+    return U.Domain(ctype, alloc, diag)
+
+
+def smooth_residual(dom, seed):
+    rng = np.random.default_rng(seed)
+    r = rng.standard_normal(dom.fluid.shape)
+    for _ in range(3):  # a few Jacobi-like averages give the low-frequency content of solver residuals
+        r = (r + np.roll(r, 1, 0) + np.roll(r, 1, 1) + np.roll(r, 1, 2)) / 4.0
+    return (r * dom.fluid).astype(np.float32)
+
+
+def test_forward_matches_oracle():
+    sim = make_sim()
+    dom = oracle_domain(sim)
+    p = U.init_params(7)
+    sim.unet_set_weights(p)
+    for seed in (0, 1):
+        r = smooth_residual(dom, seed)
+        z = sim.unet_apply(r)
+        ref = U.forward(p, dom, r)
+        err = np.linalg.norm(z - ref) / np.linalg.norm(ref)
+        assert err <= 1e-2, (seed, err)
+        assert np.all(z[dom.fluid == 0] == 0.0)
+    # positive homogeneity of z = s * NN(r / s)
+    z2 = sim.unet_apply(3.0 * r)
+    assert np.linalg.norm(z2 - 3.0 * z) <= 1e-5 * np.linalg.norm(3.0 * z)
+    sim.close()
+
+
+def test_loss_gradient_matches_oracle():
+    sim = make_sim()
+    dom = oracle_domain(sim)
+    p = U.init_params(11)
+    offs, _ = U.layer_offsets()
+    p[offs[10][1]] = 0.05
+    sim.unet_set_weights(p)
+    r = smooth_residual(dom, 3)
+    loss, g = sim.unet_loss_grad(r)
+    ref_loss, ref_g = U.loss_and_grad(p.astype(np.float64), dom, r)
+    assert abs(loss - ref_loss) <= 1e-2 * ref_loss, (loss, ref_loss)
+    for k, ((wo, bo), (name, cin, cout, _, taps, _)) in enumerate(zip(offs, U.LAYERS)):
+        a, b = g[wo:bo + cout].astype(np.float64), ref_g[wo:bo + cout]
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
+        rel = float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-300))
+        assert cos >= 0.995 and rel <= 0.1, (name, cos, rel)
+    sim.close()
+
+
+def test_training_reduces_the_loss():
+    sim = make_sim(flags=abi.FLAG_CAPTURE_RESIDUALS, steps=2)
+    sim.unet_set_weights(unet.he_init(0))
+    rep = sim.unet_train(iterations=40, batch=4, learning_rate=2e-3)
+    assert rep["systems"] >= 4 and rep["adam_steps"] == 40
+    assert rep["last_loss"] < 0.5 * rep["first_loss"], rep
+    w = sim.unet_get_weights()
+    assert w.shape == (unet.param_count(),) and np.all(np.isfinite(w))
+    sim.close()
+
+
+def fcg_run(weights, steps=3):
+    sim = make_sim(solver=abi.SOLVER_FCG_UNET, steps=0)
+    if weights is not None:
+        sim.unet_set_weights(weights)
+    out = []
+    for _ in range(steps):
+        sim.step(1.0 / 120.0, (0.0, -9.8, 0.0))
+        out.append(sim.results())
+    pres = sim.read_grid(abi.FIELD_PRESSURE_F64)
+    return sim, out, pres
+
+
+def test_fcg_stall_falls_back_to_mgpcg():
+    # a zero network gives z = 0: no progress, the stall test hands the solve to MGPCG, which meets the tolerance
+    sim, res, _ = fcg_run(np.zeros(unet.param_count(), np.float32))
+    for r in res:
+        assert r.fallback_used == 1 and r.converged == 1 and r.residual_rel <= 1e-6, r.as_dict()
+    sim.close()
+
+
+def test_fcg_without_weights_is_mgpcg():
+    sim, res, _ = fcg_run(None, steps=1)
+    assert res[0].fallback_used == 2 and res[0].converged == 1
+    sim.close()
+
+
+def test_fcg_accepted_solve_meets_tolerance_and_matches_fp64():
+    # a trained network: every accepted solve (with or without fallback) meets the MGPCG stopping rule, and the
+    # pressure matches the oracle's FP64 solve of the same system
+    trainer = make_sim(flags=abi.FLAG_CAPTURE_RESIDUALS, steps=2)
+    trainer.unet_set_weights(unet.he_init(0))
+    trainer.unet_train(iterations=60, batch=4, learning_rate=2e-3)
+    w = trainer.unet_get_weights()
+    trainer.close()
+    grid, h, reqs = scenes.dam_break(32, seed=0)
+    s = abi.make_settings(grid, h, 8 * 32 ** 3, solver=abi.SOLVER_FCG_UNET, tolerance=1e-6)
+    from slosh_amd import Sim
+
+    gpu, orc = Sim(s), O.OracleSim(abi.make_settings(grid, h, 8 * 32 ** 3, solver=abi.SOLVER_MGPCG, tolerance=1e-6))
+    gpu.unet_set_weights(w)
+    for sim in (gpu, orc):
+        for r in reqs:
+            sim.emit(r)
+    # the oracle runs the same step with SOLVER_NONE semantics up to the solve: compare the FP64 reference solve
+    gpu.step(1.0 / 120.0, (0.0, -9.8, 0.0))
+    res = gpu.results()
+    assert res.converged == 1 and res.residual_rel <= 1e-6, res.as_dict()
+    orc.step(1.0 / 120.0, (0.0, -9.8, 0.0))
+    ref, _, _ = orc.solve_fp64(1e-13, 400)
+    p = gpu.read_grid(abi.FIELD_PRESSURE_F64)
+    err = np.linalg.norm(p - ref) / np.linalg.norm(ref)
+    assert err <= 1e-5, err
+    gpu.close()
+    orc.close()
