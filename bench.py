@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--solver", default="mgpcg", choices=["mgpcg", "rbsor"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-report", action="store_true", help="print per-kernel times to stderr")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-FCG comparison run")
+    ap.add_argument("--fcg-steps", type=int, default=10, help="timed steps of the learned-FCG comparison run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run 512 MiB device copies first (scale reference for rocprofv3 --pmc FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -137,46 +139,50 @@ def main():
               "multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
 
-    from slosh_amd import Sim, abi
+    from slosh_amd import Sim, abi, unet
 
     if args.copy_calibration:
         copy_calibration()
     grid, h, reqs, dt, grav = build_scene(args.workload)
     n = sum(int(r.cell_dims[0]) * int(r.cell_dims[1]) * int(r.cell_dims[2]) * 8 for r in reqs)
-    solver = abi.SOLVER_MGPCG if args.solver == "mgpcg" else abi.SOLVER_RBSOR
-    s = abi.make_settings(grid, h, n, solver=solver, tolerance=1e-6, max_iterations=200 if solver == 0 else 20000,
-                          sor_weight=1.8)
-    sim = Sim(s)
-    for r in reqs:
-        sim.emit(r)
-    t = 0.0
-    for _ in range(args.warmup):
-        sim.step(dt, grav(t))
-        t += dt
-    # pick the dominant kernel from a fully instrumented step, then time only that kernel in the timed region
-    sim.set_kernel_timing(~0)
-    sim.kernel_times(reset=True)
-    sim.step(dt, grav(t))
-    t += dt
-    prof = sim.kernel_times(reset=True)
-    dominant = max(prof.items(), key=lambda kv: kv[1][1])[0]
-    sim.set_kernel_timing(1 << sim.kernel_index(dominant))
-    sim.kernel_times(reset=True)
 
-    iters, rels, conv = [], [], 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    def run(solver, weights, steps, warmup):
+        """Times `steps` steps after `warmup`; returns (steps/s, per-step data, dominant kernel, its timing, sim)."""
+        s = abi.make_settings(grid, h, n, solver=solver, tolerance=1e-6,
+                              max_iterations=20000 if solver == abi.SOLVER_RBSOR else 200, sor_weight=1.8)
+        sim = Sim(s)
+        for r in reqs:
+            sim.emit(r)
+        if weights is not None:
+            sim.unet_set_weights(weights)
+        t = 0.0
+        for _ in range(warmup):
+            sim.step(dt, grav(t))
+            t += dt
+        # pick the dominant kernel from a fully instrumented step, then time only that kernel in the timed region
+        sim.set_kernel_timing(~0)
+        sim.kernel_times(reset=True)
         sim.step(dt, grav(t))
         t += dt
-        r = sim.results()
-        iters.append(r.solver_iterations)
-        rels.append(r.residual_rel)
-        conv += r.converged
-    el = time.perf_counter() - t0
-    kt = sim.kernel_times(reset=True)
-    sim.set_kernel_timing(0)
+        prof = sim.kernel_times(reset=True)
+        dominant = max(prof.items(), key=lambda kv: kv[1][1])[0]
+        sim.set_kernel_timing(1 << sim.kernel_index(dominant))
+        sim.kernel_times(reset=True)
+        per = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step(dt, grav(t))
+            t += dt
+            r = sim.results()
+            per.append((r.solver_iterations, r.residual_rel, r.converged, r.fallback_used))
+        el = time.perf_counter() - t0
+        kt = sim.kernel_times(reset=True)
+        sim.set_kernel_timing(0)
+        return steps / el, el, per, dominant, kt[dominant], prof, sim
+
+    solver = {"mgpcg": abi.SOLVER_MGPCG, "rbsor": abi.SOLVER_RBSOR}[args.solver]
+    sps, el, per, dominant, (launches, total_ms), prof, sim = run(solver, None, args.steps, args.warmup)
     res = sim.results()
-    launches, total_ms = kt[dominant]
     avg_s = total_ms / 1e3 / max(launches, 1)
     nb = res.active_bricks
     nc = nb * 512
@@ -189,7 +195,33 @@ def main():
         ach = abytes / avg_s / 1e9
         roofline.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                          "algorithmic_bytes": int(abytes)})
-    sps = args.steps / el
+    sim.close()
+    iters = [p[0] for p in per]
+    rels = [p[1] for p in per]
+    conv = sum(p[2] for p in per)
+
+    # learned preconditioner: flexible CG with the trained network (MGPCG fallback), same workload and tolerance
+    learned = None
+    wpath = unet.default_weights_path()
+    if args.solver == "mgpcg" and not args.no_learned and os.path.exists(wpath):
+        fsps, fel, fper, fdom, (flaunch, fms), fprof, fsim = run(abi.SOLVER_FCG_UNET, unet.load(wpath),
+                                                                 args.fcg_steps, 2)
+        fsim.close()
+        favg = fms / 1e3 / max(flaunch, 1)
+        # level-0 forward convolutions: E0a (4->16), E0b and D0b (16->16), 27 taps, 512 cells per brick
+        conv_flops = {"k_unet_conv_l0": 2.0 * 27 * 512 * nb * (4 * 16 + 16 * 16 + 16 * 16) / 3.0,
+                      "k_unet_conv_l1": 2.0 * 27 * 64 * nb * (16 * 32 + 32 * 32 + 32 * 32 + 32 * 16) / 4.0,
+                      "k_unet_conv_l2": 2.0 * 27 * 8 * nb * (32 * 64 + 64 * 64 + 64 * 32) / 3.0}
+        fl = {"kernel": fdom, "launches": flaunch, "avg_us": round(favg * 1e6, 2)}
+        if fdom in conv_flops:
+            ach = conv_flops[fdom] / favg / 1e12
+            fl.update({"bound": "mfma", "achieved": round(ach, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                       "frac": round(ach / 2500.0, 4), "algorithmic_flops": int(conv_flops[fdom])})
+        learned = {"steps_per_s": round(fsps, 3), "speedup_vs_mgpcg": round(fsps / sps, 3),
+                   "mean_iterations": round(sum(p[0] for p in fper) / len(fper), 2),
+                   "fallback_steps": sum(1 for p in fper if p[3] == 1), "steps": args.fcg_steps,
+                   "converged_steps": sum(p[2] for p in fper), "max_rel_residual": max(p[1] for p in fper),
+                   "roofline": fl, "weights": os.path.relpath(wpath, ROOT)}
     line = {
         "metric": "steps/s @256^3,16M particles; learned-FCG vs plain MGPCG at 1e-6 residual",
         "value": round(sps, 3),
@@ -204,16 +236,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.workload], "solver": f"{args.solver} to ||r||/||b|| <= 1e-6",
-                   "particles": res.live_particles, "grid": list(grid), "fluid_cells": nf, "allocated_bricks": nb,
+                   "value_solver": args.solver, "particles": res.live_particles, "grid": list(grid),
+                   "fluid_cells": nf, "allocated_bricks": nb,
                    "mean_solver_iterations": round(sum(iters) / len(iters), 2), "max_rel_residual": max(rels),
-                   "converged_steps": conv, "particles_per_s": round(sps * res.live_particles, 1)},
+                   "converged_steps": conv, "particles_per_s": round(sps * res.live_particles, 1),
+                   "learned_fcg": learned},
         "roofline": roofline,
     }
     if args.kernel_report:
         for k, (l, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
             if l:
                 print(f"{k:22s} {l:6d} launches {ms:9.3f} ms", file=sys.stderr)
-    sim.close()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(line), flush=True)

@@ -23,7 +23,7 @@ const char *const kKernelNames[K_COUNT] = {
     "k_mg_setup", "k_coarse_factor", "k_pcg_init", "k_red_dot", "k_restrict0", "k_prolong_black", "k_direction",
     "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d3down", "k_d3up",
     "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
-    "k_fill", "k_finalize", "k_step_stats", "k_unet_conv", "k_unet_io", "k_unet_train", "k_fcg"};
+    "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io", "k_unet_train", "k_fcg"};
 } // namespace slosh
 
 static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");

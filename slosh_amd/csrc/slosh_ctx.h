@@ -111,7 +111,7 @@ enum KernelId {
     K_DIVERGENCE, K_PROJECT, K_DIV_STATS, K_EXTRAPOLATE, K_G2P, K_MG_SETUP, K_COARSE_FACTOR, K_PCG_INIT, K_RED_DOT,
     K_RESTRICT0, K_PROLONG_BLACK, K_DIRECTION, K_UPDATE_BLACK, K_PDOWN1, K_PDOWN2, K_PUP1, K_PUP2, K_D3DOWN, K_D3UP,
     K_DSMOOTH, K_DRESTRICT, K_DPROLONG, K_COARSE_SOLVE, K_COARSE_TAIL, K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL,
-    K_FINALIZE, K_STEP_STATS, K_UNET_CONV, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_COUNT
+    K_FINALIZE, K_STEP_STATS, K_UNET_CONV0, K_UNET_CONV1, K_UNET_CONV2, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_COUNT
 };
 // Names as the compiled kernels are called (what a kernel trace of the library shows).
 extern const char *const kKernelNames[K_COUNT];

@@ -43,8 +43,13 @@ inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
 
 constexpr uint32_t kSelfFluid = 1u << 6;
 
-// Bricks per workgroup of a pass at a level with S cells per brick axis.
-__host__ __device__ constexpr int group_bricks(int S) { return S == 2 ? 8 : 1; }
+// Bricks per workgroup of a pass at a level with S cells per brick axis: 512, 256 and 128 output cells, so that a
+// workgroup's weight fragments feed at least 2 M-tiles per wavefront at the coarse levels.
+__host__ __device__ constexpr int group_bricks(int S) { return S == 2 ? 16 : (S == 4 ? 4 : 1); }
+
+// LDS row stride (in BF16 elements) of one staged cell: the channels plus 4 of padding, so that the 16 rows of an A
+// fragment fall into distinct LDS banks.
+__host__ __device__ constexpr int lds_stride(int C) { return C + 4; }
 
 // Pool index (cell) of block-relative coordinates (u, v, w) in [-1, S + 1)^3 through the 27-neighbourhood slots.
 template <int S> __device__ __forceinline__ uint32_t nb_cell(const uint32_t *nb27, int u, int v, int w)
@@ -60,9 +65,9 @@ __global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nb
                                               const uint16_t *w, const float *bias, uint16_t *out)
 {
     constexpr int BG = group_bricks(S), SB = S + 2, BOX = SB * SB * SB, CPB = S * S * S, M = BG * CPB;
-    constexpr int MT = M / 16, NT = COUT / 16, KB = CIN / 16, MTW = MT / 4, CH8 = CIN / 8;
+    constexpr int MT = M / 16, NT = COUT / 16, KB = CIN / 16, MTW = MT / 4, CH8 = CIN / 8, LS = lds_stride(CIN);
     static_assert(MT % 4 == 0 && COUT % 16 == 0 && CIN % 16 == 0, "conv tiling");
-    __shared__ __attribute__((aligned(16))) uint16_t s_in[BG * BOX * CIN];
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[BG * BOX * LS];
     __shared__ uint32_t s_nb[BG][28];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t grp = xcd_remap(blockIdx.x, gridDim.x);
@@ -75,7 +80,10 @@ __global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nb
         const uint32_t c8 = k % CH8, q = k / CH8, bg = q / BOX, bi = q % BOX;
         const int u = (int)(bi % SB) - 1, v = (int)((bi / SB) % SB) - 1, ww = (int)(bi / (SB * SB)) - 1;
         const uint32_t cell = nb_cell<S>(s_nb[bg], u, v, ww);
-        *(uint4 *)(s_in + ((size_t)(bg * BOX + bi) * CIN + c8 * 8)) = *(const uint4 *)(in + (size_t)cell * CIN + c8 * 8);
+        const uint4 val = *(const uint4 *)(in + (size_t)cell * CIN + c8 * 8);
+        uint2 *dst = (uint2 *)(s_in + ((size_t)(bg * BOX + bi) * LS + c8 * 8));
+        dst[0] = make_uint2(val.x, val.y);
+        dst[1] = make_uint2(val.z, val.w);
     }
     __syncthreads();
     const uint32_t row = lane & 15u, quad = lane >> 4;
@@ -86,10 +94,10 @@ __global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nb
     for (int mt = 0; mt < MTW; ++mt) {
         const uint32_t vox = (wave * MTW + mt) * 16u + row, bg = vox / CPB, l = vox % CPB;
         const uint32_t lx = l % S, ly = (l / S) % S, lz = l / (S * S);
-        abase[mt] = (bg * BOX + lx + SB * (ly + SB * lz)) * CIN + quad * 4u;
+        abase[mt] = (bg * BOX + lx + SB * (ly + SB * lz)) * LS + quad * 4u;
     }
     for (int tap = 0; tap < 27; ++tap) {
-        const uint32_t toff = (uint32_t)((tap % 3) + SB * (((tap / 3) % 3) + SB * (tap / 9))) * CIN;
+        const uint32_t toff = (uint32_t)((tap % 3) + SB * (((tap / 3) % 3) + SB * (tap / 9))) * LS;
         for (int kb = 0; kb < KB; ++kb) {
             v4s bfr[NT];
             for (int nt = 0; nt < NT; ++nt)
@@ -550,7 +558,7 @@ void conv(slosh_ctx *c, const uint16_t *in, const uint16_t *w, const float *bias
 {
     constexpr int BG = group_bricks(S);
     const uint32_t groups = (c->nactive + BG - 1) / BG;
-    SLOSH_TIMED(c, K_UNET_CONV);
+    KTimer timer(c, S == 8 ? K_UNET_CONV0 : (S == 4 ? K_UNET_CONV1 : K_UNET_CONV2));
     hipLaunchKernelGGL((k_conv<CIN, COUT, S, RELU, BIAS>), dim3(groups), dim3(256), 0, c->stream, c->binfo, c->nactive,
                        in, w, bias, out);
 }

@@ -1,0 +1,103 @@
+"""Trains the learned preconditioner on the GPU with the library's own forward/backward kernels.
+
+Training data are pressure systems that the plain solver generates from seeded scenes: each scene is stepped with
+MGPCG, the residuals of every solve are captured (SLOSH_FLAG_CAPTURE_RESIDUALS), and a few Adam steps are taken on
+them before the next step, so the network sees the systems of whole trajectories (sloshing tanks with random
+fill depth, wave amplitude and tank acceleration; dam breaks). Usage:
+
+    python -m slosh_amd.train_unet --out slosh_amd/weights/unet_slosh.npz --seconds 300
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import Sim, abi, scenes, unet
+
+
+def scene(kind, n, rng):
+    """(grid, h, requests, dt, gravity(t)) of a random training scene."""
+    if kind == "slosh":
+        mean = float(rng.uniform(0.08, 0.3))
+        amp = float(rng.uniform(0.0, 0.3)) * mean
+        grid, h, reqs = scenes.wave_columns(n, int(rng.integers(1, 1 << 30)), mean, amp)
+        acc = float(rng.uniform(0.5, 3.0))
+        freq = float(rng.uniform(0.5, 2.0))
+        return grid, h, reqs, 1.0 / 240.0, lambda t: (-acc * math.sin(2.0 * math.pi * freq * t), -9.8, 0.0)
+    fx, fy = float(rng.uniform(0.2, 0.6)), float(rng.uniform(0.3, 0.8))
+    grid, h, reqs = scenes.dam_break(n, seed=int(rng.integers(1, 1 << 30)), fill=(fx, fy, 1.0))
+    return grid, h, reqs, 1.0 / 120.0, lambda t: (0.0, -9.8, 0.0)
+
+
+def train(out, seconds, sizes=(64, 128), steps_per_scene=24, iters_per_step=6, batch=4, lr=1e-3, seed=0,
+          init=None, log=None):
+    rng = np.random.default_rng(seed)
+    w = unet.load(init) if init else unet.he_init(seed, head_scale=0.1)
+    t0 = time.time()
+    history = []
+    nscene = 0
+    while time.time() - t0 < seconds:
+        kind = "slosh" if nscene % 3 != 2 else "dam"
+        n = int(sizes[nscene % len(sizes)])
+        grid, h, reqs, dt, grav = scene(kind, n, rng)
+        cap = sum(int(np.prod(r.cell_dims)) * 8 for r in reqs) + 1024
+        s = abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6, flags=abi.FLAG_CAPTURE_RESIDUALS)
+        sim = Sim(s)
+        for r in reqs:
+            sim.emit(r)
+        sim.unet_set_weights(w)
+        t = 0.0
+        for k in range(steps_per_scene):
+            sim.step(dt, grav(t))
+            t += dt
+            step_lr = lr * (0.3 if time.time() - t0 > 0.7 * seconds else 1.0)
+            rep = sim.unet_train(iters_per_step, batch=batch, learning_rate=step_lr, offset=int(rng.integers(0, 64)))
+            history.append({"scene": nscene, "kind": kind, "n": n, "step": k, "first_loss": rep["first_loss"],
+                            "last_loss": rep["last_loss"], "systems": rep["systems"]})
+            if time.time() - t0 >= seconds:
+                break
+        w = sim.unet_get_weights()
+        sim.close()
+        last = history[-1]
+        msg = (f"scene {nscene} {kind} {n}^3: loss {history[-steps_per_scene if len(history) >= steps_per_scene else 0]['first_loss']:.4f}"
+               f" -> {last['last_loss']:.4f}  ({time.time() - t0:.0f} s)")
+        print(msg, flush=True)
+        if log:
+            with open(log, "a") as f:
+                f.write(msg + "\n")
+        nscene += 1
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    unet.save(out, w, seconds=seconds, scenes=nscene, final_loss=history[-1]["last_loss"] if history else -1.0)
+    return w, history
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=unet.default_weights_path())
+    ap.add_argument("--seconds", type=float, default=300.0)
+    ap.add_argument("--sizes", default="64,128")
+    ap.add_argument("--steps-per-scene", type=int, default=24)
+    ap.add_argument("--iters-per-step", type=int, default=6)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init", default=None)
+    ap.add_argument("--log", default=None)
+    ap.add_argument("--history", default=None, help="write the per-step loss history (JSON) here")
+    a = ap.parse_args()
+    _, hist = train(a.out, a.seconds, tuple(int(x) for x in a.sizes.split(",")), a.steps_per_scene, a.iters_per_step,
+                    a.batch, a.lr, a.seed, a.init, a.log)
+    if a.history:
+        with open(a.history, "w") as f:
+            json.dump(hist, f)
+    print(json.dumps({"weights": a.out, "scenes": len({h["scene"] for h in hist}),
+                      "final_loss": hist[-1]["last_loss"] if hist else None}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
