@@ -73,6 +73,9 @@ typedef struct slosh_settings {
 
 #define SLOSH_FLAG_NONE 0u
 #define SLOSH_FLAG_CAPTURE_RESIDUALS 1u /* keep MGPCG residuals of each solve as training data for slosh_unet_train */
+#define SLOSH_FLAG_MG_FP16 2u           /* MGPCG: multigrid levels >= 1 stored in FP16 (FP32 arithmetic, FP32 level-0
+                                           residual, each V-cycle scaled to O(1) values); the result still meets the
+                                           tolerance */
 
 /* Emission request: FSloshEmitRequest (SloshTypes.h:169-181). shape 0 = FillCells (8 jittered particles per cell). */
 enum { SLOSH_EMIT_FILL_CELLS = 0, SLOSH_EMIT_SPHERE = 1, SLOSH_EMIT_BOX = 2 };
@@ -254,6 +257,9 @@ typedef struct slosh_unet_train_report {
     uint32_t systems;     /* captured residuals available */
     uint32_t adam_steps;  /* total Adam steps since the weights were set */
 } slosh_unet_train_report;
+/* Copies captured residual k (0 = the solve's initial residual b - A p0) of the last solve in dense order; *n
+ * receives the number of captured residuals. The training-data export of the learned preconditioner. */
+slosh_status slosh_unet_read_capture(slosh_ctx *ctx, uint32_t k, float *dst, uint64_t dst_bytes, uint32_t *n);
 /* Trains on the residuals captured by the last solve (settings flag SLOSH_FLAG_CAPTURE_RESIDUALS) of the current
  * system, with the library's forward/backward HIP kernels and Adam. */
 slosh_status slosh_unet_train(slosh_ctx *ctx, const slosh_unet_train_config *cfg, slosh_unet_train_report *out);

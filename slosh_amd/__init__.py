@@ -79,6 +79,8 @@ def load_library(path=LIB_PATH):
     lib.slosh_unet_apply.restype = st
     lib.slosh_unet_loss_grad.argtypes = [vp, vp, C.POINTER(C.c_double), vp]
     lib.slosh_unet_loss_grad.restype = st
+    lib.slosh_unet_read_capture.argtypes = [vp, C.c_uint32, vp, C.c_uint64, C.POINTER(C.c_uint32)]
+    lib.slosh_unet_read_capture.restype = st
     lib.slosh_unet_train.argtypes = [vp, C.POINTER(abi.UNetTrainConfig), C.POINTER(abi.UNetTrainReport)]
     lib.slosh_unet_train.restype = st
     _lib = lib
@@ -217,6 +219,15 @@ class Sim:
         loss = C.c_double(0.0)
         _check(self.lib.slosh_unet_loss_grad(self.h, r.ctypes.data, C.byref(loss), g.ctypes.data), "slosh_unet_loss_grad")
         return loss.value, g
+
+    def captured_residual(self, k):
+        """Captured residual k of the last solve (dense (z, y, x)), and the number captured."""
+        nx, ny, nz = self.dims
+        out = np.zeros((nz, ny, nx), np.float32)
+        n = C.c_uint32(0)
+        _check(self.lib.slosh_unet_read_capture(self.h, int(k), out.ctypes.data, out.nbytes, C.byref(n)),
+               "slosh_unet_read_capture")
+        return out, n.value
 
     def unet_train(self, iterations, batch=4, learning_rate=1e-3, offset=0):
         cfg = abi.UNetTrainConfig(int(iterations), int(batch), float(learning_rate), int(offset))

@@ -22,6 +22,7 @@ SOLVER_NONE = 2
 SOLVER_FCG_UNET = 3
 
 FLAG_CAPTURE_RESIDUALS = 1
+FLAG_MG_FP16 = 2
 
 EMIT_FILL_CELLS = 0
 EMIT_SPHERE = 1

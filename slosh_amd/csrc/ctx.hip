@@ -104,7 +104,7 @@ bool settings_valid(const slosh_settings *s)
     if (s->pressure_solver == SLOSH_SOLVER_FCG_UNET)
         for (int a = 0; a < 3; ++a)
             if (s->grid_dim[a] % 8 != 0) return false; // the network's levels need whole bricks
-    if (s->flags & ~(uint32_t)SLOSH_FLAG_CAPTURE_RESIDUALS) return false;
+    if (s->flags & ~(uint32_t)(SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_MG_FP16)) return false;
     if (!(s->tolerance > 0.0f) || s->max_iterations < 0) return false;
     if (!(s->sor_weight >= 0.5f && s->sor_weight <= 1.95f)) return false;
     if (s->extrapolation_layers < 0 || s->extrapolation_layers > 4) return false;
@@ -220,12 +220,14 @@ void create_impl(slosh_ctx *c)
         for (int a = 0; a < 3; ++a) l.dim[a] = (p.dim[a] + 1) / 2;
         l.cells = (uint32_t)l.dim[0] * (uint32_t)l.dim[1] * (uint32_t)l.dim[2];
         l.pooled = li <= 2;
+        l.half = (s.flags & SLOSH_FLAG_MG_FP16) != 0;
         const size_t n = l.pooled ? (size_t)g.npool * (size_t)(512u >> (3 * li)) : (size_t)l.cells;
+        const size_t esz = l.half ? 2u : 4u;
         dalloc(&l.type, n);
         dalloc(&l.info, n);
-        dalloc(&l.x, n);
-        dalloc(&l.b, n);
-        dalloc(&l.x1, n);
+        dalloc((uint8_t **)&l.x, n * esz);
+        dalloc((uint8_t **)&l.b, n * esz);
+        dalloc((uint8_t **)&l.x1, n * esz);
         if (!l.pooled) {
             if (l.dim[0] > 1024 || l.dim[1] > 1024 || l.dim[2] > 1024) throw std::bad_alloc();
             std::vector<uint32_t> coord(n);
@@ -861,6 +863,23 @@ slosh_status slosh_unet_loss_grad(slosh_ctx *c, const float *r_dense, double *lo
         SLOSH_HIP(hipStreamSynchronize(c->stream));
         *loss = sc.rhat2 > 0.0 ? sc.err2 / sc.rhat2 : 0.0;
         (void)hipFree(rv);
+        return SLOSH_OK;
+    })
+}
+
+slosh_status slosh_unet_read_capture(slosh_ctx *c, uint32_t k, float *dst, uint64_t dst_bytes, uint32_t *n)
+{
+    if (!c || !dst || !n) return SLOSH_EINVAL;
+    *n = c->capture_n;
+    if (!c->capture) return SLOSH_ESTATE;
+    if (k >= c->capture_n) return SLOSH_ECAPACITY;
+    const Geo &g = c->geo;
+    const size_t cells = (size_t)g.n[0] * g.n[1] * g.n[2];
+    if (dst_bytes < cells * 4u) return SLOSH_ECAPACITY;
+    ABI_GUARD({
+        std::vector<uint32_t> table = fetch(c, c->table, g.nbricks);
+        std::vector<float> h = fetch(c, c->capture + (size_t)k * g.pool_cells, g.pool_cells);
+        pool_to_dense(c, table, h.data(), dst, 1);
         return SLOSH_OK;
     })
 }

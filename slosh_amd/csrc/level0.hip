@@ -117,19 +117,20 @@ __global__ void k_brick_info(Geo g, uint32_t nactive, uint32_t *binfo)
     p[27] = b;
 }
 
-// r = b - A p (FP64 p); partials ||b||^2, ||r||^2.
+// r = b - A p (FP64 p); partials ||b||^2, ||r||^2 and the fluid cell count.
 __global__ __launch_bounds__(kThreads) void k_pcg_init(const uint32_t *binfo, const uint8_t *fmask, const float *div,
                                                        const float *diag, const double *pres, float *r,
                                                        double *partials)
 {
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
-    double v[2] = {0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
         const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
         if (!(m & kSelfFluid)) {
             r[i] = 0.0f;
             continue;
         }
+        v[2] += 1.0;
         double s = nbsum<double>(B, l, m, pres);
         float dg = diag[i] + kDiagEps;
         float ri = (float)(-(double)div[i] - ((double)dg * pres[i] - s));
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(const uint32_t *binfo, co
         v[0] += (double)div[i] * (double)div[i];
         v[1] += (double)ri * (double)ri;
     }
-    block_partial<2>(v, partials);
+    block_partial<3>(v, partials);
 }
 
 // The residual update of the iteration fused with the V-cycle's first level-0 sweeps (PCGUpdatePRCS with its
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(kThreads) void k_restrict0(const uint32_t *binfo, c
         if (fl[f]) sum += res[f];
     }
     float bv = 0.25f * sum;
+    if (L1.b.half) bv *= (float)st->mg_scale; // FP16 levels hold the scaled cycle (the V-cycle is linear)
     L1.b[ci] = bv;
     L1.x[ci] = (info & kInfoRed) ? bv / info_diag(info) : 0.0f;
 }
@@ -230,13 +232,14 @@ __global__ __launch_bounds__(kThreads) void k_prolong_black(const uint32_t *binf
 {
     if (st->done) return;
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    const float unscale = L1.x1.half ? (float)(1.0 / st->mg_scale) : 1.0f;
     for (uint32_t c = 0; c < 2u; ++c) {
         const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
         if (!(m & kSelfFluid)) continue;
         if (c == 0u) {
             float v = z[i];
             const uint32_t pi = (B.base >> 3) + parent1(l);
-            if (L1.info[pi] & kInfoSelf) v += L1.x1[pi];
+            if (L1.info[pi] & kInfoSelf) v += L1.x1[pi] * unscale;
             z1[i] = v;
         } else {
             float s = 0.0f;
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(kThreads) void k_prolong_black(const uint32_t *binf
                 const uint32_t j = nbr(B, l, n, nl);
                 float v = z[j];
                 const uint32_t pj = ((j - nl) >> 3) + parent1(nl);
-                if (L1.info[pj] & kInfoSelf) v += L1.x1[pj];
+                if (L1.info[pj] & kInfoSelf) v += L1.x1[pj] * unscale;
                 s += v;
             }
             z1[i] = (r[i] + s) / (diag[i] + kDiagEps);

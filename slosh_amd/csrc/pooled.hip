@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_pdown_lds(const uint32_t *binfo, PLevel
         // level 2 restricts into level 3, stored densely by brick index
         const uint32_t ci = S == 2 ? s_nb[27] : slot * CPB2 + k;
         const uint32_t cinfo = S == 2 ? D.info[ci] : C.info[ci];
-        float *cb = S == 2 ? D.b : C.b, *cxv = S == 2 ? D.x : C.x;
+        const LVec cb = S == 2 ? D.b : C.b, cxv = S == 2 ? D.x : C.x;
         if (!(cinfo & kInfoSelf)) {
             cb[ci] = 0.0f;
             cxv[ci] = 0.0f;
@@ -302,9 +302,9 @@ PLevel plevel_of(const slosh_ctx *c, int l)
     P.cpb = (uint32_t)(P.s * P.s * P.s);
     P.type = c->lv[l].type;
     P.info = c->lv[l].info;
-    P.x = c->lv[l].x;
-    P.x1 = c->lv[l].x1;
-    P.b = c->lv[l].b;
+    P.x = LVec{c->lv[l].x, c->lv[l].half};
+    P.x1 = LVec{c->lv[l].x1, c->lv[l].half};
+    P.b = LVec{c->lv[l].b, c->lv[l].half};
     return P;
 }
 
@@ -318,9 +318,9 @@ DenseLevel dlevel_of(const slosh_ctx *c, int l)
     D.type = c->lv[l].type;
     D.info = c->lv[l].info;
     D.coord = c->lv[l].coord;
-    D.x = c->lv[l].x;
-    D.x1 = c->lv[l].x1;
-    D.b = c->lv[l].b;
+    D.x = LVec{c->lv[l].x, c->lv[l].half};
+    D.x1 = LVec{c->lv[l].x1, c->lv[l].half};
+    D.b = LVec{c->lv[l].b, c->lv[l].half};
     return D;
 }
 

@@ -1,6 +1,7 @@
 // slosh_ctx.h -- host-side context of one simulation domain and the launchers of the gfx950 kernels.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -27,6 +28,8 @@ struct SolverState {
     double rel;    // ||r|| / ||b||
     double tol;
     double rel_true; // ||b - A p|| / ||b|| of the last true-residual check
+    double nf;       // fluid cells
+    double mg_scale; // FP16 levels: the V-cycle input scale sqrt(nf / ||r||^2), so the levels hold O(1) values
     int32_t done;  // 1 once converged or out of iterations: every solver kernel then returns at once
     int32_t iters;
     int32_t maxit;
@@ -40,10 +43,36 @@ struct Level {
     int dim[3];
     uint32_t cells;
     bool pooled;
+    bool half;       // x, x1, b stored as FP16 (SLOSH_FLAG_MG_FP16), else FP32
     uint8_t *type;
     uint16_t *info;
     uint32_t *coord; // dense levels: packed cell coordinates x | y << 10 | z << 20
-    float *x, *x1, *b;
+    void *x, *x1, *b;
+};
+
+// A coarse-level vector in FP32 or FP16 storage; arithmetic is FP32 either way (FP16 values are rounded to
+// nearest even on store). v[i] reads and writes like a float.
+struct LVec {
+    void *p;
+    int half;
+    struct Ref {
+        void *p;
+        int half;
+        uint32_t i;
+        __device__ __forceinline__ operator float() const
+        {
+            return half ? __half2float(((const __half *)p)[i]) : ((const float *)p)[i];
+        }
+        __device__ __forceinline__ Ref &operator=(float v)
+        {
+            if (half) ((__half *)p)[i] = __float2half(v);
+            else ((float *)p)[i] = v;
+            return *this;
+        }
+        __device__ __forceinline__ Ref &operator=(const Ref &o) { return *this = (float)o; }
+        __device__ __forceinline__ Ref &operator+=(float v) { return *this = (float)*this + v; }
+    };
+    __device__ __forceinline__ Ref operator[](uint32_t i) const { return Ref{p, half, i}; }
 };
 
 // Words per allocated brick in the per-solve brick table: slots of the 3x3x3 neighbourhood (entry (dx+1) +
@@ -80,7 +109,7 @@ struct PLevel {
     uint32_t cpb;  // cells per brick
     uint8_t *type;
     uint16_t *info;
-    float *x, *x1, *b;
+    LVec x, x1, b;
 };
 struct DenseLevel {
     int dim[3];
@@ -88,7 +117,7 @@ struct DenseLevel {
     uint8_t *type;
     uint16_t *info;
     const uint32_t *coord;
-    float *x, *x1, *b;
+    LVec x, x1, b;
 };
 
 __device__ __forceinline__ bool pinside(const PLevel &L, int x, int y, int z)

@@ -42,7 +42,7 @@ __device__ __forceinline__ int doff(const DenseLevel &L, int n)
     return (n & 1) ? s : -s;
 }
 
-__device__ __forceinline__ float dsum(const DenseLevel &L, const float *v, uint32_t i, uint32_t info)
+template <typename V> __device__ __forceinline__ float dsum(const DenseLevel &L, const V &v, uint32_t i, uint32_t info)
 {
     float s = 0.0f;
     for (int n = 0; n < 6; ++n)
@@ -55,7 +55,7 @@ enum FinKind { kFinInit = 0, kFinRho = 1, kFinAlpha = 2, kFinRr = 3, kFinSor = 4
 
 __host__ __device__ constexpr int fin_values(int kind)
 {
-    return (kind == kFinInit || kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck) ? 2 : 1;
+    return kind == kFinInit ? 3 : ((kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck) ? 2 : 1);
 }
 
 // Sums the partials of the preceding reduction in a fixed order (per thread: a strided run loaded four at a time;
@@ -65,39 +65,34 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
 {
     if (kind != kFinInit && kind != kFinCheck && st->done) return;
     const int nv = fin_values(kind);
-    __shared__ double sh[2][16];
+    __shared__ double sh[3][16];
     const uint32_t t = threadIdx.x;
-    double a0 = 0.0, a1 = 0.0;
+    double a[3] = {0.0, 0.0, 0.0};
     uint32_t b = t;
     for (; b + 3072u < nblocks; b += 4096u) {
-        double v0[4], v1[4];
-        for (int k = 0; k < 4; ++k) {
-            v0[k] = partials[(size_t)(b + 1024u * k) * nv];
-            v1[k] = nv == 2 ? partials[(size_t)(b + 1024u * k) * nv + 1] : 0.0;
-        }
-        for (int k = 0; k < 4; ++k) {
-            a0 += v0[k];
-            a1 += v1[k];
-        }
+        double v[4][3];
+        for (int k = 0; k < 4; ++k)
+            for (int j = 0; j < 3; ++j) v[k][j] = j < nv ? partials[(size_t)(b + 1024u * k) * nv + j] : 0.0;
+        for (int k = 0; k < 4; ++k)
+            for (int j = 0; j < 3; ++j) a[j] += v[k][j];
     }
-    for (; b < nblocks; b += 1024u) {
-        a0 += partials[(size_t)b * nv];
-        if (nv == 2) a1 += partials[(size_t)b * nv + 1];
-    }
-    for (int w = 32; w > 0; w >>= 1) {
-        a0 += __shfl_xor(a0, w);
-        a1 += __shfl_xor(a1, w);
-    }
-    if ((t & 63u) == 0u) {
-        sh[0][t >> 6] = a0;
-        sh[1][t >> 6] = a1;
-    }
+    for (; b < nblocks; b += 1024u)
+        for (int j = 0; j < nv; ++j) a[j] += partials[(size_t)b * nv + j];
+    for (int w = 32; w > 0; w >>= 1)
+        for (int j = 0; j < 3; ++j) a[j] += __shfl_xor(a[j], w);
+    if ((t & 63u) == 0u)
+        for (int j = 0; j < 3; ++j) sh[j][t >> 6] = a[j];
     __syncthreads();
     if (t != 0) return;
-    double t0 = 0.0, t1 = 0.0;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
     for (int w = 0; w < 16; ++w) {
         t0 += sh[0][w];
         t1 += sh[1][w];
+        t2 += sh[2][w];
+    }
+    if (kind == kFinInit) {
+        st->nf = t2;
+        st->mg_scale = t1 > 0.0 ? sqrt(t2 / t1) : 1.0;
     }
     if (kind == kFinInit || kind == kFinSor) {
         // t0 = ||b||^2, t1 = ||r||^2
@@ -138,6 +133,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
         st->rel_true = t0 > 0.0 ? sqrt(t1) / sqrt(t0) : 0.0;
     } else {
         st->rr = t0;
+        st->mg_scale = t0 > 0.0 ? sqrt(st->nf / t0) : 1.0;
         st->rel = sqrt(t0) / sqrt(st->bb);
         st->iters += 1;
         if (st->rel <= st->tol) {
@@ -375,8 +371,8 @@ __device__ __forceinline__ DenseLevel tview(const TailArgs &T, int k, uint16_t *
     DenseLevel V = T.L[k];
     V.info = si + T.off[k];
     V.coord = sc + T.off[k];
-    V.x = sx + T.off[k];
-    V.b = sb + T.off[k];
+    V.x = LVec{sx + T.off[k], 0};
+    V.b = LVec{sb + T.off[k], 0};
     return V;
 }
 
@@ -605,7 +601,7 @@ __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ct
 // Exact coarsest solve x = A^-1 b in FP64 (replaces MGCoarseSolveCS, SloshPressureMG.usf:386-413) when the
 // coarsest level is level 0 or pooled; b and x are indexed by the factor's cell list.
 __global__ __launch_bounds__(64) void k_coarse_solve(const uint32_t *cidx, const uint32_t *cn, const double *inv,
-                                                     const float *b, float *x, const SolverState *st)
+                                                     LVec b, LVec x, const SolverState *st)
 {
     if (st->done) return;
     constexpr int N = (int)kCoarsestCells;
@@ -640,7 +636,7 @@ void finalize(slosh_ctx *c, int kind, int iters = 0)
     (void)fin_values;
 }
 
-void coarse_solve(slosh_ctx *c, const float *b, float *x)
+void coarse_solve(slosh_ctx *c, LVec b, LVec x)
 {
     SLOSH_TIMED(c, K_COARSE_SOLVE);
     hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(64), 0, c->stream, c->coarse_idx, c->coarse_n, c->coarse_inv, b,
@@ -714,12 +710,12 @@ void vcycle(slosh_ctx *c, const float *r)
 {
     const int m = c->nlevels;
     if (m == 1) {
-        coarse_solve(c, r, c->z1);
+        coarse_solve(c, LVec{(void *)r, 0}, LVec{c->z1, 0});
     } else {
         l0_restrict(c, r);
         const int last_pooled = m - 2 < 2 ? m - 2 : 2; // pooled levels with a down pass (1, 2)
         for (int l = 1; l <= last_pooled; ++l) pooled_down(c, l);
-        if (m - 1 <= 3) coarse_solve(c, c->lv[m - 1].b, c->lv[m - 1].x1);
+        if (m - 1 <= 3) coarse_solve(c, LVec{c->lv[m - 1].b, c->lv[m - 1].half}, LVec{c->lv[m - 1].x1, c->lv[m - 1].half});
         else dense_part(c);
         for (int l = last_pooled; l >= 1; --l) pooled_up(c, l);
         l0_prolong_black(c, r);

@@ -1,0 +1,45 @@
+"""MGPCG with the multigrid levels stored in FP16 (SLOSH_FLAG_MG_FP16): every solve still meets the 1e-6 relative
+residual, and its error against the oracle's FP64 solve of the same system is reported and within the bar."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from slosh_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("kind,n", [("dam", 32), ("dam", 64), ("slosh", 64)])
+def test_fp16_levels_meet_tolerance_and_fp64_error(kind, n):
+    from slosh_amd import Sim
+
+    if kind == "dam":
+        grid, h, reqs = scenes.dam_break(n, seed=0)
+        dt, grav = 1.0 / 120.0, lambda t: (0.0, -9.8, 0.0)
+    else:
+        grid, h, reqs = scenes.sloshing_tank(n, seed=1)
+        dt, grav = 1.0 / 240.0, scenes.tank_gravity
+    cap = sum(int(np.prod(r.cell_dims)) * 8 for r in reqs) + 1024
+    runs = {}
+    for flags in (0, abi.FLAG_MG_FP16):
+        sim = Sim(abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6, flags=flags))
+        for r in reqs:
+            sim.emit(r)
+        sim.step(dt, grav(0.0))
+        runs[flags] = (sim.results(), sim.read_grid(abi.FIELD_PRESSURE_F64))
+        sim.close()
+    orc = O.OracleSim(abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6))
+    for r in reqs:
+        orc.emit(r)
+    orc.step(dt, grav(0.0))
+    ref, _, _ = orc.solve_fp64(1e-13, 600)
+    orc.close()
+    r32, p32 = runs[0]
+    r16, p16 = runs[abi.FLAG_MG_FP16]
+    assert r16.converged == 1 and r16.residual_rel <= 1e-6, r16.as_dict()
+    err16 = np.linalg.norm(p16 - ref) / np.linalg.norm(ref)
+    err32 = np.linalg.norm(p32 - ref) / np.linalg.norm(ref)
+    print(f"{kind} {n}^3: FP16 levels {r16.solver_iterations} iterations, error vs FP64 {err16:.2e}; "
+          f"FP32 levels {r32.solver_iterations} iterations, error {err32:.2e}")
+    assert err16 <= 1e-5, err16
+    assert r16.solver_iterations <= r32.solver_iterations + 4

@@ -143,3 +143,43 @@ def test_fcg_accepted_solve_meets_tolerance_and_matches_fp64():
     assert err <= 1e-5, err
     gpu.close()
     orc.close()
+
+
+def test_training_data_are_the_solver_systems():
+    # the generator's systems: the first captured residual of a step from rest is b = -divergence, bit for bit
+    # the oracle's; the operator assembled from the step's cell types and diagonal is symmetric positive definite
+    grid, h, reqs = scenes.dam_break(16, seed=0)
+    s = abi.make_settings(grid, h, 8 * 16 ** 3, solver=abi.SOLVER_MGPCG, tolerance=1e-6,
+                          flags=abi.FLAG_CAPTURE_RESIDUALS)
+    from slosh_amd import Sim
+
+    gpu, orc = Sim(s), O.OracleSim(abi.make_settings(grid, h, 8 * 16 ** 3, solver=abi.SOLVER_MGPCG, tolerance=1e-6))
+    for sim in (gpu, orc):
+        for r in reqs:
+            sim.emit(r)
+        sim.step(1.0 / 120.0, (0.0, -9.8, 0.0))
+    r0, n = gpu.captured_residual(0)
+    assert n >= 4
+    div = orc.read_grid(abi.FIELD_DIVERGENCE)
+    fluid = (orc.read_grid(abi.FIELD_CELL_FLAGS) & 3) == 1
+    assert np.array_equal((-div)[fluid].view(np.uint32), r0[fluid].view(np.uint32))
+    assert np.all(r0[~fluid] == 0.0)
+    # the operator on the fluid cells: (diag + 1e-4) on the diagonal, -1 between fluid neighbours
+    diag = gpu.read_grid(abi.FIELD_CELL_DIAG)
+    idx = -np.ones(fluid.shape, np.int64)
+    idx[fluid] = np.arange(fluid.sum())
+    A = np.zeros((fluid.sum(), fluid.sum()))
+    for z, y, x in zip(*np.nonzero(fluid)):
+        i = idx[z, y, x]
+        A[i, i] = float(np.float32(diag[z, y, x]) + np.float32(1e-4))
+        for dz, dy, dx in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            zz, yy, xx = z + dz, y + dy, x + dx
+            if 0 <= zz < 16 and 0 <= yy < 16 and 0 <= xx < 16 and fluid[zz, yy, xx]:
+                A[i, idx[zz, yy, xx]] = -1.0
+    assert np.array_equal(A, A.T)
+    assert np.linalg.eigvalsh(A).min() > 0.0
+    # and later captured residuals are the MGPCG recurrence: their norms decrease
+    norms = [np.linalg.norm(gpu.captured_residual(k)[0]) for k in range(n)]
+    assert norms[-1] < norms[0]
+    gpu.close()
+    orc.close()
