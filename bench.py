@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-report", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-FCG comparison run")
+    ap.add_argument("--no-kernel-events", action="store_true",
+                    help="no HIP events around the dominant kernel in the timed steps (no roofline figure)")
     ap.add_argument("--fcg-steps", type=int, default=10, help="timed steps of the learned-FCG comparison run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run 512 MiB device copies first (scale reference for rocprofv3 --pmc FETCH/WRITE_SIZE)")
@@ -166,7 +168,7 @@ def main():
         t += dt
         prof = sim.kernel_times(reset=True)
         dominant = max(prof.items(), key=lambda kv: kv[1][1])[0]
-        sim.set_kernel_timing(1 << sim.kernel_index(dominant))
+        sim.set_kernel_timing(0 if args.no_kernel_events else 1 << sim.kernel_index(dominant))
         sim.kernel_times(reset=True)
         per = []
         t0 = time.perf_counter()
@@ -191,7 +193,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": measured_traffic(args.workload, dominant), "launches": launches,
                 "avg_us": avg_s * 1e6}
-    if abytes:
+    if abytes and avg_s > 0:
         ach = abytes / avg_s / 1e9
         roofline.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                          "algorithmic_bytes": int(abytes)})
@@ -207,13 +209,13 @@ def main():
         fsps, fel, fper, fdom, (flaunch, fms), fprof, fsim = run(abi.SOLVER_FCG_UNET, unet.load(wpath),
                                                                  args.fcg_steps, 2)
         fsim.close()
-        favg = fms / 1e3 / max(flaunch, 1)
+        favg = fms / 1e3 / max(flaunch, 1) if fms > 0 else float("nan")
         # level-0 forward convolutions: E0a (4->16), E0b and D0b (16->16), 27 taps, 512 cells per brick
         conv_flops = {"k_unet_conv_l0": 2.0 * 27 * 512 * nb * (4 * 16 + 16 * 16 + 16 * 16) / 3.0,
                       "k_unet_conv_l1": 2.0 * 27 * 64 * nb * (16 * 32 + 32 * 32 + 32 * 32 + 32 * 16) / 4.0,
                       "k_unet_conv_l2": 2.0 * 27 * 8 * nb * (32 * 64 + 64 * 64 + 64 * 32) / 3.0}
         fl = {"kernel": fdom, "launches": flaunch, "avg_us": round(favg * 1e6, 2)}
-        if fdom in conv_flops:
+        if fdom in conv_flops and favg == favg:
             ach = conv_flops[fdom] / favg / 1e12
             fl.update({"bound": "mfma", "achieved": round(ach, 1), "peak": 2500.0, "unit": "TFLOP/s",
                        "frac": round(ach / 2500.0, 4), "algorithmic_flops": int(conv_flops[fdom])})

@@ -264,6 +264,28 @@ slosh_status slosh_unet_read_capture(slosh_ctx *ctx, uint32_t k, float *dst, uin
  * system, with the library's forward/backward HIP kernels and Adam. */
 slosh_status slosh_unet_train(slosh_ctx *ctx, const slosh_unet_train_config *cfg, slosh_unet_train_report *out);
 
+/* ---- batches of independent tanks (no reference counterpart: the reference runs one volume per proxy) ----------
+ * A batch steps existing contexts together: tanks run on several host threads at once, each on its own stream,
+ * longest first; solver iteration chunks replay as HIP graphs. The configuration changes timing only: every tank's
+ * results equal its single-tank run bit for bit. */
+typedef struct slosh_batch slosh_batch;
+typedef struct slosh_batch_config {
+    uint32_t workers;      /* tanks stepped at once (1..64) */
+    uint32_t solver_chunk; /* MGPCG iterations per read of the solver state (even, 2..32) */
+    uint32_t graphs;       /* 1: replay iteration chunks as captured HIP graphs */
+    uint32_t reserved;
+} slosh_batch_config;
+slosh_status slosh_batch_create(slosh_ctx *const *tanks, uint32_t n, slosh_batch **out);
+void slosh_batch_destroy(slosh_batch *batch);
+/* One substep of every tank; gravity holds 3 floats per tank. */
+slosh_status slosh_batch_step(slosh_batch *batch, float dt, const float *gravity);
+slosh_status slosh_batch_set_config(slosh_batch *batch, const slosh_batch_config *cfg);
+slosh_status slosh_batch_get_config(const slosh_batch *batch, slosh_batch_config *cfg);
+/* Times candidate configurations on steps_per_candidate real batch steps each (the tanks advance) and keeps the
+ * fastest. */
+slosh_status slosh_batch_autotune(slosh_batch *batch, float dt, const float *gravity, uint32_t steps_per_candidate,
+                                  slosh_batch_config *chosen);
+
 /* Library identification: ABI version and the offload target the kernels were built for. */
 uint32_t slosh_abi_version(void);
 const char *slosh_build_info(void);

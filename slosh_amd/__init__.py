@@ -79,6 +79,18 @@ def load_library(path=LIB_PATH):
     lib.slosh_unet_apply.restype = st
     lib.slosh_unet_loss_grad.argtypes = [vp, vp, C.POINTER(C.c_double), vp]
     lib.slosh_unet_loss_grad.restype = st
+    lib.slosh_batch_create.argtypes = [C.POINTER(vp), C.c_uint32, C.POINTER(vp)]
+    lib.slosh_batch_create.restype = st
+    lib.slosh_batch_destroy.argtypes = [vp]
+    lib.slosh_batch_destroy.restype = None
+    lib.slosh_batch_step.argtypes = [vp, C.c_float, C.POINTER(C.c_float)]
+    lib.slosh_batch_step.restype = st
+    lib.slosh_batch_set_config.argtypes = [vp, C.POINTER(abi.BatchConfig)]
+    lib.slosh_batch_set_config.restype = st
+    lib.slosh_batch_get_config.argtypes = [vp, C.POINTER(abi.BatchConfig)]
+    lib.slosh_batch_get_config.restype = st
+    lib.slosh_batch_autotune.argtypes = [vp, C.c_float, C.POINTER(C.c_float), C.c_uint32, C.POINTER(abi.BatchConfig)]
+    lib.slosh_batch_autotune.restype = st
     lib.slosh_unet_read_capture.argtypes = [vp, C.c_uint32, vp, C.c_uint64, C.POINTER(C.c_uint32)]
     lib.slosh_unet_read_capture.restype = st
     lib.slosh_unet_train.argtypes = [vp, C.POINTER(abi.UNetTrainConfig), C.POINTER(abi.UNetTrainReport)]
@@ -241,3 +253,52 @@ class Sim:
         cnt = C.c_uint32(0)
         _check(self.lib.slosh_device_particles(self.h, layout, C.byref(ptr), C.byref(cnt)), "slosh_device_particles")
         return ptr.value, cnt.value
+
+
+class Batch:
+    """Independent tanks (Sim objects) stepped together by the library (slosh_batch_*)."""
+
+    def __init__(self, sims):
+        self.lib = load_library()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h for s in self.sims])
+        self.h = C.c_void_p()
+        _check(self.lib.slosh_batch_create(arr, len(self.sims), C.byref(self.h)), "slosh_batch_create")
+
+    def close(self):
+        if self.h:
+            self.lib.slosh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _grav(self, gravity):
+        g = np.asarray(gravity, np.float32)
+        if g.shape == (3,):
+            g = np.tile(g, (len(self.sims), 1))
+        g = np.ascontiguousarray(g.reshape(len(self.sims), 3))
+        return g, g.ctypes.data_as(C.POINTER(C.c_float))
+
+    def step(self, dt, gravity):
+        g, ptr = self._grav(gravity)
+        _check(self.lib.slosh_batch_step(self.h, float(dt), ptr), "slosh_batch_step")
+
+    def set_config(self, workers, solver_chunk=4, graphs=1):
+        cfg = abi.BatchConfig(int(workers), int(solver_chunk), int(graphs), 0)
+        _check(self.lib.slosh_batch_set_config(self.h, C.byref(cfg)), "slosh_batch_set_config")
+
+    def config(self):
+        cfg = abi.BatchConfig()
+        _check(self.lib.slosh_batch_get_config(self.h, C.byref(cfg)), "slosh_batch_get_config")
+        return {"workers": cfg.workers, "solver_chunk": cfg.solver_chunk, "graphs": cfg.graphs}
+
+    def autotune(self, dt, gravity, steps_per_candidate=1):
+        g, ptr = self._grav(gravity)
+        cfg = abi.BatchConfig()
+        _check(self.lib.slosh_batch_autotune(self.h, float(dt), ptr, int(steps_per_candidate), C.byref(cfg)),
+               "slosh_batch_autotune")
+        return {"workers": cfg.workers, "solver_chunk": cfg.solver_chunk, "graphs": cfg.graphs}

@@ -154,6 +154,11 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint32), ("pad", C.c_uint32), ("total_ms", C.c_double)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("workers", C.c_uint32), ("solver_chunk", C.c_uint32), ("graphs", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
 class UNetTrainConfig(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("batch", C.c_uint32), ("learning_rate", C.c_float), ("offset", C.c_uint32)]
 

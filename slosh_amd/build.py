@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libslosh.so")
 BUILD_DIR = os.path.join(HERE, "_build")
-SOURCES = ["particles.hip", "grid.hip", "level0.hip", "pooled.hip", "solver.hip", "unet.hip", "ctx.hip"]
+SOURCES = ["particles.hip", "grid.hip", "level0.hip", "pooled.hip", "solver.hip", "unet.hip", "batch.hip", "ctx.hip"]
 ARCH = "gfx950"
 
 

@@ -90,6 +90,8 @@ void free_all(slosh_ctx *c)
     if (c->ev_step1) (void)hipEventDestroy(c->ev_step1);
     if (c->ev_solve0) (void)hipEventDestroy(c->ev_solve0);
     if (c->ev_solve1) (void)hipEventDestroy(c->ev_solve1);
+    if (c->mg_graph) (void)hipGraphExecDestroy(c->mg_graph);
+    c->mg_graph = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
 }

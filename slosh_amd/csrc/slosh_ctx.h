@@ -253,6 +253,10 @@ struct slosh_ctx {
     uint32_t *bp_speed = nullptr;   // [nbricks] G2P: max speed bits per brick of the sorted particles
     uint32_t *bp_clamp = nullptr;   // [nbricks]
     uint32_t n_p2g_blocks = 0, n_div_blocks = 0, n_g2p_blocks = 0;
+    // MGPCG iterations per host check of the solver state, and their replay as a captured HIP graph
+    int solver_chunk = 4;
+    bool use_graphs = true;
+    hipGraphExec_t mg_graph = nullptr;
     // learned preconditioner and the residuals captured for its training
     slosh::UNet unet;
     float *fcg_z = nullptr;           // [pool_cells] preconditioned residual of the flexible CG

@@ -758,6 +758,46 @@ void mg_setup(slosh_ctx *c)
                        c->coarse_idx, c->coarse_n, c->coarse_inv);
 }
 
+// Captures the launches of `body` on the context's stream into c->mg_graph (updating the executable graph in
+// place when the topology is unchanged, else instantiating it anew). On any capture failure the launches are
+// enqueued directly instead, graphs are switched off for the context, and false is returned.
+template <typename F> bool capture_chunk_graph(slosh_ctx *c, F body)
+{
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        c->use_graphs = false;
+        body();
+        return false;
+    }
+    body();
+    if (hipStreamEndCapture(c->stream, &g) != hipSuccess || !g) {
+        c->use_graphs = false;
+        (void)hipGetLastError();
+        body();
+        return false;
+    }
+    bool ok = false;
+    if (c->mg_graph) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res;
+        ok = hipGraphExecUpdate(c->mg_graph, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(c->mg_graph);
+            c->mg_graph = nullptr;
+        }
+    }
+    if (!ok) ok = hipGraphInstantiate(&c->mg_graph, g, nullptr, nullptr, 0) == hipSuccess;
+    (void)hipGraphDestroy(g);
+    if (!ok) {
+        (void)hipGetLastError();
+        c->mg_graph = nullptr;
+        c->use_graphs = false;
+        body();
+    }
+    return ok;
+}
+
 } // namespace
 
 // SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG), to ||r|| <= tol ||b||.
@@ -789,26 +829,37 @@ void run_mgpcg(slosh_ctx *c)
     vcycle(c, R[rc]);
     l0_direction(c, D[1], D[0], 1);
     SLOSH_HIP(hipGetLastError());
-    const int chunk = 4;
-    int done_it = 0;
-    for (int it = 0; it <= maxit; it += chunk) {
+    // Iterations are enqueued in chunks between reads of the device state (an even count, so the ping-pong
+    // buffers return to the same roles). Without residual capture or kernel timing, a chunk is captured once per
+    // solve as a HIP graph and replayed: every chunk of a solve launches the same kernels on the same buffers.
+    const int chunk = c->solver_chunk >= 2 ? c->solver_chunk & ~1 : 4;
+    const bool use_graph = c->use_graphs && !c->capture && c->timing_mask == 0;
+    auto enqueue_chunk = [&](int it0) {
         for (int k = 0; k < chunk; ++k) {
             finalize(c, kFinAlpha);
             l0_update_black(c, R[rc], R[rc ^ 1], D[dc], 0);
             rc ^= 1;
             finalize(c, kFinRr);
-            capture(R[rc], it + k + 1);
+            capture(R[rc], it0 + k + 1);
             vcycle(c, R[rc]);
             l0_direction(c, D[dc], D[dc ^ 1], 0);
             dc ^= 1;
         }
-        done_it = it + chunk;
+    };
+    bool graph_ready = false;
+    for (int it = 0; it <= maxit; it += chunk) {
+        if (use_graph && c->use_graphs) {
+            // on a capture failure the helper has enqueued the chunk directly and switched graphs off
+            if (!graph_ready) graph_ready = capture_chunk_graph(c, [&] { enqueue_chunk(it); });
+            if (graph_ready) SLOSH_HIP(hipGraphLaunch(c->mg_graph, c->stream));
+        } else {
+            enqueue_chunk(it);
+        }
         SLOSH_HIP(hipGetLastError());
         SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
         if (c->h_state->done) break;
     }
-    (void)done_it;
     if (c->capture) {
         uint32_t valid = 0;
         while (valid < ncap && c->capture_iter[valid] <= c->h_state->iters) valid++;
