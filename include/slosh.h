@@ -157,7 +157,8 @@ typedef struct slosh_results {
     float step_ms;                 /* device time of the step (HIP events) */
     float solve_ms;                /* device time of the pressure solve */
     int32_t fallback_used;         /* flexible CG: 1 when MGPCG finished the solve, 2 when no network was set */
-    int32_t reserved[5];
+    float true_residual_rel;       /* ||b - A p||_2 / ||b||_2 of the FP64 pressure after the solve */
+    int32_t reserved[4];
 } slosh_results;
 
 /* ---- lifecycle --------------------------------------------------------------------------------------------- */

@@ -143,7 +143,8 @@ class Results(C.Structure):
         ("step_ms", C.c_float),
         ("solve_ms", C.c_float),
         ("fallback_used", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("true_residual_rel", C.c_float),
+        ("reserved", C.c_int32 * 4),
     ]
 
     def as_dict(self):

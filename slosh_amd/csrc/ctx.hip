@@ -247,7 +247,7 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->coarse_idx, kCoarsestCells);
     dalloc(&c->coarse_n, 1);
     dalloc(&c->coarse_inv, (size_t)kCoarsestCells * kCoarsestCells);
-    dalloc(&c->partials, 2 * (size_t)g.npool + 2u);
+    dalloc(&c->partials, (size_t)kMaxBlockPartials * ((size_t)g.npool + 1u));
     dalloc(&c->bp_fluid, g.npool);
     dalloc(&c->bp_div_max, g.npool);
     dalloc(&c->bp_div_cnt, g.npool);
@@ -316,6 +316,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     c->res.sim_frame = c->frame;
     c->res.solver_iterations = 0;
     c->res.residual_rel = 0.0;
+    c->res.true_residual_rel = 0.0f;
     c->res.converged = 1;
     c->res.solve_ms = 0.0f;
     launch_fill_u32(c->stream, (uint32_t *)c->stats, 0u, sizeof(StepStats) / 4u);
@@ -356,6 +357,8 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
                 c->res.solver_iterations = fcg ? (c->fcg_fallback ? c->fcg_iters + c->h_state->iters : c->fcg_iters)
                                                : c->h_state->iters;
                 c->res.residual_rel = c->h_state->rel;
+                c->res.true_residual_rel = c->s.pressure_solver == SLOSH_SOLVER_RBSOR ? (float)c->h_state->rel
+                                                                                      : (float)c->h_state->rel_true;
                 c->res.rhs_norm = sqrt(c->h_state->bb);
                 c->res.converged = c->h_state->converged;
             }

@@ -85,6 +85,7 @@ __device__ __forceinline__ uint32_t parent1(uint32_t l)
 
 template <int NV> __device__ void block_partial(const double (&v)[NV], double *partials)
 {
+    static_assert(NV >= 1 && NV <= (int)kMaxBlockPartials, "partials buffer holds kMaxBlockPartials per brick");
     __shared__ double sh[NV][kThreads];
     for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = v[k];
     __syncthreads();
@@ -331,12 +332,12 @@ __global__ __launch_bounds__(kThreads) void k_sor_sweep(const uint32_t *binfo, c
     pres[i] = old + w * (gs - old);
 }
 
-// Partials ||b||^2 and ||b - A p||^2 for the SOR convergence check.
+// Partials ||b||^2 and ||b - A p||^2 (the SOR convergence check; with `always` also after the solver is done).
 __global__ __launch_bounds__(kThreads) void k_sor_residual(const uint32_t *binfo, const uint8_t *fmask,
                                                            const float *div, const float *diag, const double *pres,
-                                                           double *partials, const SolverState *st)
+                                                           double *partials, const SolverState *st, int always)
 {
-    if (st->done) return;
+    if (!always && st->done) return;
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     double v[2] = {0.0, 0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
@@ -477,7 +478,12 @@ void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w)
 
 void l0_sor_residual(slosh_ctx *c)
 {
-    L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state);
+    L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 0);
+}
+
+void l0_true_residual(slosh_ctx *c)
+{
+    L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 1);
 }
 
 } // namespace slosh

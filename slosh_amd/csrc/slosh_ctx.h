@@ -78,6 +78,9 @@ struct LVec {
 // Words per allocated brick in the per-solve brick table: slots of the 3x3x3 neighbourhood (entry (dx+1) +
 // 3(dy+1) + 9(dz+1), 0 = none), then the brick index.
 constexpr uint32_t kBinfoWords = 32u;
+// Values a level-0 reduction writes per workgroup (k_pcg_init: ||b||^2, ||r||^2, fluid cells); the partials
+// buffer holds this many per pool slot, one workgroup per allocated brick.
+constexpr uint32_t kMaxBlockPartials = 3u;
 
 // Brick handled by workgroup w of a one-brick-per-workgroup pass over n bricks. Workgroups are dispatched
 // round-robin over the MI355X's 8 XCDs, each with its own L2; giving XCD k a contiguous range of the ascending
@@ -342,6 +345,8 @@ void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, int first);
 void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w);
 // Partials ||b||^2 and ||b - A p||^2 of the FP64 pressure (the true residual).
 void l0_sor_residual(slosh_ctx *c);
+// The same partials whatever the solver state (the true residual of a finished solve).
+void l0_true_residual(slosh_ctx *c);
 void l0_fcg_zq(slosh_ctx *c, const float *z);
 void l0_fcg_direction(slosh_ctx *c, const float *z, const float *d_in, float *d_out, const float *r, int first);
 void l0_fcg_update(slosh_ctx *c, float *r, const float *d);

@@ -860,6 +860,9 @@ void run_mgpcg(slosh_ctx *c)
         SLOSH_HIP(hipStreamSynchronize(c->stream));
         if (c->h_state->done) break;
     }
+    // the true residual ||b - A p|| / ||b|| of the FP64 pressure, reported with the step's results
+    l0_true_residual(c);
+    finalize(c, kFinCheck);
     if (c->capture) {
         uint32_t valid = 0;
         while (valid < ncap && c->capture_iter[valid] <= c->h_state->iters) valid++;
@@ -926,7 +929,7 @@ void run_fcg(slosh_ctx *c)
     }
     if (!fallback) {
         // MGPCG as the judge: the true residual of the accepted pressure
-        l0_sor_residual(c);
+        l0_true_residual(c);
         finalize(c, kFinCheck);
         SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
