@@ -206,6 +206,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-report", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-FCG comparison run")
+    ap.add_argument("--mg-fluid-coarsening", action="store_true",
+                    help="MGPCG preconditioner option: fluid-first coarse cell types (SLOSH_FLAG_MG_FLUID_COARSENING)")
+    ap.add_argument("--mg-coarse-sweeps", type=int, default=0,
+                    help="MGPCG preconditioner option: red-black pairs per visit of levels 4 and coarser (0 = 1)")
     ap.add_argument("--no-kernel-events", action="store_true",
                     help="no HIP events around the dominant kernel in the timed steps (no roofline figure)")
     ap.add_argument("--fcg-steps", type=int, default=10, help="timed steps of the learned-FCG comparison run")
@@ -230,7 +234,9 @@ def main():
     def run(solver, weights, steps, warmup):
         """Times `steps` steps after `warmup`; returns (steps/s, per-step data, dominant kernel, its timing, sim)."""
         s = abi.make_settings(grid, h, n, solver=solver, tolerance=1e-6,
-                              max_iterations=20000 if solver == abi.SOLVER_RBSOR else 200, sor_weight=1.8)
+                              max_iterations=20000 if solver == abi.SOLVER_RBSOR else 200, sor_weight=1.8,
+                              flags=abi.FLAG_MG_FLUID_COARSENING if args.mg_fluid_coarsening else 0,
+                              mg_coarse_sweeps=args.mg_coarse_sweeps)
         sim = Sim(s)
         for r in reqs:
             sim.emit(r)
@@ -321,6 +327,8 @@ def main():
                    "fluid_cells": nf, "allocated_bricks": nb,
                    "mean_solver_iterations": round(sum(iters) / len(iters), 2), "max_rel_residual": max(rels),
                    "converged_steps": conv, "particles_per_s": round(sps * res.live_particles, 1),
+                   "mg_preconditioner": {"fluid_coarsening": bool(args.mg_fluid_coarsening),
+                                         "coarse_sweeps": max(args.mg_coarse_sweeps, 1)},
                    "learned_fcg": learned},
         "roofline": roofline,
     }

@@ -68,7 +68,9 @@ typedef struct slosh_settings {
     float density_correction;     /* velocity-based density correction strength (0 in the parity profile) */
     float density_deadband;
     uint32_t flags;               /* SLOSH_FLAG_* */
-    int32_t reserved[6];          /* must be zero */
+    int32_t mg_coarse_sweeps;     /* MGPCG: red-black smoothing pairs per visit of the dense coarse levels (4 and
+                                     coarser), 0..4; 0 and 1 give the reference's single pair (SloshProxy.cpp:904-959) */
+    int32_t reserved[5];          /* must be zero */
 } slosh_settings;
 
 #define SLOSH_FLAG_NONE 0u
@@ -76,6 +78,10 @@ typedef struct slosh_settings {
 #define SLOSH_FLAG_MG_FP16 2u           /* MGPCG: multigrid levels >= 1 stored in FP16 (FP32 arithmetic, FP32 level-0
                                            residual, each V-cycle scaled to O(1) values); the result still meets the
                                            tolerance */
+#define SLOSH_FLAG_MG_FLUID_COARSENING 4u /* MGPCG: a coarse cell with any fluid child is fluid (the reference's rule,
+                                             SloshPressureMG.usf:256-278, makes it air when any child is air); the
+                                             preconditioner keeps more of the free-surface band, the solution and
+                                             its tolerance are unchanged */
 
 /* Emission request: FSloshEmitRequest (SloshTypes.h:169-181). shape 0 = FillCells (8 jittered particles per cell). */
 enum { SLOSH_EMIT_FILL_CELLS = 0, SLOSH_EMIT_SPHERE = 1, SLOSH_EMIT_BOX = 2 };

@@ -705,9 +705,11 @@ static uint32_t lv_type(const osim *o, int l, int x, int y, int z)
     return o->lv[l].type[lv_index(o, l, x, y, z)] & CELL_TYPE_MASK;
 }
 
-/* SloshPressureMG.usf:256-278 (MGBuildTypeCS) */
+/* SloshPressureMG.usf:256-278 (MGBuildTypeCS): AIR if any child is air, else FLUID if any child is fluid, else
+ * SOLID. With SLOSH_FLAG_MG_FLUID_COARSENING (a preconditioner option of this build) FLUID takes precedence. */
 static void mg_build_types(osim *o)
 {
+    const int fluid_first = (o->s.flags & SLOSH_FLAG_MG_FLUID_COARSENING) != 0;
     for (int l = 1; l < o->nlevels; ++l) {
         mglevel *lv = &o->lv[l];
         for (int z = 0; z < lv->dim[2]; ++z)
@@ -721,7 +723,9 @@ static void mg_build_types(osim *o)
                         any_air |= t == CELL_AIR;
                         any_fluid |= t == CELL_FLUID;
                     }
-                    lv->type[lv_index(o, l, x, y, z)] = any_air ? CELL_AIR : (any_fluid ? CELL_FLUID : CELL_SOLID);
+                    lv->type[lv_index(o, l, x, y, z)] =
+                        fluid_first ? (any_fluid ? CELL_FLUID : (any_air ? CELL_AIR : CELL_SOLID))
+                                    : (any_air ? CELL_AIR : (any_fluid ? CELL_FLUID : CELL_SOLID));
                 }
     }
 }

@@ -23,6 +23,7 @@ SOLVER_FCG_UNET = 3
 
 FLAG_CAPTURE_RESIDUALS = 1
 FLAG_MG_FP16 = 2
+FLAG_MG_FLUID_COARSENING = 4
 
 EMIT_FILL_CELLS = 0
 EMIT_SPHERE = 1
@@ -90,7 +91,8 @@ class Settings(C.Structure):
         ("density_correction", C.c_float),
         ("density_deadband", C.c_float),
         ("flags", C.c_uint32),
-        ("reserved", C.c_int32 * 6),
+        ("mg_coarse_sweeps", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -180,7 +182,8 @@ assert PARTICLE_DTYPE.itemsize == 64 and PARTICLE_REF48_DTYPE.itemsize == 48
 
 
 def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6, max_iterations=200, sor_weight=1.8,
-                  budget=1.0, extrapolation_layers=2, ghost_fluid=1, friction=0.0, max_speed=20.0, flags=0):
+                  budget=1.0, extrapolation_layers=2, ghost_fluid=1, friction=0.0, max_speed=20.0, flags=0,
+                  mg_coarse_sweeps=0):
     """Parity-profile settings (APIC, ghost fluid, no density correction) with the given grid and solver."""
     s = Settings()
     for a in range(3):
@@ -200,6 +203,7 @@ def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6
     s.density_correction = 0.0
     s.density_deadband = 0.05
     s.flags = int(flags)
+    s.mg_coarse_sweeps = int(mg_coarse_sweeps)
     return s
 
 

@@ -65,12 +65,14 @@ void free_all(slosh_ctx *c)
     for (int k = 0; k < 3; ++k) dfree(c->vel[k]);
     dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag); dfree(c->fmask);
     dfree(c->r); dfree(c->r2); dfree(c->z); dfree(c->z1); dfree(c->d); dfree(c->d2); dfree(c->q);
-    dfree(c->binfo); dfree(c->res3); dfree(c->fcg_z); dfree(c->capture);
+    dfree(c->binfo); dfree(c->fcg_z); dfree(c->capture);
     unet_free(c);
     for (int l = 1; l < kMaxLevels; ++l) {
-        dfree(c->lv[l].type); dfree(c->lv[l].info); dfree(c->lv[l].coord);
+        dfree(c->lv[l].type);
+        if (c->lv[l].pooled) dfree(c->lv[l].info); // dense levels' words and coordinates live in the two blocks
         dfree(c->lv[l].x); dfree(c->lv[l].x1); dfree(c->lv[l].b);
     }
+    dfree(c->dense_info); dfree(c->dense_coord);
     dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->partials);
     dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
     dfree(c->bp_clamp);
@@ -106,12 +108,14 @@ bool settings_valid(const slosh_settings *s)
     if (s->pressure_solver == SLOSH_SOLVER_FCG_UNET)
         for (int a = 0; a < 3; ++a)
             if (s->grid_dim[a] % 8 != 0) return false; // the network's levels need whole bricks
-    if (s->flags & ~(uint32_t)(SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_MG_FP16)) return false;
+    if (s->flags & ~(uint32_t)(SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_MG_FP16 | SLOSH_FLAG_MG_FLUID_COARSENING))
+        return false;
     if (!(s->tolerance > 0.0f) || s->max_iterations < 0) return false;
     if (!(s->sor_weight >= 0.5f && s->sor_weight <= 1.95f)) return false;
     if (s->extrapolation_layers < 0 || s->extrapolation_layers > 4) return false;
     if (!(s->boundary_friction >= 0.0f && s->boundary_friction <= 1.0f)) return false;
-    for (int k = 0; k < 6; ++k)
+    if (s->mg_coarse_sweeps < 0 || s->mg_coarse_sweeps > 4) return false;
+    for (int k = 0; k < 5; ++k)
         if (s->reserved[k] != 0) return false;
     return true;
 }
@@ -206,7 +210,6 @@ void create_impl(slosh_ctx *c)
         c->capture_cap = 12;
         dalloc(&c->capture, (size_t)c->capture_cap * pc);
     }
-    dalloc(&c->res3, g.nbricks);
     // MG hierarchy: halve (rounding up) until a level holds <= 64 cells or 8 levels exist. Levels 1..3 live in the
     // brick pool ((8 >> l)^3 cells per slot), coarser ones are dense. dalloc zeroes them: the null slot of a
     // pooled level must read as AIR with x = b = 0.
@@ -226,24 +229,39 @@ void create_impl(slosh_ctx *c)
         const size_t n = l.pooled ? (size_t)g.npool * (size_t)(512u >> (3 * li)) : (size_t)l.cells;
         const size_t esz = l.half ? 2u : 4u;
         dalloc(&l.type, n);
-        dalloc(&l.info, n);
+        if (l.pooled) dalloc(&l.info, n);
         dalloc((uint8_t **)&l.x, n * esz);
         dalloc((uint8_t **)&l.b, n * esz);
         dalloc((uint8_t **)&l.x1, n * esz);
-        if (!l.pooled) {
-            if (l.dim[0] > 1024 || l.dim[1] > 1024 || l.dim[2] > 1024) throw std::bad_alloc();
-            std::vector<uint32_t> coord(n);
-            for (uint32_t i = 0; i < l.cells; ++i) {
-                const uint32_t x = i % (uint32_t)l.dim[0], y = (i / (uint32_t)l.dim[0]) % (uint32_t)l.dim[1],
-                               z = i / (uint32_t)(l.dim[0] * l.dim[1]);
-                coord[i] = x | y << 10 | z << 20;
-            }
-            dalloc(&l.coord, n);
-            SLOSH_HIP(hipMemcpy(l.coord, coord.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-        }
+        if (!l.pooled && (l.dim[0] > 1024 || l.dim[1] > 1024 || l.dim[2] > 1024)) throw std::bad_alloc();
         c->nlevels++;
     }
     if (c->lv[c->nlevels - 1].cells > kCoarsestCells) throw std::bad_alloc();
+    // The dense levels' stencil words and cell coordinates are two contiguous blocks, level by level from the
+    // first dense level, so the coarse tail reads all of its levels with one flat index.
+    {
+        size_t total = 0;
+        for (int l = 1; l < c->nlevels; ++l)
+            if (!c->lv[l].pooled) {
+                c->lv[l].dense_off = (uint32_t)total;
+                total += c->lv[l].cells;
+            }
+        dalloc(&c->dense_info, total);
+        dalloc(&c->dense_coord, total);
+        std::vector<uint32_t> coord(total ? total : 1);
+        for (int li = 1; li < c->nlevels; ++li) {
+            Level &l = c->lv[li];
+            if (l.pooled) continue;
+            l.info = c->dense_info + l.dense_off;
+            l.coord = c->dense_coord + l.dense_off;
+            for (uint32_t i = 0; i < l.cells; ++i) {
+                const uint32_t x = i % (uint32_t)l.dim[0], y = (i / (uint32_t)l.dim[0]) % (uint32_t)l.dim[1],
+                               z = i / (uint32_t)(l.dim[0] * l.dim[1]);
+                coord[l.dense_off + i] = x | y << 10 | z << 20;
+            }
+        }
+        if (total) SLOSH_HIP(hipMemcpy(c->dense_coord, coord.data(), total * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
     dalloc(&c->coarse_idx, kCoarsestCells);
     dalloc(&c->coarse_n, 1);
     dalloc(&c->coarse_inv, (size_t)kCoarsestCells * kCoarsestCells);

@@ -150,35 +150,58 @@ __global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo
                                                            float *r_out, float *z, const float *d, const float *q,
                                                            int first, double *partials, const SolverState *st)
 {
-    if (st->done) return;
-    const double alpha = first ? 0.0 : st->alpha;
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    const uint32_t done = st->done;
+    const double alpha = first ? 0.0 : st->alpha;
+    const uint32_t l0 = colour_cell(threadIdx.x, 0u), l1 = colour_cell(threadIdx.x, 1u);
+    const uint32_t i0 = B.base + l0, i1 = B.base + l1;
+    const uint32_t m0 = fmask[i0], m1 = fmask[i1];
+    const float g0 = diag[i0], g1 = diag[i1], ra0 = r_in[i0], ra1 = r_in[i1];
+    float q0 = 0.0f, q1 = 0.0f, d0 = 0.0f, d1 = 0.0f;
+    double p0 = 0.0, p1 = 0.0;
+    float rj[6], qj[6], gj[6];
+    for (int n = 0; n < 6; ++n) {
+        const uint32_t j = nbr(B, l1, n);
+        rj[n] = r_in[j];
+        gj[n] = diag[j];
+    }
+    if (!first) {
+        q0 = q[i0];
+        q1 = q[i1];
+        d0 = d[i0];
+        d1 = d[i1];
+        p0 = pres[i0];
+        p1 = pres[i1];
+        for (int n = 0; n < 6; ++n) qj[n] = q[nbr(B, l1, n)];
+    }
+    issue_point();
+    if (done) return;
     double v[1] = {0.0};
-    for (uint32_t c = 0; c < 2u; ++c) {
-        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
-        if (!(m & kSelfFluid)) continue;
-        float ri;
-        if (first) {
-            ri = r_in[i];
-        } else {
-            pres[i] = pres[i] + alpha * (double)d[i];
-            ri = (float)((double)r_in[i] - alpha * (double)q[i]);
+    if (m0 & kSelfFluid) {
+        float ri = ra0;
+        if (!first) {
+            pres[i0] = p0 + alpha * (double)d0;
+            ri = (float)((double)ra0 - alpha * (double)q0);
         }
-        r_out[i] = ri;
+        r_out[i0] = ri;
         v[0] += (double)ri * (double)ri;
-        float dg = diag[i] + kDiagEps;
-        if (c == 0u) {
-            z[i] = ri / dg;
-        } else {
-            float s = 0.0f;
-            for (int n = 0; n < 6; ++n) {
-                if (!(m >> n & 1u)) continue;
-                const uint32_t j = nbr(B, l, n);
-                float rj = first ? r_in[j] : (float)((double)r_in[j] - alpha * (double)q[j]);
-                s += rj / (diag[j] + kDiagEps);
-            }
-            z[i] = (ri + s) / dg;
+        z[i0] = ri / (g0 + kDiagEps);
+    }
+    if (m1 & kSelfFluid) {
+        float ri = ra1;
+        if (!first) {
+            pres[i1] = p1 + alpha * (double)d1;
+            ri = (float)((double)ra1 - alpha * (double)q1);
         }
+        r_out[i1] = ri;
+        v[0] += (double)ri * (double)ri;
+        float s = 0.0f;
+        for (int n = 0; n < 6; ++n) {
+            if (!(m1 >> n & 1u)) continue;
+            const float r = first ? rj[n] : (float)((double)rj[n] - alpha * (double)qj[n]);
+            s += r / (gj[n] + kDiagEps);
+        }
+        z[i1] = (ri + s) / (g1 + kDiagEps);
     }
     if (!first) block_partial<1>(v, partials);
 }
@@ -189,25 +212,40 @@ __global__ __launch_bounds__(kThreads) void k_restrict0(const uint32_t *binfo, c
                                                         const float *diag, const float *xf, const float *bf,
                                                         PLevel L1, const SolverState *st)
 {
-    if (st->done) return;
     __shared__ float res[kBrickCells];
     __shared__ uint8_t fl[kBrickCells];
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    const uint32_t done = st->done;
+    const double mg_scale = st->mg_scale;
+    const uint32_t t = threadIdx.x;
+    uint32_t l[2], m[2];
+    float xv[2], bv[2], gv[2], xn[2][6];
     for (uint32_t c = 0; c < 2u; ++c) {
-        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
-        fl[l] = (uint8_t)((m & kSelfFluid) ? 1u : 0u);
-        if (m & kSelfFluid) {
-            float s = nbsum<float>(B, l, m, xf);
-            float dg = diag[i] + kDiagEps;
-            res[l] = bf[i] - (dg * xf[i] - s);
+        l[c] = colour_cell(t, c);
+        const uint32_t i = B.base + l[c];
+        m[c] = fmask[i];
+        xv[c] = xf[i];
+        bv[c] = bf[i];
+        gv[c] = diag[i];
+        for (int n = 0; n < 6; ++n) xn[c][n] = xf[nbr(B, l[c], n)];
+    }
+    const uint32_t ci = (B.base >> 3) + (t & 63u);
+    const uint32_t info = L1.info[ci];
+    issue_point();
+    if (done) return;
+    for (uint32_t c = 0; c < 2u; ++c) {
+        fl[l[c]] = (uint8_t)((m[c] & kSelfFluid) ? 1u : 0u);
+        if (m[c] & kSelfFluid) {
+            float s = 0.0f;
+            for (int n = 0; n < 6; ++n)
+                if (m[c] >> n & 1u) s += xn[c][n];
+            float dg = gv[c] + kDiagEps;
+            res[l[c]] = bv[c] - (dg * xv[c] - s);
         }
     }
     __syncthreads();
-    if (threadIdx.x >= 64u) return;
-    const uint32_t t = threadIdx.x;
+    if (t >= 64u) return;
     const uint32_t lx = t & 3u, ly = (t >> 2) & 3u, lz = t >> 4;
-    const uint32_t ci = (B.base >> 3) + t;
-    const uint32_t info = L1.info[ci];
     if (!(info & kInfoSelf)) {
         L1.b[ci] = 0.0f;
         L1.x[ci] = 0.0f;
@@ -219,10 +257,10 @@ __global__ __launch_bounds__(kThreads) void k_restrict0(const uint32_t *binfo, c
         uint32_t f = fx + 8u * fy + 64u * fz;
         if (fl[f]) sum += res[f];
     }
-    float bv = 0.25f * sum;
-    if (L1.b.half) bv *= (float)st->mg_scale; // FP16 levels hold the scaled cycle (the V-cycle is linear)
-    L1.b[ci] = bv;
-    L1.x[ci] = (info & kInfoRed) ? bv / info_diag(info) : 0.0f;
+    float b = 0.25f * sum;
+    if (L1.b.half) b *= (float)mg_scale; // FP16 levels hold the scaled cycle (the V-cycle is linear)
+    L1.b[ci] = b;
+    L1.x[ci] = (info & kInfoRed) ? b / info_diag(info) : 0.0f;
 }
 
 // Prolongation from pooled level 1 (x1) fused with the black sweep: red z1 = z + x1(parent), black
@@ -231,30 +269,43 @@ __global__ __launch_bounds__(kThreads) void k_prolong_black(const uint32_t *binf
                                                             const float *diag, const float *r, const float *z,
                                                             float *z1, PLevel L1, const SolverState *st)
 {
-    if (st->done) return;
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
-    const float unscale = L1.x1.half ? (float)(1.0 / st->mg_scale) : 1.0f;
-    for (uint32_t c = 0; c < 2u; ++c) {
-        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
-        if (!(m & kSelfFluid)) continue;
-        if (c == 0u) {
-            float v = z[i];
-            const uint32_t pi = (B.base >> 3) + parent1(l);
-            if (L1.info[pi] & kInfoSelf) v += L1.x1[pi] * unscale;
-            z1[i] = v;
-        } else {
-            float s = 0.0f;
-            for (int n = 0; n < 6; ++n) {
-                if (!(m >> n & 1u)) continue;
-                uint32_t nl;
-                const uint32_t j = nbr(B, l, n, nl);
-                float v = z[j];
-                const uint32_t pj = ((j - nl) >> 3) + parent1(nl);
-                if (L1.info[pj] & kInfoSelf) v += L1.x1[pj] * unscale;
-                s += v;
-            }
-            z1[i] = (r[i] + s) / (diag[i] + kDiagEps);
+    const uint32_t done = st->done;
+    const double mg_scale = st->mg_scale;
+    const uint32_t l0 = colour_cell(threadIdx.x, 0u), l1 = colour_cell(threadIdx.x, 1u);
+    const uint32_t i0 = B.base + l0, i1 = B.base + l1;
+    const uint32_t m0 = fmask[i0], m1 = fmask[i1];
+    const uint32_t pi0 = (B.base >> 3) + parent1(l0);
+    const float z0 = z[i0], r1 = r[i1], g1 = diag[i1];
+    const uint32_t pinfo0 = L1.info[pi0];
+    const float px0 = L1.x1[pi0];
+    float zn[6], pxn[6];
+    uint32_t pinfon[6];
+    for (int n = 0; n < 6; ++n) {
+        uint32_t nl;
+        const uint32_t j = nbr(B, l1, n, nl);
+        const uint32_t pj = ((j - nl) >> 3) + parent1(nl);
+        zn[n] = z[j];
+        pinfon[n] = L1.info[pj];
+        pxn[n] = L1.x1[pj];
+    }
+    issue_point();
+    if (done) return;
+    const float unscale = L1.x1.half ? (float)(1.0 / mg_scale) : 1.0f;
+    if (m0 & kSelfFluid) {
+        float v = z0;
+        if (pinfo0 & kInfoSelf) v += px0 * unscale;
+        z1[i0] = v;
+    }
+    if (m1 & kSelfFluid) {
+        float s = 0.0f;
+        for (int n = 0; n < 6; ++n) {
+            if (!(m1 >> n & 1u)) continue;
+            float v = zn[n];
+            if (pinfon[n] & kInfoSelf) v += pxn[n] * unscale;
+            s += v;
         }
+        z1[i1] = (r1 + s) / (g1 + kDiagEps);
     }
 }
 
@@ -263,23 +314,25 @@ __global__ __launch_bounds__(kThreads) void k_red_dot(const uint32_t *binfo, con
                                                       const float *r, float *z1, double *partials,
                                                       const SolverState *st)
 {
-    if (st->done) return;
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    const uint32_t done = st->done;
+    const uint32_t l0 = colour_cell(threadIdx.x, 0u), i0 = B.base + l0, i1 = B.base + colour_cell(threadIdx.x, 1u);
+    const uint32_t m0 = fmask[i0], m1 = fmask[i1];
+    const float r0 = r[i0], g0 = diag[i0], r1 = r[i1], zb1 = z1[i1];
+    float zn[6];
+    for (int n = 0; n < 6; ++n) zn[n] = z1[nbr(B, l0, n)];
+    issue_point();
+    if (done) return;
     double v[1] = {0.0};
-    {
-        const uint32_t l = colour_cell(threadIdx.x, 0u), i = B.base + l, m = fmask[i];
-        if (m & kSelfFluid) {
-            float s = nbsum<float>(B, l, m, z1);
-            float b = r[i];
-            float xv = (b + s) / (diag[i] + kDiagEps);
-            z1[i] = xv;
-            v[0] += (double)b * (double)xv;
-        }
+    if (m0 & kSelfFluid) {
+        float s = 0.0f;
+        for (int n = 0; n < 6; ++n)
+            if (m0 >> n & 1u) s += zn[n];
+        float xv = (r0 + s) / (g0 + kDiagEps);
+        z1[i0] = xv;
+        v[0] += (double)r0 * (double)xv;
     }
-    {
-        const uint32_t i = B.base + colour_cell(threadIdx.x, 1u);
-        if (fmask[i] & kSelfFluid) v[0] += (double)r[i] * (double)z1[i];
-    }
+    if (m1 & kSelfFluid) v[0] += (double)r1 * (double)zb1;
     block_partial<1>(v, partials);
 }
 
@@ -292,25 +345,42 @@ __global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, c
                                                         float *d_out, float *q, int first, double *partials,
                                                         const SolverState *st)
 {
-    if (st->done) return;
-    const double beta = first ? 0.0 : st->beta;
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    const uint32_t done = st->done;
+    const double beta = first ? 0.0 : st->beta;
+    uint32_t i[2], m[2];
+    float zc[2], dc[2] = {0.0f, 0.0f}, gc[2], zn[2][6], dn[2][6];
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t l = colour_cell(threadIdx.x, c);
+        i[c] = B.base + l;
+        m[c] = fmask[i[c]];
+        zc[c] = z1[i[c]];
+        gc[c] = diag[i[c]];
+        for (int n = 0; n < 6; ++n) zn[c][n] = z1[nbr(B, l, n)];
+    }
+    if (!first) {
+        for (uint32_t c = 0; c < 2u; ++c) {
+            const uint32_t l = colour_cell(threadIdx.x, c);
+            dc[c] = d_in[i[c]];
+            for (int n = 0; n < 6; ++n) dn[c][n] = d_in[nbr(B, l, n)];
+        }
+    }
+    issue_point();
+    if (done) return;
     double v[1] = {0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
-        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
-        if (!(m & kSelfFluid)) continue;
-        const float dv = first ? z1[i] : (float)((double)z1[i] + beta * (double)d_in[i]);
-        d_out[i] = dv;
+        if (!(m[c] & kSelfFluid)) continue;
+        const float dv = first ? zc[c] : (float)((double)zc[c] + beta * (double)dc[c]);
+        d_out[i[c]] = dv;
         double s = 0.0;
         for (int n = 0; n < 6; ++n) {
-            if (!(m >> n & 1u)) continue;
-            const uint32_t j = nbr(B, l, n);
-            const float dj = first ? z1[j] : (float)((double)z1[j] + beta * (double)d_in[j]);
+            if (!(m[c] >> n & 1u)) continue;
+            const float dj = first ? zn[c][n] : (float)((double)zn[c][n] + beta * (double)dn[c][n]);
             s += (double)dj;
         }
-        const float dg = diag[i] + kDiagEps;
+        const float dg = gc[c] + kDiagEps;
         const double qd = (double)dg * (double)dv - s;
-        q[i] = (float)qd;
+        q[i[c]] = (float)qd;
         v[0] += (double)dv * qd;
     }
     block_partial<1>(v, partials);

@@ -51,7 +51,7 @@ __device__ __forceinline__ bool pcell(const Geo &g, const PLevel &L, uint32_t na
 }
 
 // Coarse types of pooled level 1 from level 0 (children in the same brick).
-__global__ void k_ptype1(Geo g, uint32_t nactive, const uint32_t *ctype, PLevel L)
+__global__ void k_ptype1(Geo g, uint32_t nactive, const uint32_t *ctype, PLevel L, int fluid_first)
 {
     PCell c;
     if (!pcell(g, L, nactive, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
@@ -67,11 +67,11 @@ __global__ void k_ptype1(Geo g, uint32_t nactive, const uint32_t *ctype, PLevel 
         air = air || t == kCellAir;
         fluid = fluid || t == kCellFluid;
     }
-    L.type[c.idx] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
+    L.type[c.idx] = coarse_type(air, fluid, fluid_first);
 }
 
 // Coarse types of pooled level C from pooled level F = C - 1 (children in the same brick).
-__global__ void k_ptype(Geo g, uint32_t nactive, PLevel F, PLevel C)
+__global__ void k_ptype(Geo g, uint32_t nactive, PLevel F, PLevel C, int fluid_first)
 {
     PCell c;
     if (!pcell(g, C, nactive, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
@@ -88,7 +88,7 @@ __global__ void k_ptype(Geo g, uint32_t nactive, PLevel F, PLevel C)
         air = air || t == kCellAir;
         fluid = fluid || t == kCellFluid;
     }
-    C.type[c.idx] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
+    C.type[c.idx] = coarse_type(air, fluid, fluid_first);
 }
 
 // Stencil word of every allocated cell of pooled level L (after its types).
@@ -115,7 +115,7 @@ __global__ void k_pinfo(Geo g, uint32_t nactive, PLevel L)
 }
 
 // Dense level C (level 3, one cell per brick, and coarser) types from pooled level F.
-__global__ void k_dtype_from_pooled(Geo g, PLevel F, DenseLevel C)
+__global__ void k_dtype_from_pooled(Geo g, PLevel F, DenseLevel C, int fluid_first)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C.cells) return;
@@ -129,7 +129,7 @@ __global__ void k_dtype_from_pooled(Geo g, PLevel F, DenseLevel C)
         air = air || t == kCellAir;
         fluid = fluid || t == kCellFluid;
     }
-    C.type[i] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
+    C.type[i] = coarse_type(air, fluid, fluid_first);
 }
 
 // ---- levels 1 and 2: one workgroup per brick, the brick's block and a two-cell halo staged in LDS ----------
@@ -329,10 +329,11 @@ void pooled_build_type(slosh_ctx *c, int l)
     PLevel L = plevel_of(c, l);
     SLOSH_TIMED(c, K_MG_SETUP);
     if (l == 1)
-        hipLaunchKernelGGL(k_ptype1, dim3(pass_blocks(c, L)), dim3(256), 0, c->stream, c->geo, c->nactive, c->ctype, L);
+        hipLaunchKernelGGL(k_ptype1, dim3(pass_blocks(c, L)), dim3(256), 0, c->stream, c->geo, c->nactive, c->ctype, L,
+                           fluid_coarsening(c));
     else
         hipLaunchKernelGGL(k_ptype, dim3(pass_blocks(c, L)), dim3(256), 0, c->stream, c->geo, c->nactive,
-                           plevel_of(c, l - 1), L);
+                           plevel_of(c, l - 1), L, fluid_coarsening(c));
 }
 
 void pooled_build_info(slosh_ctx *c, int l)
@@ -347,7 +348,7 @@ void dense_build_type_from_pooled(slosh_ctx *c, int l)
     DenseLevel C = dlevel_of(c, l);
     SLOSH_TIMED(c, K_MG_SETUP);
     hipLaunchKernelGGL(k_dtype_from_pooled, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, c->geo,
-                       plevel_of(c, l - 1), C);
+                       plevel_of(c, l - 1), C, fluid_coarsening(c));
 }
 
 void pooled_down(slosh_ctx *c, int l)

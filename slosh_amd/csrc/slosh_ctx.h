@@ -48,6 +48,7 @@ struct Level {
     uint16_t *info;
     uint32_t *coord; // dense levels: packed cell coordinates x | y << 10 | z << 20
     void *x, *x1, *b;
+    uint32_t dense_off; // dense levels: offset of this level's words and coordinates in the contiguous blocks
 };
 
 // A coarse-level vector in FP32 or FP16 storage; arithmetic is FP32 either way (FP16 values are rounded to
@@ -239,13 +240,14 @@ struct slosh_ctx {
     // V-cycle result; binfo holds kBinfoWords per allocated brick (its 3x3x3 neighbourhood slots, brick index)
     float *r = nullptr, *r2 = nullptr, *z = nullptr, *z1 = nullptr, *d = nullptr, *d2 = nullptr, *q = nullptr;
     uint32_t *binfo = nullptr;
-    float *res3 = nullptr; // level-3 residual (dense, by brick index), restricted into level 4
     int nlevels = 1;
     slosh::Level lv[slosh::kMaxLevels]{};
     uint32_t *coarse_idx = nullptr; // [kCoarsestCells]
     uint32_t *coarse_n = nullptr;
     double *coarse_inv = nullptr;   // [kCoarsestCells^2]
     double *partials = nullptr;
+    uint16_t *dense_info = nullptr;   // stencil words of all dense levels, level by level
+    uint32_t *dense_coord = nullptr;  // their packed cell coordinates
     slosh::SolverState *state = nullptr;
     slosh::StepStats *stats = nullptr;
     // per-workgroup statistics partials, reduced once at the end of a step (no cross-workgroup atomics)
@@ -319,6 +321,10 @@ void run_rbsor(slosh_ctx *c);
 
 // ---- pooled.hip: MG levels 1 and 2 in the brick pool ----
 PLevel plevel_of(const slosh_ctx *c, int l);
+// 1 when the settings ask for fluid-first coarse cell types (SLOSH_FLAG_MG_FLUID_COARSENING).
+inline int fluid_coarsening(const slosh_ctx *c) { return (c->s.flags & SLOSH_FLAG_MG_FLUID_COARSENING) ? 1 : 0; }
+// Red-black smoothing pairs per visit of a dense coarse level (settings.mg_coarse_sweeps; 0 means 1).
+inline int coarse_sweeps(const slosh_ctx *c) { return c->s.mg_coarse_sweeps > 0 ? c->s.mg_coarse_sweeps : 1; }
 DenseLevel dlevel_of(const slosh_ctx *c, int l);
 void pooled_build_type(slosh_ctx *c, int l);
 void pooled_build_info(slosh_ctx *c, int l);

@@ -19,6 +19,21 @@ constexpr uint32_t kNoBrick = 0xFFFFFFFFu;
 constexpr uint32_t kCellAir = 0u;
 constexpr uint32_t kCellFluid = 1u;
 constexpr uint32_t kCellSolid = 2u;
+
+// Ordering point for the compiler: memory operations written before it stay before it. The solver passes issue
+// every load a thread can need (its cells and their stencil neighbours) and only then test the solver's done
+// flag, so the flag, the brick table row and the vector data arrive in one memory latency. Without
+// this point the compiler would sink the loads past the test (they are used only on the continuing path) and each
+// group of loads would wait for the one before it.
+__device__ __forceinline__ void issue_point() { __asm__ volatile("" ::: "memory"); }
+
+// Type of a coarse multigrid cell from its children (SloshPressureMG.usf:256-278, MGBuildTypeCS): AIR if any child
+// is air, else FLUID if any child is fluid, else SOLID. With fluid_first, FLUID takes precedence over AIR.
+__host__ __device__ constexpr uint8_t coarse_type(bool air, bool fluid, int fluid_first)
+{
+    return (uint8_t)(fluid_first ? (fluid ? kCellFluid : (air ? kCellAir : kCellSolid))
+                                 : (air ? kCellAir : (fluid ? kCellFluid : kCellSolid)));
+}
 constexpr uint32_t kCellTypeMask = 3u;
 constexpr uint32_t kCellDense = 4u;
 constexpr uint32_t kColliderShift = 8u;

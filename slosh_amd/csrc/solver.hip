@@ -147,7 +147,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
 
 // ---- dense levels (>= 4) ----------------------------------------------------------------------------------
 // SloshPressureMG.usf:256-278 (MGBuildTypeCS): AIR if any child is air, else FLUID if any child is fluid.
-__global__ void k_build_type(DenseLevel F, DenseLevel C)
+__global__ void k_build_type(DenseLevel F, DenseLevel C, int fluid_first)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C.cells) return;
@@ -161,7 +161,7 @@ __global__ void k_build_type(DenseLevel F, DenseLevel C)
         air = air || t == kCellAir;
         fluid = fluid || t == kCellFluid;
     }
-    C.type[i] = (uint8_t)(air ? kCellAir : (fluid ? kCellFluid : kCellSolid));
+    C.type[i] = coarse_type(air, fluid, fluid_first);
 }
 
 // Stencil word of a dense level's cells (after its types).
@@ -222,30 +222,6 @@ __device__ __forceinline__ void restrict_cell(const DenseLevel &F, const DenseLe
     C.x[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
 }
 
-// The same restriction into level 4 from level 3's residuals (dense, of dimensions fdim, zero on every cell that is
-// not fluid: adding +0 for a skipped child leaves the sum unchanged, since the sum starts at +0 and never becomes
-// -0).
-__device__ __forceinline__ void restrict_res_cell(const int fdim[3], const float *res, const DenseLevel &C, uint32_t ci)
-{
-    const uint32_t cinfo = C.info[ci];
-    if (!(cinfo & kInfoSelf)) {
-        C.b[ci] = 0.0f;
-        C.x[ci] = 0.0f;
-        return;
-    }
-    int cx, cy, cz;
-    dcoord(C, ci, cx, cy, cz);
-    float sum = 0.0f;
-    for (uint32_t oi = 0; oi < 8u; ++oi) {
-        int fx = 2 * cx + (int)(oi & 1u), fy = 2 * cy + (int)((oi >> 1) & 1u), fz = 2 * cz + (int)(oi >> 2);
-        if (fx >= fdim[0] || fy >= fdim[1] || fz >= fdim[2]) continue;
-        sum += res[fx + fdim[0] * (fy + fdim[1] * fz)];
-    }
-    const float bv = 0.25f * sum;
-    C.b[ci] = bv;
-    C.x[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
-}
-
 // SloshPressureMG.usf:418-433 (MGProlongCS)
 __device__ __forceinline__ void prolong_cell(const DenseLevel &F, const DenseLevel &C, uint32_t i)
 {
@@ -270,26 +246,13 @@ __global__ void k_drestrict(DenseLevel F, DenseLevel C, const SolverState *st)
     if (i < C.cells) restrict_cell(F, C, i);
 }
 
-__global__ void k_drestrict_res(DenseLevel F, const float *res, DenseLevel C, const SolverState *st)
+// Residual of level-3 cell i after the black sweep (0 where not fluid); a black cell's new value is stored. The
+// residual needs the black value of a neighbour that another lane or workgroup is producing: it is recomputed from
+// the red values, which this pass does not write.
+__device__ __forceinline__ float d3_residual(const DenseLevel &L, uint32_t i)
 {
-    if (st->done) return;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < C.cells) restrict_res_cell(F.dim, res, C, i);
-}
-
-// Level 3 (one cell per brick), down pass: black sweep, then the residual of every cell (0 where not fluid). The
-// residual needs the black value of a neighbour that another workgroup is producing: it is recomputed from the
-// red values, which this pass does not write.
-__global__ void k_d3down(DenseLevel L, float *res, const SolverState *st)
-{
-    if (st->done) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L.cells) return;
     const uint32_t info = L.info[i];
-    if (!(info & kInfoSelf)) {
-        res[i] = 0.0f;
-        return;
-    }
+    if (!(info & kInfoSelf)) return 0.0f;
     float xc;
     if (!(info & kInfoRed)) {
         xc = (L.b[i] + dsum(L, L.x, i, info)) / info_diag(info);
@@ -308,7 +271,38 @@ __global__ void k_d3down(DenseLevel L, float *res, const SolverState *st)
             s += L.x[j];
         }
     }
-    res[i] = L.b[i] - (info_diag(info) * xc - s);
+    return L.b[i] - (info_diag(info) * xc - s);
+}
+
+// Level 3 (one cell per brick), down pass fused with the restriction into level 4 (MGSmoothCS black, then
+// MGRestrictCS with the fused red sweep): lanes 8p .. 8p+7 of a wavefront take the eight children of level-4 cell p
+// (a child outside level 3 contributes 0), and lane 8p sums their residuals in child order from the shuffled values:
+// b_p = 1/4 of the sum, x_p = b_p / diag_p on red cells. Non-fluid children have residual 0, and adding +0 leaves
+// the sum unchanged (it starts at +0 and never becomes -0).
+__global__ __launch_bounds__(256) void k_d3down(DenseLevel L, DenseLevel C, const SolverState *st)
+{
+    if (st->done) return;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, p = gid >> 3, oi = gid & 7u;
+    float r = 0.0f;
+    if (p < C.cells) {
+        int px, py, pz;
+        dcoord(C, p, px, py, pz);
+        const int x = 2 * px + (int)(oi & 1u), y = 2 * py + (int)((oi >> 1) & 1u), z = 2 * pz + (int)(oi >> 2);
+        if (x < L.dim[0] && y < L.dim[1] && z < L.dim[2]) r = d3_residual(L, (uint32_t)(x + L.dim[0] * (y + L.dim[1] * z)));
+    }
+    const int base = (int)(threadIdx.x & 63u & ~7u);
+    float sum = 0.0f;
+    for (int k = 0; k < 8; ++k) sum += __shfl(r, base + k);
+    if (oi != 0u || p >= C.cells) return;
+    const uint32_t cinfo = C.info[p];
+    if (!(cinfo & kInfoSelf)) {
+        C.b[p] = 0.0f;
+        C.x[p] = 0.0f;
+        return;
+    }
+    const float bv = 0.25f * sum;
+    C.b[p] = bv;
+    C.x[p] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
 }
 
 // Down-sweep iterate of level-3 cell j prolonged from level 4 (a parent contributes only when it is fluid).
@@ -359,147 +353,262 @@ __global__ void k_dprolong(DenseLevel F, DenseLevel C, const SolverState *st)
 }
 
 struct TailArgs {
-    int n;          // dense levels in the tail, the last one the coarsest
-    int from_res;   // the first tail level is restricted from level 3's residuals, else its b and x are in memory
-    int fdim[3];    // level 3's dimensions
-    uint32_t off[kMaxLevels];
-    DenseLevel L[kMaxLevels];
+    int n;                   // dense levels in the tail, the last one the coarsest
+    int sweeps;              // red-black smoothing pairs per visit of a level
+    uint32_t total;          // cells of all tail levels
+    const uint16_t *info;    // stencil words of the tail's levels, contiguous from the first one
+    const uint32_t *coord;   // their packed cell coordinates
+    uint32_t off[kMaxLevels];  // first cell of each tail level in the two arrays above and in LDS
+    DenseLevel L[kMaxLevels];  // per level: dimensions and cell count; b and the red-initialised x of the first
+                               // level are in memory
 };
 
-__device__ __forceinline__ DenseLevel tview(const TailArgs &T, int k, uint16_t *si, uint32_t *sc, float *sx, float *sb)
+// Cells per thread of any tail pass (1024 threads).
+constexpr int kTailPer = (int)((kTailCells + 1023u) / 1024u);
+
+// LDS image of the tail: per cell the stencil word, packed coordinates, iterate, right-hand side, residual and
+// diagonal (info_diag of the word, summed once at staging).
+struct TailLds {
+    uint16_t *info;
+    uint32_t *coord;
+    float *x, *b, *r, *dg;
+};
+
+// A tail level as the LDS passes see it (uniform across the workgroup).
+struct TailLevel {
+    uint32_t off, cells;
+    int dx, dy, dz;
+};
+
+__device__ __forceinline__ TailLevel tail_level(const TailArgs &T, int k)
 {
-    DenseLevel V = T.L[k];
-    V.info = si + T.off[k];
-    V.coord = sc + T.off[k];
-    V.x = LVec{sx + T.off[k], 0};
-    V.b = LVec{sb + T.off[k], 0};
+    TailLevel V;
+    V.off = T.off[k];
+    V.cells = T.L[k].cells;
+    V.dx = T.L[k].dim[0];
+    V.dy = T.L[k].dim[1];
+    V.dz = T.L[k].dim[2];
     return V;
 }
 
-// The dense levels of the V-cycle (SloshProxy.cpp:904-959) in one 1024-thread workgroup, entirely in LDS (stencil
-// words, cell coordinates and both vectors): the restriction into the first tail level, its black pre-sweep,
-// restrictions with fused red sweeps and black pre-sweeps down to the coarsest level, the exact FP64 coarsest
-// solve, then prolongation with black and red post-sweeps back up. Only the first level's result goes back to
-// memory.
-__global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const float *res3, const uint32_t *cidx,
+// LDS index of neighbour n (order -x +x -y +y -z +z) of cell k, or k itself when the word has no fluid neighbour
+// there (so every load address is valid and the value is masked out).
+__device__ __forceinline__ uint32_t tail_nbr(const TailLevel &V, uint32_t k, uint32_t info, int n)
+{
+    const int s = n < 2 ? 1 : (n < 4 ? V.dx : V.dx * V.dy);
+    const int d = (n & 1) ? s : -s;
+    return (info >> n & 1u) ? (uint32_t)((int)k + d) : k;
+}
+
+// Sum over the fluid neighbours of cell k of the LDS iterate, in neighbour order.
+__device__ __forceinline__ float tail_nbsum(const TailLds &S, const TailLevel &V, uint32_t k, uint32_t info)
+{
+    float v[6];
+    for (int n = 0; n < 6; ++n) v[n] = S.x[tail_nbr(V, k, info, n)];
+    float s = 0.0f;
+    for (int n = 0; n < 6; ++n)
+        if (info >> n & 1u) s += v[n];
+    return s;
+}
+
+// One Gauss-Seidel colour sweep (SloshPressureMG.usf:311-338): new values of all of a thread's cells first, then
+// the stores (cells of one colour never neighbour each other). When rows have even length, cells 2j and 2j + 1 hold
+// one cell of each colour and thread j takes the one of this colour (no lanes idle on the other colour).
+__device__ __forceinline__ void tail_sweep(const TailLds &S, const TailLevel &V, uint32_t colour)
+{
+    if ((V.dx & 1) == 0) {
+        constexpr int kPairs = (kTailPer + 1) / 2;
+        float xp[kPairs];
+        uint32_t kp[kPairs];
+        bool wp[kPairs];
+        for (int u = 0; u < kPairs; ++u) {
+            const uint32_t j = threadIdx.x + 1024u * (uint32_t)u;
+            wp[u] = false;
+            if (2u * j >= V.cells) continue;
+            const uint32_t k0 = V.off + 2u * j, pc = S.coord[k0];
+            // x of cell 2j is even, so the cell of colour c (0 = red = even x+y+z) is 2j + ((y + z + c) & 1)
+            const uint32_t k = k0 + ((((pc >> 10) & 1023u) + (pc >> 20) + colour) & 1u);
+            const uint32_t info = S.info[k];
+            if (!(info & kInfoSelf)) continue;
+            wp[u] = true;
+            kp[u] = k;
+            xp[u] = (S.b[k] + tail_nbsum(S, V, k, info)) / S.dg[k];
+        }
+        for (int u = 0; u < kPairs; ++u)
+            if (wp[u]) S.x[kp[u]] = xp[u];
+        return;
+    }
+    float xn[kTailPer];
+    bool w[kTailPer];
+    for (int u = 0; u < kTailPer; ++u) {
+        const uint32_t i = threadIdx.x + 1024u * (uint32_t)u, k = V.off + i;
+        w[u] = false;
+        if (i >= V.cells) continue;
+        const uint32_t info = S.info[k];
+        if (!(info & kInfoSelf) || ((info & kInfoRed) ? 0u : 1u) != colour) continue;
+        w[u] = true;
+        xn[u] = (S.b[k] + tail_nbsum(S, V, k, info)) / S.dg[k];
+    }
+    for (int u = 0; u < kTailPer; ++u)
+        if (w[u]) S.x[V.off + threadIdx.x + 1024u * (uint32_t)u] = xn[u];
+}
+
+// Residual b - A x of every fluid cell of a level (0 elsewhere), the children's values of the next restriction.
+__device__ __forceinline__ void tail_residual(const TailLds &S, const TailLevel &V)
+{
+    for (int u = 0; u < kTailPer; ++u) {
+        const uint32_t i = threadIdx.x + 1024u * (uint32_t)u, k = V.off + i;
+        if (i >= V.cells) continue;
+        const uint32_t info = S.info[k];
+        float r = 0.0f;
+        if (info & kInfoSelf) r = S.b[k] - (S.dg[k] * S.x[k] - tail_nbsum(S, V, k, info));
+        S.r[k] = r;
+    }
+}
+
+// SloshPressureMG.usf:343-381 (MGRestrictCS) from the staged residuals, with the fused red sweep x_c = b_c / diag_c.
+__device__ __forceinline__ void tail_restrict(const TailLds &S, const TailLevel &F, const TailLevel &C)
+{
+    for (int u = 0; u < kTailPer; ++u) {
+        const uint32_t ci = threadIdx.x + 1024u * (uint32_t)u, kc = C.off + ci;
+        if (ci >= C.cells) continue;
+        const uint32_t cinfo = S.info[kc];
+        if (!(cinfo & kInfoSelf)) {
+            S.b[kc] = 0.0f;
+            S.x[kc] = 0.0f;
+            continue;
+        }
+        const uint32_t pc = S.coord[kc];
+        const int cx = (int)(pc & 1023u), cy = (int)((pc >> 10) & 1023u), cz = (int)(pc >> 20);
+        float rv[8];
+        bool fl[8];
+        for (uint32_t oi = 0; oi < 8u; ++oi) {
+            const int fx = 2 * cx + (int)(oi & 1u), fy = 2 * cy + (int)((oi >> 1) & 1u), fz = 2 * cz + (int)(oi >> 2);
+            const bool in = fx < F.dx && fy < F.dy && fz < F.dz;
+            const uint32_t fi = in ? F.off + (uint32_t)(fx + F.dx * (fy + F.dy * fz)) : kc;
+            rv[oi] = S.r[fi];
+            fl[oi] = in && (S.info[fi] & kInfoSelf);
+        }
+        float sum = 0.0f;
+        for (uint32_t oi = 0; oi < 8u; ++oi)
+            if (fl[oi]) sum += rv[oi];
+        const float bv = 0.25f * sum;
+        S.b[kc] = bv;
+        S.x[kc] = (cinfo & kInfoRed) ? bv / S.dg[kc] : 0.0f;
+    }
+}
+
+// SloshPressureMG.usf:418-433 (MGProlongCS): x_f += x_c(parent) for fluid cells with a fluid parent.
+__device__ __forceinline__ void tail_prolong(const TailLds &S, const TailLevel &F, const TailLevel &C)
+{
+    for (int u = 0; u < kTailPer; ++u) {
+        const uint32_t i = threadIdx.x + 1024u * (uint32_t)u, k = F.off + i;
+        if (i >= F.cells || !(S.info[k] & kInfoSelf)) continue;
+        const uint32_t pc = S.coord[k];
+        const int x = (int)(pc & 1023u), y = (int)((pc >> 10) & 1023u), z = (int)(pc >> 20);
+        const uint32_t pi = C.off + (uint32_t)((x >> 1) + C.dx * ((y >> 1) + C.dy * (z >> 1)));
+        if (S.info[pi] & kInfoSelf) S.x[k] += S.x[pi];
+    }
+}
+
+// The dense levels of the V-cycle (SloshPressureMG.usf via SloshProxy.cpp:904-959) in one 1024-thread workgroup,
+// entirely in LDS, from the first tail level's b and red-initialised x: `sweeps` red-black pairs on each level on
+// the way down (the first red sweep fused into the restriction), residuals and restrictions down to the coarsest
+// level, the exact FP64 coarsest solve, then prolongation with `sweeps` black-red pairs back up. Every global read
+// is issued at the start (solver flag, stencil words, coordinates, the coarsest inverse, the first level's inputs);
+// only the first level's result goes back to memory.
+__global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t *cidx,
                                                       const uint32_t *cn, const double *inv, const SolverState *st)
 {
-    if (st->done) return;
     constexpr int N = (int)kCoarsestCells;
-    constexpr int kPer = (int)((kTailCells + 1023u) / 1024u); // cells per thread of any tail pass
     __shared__ uint16_t s_info[kTailCells];
     __shared__ uint32_t s_coord[kTailCells];
-    __shared__ float s_x[kTailCells], s_b[kTailCells];
+    __shared__ float s_x[kTailCells], s_b[kTailCells], s_r[kTailCells], s_dg[kTailCells];
     __shared__ double bs[N];
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const TailLds S{s_info, s_coord, s_x, s_b, s_r, s_dg};
+    const uint32_t tid = threadIdx.x;
     const int m = T.n;
-    // The global reads are issued up front and all at once (one memory latency each, not one per loop trip): the
-    // coarsest level's inverse rows and cell list, then the levels' stencil words and coordinates.
+    const uint32_t done = st->done;
     const int row = (int)(tid >> 4), part = (int)(tid & 15u);
     const int n = (int)*cn;
     double invr[N / 16];
     for (int k = 0; k < N / 16; ++k) invr[k] = inv[(size_t)row * N + part + 16 * k];
     const uint32_t c_row = cidx[row], c_tid = cidx[tid & (uint32_t)(N - 1)];
-    const uint32_t total = T.off[m - 1] + T.L[m - 1].cells;
-    {
-        uint16_t vi[kPer];
-        uint32_t vc[kPer];
-        for (int u = 0; u < kPer; ++u) {
-            const uint32_t f = tid + nt * (uint32_t)u;
-            vi[u] = 0;
-            vc[u] = 0;
-            if (f < total) {
-                int k = 0;
-                while (k + 1 < m && f >= T.off[k + 1]) ++k;
-                vi[u] = T.L[k].info[f - T.off[k]];
-                vc[u] = T.L[k].coord[f - T.off[k]];
-            }
-        }
-        for (int u = 0; u < kPer; ++u) {
-            const uint32_t f = tid + nt * (uint32_t)u;
-            if (f < total) {
-                s_info[f] = vi[u];
-                s_coord[f] = vc[u];
-            }
-        }
+    uint16_t vi[kTailPer];
+    uint32_t vc[kTailPer];
+    for (int u = 0; u < kTailPer; ++u) {
+        const uint32_t f = tid + 1024u * (uint32_t)u;
+        vi[u] = f < T.total ? T.info[f] : (uint16_t)0;
+        vc[u] = f < T.total ? T.coord[f] : 0u;
+    }
+    // the first tail level's b and red-initialised x (cells tid + 1024 u)
+    const TailLevel V0 = tail_level(T, 0);
+    float gx[kTailPer], gb[kTailPer];
+    for (int u = 0; u < kTailPer; ++u) {
+        const uint32_t ci = tid + 1024u * (uint32_t)u;
+        gx[u] = gb[u] = 0.0f;
+        if (ci >= V0.cells) continue;
+        gx[u] = T.L[0].x[ci];
+        gb[u] = T.L[0].b[ci];
+    }
+    issue_point();
+    if (done) return;
+    for (int u = 0; u < kTailPer; ++u) {
+        const uint32_t f = tid + 1024u * (uint32_t)u;
+        if (f >= T.total) continue;
+        s_info[f] = vi[u];
+        s_coord[f] = vc[u];
+        s_dg[f] = info_diag(vi[u]);
+    }
+    for (int u = 0; u < kTailPer; ++u) {
+        const uint32_t ci = tid + 1024u * (uint32_t)u;
+        if (ci >= V0.cells) continue;
+        s_b[ci] = gb[u];
+        s_x[ci] = gx[u];
     }
     __syncthreads();
-    {
-        const DenseLevel V0 = tview(T, 0, s_info, s_coord, s_x, s_b);
-        if (T.from_res) {
-            // restriction from level 3's residuals (see restrict_res_cell): the 8 children of each of this
-            // thread's cells are read first, then summed in child order
-            float rv[kPer][8];
-            for (int u = 0; u < kPer; ++u) {
-                const uint32_t ci = tid + nt * (uint32_t)u;
-                for (int oi = 0; oi < 8; ++oi) rv[u][oi] = 0.0f;
-                if (ci >= V0.cells || !(V0.info[ci] & kInfoSelf)) continue;
-                int cx, cy, cz;
-                dcoord(V0, ci, cx, cy, cz);
-                for (int oi = 0; oi < 8; ++oi) {
-                    const int fx = 2 * cx + (oi & 1), fy = 2 * cy + ((oi >> 1) & 1), fz = 2 * cz + (oi >> 2);
-                    if (fx < T.fdim[0] && fy < T.fdim[1] && fz < T.fdim[2])
-                        rv[u][oi] = res3[fx + T.fdim[0] * (fy + T.fdim[1] * fz)];
-                }
-            }
-            for (int u = 0; u < kPer; ++u) {
-                const uint32_t ci = tid + nt * (uint32_t)u;
-                if (ci >= V0.cells) continue;
-                const uint32_t cinfo = V0.info[ci];
-                if (!(cinfo & kInfoSelf)) {
-                    V0.b[ci] = 0.0f;
-                    V0.x[ci] = 0.0f;
-                    continue;
-                }
-                float sum = 0.0f;
-                for (int oi = 0; oi < 8; ++oi) sum += rv[u][oi];
-                const float bv = 0.25f * sum;
-                V0.b[ci] = bv;
-                V0.x[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
-            }
-        } else {
-            for (uint32_t i = tid; i < V0.cells; i += nt) {
-                V0.x[i] = T.L[0].x[i];
-                V0.b[i] = T.L[0].b[i];
-            }
-        }
-        __syncthreads();
-        if (m > 1) {
-            for (uint32_t i = tid; i < V0.cells; i += nt) smooth_cell(V0, 1u, i);
-            __syncthreads();
-        }
-    }
     for (int l = 0; l < m - 1; ++l) {
-        const DenseLevel F = tview(T, l, s_info, s_coord, s_x, s_b), C = tview(T, l + 1, s_info, s_coord, s_x, s_b);
-        for (uint32_t ci = tid; ci < C.cells; ci += nt) restrict_cell(F, C, ci);
-        __syncthreads();
-        if (l + 1 < m - 1) {
-            for (uint32_t i = tid; i < C.cells; i += nt) smooth_cell(C, 1u, i);
+        const TailLevel F = tail_level(T, l), C = tail_level(T, l + 1);
+        for (int k = 0; k < T.sweeps; ++k) {
+            if (k > 0) {
+                tail_sweep(S, F, 0u);
+                __syncthreads();
+            }
+            tail_sweep(S, F, 1u);
             __syncthreads();
         }
+        tail_residual(S, F);
+        __syncthreads();
+        tail_restrict(S, F, C);
+        __syncthreads();
     }
     {
         // x = A^-1 b on the coarsest level: 16 lanes per row, partial sums combined by a fixed butterfly
-        const DenseLevel Vc = tview(T, m - 1, s_info, s_coord, s_x, s_b);
-        if ((int)tid < n) bs[tid] = (double)Vc.b[c_tid];
+        const TailLevel Vc = tail_level(T, m - 1);
+        if ((int)tid < n) bs[tid] = (double)s_b[Vc.off + c_tid];
         __syncthreads();
         double s = 0.0;
         for (int k = 0; k < N / 16; ++k)
             if (row < n && part + 16 * k < n) s += invr[k] * bs[part + 16 * k];
         for (int w = 8; w > 0; w >>= 1) s += __shfl_xor(s, w, 16);
-        if (row < n && part == 0) Vc.x[c_row] = (float)s;
+        if (row < n && part == 0) s_x[Vc.off + c_row] = (float)s;
         __syncthreads();
     }
     for (int l = m - 2; l >= 0; --l) {
-        const DenseLevel F = tview(T, l, s_info, s_coord, s_x, s_b), C = tview(T, l + 1, s_info, s_coord, s_x, s_b);
-        for (uint32_t i = tid; i < F.cells; i += nt) prolong_cell(F, C, i);
+        const TailLevel F = tail_level(T, l), C = tail_level(T, l + 1);
+        tail_prolong(S, F, C);
         __syncthreads();
-        for (uint32_t i = tid; i < F.cells; i += nt) smooth_cell(F, 1u, i);
-        __syncthreads();
-        for (uint32_t i = tid; i < F.cells; i += nt) smooth_cell(F, 0u, i);
-        __syncthreads();
+        for (int k = 0; k < T.sweeps; ++k) {
+            tail_sweep(S, F, 1u);
+            __syncthreads();
+            tail_sweep(S, F, 0u);
+            __syncthreads();
+        }
     }
-    for (uint32_t i = tid; i < T.L[0].cells; i += nt) T.L[0].x[i] = s_x[T.off[0] + i];
+    for (int u = 0; u < kTailPer; ++u) {
+        const uint32_t i = tid + 1024u * (uint32_t)u;
+        if (i < V0.cells) T.L[0].x[i] = s_x[i];
+    }
 }
 
 // ---- coarsest level: FP64 Cholesky of the <= 64 fluid unknowns and its explicit inverse (once per solve) ----
@@ -643,48 +752,48 @@ void coarse_solve(slosh_ctx *c, LVec b, LVec x)
                        x, c->state);
 }
 
-// Levels 3 .. m-1 (m >= 5): level 3's down pass, grid-wide passes for dense levels kFirstDense .. t-1 (only when
-// the dense levels do not all fit the tail's LDS), the tail on levels t .. m-1, the grid-wide up passes, level 3's
-// up pass.
+// Levels 3 .. m-1 (m >= 5): level 3's down pass with the restriction into level 4, grid-wide passes for dense
+// levels kFirstDense .. t-1 (only when the dense levels do not all fit the tail's LDS), the tail on levels
+// t .. m-1, the grid-wide up passes, level 3's up pass.
 void dense_part(slosh_ctx *c)
 {
     const int m = c->nlevels;
     const DenseLevel L3 = dlevel_of(c, 3);
     {
         SLOSH_TIMED(c, K_D3DOWN);
-        hipLaunchKernelGGL(k_d3down, dim3(blocks_for(L3.cells, 256)), dim3(256), 0, c->stream, L3, c->res3, c->state);
+        const DenseLevel L4 = dlevel_of(c, kFirstDense);
+        hipLaunchKernelGGL(k_d3down, dim3(blocks_for((uint64_t)L4.cells * 8u, 256)), dim3(256), 0, c->stream, L3, L4,
+                           c->state);
     }
     int t = m - 1;
     uint32_t tail_cells = c->lv[m - 1].cells;
     while (t - 1 >= kFirstDense && tail_cells + c->lv[t - 1].cells <= kTailCells) tail_cells += c->lv[--t].cells;
-    if (t > kFirstDense) {
-        DenseLevel C = dlevel_of(c, kFirstDense);
-        SLOSH_TIMED(c, K_DRESTRICT);
-        hipLaunchKernelGGL(k_drestrict_res, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, L3, c->res3, C,
-                           c->state);
-    }
+    const int sweeps = coarse_sweeps(c);
     for (int l = kFirstDense; l < t; ++l) {
         DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
-        {
-            SLOSH_TIMED(c, K_DSMOOTH);
-            hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, 1u, c->state);
-        }
+        for (int k = 0; k < sweeps; ++k)
+            for (uint32_t colour : {0u, 1u}) {
+                if (k == 0 && colour == 0u) continue; // the first red sweep is fused into the restriction
+                SLOSH_TIMED(c, K_DSMOOTH);
+                hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, colour,
+                                   c->state);
+            }
         SLOSH_TIMED(c, K_DRESTRICT);
         hipLaunchKernelGGL(k_drestrict, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
     }
     TailArgs T{};
     T.n = m - t;
-    T.from_res = t == kFirstDense ? 1 : 0;
-    for (int a = 0; a < 3; ++a) T.fdim[a] = L3.dim[a];
-    uint32_t off = 0;
+    T.sweeps = sweeps;
+    T.info = c->dense_info + c->lv[t].dense_off;
+    T.coord = c->dense_coord + c->lv[t].dense_off;
     for (int k = 0; k < T.n; ++k) {
         T.L[k] = dlevel_of(c, t + k);
-        T.off[k] = off;
-        off += T.L[k].cells;
+        T.off[k] = c->lv[t + k].dense_off - c->lv[t].dense_off; // levels are contiguous in the blocks
+        T.total = T.off[k] + T.L[k].cells;
     }
     {
         SLOSH_TIMED(c, K_COARSE_TAIL);
-        hipLaunchKernelGGL(k_coarse_tail, dim3(1), dim3(1024), 0, c->stream, T, c->res3, c->coarse_idx, c->coarse_n,
+        hipLaunchKernelGGL(k_coarse_tail, dim3(1), dim3(1024), 0, c->stream, T, c->coarse_idx, c->coarse_n,
                            c->coarse_inv, c->state);
     }
     for (int l = t - 1; l >= kFirstDense; --l) {
@@ -693,11 +802,12 @@ void dense_part(slosh_ctx *c)
             SLOSH_TIMED(c, K_DPROLONG);
             hipLaunchKernelGGL(k_dprolong, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
         }
-        for (uint32_t colour : {1u, 0u}) {
-            SLOSH_TIMED(c, K_DSMOOTH);
-            hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, colour,
-                               c->state);
-        }
+        for (int k = 0; k < sweeps; ++k)
+            for (uint32_t colour : {1u, 0u}) {
+                SLOSH_TIMED(c, K_DSMOOTH);
+                hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, colour,
+                                   c->state);
+            }
     }
     SLOSH_TIMED(c, K_D3UP);
     hipLaunchKernelGGL(k_d3up, dim3(blocks_for(L3.cells, 256)), dim3(256), 0, c->stream, L3, dlevel_of(c, kFirstDense),
@@ -738,7 +848,7 @@ void mg_setup(slosh_ctx *c)
             DenseLevel C = dlevel_of(c, l);
             SLOSH_TIMED(c, K_MG_SETUP);
             hipLaunchKernelGGL(k_build_type, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, dlevel_of(c, l - 1),
-                               C);
+                               C, fluid_coarsening(c));
         }
     }
     for (int l = 1; l < m; ++l) {

@@ -244,3 +244,36 @@ def test_mgpcg_mostly_allocated_tank():
     assert rel_linf(gpu.read_grid(abi.FIELD_VEL), orc.read_grid(abi.FIELD_VEL)) <= 1e-5
     gpu.close()
     orc.close()
+
+
+@pytest.mark.parametrize("flags,sweeps", [(0, 0), (abi.FLAG_MG_FLUID_COARSENING, 0),
+                                          (abi.FLAG_MG_FLUID_COARSENING, 3), (0, 2)])
+def test_mgpcg_preconditioner_options(flags, sweeps):
+    """MGPCG with the preconditioner options (fluid-first coarse cell types; extra red-black pairs on the coarse
+    levels 4 and coarser) on a 128 x 64 x 64 dam break, whose levels 3, 4, 5 hold 1024, 128 and 16 cells (level 4 is
+    smoothed in the coarse tail): the oracle's iteration count within 2, the FP64 solution within 1e-5 and the true
+    residual, over two steps."""
+    grid, h = (128, 64, 64), 1.0 / 64
+    cmin, dims = scenes.snap_box((0.0, 0.0, 0.0), (1.1, 0.55, 1.0), grid, h)
+    reqs = [abi.fill_request(cmin, dims, 5)]
+    s = abi.make_settings(grid, h, 8 * int(np.prod(dims)), solver=abi.SOLVER_MGPCG, tolerance=1e-6, flags=flags,
+                          mg_coarse_sweeps=sweeps)
+    print(f"case options flags={flags} sweeps={sweeps}", flush=True)
+    gpu, orc = Sim(s), O.OracleSim(s)
+    for sim in (gpu, orc):
+        for r in reqs:
+            sim.emit(r)
+    for k in range(2):
+        gpu.step(1.0 / 120.0, G)
+        orc.step(1.0 / 120.0, G)
+        rg, ro = gpu.results(), orc.results()
+        assert rg.converged == 1 and rg.residual_rel <= 1e-6, rg.as_dict()
+        assert abs(rg.solver_iterations - ro.solver_iterations) <= 2, (rg.solver_iterations, ro.solver_iterations)
+        if k == 0:
+            pg = gpu.read_grid(abi.FIELD_PRESSURE_F64)
+            ref, _, rel = orc.solve_fp64(1e-13, 600)
+            assert rel <= 1e-12
+            assert np.linalg.norm(pg - ref) / np.linalg.norm(ref) <= 1e-5
+            assert orc.true_residual(pg) <= 1.2e-6
+    gpu.close()
+    orc.close()
