@@ -164,40 +164,65 @@ template <int S> __device__ __forceinline__ float box_sum(const float *v, uint32
     return s;
 }
 
+// Workgroup size of the pooled passes: one wavefront per brick, so that every brick's workgroup is resident at once
+// (the passes move little data; their time is the latency of a few dependent loads, paid once instead of once per
+// round of workgroups).
+constexpr uint32_t kPoolThreads = 64;
+
 // Down pass of pooled level L (S = 8 >> l): black sweep, residual, restriction into pooled level C.
 template <int S>
-__global__ __launch_bounds__(256) void k_pdown_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+__global__ __launch_bounds__(kPoolThreads) void k_pdown_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
                                                    const SolverState *st)
 {
-    if (st->done) return;
     using Bx = Box<S>;
     constexpr int S2 = S / 2, CPB2 = S2 * S2 * S2;
+    constexpr int kPer = (Bx::NB + (int)kPoolThreads - 1) / (int)kPoolThreads;
     __shared__ uint32_t s_nb[28];
     __shared__ float s_x[Bx::NB], s_b[Bx::NB];
     __shared__ uint16_t s_info[Bx::NB];
     __shared__ float s_res[Bx::CPB];
     const uint32_t t = threadIdx.x;
-    if (t < 28u) s_nb[t] = binfo[(size_t)xcd_remap(blockIdx.x, gridDim.x) * kBinfoWords + t];
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    if (t < 28u) s_nb[t] = binfo[(size_t)e * kBinfoWords + t];
+    const uint32_t done = st->done;
+    // the level-2 pass restricts into level 3 (dense by brick index): its target word is known from the brick index
+    const uint32_t b3 = binfo[(size_t)e * kBinfoWords + 27];
     __syncthreads();
-    for (uint32_t k = t; k < (uint32_t)Bx::NB; k += 256u) {
+    uint32_t vi[kPer];
+    float vx[kPer], vb[kPer];
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t k = t + kPoolThreads * (uint32_t)q;
+        vi[q] = 0u;
+        vx[q] = vb[q] = 0.0f;
+        if (k >= (uint32_t)Bx::NB) continue;
         const int u = (int)(k % Bx::BD) - 2, v = (int)((k / Bx::BD) % Bx::BD) - 2, w = (int)(k / (Bx::BD * Bx::BD)) - 2;
         const uint32_t gi = Bx::index(s_nb, u, v, w);
-        const uint32_t info = L.info[gi];
-        const bool f = (info & kInfoSelf) != 0u;
-        s_info[k] = (uint16_t)info;
-        s_x[k] = f ? L.x[gi] : 0.0f;
-        s_b[k] = f ? L.b[gi] : 0.0f;
+        vi[q] = L.info[gi];
+        vx[q] = L.x[gi];
+        vb[q] = L.b[gi];
+    }
+    const uint32_t slot = s_nb[13];
+    const uint32_t ci = S == 2 ? b3 : slot * CPB2 + (t & (uint32_t)(CPB2 - 1));
+    const uint32_t cinfo = S == 2 ? D.info[ci] : C.info[ci];
+    issue_point();
+    if (done) return;
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t k = t + kPoolThreads * (uint32_t)q;
+        if (k >= (uint32_t)Bx::NB) continue;
+        const bool f = (vi[q] & kInfoSelf) != 0u;
+        s_info[k] = (uint16_t)vi[q];
+        s_x[k] = f ? vx[q] : 0.0f;
+        s_b[k] = f ? vb[q] : 0.0f;
     }
     __syncthreads();
-    for (uint32_t k = t; k < (uint32_t)(Bx::IN * Bx::IN * Bx::IN); k += 256u) {
+    for (uint32_t k = t; k < (uint32_t)(Bx::IN * Bx::IN * Bx::IN); k += kPoolThreads) {
         const int u = (int)(k % Bx::IN) - 1, v = (int)((k / Bx::IN) % Bx::IN) - 1, w = (int)(k / (Bx::IN * Bx::IN)) - 1;
         const uint32_t kb = Bx::box_of(u, v, w), info = s_info[kb];
         if (!(info & kInfoSelf) || (info & kInfoRed)) continue;
         s_x[kb] = (s_b[kb] + box_sum<S>(s_x, kb, info)) / info_diag(info);
     }
     __syncthreads();
-    const uint32_t slot = s_nb[13];
-    for (uint32_t k = t; k < (uint32_t)Bx::CPB; k += 256u) {
+    for (uint32_t k = t; k < (uint32_t)Bx::CPB; k += kPoolThreads) {
         const int u = (int)(k % S), v = (int)((k / S) % S), w = (int)(k / (S * S));
         const uint32_t kb = Bx::box_of(u, v, w), info = s_info[kb];
         float r = 0.0f;
@@ -208,16 +233,15 @@ __global__ __launch_bounds__(256) void k_pdown_lds(const uint32_t *binfo, PLevel
         s_res[k] = r;
     }
     __syncthreads();
-    for (uint32_t k = t; k < (uint32_t)CPB2; k += 256u) {
+    if (t >= (uint32_t)CPB2) return;
+    {
+        const uint32_t k = t;
         const int cx = (int)(k % S2), cy = (int)((k / S2) % S2), cz = (int)(k / (S2 * S2));
-        // level 2 restricts into level 3, stored densely by brick index
-        const uint32_t ci = S == 2 ? s_nb[27] : slot * CPB2 + k;
-        const uint32_t cinfo = S == 2 ? D.info[ci] : C.info[ci];
         const LVec cb = S == 2 ? D.b : C.b, cxv = S == 2 ? D.x : C.x;
         if (!(cinfo & kInfoSelf)) {
             cb[ci] = 0.0f;
             cxv[ci] = 0.0f;
-            continue;
+            return;
         }
         float sum = 0.0f;
         for (uint32_t oi = 0; oi < 8u; ++oi) {
@@ -232,45 +256,67 @@ __global__ __launch_bounds__(256) void k_pdown_lds(const uint32_t *binfo, PLevel
 
 // Up pass of pooled level L (S = 8 >> l) from pooled parent level C: prolongation, black sweep, red sweep.
 template <int S>
-__global__ __launch_bounds__(256) void k_pup_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+__global__ __launch_bounds__(kPoolThreads) void k_pup_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
                                                  const SolverState *st)
 {
-    if (st->done) return;
     using Bx = Box<S>;
     constexpr int S2 = S / 2, CPB2 = S2 * S2 * S2;
+    constexpr int kPer = (Bx::NB + (int)kPoolThreads - 1) / (int)kPoolThreads;
     __shared__ uint32_t s_nb[28];
     __shared__ float s_v[Bx::NB], s_b[Bx::NB];
     __shared__ uint16_t s_info[Bx::NB];
     const uint32_t t = threadIdx.x;
-    if (t < 28u) s_nb[t] = binfo[(size_t)xcd_remap(blockIdx.x, gridDim.x) * kBinfoWords + t];
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    if (t < 28u) s_nb[t] = binfo[(size_t)e * kBinfoWords + t];
+    const uint32_t done = st->done;
     __syncthreads();
-    for (uint32_t k = t; k < (uint32_t)Bx::NB; k += 256u) {
+    // every box cell's word, iterate, right-hand side and its parent's word and result, all issued at once (a
+    // cell outside the grid reads the null slot, and for level 2 its own brick's level-3 cell)
+    uint32_t vi[kPer], pinf[kPer];
+    float vx[kPer], vb[kPer], px[kPer];
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t k = t + kPoolThreads * (uint32_t)q;
+        vi[q] = pinf[q] = 0u;
+        vx[q] = vb[q] = px[q] = 0.0f;
+        if (k >= (uint32_t)Bx::NB) continue;
         const int u = (int)(k % Bx::BD) - 2, v = (int)((k / Bx::BD) % Bx::BD) - 2, w = (int)(k / (Bx::BD * Bx::BD)) - 2;
         const int ox = u < 0 ? -1 : (u >= S ? 1 : 0), oy = v < 0 ? -1 : (v >= S ? 1 : 0),
                   oz = w < 0 ? -1 : (w >= S ? 1 : 0);
         const int lu = u - ox * S, lv = v - oy * S, lw = w - oz * S;
         const uint32_t nslot = s_nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
         const uint32_t gi = nslot * (uint32_t)Bx::CPB + (uint32_t)(lu + S * (lv + S * lw));
-        const uint32_t info = L.info[gi];
-        float val = 0.0f, bv = 0.0f;
-        if (info & kInfoSelf) {
-            val = L.x[gi];
-            if (S == 2) {
-                // parent: the level-3 cell of the (allocated, in-grid) brick holding this cell
-                const uint32_t pi = (uint32_t)((int)s_nb[27] + ox + D.dim[0] * (oy + D.dim[1] * oz));
-                if (D.info[pi] & kInfoSelf) val += D.x1[pi];
-            } else {
-                const uint32_t pi = nslot * (uint32_t)CPB2 + (uint32_t)((lu >> 1) + S2 * ((lv >> 1) + S2 * (lw >> 1)));
-                if (C.info[pi] & kInfoSelf) val += C.x1[pi];
-            }
-            bv = L.b[gi];
+        vi[q] = L.info[gi];
+        vx[q] = L.x[gi];
+        vb[q] = L.b[gi];
+        if (S == 2) {
+            // parent: the level-3 cell of the (allocated, in-grid) brick holding this cell
+            const uint32_t pi =
+                nslot ? (uint32_t)((int)s_nb[27] + ox + D.dim[0] * (oy + D.dim[1] * oz)) : s_nb[27];
+            pinf[q] = D.info[pi];
+            px[q] = D.x1[pi];
+        } else {
+            const uint32_t pi = nslot * (uint32_t)CPB2 + (uint32_t)((lu >> 1) + S2 * ((lv >> 1) + S2 * (lw >> 1)));
+            pinf[q] = C.info[pi];
+            px[q] = C.x1[pi];
         }
-        s_info[k] = (uint16_t)info;
+    }
+    issue_point();
+    if (done) return;
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t k = t + kPoolThreads * (uint32_t)q;
+        if (k >= (uint32_t)Bx::NB) continue;
+        float val = 0.0f, bv = 0.0f;
+        if (vi[q] & kInfoSelf) {
+            val = vx[q];
+            if (pinf[q] & kInfoSelf) val += px[q];
+            bv = vb[q];
+        }
+        s_info[k] = (uint16_t)vi[q];
         s_v[k] = val;
         s_b[k] = bv;
     }
     __syncthreads();
-    for (uint32_t k = t; k < (uint32_t)(Bx::IN * Bx::IN * Bx::IN); k += 256u) {
+    for (uint32_t k = t; k < (uint32_t)(Bx::IN * Bx::IN * Bx::IN); k += kPoolThreads) {
         const int u = (int)(k % Bx::IN) - 1, v = (int)((k / Bx::IN) % Bx::IN) - 1, w = (int)(k / (Bx::IN * Bx::IN)) - 1;
         const uint32_t kb = Bx::box_of(u, v, w), info = s_info[kb];
         if (!(info & kInfoSelf) || (info & kInfoRed)) continue;
@@ -278,7 +324,7 @@ __global__ __launch_bounds__(256) void k_pup_lds(const uint32_t *binfo, PLevel L
     }
     __syncthreads();
     const uint32_t slot = s_nb[13];
-    for (uint32_t k = t; k < (uint32_t)Bx::CPB; k += 256u) {
+    for (uint32_t k = t; k < (uint32_t)Bx::CPB; k += kPoolThreads) {
         const int u = (int)(k % S), v = (int)((k / S) % S), w = (int)(k / (S * S));
         const uint32_t kb = Bx::box_of(u, v, w), info = s_info[kb];
         if (!(info & kInfoSelf)) continue;
@@ -358,9 +404,9 @@ void pooled_down(slosh_ctx *c, int l)
     DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
     KTimer timer(c, l == 1 ? K_PDOWN1 : K_PDOWN2);
     if (l == 1)
-        hipLaunchKernelGGL(k_pdown_lds<4>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pdown_lds<4>, dim3(c->nactive), dim3(kPoolThreads), 0, c->stream, c->binfo, L, C, D, c->state);
     else
-        hipLaunchKernelGGL(k_pdown_lds<2>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pdown_lds<2>, dim3(c->nactive), dim3(kPoolThreads), 0, c->stream, c->binfo, L, C, D, c->state);
 }
 
 void pooled_up(slosh_ctx *c, int l)
@@ -370,9 +416,9 @@ void pooled_up(slosh_ctx *c, int l)
     DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
     KTimer timer(c, l == 1 ? K_PUP1 : K_PUP2);
     if (l == 1)
-        hipLaunchKernelGGL(k_pup_lds<4>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pup_lds<4>, dim3(c->nactive), dim3(kPoolThreads), 0, c->stream, c->binfo, L, C, D, c->state);
     else
-        hipLaunchKernelGGL(k_pup_lds<2>, dim3(c->nactive), dim3(256), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pup_lds<2>, dim3(c->nactive), dim3(kPoolThreads), 0, c->stream, c->binfo, L, C, D, c->state);
 }
 
 } // namespace slosh

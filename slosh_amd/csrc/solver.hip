@@ -246,103 +246,181 @@ __global__ void k_drestrict(DenseLevel F, DenseLevel C, const SolverState *st)
     if (i < C.cells) restrict_cell(F, C, i);
 }
 
-// Residual of level-3 cell i after the black sweep (0 where not fluid); a black cell's new value is stored. The
-// residual needs the black value of a neighbour that another lane or workgroup is producing: it is recomputed from
-// the red values, which this pass does not write.
-__device__ __forceinline__ float d3_residual(const DenseLevel &L, uint32_t i)
-{
-    const uint32_t info = L.info[i];
-    if (!(info & kInfoSelf)) return 0.0f;
-    float xc;
-    if (!(info & kInfoRed)) {
-        xc = (L.b[i] + dsum(L, L.x, i, info)) / info_diag(info);
-        L.x[i] = xc;
-    } else {
-        xc = L.x[i];
-    }
-    float s = 0.0f;
-    for (int n = 0; n < 6; ++n) {
-        if (!(info >> n & 1u)) continue;
-        const uint32_t j = (uint32_t)((int)i + doff(L, n));
-        if (info & kInfoRed) {
-            const uint32_t jinfo = L.info[j];
-            s += (L.b[j] + dsum(L, L.x, j, jinfo)) / info_diag(jinfo);
-        } else {
-            s += L.x[j];
-        }
-    }
-    return L.b[i] - (info_diag(info) * xc - s);
-}
+// ---- level 3 (one cell per brick, dense by brick index): one wavefront per 4^3 tile, the tile and a two-cell halo
+// staged in LDS (the passes move little data; all of a tile's loads are issued at once, addresses from arithmetic).
+constexpr int kD3T = 4;                       // tile edge
+constexpr int kD3B = kD3T + 4;                // box edge: the tile and a two-cell halo
+constexpr int kD3NB = kD3B * kD3B * kD3B;     // box cells
+constexpr int kD3In = kD3T + 2;               // the tile and a one-cell halo
+constexpr uint32_t kD3Threads = 64;
+constexpr int kD3Per = kD3NB / (int)kD3Threads;
 
-// Level 3 (one cell per brick), down pass fused with the restriction into level 4 (MGSmoothCS black, then
-// MGRestrictCS with the fused red sweep): lanes 8p .. 8p+7 of a wavefront take the eight children of level-4 cell p
-// (a child outside level 3 contributes 0), and lane 8p sums their residuals in child order from the shuffled values:
-// b_p = 1/4 of the sum, x_p = b_p / diag_p on red cells. Non-fluid children have residual 0, and adding +0 leaves
-// the sum unchanged (it starts at +0 and never becomes -0).
-__global__ __launch_bounds__(256) void k_d3down(DenseLevel L, DenseLevel C, const SolverState *st)
-{
-    if (st->done) return;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, p = gid >> 3, oi = gid & 7u;
-    float r = 0.0f;
-    if (p < C.cells) {
-        int px, py, pz;
-        dcoord(C, p, px, py, pz);
-        const int x = 2 * px + (int)(oi & 1u), y = 2 * py + (int)((oi >> 1) & 1u), z = 2 * pz + (int)(oi >> 2);
-        if (x < L.dim[0] && y < L.dim[1] && z < L.dim[2]) r = d3_residual(L, (uint32_t)(x + L.dim[0] * (y + L.dim[1] * z)));
-    }
-    const int base = (int)(threadIdx.x & 63u & ~7u);
-    float sum = 0.0f;
-    for (int k = 0; k < 8; ++k) sum += __shfl(r, base + k);
-    if (oi != 0u || p >= C.cells) return;
-    const uint32_t cinfo = C.info[p];
-    if (!(cinfo & kInfoSelf)) {
-        C.b[p] = 0.0f;
-        C.x[p] = 0.0f;
-        return;
-    }
-    const float bv = 0.25f * sum;
-    C.b[p] = bv;
-    C.x[p] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
-}
+__device__ __forceinline__ int d3_box(int u, int v, int w) { return (u + 2) + kD3B * ((v + 2) + kD3B * (w + 2)); }
 
-// Down-sweep iterate of level-3 cell j prolonged from level 4 (a parent contributes only when it is fluid).
-__device__ __forceinline__ float d3_prolonged(const DenseLevel &L, const DenseLevel &P, uint32_t j)
-{
-    float v = L.x[j];
-    int x, y, z;
-    dcoord(L, j, x, y, z);
-    const uint32_t pi = (uint32_t)((x >> 1) + P.dim[0] * ((y >> 1) + P.dim[1] * (z >> 1)));
-    if (P.info[pi] & kInfoSelf) v += P.x[pi];
-    return v;
-}
-
-__device__ __forceinline__ float d3_black_up(const DenseLevel &L, const DenseLevel &P, uint32_t j, uint32_t info)
+__device__ __forceinline__ float d3_box_sum(const float *v, int k, uint32_t info)
 {
     float s = 0.0f;
     for (int n = 0; n < 6; ++n)
-        if (info >> n & 1u) s += d3_prolonged(L, P, (uint32_t)((int)j + doff(L, n)));
-    return (L.b[j] + s) / info_diag(info);
+        if (info >> n & 1u) s += v[k + ((n < 2) ? 1 : (n < 4 ? kD3B : kD3B * kD3B)) * ((n & 1) ? 1 : -1)];
+    return s;
 }
 
-// Level 3, up pass from level 4: prolongation, black sweep, red sweep; result in x1.
-__global__ void k_d3up(DenseLevel L, DenseLevel P, const SolverState *st)
+// Level-3 cell of box cell k of tile (tx, ty, tz): its coordinates and whether it lies in the level.
+__device__ __forceinline__ bool d3_cell(const DenseLevel &L, int tx, int ty, int tz, int k, int &x, int &y, int &z)
 {
-    if (st->done) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L.cells) return;
-    const uint32_t info = L.info[i];
-    if (!(info & kInfoSelf)) return;
-    if (!(info & kInfoRed)) {
-        L.x1[i] = d3_black_up(L, P, i, info);
+    x = tx * kD3T + (k % kD3B) - 2;
+    y = ty * kD3T + ((k / kD3B) % kD3B) - 2;
+    z = tz * kD3T + k / (kD3B * kD3B) - 2;
+    return x >= 0 && y >= 0 && z >= 0 && x < L.dim[0] && y < L.dim[1] && z < L.dim[2];
+}
+
+__device__ __forceinline__ void d3_tile(const DenseLevel &L, int &tx, int &ty, int &tz)
+{
+    const int ntx = (L.dim[0] + kD3T - 1) / kD3T, nty = (L.dim[1] + kD3T - 1) / kD3T;
+    tx = (int)blockIdx.x % ntx;
+    ty = ((int)blockIdx.x / ntx) % nty;
+    tz = (int)blockIdx.x / (ntx * nty);
+}
+
+// Level 3, down pass fused with the restriction into level 4 (MGSmoothCS black, then MGRestrictCS with the fused red
+// sweep): the black sweep on the tile and its one-cell halo (from the staged red values), the residual of every
+// tile cell (0 where not fluid), then b_p = 1/4 of the children's residuals summed in child order for the tile's 2^3
+// level-4 cells and x_p = b_p / diag_p on red ones. Black values of the tile go back to memory.
+__global__ __launch_bounds__(kD3Threads) void k_d3down(DenseLevel L, DenseLevel C, const SolverState *st)
+{
+    __shared__ float s_x[kD3NB], s_b[kD3NB];
+    __shared__ uint16_t s_info[kD3NB];
+    __shared__ float s_res[kD3T * kD3T * kD3T];
+    int tx, ty, tz;
+    d3_tile(L, tx, ty, tz);
+    const uint32_t t = threadIdx.x;
+    const uint32_t done = st->done;
+    uint32_t vi[kD3Per];
+    float vx[kD3Per], vb[kD3Per];
+    for (int q = 0; q < kD3Per; ++q) {
+        int x, y, z;
+        const bool in = d3_cell(L, tx, ty, tz, (int)t + (int)kD3Threads * q, x, y, z);
+        const uint32_t gi = in ? (uint32_t)(x + L.dim[0] * (y + L.dim[1] * z)) : 0u;
+        vi[q] = in ? (uint32_t)L.info[gi] : 0u;
+        vx[q] = L.x[gi];
+        vb[q] = L.b[gi];
+    }
+    // the tile's level-4 cells: thread t < 8 takes child block t
+    const int cx = 2 * tx + (int)(t & 1u), cy = 2 * ty + (int)((t >> 1) & 1u), cz = 2 * tz + (int)((t >> 2) & 1u);
+    const bool cin = t < 8u && cx < C.dim[0] && cy < C.dim[1] && cz < C.dim[2];
+    const uint32_t ci = cin ? (uint32_t)(cx + C.dim[0] * (cy + C.dim[1] * cz)) : 0u;
+    const uint32_t cinfo = C.info[ci];
+    issue_point();
+    if (done) return;
+    for (int q = 0; q < kD3Per; ++q) {
+        const int k = (int)t + (int)kD3Threads * q;
+        const bool f = (vi[q] & kInfoSelf) != 0u;
+        s_info[k] = (uint16_t)vi[q];
+        s_x[k] = f ? vx[q] : 0.0f;
+        s_b[k] = f ? vb[q] : 0.0f;
+    }
+    __syncthreads();
+    for (int k = (int)t; k < kD3In * kD3In * kD3In; k += (int)kD3Threads) {
+        const int kb = d3_box(k % kD3In - 1, (k / kD3In) % kD3In - 1, k / (kD3In * kD3In) - 1);
+        const uint32_t info = s_info[kb];
+        if (!(info & kInfoSelf) || (info & kInfoRed)) continue;
+        s_x[kb] = (s_b[kb] + d3_box_sum(s_x, kb, info)) / info_diag(info);
+    }
+    __syncthreads();
+    {
+        const int u = (int)(t % kD3T), v = (int)((t / kD3T) % kD3T), w = (int)(t / (kD3T * kD3T));
+        const int kb = d3_box(u, v, w);
+        const uint32_t info = s_info[kb];
+        float r = 0.0f;
+        if (info & kInfoSelf) {
+            if (!(info & kInfoRed)) {
+                const int x = tx * kD3T + u, y = ty * kD3T + v, z = tz * kD3T + w;
+                L.x[(uint32_t)(x + L.dim[0] * (y + L.dim[1] * z))] = s_x[kb];
+            }
+            r = s_b[kb] - (info_diag(info) * s_x[kb] - d3_box_sum(s_x, kb, info));
+        }
+        s_res[t] = r;
+    }
+    __syncthreads();
+    if (!cin) return;
+    if (!(cinfo & kInfoSelf)) {
+        C.b[ci] = 0.0f;
+        C.x[ci] = 0.0f;
         return;
     }
-    float s = 0.0f;
-    for (int n = 0; n < 6; ++n) {
-        if (!(info >> n & 1u)) continue;
-        const uint32_t j = (uint32_t)((int)i + doff(L, n));
-        s += d3_black_up(L, P, j, L.info[j]);
+    // children in child order; one outside level 3 has residual 0 (adding +0 leaves the sum, which starts at +0 and
+    // never becomes -0, unchanged)
+    float sum = 0.0f;
+    for (uint32_t oi = 0; oi < 8u; ++oi) {
+        const int u = 2 * (int)(t & 1u) + (int)(oi & 1u), v = 2 * (int)((t >> 1) & 1u) + (int)((oi >> 1) & 1u),
+                  w = 2 * (int)((t >> 2) & 1u) + (int)(oi >> 2);
+        sum += s_res[u + kD3T * (v + kD3T * w)];
     }
-    L.x1[i] = (L.b[i] + s) / info_diag(info);
+    const float bv = 0.25f * sum;
+    C.b[ci] = bv;
+    C.x[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
+}
+
+// Level 3, up pass from level 4 (MGProlongCS, then MGSmoothCS black and red): the prolonged down-sweep iterate
+// x + x4(parent) (a parent contributes only when it is fluid) on the box, the black sweep on the tile and its
+// one-cell halo, the red sweep on the tile; result in x1.
+__global__ __launch_bounds__(kD3Threads) void k_d3up(DenseLevel L, DenseLevel P, const SolverState *st)
+{
+    __shared__ float s_v[kD3NB], s_b[kD3NB];
+    __shared__ uint16_t s_info[kD3NB];
+    int tx, ty, tz;
+    d3_tile(L, tx, ty, tz);
+    const uint32_t t = threadIdx.x;
+    const uint32_t done = st->done;
+    uint32_t vi[kD3Per], pinf[kD3Per];
+    float vx[kD3Per], vb[kD3Per], px[kD3Per];
+    for (int q = 0; q < kD3Per; ++q) {
+        int x, y, z;
+        const bool in = d3_cell(L, tx, ty, tz, (int)t + (int)kD3Threads * q, x, y, z);
+        const uint32_t gi = in ? (uint32_t)(x + L.dim[0] * (y + L.dim[1] * z)) : 0u;
+        const uint32_t pi = in ? (uint32_t)((x >> 1) + P.dim[0] * ((y >> 1) + P.dim[1] * (z >> 1))) : 0u;
+        vi[q] = in ? (uint32_t)L.info[gi] : 0u;
+        vx[q] = L.x[gi];
+        vb[q] = L.b[gi];
+        pinf[q] = P.info[pi];
+        px[q] = P.x[pi];
+    }
+    issue_point();
+    if (done) return;
+    for (int q = 0; q < kD3Per; ++q) {
+        const int k = (int)t + (int)kD3Threads * q;
+        float val = 0.0f, bv = 0.0f;
+        if (vi[q] & kInfoSelf) {
+            val = vx[q];
+            if (pinf[q] & kInfoSelf) val += px[q];
+            bv = vb[q];
+        }
+        s_info[k] = (uint16_t)vi[q];
+        s_v[k] = val;
+        s_b[k] = bv;
+    }
+    __syncthreads();
+    for (int k = (int)t; k < kD3In * kD3In * kD3In; k += (int)kD3Threads) {
+        const int kb = d3_box(k % kD3In - 1, (k / kD3In) % kD3In - 1, k / (kD3In * kD3In) - 1);
+        const uint32_t info = s_info[kb];
+        if (!(info & kInfoSelf) || (info & kInfoRed)) continue;
+        s_v[kb] = (s_b[kb] + d3_box_sum(s_v, kb, info)) / info_diag(info);
+    }
+    __syncthreads();
+    const int u = (int)(t % kD3T), v = (int)((t / kD3T) % kD3T), w = (int)(t / (kD3T * kD3T));
+    const int x = tx * kD3T + u, y = ty * kD3T + v, z = tz * kD3T + w;
+    if (x >= L.dim[0] || y >= L.dim[1] || z >= L.dim[2]) return;
+    const int kb = d3_box(u, v, w);
+    const uint32_t info = s_info[kb];
+    if (!(info & kInfoSelf)) return;
+    L.x1[(uint32_t)(x + L.dim[0] * (y + L.dim[1] * z))] =
+        (info & kInfoRed) ? (s_b[kb] + d3_box_sum(s_v, kb, info)) / info_diag(info) : s_v[kb];
+}
+
+// Tiles of the level-3 passes.
+uint32_t d3_tiles(const DenseLevel &L)
+{
+    return (uint32_t)(((L.dim[0] + kD3T - 1) / kD3T) * ((L.dim[1] + kD3T - 1) / kD3T) * ((L.dim[2] + kD3T - 1) / kD3T));
 }
 
 __global__ void k_dprolong(DenseLevel F, DenseLevel C, const SolverState *st)
@@ -762,8 +840,7 @@ void dense_part(slosh_ctx *c)
     {
         SLOSH_TIMED(c, K_D3DOWN);
         const DenseLevel L4 = dlevel_of(c, kFirstDense);
-        hipLaunchKernelGGL(k_d3down, dim3(blocks_for((uint64_t)L4.cells * 8u, 256)), dim3(256), 0, c->stream, L3, L4,
-                           c->state);
+        hipLaunchKernelGGL(k_d3down, dim3(d3_tiles(L3)), dim3(kD3Threads), 0, c->stream, L3, L4, c->state);
     }
     int t = m - 1;
     uint32_t tail_cells = c->lv[m - 1].cells;
@@ -810,7 +887,7 @@ void dense_part(slosh_ctx *c)
             }
     }
     SLOSH_TIMED(c, K_D3UP);
-    hipLaunchKernelGGL(k_d3up, dim3(blocks_for(L3.cells, 256)), dim3(256), 0, c->stream, L3, dlevel_of(c, kFirstDense),
+    hipLaunchKernelGGL(k_d3up, dim3(d3_tiles(L3)), dim3(kD3Threads), 0, c->stream, L3, dlevel_of(c, kFirstDense),
                        c->state);
 }
 
