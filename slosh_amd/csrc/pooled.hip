@@ -164,18 +164,20 @@ template <int S> __device__ __forceinline__ float box_sum(const float *v, uint32
     return s;
 }
 
-// Workgroup size of the pooled passes: one wavefront per brick, so that every brick's workgroup is resident at once
-// (the passes move little data; their time is the latency of a few dependent loads, paid once instead of once per
-// round of workgroups).
-constexpr uint32_t kPoolThreads = 64;
+// Workgroup size of the pooled passes. Level 2 (S = 2, a 6^3 box per brick) takes one wavefront per brick, so that
+// every brick's workgroup is resident at once: the pass moves little data and its time is the latency of a few
+// dependent loads, paid once instead of once per round of workgroups. Level 1 (an 8^3 box) has enough work per brick
+// that four wavefronts finish it sooner.
+template <int S> constexpr uint32_t pool_threads() { return S == 4 ? 256u : 64u; }
 
 // Down pass of pooled level L (S = 8 >> l): black sweep, residual, restriction into pooled level C.
 template <int S>
-__global__ __launch_bounds__(kPoolThreads) void k_pdown_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+__global__ __launch_bounds__(pool_threads<S>()) void k_pdown_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
                                                    const SolverState *st)
 {
     using Bx = Box<S>;
     constexpr int S2 = S / 2, CPB2 = S2 * S2 * S2;
+    constexpr uint32_t kPoolThreads = pool_threads<S>();
     constexpr int kPer = (Bx::NB + (int)kPoolThreads - 1) / (int)kPoolThreads;
     __shared__ uint32_t s_nb[28];
     __shared__ float s_x[Bx::NB], s_b[Bx::NB];
@@ -256,11 +258,12 @@ __global__ __launch_bounds__(kPoolThreads) void k_pdown_lds(const uint32_t *binf
 
 // Up pass of pooled level L (S = 8 >> l) from pooled parent level C: prolongation, black sweep, red sweep.
 template <int S>
-__global__ __launch_bounds__(kPoolThreads) void k_pup_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+__global__ __launch_bounds__(pool_threads<S>()) void k_pup_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
                                                  const SolverState *st)
 {
     using Bx = Box<S>;
     constexpr int S2 = S / 2, CPB2 = S2 * S2 * S2;
+    constexpr uint32_t kPoolThreads = pool_threads<S>();
     constexpr int kPer = (Bx::NB + (int)kPoolThreads - 1) / (int)kPoolThreads;
     __shared__ uint32_t s_nb[28];
     __shared__ float s_v[Bx::NB], s_b[Bx::NB];
@@ -404,9 +407,9 @@ void pooled_down(slosh_ctx *c, int l)
     DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
     KTimer timer(c, l == 1 ? K_PDOWN1 : K_PDOWN2);
     if (l == 1)
-        hipLaunchKernelGGL(k_pdown_lds<4>, dim3(c->nactive), dim3(kPoolThreads), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pdown_lds<4>, dim3(c->nactive), dim3(pool_threads<4>()), 0, c->stream, c->binfo, L, C, D, c->state);
     else
-        hipLaunchKernelGGL(k_pdown_lds<2>, dim3(c->nactive), dim3(kPoolThreads), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pdown_lds<2>, dim3(c->nactive), dim3(pool_threads<2>()), 0, c->stream, c->binfo, L, C, D, c->state);
 }
 
 void pooled_up(slosh_ctx *c, int l)
@@ -416,9 +419,9 @@ void pooled_up(slosh_ctx *c, int l)
     DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
     KTimer timer(c, l == 1 ? K_PUP1 : K_PUP2);
     if (l == 1)
-        hipLaunchKernelGGL(k_pup_lds<4>, dim3(c->nactive), dim3(kPoolThreads), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pup_lds<4>, dim3(c->nactive), dim3(pool_threads<4>()), 0, c->stream, c->binfo, L, C, D, c->state);
     else
-        hipLaunchKernelGGL(k_pup_lds<2>, dim3(c->nactive), dim3(kPoolThreads), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pup_lds<2>, dim3(c->nactive), dim3(pool_threads<2>()), 0, c->stream, c->binfo, L, C, D, c->state);
 }
 
 } // namespace slosh

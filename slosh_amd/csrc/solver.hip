@@ -445,17 +445,23 @@ struct TailArgs {
 constexpr int kTailPer = (int)((kTailCells + 1023u) / 1024u);
 
 // LDS image of the tail: per cell the stencil word, packed coordinates, iterate, right-hand side, residual and
-// diagonal (info_diag of the word, summed once at staging).
+// diagonal (info_diag of the word, summed once at staging); per level the LDS indices of its fluid cells, red ones
+// from the front of the level's segment of `list`, black ones from its back, and their counts cnt[2 level + colour].
+// The passes visit fluid cells only. Which thread takes which cell does not matter: a colour sweep reads only the
+// other colour, and every other pass writes each output from values it does not write.
 struct TailLds {
     uint16_t *info;
     uint32_t *coord;
     float *x, *b, *r, *dg;
+    uint16_t *list;
+    uint32_t *cnt;
 };
 
 // A tail level as the LDS passes see it (uniform across the workgroup).
 struct TailLevel {
     uint32_t off, cells;
     int dx, dy, dz;
+    int k; // index in the tail
 };
 
 __device__ __forceinline__ TailLevel tail_level(const TailArgs &T, int k)
@@ -466,7 +472,14 @@ __device__ __forceinline__ TailLevel tail_level(const TailArgs &T, int k)
     V.dx = T.L[k].dim[0];
     V.dy = T.L[k].dim[1];
     V.dz = T.L[k].dim[2];
+    V.k = k;
     return V;
+}
+
+// LDS index of the j-th fluid cell of colour c (0 red, 1 black) of level V.
+__device__ __forceinline__ uint32_t tail_cell(const TailLds &S, const TailLevel &V, uint32_t c, uint32_t j)
+{
+    return S.list[c ? V.off + V.cells - 1u - j : V.off + j];
 }
 
 // LDS index of neighbour n (order -x +x -y +y -z +z) of cell k, or k itself when the word has no fluid neighbour
@@ -489,74 +502,34 @@ __device__ __forceinline__ float tail_nbsum(const TailLds &S, const TailLevel &V
     return s;
 }
 
-// One Gauss-Seidel colour sweep (SloshPressureMG.usf:311-338): new values of all of a thread's cells first, then
-// the stores (cells of one colour never neighbour each other). When rows have even length, cells 2j and 2j + 1 hold
-// one cell of each colour and thread j takes the one of this colour (no lanes idle on the other colour).
+// One Gauss-Seidel colour sweep (SloshPressureMG.usf:311-338) over the level's fluid cells of that colour.
 __device__ __forceinline__ void tail_sweep(const TailLds &S, const TailLevel &V, uint32_t colour)
 {
-    if ((V.dx & 1) == 0) {
-        constexpr int kPairs = (kTailPer + 1) / 2;
-        float xp[kPairs];
-        uint32_t kp[kPairs];
-        bool wp[kPairs];
-        for (int u = 0; u < kPairs; ++u) {
-            const uint32_t j = threadIdx.x + 1024u * (uint32_t)u;
-            wp[u] = false;
-            if (2u * j >= V.cells) continue;
-            const uint32_t k0 = V.off + 2u * j, pc = S.coord[k0];
-            // x of cell 2j is even, so the cell of colour c (0 = red = even x+y+z) is 2j + ((y + z + c) & 1)
-            const uint32_t k = k0 + ((((pc >> 10) & 1023u) + (pc >> 20) + colour) & 1u);
-            const uint32_t info = S.info[k];
-            if (!(info & kInfoSelf)) continue;
-            wp[u] = true;
-            kp[u] = k;
-            xp[u] = (S.b[k] + tail_nbsum(S, V, k, info)) / S.dg[k];
-        }
-        for (int u = 0; u < kPairs; ++u)
-            if (wp[u]) S.x[kp[u]] = xp[u];
-        return;
+    const uint32_t n = S.cnt[2 * V.k + (int)colour];
+    for (uint32_t j = threadIdx.x; j < n; j += 1024u) {
+        const uint32_t k = tail_cell(S, V, colour, j), info = S.info[k];
+        S.x[k] = (S.b[k] + tail_nbsum(S, V, k, info)) / S.dg[k];
     }
-    float xn[kTailPer];
-    bool w[kTailPer];
-    for (int u = 0; u < kTailPer; ++u) {
-        const uint32_t i = threadIdx.x + 1024u * (uint32_t)u, k = V.off + i;
-        w[u] = false;
-        if (i >= V.cells) continue;
-        const uint32_t info = S.info[k];
-        if (!(info & kInfoSelf) || ((info & kInfoRed) ? 0u : 1u) != colour) continue;
-        w[u] = true;
-        xn[u] = (S.b[k] + tail_nbsum(S, V, k, info)) / S.dg[k];
-    }
-    for (int u = 0; u < kTailPer; ++u)
-        if (w[u]) S.x[V.off + threadIdx.x + 1024u * (uint32_t)u] = xn[u];
 }
 
-// Residual b - A x of every fluid cell of a level (0 elsewhere), the children's values of the next restriction.
+// Residual b - A x of the level's fluid cells, the children's values of the next restriction.
 __device__ __forceinline__ void tail_residual(const TailLds &S, const TailLevel &V)
 {
-    for (int u = 0; u < kTailPer; ++u) {
-        const uint32_t i = threadIdx.x + 1024u * (uint32_t)u, k = V.off + i;
-        if (i >= V.cells) continue;
-        const uint32_t info = S.info[k];
-        float r = 0.0f;
-        if (info & kInfoSelf) r = S.b[k] - (S.dg[k] * S.x[k] - tail_nbsum(S, V, k, info));
-        S.r[k] = r;
+    const uint32_t n0 = S.cnt[2 * V.k], n = n0 + S.cnt[2 * V.k + 1];
+    for (uint32_t j = threadIdx.x; j < n; j += 1024u) {
+        const uint32_t k = j < n0 ? tail_cell(S, V, 0u, j) : tail_cell(S, V, 1u, j - n0), info = S.info[k];
+        S.r[k] = S.b[k] - (S.dg[k] * S.x[k] - tail_nbsum(S, V, k, info));
     }
 }
 
-// SloshPressureMG.usf:343-381 (MGRestrictCS) from the staged residuals, with the fused red sweep x_c = b_c / diag_c.
+// SloshPressureMG.usf:343-381 (MGRestrictCS) from the staged residuals of the fluid children, with the fused red
+// sweep x_c = b_c / diag_c, for the coarse level's fluid cells (the values of other cells are never read).
 __device__ __forceinline__ void tail_restrict(const TailLds &S, const TailLevel &F, const TailLevel &C)
 {
-    for (int u = 0; u < kTailPer; ++u) {
-        const uint32_t ci = threadIdx.x + 1024u * (uint32_t)u, kc = C.off + ci;
-        if (ci >= C.cells) continue;
-        const uint32_t cinfo = S.info[kc];
-        if (!(cinfo & kInfoSelf)) {
-            S.b[kc] = 0.0f;
-            S.x[kc] = 0.0f;
-            continue;
-        }
-        const uint32_t pc = S.coord[kc];
+    const uint32_t n0 = S.cnt[2 * C.k], n = n0 + S.cnt[2 * C.k + 1];
+    for (uint32_t j = threadIdx.x; j < n; j += 1024u) {
+        const uint32_t kc = j < n0 ? tail_cell(S, C, 0u, j) : tail_cell(S, C, 1u, j - n0);
+        const uint32_t cinfo = S.info[kc], pc = S.coord[kc];
         const int cx = (int)(pc & 1023u), cy = (int)((pc >> 10) & 1023u), cz = (int)(pc >> 20);
         float rv[8];
         bool fl[8];
@@ -576,13 +549,12 @@ __device__ __forceinline__ void tail_restrict(const TailLds &S, const TailLevel 
     }
 }
 
-// SloshPressureMG.usf:418-433 (MGProlongCS): x_f += x_c(parent) for fluid cells with a fluid parent.
+// SloshPressureMG.usf:418-433 (MGProlongCS): x_f += x_c(parent) for the fluid cells with a fluid parent.
 __device__ __forceinline__ void tail_prolong(const TailLds &S, const TailLevel &F, const TailLevel &C)
 {
-    for (int u = 0; u < kTailPer; ++u) {
-        const uint32_t i = threadIdx.x + 1024u * (uint32_t)u, k = F.off + i;
-        if (i >= F.cells || !(S.info[k] & kInfoSelf)) continue;
-        const uint32_t pc = S.coord[k];
+    const uint32_t n0 = S.cnt[2 * F.k], n = n0 + S.cnt[2 * F.k + 1];
+    for (uint32_t j = threadIdx.x; j < n; j += 1024u) {
+        const uint32_t k = j < n0 ? tail_cell(S, F, 0u, j) : tail_cell(S, F, 1u, j - n0), pc = S.coord[k];
         const int x = (int)(pc & 1023u), y = (int)((pc >> 10) & 1023u), z = (int)(pc >> 20);
         const uint32_t pi = C.off + (uint32_t)((x >> 1) + C.dx * ((y >> 1) + C.dy * (z >> 1)));
         if (S.info[pi] & kInfoSelf) S.x[k] += S.x[pi];
@@ -594,7 +566,8 @@ __device__ __forceinline__ void tail_prolong(const TailLds &S, const TailLevel &
 // the way down (the first red sweep fused into the restriction), residuals and restrictions down to the coarsest
 // level, the exact FP64 coarsest solve, then prolongation with `sweeps` black-red pairs back up. Every global read
 // is issued at the start (solver flag, stencil words, coordinates, the coarsest inverse, the first level's inputs);
-// only the first level's result goes back to memory.
+// the passes then visit only fluid cells, from per-level lists built once; only the first level's result goes back
+// to memory.
 __global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t *cidx,
                                                       const uint32_t *cn, const double *inv, const SolverState *st)
 {
@@ -602,8 +575,10 @@ __global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t
     __shared__ uint16_t s_info[kTailCells];
     __shared__ uint32_t s_coord[kTailCells];
     __shared__ float s_x[kTailCells], s_b[kTailCells], s_r[kTailCells], s_dg[kTailCells];
+    __shared__ uint16_t s_list[kTailCells];
+    __shared__ uint32_t s_cnt[2 * kMaxLevels];
     __shared__ double bs[N];
-    const TailLds S{s_info, s_coord, s_x, s_b, s_r, s_dg};
+    const TailLds S{s_info, s_coord, s_x, s_b, s_r, s_dg, s_list, s_cnt};
     const uint32_t tid = threadIdx.x;
     const int m = T.n;
     const uint32_t done = st->done;
@@ -631,6 +606,7 @@ __global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t
     }
     issue_point();
     if (done) return;
+    if (tid < 2u * kMaxLevels) s_cnt[tid] = 0u;
     for (int u = 0; u < kTailPer; ++u) {
         const uint32_t f = tid + 1024u * (uint32_t)u;
         if (f >= T.total) continue;
@@ -643,6 +619,17 @@ __global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t
         if (ci >= V0.cells) continue;
         s_b[ci] = gb[u];
         s_x[ci] = gx[u];
+    }
+    __syncthreads();
+    // fluid cells of each level and colour (the order within a list does not change any result)
+    for (int l = 0; l < m; ++l) {
+        const TailLevel V = tail_level(T, l);
+        for (uint32_t i = tid; i < V.cells; i += 1024u) {
+            const uint32_t f = V.off + i, info = s_info[f];
+            if (!(info & kInfoSelf)) continue;
+            if (info & kInfoRed) s_list[V.off + atomicAdd(&s_cnt[2 * l], 1u)] = (uint16_t)f;
+            else s_list[V.off + V.cells - 1u - atomicAdd(&s_cnt[2 * l + 1], 1u)] = (uint16_t)f;
+        }
     }
     __syncthreads();
     for (int l = 0; l < m - 1; ++l) {
