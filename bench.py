@@ -138,7 +138,8 @@ def bench_other(args):
         grid, h, reqs = scenes.spheres_pool(512, seed=2)
         n = sum(int(r.cell_dims[0]) * int(r.cell_dims[1]) * int(r.cell_dims[2]) * 8 for r in reqs)
         sim = Sim(abi.make_settings(grid, h, n, solver=abi.SOLVER_MGPCG, tolerance=1e-6, budget=0.5,
-                                    flags=abi.FLAG_MG_FP16))
+                                    flags=abi.FLAG_MG_FP16 | abi.FLAG_MG_FLUID_COARSENING,
+                                    mg_coarse_sweeps=args.mg_coarse_sweeps))
         for r in reqs:
             sim.emit(r)
         for _ in range(args.warmup):
@@ -160,12 +161,15 @@ def bench_other(args):
                             "max_rel_residual": max(p[1] for p in per),
                             "max_true_rel_residual_fp64": max(p[2] for p in per),
                             "converged_steps": sum(p[3] for p in per),
+                            "mg_preconditioner": {"fluid_first_coarse_types": True,
+                                                  "coarse_sweeps": max(args.mg_coarse_sweeps, 1)},
                             "particles_per_s": round(args.steps / el * res.live_particles, 1)})
     specs = scenes.tank_batch(64, seed=4)
     sims = []
     for grid, h, reqs, cols in specs:
         n = sum(int(r.cell_dims[0]) * int(r.cell_dims[1]) * int(r.cell_dims[2]) * 8 for r in reqs) + 1024
-        sim = Sim(abi.make_settings(grid, h, n, solver=abi.SOLVER_MGPCG, tolerance=1e-6))
+        sim = Sim(abi.make_settings(grid, h, n, solver=abi.SOLVER_MGPCG, tolerance=1e-6,
+                                    flags=abi.FLAG_MG_FLUID_COARSENING, mg_coarse_sweeps=args.mg_coarse_sweeps))
         sim.set_colliders(cols)
         for r in reqs:
             sim.emit(r)
@@ -192,6 +196,8 @@ def bench_other(args):
                 value=round(args.steps / el, 3), ms_per_step=round(1e3 * el / args.steps, 3),
                 config={"workload": WORKLOADS[args.workload], "tanks": len(specs), "particles": parts,
                         "autotuned": chosen, "mean_solver_iterations": round(its / (len(specs) * args.steps), 2),
+                        "mg_preconditioner": {"fluid_first_coarse_types": True,
+                                              "coarse_sweeps": max(args.mg_coarse_sweeps, 1)},
                         "converged_tank_steps": conv, "tank_steps": len(specs) * args.steps,
                         "particles_per_s": round(args.steps / el * parts, 1)})
 
@@ -206,10 +212,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-report", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-FCG comparison run")
-    ap.add_argument("--mg-fluid-coarsening", action="store_true",
-                    help="MGPCG preconditioner option: fluid-first coarse cell types (SLOSH_FLAG_MG_FLUID_COARSENING)")
-    ap.add_argument("--mg-coarse-sweeps", type=int, default=0,
-                    help="MGPCG preconditioner option: red-black pairs per visit of levels 4 and coarser (0 = 1)")
+    ap.add_argument("--mg-reference-cycle", action="store_true",
+                    help="time the reference's V-cycle (air-first coarse types, one red-black pair per level) for "
+                         "`value` instead of the tuned preconditioner")
+    ap.add_argument("--mg-coarse-sweeps", type=int, default=3,
+                    help="tuned preconditioner: red-black pairs per visit of levels 4 and coarser")
+    ap.add_argument("--no-reference-cycle-run", action="store_true",
+                    help="skip the comparison run with the reference's V-cycle")
     ap.add_argument("--no-kernel-events", action="store_true",
                     help="no HIP events around the dominant kernel in the timed steps (no roofline figure)")
     ap.add_argument("--fcg-steps", type=int, default=10, help="timed steps of the learned-FCG comparison run")
@@ -231,12 +240,17 @@ def main():
     grid, h, reqs, dt, grav = build_scene(args.workload)
     n = sum(int(r.cell_dims[0]) * int(r.cell_dims[1]) * int(r.cell_dims[2]) * 8 for r in reqs)
 
-    def run(solver, weights, steps, warmup):
+    # MGPCG preconditioner: tuned (fluid-first coarse cell types, extra red-black pairs on levels 4 and coarser) or
+    # the reference's cycle; both solve to the same tolerance
+    tuned = {"flags": abi.FLAG_MG_FLUID_COARSENING, "mg_coarse_sweeps": args.mg_coarse_sweeps}
+    reference_cycle = {"flags": 0, "mg_coarse_sweeps": 0}
+    mg = reference_cycle if args.mg_reference_cycle else tuned
+
+    def run(solver, weights, steps, warmup, mg_opts):
         """Times `steps` steps after `warmup`; returns (steps/s, per-step data, dominant kernel, its timing, sim)."""
         s = abi.make_settings(grid, h, n, solver=solver, tolerance=1e-6,
                               max_iterations=20000 if solver == abi.SOLVER_RBSOR else 200, sor_weight=1.8,
-                              flags=abi.FLAG_MG_FLUID_COARSENING if args.mg_fluid_coarsening else 0,
-                              mg_coarse_sweeps=args.mg_coarse_sweeps)
+                              flags=mg_opts["flags"], mg_coarse_sweeps=mg_opts["mg_coarse_sweeps"])
         sim = Sim(s)
         for r in reqs:
             sim.emit(r)
@@ -268,7 +282,7 @@ def main():
         return steps / el, el, per, dominant, kt[dominant], prof, sim
 
     solver = {"mgpcg": abi.SOLVER_MGPCG, "rbsor": abi.SOLVER_RBSOR}[args.solver]
-    sps, el, per, dominant, (launches, total_ms), prof, sim = run(solver, None, args.steps, args.warmup)
+    sps, el, per, dominant, (launches, total_ms), prof, sim = run(solver, None, args.steps, args.warmup, mg)
     res = sim.results()
     avg_s = total_ms / 1e3 / max(launches, 1)
     nb = res.active_bricks
@@ -292,7 +306,7 @@ def main():
     wpath = unet.default_weights_path()
     if args.solver == "mgpcg" and not args.no_learned and os.path.exists(wpath):
         fsps, fel, fper, fdom, (flaunch, fms), fprof, fsim = run(abi.SOLVER_FCG_UNET, unet.load(wpath),
-                                                                 args.fcg_steps, 2)
+                                                                 args.fcg_steps, 2, mg)
         fsim.close()
         favg = fms / 1e3 / max(flaunch, 1) if fms > 0 else float("nan")
         # level-0 forward convolutions: E0a (4->16), E0b and D0b (16->16), 27 taps, 512 cells per brick
@@ -309,6 +323,13 @@ def main():
                    "fallback_steps": sum(1 for p in fper if p[3] == 1), "steps": args.fcg_steps,
                    "converged_steps": sum(p[2] for p in fper), "max_rel_residual": max(p[1] for p in fper),
                    "roofline": fl, "weights": os.path.relpath(wpath, ROOT)}
+    # the same workload with the reference's V-cycle, for comparison
+    ref_cycle = None
+    if args.solver == "mgpcg" and not args.mg_reference_cycle and not args.no_reference_cycle_run:
+        rsps, rel_, rper, _, _, _, rsim = run(abi.SOLVER_MGPCG, None, args.steps, args.warmup, reference_cycle)
+        rsim.close()
+        ref_cycle = {"steps_per_s": round(rsps, 3), "mean_solver_iterations": round(sum(p[0] for p in rper) / len(rper), 2),
+                     "max_rel_residual": max(p[1] for p in rper), "converged_steps": sum(p[2] for p in rper)}
     line = {
         "metric": "steps/s @256^3,16M particles; learned-FCG vs plain MGPCG at 1e-6 residual",
         "value": round(sps, 3),
@@ -327,8 +348,9 @@ def main():
                    "fluid_cells": nf, "allocated_bricks": nb,
                    "mean_solver_iterations": round(sum(iters) / len(iters), 2), "max_rel_residual": max(rels),
                    "converged_steps": conv, "particles_per_s": round(sps * res.live_particles, 1),
-                   "mg_preconditioner": {"fluid_coarsening": bool(args.mg_fluid_coarsening),
-                                         "coarse_sweeps": max(args.mg_coarse_sweeps, 1)},
+                   "mg_preconditioner": {"fluid_first_coarse_types": mg["flags"] == abi.FLAG_MG_FLUID_COARSENING,
+                                         "coarse_sweeps": max(mg["mg_coarse_sweeps"], 1)},
+                   "mg_reference_cycle": ref_cycle,
                    "learned_fcg": learned},
         "roofline": roofline,
     }
