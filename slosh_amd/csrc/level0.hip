@@ -153,6 +153,8 @@ __global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     const uint32_t done = st->done;
     const double alpha = first ? 0.0 : st->alpha;
+    issue_point();
+    if (done) return;
     const uint32_t l0 = colour_cell(threadIdx.x, 0u), l1 = colour_cell(threadIdx.x, 1u);
     const uint32_t i0 = B.base + l0, i1 = B.base + l1;
     const uint32_t m0 = fmask[i0], m1 = fmask[i1];
@@ -174,8 +176,6 @@ __global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo
         p1 = pres[i1];
         for (int n = 0; n < 6; ++n) qj[n] = q[nbr(B, l1, n)];
     }
-    issue_point();
-    if (done) return;
     double v[1] = {0.0};
     if (m0 & kSelfFluid) {
         float ri = ra0;
@@ -217,6 +217,8 @@ __global__ __launch_bounds__(kThreads) void k_restrict0(const uint32_t *binfo, c
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     const uint32_t done = st->done;
     const double mg_scale = st->mg_scale;
+    issue_point();
+    if (done) return;
     const uint32_t t = threadIdx.x;
     uint32_t l[2], m[2];
     float xv[2], bv[2], gv[2], xn[2][6];
@@ -231,8 +233,6 @@ __global__ __launch_bounds__(kThreads) void k_restrict0(const uint32_t *binfo, c
     }
     const uint32_t ci = (B.base >> 3) + (t & 63u);
     const uint32_t info = L1.info[ci];
-    issue_point();
-    if (done) return;
     for (uint32_t c = 0; c < 2u; ++c) {
         fl[l[c]] = (uint8_t)((m[c] & kSelfFluid) ? 1u : 0u);
         if (m[c] & kSelfFluid) {
@@ -272,6 +272,8 @@ __global__ __launch_bounds__(kThreads) void k_prolong_black(const uint32_t *binf
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     const uint32_t done = st->done;
     const double mg_scale = st->mg_scale;
+    issue_point();
+    if (done) return;
     const uint32_t l0 = colour_cell(threadIdx.x, 0u), l1 = colour_cell(threadIdx.x, 1u);
     const uint32_t i0 = B.base + l0, i1 = B.base + l1;
     const uint32_t m0 = fmask[i0], m1 = fmask[i1];
@@ -289,8 +291,6 @@ __global__ __launch_bounds__(kThreads) void k_prolong_black(const uint32_t *binf
         pinfon[n] = L1.info[pj];
         pxn[n] = L1.x1[pj];
     }
-    issue_point();
-    if (done) return;
     const float unscale = L1.x1.half ? (float)(1.0 / mg_scale) : 1.0f;
     if (m0 & kSelfFluid) {
         float v = z0;
@@ -316,13 +316,13 @@ __global__ __launch_bounds__(kThreads) void k_red_dot(const uint32_t *binfo, con
 {
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     const uint32_t done = st->done;
+    issue_point();
+    if (done) return;
     const uint32_t l0 = colour_cell(threadIdx.x, 0u), i0 = B.base + l0, i1 = B.base + colour_cell(threadIdx.x, 1u);
     const uint32_t m0 = fmask[i0], m1 = fmask[i1];
     const float r0 = r[i0], g0 = diag[i0], r1 = r[i1], zb1 = z1[i1];
     float zn[6];
     for (int n = 0; n < 6; ++n) zn[n] = z1[nbr(B, l0, n)];
-    issue_point();
-    if (done) return;
     double v[1] = {0.0};
     if (m0 & kSelfFluid) {
         float s = 0.0f;
@@ -348,6 +348,8 @@ __global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, c
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     const uint32_t done = st->done;
     const double beta = first ? 0.0 : st->beta;
+    issue_point();
+    if (done) return;
     uint32_t i[2], m[2];
     float zc[2], dc[2] = {0.0f, 0.0f}, gc[2], zn[2][6], dn[2][6];
     for (uint32_t c = 0; c < 2u; ++c) {
@@ -365,8 +367,6 @@ __global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, c
             for (int n = 0; n < 6; ++n) dn[c][n] = d_in[nbr(B, l, n)];
         }
     }
-    issue_point();
-    if (done) return;
     double v[1] = {0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
         if (!(m[c] & kSelfFluid)) continue;

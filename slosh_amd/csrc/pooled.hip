@@ -190,6 +190,7 @@ __global__ __launch_bounds__(pool_threads<S>()) void k_pdown_lds(const uint32_t 
     // the level-2 pass restricts into level 3 (dense by brick index): its target word is known from the brick index
     const uint32_t b3 = binfo[(size_t)e * kBinfoWords + 27];
     __syncthreads();
+    if (done) return;
     uint32_t vi[kPer];
     float vx[kPer], vb[kPer];
     for (int q = 0; q < kPer; ++q) {
@@ -206,8 +207,6 @@ __global__ __launch_bounds__(pool_threads<S>()) void k_pdown_lds(const uint32_t 
     const uint32_t slot = s_nb[13];
     const uint32_t ci = S == 2 ? b3 : slot * CPB2 + (t & (uint32_t)(CPB2 - 1));
     const uint32_t cinfo = S == 2 ? D.info[ci] : C.info[ci];
-    issue_point();
-    if (done) return;
     for (int q = 0; q < kPer; ++q) {
         const uint32_t k = t + kPoolThreads * (uint32_t)q;
         if (k >= (uint32_t)Bx::NB) continue;
@@ -273,6 +272,7 @@ __global__ __launch_bounds__(pool_threads<S>()) void k_pup_lds(const uint32_t *b
     if (t < 28u) s_nb[t] = binfo[(size_t)e * kBinfoWords + t];
     const uint32_t done = st->done;
     __syncthreads();
+    if (done) return;
     // every box cell's word, iterate, right-hand side and its parent's word and result, all issued at once (a
     // cell outside the grid reads the null slot, and for level 2 its own brick's level-3 cell)
     uint32_t vi[kPer], pinf[kPer];
@@ -303,8 +303,6 @@ __global__ __launch_bounds__(pool_threads<S>()) void k_pup_lds(const uint32_t *b
             px[q] = C.x1[pi];
         }
     }
-    issue_point();
-    if (done) return;
     for (int q = 0; q < kPer; ++q) {
         const uint32_t k = t + kPoolThreads * (uint32_t)q;
         if (k >= (uint32_t)Bx::NB) continue;

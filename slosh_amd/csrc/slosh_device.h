@@ -20,11 +20,11 @@ constexpr uint32_t kCellAir = 0u;
 constexpr uint32_t kCellFluid = 1u;
 constexpr uint32_t kCellSolid = 2u;
 
-// Ordering point for the compiler: memory operations written before it stay before it. The solver passes issue
-// every load a thread can need (its cells and their stencil neighbours) and only then test the solver's done
-// flag, so the flag, the brick table row and the vector data arrive in one memory latency. Without
-// this point the compiler would sink the loads past the test (they are used only on the continuing path) and each
-// group of loads would wait for the one before it.
+// Ordering point for the compiler: memory operations written before it stay before it. The solver passes read their
+// scalars (the brick table row, the solver's done flag, alpha or beta) and test the flag after this point, so all of
+// them arrive in one memory latency; without it the compiler would sink the row's loads past the test (they are
+// used only on the continuing path) and the row would wait for the flag. The coarse tail issues every global read
+// before its test.
 __device__ __forceinline__ void issue_point() { __asm__ volatile("" ::: "memory"); }
 
 // Type of a coarse multigrid cell from its children (SloshPressureMG.usf:256-278, MGBuildTypeCS): AIR if any child
