@@ -123,37 +123,71 @@ __device__ void bitonic_lds(uint32_t *s, uint32_t size)
     }
 }
 
+// Ascending bitonic sort of one value per thread over a 256-thread workgroup: stages with partner distance < 64
+// exchange through wavefront shuffles, the three with distance 64 or 128 through LDS.
+__device__ __forceinline__ uint32_t bitonic_tile(uint32_t x, uint32_t *s)
+{
+    const uint32_t t = threadIdx.x;
+    for (uint32_t k = 2; k <= kTile; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            uint32_t y;
+            if (j >= 64u) {
+                s[t] = x;
+                __syncthreads();
+                y = s[t ^ j];
+                __syncthreads();
+            } else {
+                y = (uint32_t)__shfl_xor((int)x, (int)j);
+            }
+            const bool lower = (t & j) == 0u, up = (t & k) == 0u;
+            x = (lower == up) ? min(x, y) : max(x, y);
+        }
+    }
+    return x;
+}
+
 // Sort pass 1. Per tile of 256 consecutive particles: key = 8^3 brick of the clamped cell (dead: last key), a
-// stable in-tile rank per key from an LDS bitonic sort of (key << 8 | lane), and the tile's runs of equal keys.
+// stable in-tile rank per key from a sort of (key << 8 | lane), and the tile's runs of equal keys. The values are
+// distinct, so their sorted order is unique; a tile whose keys already ascend (the common case: particles were
+// sorted last step and few cross a brick face) skips the sort. Run starts are counted with wavefront ballots.
 // Reference counterpart: SortCountCS (SloshSort.usf:65-76), whose atomic rank is not stable.
 __global__ __launch_bounds__(256) void k_sort_tiles(Geo g, const float4 *pa, uint32_t np, uint32_t dead_key,
                                                     uint32_t *packed, uint32_t *tile_nruns, uint32_t *run_key,
                                                     uint32_t *run_len, uint32_t *key_count, uint32_t *key_runs)
 {
     __shared__ uint32_t s[kTile];
-    __shared__ uint32_t sc[kTile];
     __shared__ uint32_t start[kTile];
-    const uint32_t t = threadIdx.x;
+    __shared__ uint32_t wruns[kTile / 64];
+    __shared__ uint32_t unsorted;
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
     const uint32_t i = blockIdx.x * kTile + t;
     uint32_t v = 0xFFFFFFFFu;
     if (i < np) v = (key_of(g, pa[i], dead_key) << 8) | t;
     s[t] = v;
+    if (t == 0u) unsorted = 0u;
     __syncthreads();
-    bitonic_lds(s, kTile);
-    const uint32_t e = s[t];
-    const bool valid = e != 0xFFFFFFFFu;
-    const uint32_t k = e >> 8;
-    const bool flag = valid && (t == 0u || (s[t - 1] >> 8) != k);
-    sc[t] = flag ? 1u : 0u;
+    if (t > 0u && s[t - 1u] > v) unsorted = 1u;
     __syncthreads();
-    for (uint32_t off = 1; off < kTile; off <<= 1) {
-        uint32_t add = t >= off ? sc[t - off] : 0u;
-        __syncthreads();
-        sc[t] += add;
+    uint32_t e = v;
+    if (unsorted) {
+        e = bitonic_tile(v, s);
+        s[t] = e;
         __syncthreads();
     }
-    const uint32_t nruns = sc[kTile - 1];
-    const uint32_t run = sc[t] - 1u;
+    const bool valid = e != 0xFFFFFFFFu;
+    const uint32_t k = e >> 8;
+    const bool flag = valid && (t == 0u || (s[t - 1u] >> 8) != k);
+    // run index of every entry = run starts at or before it, minus one
+    const uint64_t m = __ballot(flag);
+    const uint32_t upto = (uint32_t)__popcll(m & (lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull)));
+    if (lane == 0u) wruns[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = 0u, nruns = 0u;
+    for (uint32_t q = 0; q < kTile / 64; ++q) {
+        base += q < w ? wruns[q] : 0u;
+        nruns += wruns[q];
+    }
+    const uint32_t run = base + upto - 1u;
     if (flag) start[run] = t;
     __syncthreads();
     const uint32_t nvalid = np - blockIdx.x * kTile < kTile ? np - blockIdx.x * kTile : kTile;
