@@ -58,25 +58,30 @@ __host__ __device__ constexpr int fin_values(int kind)
     return kind == kFinInit ? 3 : ((kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck) ? 2 : 1);
 }
 
-// Sums the partials of the preceding reduction in a fixed order (per thread: a strided run loaded four at a time;
+// Sums the partials of the preceding reduction in a fixed order (per thread: its strided run in increasing order;
 // then a wavefront butterfly and the 16 wavefront sums in order) and applies them to the solver state.
 __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
                                                    SolverState *st)
 {
-    if (kind != kFinInit && kind != kFinCheck && st->done) return;
     const int nv = fin_values(kind);
     __shared__ double sh[3][16];
     const uint32_t t = threadIdx.x;
-    double a[3] = {0.0, 0.0, 0.0};
-    uint32_t b = t;
-    for (; b + 3072u < nblocks; b += 4096u) {
-        double v[4][3];
-        for (int k = 0; k < 4; ++k)
-            for (int j = 0; j < 3; ++j) v[k][j] = j < nv ? partials[(size_t)(b + 1024u * k) * nv + j] : 0.0;
-        for (int k = 0; k < 4; ++k)
-            for (int j = 0; j < 3; ++j) a[j] += v[k][j];
+    // the first 8 partials of this thread (b = t + 1024 k) are read together with the done flag; a thread sums its
+    // partials in increasing b (an absent one adds +0, which leaves the sum unchanged: it starts at +0 and never
+    // becomes -0)
+    constexpr int kFirst = 8;
+    const uint32_t done = st->done;
+    double v[kFirst][3];
+    for (int k = 0; k < kFirst; ++k) {
+        const uint32_t b = t + 1024u * (uint32_t)k;
+        for (int j = 0; j < 3; ++j) v[k][j] = (b < nblocks && j < nv) ? partials[(size_t)b * nv + j] : 0.0;
     }
-    for (; b < nblocks; b += 1024u)
+    issue_point();
+    if (kind != kFinInit && kind != kFinCheck && done) return;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < kFirst; ++k)
+        for (int j = 0; j < 3; ++j) a[j] += v[k][j];
+    for (uint32_t b = t + 1024u * kFirst; b < nblocks; b += 1024u)
         for (int j = 0; j < nv; ++j) a[j] += partials[(size_t)b * nv + j];
     for (int w = 32; w > 0; w >>= 1)
         for (int j = 0; j < 3; ++j) a[j] += __shfl_xor(a[j], w);
