@@ -404,26 +404,54 @@ __global__ __launch_bounds__(512) void k_div_stats(Geo g, const float *vel, cons
 }
 
 // End of step: one workgroup reduces the per-workgroup statistics of P2G, G2P and the divergence pass (counts and
-// maxima in any order, the FP64 divergence sum in a fixed order).
-__global__ __launch_bounds__(256) void k_step_stats(const uint32_t *bp_fluid, uint32_t n_p2g, const uint32_t *bp_div_max,
-                                                    const uint32_t *bp_div_cnt, const double *bp_div_sum,
-                                                    uint32_t n_div, const uint32_t *bp_speed,
-                                                    const uint32_t *bp_clamp, uint32_t n_g2p, StepStats *stats)
+// maxima in any order, the FP64 divergence sum in a fixed order: per thread in increasing index, then a tree).
+// Each thread keeps eight loads in flight per loop trip.
+constexpr uint32_t kStatsThreads = 1024;
+__global__ __launch_bounds__(kStatsThreads) void k_step_stats(const uint32_t *bp_fluid, uint32_t n_p2g,
+                                                              const uint32_t *bp_div_max, const uint32_t *bp_div_cnt,
+                                                              const double *bp_div_sum, uint32_t n_div,
+                                                              const uint32_t *bp_speed, const uint32_t *bp_clamp,
+                                                              uint32_t n_g2p, StepStats *stats)
 {
-    __shared__ uint32_t su[5][256];
-    __shared__ double sd[256];
+    constexpr uint32_t T = kStatsThreads, U = 8;
+    __shared__ uint32_t su[5][T];
+    __shared__ double sd[T];
     const uint32_t t = threadIdx.x;
     uint32_t fluid = 0, dmax = 0, dcnt = 0, smax = 0, clamp = 0;
     double dsum = 0.0;
-    for (uint32_t b = t; b < n_p2g; b += 256) fluid += bp_fluid[b];
-    for (uint32_t b = t; b < n_div; b += 256) {
-        dmax = max(dmax, bp_div_max[b]);
-        dcnt += bp_div_cnt[b];
-        dsum += bp_div_sum[b];
+    for (uint32_t b0 = t; b0 < n_p2g; b0 += T * U) {
+        uint32_t v[U];
+        for (uint32_t u = 0; u < U; ++u) v[u] = b0 + T * u < n_p2g ? bp_fluid[b0 + T * u] : 0u;
+        for (uint32_t u = 0; u < U; ++u) fluid += v[u];
     }
-    for (uint32_t b = t; b < n_g2p; b += 256) {
-        smax = max(smax, bp_speed[b]);
-        clamp += bp_clamp[b];
+    for (uint32_t b0 = t; b0 < n_div; b0 += T * U) {
+        uint32_t m[U], k[U];
+        double d[U];
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t b = b0 + T * u;
+            const bool in = b < n_div;
+            m[u] = in ? bp_div_max[b] : 0u;
+            k[u] = in ? bp_div_cnt[b] : 0u;
+            d[u] = in ? bp_div_sum[b] : 0.0;
+        }
+        for (uint32_t u = 0; u < U; ++u) {
+            dmax = max(dmax, m[u]);
+            dcnt += k[u];
+            dsum += d[u];
+        }
+    }
+    for (uint32_t b0 = t; b0 < n_g2p; b0 += T * U) {
+        uint32_t sp[U], cl[U];
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t b = b0 + T * u;
+            const bool in = b < n_g2p;
+            sp[u] = in ? bp_speed[b] : 0u;
+            cl[u] = in ? bp_clamp[b] : 0u;
+        }
+        for (uint32_t u = 0; u < U; ++u) {
+            smax = max(smax, sp[u]);
+            clamp += cl[u];
+        }
     }
     su[0][t] = fluid;
     su[1][t] = dmax;
@@ -432,7 +460,7 @@ __global__ __launch_bounds__(256) void k_step_stats(const uint32_t *bp_fluid, ui
     su[4][t] = clamp;
     sd[t] = dsum;
     __syncthreads();
-    for (uint32_t s = 128; s > 0; s >>= 1) {
+    for (uint32_t s = T / 2; s > 0; s >>= 1) {
         if (t < s) {
             su[0][t] += su[0][t + s];
             su[1][t] = max(su[1][t], su[1][t + s]);
@@ -613,7 +641,7 @@ void launch_project(slosh_ctx *c)
 void launch_step_stats(slosh_ctx *c)
 {
     SLOSH_TIMED(c, K_STEP_STATS);
-    hipLaunchKernelGGL(k_step_stats, dim3(1), dim3(256), 0, c->stream, c->bp_fluid, c->n_p2g_blocks, c->bp_div_max,
+    hipLaunchKernelGGL(k_step_stats, dim3(1), dim3(kStatsThreads), 0, c->stream, c->bp_fluid, c->n_p2g_blocks, c->bp_div_max,
                        c->bp_div_cnt, c->bp_div_sum, c->n_div_blocks, c->bp_speed, c->bp_clamp, c->n_g2p_blocks,
                        c->stats);
     SLOSH_HIP(hipGetLastError());

@@ -681,14 +681,15 @@ __global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t
     }
 }
 
-// ---- coarsest level: FP64 Cholesky of the <= 64 fluid unknowns and its explicit inverse (once per solve) ----
+// ---- coarsest level: FP64 explicit inverse of the <= 64 fluid unknowns' operator (once per solve) ----
 // The coarsest level is level 0 (mode 0, grids of <= 64 cells), a pooled level (mode 1) or a dense level (mode 2).
 __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ctype, const float *diag, int mode,
                                                        PLevel P, DenseLevel D, uint32_t *cidx, uint32_t *cn,
                                                        double *inv)
 {
     constexpr int N = (int)kCoarsestCells;
-    __shared__ double a[N][N + 1];
+    __shared__ double a[N][2 * N + 1]; // [A | I], becoming [I | A^-1]
+    __shared__ double col[N];
     __shared__ int px[N], py[N], pz[N];
     __shared__ uint32_t pidx[N];
     __shared__ uint32_t n_s;
@@ -725,7 +726,7 @@ __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ct
     }
     __syncthreads();
     const int n = (int)n_s;
-    for (int e = t; e < N * N; e += blockDim.x) a[e / N][e % N] = 0.0;
+    for (int e = t; e < N * 2 * N; e += blockDim.x) a[e / (2 * N)][e % (2 * N)] = (e % (2 * N)) == N + e / (2 * N) ? 1.0 : 0.0;
     __syncthreads();
     if (t < n) {
         cidx[t] = pidx[t];
@@ -745,35 +746,23 @@ __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ct
         a[t][t] = (double)dg;
     }
     __syncthreads();
-    // left-looking Cholesky, lower factor in place
-    for (int j = 0; j < n; ++j) {
-        if (t == 0) {
-            double s = a[j][j];
-            for (int k = 0; k < j; ++k) s -= a[j][k] * a[j][k];
-            a[j][j] = sqrt(s > 0.0 ? s : 1e-300);
-        }
+    // Gauss-Jordan elimination on [A | I] (A is symmetric positive definite: no pivoting), all threads per step
+    for (int k = 0; k < n; ++k) {
+        const double piv = a[k][k];
         __syncthreads();
-        if (t > j && t < n) {
-            double s = a[t][j];
-            for (int k = 0; k < j; ++k) s -= a[t][k] * a[j][k];
-            a[t][j] = s / a[j][j];
+        for (int j = t; j < 2 * N; j += blockDim.x) a[k][j] /= piv;
+        __syncthreads();
+        if (t < n) col[t] = a[t][k];
+        __syncthreads();
+        for (int e = t; e < n * 2 * N; e += blockDim.x) {
+            const int i = e / (2 * N), j = e % (2 * N);
+            if (i != k) a[i][j] -= col[i] * a[k][j];
         }
         __syncthreads();
     }
-    // column t of the inverse: solve L L^T x = e_t
-    if (t < n) {
-        double y[N];
-        for (int i = 0; i < n; ++i) {
-            double s = i == t ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) s -= a[i][k] * y[k];
-            y[i] = s / a[i][i];
-        }
-        for (int i = n - 1; i >= 0; --i) {
-            double s = y[i];
-            for (int k = i + 1; k < n; ++k) s -= a[k][i] * y[k];
-            y[i] = s / a[i][i];
-        }
-        for (int i = 0; i < n; ++i) inv[(size_t)i * N + t] = y[i];
+    for (int e = t; e < N * N; e += blockDim.x) {
+        const int i = e / N, j = e % N;
+        inv[(size_t)i * N + j] = (i < n && j < n) ? a[i][N + j] : 0.0;
     }
 }
 
