@@ -709,20 +709,30 @@ __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ct
         if (mode == 0) return ctype[i] & kCellTypeMask;
         return mode == 1 ? P.type[i] : D.type[i];
     };
-    if (t == 0) {
-        uint32_t n = 0;
-        for (uint32_t i = 0; i < cells && n < (uint32_t)N; ++i) {
-            int x = (int)(i % (uint32_t)dims[0]), y = (int)((i / (uint32_t)dims[0]) % (uint32_t)dims[1]),
-                z = (int)(i / (uint32_t)(dims[0] * dims[1]));
-            if (type_of(x, y, z) != kCellFluid) continue;
-            px[n] = x;
-            py[n] = y;
-            pz[n] = z;
-            pidx[n] = index_of(x, y, z);
-            n++;
+    // the fluid cells in increasing index order (the level has at most N = 64 cells, which slosh_create checks: one
+    // wavefront, compacted by a ballot)
+    static_assert(N == 64, "one wavefront scans the coarsest level");
+    if (t < N) {
+        bool f = false;
+        int x = 0, y = 0, z = 0;
+        if ((uint32_t)t < cells) {
+            x = t % dims[0];
+            y = (t / dims[0]) % dims[1];
+            z = t / (dims[0] * dims[1]);
+            f = type_of(x, y, z) == kCellFluid;
         }
-        n_s = n;
-        *cn = n;
+        const uint64_t m = __ballot(f);
+        const uint32_t pos = (uint32_t)__popcll(m & ((1ull << t) - 1ull));
+        if (f) {
+            px[pos] = x;
+            py[pos] = y;
+            pz[pos] = z;
+            pidx[pos] = index_of(x, y, z);
+        }
+        if (t == 0) {
+            n_s = (uint32_t)__popcll(m);
+            *cn = n_s;
+        }
     }
     __syncthreads();
     const int n = (int)n_s;
