@@ -65,7 +65,7 @@ void free_all(slosh_ctx *c)
     for (int k = 0; k < 3; ++k) dfree(c->vel[k]);
     dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag); dfree(c->fmask);
     dfree(c->r); dfree(c->r2); dfree(c->z); dfree(c->z1); dfree(c->d); dfree(c->d2); dfree(c->q);
-    dfree(c->binfo); dfree(c->fcg_z); dfree(c->capture);
+    dfree(c->binfo); dfree(c->binfo_f); dfree(c->fluid_flag); dfree(c->fluid_rank); dfree(c->fcg_z); dfree(c->capture);
     unet_free(c);
     for (int l = 1; l < kMaxLevels; ++l) {
         dfree(c->lv[l].type);
@@ -86,6 +86,7 @@ void free_all(slosh_ctx *c)
     c->tev_cap = 0;
     if (c->h_stats) (void)hipHostFree(c->h_stats);
     if (c->h_state) (void)hipHostFree(c->h_state);
+    if (c->h_nsolve) (void)hipHostFree(c->h_nsolve);
     c->h_stats = nullptr;
     c->h_state = nullptr;
     if (c->ev_step0) (void)hipEventDestroy(c->ev_step0);
@@ -205,6 +206,9 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->d2, pc);
     dalloc(&c->q, pc);
     dalloc(&c->binfo, (size_t)g.npool * kBinfoWords);
+    dalloc(&c->binfo_f, (size_t)g.npool * kBinfoWords);
+    dalloc(&c->fluid_flag, g.npool);
+    dalloc(&c->fluid_rank, (size_t)g.npool + 1u);
     if (s.pressure_solver == SLOSH_SOLVER_FCG_UNET) dalloc(&c->fcg_z, pc);
     if (s.flags & SLOSH_FLAG_CAPTURE_RESIDUALS) {
         c->capture_cap = 12;
@@ -276,6 +280,7 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->stats, 1);
     dalloc(&c->d_col, SLOSH_MAX_COLLIDERS);
     if (hipHostMalloc((void **)&c->h_stats, sizeof(StepStats), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+    if (hipHostMalloc((void **)&c->h_nsolve, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
     if (hipHostMalloc((void **)&c->h_state, sizeof(SolverState), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
     memset(c->h_stats, 0, sizeof(StepStats));
     memset(c->h_state, 0, sizeof(SolverState));

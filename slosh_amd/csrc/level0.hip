@@ -106,24 +106,38 @@ template <int NV> __device__ void block_partial(const double (&v)[NV], double *p
         for (int k = 0; k < NV; ++k) partials[(size_t)blockIdx.x * NV + k] = a[k];
 }
 
+// 1 for the allocated bricks holding fluid cells (their count from the P2G classification).
+__global__ void k_fluid_flags(uint32_t nactive, const uint32_t *bp_fluid, uint32_t *flag)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nactive) flag[e] = bp_fluid[e] != 0u ? 1u : 0u;
+}
+
 // Slots of the 3x3x3 brick neighbourhood (0 outside the grid or unallocated) and the brick index of every
-// allocated brick, once per solve.
-__global__ void k_brick_info(Geo g, uint32_t nactive, uint32_t *binfo)
+// allocated brick, once per solve; the rows of bricks with fluid also go, in active order, to binfo_f.
+__global__ void k_brick_info(Geo g, uint32_t nactive, uint32_t *binfo, const uint32_t *flag, const uint32_t *rank,
+                             uint32_t *binfo_f)
 {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nactive) return;
     const uint32_t b = g.active[e];
     int bc[3];
     brick_coord(g, b, bc);
-    uint32_t *p = binfo + (size_t)e * kBinfoWords;
+    uint32_t row[28];
     for (int k = 0; k < 27; ++k) {
         const int x = bc[0] + k % 3 - 1, y = bc[1] + (k / 3) % 3 - 1, z = bc[2] + k / 9 - 1;
         uint32_t s = 0u;
         if (x >= 0 && y >= 0 && z >= 0 && x < g.bd[0] && y < g.bd[1] && z < g.bd[2])
             s = g.table[(uint32_t)(x + g.bd[0] * (y + g.bd[1] * z))];
-        p[k] = s;
+        row[k] = s;
     }
-    p[27] = b;
+    row[27] = b;
+    uint32_t *p = binfo + (size_t)e * kBinfoWords;
+    for (int k = 0; k < 28; ++k) p[k] = row[k];
+    if (flag[e]) {
+        uint32_t *q = binfo_f + (size_t)rank[e] * kBinfoWords;
+        for (int k = 0; k < 28; ++k) q[k] = row[k];
+    }
 }
 
 // r = b - A p (FP64 p); partials ||b||^2, ||r||^2 and the fluid cell count.
@@ -500,19 +514,30 @@ __global__ __launch_bounds__(kThreads) void k_fcg_update(const uint32_t *binfo, 
 #define L0_LAUNCH(kid, kernel, ...)                                                                              \
     do {                                                                                                         \
         SLOSH_TIMED(c, kid);                                                                                     \
-        hipLaunchKernelGGL(kernel, dim3(c->nactive), dim3(kThreads), 0, c->stream, c->binfo, c->fmask, __VA_ARGS__); \
+        hipLaunchKernelGGL(kernel, dim3(c->nsolve), dim3(kThreads), 0, c->stream, c->binfo_f, c->fmask, __VA_ARGS__); \
     } while (0)
 
 void l0_brick_info(slosh_ctx *c)
 {
     SLOSH_TIMED(c, K_MG_SETUP);
-    hipLaunchKernelGGL(k_brick_info, dim3((c->nactive + 255u) / 256u), dim3(256), 0, c->stream, c->geo, c->nactive,
-                       c->binfo);
+    const uint32_t nb = (c->nactive + 255u) / 256u;
+    hipLaunchKernelGGL(k_fluid_flags, dim3(nb), dim3(256), 0, c->stream, c->nactive, c->bp_fluid, c->fluid_flag);
+    scan_exclusive(c, c->fluid_flag, c->fluid_rank, c->nactive);
+    hipLaunchKernelGGL(k_brick_info, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nactive, c->binfo, c->fluid_flag,
+                       c->fluid_rank, c->binfo_f);
+    // the level-0 and pooled passes launch one workgroup per brick with fluid: their count comes back here
+    SLOSH_HIP(hipMemcpyAsync(c->h_nsolve, c->fluid_rank + c->nactive, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    SLOSH_HIP(hipStreamSynchronize(c->stream));
+    c->nsolve = *c->h_nsolve;
 }
 
+// Over every allocated brick (non-fluid cells get r = 0, also in bricks without fluid).
 void l0_pcg_init(slosh_ctx *c, float *r_out)
 {
-    L0_LAUNCH(K_PCG_INIT, k_pcg_init, c->div, c->diag, c->pres, r_out, c->partials);
+    SLOSH_TIMED(c, K_PCG_INIT);
+    hipLaunchKernelGGL(k_pcg_init, dim3(c->nactive), dim3(kThreads), 0, c->stream, c->binfo, c->fmask, c->div, c->diag,
+                       c->pres, r_out, c->partials);
 }
 
 void l0_update_black(slosh_ctx *c, const float *r_in, float *r_out, const float *d, int first)

@@ -405,9 +405,11 @@ void pooled_down(slosh_ctx *c, int l)
     DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
     KTimer timer(c, l == 1 ? K_PDOWN1 : K_PDOWN2);
     if (l == 1)
-        hipLaunchKernelGGL(k_pdown_lds<4>, dim3(c->nactive), dim3(pool_threads<4>()), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pdown_lds<4>, dim3(c->nsolve), dim3(pool_threads<4>()), 0, c->stream, c->binfo_f, L, C, D,
+                           c->state);
     else
-        hipLaunchKernelGGL(k_pdown_lds<2>, dim3(c->nactive), dim3(pool_threads<2>()), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pdown_lds<2>, dim3(c->nsolve), dim3(pool_threads<2>()), 0, c->stream, c->binfo_f, L, C, D,
+                           c->state);
 }
 
 void pooled_up(slosh_ctx *c, int l)
@@ -417,9 +419,11 @@ void pooled_up(slosh_ctx *c, int l)
     DenseLevel D = l == 2 ? dlevel_of(c, 3) : DenseLevel{};
     KTimer timer(c, l == 1 ? K_PUP1 : K_PUP2);
     if (l == 1)
-        hipLaunchKernelGGL(k_pup_lds<4>, dim3(c->nactive), dim3(pool_threads<4>()), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pup_lds<4>, dim3(c->nsolve), dim3(pool_threads<4>()), 0, c->stream, c->binfo_f, L, C, D,
+                           c->state);
     else
-        hipLaunchKernelGGL(k_pup_lds<2>, dim3(c->nactive), dim3(pool_threads<2>()), 0, c->stream, c->binfo, L, C, D, c->state);
+        hipLaunchKernelGGL(k_pup_lds<2>, dim3(c->nsolve), dim3(pool_threads<2>()), 0, c->stream, c->binfo_f, L, C, D,
+                           c->state);
 }
 
 } // namespace slosh

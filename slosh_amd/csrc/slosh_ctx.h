@@ -240,6 +240,12 @@ struct slosh_ctx {
     // V-cycle result; binfo holds kBinfoWords per allocated brick (its 3x3x3 neighbourhood slots, brick index)
     float *r = nullptr, *r2 = nullptr, *z = nullptr, *z1 = nullptr, *d = nullptr, *d2 = nullptr, *q = nullptr;
     uint32_t *binfo = nullptr;
+    // rows of binfo for the bricks holding fluid cells, in active order: the solver's level-0 and pooled passes
+    // visit these only (a brick without fluid has no fluid cell on levels 0-2 and nothing reads its values)
+    uint32_t *binfo_f = nullptr;
+    uint32_t *fluid_flag = nullptr, *fluid_rank = nullptr;
+    uint32_t *h_nsolve = nullptr; // pinned: the count of such bricks, read back once per solve
+    uint32_t nsolve = 0;
     int nlevels = 1;
     slosh::Level lv[slosh::kMaxLevels]{};
     uint32_t *coarse_idx = nullptr; // [kCoarsestCells]

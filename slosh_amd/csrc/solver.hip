@@ -807,10 +807,13 @@ __global__ void k_state_reset(SolverState *st, double tol, int maxit)
     st->rel = 0.0;
 }
 
+// Reduction of the partials of the preceding level-0 pass: one per brick with fluid, except the initial residual,
+// which covers every allocated brick.
 void finalize(slosh_ctx *c, int kind, int iters = 0)
 {
     SLOSH_TIMED(c, K_FINALIZE);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, kind, c->partials, c->nactive, iters, c->state);
+    const uint32_t nblocks = kind == kFinInit ? c->nactive : c->nsolve;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, kind, c->partials, nblocks, iters, c->state);
     (void)fin_values;
 }
 
@@ -989,6 +992,11 @@ void run_mgpcg(slosh_ctx *c)
     float *D[2] = {c->d, c->d2};
     l0_pcg_init(c, R[0]);
     finalize(c, kFinInit);
+    if (!c->nsolve) {
+        // no fluid cell: b = 0, the solve is converged as it stands
+        c->capture_n = 0;
+        return;
+    }
     // residuals kept for training the learned preconditioner: the initial one and those after iterations 1, 2,
     // 3, 4, 6, 8, 11, 15, 20, ... (slots past the solve's last iteration are dropped afterwards)
     uint32_t ncap = 0;
@@ -1070,6 +1078,7 @@ void run_fcg(slosh_ctx *c)
     float *r = c->r;
     l0_pcg_init(c, r);
     finalize(c, kFinInit);
+    if (!c->nsolve) return;
     int dc = 0;
     double hist[4] = {0, 0, 0, 0};
     bool fallback = false;
@@ -1126,6 +1135,12 @@ void run_rbsor(slosh_ctx *c)
     const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 1000;
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
     l0_brick_info(c);
+    if (!c->nsolve) {
+        // no fluid cell: b = 0, the solve is converged as it stands
+        l0_pcg_init(c, c->r);
+        finalize(c, kFinInit);
+        return;
+    }
     const double w = (double)c->s.sor_weight;
     int it = 0;
     auto residual = [&](int iters) {
