@@ -70,7 +70,8 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
     // partials in increasing b (an absent one adds +0, which leaves the sum unchanged: it starts at +0 and never
     // becomes -0)
     constexpr int kFirst = 8;
-    const uint32_t done = st->done;
+    const SolverState s0 = *st; // read with the partials; thread 0 updates the state from it
+    const uint32_t done = (uint32_t)s0.done;
     double v[kFirst][3];
     for (int k = 0; k < kFirst; ++k) {
         const uint32_t b = t + 1024u * (uint32_t)k;
@@ -103,33 +104,34 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
         // t0 = ||b||^2, t1 = ||r||^2
         st->bb = t0;
         st->rr = t1;
-        double bn = sqrt(t0);
-        st->rel = bn > 0.0 ? sqrt(t1) / bn : 0.0;
+        const double bn = sqrt(t0), rel = bn > 0.0 ? sqrt(t1) / bn : 0.0;
+        st->rel = rel;
         if (kind == kFinInit) {
             st->iters = 0;
             st->rho = 0.0;
             st->beta = 0.0;
             st->alpha = 0.0;
-            st->converged = (bn == 0.0 || st->rel <= st->tol) ? 1 : 0;
-            st->done = st->converged;
+            const int conv = (bn == 0.0 || rel <= s0.tol) ? 1 : 0;
+            st->converged = conv;
+            st->done = conv;
         } else {
             st->iters = iters;
-            if (bn == 0.0 || st->rel <= st->tol) {
+            if (bn == 0.0 || rel <= s0.tol) {
                 st->converged = 1;
                 st->done = 1;
-            } else if (iters >= st->maxit) {
+            } else if (iters >= s0.maxit) {
                 st->done = 1;
             }
         }
     } else if (kind == kFinRho) {
-        st->beta = fabs(st->rho) > 1e-30 ? t0 / st->rho : 0.0;
+        st->beta = fabs(s0.rho) > 1e-30 ? t0 / s0.rho : 0.0;
         st->rho = t0;
     } else if (kind == kFinAlpha) {
         st->dq = t0;
-        st->alpha = fabs(t0) > 1e-30 ? st->rho / t0 : 0.0;
+        st->alpha = fabs(t0) > 1e-30 ? s0.rho / t0 : 0.0;
     } else if (kind == kFinBetaF) {
         // flexible CG: beta = -(z . q_prev) / (d_prev . q_prev)
-        st->beta = fabs(st->dq) > 1e-30 ? -t0 / st->dq : 0.0;
+        st->beta = fabs(s0.dq) > 1e-30 ? -t0 / s0.dq : 0.0;
     } else if (kind == kFinAlphaF) {
         // flexible CG: alpha = (d . r) / (d . q)
         st->dq = t0;
@@ -138,13 +140,15 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
         st->rel_true = t0 > 0.0 ? sqrt(t1) / sqrt(t0) : 0.0;
     } else {
         st->rr = t0;
-        st->mg_scale = t0 > 0.0 ? sqrt(st->nf / t0) : 1.0;
-        st->rel = sqrt(t0) / sqrt(st->bb);
-        st->iters += 1;
-        if (st->rel <= st->tol) {
+        st->mg_scale = t0 > 0.0 ? sqrt(s0.nf / t0) : 1.0;
+        const double rel = sqrt(t0) / sqrt(s0.bb);
+        st->rel = rel;
+        const int it = s0.iters + 1;
+        st->iters = it;
+        if (rel <= s0.tol) {
             st->converged = 1;
             st->done = 1;
-        } else if (st->iters >= st->maxit) {
+        } else if (it >= s0.maxit) {
             st->done = 1;
         }
     }
