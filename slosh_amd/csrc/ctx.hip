@@ -21,7 +21,7 @@ const char *const kKernelNames[K_COUNT] = {
     "k_emit_fill", "k_sort_tiles", "k_sort_runs_scatter", "k_sort_key_runs", "k_sort_scatter", "k_scan",
     "k_pool", "k_p2g", "k_normalize", "k_divergence", "k_project", "k_div_stats", "k_extrapolate", "k_g2p",
     "k_mg_setup", "k_coarse_factor", "k_pcg_init", "k_red_dot", "k_restrict0", "k_prolong_black", "k_direction",
-    "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d3down", "k_d3up",
+    "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d23down", "k_d23up",
     "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
     "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io", "k_unet_train", "k_fcg"};
 } // namespace slosh
@@ -65,7 +65,8 @@ void free_all(slosh_ctx *c)
     for (int k = 0; k < 3; ++k) dfree(c->vel[k]);
     dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag); dfree(c->fmask);
     dfree(c->r); dfree(c->r2); dfree(c->z); dfree(c->z1); dfree(c->d); dfree(c->d2); dfree(c->q);
-    dfree(c->binfo); dfree(c->binfo_f); dfree(c->fluid_flag); dfree(c->fluid_rank); dfree(c->fcg_z); dfree(c->capture);
+    dfree(c->binfo); dfree(c->binfo_f); dfree(c->fluid_flag); dfree(c->fluid_rank); dfree(c->tile_flag); dfree(c->tile_rank);
+    dfree(c->tile_list); dfree(c->fcg_z); dfree(c->capture);
     unet_free(c);
     for (int l = 1; l < kMaxLevels; ++l) {
         dfree(c->lv[l].type);
@@ -208,6 +209,13 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->binfo, (size_t)g.npool * kBinfoWords);
     dalloc(&c->binfo_f, (size_t)g.npool * kBinfoWords);
     dalloc(&c->fluid_flag, g.npool);
+    {
+        const uint32_t t3 = (uint32_t)(((g.bd[0] + 3) / 4) * ((g.bd[1] + 3) / 4) * ((g.bd[2] + 3) / 4));
+        c->ntiles = t3;
+        dalloc(&c->tile_flag, t3);
+        dalloc(&c->tile_rank, (size_t)t3 + 1u);
+        dalloc(&c->tile_list, t3);
+    }
     dalloc(&c->fluid_rank, (size_t)g.npool + 1u);
     if (s.pressure_solver == SLOSH_SOLVER_FCG_UNET) dalloc(&c->fcg_z, pc);
     if (s.flags & SLOSH_FLAG_CAPTURE_RESIDUALS) {
@@ -280,7 +288,7 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->stats, 1);
     dalloc(&c->d_col, SLOSH_MAX_COLLIDERS);
     if (hipHostMalloc((void **)&c->h_stats, sizeof(StepStats), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
-    if (hipHostMalloc((void **)&c->h_nsolve, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+    if (hipHostMalloc((void **)&c->h_nsolve, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
     if (hipHostMalloc((void **)&c->h_state, sizeof(SolverState), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
     memset(c->h_stats, 0, sizeof(StepStats));
     memset(c->h_state, 0, sizeof(SolverState));

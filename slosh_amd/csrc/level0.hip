@@ -517,7 +517,7 @@ __global__ __launch_bounds__(kThreads) void k_fcg_update(const uint32_t *binfo, 
         hipLaunchKernelGGL(kernel, dim3(c->nsolve), dim3(kThreads), 0, c->stream, c->binfo_f, c->fmask, __VA_ARGS__); \
     } while (0)
 
-void l0_brick_info(slosh_ctx *c)
+void l0_brick_info(slosh_ctx *c, bool readback)
 {
     SLOSH_TIMED(c, K_MG_SETUP);
     const uint32_t nb = (c->nactive + 255u) / 256u;
@@ -525,11 +525,14 @@ void l0_brick_info(slosh_ctx *c)
     scan_exclusive(c, c->fluid_flag, c->fluid_rank, c->nactive);
     hipLaunchKernelGGL(k_brick_info, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nactive, c->binfo, c->fluid_flag,
                        c->fluid_rank, c->binfo_f);
-    // the level-0 and pooled passes launch one workgroup per brick with fluid: their count comes back here
+    // the level-0 and pooled passes launch one workgroup per brick with fluid: their count comes back here (or with
+    // the caller's readback)
     SLOSH_HIP(hipMemcpyAsync(c->h_nsolve, c->fluid_rank + c->nactive, sizeof(uint32_t), hipMemcpyDeviceToHost,
                              c->stream));
-    SLOSH_HIP(hipStreamSynchronize(c->stream));
-    c->nsolve = *c->h_nsolve;
+    if (readback) {
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        c->nsolve = c->h_nsolve[0];
+    }
 }
 
 // Over every allocated brick (non-fluid cells get r = 0, also in bricks without fluid).

@@ -244,8 +244,11 @@ struct slosh_ctx {
     // visit these only (a brick without fluid has no fluid cell on levels 0-2 and nothing reads its values)
     uint32_t *binfo_f = nullptr;
     uint32_t *fluid_flag = nullptr, *fluid_rank = nullptr;
-    uint32_t *h_nsolve = nullptr; // pinned: the count of such bricks, read back once per solve
+    uint32_t *h_nsolve = nullptr; // pinned: [0] the count of such bricks, [1] of level-3 tiles with fluid bricks
     uint32_t nsolve = 0;
+    // 4^3 tiles of level 3 whose bricks hold fluid: the fused level-2/3 passes visit these only
+    uint32_t *tile_flag = nullptr, *tile_rank = nullptr, *tile_list = nullptr;
+    uint32_t ntiles = 0, ntiles_f = 0;
     int nlevels = 1;
     slosh::Level lv[slosh::kMaxLevels]{};
     uint32_t *coarse_idx = nullptr; // [kCoarsestCells]
@@ -342,7 +345,7 @@ void pooled_down(slosh_ctx *c, int l);
 void pooled_up(slosh_ctx *c, int l);
 
 // ---- level0.hip: finest-level solver passes (one brick per workgroup; one reduction partial per brick) ----
-void l0_brick_info(slosh_ctx *c);
+void l0_brick_info(slosh_ctx *c, bool readback = true);
 void l0_pcg_init(slosh_ctx *c, float *r_out);
 // p += alpha d, r_out = r_in - alpha q (first: r_out = r_in), z = r_out / diag on red cells, then the black sweep;
 // partials ||r||^2 unless first.

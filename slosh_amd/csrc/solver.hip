@@ -255,182 +255,360 @@ __global__ void k_drestrict(DenseLevel F, DenseLevel C, const SolverState *st)
     if (i < C.cells) restrict_cell(F, C, i);
 }
 
-// ---- level 3 (one cell per brick, dense by brick index): one wavefront per 4^3 tile, the tile and a two-cell halo
-// staged in LDS (the passes move little data; all of a tile's loads are issued at once, addresses from arithmetic).
-constexpr int kD3T = 4;                       // tile edge
-constexpr int kD3B = kD3T + 4;                // box edge: the tile and a two-cell halo
-constexpr int kD3NB = kD3B * kD3B * kD3B;     // box cells
-constexpr int kD3In = kD3T + 2;               // the tile and a one-cell halo
-constexpr uint32_t kD3Threads = 64;
-constexpr int kD3Per = kD3NB / (int)kD3Threads;
+// ---- levels 2 and 3 fused: one workgroup per 4^3 tile of level 3 holding fluid (the tile list is built once per
+// solve). Level 2 is pooled (2^3 cells per brick slot) and is gathered through the brick table; level 3 is dense by
+// brick index. Each pass stages the halos its sweeps need and recomputes there what neighbouring tiles also compute,
+// with the same expressions and order as the separate level-2 and level-3 passes.
+constexpr int kT3 = 4;                          // level-3 tile edge
+constexpr int kDB2 = 2 * kT3 + 12;              // down: level-2 box edge (the tile's 8^3 cells and a 6-cell halo)
+constexpr int kDNB2 = kDB2 * kDB2 * kDB2;
+constexpr int kDBB = kT3 + 6;                   // down: bricks per edge under the level-2 box
+constexpr int kDR3 = kT3 + 4;                   // down: level-3 region edge (the tile and a 2-cell halo)
+constexpr int kDR2 = 2 * kDR3;                  // down: level-2 children of the level-3 region
+constexpr uint32_t kD23DownThreads = 1024;
+constexpr int kUB3 = kT3 + 6;                   // up: level-3 box edge (a 3-cell halo)
+constexpr int kUB2 = 2 * kT3 + 4;               // up: level-2 box edge (a 2-cell halo)
+constexpr int kUBB = kT3 + 2;                   // up: bricks per edge under the level-2 box
+constexpr uint32_t kD23UpThreads = 512;
 
-__device__ __forceinline__ int d3_box(int u, int v, int w) { return (u + 2) + kD3B * ((v + 2) + kD3B * (w + 2)); }
+__device__ __forceinline__ void tile_of(const DenseLevel &L3, uint32_t tile, int &tx, int &ty, int &tz)
+{
+    const int ntx = (L3.dim[0] + kT3 - 1) / kT3, nty = (L3.dim[1] + kT3 - 1) / kT3;
+    tx = (int)tile % ntx;
+    ty = ((int)tile / ntx) % nty;
+    tz = (int)tile / (ntx * nty);
+}
 
-__device__ __forceinline__ float d3_box_sum(const float *v, int k, uint32_t info)
+// Offset of neighbour n (order -x +x -y +y -z +z) in a box of edge E.
+template <int E> __device__ __forceinline__ int box_off(int n)
+{
+    const int s = n < 2 ? 1 : (n < 4 ? E : E * E);
+    return (n & 1) ? s : -s;
+}
+
+template <int E> __device__ __forceinline__ float box_nbsum(const float *v, int k, uint32_t info)
 {
     float s = 0.0f;
     for (int n = 0; n < 6; ++n)
-        if (info >> n & 1u) s += v[k + ((n < 2) ? 1 : (n < 4 ? kD3B : kD3B * kD3B)) * ((n & 1) ? 1 : -1)];
+        if (info >> n & 1u) s += v[k + box_off<E>(n)];
     return s;
 }
 
-// Level-3 cell of box cell k of tile (tx, ty, tz): its coordinates and whether it lies in the level.
-__device__ __forceinline__ bool d3_cell(const DenseLevel &L, int tx, int ty, int tz, int k, int &x, int &y, int &z)
+// Brick slots under a box of level-2 cells starting at brick (b0x, b0y, b0z), NB bricks per edge (0 outside the
+// grid or unallocated).
+template <int NB>
+__device__ __forceinline__ void stage_slots(const Geo &g, int b0x, int b0y, int b0z, uint32_t *s_slot, uint32_t nt)
 {
-    x = tx * kD3T + (k % kD3B) - 2;
-    y = ty * kD3T + ((k / kD3B) % kD3B) - 2;
-    z = tz * kD3T + k / (kD3B * kD3B) - 2;
-    return x >= 0 && y >= 0 && z >= 0 && x < L.dim[0] && y < L.dim[1] && z < L.dim[2];
+    for (uint32_t k = threadIdx.x; k < (uint32_t)(NB * NB * NB); k += nt) {
+        const int bx = b0x + (int)(k % NB), by = b0y + (int)((k / NB) % NB), bz = b0z + (int)(k / (NB * NB));
+        const bool in = bx >= 0 && by >= 0 && bz >= 0 && bx < g.bd[0] && by < g.bd[1] && bz < g.bd[2];
+        s_slot[k] = in ? g.table[(uint32_t)(bx + g.bd[0] * (by + g.bd[1] * bz))] : 0u;
+    }
 }
 
-__device__ __forceinline__ void d3_tile(const DenseLevel &L, int &tx, int &ty, int &tz)
+// Pool index of level-2 cell (x, y, z) through the staged slots (bricks from (b0x, b0y, b0z), NB per edge); cells
+// outside level 2 read the null slot, whose words are 0.
+template <int NB>
+__device__ __forceinline__ uint32_t l2_index(const PLevel &L2, const uint32_t *s_slot, int b0x, int b0y, int b0z, int x,
+                                             int y, int z)
 {
-    const int ntx = (L.dim[0] + kD3T - 1) / kD3T, nty = (L.dim[1] + kD3T - 1) / kD3T;
-    tx = (int)blockIdx.x % ntx;
-    ty = ((int)blockIdx.x / ntx) % nty;
-    tz = (int)blockIdx.x / (ntx * nty);
+    if (!pinside(L2, x, y, z)) return 0u;
+    const uint32_t slot = s_slot[((x >> 1) - b0x) + NB * (((y >> 1) - b0y) + NB * ((z >> 1) - b0z))];
+    return slot * 8u + (uint32_t)((x & 1) + 2 * ((y & 1) + 2 * (z & 1)));
 }
 
-// Level 3, down pass fused with the restriction into level 4 (MGSmoothCS black, then MGRestrictCS with the fused red
-// sweep): the black sweep on the tile and its one-cell halo (from the staged red values), the residual of every
-// tile cell (0 where not fluid), then b_p = 1/4 of the children's residuals summed in child order for the tile's 2^3
-// level-4 cells and x_p = b_p / diag_p on red ones. Black values of the tile go back to memory.
-__global__ __launch_bounds__(kD3Threads) void k_d3down(DenseLevel L, DenseLevel C, const SolverState *st)
+// Down pass of levels 2 and 3 for one tile: the level-2 black sweep (MGSmoothCS) and residual on the children of the
+// tile's level-3 cells and their 2-cell halo, the restriction into level 3 with its fused red sweep (MGRestrictCS),
+// the level-3 black sweep and residual, and the restriction into the tile's 2^3 level-4 cells. Stored: the level-2
+// black values of the tile's cells, level 3's b and x on the tile, level 4's b and x.
+__global__ __launch_bounds__(kD23DownThreads) void k_d23down(Geo g, const uint32_t *tiles, PLevel L2, DenseLevel L3,
+                                                             DenseLevel L4, const SolverState *st)
 {
-    __shared__ float s_x[kD3NB], s_b[kD3NB];
-    __shared__ uint16_t s_info[kD3NB];
-    __shared__ float s_res[kD3T * kD3T * kD3T];
-    int tx, ty, tz;
-    d3_tile(L, tx, ty, tz);
+    constexpr uint32_t NT = kD23DownThreads;
+    constexpr int kPer = (kDNB2 + (int)NT - 1) / (int)NT;
+    __shared__ uint32_t s_slot[kDBB * kDBB * kDBB];
+    __shared__ float s_x[kDNB2], s_b[kDNB2];
+    __shared__ uint16_t s_info[kDNB2];
+    __shared__ float s_r2[kDR2 * kDR2 * kDR2];
+    __shared__ float s_x3[kDR3 * kDR3 * kDR3], s_b3[kDR3 * kDR3 * kDR3];
+    __shared__ uint16_t s_i3[kDR3 * kDR3 * kDR3];
+    __shared__ float s_r3[kT3 * kT3 * kT3];
     const uint32_t t = threadIdx.x;
+    const uint32_t tile = tiles[blockIdx.x];
     const uint32_t done = st->done;
-    uint32_t vi[kD3Per];
-    float vx[kD3Per], vb[kD3Per];
-    for (int q = 0; q < kD3Per; ++q) {
-        int x, y, z;
-        const bool in = d3_cell(L, tx, ty, tz, (int)t + (int)kD3Threads * q, x, y, z);
-        const uint32_t gi = in ? (uint32_t)(x + L.dim[0] * (y + L.dim[1] * z)) : 0u;
-        vi[q] = in ? (uint32_t)L.info[gi] : 0u;
-        vx[q] = L.x[gi];
-        vb[q] = L.b[gi];
+    issue_point();
+    if (done) return;
+    int tx, ty, tz;
+    tile_of(L3, tile, tx, ty, tz);
+    const int o2x = 8 * tx - 6, o2y = 8 * ty - 6, o2z = 8 * tz - 6;  // level-2 box origin
+    const int bbx = 4 * tx - 3, bby = 4 * ty - 3, bbz = 4 * tz - 3;  // first brick under it
+    const int o3x = 4 * tx - 2, o3y = 4 * ty - 2, o3z = 4 * tz - 2;  // level-3 region origin
+    stage_slots<kDBB>(g, bbx, bby, bbz, s_slot, NT);
+    __syncthreads();
+    uint32_t vi[kPer];
+    float vx[kPer], vb[kPer];
+    for (int q = 0; q < kPer; ++q) {
+        const int k = (int)t + (int)NT * q;
+        vi[q] = 0u;
+        vx[q] = vb[q] = 0.0f;
+        if (k >= kDNB2) continue;
+        const uint32_t gi = l2_index<kDBB>(L2, s_slot, bbx, bby, bbz, o2x + k % kDB2, o2y + (k / kDB2) % kDB2,
+                                           o2z + k / (kDB2 * kDB2));
+        vi[q] = L2.info[gi];
+        vx[q] = L2.x[gi];
+        vb[q] = L2.b[gi];
+    }
+    uint32_t wi3 = 0u;
+    if (t < (uint32_t)(kDR3 * kDR3 * kDR3)) {
+        const int x = o3x + (int)t % kDR3, y = o3y + ((int)t / kDR3) % kDR3, z = o3z + (int)t / (kDR3 * kDR3);
+        if (x >= 0 && y >= 0 && z >= 0 && x < L3.dim[0] && y < L3.dim[1] && z < L3.dim[2])
+            wi3 = L3.info[(uint32_t)(x + L3.dim[0] * (y + L3.dim[1] * z))];
     }
     // the tile's level-4 cells: thread t < 8 takes child block t
     const int cx = 2 * tx + (int)(t & 1u), cy = 2 * ty + (int)((t >> 1) & 1u), cz = 2 * tz + (int)((t >> 2) & 1u);
-    const bool cin = t < 8u && cx < C.dim[0] && cy < C.dim[1] && cz < C.dim[2];
-    const uint32_t ci = cin ? (uint32_t)(cx + C.dim[0] * (cy + C.dim[1] * cz)) : 0u;
-    const uint32_t cinfo = C.info[ci];
-    issue_point();
-    if (done) return;
-    for (int q = 0; q < kD3Per; ++q) {
-        const int k = (int)t + (int)kD3Threads * q;
+    const bool cin = t < 8u && cx < L4.dim[0] && cy < L4.dim[1] && cz < L4.dim[2];
+    const uint32_t ci = cin ? (uint32_t)(cx + L4.dim[0] * (cy + L4.dim[1] * cz)) : 0u;
+    const uint32_t cinfo = L4.info[ci];
+    for (int q = 0; q < kPer; ++q) {
+        const int k = (int)t + (int)NT * q;
+        if (k >= kDNB2) continue;
         const bool f = (vi[q] & kInfoSelf) != 0u;
         s_info[k] = (uint16_t)vi[q];
         s_x[k] = f ? vx[q] : 0.0f;
         s_b[k] = f ? vb[q] : 0.0f;
     }
+    if (t < (uint32_t)(kDR3 * kDR3 * kDR3)) s_i3[t] = (uint16_t)wi3;
     __syncthreads();
-    for (int k = (int)t; k < kD3In * kD3In * kD3In; k += (int)kD3Threads) {
-        const int kb = d3_box(k % kD3In - 1, (k / kD3In) % kD3In - 1, k / (kD3In * kD3In) - 1);
+    // level 2: black sweep on the box less a 1-cell rim
+    for (int k = (int)t; k < (kDB2 - 2) * (kDB2 - 2) * (kDB2 - 2); k += (int)NT) {
+        constexpr int E = kDB2 - 2;
+        const int kb = (1 + k % E) + kDB2 * ((1 + (k / E) % E) + kDB2 * (1 + k / (E * E)));
         const uint32_t info = s_info[kb];
         if (!(info & kInfoSelf) || (info & kInfoRed)) continue;
-        s_x[kb] = (s_b[kb] + d3_box_sum(s_x, kb, info)) / info_diag(info);
+        s_x[kb] = (s_b[kb] + box_nbsum<kDB2>(s_x, kb, info)) / info_diag(info);
     }
     __syncthreads();
-    {
-        const int u = (int)(t % kD3T), v = (int)((t / kD3T) % kD3T), w = (int)(t / (kD3T * kD3T));
-        const int kb = d3_box(u, v, w);
+    // level 2: residual of the children of the level-3 region (the box less a 2-cell rim); black values of the
+    // tile's own level-2 cells go back to memory
+    for (int k = (int)t; k < kDR2 * kDR2 * kDR2; k += (int)NT) {
+        const int u = 2 + k % kDR2, v = 2 + (k / kDR2) % kDR2, w = 2 + k / (kDR2 * kDR2);
+        const int kb = u + kDB2 * (v + kDB2 * w);
         const uint32_t info = s_info[kb];
         float r = 0.0f;
         if (info & kInfoSelf) {
-            if (!(info & kInfoRed)) {
-                const int x = tx * kD3T + u, y = ty * kD3T + v, z = tz * kD3T + w;
-                L.x[(uint32_t)(x + L.dim[0] * (y + L.dim[1] * z))] = s_x[kb];
-            }
-            r = s_b[kb] - (info_diag(info) * s_x[kb] - d3_box_sum(s_x, kb, info));
+            if (!(info & kInfoRed) && u >= 6 && u < 14 && v >= 6 && v < 14 && w >= 6 && w < 14)
+                L2.x[l2_index<kDBB>(L2, s_slot, bbx, bby, bbz, o2x + u, o2y + v, o2z + w)] = s_x[kb];
+            r = s_b[kb] - (info_diag(info) * s_x[kb] - box_nbsum<kDB2>(s_x, kb, info));
         }
-        s_res[t] = r;
+        s_r2[k] = r;
+    }
+    __syncthreads();
+    // restriction into the level-3 region, in child order, with the fused red sweep
+    if (t < (uint32_t)(kDR3 * kDR3 * kDR3)) {
+        const int i = (int)t % kDR3, j = ((int)t / kDR3) % kDR3, l = (int)t / (kDR3 * kDR3);
+        const uint32_t c3 = s_i3[t];
+        float bv = 0.0f, xv = 0.0f;
+        if (c3 & kInfoSelf) {
+            float sum = 0.0f;
+            for (uint32_t oi = 0; oi < 8u; ++oi) {
+                const int fu = 2 * i + (int)(oi & 1u), fv = 2 * j + (int)((oi >> 1) & 1u), fw = 2 * l + (int)(oi >> 2);
+                if (s_info[(2 + fu) + kDB2 * ((2 + fv) + kDB2 * (2 + fw))] & kInfoSelf)
+                    sum += s_r2[fu + kDR2 * (fv + kDR2 * fw)];
+            }
+            bv = 0.25f * sum;
+            xv = (c3 & kInfoRed) ? bv / info_diag(c3) : 0.0f;
+        }
+        s_b3[t] = bv;
+        s_x3[t] = xv;
+    }
+    __syncthreads();
+    // level 3: black sweep on the region less a 1-cell rim
+    if (t < (uint32_t)((kDR3 - 2) * (kDR3 - 2) * (kDR3 - 2))) {
+        constexpr int E = kDR3 - 2;
+        const int k3 = (1 + (int)t % E) + kDR3 * ((1 + ((int)t / E) % E) + kDR3 * (1 + (int)t / (E * E)));
+        const uint32_t info = s_i3[k3];
+        if ((info & kInfoSelf) && !(info & kInfoRed))
+            s_x3[k3] = (s_b3[k3] + box_nbsum<kDR3>(s_x3, k3, info)) / info_diag(info);
+    }
+    __syncthreads();
+    // level 3: residual of the tile; the tile's b and x go back to memory
+    if (t < (uint32_t)(kT3 * kT3 * kT3)) {
+        const int u = (int)t % kT3, v = ((int)t / kT3) % kT3, w = (int)t / (kT3 * kT3);
+        const int k3 = (2 + u) + kDR3 * ((2 + v) + kDR3 * (2 + w));
+        const uint32_t info = s_i3[k3];
+        const int x = 4 * tx + u, y = 4 * ty + v, z = 4 * tz + w;
+        float r = 0.0f;
+        if (x < L3.dim[0] && y < L3.dim[1] && z < L3.dim[2]) {
+            const uint32_t gi = (uint32_t)(x + L3.dim[0] * (y + L3.dim[1] * z));
+            L3.b[gi] = s_b3[k3];
+            L3.x[gi] = s_x3[k3];
+            if (info & kInfoSelf) r = s_b3[k3] - (info_diag(info) * s_x3[k3] - box_nbsum<kDR3>(s_x3, k3, info));
+        }
+        s_r3[t] = r;
     }
     __syncthreads();
     if (!cin) return;
     if (!(cinfo & kInfoSelf)) {
-        C.b[ci] = 0.0f;
-        C.x[ci] = 0.0f;
+        L4.b[ci] = 0.0f;
+        L4.x[ci] = 0.0f;
         return;
     }
-    // children in child order; one outside level 3 has residual 0 (adding +0 leaves the sum, which starts at +0 and
-    // never becomes -0, unchanged)
+    // children in child order; one outside level 3 has residual 0 (adding +0 leaves the sum, which starts at +0
+    // and never becomes -0, unchanged)
     float sum = 0.0f;
     for (uint32_t oi = 0; oi < 8u; ++oi) {
         const int u = 2 * (int)(t & 1u) + (int)(oi & 1u), v = 2 * (int)((t >> 1) & 1u) + (int)((oi >> 1) & 1u),
                   w = 2 * (int)((t >> 2) & 1u) + (int)(oi >> 2);
-        sum += s_res[u + kD3T * (v + kD3T * w)];
+        sum += s_r3[u + kT3 * (v + kT3 * w)];
     }
     const float bv = 0.25f * sum;
-    C.b[ci] = bv;
-    C.x[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
+    L4.b[ci] = bv;
+    L4.x[ci] = (cinfo & kInfoRed) ? bv / info_diag(cinfo) : 0.0f;
 }
 
-// Level 3, up pass from level 4 (MGProlongCS, then MGSmoothCS black and red): the prolonged down-sweep iterate
-// x + x4(parent) (a parent contributes only when it is fluid) on the box, the black sweep on the tile and its
-// one-cell halo, the red sweep on the tile; result in x1.
-__global__ __launch_bounds__(kD3Threads) void k_d3up(DenseLevel L, DenseLevel P, const SolverState *st)
+// Up pass of levels 3 and 2 for one tile (MGProlongCS, then MGSmoothCS black and red on each level): level 3's
+// prolonged iterate x + x4(parent) on the tile and a 3-cell halo, its black sweep and red sweep, which give level 3's
+// result on the tile and a 1-cell halo; level 2's prolonged iterate from it, its black sweep and red sweep; level 2's
+// result x1 on the tile's cells goes back to memory.
+__global__ __launch_bounds__(kD23UpThreads) void k_d23up(Geo g, const uint32_t *tiles, PLevel L2, DenseLevel L3,
+                                                         DenseLevel L4, const SolverState *st)
 {
-    __shared__ float s_v[kD3NB], s_b[kD3NB];
-    __shared__ uint16_t s_info[kD3NB];
-    int tx, ty, tz;
-    d3_tile(L, tx, ty, tz);
+    constexpr uint32_t NT = kD23UpThreads;
+    constexpr int kN3 = kUB3 * kUB3 * kUB3, kN2 = kUB2 * kUB2 * kUB2;
+    constexpr int kPer3 = (kN3 + (int)NT - 1) / (int)NT, kPer2 = (kN2 + (int)NT - 1) / (int)NT;
+    __shared__ uint32_t s_slot[kUBB * kUBB * kUBB];
+    __shared__ float s_v3[kN3], s_b3[kN3];
+    __shared__ uint16_t s_i3[kN3];
+    __shared__ float s_v2[kN2], s_b2[kN2];
+    __shared__ uint16_t s_i2[kN2];
     const uint32_t t = threadIdx.x;
+    const uint32_t tile = tiles[blockIdx.x];
     const uint32_t done = st->done;
-    uint32_t vi[kD3Per], pinf[kD3Per];
-    float vx[kD3Per], vb[kD3Per], px[kD3Per];
-    for (int q = 0; q < kD3Per; ++q) {
-        int x, y, z;
-        const bool in = d3_cell(L, tx, ty, tz, (int)t + (int)kD3Threads * q, x, y, z);
-        const uint32_t gi = in ? (uint32_t)(x + L.dim[0] * (y + L.dim[1] * z)) : 0u;
-        const uint32_t pi = in ? (uint32_t)((x >> 1) + P.dim[0] * ((y >> 1) + P.dim[1] * (z >> 1))) : 0u;
-        vi[q] = in ? (uint32_t)L.info[gi] : 0u;
-        vx[q] = L.x[gi];
-        vb[q] = L.b[gi];
-        pinf[q] = P.info[pi];
-        px[q] = P.x[pi];
-    }
     issue_point();
     if (done) return;
-    for (int q = 0; q < kD3Per; ++q) {
-        const int k = (int)t + (int)kD3Threads * q;
+    int tx, ty, tz;
+    tile_of(L3, tile, tx, ty, tz);
+    const int o3x = 4 * tx - 3, o3y = 4 * ty - 3, o3z = 4 * tz - 3;  // level-3 box origin
+    const int o2x = 8 * tx - 2, o2y = 8 * ty - 2, o2z = 8 * tz - 2;  // level-2 box origin
+    const int bbx = 4 * tx - 1, bby = 4 * ty - 1, bbz = 4 * tz - 1;  // first brick under it
+    stage_slots<kUBB>(g, bbx, bby, bbz, s_slot, NT);
+    uint32_t wi3[kPer3], wp3[kPer3];
+    float wx3[kPer3], wb3[kPer3], wpx[kPer3];
+    for (int q = 0; q < kPer3; ++q) {
+        const int k = (int)t + (int)NT * q;
+        wi3[q] = wp3[q] = 0u;
+        wx3[q] = wb3[q] = wpx[q] = 0.0f;
+        if (k >= kN3) continue;
+        const int x = o3x + k % kUB3, y = o3y + (k / kUB3) % kUB3, z = o3z + k / (kUB3 * kUB3);
+        const bool in = x >= 0 && y >= 0 && z >= 0 && x < L3.dim[0] && y < L3.dim[1] && z < L3.dim[2];
+        const uint32_t gi = in ? (uint32_t)(x + L3.dim[0] * (y + L3.dim[1] * z)) : 0u;
+        const uint32_t pi = in ? (uint32_t)((x >> 1) + L4.dim[0] * ((y >> 1) + L4.dim[1] * (z >> 1))) : 0u;
+        wi3[q] = in ? (uint32_t)L3.info[gi] : 0u;
+        wx3[q] = L3.x[gi];
+        wb3[q] = L3.b[gi];
+        wp3[q] = L4.info[pi];
+        wpx[q] = L4.x[pi];
+    }
+    __syncthreads();
+    uint32_t wi2[kPer2];
+    float wx2[kPer2], wb2[kPer2];
+    for (int q = 0; q < kPer2; ++q) {
+        const int k = (int)t + (int)NT * q;
+        wi2[q] = 0u;
+        wx2[q] = wb2[q] = 0.0f;
+        if (k >= kN2) continue;
+        const uint32_t gi = l2_index<kUBB>(L2, s_slot, bbx, bby, bbz, o2x + k % kUB2, o2y + (k / kUB2) % kUB2,
+                                           o2z + k / (kUB2 * kUB2));
+        wi2[q] = L2.info[gi];
+        wx2[q] = L2.x[gi];
+        wb2[q] = L2.b[gi];
+    }
+    for (int q = 0; q < kPer3; ++q) {
+        const int k = (int)t + (int)NT * q;
+        if (k >= kN3) continue;
         float val = 0.0f, bv = 0.0f;
-        if (vi[q] & kInfoSelf) {
-            val = vx[q];
-            if (pinf[q] & kInfoSelf) val += px[q];
-            bv = vb[q];
+        if (wi3[q] & kInfoSelf) {
+            val = wx3[q];
+            if (wp3[q] & kInfoSelf) val += wpx[q];
+            bv = wb3[q];
         }
-        s_info[k] = (uint16_t)vi[q];
-        s_v[k] = val;
-        s_b[k] = bv;
+        s_i3[k] = (uint16_t)wi3[q];
+        s_v3[k] = val;
+        s_b3[k] = bv;
+    }
+    for (int q = 0; q < kPer2; ++q) {
+        const int k = (int)t + (int)NT * q;
+        if (k >= kN2) continue;
+        const bool f = (wi2[q] & kInfoSelf) != 0u;
+        s_i2[k] = (uint16_t)wi2[q];
+        s_v2[k] = f ? wx2[q] : 0.0f;
+        s_b2[k] = f ? wb2[q] : 0.0f;
     }
     __syncthreads();
-    for (int k = (int)t; k < kD3In * kD3In * kD3In; k += (int)kD3Threads) {
-        const int kb = d3_box(k % kD3In - 1, (k / kD3In) % kD3In - 1, k / (kD3In * kD3In) - 1);
-        const uint32_t info = s_info[kb];
-        if (!(info & kInfoSelf) || (info & kInfoRed)) continue;
-        s_v[kb] = (s_b[kb] + d3_box_sum(s_v, kb, info)) / info_diag(info);
+    // level 3: black sweep on the box less a 1-cell rim, then red sweep on the box less a 2-cell rim (in place:
+    // a red cell reads only black neighbours)
+    for (int k = (int)t; k < (kUB3 - 2) * (kUB3 - 2) * (kUB3 - 2); k += (int)NT) {
+        constexpr int E = kUB3 - 2;
+        const int kb = (1 + k % E) + kUB3 * ((1 + (k / E) % E) + kUB3 * (1 + k / (E * E)));
+        const uint32_t info = s_i3[kb];
+        if ((info & kInfoSelf) && !(info & kInfoRed))
+            s_v3[kb] = (s_b3[kb] + box_nbsum<kUB3>(s_v3, kb, info)) / info_diag(info);
     }
     __syncthreads();
-    const int u = (int)(t % kD3T), v = (int)((t / kD3T) % kD3T), w = (int)(t / (kD3T * kD3T));
-    const int x = tx * kD3T + u, y = ty * kD3T + v, z = tz * kD3T + w;
-    if (x >= L.dim[0] || y >= L.dim[1] || z >= L.dim[2]) return;
-    const int kb = d3_box(u, v, w);
-    const uint32_t info = s_info[kb];
-    if (!(info & kInfoSelf)) return;
-    L.x1[(uint32_t)(x + L.dim[0] * (y + L.dim[1] * z))] =
-        (info & kInfoRed) ? (s_b[kb] + d3_box_sum(s_v, kb, info)) / info_diag(info) : s_v[kb];
+    for (int k = (int)t; k < (kUB3 - 4) * (kUB3 - 4) * (kUB3 - 4); k += (int)NT) {
+        constexpr int E = kUB3 - 4;
+        const int kb = (2 + k % E) + kUB3 * ((2 + (k / E) % E) + kUB3 * (2 + k / (E * E)));
+        const uint32_t info = s_i3[kb];
+        if ((info & kInfoSelf) && (info & kInfoRed))
+            s_v3[kb] = (s_b3[kb] + box_nbsum<kUB3>(s_v3, kb, info)) / info_diag(info);
+    }
+    __syncthreads();
+    // level 2: prolonged iterate (a parent contributes only when it is fluid), black sweep on the box less a
+    // 1-cell rim, red sweep on the tile
+    for (int k = (int)t; k < kN2; k += (int)NT) {
+        if (!(s_i2[k] & kInfoSelf)) continue;
+        const int x = o2x + k % kUB2, y = o2y + (k / kUB2) % kUB2, z = o2z + k / (kUB2 * kUB2);
+        const int k3 = ((x >> 1) - o3x) + kUB3 * (((y >> 1) - o3y) + kUB3 * ((z >> 1) - o3z));
+        if (s_i3[k3] & kInfoSelf) s_v2[k] += s_v3[k3];
+    }
+    __syncthreads();
+    for (int k = (int)t; k < (kUB2 - 2) * (kUB2 - 2) * (kUB2 - 2); k += (int)NT) {
+        constexpr int E = kUB2 - 2;
+        const int kb = (1 + k % E) + kUB2 * ((1 + (k / E) % E) + kUB2 * (1 + k / (E * E)));
+        const uint32_t info = s_i2[kb];
+        if ((info & kInfoSelf) && !(info & kInfoRed))
+            s_v2[kb] = (s_b2[kb] + box_nbsum<kUB2>(s_v2, kb, info)) / info_diag(info);
+    }
+    __syncthreads();
+    for (int k = (int)t; k < (kUB2 - 4) * (kUB2 - 4) * (kUB2 - 4); k += (int)NT) {
+        constexpr int E = kUB2 - 4;
+        const int u = 2 + k % E, v = 2 + (k / E) % E, w = 2 + k / (E * E);
+        const int kb = u + kUB2 * (v + kUB2 * w);
+        const uint32_t info = s_i2[kb];
+        if (!(info & kInfoSelf)) continue;
+        L2.x1[l2_index<kUBB>(L2, s_slot, bbx, bby, bbz, o2x + u, o2y + v, o2z + w)] =
+            (info & kInfoRed) ? (s_b2[kb] + box_nbsum<kUB2>(s_v2, kb, info)) / info_diag(info) : s_v2[kb];
+    }
 }
 
-// Tiles of the level-3 passes.
-uint32_t d3_tiles(const DenseLevel &L)
+// Tiles of level 3 whose bricks hold fluid cells (fluid on levels 2 and 3 needs a fluid cell below it in the same
+// brick under either coarsening rule): flag per tile, set by the bricks (the flags start at 0).
+__global__ void k_tile_flags(Geo g, uint32_t nactive, const uint32_t *fluid_flag, DenseLevel L3, uint32_t *flag)
 {
-    return (uint32_t)(((L.dim[0] + kD3T - 1) / kD3T) * ((L.dim[1] + kD3T - 1) / kD3T) * ((L.dim[2] + kD3T - 1) / kD3T));
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nactive || !fluid_flag[e]) return;
+    int bc[3];
+    brick_coord(g, g.active[e], bc);
+    const int ntx = (L3.dim[0] + kT3 - 1) / kT3, nty = (L3.dim[1] + kT3 - 1) / kT3;
+    flag[(uint32_t)(bc[0] / kT3 + ntx * (bc[1] / kT3 + nty * (bc[2] / kT3)))] = 1u;
 }
+
+__global__ void k_tile_list(uint32_t ntiles, const uint32_t *flag, const uint32_t *rank, uint32_t *list)
+{
+    const uint32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile < ntiles && flag[tile]) list[rank[tile]] = tile;
+}
+
 
 __global__ void k_dprolong(DenseLevel F, DenseLevel C, const SolverState *st)
 {
@@ -828,17 +1006,18 @@ void coarse_solve(slosh_ctx *c, LVec b, LVec x)
                        x, c->state);
 }
 
-// Levels 3 .. m-1 (m >= 5): level 3's down pass with the restriction into level 4, grid-wide passes for dense
-// levels kFirstDense .. t-1 (only when the dense levels do not all fit the tail's LDS), the tail on levels
-// t .. m-1, the grid-wide up passes, level 3's up pass.
+// Levels 2 .. m-1 (m >= 5): the fused level-2/3 down pass with the restriction into level 4, grid-wide passes for
+// dense levels kFirstDense .. t-1 (only when the dense levels do not all fit the tail's LDS), the tail on levels
+// t .. m-1, the grid-wide up passes, the fused level-3/2 up pass.
 void dense_part(slosh_ctx *c)
 {
     const int m = c->nlevels;
-    const DenseLevel L3 = dlevel_of(c, 3);
+    const DenseLevel L3 = dlevel_of(c, 3), L4 = dlevel_of(c, kFirstDense);
+    const PLevel L2 = plevel_of(c, 2);
     {
         SLOSH_TIMED(c, K_D3DOWN);
-        const DenseLevel L4 = dlevel_of(c, kFirstDense);
-        hipLaunchKernelGGL(k_d3down, dim3(d3_tiles(L3)), dim3(kD3Threads), 0, c->stream, L3, L4, c->state);
+        hipLaunchKernelGGL(k_d23down, dim3(c->ntiles_f), dim3(kD23DownThreads), 0, c->stream, c->geo, c->tile_list, L2,
+                           L3, L4, c->state);
     }
     int t = m - 1;
     uint32_t tail_cells = c->lv[m - 1].cells;
@@ -885,7 +1064,7 @@ void dense_part(slosh_ctx *c)
             }
     }
     SLOSH_TIMED(c, K_D3UP);
-    hipLaunchKernelGGL(k_d3up, dim3(d3_tiles(L3)), dim3(kD3Threads), 0, c->stream, L3, dlevel_of(c, kFirstDense),
+    hipLaunchKernelGGL(k_d23up, dim3(c->ntiles_f), dim3(kD23UpThreads), 0, c->stream, c->geo, c->tile_list, L2, L3, L4,
                        c->state);
 }
 
@@ -898,7 +1077,9 @@ void vcycle(slosh_ctx *c, const float *r)
         coarse_solve(c, LVec{(void *)r, 0}, LVec{c->z1, 0});
     } else {
         l0_restrict(c, r);
-        const int last_pooled = m - 2 < 2 ? m - 2 : 2; // pooled levels with a down pass (1, 2)
+        // pooled levels with a down pass of their own: 1 and 2 when level 3 is the coarsest, else 1 (level 2 is
+        // fused with level 3 in the dense part)
+        const int last_pooled = m - 2 < 2 ? m - 2 : (m - 1 <= 3 ? 2 : 1);
         for (int l = 1; l <= last_pooled; ++l) pooled_down(c, l);
         if (m - 1 <= 3) coarse_solve(c, LVec{c->lv[m - 1].b, c->lv[m - 1].half}, LVec{c->lv[m - 1].x1, c->lv[m - 1].half});
         else dense_part(c);
@@ -912,7 +1093,7 @@ void vcycle(slosh_ctx *c, const float *r)
 // Per-solve set-up: brick table of the allocated bricks, coarse types and stencil words, coarsest factor.
 void mg_setup(slosh_ctx *c)
 {
-    l0_brick_info(c);
+    l0_brick_info(c, false);
     const int m = c->nlevels;
     for (int l = 1; l < m; ++l) {
         if (c->lv[l].pooled) {
@@ -935,6 +1116,24 @@ void mg_setup(slosh_ctx *c)
             hipLaunchKernelGGL(k_dinfo, dim3(blocks_for(D.cells, 256)), dim3(256), 0, c->stream, D);
         }
     }
+    // tiles of level 3 for the fused level-2/3 passes (levels 3 and 4 exist below a coarser level)
+    c->ntiles_f = 0;
+    if (m >= 5) {
+        const DenseLevel L3 = dlevel_of(c, 3);
+        SLOSH_TIMED(c, K_MG_SETUP);
+        launch_fill_u32(c->stream, c->tile_flag, 0u, c->ntiles);
+        hipLaunchKernelGGL(k_tile_flags, dim3((c->nactive + 255u) / 256u), dim3(256), 0, c->stream, c->geo, c->nactive,
+                           c->fluid_flag, L3, c->tile_flag);
+        scan_exclusive(c, c->tile_flag, c->tile_rank, c->ntiles);
+        hipLaunchKernelGGL(k_tile_list, dim3((c->ntiles + 255u) / 256u), dim3(256), 0, c->stream, c->ntiles,
+                           c->tile_flag, c->tile_rank, c->tile_list);
+        SLOSH_HIP(hipMemcpyAsync(c->h_nsolve + 1, c->tile_rank + c->ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    // one readback for the counts of bricks with fluid and of tiles
+    SLOSH_HIP(hipStreamSynchronize(c->stream));
+    c->nsolve = c->h_nsolve[0];
+    if (m >= 5) c->ntiles_f = c->h_nsolve[1];
     const int mode = m == 1 ? 0 : (c->lv[m - 1].pooled ? 1 : 2);
     PLevel P = mode == 1 ? plevel_of(c, m - 1) : PLevel{};
     DenseLevel D = mode == 2 ? dlevel_of(c, m - 1) : DenseLevel{};
