@@ -48,9 +48,12 @@ def build_scene(workload):
     return grid, h, reqs, 1.0 / 120.0, lambda t: (0.0, -9.8, 0.0)
 
 
-def algorithmic_bytes(kernel, np_, nc, nf, nb):
+def algorithmic_bytes(kernel, np_, nc, nf, nb, ns=None):
     """Bytes a kernel must move per launch: every array element it reads or writes, once (neighbour reuse
-    perfect). np_ particles, nc allocated cells, nf fluid cells, nb allocated bricks."""
+    perfect). np_ particles, nc allocated cells, nf fluid cells, nb allocated bricks, ns cells of the bricks that
+    hold fluid (the level-0 solver passes visit only those bricks; defaults to nc)."""
+    if ns is None:
+        ns = nc
     table = {
         # pos+age 16 B read; pos+age, vel, C written (64 B); staged face velocities once per cell
         "k_g2p": 80 * np_ + 12 * nc,
@@ -65,21 +68,32 @@ def algorithmic_bytes(kernel, np_, nc, nf, nb):
         "k_divergence": 24 * nc + 4 * nc + 8 * nc,
         "k_project": 12 * nc + 12 * nc + 4 * nc + 8 * nc + 3 * nc + 15 * nc,
         "k_extrapolate": 12 * nc + 3 * nc + 12 * nc + 3 * nc,
-        # level-0 MGPCG passes: vectors on fluid cells, the one-byte stencil mask on every allocated cell
-        # p (8+8), d, r_in, q, diag read; r_out, z written
-        "k_update_black": 40 * nf + nc,
+        # level-0 MGPCG passes: vectors on fluid cells, the one-byte stencil mask on every cell of a brick that
+        # holds fluid; p (8+8), d, r_in, q, diag read; r_out, z written
+        "k_update_black": 40 * nf + ns,
         # z1, d_in, diag read; d_out, q written
-        "k_direction": 20 * nf + nc,
+        "k_direction": 20 * nf + ns,
         # diag, z, r read; level-1 b and x written (one level-1 cell per 8 cells)
-        "k_restrict0": 12 * nf + 2 * nc,
+        "k_restrict0": 12 * nf + 2 * ns,
         # diag, r, z read, z1 written; level-1 stencil word and x1
-        "k_prolong_black": 16 * nf + nc + 6 * nc // 8,
+        "k_prolong_black": 16 * nf + ns + 6 * ns // 8,
         # diag, r, z1 read; z1 written on red cells
-        "k_red_dot": 14 * nf + nc,
+        "k_red_dot": 14 * nf + ns,
         "k_pcg_init": 8 * nf + 16 * nf + 12 * nc + 4 * nc,
         "k_sor_sweep": 16 * nf + 8 * nf // 2 + nc,
     }
     return table.get(kernel)
+
+
+def fluid_brick_cells(sim):
+    """512 x the number of 8^3 bricks holding at least one fluid cell (read back after the timed region)."""
+    from slosh_amd import abi
+    t = sim.read_grid(abi.FIELD_CELL_FLAGS) & 3
+    nz, ny, nx = t.shape
+    pz, py, px = (-nz) % 8, (-ny) % 8, (-nx) % 8
+    f = np.pad(t == abi.CELL_FLUID, ((0, pz), (0, py), (0, px)))
+    b = f.reshape((nz + pz) // 8, 8, (ny + py) // 8, 8, (nx + px) // 8, 8).any(axis=(1, 3, 5))
+    return int(b.sum()) * 512
 
 
 def measured_traffic(workload, kernel):
@@ -300,7 +314,8 @@ def main():
     nb = res.active_bricks
     nc = nb * 512
     nf = res.fluid_cells
-    abytes = algorithmic_bytes(dominant, res.live_particles, nc, nf, nb)
+    ns = fluid_brick_cells(sim)
+    abytes = algorithmic_bytes(dominant, res.live_particles, nc, nf, nb, ns)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": measured_traffic(args.workload, dominant), "launches": launches,
                 "avg_us": avg_s * 1e6,
@@ -359,7 +374,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.workload], "solver": f"{args.solver} to ||r||/||b|| <= 1e-6",
                    "value_solver": args.solver, "particles": res.live_particles, "grid": list(grid),
-                   "fluid_cells": nf, "allocated_bricks": nb,
+                   "fluid_cells": nf, "allocated_bricks": nb, "fluid_bricks": ns // 512,
                    "mean_solver_iterations": round(sum(iters) / len(iters), 2), "max_rel_residual": max(rels),
                    "converged_steps": conv, "particles_per_s": round(sps * res.live_particles, 1),
                    "mg_preconditioner": {"fluid_first_coarse_types": mg["flags"] == abi.FLAG_MG_FLUID_COARSENING,

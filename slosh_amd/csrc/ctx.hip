@@ -365,6 +365,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
         c->nactive = c->h_stats->nactive;
         c->res.active_bricks = c->nactive;
         c->res.pool_overflow_bricks = c->h_stats->overflow;
+        build_brick_rows(c);
     }
     if (stop > SLOSH_STAGE_POOL) launch_p2g(c, warm);
     if (stop > SLOSH_STAGE_P2G) launch_normalize(c, dt, gv);

@@ -47,6 +47,64 @@ __device__ uint32_t block_scan_excl_256(uint32_t v, uint32_t *sh, uint32_t *tota
     return incl - v;
 }
 
+// Exclusive scan of n <= kScanOneMax values by one workgroup, out[n] = the total. Values are staged in LDS with one
+// pad word after every 32 (value i at i + i/32), so the 32 consecutive values a thread scans lie in distinct banks;
+// each thread scans its run serially, the 1024 run totals are scanned by wavefront shuffles and one LDS stage.
+constexpr uint32_t kScanOneThreads = 1024u;
+constexpr uint32_t kScanOneMax = 33u * kScanOneThreads;
+constexpr uint32_t kScanOneLds = kScanOneMax + kScanOneMax / 32u;
+
+__device__ __forceinline__ uint32_t scan_pad(uint32_t i) { return i + (i >> 5); }
+
+__global__ __launch_bounds__(kScanOneThreads) void k_scan_one(const uint32_t *in, uint32_t n, uint32_t *out)
+{
+    constexpr uint32_t T = kScanOneThreads, U = kScanOneMax / kScanOneThreads, W = 64u;
+    __shared__ uint32_t v[kScanOneLds];
+    __shared__ uint32_t wsum[T / W];
+    const uint32_t t = threadIdx.x;
+    {
+        // every load in flight before the first store
+        uint32_t x[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) x[u] = t + T * u < n ? in[t + T * u] : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (t + T * u < n) v[scan_pad(t + T * u)] = x[u];
+    }
+    __syncthreads();
+    const uint32_t per = (n + T - 1u) / T;
+    const uint32_t beg = min(t * per, n), end = min(beg + per, n);
+    uint32_t sum = 0u;
+    for (uint32_t i = beg; i < end; ++i) sum += v[scan_pad(i)];
+    // inclusive scan over the wavefront, then over the wavefront totals
+    const uint32_t lane = t % W, wave = t / W;
+    uint32_t inc = sum;
+    for (uint32_t off = 1; off < W; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, W);
+        if (lane >= off) inc += y;
+    }
+    if (lane == W - 1u) wsum[wave] = inc;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < T / W ? wsum[lane] : 0u;
+        for (uint32_t off = 1; off < T / W; off <<= 1) {
+            const uint32_t y = __shfl_up(w, off, W);
+            if (lane >= off) w += y;
+        }
+        if (lane < T / W) wsum[lane] = w; // inclusive
+    }
+    __syncthreads();
+    uint32_t run = inc - sum + (wave ? wsum[wave - 1u] : 0u);
+    for (uint32_t i = beg; i < end; ++i) {
+        const uint32_t x = v[scan_pad(i)];
+        v[scan_pad(i)] = run;
+        run += x;
+    }
+    if (t == 0) out[n] = wsum[T / W - 1u];
+    __syncthreads();
+    for (uint32_t i = t; i < n; i += T) out[i] = v[scan_pad(i)];
+}
+
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *block_sums)
 {
     __shared__ uint32_t sh[256];
@@ -165,35 +223,80 @@ __global__ void k_pool_active(Geo g, const uint32_t *alloc_flag, const uint32_t 
     if (alloc_flag[b]) active[alloc_rank[b]] = b;
 }
 
+// Per allocated brick (active order), once per step after the pool: the slots of its 3x3x3 brick neighbourhood
+// (word (dx+1) + 3(dy+1) + 9(dz+1); 0 outside the grid or unallocated, as the brick table gives) and, in word 27,
+// its brick index. One thread per word.
+__global__ void k_brick_rows(Geo g, uint32_t nactive, uint32_t *rows)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t e = t / kBinfoWords, k = t % kBinfoWords;
+    if (e >= nactive) return;
+    const uint32_t b = g.active[e];
+    uint32_t v = 0u;
+    if (k == 27u) {
+        v = b;
+    } else if (k < 27u) {
+        int bc[3];
+        brick_coord(g, b, bc);
+        const int x = bc[0] + (int)(k % 3u) - 1, y = bc[1] + (int)(k / 3u % 3u) - 1, z = bc[2] + (int)(k / 9u) - 1;
+        if (x >= 0 && y >= 0 && z >= 0 && x < g.bd[0] && y < g.bd[1] && z < g.bd[2])
+            v = g.table[(uint32_t)(x + g.bd[0] * (y + g.bd[1] * z))];
+    }
+    rows[t] = v;
+}
+
 // ---- per-brick helpers -------------------------------------------------------------------------------------
+// One workgroup per allocated brick, one thread per cell. The brick's row of neighbour slots is staged in LDS, so
+// a neighbour's pool index costs one LDS read instead of a brick-table load that the data load has to wait for.
 struct CellPos {
     int x, y, z;
     uint32_t idx;
     uint32_t slot;
     int bc[3];
+    const uint32_t *row; // LDS copy of the brick's neighbourhood row
 };
 
-__device__ __forceinline__ bool cell_of_block(const Geo &g, CellPos &c)
+// Workgroup -> active brick e (bricks of one XCD contiguous); stages the row (ends with a barrier).
+__device__ __forceinline__ bool cell_of_block(const Geo &g, const uint32_t *rows, uint32_t *s_row, CellPos &c,
+                                              uint32_t &e)
 {
-    const uint32_t b = g.active[blockIdx.x];
+    e = xcd_remap(blockIdx.x, gridDim.x);
+    if (threadIdx.x < 28u) s_row[threadIdx.x] = rows[(size_t)e * kBinfoWords + threadIdx.x];
+    __syncthreads();
+    const uint32_t b = s_row[27];
     const uint32_t l = threadIdx.x;
     brick_coord(g, b, c.bc);
     c.x = c.bc[0] * 8 + (int)(l & 7u);
     c.y = c.bc[1] * 8 + (int)((l >> 3) & 7u);
     c.z = c.bc[2] * 8 + (int)(l >> 6);
-    c.slot = g.table[b];
+    c.slot = s_row[13];
     c.idx = c.slot * kBrickCells + l;
+    c.row = s_row;
     return in_grid(g, c.x, c.y, c.z);
 }
 
-__device__ __forceinline__ uint32_t type_at(const Geo &g, const uint32_t *ctype, int x, int y, int z)
+// Slot of the brick holding cell (x, y, z), which lies in the 3x3x3 neighbourhood of c's brick.
+__device__ __forceinline__ uint32_t nb_slot(const CellPos &c, int x, int y, int z)
 {
-    return in_grid(g, x, y, z) ? ctype[cell_index(g, x, y, z)] : kCellSolid;
+    const int k = ((x >> 3) - c.bc[0] + 1) + 3 * ((y >> 3) - c.bc[1] + 1) + 9 * ((z >> 3) - c.bc[2] + 1);
+    return c.row[k];
 }
 
-__device__ __forceinline__ uint32_t face_index(const Geo &g, int axis, int x, int y, int z)
+// CellIndex (Slosh.usf:136-144) for a cell of the neighbourhood: the same slot the brick table gives.
+__device__ __forceinline__ uint32_t cell_index(const Geo &g, const CellPos &c, int x, int y, int z)
 {
-    return (uint32_t)axis * g.pool_cells + cell_index(g, x, y, z);
+    const uint32_t slot = in_grid(g, x, y, z) ? nb_slot(c, x, y, z) : 0u;
+    return slot * kBrickCells + local_of_cell(x, y, z);
+}
+
+__device__ __forceinline__ uint32_t type_at(const Geo &g, const CellPos &c, const uint32_t *ctype, int x, int y, int z)
+{
+    return in_grid(g, x, y, z) ? ctype[cell_index(g, c, x, y, z)] : kCellSolid;
+}
+
+__device__ __forceinline__ uint32_t face_index(const Geo &g, const CellPos &c, int axis, int x, int y, int z)
+{
+    return (uint32_t)axis * g.pool_cells + cell_index(g, c, x, y, z);
 }
 
 // Slosh.usf:232-241
@@ -207,12 +310,15 @@ __device__ __forceinline__ float solid_face_velocity(const slosh_collider *col, 
 
 // Slosh.usf:604-651 (NormalizeFace, APIC) over the P2G tiles: each face sums the tiles of the (up to 8) bricks
 // whose node range covers it, then ratio, weight, gravity and solid boundary velocities as the reference.
-__global__ __launch_bounds__(512) void k_normalize(Geo g, const int32_t *tiles, const uint8_t *tile_valid,
-                                                   const uint32_t *ctype, float *vel, float *fweight, uint8_t *fflags,
-                                                   float gdt0, float gdt1, float gdt2, const slosh_collider *col)
+__global__ __launch_bounds__(512) void k_normalize(Geo g, const uint32_t *rows, const int32_t *tiles,
+                                                   const uint8_t *tile_valid, const uint32_t *ctype, float *vel,
+                                                   float *fweight, uint8_t *fflags, float gdt0, float gdt1, float gdt2,
+                                                   const slosh_collider *col)
 {
+    __shared__ uint32_t s_row[28];
     CellPos c;
-    if (!cell_of_block(g, c)) return;
+    uint32_t e;
+    if (!cell_of_block(g, rows, s_row, c, e)) return;
     const int lc[3] = {c.x & 7, c.y & 7, c.z & 7};
     int lo[3], hi[3];
     for (int a = 0; a < 3; ++a) {
@@ -224,9 +330,7 @@ __global__ __launch_bounds__(512) void k_normalize(Geo g, const int32_t *tiles, 
         for (int dy = lo[1]; dy <= hi[1]; ++dy)
             for (int dx = lo[0]; dx <= hi[0]; ++dx) {
                 int nb[3] = {c.bc[0] + dx, c.bc[1] + dy, c.bc[2] + dz};
-                if (nb[0] < 0 || nb[1] < 0 || nb[2] < 0 || nb[0] >= g.bd[0] || nb[1] >= g.bd[1] || nb[2] >= g.bd[2])
-                    continue;
-                uint32_t ns = g.table[(uint32_t)(nb[0] + g.bd[0] * (nb[1] + g.bd[1] * nb[2]))];
+                const uint32_t ns = c.row[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)]; // 0 outside the grid
                 if (ns == 0u || !tile_valid[ns]) continue;
                 int tx = c.x - (nb[0] * 8 - 1), ty = c.y - (nb[1] * 8 - 1), tz = c.z - (nb[2] * 8 - 1);
                 const int32_t *t = tiles + (size_t)ns * kTileInts + (tx + kTileDim * (ty + kTileDim * tz));
@@ -244,7 +348,7 @@ __global__ __launch_bounds__(512) void k_normalize(Geo g, const int32_t *tiles, 
         uint32_t flags = wt[a] > 0 ? kFaceHasWeight : 0u;
         int ca[3] = {c.x, c.y, c.z};
         ca[a] -= 1;
-        uint32_t ta = type_at(g, ctype, ca[0], ca[1], ca[2]);
+        uint32_t ta = type_at(g, c, ctype, ca[0], ca[1], ca[2]);
         uint32_t sa = ta & kCellTypeMask, sb = tb & kCellTypeMask;
         float fp[3] = {(float)c.x + 0.5f, (float)c.y + 0.5f, (float)c.z + 0.5f};
         fp[a] = (float)(a == 0 ? c.x : (a == 1 ? c.y : c.z));
@@ -266,33 +370,35 @@ __global__ __launch_bounds__(512) void k_normalize(Geo g, const int32_t *tiles, 
 }
 
 // Slosh.usf:211-224
-__device__ __forceinline__ float cell_density(const Geo &g, const float *fw, int x, int y, int z)
+__device__ __forceinline__ float cell_density(const Geo &g, const CellPos &c, const float *fw, int x, int y, int z)
 {
     if (!in_grid(g, x, y, z)) return 0.0f;
     float ws = 0.0f;
     for (int a = 0; a < 3; ++a) {
         int p[3] = {x, y, z};
         p[a] += 1;
-        ws += fw[face_index(g, a, x, y, z)] + fw[face_index(g, a, p[0], p[1], p[2])];
+        ws += fw[face_index(g, c, a, x, y, z)] + fw[face_index(g, c, a, p[0], p[1], p[2])];
     }
     return ws / (6.0f * kTargetPpc);
 }
 
-__device__ __forceinline__ float air_face_weight(const Geo &g, const float *fw, int ghost, float rho_fluid, int x,
-                                                 int y, int z)
+__device__ __forceinline__ float air_face_weight(const Geo &g, const CellPos &c, const float *fw, int ghost,
+                                                 float rho_fluid, int x, int y, int z)
 {
-    return ghost ? 1.0f / ghost_theta(rho_fluid, cell_density(g, fw, x, y, z)) : 1.0f;
+    return ghost ? 1.0f / ghost_theta(rho_fluid, cell_density(g, c, fw, x, y, z)) : 1.0f;
 }
 
 // Slosh.usf:692-741 (DivergenceCS)
 // Also writes the solver's per-cell stencil mask: bit n (0..5, order -x +x -y +y -z +z) = neighbour n is fluid,
 // bit 6 = the cell itself is fluid (0 for every other cell).
-__global__ __launch_bounds__(512) void k_divergence(Geo g, const float *vel, const float *fw, const uint32_t *ctype,
-                                                    float *div, float *diag, uint8_t *fmask, int ghost, float dc,
-                                                    float deadband, float h, float dt)
+__global__ __launch_bounds__(512) void k_divergence(Geo g, const uint32_t *rows, const float *vel, const float *fw,
+                                                    const uint32_t *ctype, float *div, float *diag, uint8_t *fmask,
+                                                    int ghost, float dc, float deadband, float h, float dt)
 {
+    __shared__ uint32_t s_row[28];
     CellPos c;
-    if (!cell_of_block(g, c) || (ctype[c.idx] & kCellTypeMask) != kCellFluid) {
+    uint32_t e;
+    if (!cell_of_block(g, rows, s_row, c, e) || (ctype[c.idx] & kCellTypeMask) != kCellFluid) {
         // cells of a partial brick beyond the grid get a zero mask too (their slot may be recycled)
         div[c.idx] = 0.0f;
         diag[c.idx] = 0.0f;
@@ -303,13 +409,13 @@ __global__ __launch_bounds__(512) void k_divergence(Geo g, const float *vel, con
     for (int a = 0; a < 3; ++a) {
         int p[3] = {c.x, c.y, c.z};
         p[a] += 1;
-        dv += vel[face_index(g, a, p[0], p[1], p[2])] - vel[(uint32_t)a * g.pool_cells + c.idx];
+        dv += vel[face_index(g, c, a, p[0], p[1], p[2])] - vel[(uint32_t)a * g.pool_cells + c.idx];
     }
     float ws = 0.0f;
     for (int a = 0; a < 3; ++a) {
         int p[3] = {c.x, c.y, c.z};
         p[a] += 1;
-        ws += fw[(uint32_t)a * g.pool_cells + c.idx] + fw[face_index(g, a, p[0], p[1], p[2])];
+        ws += fw[(uint32_t)a * g.pool_cells + c.idx] + fw[face_index(g, c, a, p[0], p[1], p[2])];
     }
     float rho = ws / (6.0f * kTargetPpc);
     float excess = fmaxf(rho - (1.0f + deadband), 0.0f);
@@ -321,26 +427,28 @@ __global__ __launch_bounds__(512) void k_divergence(Geo g, const float *vel, con
         int d[3];
         nb_offset(nb, d);
         int nx = c.x + d[0], ny = c.y + d[1], nz = c.z + d[2];
-        uint32_t t = type_at(g, ctype, nx, ny, nz) & kCellTypeMask;
+        uint32_t t = type_at(g, c, ctype, nx, ny, nz) & kCellTypeMask;
         if (t == kCellFluid) mask |= 1u << nb;
-        dg += t == kCellFluid ? 1.0f : (t == kCellAir ? air_face_weight(g, fw, ghost, rho, nx, ny, nz) : 0.0f);
+        dg += t == kCellFluid ? 1.0f : (t == kCellAir ? air_face_weight(g, c, fw, ghost, rho, nx, ny, nz) : 0.0f);
     }
     diag[c.idx] = dg;
     fmask[c.idx] = (uint8_t)mask;
 }
 
 // Slosh.usf:995-1052 (ProjectFace) on the FP64 pressure, without collider-coupling accumulation.
-__global__ __launch_bounds__(512) void k_project(Geo g, float *vel, const float *fw, const uint32_t *ctype,
-                                                 const double *pres, uint8_t *fflags, int ghost)
+__global__ __launch_bounds__(512) void k_project(Geo g, const uint32_t *rows, float *vel, const float *fw,
+                                                 const uint32_t *ctype, const double *pres, uint8_t *fflags, int ghost)
 {
+    __shared__ uint32_t s_row[28];
     CellPos c;
-    if (!cell_of_block(g, c)) return;
+    uint32_t e;
+    if (!cell_of_block(g, rows, s_row, c, e)) return;
     const uint32_t tb = ctype[c.idx] & kCellTypeMask;
     for (int a = 0; a < 3; ++a) {
         uint32_t f = (uint32_t)a * g.pool_cells + c.idx;
         int ca[3] = {c.x, c.y, c.z};
         ca[a] -= 1;
-        uint32_t ta = type_at(g, ctype, ca[0], ca[1], ca[2]) & kCellTypeMask;
+        uint32_t ta = type_at(g, c, ctype, ca[0], ca[1], ca[2]) & kCellTypeMask;
         uint32_t flags = (uint32_t)fflags[f] & kFaceHasWeight;
         if (ta == kCellSolid && tb == kCellSolid) {
         } else if (ta == kCellSolid || tb == kCellSolid) {
@@ -348,14 +456,16 @@ __global__ __launch_bounds__(512) void k_project(Geo g, float *vel, const float 
         } else if (ta == kCellFluid || tb == kCellFluid) {
             double pa, pb;
             if (ta == kCellFluid && tb == kCellFluid) {
-                pa = pres[cell_index(g, ca[0], ca[1], ca[2])];
+                pa = pres[cell_index(g, c, ca[0], ca[1], ca[2])];
                 pb = pres[c.idx];
             } else if (ta == kCellFluid) {
-                pa = pres[cell_index(g, ca[0], ca[1], ca[2])];
-                pb = pa - pa * (double)air_face_weight(g, fw, ghost, cell_density(g, fw, ca[0], ca[1], ca[2]), c.x, c.y, c.z);
+                pa = pres[cell_index(g, c, ca[0], ca[1], ca[2])];
+                pb = pa - pa * (double)air_face_weight(g, c, fw, ghost, cell_density(g, c, fw, ca[0], ca[1], ca[2]),
+                                                       c.x, c.y, c.z);
             } else {
                 pb = pres[c.idx];
-                pa = pb - pb * (double)air_face_weight(g, fw, ghost, cell_density(g, fw, c.x, c.y, c.z), ca[0], ca[1], ca[2]);
+                pa = pb - pb * (double)air_face_weight(g, c, fw, ghost, cell_density(g, c, fw, c.x, c.y, c.z), ca[0],
+                                                       ca[1], ca[2]);
             }
             vel[f] -= (float)(pb - pa);
             flags |= kFaceValid;
@@ -366,24 +476,25 @@ __global__ __launch_bounds__(512) void k_project(Geo g, float *vel, const float 
 
 // Slosh.usf:1222-1240 (ResidualCS): max and sum of |div|/h over fluid cells after projection, one partial per
 // brick (reduced with the other step statistics by k_step_stats).
-__global__ __launch_bounds__(512) void k_div_stats(Geo g, const float *vel, const uint32_t *ctype, uint32_t *bp_max,
-                                                   uint32_t *bp_cnt, double *bp_sum)
+__global__ __launch_bounds__(512) void k_div_stats(Geo g, const uint32_t *rows, const float *vel, const uint32_t *ctype,
+                                                   uint32_t *bp_max, uint32_t *bp_cnt, double *bp_sum)
 {
     __shared__ double ssum[512];
     __shared__ uint32_t smax, scount;
+    __shared__ uint32_t s_row[28];
     if (threadIdx.x == 0) {
         smax = 0u;
         scount = 0u;
     }
-    __syncthreads();
     CellPos c;
+    uint32_t e;
     double rate_d = 0.0;
-    if (cell_of_block(g, c) && (ctype[c.idx] & kCellTypeMask) == kCellFluid) {
+    if (cell_of_block(g, rows, s_row, c, e) && (ctype[c.idx] & kCellTypeMask) == kCellFluid) {
         float dv = 0.0f;
         for (int a = 0; a < 3; ++a) {
             int p[3] = {c.x, c.y, c.z};
             p[a] += 1;
-            dv += vel[face_index(g, a, p[0], p[1], p[2])] - vel[(uint32_t)a * g.pool_cells + c.idx];
+            dv += vel[face_index(g, c, a, p[0], p[1], p[2])] - vel[(uint32_t)a * g.pool_cells + c.idx];
         }
         float rate = fabsf(dv) * g.inv_h;
         atomicMax(&smax, __float_as_uint(rate));
@@ -397,9 +508,9 @@ __global__ __launch_bounds__(512) void k_div_stats(Geo g, const float *vel, cons
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        bp_sum[blockIdx.x] = ssum[0];
-        bp_max[blockIdx.x] = smax;
-        bp_cnt[blockIdx.x] = scount;
+        bp_sum[e] = ssum[0];
+        bp_max[e] = smax;
+        bp_cnt[e] = scount;
     }
 }
 
@@ -482,10 +593,13 @@ __global__ __launch_bounds__(kStatsThreads) void k_step_stats(const uint32_t *bp
 }
 
 // Slosh.usf:1263-1310 (ExtrapolateFace) for layer `layer`: src -> dst, marking filled faces with bit 4+layer.
-__global__ __launch_bounds__(512) void k_extrapolate(Geo g, const float *src, float *dst, uint8_t *fflags, uint32_t layer)
+__global__ __launch_bounds__(512) void k_extrapolate(Geo g, const uint32_t *rows, const float *src, float *dst,
+                                                     uint8_t *fflags, uint32_t layer)
 {
+    __shared__ uint32_t s_row[28];
     CellPos c;
-    if (!cell_of_block(g, c)) return;
+    uint32_t e;
+    if (!cell_of_block(g, rows, s_row, c, e)) return;
     uint32_t known = kFaceValid;
     for (uint32_t k = 0; k < layer; ++k) known |= 1u << (kFaceExtrapShift + k);
     for (int a = 0; a < 3; ++a) {
@@ -502,7 +616,7 @@ __global__ __launch_bounds__(512) void k_extrapolate(Geo g, const float *src, fl
             nb_offset(nb, d);
             int nx = c.x + d[0], ny = c.y + d[1], nz = c.z + d[2];
             if (!in_grid(g, nx, ny, nz)) continue;
-            uint32_t ni = face_index(g, a, nx, ny, nz);
+            uint32_t ni = face_index(g, c, a, nx, ny, nz);
             uint32_t nf = fflags[ni];
             if ((nf & known) != 0u && (nf & kFaceSolid) == 0u) {
                 sum += src[ni];
@@ -542,18 +656,20 @@ void launch_fill_f64(hipStream_t st, double *p, double v, size_t n)
 // out[0..n] = exclusive prefix sums of in[0..n), out[n] = total. Supports n <= 256 * 1024 * 1024.
 void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n)
 {
-    const uint32_t nb = n ? blocks_for(n, kScanBlock) : 1u;
+    if (n <= kScanOneMax) {
+        launch_timed(c, K_SCAN, k_scan_one, dim3(1), dim3(kScanOneThreads), 0, in, n, out);
+        SLOSH_HIP(hipGetLastError());
+        return;
+    }
+    const uint32_t nb = blocks_for(n, kScanBlock);
     {
-        SLOSH_TIMED(c, K_SCAN);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, in, n, c->scan_blocks);
+        launch_timed(c, K_SCAN, k_scan_reduce, dim3(nb), dim3(256), 0, in, n, c->scan_blocks);
     }
     {
-        SLOSH_TIMED(c, K_SCAN);
-        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(256), 0, c->stream, c->scan_blocks, nb);
+        launch_timed(c, K_SCAN, k_scan_blocks, dim3(1), dim3(256), 0, c->scan_blocks, nb);
     }
     {
-        SLOSH_TIMED(c, K_SCAN);
-        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, c->stream, in, n, c->scan_blocks, out);
+        launch_timed(c, K_SCAN, k_scan_down, dim3(nb), dim3(256), 0, in, n, c->scan_blocks, out);
     }
     SLOSH_HIP(hipGetLastError());
 }
@@ -563,37 +679,41 @@ void run_pool(slosh_ctx *c)
     const Geo &g = c->geo;
     uint32_t *table = c->table;
     {
-        SLOSH_TIMED(c, K_POOL);
-        hipLaunchKernelGGL(k_pool_want, dim3(blocks_for(g.nbricks, 256)), dim3(256), 0, c->stream, g, c->touch, table,
-                           c->slot_brick, c->need);
+        launch_timed(c, K_POOL, k_pool_want, dim3(blocks_for(g.nbricks, 256)), dim3(256), 0, g, c->touch, table,
+                     c->slot_brick, c->need);
     }
     {
-        SLOSH_TIMED(c, K_POOL);
-        hipLaunchKernelGGL(k_pool_freeflags, dim3(blocks_for(g.npool, 256)), dim3(256), 0, c->stream, c->slot_brick, g.npool,
-                           c->freeflag);
+        launch_timed(c, K_POOL, k_pool_freeflags, dim3(blocks_for(g.npool, 256)), dim3(256), 0, c->slot_brick, g.npool,
+                     c->freeflag);
     }
     SLOSH_HIP(hipGetLastError());
     scan_exclusive(c, c->need, c->need_rank, g.nbricks);
     scan_exclusive(c, c->freeflag, c->free_rank, g.npool);
     {
-        SLOSH_TIMED(c, K_POOL);
-        hipLaunchKernelGGL(k_pool_free_list, dim3(blocks_for(g.npool, 256)), dim3(256), 0, c->stream, c->freeflag,
-                           c->free_rank, g.npool, c->free_list);
+        launch_timed(c, K_POOL, k_pool_free_list, dim3(blocks_for(g.npool, 256)), dim3(256), 0, c->freeflag,
+                     c->free_rank, g.npool, c->free_list);
     }
     {
-        SLOSH_TIMED(c, K_POOL);
-        hipLaunchKernelGGL(k_pool_assign, dim3(blocks_for(g.nbricks, 256)), dim3(256), 0, c->stream, g, c->need,
-                           c->need_rank, c->free_list, c->free_rank, g.npool, table, c->slot_brick,
-                           c->new_slot, c->touch, c->alloc_flag,
-                           c->stats);
+        launch_timed(c, K_POOL, k_pool_assign, dim3(blocks_for(g.nbricks, 256)), dim3(256), 0, g, c->need,
+                     c->need_rank, c->free_list, c->free_rank, g.npool, table, c->slot_brick,
+                     c->new_slot, c->touch, c->alloc_flag,
+                     c->stats);
     }
     SLOSH_HIP(hipGetLastError());
     scan_exclusive(c, c->alloc_flag, c->alloc_rank, g.nbricks);
     {
-        SLOSH_TIMED(c, K_POOL);
-        hipLaunchKernelGGL(k_pool_active, dim3(blocks_for(g.nbricks, 256)), dim3(256), 0, c->stream, g, c->alloc_flag,
-                           c->alloc_rank, c->active, c->stats);
+        launch_timed(c, K_POOL, k_pool_active, dim3(blocks_for(g.nbricks, 256)), dim3(256), 0, g, c->alloc_flag,
+                     c->alloc_rank, c->active, c->stats);
     }
+    SLOSH_HIP(hipGetLastError());
+}
+
+// The neighbourhood rows of the allocated bricks (once per step, after the pool; every per-brick pass uses them).
+void build_brick_rows(slosh_ctx *c)
+{
+    if (!c->nactive) return;
+    launch_timed(c, K_POOL, k_brick_rows, dim3(blocks_for((size_t)c->nactive * kBinfoWords, 256)), dim3(256), 0,
+                 c->geo, c->nactive, c->binfo);
     SLOSH_HIP(hipGetLastError());
 }
 
@@ -602,9 +722,9 @@ void launch_normalize(slosh_ctx *c, float dt, const float g[3])
     if (!c->nactive) return;
     c->vcur = 0;
     {
-        SLOSH_TIMED(c, K_NORMALIZE);
-        hipLaunchKernelGGL(k_normalize, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->tiles, c->tile_valid,
-                           c->ctype, c->vel[0], c->fweight, c->fflags, g[0] * dt, g[1] * dt, g[2] * dt, c->d_col);
+        launch_timed(c, K_NORMALIZE, k_normalize, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo, c->tiles,
+                     c->tile_valid, c->ctype, c->vel[0], c->fweight, c->fflags, g[0] * dt, g[1] * dt, g[2] * dt,
+                     c->d_col);
     }
     SLOSH_HIP(hipGetLastError());
 }
@@ -613,10 +733,9 @@ void launch_divergence(slosh_ctx *c, float dt)
 {
     if (!c->nactive) return;
     {
-        SLOSH_TIMED(c, K_DIVERGENCE);
-        hipLaunchKernelGGL(k_divergence, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->fweight,
-                           c->ctype, c->div, c->diag, c->fmask, c->s.ghost_fluid, c->s.density_correction,
-                           c->s.density_deadband, c->geo.h, dt);
+        launch_timed(c, K_DIVERGENCE, k_divergence, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo,
+                     c->vel[c->vcur], c->fweight, c->ctype, c->div, c->diag, c->fmask, c->s.ghost_fluid,
+                     c->s.density_correction, c->s.density_deadband, c->geo.h, dt);
     }
     SLOSH_HIP(hipGetLastError());
 }
@@ -625,14 +744,12 @@ void launch_project(slosh_ctx *c)
 {
     if (!c->nactive) return;
     {
-        SLOSH_TIMED(c, K_PROJECT);
-        hipLaunchKernelGGL(k_project, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->fweight,
-                           c->ctype, c->pres, c->fflags, c->s.ghost_fluid);
+        launch_timed(c, K_PROJECT, k_project, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo, c->vel[c->vcur],
+                     c->fweight, c->ctype, c->pres, c->fflags, c->s.ghost_fluid);
     }
     {
-        SLOSH_TIMED(c, K_DIV_STATS);
-        hipLaunchKernelGGL(k_div_stats, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->ctype,
-                           c->bp_div_max, c->bp_div_cnt, c->bp_div_sum);
+        launch_timed(c, K_DIV_STATS, k_div_stats, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo, c->vel[c->vcur],
+                     c->ctype, c->bp_div_max, c->bp_div_cnt, c->bp_div_sum);
     }
     c->n_div_blocks = c->nactive;
     SLOSH_HIP(hipGetLastError());
@@ -652,9 +769,8 @@ void launch_extrapolate(slosh_ctx *c, uint32_t layer)
     if (!c->nactive) return;
     int dst = (c->vcur + 1) % 3;
     {
-        SLOSH_TIMED(c, K_EXTRAPOLATE);
-        hipLaunchKernelGGL(k_extrapolate, dim3(c->nactive), dim3(512), 0, c->stream, c->geo, c->vel[c->vcur], c->vel[dst],
-                           c->fflags, layer);
+        launch_timed(c, K_EXTRAPOLATE, k_extrapolate, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo,
+                     c->vel[c->vcur], c->vel[dst], c->fflags, layer);
     }
     SLOSH_HIP(hipGetLastError());
     c->vcur = dst;

@@ -113,31 +113,14 @@ __global__ void k_fluid_flags(uint32_t nactive, const uint32_t *bp_fluid, uint32
     if (e < nactive) flag[e] = bp_fluid[e] != 0u ? 1u : 0u;
 }
 
-// Slots of the 3x3x3 brick neighbourhood (0 outside the grid or unallocated) and the brick index of every
-// allocated brick, once per solve; the rows of bricks with fluid also go, in active order, to binfo_f.
-__global__ void k_brick_info(Geo g, uint32_t nactive, uint32_t *binfo, const uint32_t *flag, const uint32_t *rank,
+// The neighbourhood rows (built once per step after the pool) of the bricks with fluid, in active order, to binfo_f.
+__global__ void k_fluid_rows(uint32_t nactive, const uint32_t *binfo, const uint32_t *flag, const uint32_t *rank,
                              uint32_t *binfo_f)
 {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nactive) return;
-    const uint32_t b = g.active[e];
-    int bc[3];
-    brick_coord(g, b, bc);
-    uint32_t row[28];
-    for (int k = 0; k < 27; ++k) {
-        const int x = bc[0] + k % 3 - 1, y = bc[1] + (k / 3) % 3 - 1, z = bc[2] + k / 9 - 1;
-        uint32_t s = 0u;
-        if (x >= 0 && y >= 0 && z >= 0 && x < g.bd[0] && y < g.bd[1] && z < g.bd[2])
-            s = g.table[(uint32_t)(x + g.bd[0] * (y + g.bd[1] * z))];
-        row[k] = s;
-    }
-    row[27] = b;
-    uint32_t *p = binfo + (size_t)e * kBinfoWords;
-    for (int k = 0; k < 28; ++k) p[k] = row[k];
-    if (flag[e]) {
-        uint32_t *q = binfo_f + (size_t)rank[e] * kBinfoWords;
-        for (int k = 0; k < 28; ++k) q[k] = row[k];
-    }
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t e = t / kBinfoWords, k = t % kBinfoWords;
+    if (e >= nactive || !flag[e]) return;
+    binfo_f[(size_t)rank[e] * kBinfoWords + k] = binfo[t];
 }
 
 // r = b - A p (FP64 p); partials ||b||^2, ||r||^2 and the fluid cell count.
@@ -512,10 +495,7 @@ __global__ __launch_bounds__(kThreads) void k_fcg_update(const uint32_t *binfo, 
 } // namespace
 
 #define L0_LAUNCH(kid, kernel, ...)                                                                              \
-    do {                                                                                                         \
-        SLOSH_TIMED(c, kid);                                                                                     \
-        hipLaunchKernelGGL(kernel, dim3(c->nsolve), dim3(kThreads), 0, c->stream, c->binfo_f, c->fmask, __VA_ARGS__); \
-    } while (0)
+    launch_timed(c, slosh::kid, kernel, dim3(c->nsolve), dim3(kThreads), 0u, c->binfo_f, c->fmask, __VA_ARGS__)
 
 void l0_brick_info(slosh_ctx *c, bool readback)
 {
@@ -523,8 +503,8 @@ void l0_brick_info(slosh_ctx *c, bool readback)
     const uint32_t nb = (c->nactive + 255u) / 256u;
     hipLaunchKernelGGL(k_fluid_flags, dim3(nb), dim3(256), 0, c->stream, c->nactive, c->bp_fluid, c->fluid_flag);
     scan_exclusive(c, c->fluid_flag, c->fluid_rank, c->nactive);
-    hipLaunchKernelGGL(k_brick_info, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nactive, c->binfo, c->fluid_flag,
-                       c->fluid_rank, c->binfo_f);
+    hipLaunchKernelGGL(k_fluid_rows, dim3(nb * kBinfoWords), dim3(256), 0, c->stream, c->nactive, c->binfo,
+                       c->fluid_flag, c->fluid_rank, c->binfo_f);
     // the level-0 and pooled passes launch one workgroup per brick with fluid: their count comes back here (or with
     // the caller's readback)
     SLOSH_HIP(hipMemcpyAsync(c->h_nsolve, c->fluid_rank + c->nactive, sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -538,9 +518,8 @@ void l0_brick_info(slosh_ctx *c, bool readback)
 // Over every allocated brick (non-fluid cells get r = 0, also in bricks without fluid).
 void l0_pcg_init(slosh_ctx *c, float *r_out)
 {
-    SLOSH_TIMED(c, K_PCG_INIT);
-    hipLaunchKernelGGL(k_pcg_init, dim3(c->nactive), dim3(kThreads), 0, c->stream, c->binfo, c->fmask, c->div, c->diag,
-                       c->pres, r_out, c->partials);
+    launch_timed(c, K_PCG_INIT, k_pcg_init, dim3(c->nactive), dim3(kThreads), 0, c->binfo, c->fmask, c->div, c->diag,
+                 c->pres, r_out, c->partials);
 }
 
 void l0_update_black(slosh_ctx *c, const float *r_in, float *r_out, const float *d, int first)

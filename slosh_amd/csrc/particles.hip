@@ -314,12 +314,25 @@ __global__ __launch_bounds__(256) void k_p2g(Geo g, Parts p, const uint32_t *key
     if (threadIdx.x == 0) fluid_sum = 0u;
     __syncthreads();
     const uint32_t i0 = key_off[b], i1 = key_off[b + 1];
+    // each thread's next record is loaded while the current one is splatted
+    auto load = [&](uint32_t i, float4 &pa, float v[3], float c[9]) {
+        pa = p.pa[i];
+        for (int a = 0; a < 3; ++a) v[a] = p.v[a][i];
+        for (int k = 0; k < 9; ++k) c[k] = p.c[k][i];
+    };
+    float4 pa_n;
+    float v_n[3], c_n[9];
+    if (i0 + threadIdx.x < i1) load(i0 + threadIdx.x, pa_n, v_n, c_n);
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        float4 pa = p.pa[i];
+        const float4 pa = pa_n;
+        float vp[3], cp[9];
+        for (int a = 0; a < 3; ++a) vp[a] = v_n[a];
+        for (int k = 0; k < 9; ++k) cp[k] = c_n[k];
+        if (i + blockDim.x < i1) load(i + blockDim.x, pa_n, v_n, c_n);
         float xp[3] = {pa.x, pa.y, pa.z};
         for (int axis = 0; axis < 3; ++axis) {
-            float up = p.v[axis][i];
-            float cp0 = p.c[3 * axis][i], cp1 = p.c[3 * axis + 1][i], cp2 = p.c[3 * axis + 2][i];
+            float up = vp[axis];
+            float cp0 = cp[3 * axis], cp1 = cp[3 * axis + 1], cp2 = cp[3 * axis + 2];
             float off[3] = {0.5f, 0.5f, 0.5f};
             off[axis] = 0.0f;
             float gg[3], fr[3];
@@ -579,7 +592,7 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key
                                              uint32_t ncol, uint8_t *touch, uint32_t *bp_speed, uint32_t *bp_clamp)
 {
     __shared__ float s_box[3 * kG2PBoxNodes];
-    __shared__ uint32_t tb[256];
+    __shared__ uint32_t s_nb[27]; // slots of the 3x3x3 brick neighbourhood, (dx+1) + 3(dy+1) + 9(dz+1)
     __shared__ uint32_t smax, sclamp;
     const uint32_t key = blockIdx.x;
     const uint32_t beg = key_off[key], end = key_off[key + 1];
@@ -596,18 +609,46 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key
     }
     int bc[3];
     brick_coord(g, key, bc);
-    const bool staged = g.table[key] != 0u;
+    if (threadIdx.x < 27u) {
+        const int x = bc[0] + (int)(threadIdx.x % 3u) - 1, y = bc[1] + (int)(threadIdx.x / 3u % 3u) - 1,
+                  z = bc[2] + (int)(threadIdx.x / 9u) - 1;
+        s_nb[threadIdx.x] = x >= 0 && y >= 0 && z >= 0 && x < g.bd[0] && y < g.bd[1] && z < g.bd[2]
+                                ? g.table[(uint32_t)(x + g.bd[0] * (y + g.bd[1] * z))]
+                                : 0u;
+    }
+    // the first position records, loaded while the box is staged
+    const uint32_t i0 = beg + threadIdx.x;
+    float4 pa_next = i0 < end ? p.pa[i0] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    __syncthreads();
+    const bool staged = s_nb[13] != 0u;
     FaceSource src;
     src.vel = vel;
     src.box = s_box;
     src.staged = staged;
     for (int a = 0; a < 3; ++a) src.org[a] = bc[a] * 8 - 2;
     if (staged) {
-        for (uint32_t k = threadIdx.x; k < (uint32_t)(3 * kG2PBoxNodes); k += 256u) {
-            const int axis = (int)(k / kG2PBoxNodes), q = (int)(k % kG2PBoxNodes);
-            const int x = src.org[0] + q % kG2PBox, y = src.org[1] + (q / kG2PBox) % kG2PBox,
-                      z = src.org[2] + q / (kG2PBox * kG2PBox);
-            s_box[k] = in_grid(g, x, y, z) ? vel[face_index(g, axis, x, y, z)] : 0.0f;
+        // the box's nodes lie in the 3x3x3 neighbourhood; U loads in flight per thread before their stores
+        constexpr uint32_t U = 7u, N = 3u * (uint32_t)kG2PBoxNodes;
+        for (uint32_t k0 = threadIdx.x; k0 < N; k0 += 256u * U) {
+            float v[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t k = k0 + 256u * u;
+                v[u] = 0.0f;
+                if (k < N) {
+                    const int axis = (int)(k / kG2PBoxNodes), q = (int)(k % kG2PBoxNodes);
+                    const int x = src.org[0] + q % kG2PBox, y = src.org[1] + (q / kG2PBox) % kG2PBox,
+                              z = src.org[2] + q / (kG2PBox * kG2PBox);
+                    if (in_grid(g, x, y, z)) {
+                        const uint32_t slot =
+                            s_nb[((x >> 3) - bc[0] + 1) + 3 * ((y >> 3) - bc[1] + 1) + 9 * ((z >> 3) - bc[2] + 1)];
+                        v[u] = vel[(uint32_t)axis * g.pool_cells + slot * kBrickCells + local_of_cell(x, y, z)];
+                    }
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                if (k0 + 256u * u < N) s_box[k0 + 256u * u] = v[u];
         }
     }
     __syncthreads();
@@ -615,8 +656,9 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key
     for (uint32_t base_i = beg; base_i < end; base_i += 256u) {
         const uint32_t i = base_i + threadIdx.x;
         uint32_t brick = kNoBrick;
+        const float4 pa = pa_next;
+        if (i + 256u < end) pa_next = p.pa[i + 256u];
         if (i < end) {
-            float4 pa = p.pa[i];
             if (pa.w >= 0.0f) {
                 const float grav[3] = {gx, gy, gz};
                 float pos[3] = {pa.x, pa.y, pa.z};
@@ -653,8 +695,9 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key
                 float x[3];
                 for (int a = 0; a < 3; ++a) x[a] = pos[a] + dt * v2[a];
                 int c[3];
-                pos_cell(g, pos[0], pos[1], pos[2], c);
-                if (cell_index(g, c[0], c[1], c[2]) < kBrickCells) {
+                // Every live particle of this key lies in brick `key` (its sort key is the brick of its clamped
+                // cell), so its cell is in an unallocated brick exactly when the brick is not staged.
+                if (!staged) {
                     for (int a = 0; a < 3; ++a) {
                         nv[a] = p.v[a][i] + grav[a] * dt;
                         x[a] = pos[a] + dt * nv[a];
@@ -708,10 +751,9 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key
                 brick = brick_of_cell(g, c[0], c[1], c[2]);
             }
         }
-        tb[threadIdx.x] = brick;
-        __syncthreads();
-        if (brick != kNoBrick && (threadIdx.x == 0 || tb[threadIdx.x - 1] != brick)) touch[brick] = 1u;
-        __syncthreads();
+        // mark the destination bricks, once per run of equal bricks within the wavefront (flags are a set)
+        const uint32_t prev = (uint32_t)__shfl_up((int)brick, 1u, 64);
+        if (brick != kNoBrick && ((threadIdx.x & 63u) == 0u || prev != brick)) touch[brick] = 1u;
     }
     for (int w = 32; w > 0; w >>= 1) {
         vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, w));
@@ -736,9 +778,8 @@ void launch_emit_fill(slosh_ctx *c, const slosh_emit_request *r, uint32_t count)
 {
     if (!count) return;
     {
-        SLOSH_TIMED(c, K_EMIT);
-        hipLaunchKernelGGL(k_emit_fill, dim3(blocks_for(count, 256)), dim3(256), 0, c->stream, c->geo, c->parts[c->cur],
-                           c->np, count, *r, c->touch);
+        launch_timed(c, K_EMIT, k_emit_fill, dim3(blocks_for(count, 256)), dim3(256), 0, c->geo, c->parts[c->cur],
+                     c->np, count, *r, c->touch);
     }
     SLOSH_HIP(hipGetLastError());
 }
@@ -772,29 +813,25 @@ void run_sort(slosh_ctx *c)
     }
     const uint32_t ntiles = blocks_for(np, kTile);
     {
-        SLOSH_TIMED(c, K_SORT_TILES);
-        hipLaunchKernelGGL(k_sort_tiles, dim3(ntiles), dim3(kTile), 0, c->stream, c->geo, c->parts[c->cur].pa, np, dead,
-                           c->packed, c->tile_nruns, c->run_key, c->run_len, c->key_count, c->key_runs);
+        launch_timed(c, K_SORT_TILES, k_sort_tiles, dim3(ntiles), dim3(kTile), 0, c->geo, c->parts[c->cur].pa, np, dead,
+                     c->packed, c->tile_nruns, c->run_key, c->run_len, c->key_count, c->key_runs);
     }
     SLOSH_HIP(hipGetLastError());
     scan_exclusive(c, c->key_count, c->key_off, c->nkeys);
     scan_exclusive(c, c->key_runs, c->run_off, c->nkeys);
     {
-        SLOSH_TIMED(c, K_SORT_RUNS);
-        hipLaunchKernelGGL(k_sort_runs_scatter, dim3(ntiles), dim3(kTile), 0, c->stream, c->tile_nruns, c->run_key,
-                           c->run_off, c->run_cursor, c->run_sorted);
+        launch_timed(c, K_SORT_RUNS, k_sort_runs_scatter, dim3(ntiles), dim3(kTile), 0, c->tile_nruns, c->run_key,
+                     c->run_off, c->run_cursor, c->run_sorted);
     }
     SLOSH_HIP(hipGetLastError());
     {
-        SLOSH_TIMED(c, K_SORT_KEY_RUNS);
-        hipLaunchKernelGGL(k_sort_key_runs, dim3(c->nkeys - 1u), dim3(256), 0, c->stream, c->run_off, c->key_off,
-                           c->run_len, c->run_sorted, c->run_base);
+        launch_timed(c, K_SORT_KEY_RUNS, k_sort_key_runs, dim3(c->nkeys - 1u), dim3(256), 0, c->run_off, c->key_off,
+                     c->run_len, c->run_sorted, c->run_base);
     }
     SLOSH_HIP(hipGetLastError());
     {
-        SLOSH_TIMED(c, K_SORT_SCATTER);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kTile), 0, c->stream, c->parts[c->cur],
-                           c->parts[1 - c->cur], np, c->packed, c->run_key, c->run_base, dead);
+        launch_timed(c, K_SORT_SCATTER, k_sort_scatter, dim3(ntiles), dim3(kTile), 0, c->parts[c->cur],
+                     c->parts[1 - c->cur], np, c->packed, c->run_key, c->run_base, dead);
     }
     SLOSH_HIP(hipGetLastError());
     c->cur = 1 - c->cur;

@@ -386,16 +386,14 @@ void pooled_build_type(slosh_ctx *c, int l)
 void pooled_build_info(slosh_ctx *c, int l)
 {
     PLevel L = plevel_of(c, l);
-    SLOSH_TIMED(c, K_MG_SETUP);
-    hipLaunchKernelGGL(k_pinfo, dim3(pass_blocks(c, L)), dim3(256), 0, c->stream, c->geo, c->nactive, L);
+    launch_timed(c, K_MG_SETUP, k_pinfo, dim3(pass_blocks(c, L)), dim3(256), 0, c->geo, c->nactive, L);
 }
 
 void dense_build_type_from_pooled(slosh_ctx *c, int l)
 {
     DenseLevel C = dlevel_of(c, l);
-    SLOSH_TIMED(c, K_MG_SETUP);
-    hipLaunchKernelGGL(k_dtype_from_pooled, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, c->geo,
-                       plevel_of(c, l - 1), C, fluid_coarsening(c));
+    launch_timed(c, K_MG_SETUP, k_dtype_from_pooled, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->geo,
+                 plevel_of(c, l - 1), C, fluid_coarsening(c));
 }
 
 void pooled_down(slosh_ctx *c, int l)

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_fp16.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -314,6 +315,7 @@ void launch_g2p(slosh_ctx *c, float dt, const float g[3]);
 
 // ---- grid.hip ----
 void run_pool(slosh_ctx *c);
+void build_brick_rows(slosh_ctx *c);
 void launch_normalize(slosh_ctx *c, float dt, const float g[3]);
 void launch_divergence(slosh_ctx *c, float dt);
 void launch_project(slosh_ctx *c);
@@ -412,6 +414,22 @@ struct KTimer {
         if (slot >= 0) (void)hipEventRecord(c->tev[slot + 1], c->stream);
     }
 };
+
+// Launches one kernel on the context's stream. When kernel `id` is selected for timing, the event pair rides on
+// the dispatch itself (hipExtLaunchKernelGGL), so the elapsed time is the kernel's own begin-to-end time, without
+// the gaps an event recorded before and after the launch would add.
+template <typename F, typename... Args>
+inline void launch_timed(slosh_ctx *c, int id, F kernel, dim3 grid, dim3 block, uint32_t shm, Args... args)
+{
+    if ((c->timing_mask >> id & 1ull) && c->tev_used + 2 <= c->tev_cap) {
+        const uint32_t slot = c->tev_used;
+        c->tev_used += 2;
+        c->tev_kernel[slot / 2] = id;
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, c->stream, c->tev[slot], c->tev[slot + 1], 0u, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shm, c->stream, args...);
+    }
+}
 } // namespace slosh
 
 #define SLOSH_TIMED(c, id) slosh::KTimer slosh_ktimer_(c, slosh::id)

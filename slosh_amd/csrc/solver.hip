@@ -1001,9 +1001,8 @@ void finalize(slosh_ctx *c, int kind, int iters = 0)
 
 void coarse_solve(slosh_ctx *c, LVec b, LVec x)
 {
-    SLOSH_TIMED(c, K_COARSE_SOLVE);
-    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(64), 0, c->stream, c->coarse_idx, c->coarse_n, c->coarse_inv, b,
-                       x, c->state);
+    launch_timed(c, K_COARSE_SOLVE, k_coarse_solve, dim3(1), dim3(64), 0, c->coarse_idx, c->coarse_n, c->coarse_inv, b,
+                 x, c->state);
 }
 
 // Levels 2 .. m-1 (m >= 5): the fused level-2/3 down pass with the restriction into level 4, grid-wide passes for
@@ -1015,9 +1014,8 @@ void dense_part(slosh_ctx *c)
     const DenseLevel L3 = dlevel_of(c, 3), L4 = dlevel_of(c, kFirstDense);
     const PLevel L2 = plevel_of(c, 2);
     {
-        SLOSH_TIMED(c, K_D3DOWN);
-        hipLaunchKernelGGL(k_d23down, dim3(c->ntiles_f), dim3(kD23DownThreads), 0, c->stream, c->geo, c->tile_list, L2,
-                           L3, L4, c->state);
+        launch_timed(c, K_D3DOWN, k_d23down, dim3(c->ntiles_f), dim3(kD23DownThreads), 0, c->geo, c->tile_list, L2,
+                     L3, L4, c->state);
     }
     int t = m - 1;
     uint32_t tail_cells = c->lv[m - 1].cells;
@@ -1028,12 +1026,10 @@ void dense_part(slosh_ctx *c)
         for (int k = 0; k < sweeps; ++k)
             for (uint32_t colour : {0u, 1u}) {
                 if (k == 0 && colour == 0u) continue; // the first red sweep is fused into the restriction
-                SLOSH_TIMED(c, K_DSMOOTH);
-                hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, colour,
-                                   c->state);
+                launch_timed(c, K_DSMOOTH, k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, F, colour,
+                             c->state);
             }
-        SLOSH_TIMED(c, K_DRESTRICT);
-        hipLaunchKernelGGL(k_drestrict, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
+        launch_timed(c, K_DRESTRICT, k_drestrict, dim3(blocks_for(C.cells, 256)), dim3(256), 0, F, C, c->state);
     }
     TailArgs T{};
     T.n = m - t;
@@ -1046,26 +1042,22 @@ void dense_part(slosh_ctx *c)
         T.total = T.off[k] + T.L[k].cells;
     }
     {
-        SLOSH_TIMED(c, K_COARSE_TAIL);
-        hipLaunchKernelGGL(k_coarse_tail, dim3(1), dim3(1024), 0, c->stream, T, c->coarse_idx, c->coarse_n,
-                           c->coarse_inv, c->state);
+        launch_timed(c, K_COARSE_TAIL, k_coarse_tail, dim3(1), dim3(1024), 0, T, c->coarse_idx, c->coarse_n,
+                     c->coarse_inv, c->state);
     }
     for (int l = t - 1; l >= kFirstDense; --l) {
         DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
         {
-            SLOSH_TIMED(c, K_DPROLONG);
-            hipLaunchKernelGGL(k_dprolong, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, C, c->state);
+            launch_timed(c, K_DPROLONG, k_dprolong, dim3(blocks_for(F.cells, 256)), dim3(256), 0, F, C, c->state);
         }
         for (int k = 0; k < sweeps; ++k)
             for (uint32_t colour : {1u, 0u}) {
-                SLOSH_TIMED(c, K_DSMOOTH);
-                hipLaunchKernelGGL(k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, c->stream, F, colour,
-                                   c->state);
+                launch_timed(c, K_DSMOOTH, k_dsmooth, dim3(blocks_for(F.cells, 256)), dim3(256), 0, F, colour,
+                             c->state);
             }
     }
-    SLOSH_TIMED(c, K_D3UP);
-    hipLaunchKernelGGL(k_d23up, dim3(c->ntiles_f), dim3(kD23UpThreads), 0, c->stream, c->geo, c->tile_list, L2, L3, L4,
-                       c->state);
+    launch_timed(c, K_D3UP, k_d23up, dim3(c->ntiles_f), dim3(kD23UpThreads), 0, c->geo, c->tile_list, L2, L3, L4,
+                 c->state);
 }
 
 // SloshProxy.cpp:904-959 (VCycle) with the exact coarsest solve, from the level-0 down-sweep iterate (red
@@ -1102,9 +1094,8 @@ void mg_setup(slosh_ctx *c)
             dense_build_type_from_pooled(c, l);
         } else {
             DenseLevel C = dlevel_of(c, l);
-            SLOSH_TIMED(c, K_MG_SETUP);
-            hipLaunchKernelGGL(k_build_type, dim3(blocks_for(C.cells, 256)), dim3(256), 0, c->stream, dlevel_of(c, l - 1),
-                               C, fluid_coarsening(c));
+            launch_timed(c, K_MG_SETUP, k_build_type, dim3(blocks_for(C.cells, 256)), dim3(256), 0, dlevel_of(c, l - 1),
+                         C, fluid_coarsening(c));
         }
     }
     for (int l = 1; l < m; ++l) {
@@ -1112,8 +1103,7 @@ void mg_setup(slosh_ctx *c)
             pooled_build_info(c, l);
         } else {
             DenseLevel D = dlevel_of(c, l);
-            SLOSH_TIMED(c, K_MG_SETUP);
-            hipLaunchKernelGGL(k_dinfo, dim3(blocks_for(D.cells, 256)), dim3(256), 0, c->stream, D);
+            launch_timed(c, K_MG_SETUP, k_dinfo, dim3(blocks_for(D.cells, 256)), dim3(256), 0, D);
         }
     }
     // tiles of level 3 for the fused level-2/3 passes (levels 3 and 4 exist below a coarser level)
@@ -1137,9 +1127,8 @@ void mg_setup(slosh_ctx *c)
     const int mode = m == 1 ? 0 : (c->lv[m - 1].pooled ? 1 : 2);
     PLevel P = mode == 1 ? plevel_of(c, m - 1) : PLevel{};
     DenseLevel D = mode == 2 ? dlevel_of(c, m - 1) : DenseLevel{};
-    SLOSH_TIMED(c, K_COARSE_FACTOR);
-    hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(256), 0, c->stream, c->geo, c->ctype, c->diag, mode, P, D,
-                       c->coarse_idx, c->coarse_n, c->coarse_inv);
+    launch_timed(c, K_COARSE_FACTOR, k_coarse_factor, dim3(1), dim3(256), 0, c->geo, c->ctype, c->diag, mode, P, D,
+                 c->coarse_idx, c->coarse_n, c->coarse_inv);
 }
 
 // Captures the launches of `body` on the context's stream into c->mg_graph (updating the executable graph in

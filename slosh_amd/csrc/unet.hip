@@ -569,9 +569,8 @@ void conv_dw(slosh_ctx *c, const uint16_t *in, const uint16_t *g, int layer)
     constexpr int BG = group_bricks(S);
     const uint32_t groups = (c->nactive + BG - 1) / BG;
     UNet &u = c->unet;
-    SLOSH_TIMED(c, K_UNET_TRAIN);
-    hipLaunchKernelGGL((k_conv_dw<CIN, COUT, S, CINW>), dim3(groups), dim3(256), 0, c->stream, c->binfo, c->nactive, in, g,
-                       u.grad + u.w_off[layer], u.grad + u.b_off[layer]);
+    launch_timed(c, K_UNET_TRAIN, (k_conv_dw<CIN, COUT, S, CINW>), dim3(groups), dim3(256), 0, c->binfo, c->nactive, in, g,
+                 u.grad + u.w_off[layer], u.grad + u.b_off[layer]);
 }
 
 uint32_t chunks(const slosh_ctx *c, int level, int C) { return blocks_for((uint64_t)c->nactive * (kBrickCells >> (3 * level)) * (C / 8), 256); }
@@ -664,34 +663,29 @@ void unet_apply(slosh_ctx *c, const float *r, float *z)
     conv<16, 16, 8, true, true>(c, u.x0, wf(0), bias(0), u.a0);
     conv<16, 16, 8, true, true>(c, u.a0, wf(1), bias(1), u.s0);
     {
-        SLOSH_TIMED(c, K_UNET_IO);
-        hipLaunchKernelGGL((k_pool<16, 8>), dim3(chunks(c, 1, 16)), dim3(256), 0, c->stream, c->binfo, nb, u.s0, u.p0);
+        launch_timed(c, K_UNET_IO, (k_pool<16, 8>), dim3(chunks(c, 1, 16)), dim3(256), 0, c->binfo, nb, u.s0, u.p0);
     }
     conv<16, 32, 4, true, true>(c, u.p0, wf(2), bias(2), u.a1);
     conv<32, 32, 4, true, true>(c, u.a1, wf(3), bias(3), u.s1);
     {
-        SLOSH_TIMED(c, K_UNET_IO);
-        hipLaunchKernelGGL((k_pool<32, 4>), dim3(chunks(c, 2, 32)), dim3(256), 0, c->stream, c->binfo, nb, u.s1, u.p1);
+        launch_timed(c, K_UNET_IO, (k_pool<32, 4>), dim3(chunks(c, 2, 32)), dim3(256), 0, c->binfo, nb, u.s1, u.p1);
     }
     conv<32, 64, 2, true, true>(c, u.p1, wf(4), bias(4), u.a2);
     conv<64, 64, 2, true, true>(c, u.a2, wf(5), bias(5), u.b2);
     conv<64, 32, 2, false, true>(c, u.b2, wf(6), bias(6), u.c2);
     {
-        SLOSH_TIMED(c, K_UNET_IO);
-        hipLaunchKernelGGL((k_upadd<32, 4>), dim3(chunks(c, 1, 32)), dim3(256), 0, c->stream, c->binfo, nb, u.c2, u.s1,
-                           u.t1);
+        launch_timed(c, K_UNET_IO, (k_upadd<32, 4>), dim3(chunks(c, 1, 32)), dim3(256), 0, c->binfo, nb, u.c2, u.s1,
+                     u.t1);
     }
     conv<32, 32, 4, true, true>(c, u.t1, wf(7), bias(7), u.d1);
     conv<32, 16, 4, false, true>(c, u.d1, wf(8), bias(8), u.c1);
     {
-        SLOSH_TIMED(c, K_UNET_IO);
-        hipLaunchKernelGGL((k_upadd<16, 8>), dim3(chunks(c, 0, 16)), dim3(256), 0, c->stream, c->binfo, nb, u.c1, u.s0,
-                           u.t0);
+        launch_timed(c, K_UNET_IO, (k_upadd<16, 8>), dim3(chunks(c, 0, 16)), dim3(256), 0, c->binfo, nb, u.c1, u.s0,
+                     u.t0);
     }
     conv<16, 16, 8, true, true>(c, u.t0, wf(9), bias(9), u.d0);
-    SLOSH_TIMED(c, K_UNET_IO);
-    hipLaunchKernelGGL(k_unet_head, dim3(nb), dim3(256), 0, c->stream, c->binfo, c->fmask, u.d0, wf(10), bias(10), u.sc,
-                       u.zhat, z);
+    launch_timed(c, K_UNET_IO, k_unet_head, dim3(nb), dim3(256), 0, c->binfo, c->fmask, u.d0, wf(10), bias(10), u.sc,
+                 u.zhat, z);
 }
 
 // Accumulates dL/dparams for the system (fmask, diag) and residual r into u.grad; the loss terms go to u.sc.
