@@ -71,8 +71,8 @@ def algorithmic_bytes(kernel, np_, nc, nf, nb, ns=None):
         # level-0 MGPCG passes: vectors on fluid cells, the one-byte stencil mask on every cell of a brick that
         # holds fluid; p (8+8), d, r_in, q, diag read; r_out, z written
         "k_update_black": 40 * nf + ns,
-        # z1, d_in, diag read; d_out, q written
-        "k_direction": 20 * nf + ns,
+        # z1, d_in, diag, r read; d_out, q written
+        "k_direction": 24 * nf + ns,
         # diag, z, r read; level-1 b and x written (one level-1 cell per 8 cells)
         "k_restrict0": 12 * nf + 2 * ns,
         # diag, r, z read, z1 written; level-1 stencil word and x1

@@ -153,13 +153,13 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(const uint32_t *binfo, co
 __global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo, const uint8_t *fmask,
                                                            const float *diag, double *pres, const float *r_in,
                                                            float *r_out, float *z, const float *d, const float *q,
-                                                           int first, double *partials, const SolverState *st)
+                                                           int first, const SolverState *st)
 {
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     const uint32_t done = st->done;
     const double alpha = first ? 0.0 : st->alpha;
     issue_point();
-    if (done) return;
+    if (done == 1u) return; // done == 2: the solve's last update, made before every other pass stops
     const uint32_t l0 = colour_cell(threadIdx.x, 0u), l1 = colour_cell(threadIdx.x, 1u);
     const uint32_t i0 = B.base + l0, i1 = B.base + l1;
     const uint32_t m0 = fmask[i0], m1 = fmask[i1];
@@ -181,7 +181,6 @@ __global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo
         p1 = pres[i1];
         for (int n = 0; n < 6; ++n) qj[n] = q[nbr(B, l1, n)];
     }
-    double v[1] = {0.0};
     if (m0 & kSelfFluid) {
         float ri = ra0;
         if (!first) {
@@ -189,7 +188,6 @@ __global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo
             ri = (float)((double)ra0 - alpha * (double)q0);
         }
         r_out[i0] = ri;
-        v[0] += (double)ri * (double)ri;
         z[i0] = ri / (g0 + kDiagEps);
     }
     if (m1 & kSelfFluid) {
@@ -199,7 +197,6 @@ __global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo
             ri = (float)((double)ra1 - alpha * (double)q1);
         }
         r_out[i1] = ri;
-        v[0] += (double)ri * (double)ri;
         float s = 0.0f;
         for (int n = 0; n < 6; ++n) {
             if (!(m1 >> n & 1u)) continue;
@@ -208,7 +205,6 @@ __global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo
         }
         z[i1] = (ri + s) / (g1 + kDiagEps);
     }
-    if (!first) block_partial<1>(v, partials);
 }
 
 // Restriction into pooled level 1: the residuals of the brick's 512 cells in LDS, then b_c = 1/4 of the fluid
@@ -347,8 +343,8 @@ __global__ __launch_bounds__(kThreads) void k_red_dot(const uint32_t *binfo, con
 // sees); partials d.q.
 __global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, const uint8_t *fmask,
                                                         const float *diag, const float *z1, const float *d_in,
-                                                        float *d_out, float *q, int first, double *partials,
-                                                        const SolverState *st)
+                                                        float *d_out, float *q, const float *r, int first,
+                                                        double *partials, const SolverState *st)
 {
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     const uint32_t done = st->done;
@@ -356,13 +352,14 @@ __global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, c
     issue_point();
     if (done) return;
     uint32_t i[2], m[2];
-    float zc[2], dc[2] = {0.0f, 0.0f}, gc[2], zn[2][6], dn[2][6];
+    float zc[2], dc[2] = {0.0f, 0.0f}, gc[2], rc[2], zn[2][6], dn[2][6];
     for (uint32_t c = 0; c < 2u; ++c) {
         const uint32_t l = colour_cell(threadIdx.x, c);
         i[c] = B.base + l;
         m[c] = fmask[i[c]];
         zc[c] = z1[i[c]];
         gc[c] = diag[i[c]];
+        rc[c] = r[i[c]];
         for (int n = 0; n < 6; ++n) zn[c][n] = z1[nbr(B, l, n)];
     }
     if (!first) {
@@ -372,7 +369,8 @@ __global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, c
             for (int n = 0; n < 6; ++n) dn[c][n] = d_in[nbr(B, l, n)];
         }
     }
-    double v[1] = {0.0};
+    // d.q, and for the stopping test of the coming update r - alpha q: r.q, q.q and r.r (q as stored, in FP32)
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
         if (!(m[c] & kSelfFluid)) continue;
         const float dv = first ? zc[c] : (float)((double)zc[c] + beta * (double)dc[c]);
@@ -385,10 +383,14 @@ __global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, c
         }
         const float dg = gc[c] + kDiagEps;
         const double qd = (double)dg * (double)dv - s;
-        q[i[c]] = (float)qd;
+        const float qf = (float)qd;
+        q[i[c]] = qf;
         v[0] += (double)dv * qd;
+        v[1] += (double)rc[c] * (double)qf;
+        v[2] += (double)qf * (double)qf;
+        v[3] += (double)rc[c] * (double)rc[c];
     }
-    block_partial<1>(v, partials);
+    block_partial<4>(v, partials);
 }
 
 // Red-black SOR sweep on the FP64 pressure: p += w ((sum p_n - div) / diag - p).
@@ -524,7 +526,7 @@ void l0_pcg_init(slosh_ctx *c, float *r_out)
 
 void l0_update_black(slosh_ctx *c, const float *r_in, float *r_out, const float *d, int first)
 {
-    L0_LAUNCH(K_UPDATE_BLACK, k_update_black, c->diag, c->pres, r_in, r_out, c->z, d, c->q, first, c->partials, c->state);
+    L0_LAUNCH(K_UPDATE_BLACK, k_update_black, c->diag, c->pres, r_in, r_out, c->z, d, c->q, first, c->state);
 }
 
 void l0_restrict(slosh_ctx *c, const float *r)
@@ -539,9 +541,9 @@ void l0_prolong_black(slosh_ctx *c, const float *r)
 
 void l0_red_dot(slosh_ctx *c, const float *r) { L0_LAUNCH(K_RED_DOT, k_red_dot, c->diag, r, c->z1, c->partials, c->state); }
 
-void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, int first)
+void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, const float *r, int first)
 {
-    L0_LAUNCH(K_DIRECTION, k_direction, c->diag, c->z1, d_in, d_out, c->q, first, c->partials, c->state);
+    L0_LAUNCH(K_DIRECTION, k_direction, c->diag, c->z1, d_in, d_out, c->q, r, first, c->partials, c->state);
 }
 
 void l0_fcg_zq(slosh_ctx *c, const float *z) { L0_LAUNCH(K_FCG, k_fcg_zq, z, c->q, c->partials, c->state); }

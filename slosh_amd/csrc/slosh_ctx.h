@@ -31,7 +31,8 @@ struct SolverState {
     double rel_true; // ||b - A p|| / ||b|| of the last true-residual check
     double nf;       // fluid cells
     double mg_scale; // FP16 levels: the V-cycle input scale sqrt(nf / ||r||^2), so the levels hold O(1) values
-    int32_t done;  // 1 once converged or out of iterations: every solver kernel then returns at once
+    int32_t done;  // 1 once converged or out of iterations: every solver kernel then returns at once; 2 when the
+                   // MGPCG update being made is the last (only the update pass runs; the next finalize sets 1)
     int32_t iters;
     int32_t maxit;
     int32_t converged;
@@ -82,7 +83,7 @@ struct LVec {
 constexpr uint32_t kBinfoWords = 32u;
 // Values a level-0 reduction writes per workgroup (k_pcg_init: ||b||^2, ||r||^2, fluid cells); the partials
 // buffer holds this many per pool slot, one workgroup per allocated brick.
-constexpr uint32_t kMaxBlockPartials = 3u;
+constexpr uint32_t kMaxBlockPartials = 4u;
 
 // Brick handled by workgroup w of a one-brick-per-workgroup pass over n bricks. Workgroups are dispatched
 // round-robin over the MI355X's 8 XCDs, each with its own L2; giving XCD k a contiguous range of the ascending
@@ -270,6 +271,7 @@ struct slosh_ctx {
     uint32_t n_p2g_blocks = 0, n_div_blocks = 0, n_g2p_blocks = 0;
     // MGPCG iterations per host check of the solver state, and their replay as a captured HIP graph
     int solver_chunk = 4;
+    int last_mg_iters = 0; // iterations of the previous MGPCG solve (chunks enqueued before the first state read)
     bool use_graphs = true;
     hipGraphExec_t mg_graph = nullptr;
     // learned preconditioner and the residuals captured for its training
@@ -358,7 +360,7 @@ void l0_prolong_black(slosh_ctx *c, const float *r);
 // Red sweep on z1 and the partials r.z1.
 void l0_red_dot(slosh_ctx *c, const float *r);
 // d_out = z1 + beta d_in (first: d_out = z1), q = A d_out; partials d.q.
-void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, int first);
+void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, const float *r, int first);
 void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w);
 // Partials ||b||^2 and ||b - A p||^2 of the FP64 pressure (the true residual).
 void l0_sor_residual(slosh_ctx *c);

@@ -50,12 +50,15 @@ template <typename V> __device__ __forceinline__ float dsum(const DenseLevel &L,
     return s;
 }
 
-enum FinKind { kFinInit = 0, kFinRho = 1, kFinAlpha = 2, kFinRr = 3, kFinSor = 4, kFinBetaF = 5, kFinAlphaF = 6,
-               kFinCheck = 7 };
+enum FinKind { kFinInit = 0, kFinRho = 1, kFinAlphaStop = 2, kFinRr = 3, kFinSor = 4, kFinBetaF = 5,
+               kFinAlphaF = 6, kFinCheck = 7 };
 
 __host__ __device__ constexpr int fin_values(int kind)
 {
-    return kind == kFinInit ? 3 : ((kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck) ? 2 : 1);
+    if (kind == kFinAlphaStop) return 4;
+    if (kind == kFinInit) return 3;
+    if (kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck) return 2;
+    return 1;
 }
 
 // Sums the partials of the preceding reduction in a fixed order (per thread: its strided run in increasing order;
@@ -63,8 +66,9 @@ __host__ __device__ constexpr int fin_values(int kind)
 __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
                                                    SolverState *st)
 {
+    constexpr int kMaxV = 4;
     const int nv = fin_values(kind);
-    __shared__ double sh[3][16];
+    __shared__ double sh[kMaxV][16];
     const uint32_t t = threadIdx.x;
     // the first 8 partials of this thread (b = t + 1024 k) are read together with the done flag; a thread sums its
     // partials in increasing b (an absent one adds +0, which leaves the sum unchanged: it starts at +0 and never
@@ -72,29 +76,34 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
     constexpr int kFirst = 8;
     const SolverState s0 = *st; // read with the partials; thread 0 updates the state from it
     const uint32_t done = (uint32_t)s0.done;
-    double v[kFirst][3];
+    double v[kFirst][kMaxV];
     for (int k = 0; k < kFirst; ++k) {
         const uint32_t b = t + 1024u * (uint32_t)k;
-        for (int j = 0; j < 3; ++j) v[k][j] = (b < nblocks && j < nv) ? partials[(size_t)b * nv + j] : 0.0;
+        for (int j = 0; j < kMaxV; ++j) v[k][j] = (b < nblocks && j < nv) ? partials[(size_t)b * nv + j] : 0.0;
     }
     issue_point();
-    if (kind != kFinInit && kind != kFinCheck && done) return;
-    double a[3] = {0.0, 0.0, 0.0};
+    if (kind != kFinInit && kind != kFinCheck && done) {
+        // the update that met the stopping test has run: from here on every pass returns at once
+        if (kind == kFinAlphaStop && done == 2u && t == 0) st->done = 1;
+        return;
+    }
+    double a[kMaxV] = {0.0, 0.0, 0.0, 0.0};
     for (int k = 0; k < kFirst; ++k)
-        for (int j = 0; j < 3; ++j) a[j] += v[k][j];
+        for (int j = 0; j < kMaxV; ++j) a[j] += v[k][j];
     for (uint32_t b = t + 1024u * kFirst; b < nblocks; b += 1024u)
         for (int j = 0; j < nv; ++j) a[j] += partials[(size_t)b * nv + j];
     for (int w = 32; w > 0; w >>= 1)
-        for (int j = 0; j < 3; ++j) a[j] += __shfl_xor(a[j], w);
+        for (int j = 0; j < kMaxV; ++j) a[j] += __shfl_xor(a[j], w);
     if ((t & 63u) == 0u)
-        for (int j = 0; j < 3; ++j) sh[j][t >> 6] = a[j];
+        for (int j = 0; j < kMaxV; ++j) sh[j][t >> 6] = a[j];
     __syncthreads();
     if (t != 0) return;
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
     for (int w = 0; w < 16; ++w) {
         t0 += sh[0][w];
         t1 += sh[1][w];
         t2 += sh[2][w];
+        t3 += sh[3][w];
     }
     if (kind == kFinInit) {
         st->nf = t2;
@@ -126,9 +135,25 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
     } else if (kind == kFinRho) {
         st->beta = fabs(s0.rho) > 1e-30 ? t0 / s0.rho : 0.0;
         st->rho = t0;
-    } else if (kind == kFinAlpha) {
+    } else if (kind == kFinAlphaStop) {
+        // t0 = d.q, t1 = r.q, t2 = q.q, t3 = r.r of the current residual r: alpha, and the stopping test on
+        // ||r - alpha q||^2 = r.r - 2 alpha r.q + alpha^2 q.q (each term summed afresh from this iteration's vectors)
         st->dq = t0;
-        st->alpha = fabs(t0) > 1e-30 ? s0.rho / t0 : 0.0;
+        const double alpha = fabs(t0) > 1e-30 ? s0.rho / t0 : 0.0;
+        st->alpha = alpha;
+        const double rr = fmax(t3 - 2.0 * alpha * t1 + alpha * alpha * t2, 0.0);
+        st->rr = rr;
+        st->mg_scale = rr > 0.0 ? sqrt(s0.nf / rr) : 1.0;
+        const double rel = sqrt(rr) / sqrt(s0.bb);
+        st->rel = rel;
+        const int it = s0.iters + 1;
+        st->iters = it;
+        if (rel <= s0.tol) {
+            st->converged = 1;
+            st->done = 2;
+        } else if (it >= s0.maxit) {
+            st->done = 2;
+        }
     } else if (kind == kFinBetaF) {
         // flexible CG: beta = -(z . q_prev) / (d_prev . q_prev)
         st->beta = fabs(s0.dq) > 1e-30 ? -t0 / s0.dq : 0.0;
@@ -1205,7 +1230,7 @@ void run_mgpcg(slosh_ctx *c)
     l0_update_black(c, R[0], R[1], D[0], 1);
     int rc = 1, dc = 0;
     vcycle(c, R[rc]);
-    l0_direction(c, D[1], D[0], 1);
+    l0_direction(c, D[1], D[0], R[rc], 1);
     SLOSH_HIP(hipGetLastError());
     // Iterations are enqueued in chunks between reads of the device state (an even count, so the ping-pong
     // buffers return to the same roles). Without residual capture or kernel timing, a chunk is captured once per
@@ -1214,30 +1239,39 @@ void run_mgpcg(slosh_ctx *c)
     const bool use_graph = c->use_graphs && !c->capture && c->timing_mask == 0;
     auto enqueue_chunk = [&](int it0) {
         for (int k = 0; k < chunk; ++k) {
-            finalize(c, kFinAlpha);
+            // alpha and the stopping test of the update it makes (done == 2: only that update still runs)
+            finalize(c, kFinAlphaStop);
             l0_update_black(c, R[rc], R[rc ^ 1], D[dc], 0);
             rc ^= 1;
-            finalize(c, kFinRr);
             capture(R[rc], it0 + k + 1);
             vcycle(c, R[rc]);
-            l0_direction(c, D[dc], D[dc ^ 1], 0);
+            l0_direction(c, D[dc], D[dc ^ 1], R[rc], 0);
             dc ^= 1;
         }
     };
     bool graph_ready = false;
-    for (int it = 0; it <= maxit; it += chunk) {
-        if (use_graph && c->use_graphs) {
-            // on a capture failure the helper has enqueued the chunk directly and switched graphs off
-            if (!graph_ready) graph_ready = capture_chunk_graph(c, [&] { enqueue_chunk(it); });
-            if (graph_ready) SLOSH_HIP(hipGraphLaunch(c->mg_graph, c->stream));
-        } else {
-            enqueue_chunk(it);
+    // Before the first read of the state, as many chunks as the previous solve's iteration count fills (less one
+    // iteration): consecutive solves of a simulation take similar counts, and a chunk enqueued past convergence
+    // only costs its early-exit launches. Afterwards the state is read after every chunk.
+    int ahead = c->last_mg_iters > 1 ? (c->last_mg_iters - 1) / chunk : 1;
+    if (ahead < 1) ahead = 1;
+    for (int it = 0; it <= maxit;) {
+        for (int k = 0; k < ahead && it <= maxit; ++k, it += chunk) {
+            if (use_graph && c->use_graphs) {
+                // on a capture failure the helper has enqueued the chunk directly and switched graphs off
+                if (!graph_ready) graph_ready = capture_chunk_graph(c, [&] { enqueue_chunk(it); });
+                if (graph_ready) SLOSH_HIP(hipGraphLaunch(c->mg_graph, c->stream));
+            } else {
+                enqueue_chunk(it);
+            }
+            SLOSH_HIP(hipGetLastError());
         }
-        SLOSH_HIP(hipGetLastError());
+        ahead = 1;
         SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
         if (c->h_state->done) break;
     }
+    c->last_mg_iters = c->h_state->iters;
     // the true residual ||b - A p|| / ||b|| of the FP64 pressure, reported with the step's results
     l0_true_residual(c);
     finalize(c, kFinCheck);
