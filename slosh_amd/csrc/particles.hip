@@ -342,23 +342,52 @@ __global__ __launch_bounds__(256) void k_p2g(Geo g, Parts p, const uint32_t *key
                 base[a] = (int)floorf(gg[a]);
                 fr[a] = gg[a] - (float)base[a];
             }
-            for (uint32_t oi = 0; oi < 8u; ++oi) {
-                int od[3] = {(int)(oi & 1u), (int)((oi >> 1) & 1u), (int)((oi >> 2) & 1u)};
-                int node[3] = {base[0] + od[0], base[1] + od[1], base[2] + od[2]};
-                if (!in_grid(g, node[0], node[1], node[2])) continue;
-                float w3[3];
-                for (int a = 0; a < 3; ++a) w3[a] = od[a] ? fr[a] : 1.0f - fr[a];
-                float w = w3[0] * w3[1] * w3[2];
-                if (w <= 0.0f) continue;
-                float dxn[3];
-                for (int a = 0; a < 3; ++a) dxn[a] = ((float)node[a] + off[a]) * g.h - xp[a];
-                float dot = cp0 * dxn[0] + cp1 * dxn[1] + cp2 * dxn[2];
-                int32_t mom = (int32_t)rintf(w * (up + dot) * kFixedScale);
-                int32_t wt = (int32_t)rintf(w * kFixedScale);
-                int tx = node[0] - ox, ty = node[1] - oy, tz = node[2] - oz;
-                uint32_t li = (uint32_t)(tx + kTileDim * (ty + kTileDim * tz));
-                atomicAdd(&tile[axis * kTileNodes + li], mom);
-                atomicAdd(&tile[(3 + axis) * kTileNodes + li], wt);
+            // Per axis a and corner offset o in {0, 1}: the weight factor, whether the node is in the grid, and
+            // the affine term cp_a * ((node + off) h - x). A corner combines them as w = (w_x w_y) w_z and
+            // dot = (cp0 dx + cp1 dy) + cp2 dz, the same products and sums, in the same order, as per corner.
+            float w1[3][2], cd[3][2];
+            bool inb[3][2];
+            const float cpa[3] = {cp0, cp1, cp2};
+            for (int a = 0; a < 3; ++a)
+                for (int o = 0; o < 2; ++o) {
+                    const int node = base[a] + o;
+                    inb[a][o] = node >= 0 && node < g.n[a];
+                    w1[a][o] = o ? fr[a] : 1.0f - fr[a];
+                    cd[a][o] = cpa[a] * (((float)node + off[a]) * g.h - xp[a]);
+                }
+            float wxy[2][2], cxy[2][2];
+            for (int oy = 0; oy < 2; ++oy)
+                for (int ox = 0; ox < 2; ++ox) {
+                    wxy[oy][ox] = w1[0][ox] * w1[1][oy];
+                    cxy[oy][ox] = cd[0][ox] + cd[1][oy];
+                }
+            // the particle lies in brick b, so every corner node lies in its tile (node - tile origin in [0, 9])
+            const uint32_t li0 = (uint32_t)((base[0] - ox) + kTileDim * ((base[1] - oy) + kTileDim * (base[2] - oz)));
+            int32_t *tm = tile + axis * kTileNodes + li0, *tw = tile + (3 + axis) * kTileNodes + li0;
+            // All eight nodes in the grid and no fractional part zero: every weight is positive (a nonzero
+            // fraction is at least 2^-15 here, so the product of three factors stays a normal float) and every
+            // corner adds its sums. Otherwise a node outside the grid or of zero weight adds nothing.
+            const bool all = inb[0][0] & inb[0][1] & inb[1][0] & inb[1][1] & inb[2][0] & inb[2][1] & (fr[0] != 0.0f) &
+                             (fr[1] != 0.0f) & (fr[2] != 0.0f);
+            if (all) {
+                for (uint32_t oi = 0; oi < 8u; ++oi) {
+                    const int cx = (int)(oi & 1u), cy = (int)((oi >> 1) & 1u), cz = (int)(oi >> 2);
+                    const float w = wxy[cy][cx] * w1[2][cz];
+                    const float dot = cxy[cy][cx] + cd[2][cz];
+                    const uint32_t lo = (uint32_t)(cx + kTileDim * (cy + kTileDim * cz));
+                    atomicAdd(&tm[lo], (int32_t)rintf(w * (up + dot) * kFixedScale));
+                    atomicAdd(&tw[lo], (int32_t)rintf(w * kFixedScale));
+                }
+            } else {
+                for (uint32_t oi = 0; oi < 8u; ++oi) {
+                    const int cx = (int)(oi & 1u), cy = (int)((oi >> 1) & 1u), cz = (int)(oi >> 2);
+                    const float w = wxy[cy][cx] * w1[2][cz];
+                    if (!(inb[0][cx] & inb[1][cy] & inb[2][cz]) || w <= 0.0f) continue;
+                    const float dot = cxy[cy][cx] + cd[2][cz];
+                    const uint32_t lo = (uint32_t)(cx + kTileDim * (cy + kTileDim * cz));
+                    atomicAdd(&tm[lo], (int32_t)rintf(w * (up + dot) * kFixedScale));
+                    atomicAdd(&tw[lo], (int32_t)rintf(w * kFixedScale));
+                }
             }
         }
         int c[3];
