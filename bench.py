@@ -55,12 +55,14 @@ def algorithmic_bytes(kernel, np_, nc, nf, nb, ns=None):
     if ns is None:
         ns = nc
     table = {
-        # pos+age 16 B read; pos+age, vel, C written (64 B); staged face velocities once per cell
-        "k_g2p": 80 * np_ + 12 * nc,
+        # G2P first pass: pos+age 16 B read, new brick key 4 B written; staged face velocities once per cell
+        "k_g2p_keys": 20 * np_ + 12 * nc,
+        # G2P second pass: pos+age, key and in-tile rank read (24 B), the 64 B record written at its sorted place
+        "k_g2p_move": 88 * np_ + 12 * nc,
         # 64-byte record read and written, 4 B in-tile rank read
         "k_sort_scatter": 132 * np_,
-        # pos+age 16 B read, 4 B rank written
-        "k_sort_tiles": 20 * np_,
+        # the first G2P pass's key read (4 B), 4 B rank written (a step after emission reads pos+age instead)
+        "k_sort_tiles": 8 * np_,
         # pos, vel, C read (64 B); brick tile of 6000 ints written; flags written; FP64 pressure read+written
         "k_p2g": 64 * np_ + 24000 * nb + 4 * nc + 16 * nc,
         # tiles read; flags read; velocity, weight (FP32 x3) and face flags (1 B x3) written

@@ -118,6 +118,10 @@ class OracleSim:
         self.lib.osim_read_particles(self.h, out.ctypes.data, n)
         return out
 
+    def read_particles_brick_order(self):
+        """The records in the order the next step's sort gives them (the library's order after a full step)."""
+        return sort_by_brick(self.read_particles(), self.dims, self.settings.cell_size)
+
     def read_grid(self, field):
         nx, ny, nz = self.dims
         if field == abi.FIELD_BRICK_TABLE:
@@ -155,6 +159,24 @@ class OracleSim:
             return float(self.lib.osim_true_residual_d(self.h, dense_p.ctypes.data))
         dense_p = np.ascontiguousarray(dense_p, dtype=np.float32)
         return float(self.lib.osim_true_residual(self.h, dense_p.ctypes.data))
+
+
+def sort_by_brick(parts, grid, cell_size):
+    """`parts` (PARTICLE_DTYPE records) in the stable order by 8^3 brick key of the clamped cell, dead records
+    dropped: the order the reference's sort at the start of the next step gives (SloshSort.usf:65-135, stable as
+    restated in oracle.c). The library sorts by the new keys at the end of a step, so after a full step its records
+    are the oracle's records in this order."""
+    grid = [int(v) for v in grid]
+    inv_h = np.float32(1.0) / np.float32(cell_size)
+    pos = parts["pos"].astype(np.float32)
+    cell = np.floor(pos * inv_h).astype(np.int64)
+    for a in range(3):
+        cell[:, a] = np.clip(cell[:, a], 0, grid[a] - 1)
+    bd = [(g + 7) // 8 for g in grid]
+    key = (cell[:, 0] >> 3) + bd[0] * ((cell[:, 1] >> 3) + bd[1] * (cell[:, 2] >> 3))
+    live = parts["age"] >= 0.0
+    order = np.argsort(np.where(live, key, bd[0] * bd[1] * bd[2]), kind="stable")
+    return parts[order[: int(live.sum())]]
 
 
 def pcg_hash(v):

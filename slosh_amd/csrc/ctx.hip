@@ -19,11 +19,12 @@ using namespace slosh;
 namespace slosh {
 const char *const kKernelNames[K_COUNT] = {
     "k_emit_fill", "k_sort_tiles", "k_sort_runs_scatter", "k_sort_key_runs", "k_sort_scatter", "k_scan",
-    "k_pool", "k_p2g", "k_normalize", "k_divergence", "k_project", "k_div_stats", "k_extrapolate", "k_g2p",
+    "k_pool", "k_p2g", "k_normalize", "k_divergence", "k_project", "k_div_stats", "k_extrapolate", "k_g2p_keys",
     "k_mg_setup", "k_coarse_factor", "k_pcg_init", "k_red_dot", "k_restrict0", "k_prolong_black", "k_direction",
     "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d23down", "k_d23up",
     "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
-    "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io", "k_unet_train", "k_fcg"};
+    "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io", "k_unet_train", "k_fcg",
+    "k_g2p_move"};
 } // namespace slosh
 
 static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");
@@ -56,7 +57,7 @@ void free_all(slosh_ctx *c)
         for (int q = 0; q < 9; ++q) dfree(c->parts[k].c[q]);
     }
     dfree(c->packed); dfree(c->tile_nruns); dfree(c->run_key); dfree(c->run_len); dfree(c->run_base);
-    dfree(c->run_sorted); dfree(c->key_count); dfree(c->key_off); dfree(c->key_runs); dfree(c->run_off);
+    dfree(c->run_sorted); dfree(c->key_count); dfree(c->key_off); dfree(c->key_off_next); dfree(c->g2p_keys); dfree(c->key_runs); dfree(c->run_off);
     dfree(c->run_cursor); dfree(c->scan_blocks);
     dfree(c->table); dfree(c->slot_brick); dfree(c->active); dfree(c->need); dfree(c->need_rank);
     dfree(c->freeflag); dfree(c->free_rank); dfree(c->free_list); dfree(c->alloc_flag); dfree(c->alloc_rank);
@@ -174,6 +175,8 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->key_runs, c->nkeys);
     dalloc(&c->run_cursor, c->nkeys);
     dalloc(&c->key_off, (size_t)c->nkeys + 1u);
+    dalloc(&c->key_off_next, (size_t)c->nkeys + 1u);
+    dalloc(&c->g2p_keys, (size_t)c->cap);
     dalloc(&c->run_off, (size_t)c->nkeys + 1u);
     uint32_t maxn = c->nkeys > g.npool ? c->nkeys : g.npool;
     dalloc(&c->scan_blocks, (size_t)(maxn / 1024u + 2u));
@@ -303,6 +306,7 @@ void reset_impl(slosh_ctx *c)
     const Geo &g = c->geo;
     c->np = 0;
     c->ndead = 0;
+    c->sorted = false;
     c->cur = 0;
     c->vcur = 0;
     c->nactive = 0;
@@ -355,7 +359,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     c->n_div_blocks = 0;
     c->n_g2p_blocks = 0;
     SLOSH_HIP(hipEventRecord(c->ev_step0, c->stream));
-    run_sort(c);
+    if (!c->sorted) run_sort(c);
     c->np -= c->ndead;
     c->ndead = 0;
     c->res.live_particles = c->np;
@@ -415,6 +419,9 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     c->res.max_divergence = f;
     c->res.mean_divergence = st.div_count ? (float)(st.div_sum / (double)st.div_count) : 0.0f;
     c->res.clamped_particles = st.clamped;
+    if (st.misplaced)
+        throw HipError(hipErrorUnknown, "G2P passes disagree on particles' bricks (record order inconsistent)",
+                       __FILE__, __LINE__);
     float ms = 0.0f;
     SLOSH_HIP(hipEventElapsedTime(&ms, c->ev_step0, c->ev_step1));
     c->res.step_ms = ms;
@@ -667,7 +674,7 @@ slosh_status slosh_emit(slosh_ctx *c, const slosh_emit_request *r)
     uint64_t count = (uint64_t)r->cell_dims[0] * (uint64_t)r->cell_dims[1] * (uint64_t)r->cell_dims[2] * 8u;
     if (count == 0) return SLOSH_OK;
     if ((uint64_t)c->np + count > (uint64_t)c->cap) return SLOSH_ECAPACITY;
-    ABI_GUARD(launch_emit_fill(c, r, (uint32_t)count); c->np += (uint32_t)count;
+    ABI_GUARD(launch_emit_fill(c, r, (uint32_t)count); c->np += (uint32_t)count; c->sorted = false;
               SLOSH_HIP(hipStreamSynchronize(c->stream)); return SLOSH_OK;)
 }
 
@@ -720,7 +727,8 @@ slosh_status slosh_write_particles(slosh_ctx *c, const void *src, uint32_t n)
 {
     if (!c || (n && !src)) return SLOSH_EINVAL;
     if (n > c->cap) return SLOSH_ECAPACITY;
-    ABI_GUARD(launch_fill_u8(c->stream, c->touch, 0u, c->geo.nbricks); c->np = 0; c->ndead = 0; if (n) {
+    ABI_GUARD(launch_fill_u8(c->stream, c->touch, 0u, c->geo.nbricks); c->np = 0; c->ndead = 0; c->sorted = false;
+              if (n) {
         ensure_staging(c, (size_t)n * sizeof(slosh_particle));
         SLOSH_HIP(hipMemcpyAsync(c->staging, src, (size_t)n * sizeof(slosh_particle), hipMemcpyHostToDevice, c->stream));
         launch_unpack_particles(c, (const slosh_particle *)c->staging, n);

@@ -2,6 +2,7 @@
 //
 // Launch shapes are wave64-native: 256-thread workgroups (4 waves) for particle passes and one workgroup per
 // allocated 8^3 brick for the P2G gather, whose LDS tile holds every node the brick's particles touch.
+#include <utility>
 #include "slosh_ctx.h"
 
 namespace slosh {
@@ -146,14 +147,16 @@ __device__ __forceinline__ uint32_t bitonic_tile(uint32_t x, uint32_t *s)
     return x;
 }
 
-// Sort pass 1. Per tile of 256 consecutive particles: key = 8^3 brick of the clamped cell (dead: last key), a
+// Sort pass 1. Per tile of 256 consecutive particles: key = 8^3 brick of the clamped cell (dead: last key), or the
+// given key when `keys` is set (the keys of G2P's first pass), a
 // stable in-tile rank per key from a sort of (key << 8 | lane), and the tile's runs of equal keys. The values are
 // distinct, so their sorted order is unique; a tile whose keys already ascend (the common case: particles were
 // sorted last step and few cross a brick face) skips the sort. Run starts are counted with wavefront ballots.
 // Reference counterpart: SortCountCS (SloshSort.usf:65-76), whose atomic rank is not stable.
-__global__ __launch_bounds__(256) void k_sort_tiles(Geo g, const float4 *pa, uint32_t np, uint32_t dead_key,
-                                                    uint32_t *packed, uint32_t *tile_nruns, uint32_t *run_key,
-                                                    uint32_t *run_len, uint32_t *key_count, uint32_t *key_runs)
+__global__ __launch_bounds__(256) void k_sort_tiles(Geo g, const float4 *pa, const uint32_t *keys, uint32_t np,
+                                                    uint32_t dead_key, uint32_t *packed, uint32_t *tile_nruns,
+                                                    uint32_t *run_key, uint32_t *run_len, uint32_t *key_count,
+                                                    uint32_t *key_runs)
 {
     __shared__ uint32_t s[kTile];
     __shared__ uint32_t start[kTile];
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(256) void k_sort_tiles(Geo g, const float4 *pa, uin
     const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
     const uint32_t i = blockIdx.x * kTile + t;
     uint32_t v = 0xFFFFFFFFu;
-    if (i < np) v = (key_of(g, pa[i], dead_key) << 8) | t;
+    if (i < np) v = ((keys ? keys[i] : key_of(g, pa[i], dead_key)) << 8) | t;
     s[t] = v;
     if (t == 0u) unsorted = 0u;
     __syncthreads();
@@ -612,13 +615,28 @@ __device__ __forceinline__ float sample_component(const Geo &g, const FaceSource
     return u;
 }
 
+// Where the second G2P pass writes: each particle's new record goes straight to its place in the order sorted by
+// the new brick keys (the first pass's keys, sorted by the same passes as at the start of a step).
+struct G2PMove {
+    Parts dst;
+    const uint32_t *packed;   // per particle: in-tile run index << 8 | rank within the run
+    const uint32_t *run_base; // per run: its first destination
+    uint32_t *misplaced;      // count of particles whose recomputed key differs from the first pass's
+};
+
 // Slosh.usf:1451-1678 (AdvectParticle + G2PAdvectCS), APIC; manipulators, adhesion and drains are off. One
 // workgroup per brick key of the sorted particle array: the brick's particles all lie inside it, so their gathers
 // read the staged box (an unallocated brick, whose particles are ballistic, reads through the table).
-__global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key_off, const float *vel,
-                                             const uint8_t *fflags, float dt, float gx, float gy, float gz,
-                                             float friction, float radius, float max_speed, const slosh_collider *col,
-                                             uint32_t ncol, uint8_t *touch, uint32_t *bp_speed, uint32_t *bp_clamp)
+// The step runs it twice on the same inputs. The first pass (kMove false) writes each particle's new brick key, the
+// destination-brick flags and the speed statistics; the second recomputes the same values (the same operations on
+// the same operands) and writes the 64-byte record to its sorted place, so the records are written once and never
+// moved by a separate scatter.
+template <bool kMove>
+__device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off, const float *vel,
+                                         const uint8_t *fflags, float dt, float gx, float gy, float gz, float friction,
+                                         float radius, float max_speed, const slosh_collider *col, uint32_t ncol,
+                                         uint8_t *touch, uint32_t *bp_speed, uint32_t *bp_clamp, uint32_t *keys,
+                                         uint32_t dead_key, const G2PMove &mv)
 {
     __shared__ float s_box[3 * kG2PBoxNodes];
     __shared__ uint32_t s_nb[27]; // slots of the 3x3x3 brick neighbourhood, (dx+1) + 3(dy+1) + 9(dz+1)
@@ -630,7 +648,7 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key
         sclamp = 0u;
     }
     if (beg == end) {
-        if (threadIdx.x == 0) {
+        if (!kMove && threadIdx.x == 0) {
             bp_speed[key] = 0u;
             bp_clamp[key] = 0u;
         }
@@ -688,6 +706,7 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key
         const float4 pa = pa_next;
         if (i + 256u < end) pa_next = p.pa[i + 256u];
         if (i < end) {
+            if (!kMove && !(pa.w >= 0.0f)) keys[i] = dead_key; // a dead record is dropped by the sort
             if (pa.w >= 0.0f) {
                 const float grav[3] = {gx, gy, gz};
                 float pos[3] = {pa.x, pa.y, pa.z};
@@ -771,19 +790,29 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key
                     for (int a = 0; a < 3; ++a)
                         if ((contact & (1u << a)) == 0u) nv[a] *= 1.0f - friction;
                 float speed = len3(nv[0], nv[1], nv[2]);
-                if (speed > max_speed) nclamp++;
-                p.pa[i] = make_float4(x[0], x[1], x[2], pa.w + dt);
-                for (int a = 0; a < 3; ++a) p.v[a][i] = nv[a];
-                for (int k = 0; k < 9; ++k) p.c[k][i] = cm[k];
-                vmax = max(vmax, __float_as_uint(speed));
                 pos_cell(g, x[0], x[1], x[2], c);
                 brick = brick_of_cell(g, c[0], c[1], c[2]);
+                if (kMove) {
+                    const uint32_t pk = mv.packed[i];
+                    const uint32_t j = mv.run_base[(i & ~(kTile - 1u)) + (pk >> 8)] + (pk & 255u);
+                    mv.dst.pa[j] = make_float4(x[0], x[1], x[2], pa.w + dt);
+                    for (int a = 0; a < 3; ++a) mv.dst.v[a][j] = nv[a];
+                    for (int k = 0; k < 9; ++k) mv.dst.c[k][j] = cm[k];
+                    if (brick != keys[i]) atomicAdd(mv.misplaced, 1u);
+                } else {
+                    if (speed > max_speed) nclamp++;
+                    vmax = max(vmax, __float_as_uint(speed));
+                    keys[i] = brick;
+                }
             }
         }
-        // mark the destination bricks, once per run of equal bricks within the wavefront (flags are a set)
-        const uint32_t prev = (uint32_t)__shfl_up((int)brick, 1u, 64);
-        if (brick != kNoBrick && ((threadIdx.x & 63u) == 0u || prev != brick)) touch[brick] = 1u;
+        if (!kMove) {
+            // mark the destination bricks, once per run of equal bricks within the wavefront (flags are a set)
+            const uint32_t prev = (uint32_t)__shfl_up((int)brick, 1u, 64);
+            if (brick != kNoBrick && ((threadIdx.x & 63u) == 0u || prev != brick)) touch[brick] = 1u;
+        }
     }
+    if (kMove) return;
     for (int w = 32; w > 0; w >>= 1) {
         vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, w));
         nclamp += (uint32_t)__shfl_xor((int)nclamp, w);
@@ -797,6 +826,26 @@ __global__ __launch_bounds__(256) void k_g2p(Geo g, Parts p, const uint32_t *key
         bp_speed[key] = smax;
         bp_clamp[key] = sclamp;
     }
+}
+
+__global__ __launch_bounds__(256) void k_g2p_keys(Geo g, Parts p, const uint32_t *key_off, const float *vel,
+                                                  const uint8_t *fflags, float dt, float gx, float gy, float gz,
+                                                  float friction, float radius, float max_speed,
+                                                  const slosh_collider *col, uint32_t ncol, uint8_t *touch,
+                                                  uint32_t *bp_speed, uint32_t *bp_clamp, uint32_t *keys,
+                                                  uint32_t dead_key)
+{
+    g2p_body<false>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, touch, bp_speed,
+                    bp_clamp, keys, dead_key, G2PMove{});
+}
+
+__global__ __launch_bounds__(256) void k_g2p_move(Geo g, Parts p, const uint32_t *key_off, const float *vel,
+                                                  const uint8_t *fflags, float dt, float gx, float gy, float gz,
+                                                  float friction, float radius, float max_speed,
+                                                  const slosh_collider *col, uint32_t ncol, uint32_t *keys, G2PMove mv)
+{
+    g2p_body<true>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, nullptr, nullptr,
+                   nullptr, keys, 0u, mv);
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
@@ -829,7 +878,10 @@ void launch_pack_particles(slosh_ctx *c, void *d_dst, int layout)
     SLOSH_HIP(hipGetLastError());
 }
 
-void run_sort(slosh_ctx *c)
+// The sort's key passes: stable in-tile ranks and runs, per-key counts and offsets, and each run's first
+// destination. Keys come from the positions, or from `keys` when given. The offsets go to the spare buffer
+// (key_off_next); the caller swaps it in once nothing reads the current array's offsets any more.
+static void sort_keys(slosh_ctx *c, const uint32_t *keys)
 {
     const uint32_t np = c->np;
     const uint32_t dead = c->nkeys - 1u;
@@ -837,59 +889,67 @@ void run_sort(slosh_ctx *c)
     launch_fill_u32(c->stream, c->key_runs, 0u, c->nkeys);
     launch_fill_u32(c->stream, c->run_cursor, 0u, c->nkeys);
     if (np == 0) {
-        scan_exclusive(c, c->key_count, c->key_off, c->nkeys);
+        scan_exclusive(c, c->key_count, c->key_off_next, c->nkeys);
         return;
     }
     const uint32_t ntiles = blocks_for(np, kTile);
-    {
-        launch_timed(c, K_SORT_TILES, k_sort_tiles, dim3(ntiles), dim3(kTile), 0, c->geo, c->parts[c->cur].pa, np, dead,
-                     c->packed, c->tile_nruns, c->run_key, c->run_len, c->key_count, c->key_runs);
-    }
+    launch_timed(c, K_SORT_TILES, k_sort_tiles, dim3(ntiles), dim3(kTile), 0, c->geo, c->parts[c->cur].pa, keys, np,
+                 dead, c->packed, c->tile_nruns, c->run_key, c->run_len, c->key_count, c->key_runs);
     SLOSH_HIP(hipGetLastError());
-    scan_exclusive(c, c->key_count, c->key_off, c->nkeys);
+    scan_exclusive(c, c->key_count, c->key_off_next, c->nkeys);
     scan_exclusive(c, c->key_runs, c->run_off, c->nkeys);
-    {
-        launch_timed(c, K_SORT_RUNS, k_sort_runs_scatter, dim3(ntiles), dim3(kTile), 0, c->tile_nruns, c->run_key,
-                     c->run_off, c->run_cursor, c->run_sorted);
-    }
+    launch_timed(c, K_SORT_RUNS, k_sort_runs_scatter, dim3(ntiles), dim3(kTile), 0, c->tile_nruns, c->run_key,
+                 c->run_off, c->run_cursor, c->run_sorted);
     SLOSH_HIP(hipGetLastError());
-    {
-        launch_timed(c, K_SORT_KEY_RUNS, k_sort_key_runs, dim3(c->nkeys - 1u), dim3(256), 0, c->run_off, c->key_off,
-                     c->run_len, c->run_sorted, c->run_base);
-    }
+    launch_timed(c, K_SORT_KEY_RUNS, k_sort_key_runs, dim3(c->nkeys - 1u), dim3(256), 0, c->run_off, c->key_off_next,
+                 c->run_len, c->run_sorted, c->run_base);
     SLOSH_HIP(hipGetLastError());
-    {
-        launch_timed(c, K_SORT_SCATTER, k_sort_scatter, dim3(ntiles), dim3(kTile), 0, c->parts[c->cur],
-                     c->parts[1 - c->cur], np, c->packed, c->run_key, c->run_base, dead);
-    }
+}
+
+void run_sort(slosh_ctx *c)
+{
+    sort_keys(c, nullptr);
+    std::swap(c->key_off, c->key_off_next);
+    if (c->np == 0) return;
+    launch_timed(c, K_SORT_SCATTER, k_sort_scatter, dim3(blocks_for(c->np, kTile)), dim3(kTile), 0, c->parts[c->cur],
+                 c->parts[1 - c->cur], c->np, c->packed, c->run_key, c->run_base, c->nkeys - 1u);
     SLOSH_HIP(hipGetLastError());
     c->cur = 1 - c->cur;
+    c->sorted = true;
 }
 
 void launch_p2g(slosh_ctx *c, float warm_scale)
 {
     if (!c->nactive) return;
-    {
-        SLOSH_TIMED(c, K_P2G);
-        hipLaunchKernelGGL(k_p2g, dim3(c->nactive), dim3(256), 0, c->stream, c->geo, c->parts[c->cur], c->key_off, c->tiles,
-                           c->tile_valid, c->ctype, c->pres, c->new_slot, warm_scale, c->d_col, c->ncol, c->bp_fluid);
-        c->n_p2g_blocks = c->nactive;
-    }
+    launch_timed(c, K_P2G, k_p2g, dim3(c->nactive), dim3(256), 0, c->geo, c->parts[c->cur], c->key_off, c->tiles,
+                 c->tile_valid, c->ctype, c->pres, c->new_slot, warm_scale, c->d_col, c->ncol, c->bp_fluid);
+    c->n_p2g_blocks = c->nactive;
     SLOSH_HIP(hipGetLastError());
 }
 
+// G2P and the sort by the new brick keys in one: the first pass writes each particle's new key, the key passes sort
+// them, and the second pass writes every new record straight to its sorted place in the other buffer. The next step
+// then starts from a sorted array and skips its own sort (until particles are emitted or written).
 void launch_g2p(slosh_ctx *c, float dt, const float g[3])
 {
     if (!c->np) return;
-    {
-        SLOSH_TIMED(c, K_G2P);
-        hipLaunchKernelGGL(k_g2p, dim3(c->geo.nbricks), dim3(256), 0, c->stream, c->geo, c->parts[c->cur], c->key_off,
-                           c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction,
-                           c->s.particle_radius_cells * c->geo.h, c->s.max_speed, c->d_col, c->ncol, c->touch,
-                           c->bp_speed, c->bp_clamp);
-        c->n_g2p_blocks = c->geo.nbricks;
-    }
+    const Geo &geo = c->geo;
+    const float radius = c->s.particle_radius_cells * geo.h;
+    launch_timed(c, K_G2P, k_g2p_keys, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
+                 c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction, radius, c->s.max_speed,
+                 c->d_col, c->ncol, c->touch, c->bp_speed, c->bp_clamp, c->g2p_keys, c->nkeys - 1u);
+    c->n_g2p_blocks = geo.nbricks;
     SLOSH_HIP(hipGetLastError());
+    // the new offsets go to the spare buffer: the second pass walks the current array by its own offsets
+    sort_keys(c, c->g2p_keys);
+    G2PMove mv{c->parts[1 - c->cur], c->packed, c->run_base, &c->stats->misplaced};
+    launch_timed(c, K_G2P_MOVE, k_g2p_move, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
+                 c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction, radius, c->s.max_speed,
+                 c->d_col, c->ncol, c->g2p_keys, mv);
+    SLOSH_HIP(hipGetLastError());
+    std::swap(c->key_off, c->key_off_next);
+    c->cur = 1 - c->cur;
+    c->sorted = true;
 }
 
 } // namespace slosh

@@ -146,7 +146,7 @@ enum KernelId {
     K_DIVERGENCE, K_PROJECT, K_DIV_STATS, K_EXTRAPOLATE, K_G2P, K_MG_SETUP, K_COARSE_FACTOR, K_PCG_INIT, K_RED_DOT,
     K_RESTRICT0, K_PROLONG_BLACK, K_DIRECTION, K_UPDATE_BLACK, K_PDOWN1, K_PDOWN2, K_PUP1, K_PUP2, K_D3DOWN, K_D3UP,
     K_DSMOOTH, K_DRESTRICT, K_DPROLONG, K_COARSE_SOLVE, K_COARSE_TAIL, K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL,
-    K_FINALIZE, K_STEP_STATS, K_UNET_CONV0, K_UNET_CONV1, K_UNET_CONV2, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_COUNT
+    K_FINALIZE, K_STEP_STATS, K_UNET_CONV0, K_UNET_CONV1, K_UNET_CONV2, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_G2P_MOVE, K_COUNT
 };
 // Names as the compiled kernels are called (what a kernel trace of the library shows).
 extern const char *const kKernelNames[K_COUNT];
@@ -192,7 +192,7 @@ struct StepStats {
     uint32_t overflow;
     uint32_t nactive;
     uint32_t nfree;
-    uint32_t pad;
+    uint32_t misplaced; // particles whose two G2P passes disagree on the new brick (never expected; fails the step)
     double div_sum;
     uint32_t div_count;
     uint32_t pad2;
@@ -209,6 +209,8 @@ struct slosh_ctx {
     uint32_t cap = 0;      // particle capacity
     uint32_t np = 0;       // particle slots in use (host-exact)
     uint32_t ndead = 0;    // dead records among them, dropped by the next sort
+    bool sorted = false;   // the records are in brick-key order with key_off current (the last step's G2P sorted
+                           // them); emitting or writing particles clears it and the next step sorts
     uint32_t ntiles_max = 0;
     uint32_t nactive = 0;  // allocated bricks after the last pool pass (read back once per step)
 
@@ -217,7 +219,8 @@ struct slosh_ctx {
 
     // sort scratch
     uint32_t *packed = nullptr, *tile_nruns = nullptr, *run_key = nullptr, *run_len = nullptr, *run_base = nullptr,
-             *run_sorted = nullptr, *key_count = nullptr, *key_off = nullptr, *key_runs = nullptr, *run_off = nullptr,
+             *run_sorted = nullptr, *key_count = nullptr, *key_off = nullptr, *key_runs = nullptr,
+             *key_off_next = nullptr, *g2p_keys = nullptr, *run_off = nullptr,
              *run_cursor = nullptr;
     // scan scratch
     uint32_t *scan_blocks = nullptr;

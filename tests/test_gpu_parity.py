@@ -115,8 +115,11 @@ def test_stage_bitexact(name, stage):
     orc.write_pressure(pres)
     gpu.step(DT, G, stop_after=stage)
     orc.step(DT, G, stop_after=stage)
-    if stage in (abi.STAGE_SORT, abi.STAGE_ALL):
+    if stage == abi.STAGE_SORT:
         assert_bits(gpu.read_particles(), orc.read_particles(), "particles")
+    if stage == abi.STAGE_ALL:
+        # the library leaves the records sorted by their new bricks, as the reference's next sort orders them
+        assert_bits(gpu.read_particles(), orc.read_particles_brick_order(), "particles")
     for f in STAGE_FIELDS.get(stage, []):
         assert_bits(gpu.read_grid(f), orc.read_grid(f), f"field {f}")
     rg, ro = gpu.results(), orc.results()
@@ -141,7 +144,7 @@ def test_multistep_bitexact(name):
         g = (0.3 * np.sin(k), -9.8, 0.1)
         gpu.step(DT, g)
         orc.step(DT, g)
-        assert_bits(gpu.read_particles(), orc.read_particles(), f"particles after step {k}")
+        assert_bits(gpu.read_particles(), orc.read_particles_brick_order(), f"particles after step {k}")
         rg, ro = gpu.results(), orc.results()
         assert rg.max_particle_speed == ro.max_particle_speed
         assert rg.clamped_particles == ro.clamped_particles
@@ -164,7 +167,7 @@ def test_large_sparse_grid_bitexact():
     for k in range(3):
         gpu.step(DT, G)
         orc.step(DT, G)
-        assert_bits(gpu.read_particles(), orc.read_particles(), f"particles after step {k}")
+        assert_bits(gpu.read_particles(), orc.read_particles_brick_order(), f"particles after step {k}")
         assert_bits(gpu.read_grid(abi.FIELD_BRICK_TABLE), orc.read_grid(abi.FIELD_BRICK_TABLE), "brick table")
         rg, ro = gpu.results(), orc.results()
         assert rg.active_bricks == ro.active_bricks and rg.fluid_cells == ro.fluid_cells
@@ -203,7 +206,7 @@ def test_mgpcg_parity(n, steps):
             assert orc.true_residual(pg) <= 1.2e-6
             vg, vo = gpu.read_grid(abi.FIELD_VEL), orc.read_grid(abi.FIELD_VEL)
             assert rel_linf(vg, vo) <= 1e-5
-            pgp, pop = gpu.read_particles(), orc.read_particles()
+            pgp, pop = gpu.read_particles(), orc.read_particles_brick_order()
             assert rel_linf(pgp["vel"], pop["vel"]) <= 1e-5
 
 
