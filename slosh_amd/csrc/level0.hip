@@ -83,27 +83,25 @@ __device__ __forceinline__ uint32_t parent1(uint32_t l)
     return ((l & 7u) >> 1) + 4u * (((l >> 3) & 7u) >> 1) + 16u * ((l >> 6) >> 1);
 }
 
-// Workgroup sum of each thread's values into partials[blockIdx.x * NV + k], as the tree a[t] += a[t + s] for
-// s = 128, 64, ..., 1: the two cross-wavefront stages through LDS, the in-wavefront ones (s < 64) as shuffles
-// down in wavefront 0 (the same additions in the same order).
+// Workgroup sum of each thread's values into partials[blockIdx.x * NV + k]: a shuffle tree within each wavefront
+// (s = 32, 16, ..., 1), then the four wavefront sums added in wavefront order by thread 0 (one barrier). The order
+// is fixed, so repeated runs give identical partials.
 template <int NV> __device__ void block_partial(const double (&v)[NV], double *partials)
 {
     static_assert(NV >= 1 && NV <= (int)kMaxBlockPartials, "partials buffer holds kMaxBlockPartials per brick");
-    static_assert(kThreads == 256u, "two cross-wavefront stages");
-    __shared__ double sh[NV][kThreads];
+    static_assert(kThreads == 256u, "four wavefronts");
+    __shared__ double sh[NV][kThreads / 64u];
     const uint32_t t = threadIdx.x;
-    for (int k = 0; k < NV; ++k) sh[k][t] = v[k];
-    __syncthreads();
-    if (t < 128u)
-        for (int k = 0; k < NV; ++k) sh[k][t] += sh[k][t + 128u];
-    __syncthreads();
-    if (t >= 64u) return;
     double a[NV];
-    for (int k = 0; k < NV; ++k) a[k] = sh[k][t] + sh[k][t + 64u];
+    for (int k = 0; k < NV; ++k) a[k] = v[k];
     for (int s = 32; s > 0; s >>= 1)
         for (int k = 0; k < NV; ++k) a[k] += __shfl_down(a[k], s);
+    if ((t & 63u) == 0u)
+        for (int k = 0; k < NV; ++k) sh[k][t >> 6] = a[k];
+    __syncthreads();
     if (t == 0)
-        for (int k = 0; k < NV; ++k) partials[(size_t)blockIdx.x * NV + k] = a[k];
+        for (int k = 0; k < NV; ++k)
+            partials[(size_t)blockIdx.x * NV + k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
 }
 
 // 1 for the allocated bricks holding fluid cells (their count from the P2G classification).
