@@ -321,8 +321,8 @@ def main():
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": measured_traffic(args.workload, dominant), "launches": launches,
                 "avg_us": avg_s * 1e6,
-                "timing": f"HIP events around each launch on the library's stream, {args.roofline_steps} steps after "
-                          "the timed ones"}
+                "timing": f"a HIP event pair carried by each launch's dispatch on the library's stream (the kernel's "
+                          f"own begin and end), {args.roofline_steps} steps after the timed ones"}
     if abytes and avg_s > 0:
         ach = abytes / avg_s / 1e9
         roofline.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),

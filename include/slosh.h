@@ -231,7 +231,9 @@ typedef struct slosh_kernel_time {
     double total_ms;    /* summed HIP-event time of those launches */
 } slosh_kernel_time;
 /* Selects the kernels to time with HIP events on the context's stream: bit i = kernel i of slosh_kernel_times'
- * order, ~0 = all, 0 = none (the default). */
+ * order, ~0 = all, 0 = none (the default). A kernel launched alone carries the event pair on its dispatch (its own
+ * begin and end); a multi-launch set-up stage is bracketed by events recorded around it. Timing turns the replay of
+ * solver chunks as HIP graphs off. */
 slosh_status slosh_set_kernel_timing(slosh_ctx *ctx, uint64_t mask);
 /* Writes up to cap entries (one per kernel, fixed order) and sets *n to the kernel count; out == NULL resets
  * the totals. */
