@@ -23,8 +23,8 @@ const char *const kKernelNames[K_COUNT] = {
     "k_mg_setup", "k_coarse_factor", "k_pcg_init", "k_red_dot", "k_restrict0", "k_prolong_black", "k_direction",
     "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d23down", "k_d23up",
     "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
-    "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io", "k_unet_train", "k_fcg",
-    "k_g2p_move"};
+    "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io",
+    "k_unet_train", "k_fcg", "k_g2p_move"};
 } // namespace slosh
 
 static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");
@@ -57,7 +57,8 @@ void free_all(slosh_ctx *c)
         for (int q = 0; q < 9; ++q) dfree(c->parts[k].c[q]);
     }
     dfree(c->packed); dfree(c->tile_nruns); dfree(c->run_key); dfree(c->run_len); dfree(c->run_base);
-    dfree(c->run_sorted); dfree(c->key_count); dfree(c->key_off); dfree(c->key_off_next); dfree(c->g2p_keys); dfree(c->key_runs); dfree(c->run_off);
+    dfree(c->run_sorted); dfree(c->key_count); dfree(c->key_off); dfree(c->key_off_next); dfree(c->g2p_keys);
+    dfree(c->key_runs); dfree(c->run_off);
     dfree(c->run_cursor); dfree(c->scan_blocks);
     dfree(c->table); dfree(c->slot_brick); dfree(c->active); dfree(c->need); dfree(c->need_rank);
     dfree(c->freeflag); dfree(c->free_rank); dfree(c->free_list); dfree(c->alloc_flag); dfree(c->alloc_rank);

@@ -5,7 +5,7 @@
 // Slosh.usf:856-892 (PressureRBGSCS), restated for the MI355X:
 //   - one workgroup per allocated brick, 256 threads, thread t owning the red/black pair of cells
 //     (colour_cell(t, 0), colour_cell(t, 1)); the brick's slot and its six face neighbours' slots come from a
-//     per-solve table of every allocated brick's 3x3x3 neighbourhood (scalar loads), so no pass walks the brick
+//     per-step table of every allocated brick's 3x3x3 neighbourhood (scalar loads), so no pass walks the brick
 //     table;
 //   - the stencil reads a per-cell mask of fluid neighbours (written by the divergence pass) instead of six
 //     neighbour cell types;

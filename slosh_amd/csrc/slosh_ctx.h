@@ -78,11 +78,11 @@ struct LVec {
     __device__ __forceinline__ Ref operator[](uint32_t i) const { return Ref{p, half, i}; }
 };
 
-// Words per allocated brick in the per-solve brick table: slots of the 3x3x3 neighbourhood (entry (dx+1) +
+// Words per allocated brick in the per-step brick table: slots of the 3x3x3 neighbourhood (entry (dx+1) +
 // 3(dy+1) + 9(dz+1), 0 = none), then the brick index.
 constexpr uint32_t kBinfoWords = 32u;
-// Values a level-0 reduction writes per workgroup (k_pcg_init: ||b||^2, ||r||^2, fluid cells); the partials
-// buffer holds this many per pool slot, one workgroup per allocated brick.
+// Most values a level-0 reduction writes per workgroup (k_direction: d.q, r.q, q.q, r.r; k_pcg_init three); the
+// partials buffer holds this many per pool slot, one workgroup per allocated brick.
 constexpr uint32_t kMaxBlockPartials = 4u;
 
 // Brick handled by workgroup w of a one-brick-per-workgroup pass over n bricks. Workgroups are dispatched
