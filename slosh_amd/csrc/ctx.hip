@@ -76,7 +76,7 @@ void free_all(slosh_ctx *c)
         dfree(c->lv[l].x); dfree(c->lv[l].x1); dfree(c->lv[l].b);
     }
     dfree(c->dense_info); dfree(c->dense_coord);
-    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->partials);
+    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->coarse_sig); dfree(c->partials);
     dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
     dfree(c->bp_clamp);
     dfree(c->state); dfree(c->stats); dfree(c->d_col);
@@ -281,6 +281,7 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->coarse_idx, kCoarsestCells);
     dalloc(&c->coarse_n, 1);
     dalloc(&c->coarse_inv, (size_t)kCoarsestCells * kCoarsestCells);
+    dalloc(&c->coarse_sig, (size_t)kCoarsestCells + 1u);
     dalloc(&c->partials, (size_t)kMaxBlockPartials * ((size_t)g.npool + 1u));
     dalloc(&c->bp_fluid, g.npool);
     dalloc(&c->bp_div_max, g.npool);
@@ -375,6 +376,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     if (stop > SLOSH_STAGE_POOL) launch_p2g(c, warm);
     if (stop > SLOSH_STAGE_P2G) launch_normalize(c, dt, gv);
     if (stop > SLOSH_STAGE_NORMALIZE) launch_divergence(c, dt);
+    bool solved = false;
     if (stop > SLOSH_STAGE_DIVERGENCE) {
         SLOSH_HIP(hipEventRecord(c->ev_solve0, c->stream));
         c->fcg_iters = 0;
@@ -384,25 +386,10 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
         else if (c->s.pressure_solver == SLOSH_SOLVER_FCG_UNET) run_fcg(c);
         SLOSH_HIP(hipEventRecord(c->ev_solve1, c->stream));
         c->res.fallback_used = c->fcg_fallback;
-        if (c->s.pressure_solver != SLOSH_SOLVER_NONE) {
+        // the solver's final state comes back with the step's statistics (no host wait between solve and G2P)
+        solved = c->s.pressure_solver != SLOSH_SOLVER_NONE;
+        if (solved)
             SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
-            SLOSH_HIP(hipStreamSynchronize(c->stream));
-            harvest_timing(c);
-            if (c->nactive) {
-                // flexible CG: its own iterations, plus those of the MGPCG fallback when it ran
-                const bool fcg = c->s.pressure_solver == SLOSH_SOLVER_FCG_UNET && c->fcg_fallback != 2;
-                c->res.solver_iterations = fcg ? (c->fcg_fallback ? c->fcg_iters + c->h_state->iters : c->fcg_iters)
-                                               : c->h_state->iters;
-                c->res.residual_rel = c->h_state->rel;
-                c->res.true_residual_rel = c->s.pressure_solver == SLOSH_SOLVER_RBSOR ? (float)c->h_state->rel
-                                                                                      : (float)c->h_state->rel_true;
-                c->res.rhs_norm = sqrt(c->h_state->bb);
-                c->res.converged = c->h_state->converged;
-            }
-            float ms = 0.0f;
-            SLOSH_HIP(hipEventElapsedTime(&ms, c->ev_solve0, c->ev_solve1));
-            c->res.solve_ms = ms;
-        }
     }
     if (stop > SLOSH_STAGE_SOLVE) launch_project(c);
     if (stop > SLOSH_STAGE_PROJECT)
@@ -411,6 +398,22 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     launch_step_stats(c);
     SLOSH_HIP(hipEventRecord(c->ev_step1, c->stream));
     read_stats(c);
+    if (solved) {
+        if (c->nactive) {
+            // flexible CG: its own iterations, plus those of the MGPCG fallback when it ran
+            const bool fcg = c->s.pressure_solver == SLOSH_SOLVER_FCG_UNET && c->fcg_fallback != 2;
+            c->res.solver_iterations = fcg ? (c->fcg_fallback ? c->fcg_iters + c->h_state->iters : c->fcg_iters)
+                                           : c->h_state->iters;
+            c->res.residual_rel = c->h_state->rel;
+            c->res.true_residual_rel = c->s.pressure_solver == SLOSH_SOLVER_RBSOR ? (float)c->h_state->rel
+                                                                                  : (float)c->h_state->rel_true;
+            c->res.rhs_norm = sqrt(c->h_state->bb);
+            c->res.converged = c->h_state->converged;
+        }
+        float ms = 0.0f;
+        SLOSH_HIP(hipEventElapsedTime(&ms, c->ev_solve0, c->ev_solve1));
+        c->res.solve_ms = ms;
+    }
     const StepStats &st = *c->h_stats;
     c->res.fluid_cells = st.fluid_cells;
     float f;

@@ -259,6 +259,7 @@ struct slosh_ctx {
     uint32_t *coarse_idx = nullptr; // [kCoarsestCells]
     uint32_t *coarse_n = nullptr;
     double *coarse_inv = nullptr;   // [kCoarsestCells^2]
+    uint32_t *coarse_sig = nullptr; // the coarsest level's cell types and indices at its last factorisation
     double *partials = nullptr;
     uint16_t *dense_info = nullptr;   // stencil words of all dense levels, level by level
     uint32_t *dense_coord = nullptr;  // their packed cell coordinates

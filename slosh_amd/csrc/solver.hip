@@ -892,7 +892,7 @@ __global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t
 // The coarsest level is level 0 (mode 0, grids of <= 64 cells), a pooled level (mode 1) or a dense level (mode 2).
 __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ctype, const float *diag, int mode,
                                                        PLevel P, DenseLevel D, uint32_t *cidx, uint32_t *cn,
-                                                       double *inv)
+                                                       double *inv, uint32_t *sig)
 {
     constexpr int N = (int)kCoarsestCells;
     __shared__ double a[N][2 * N + 1]; // [A | I], becoming [I | A^-1]
@@ -919,6 +919,27 @@ __global__ __launch_bounds__(256) void k_coarse_factor(Geo g, const uint32_t *ct
     // the fluid cells in increasing index order (the level has at most N = 64 cells, which slosh_create checks: one
     // wavefront, compacted by a ballot)
     static_assert(N == 64, "one wavefront scans the coarsest level");
+    // A coarse level's operator, fluid list and inverse follow from its cells' types and pool indices alone: when
+    // they equal those of the last factorisation (sig[0..63], sig[64] = 1 once one was made), the inverse stands.
+    // A level-0 coarsest level (grids of at most 64 cells) also reads the diagonal and is always refactored.
+    __shared__ uint32_t s_same;
+    if (mode != 0 && t < N) {
+        uint32_t v = 0xFFFFFFFFu;
+        if ((uint32_t)t < cells) {
+            const int x = t % dims[0], y = (t / dims[0]) % dims[1], z = t / (dims[0] * dims[1]);
+            v = type_of(x, y, z) | (index_of(x, y, z) << 2);
+        }
+        const bool differ = v != sig[t];
+        const uint64_t dm = __ballot(differ);
+        if (differ) sig[t] = v;
+        if (t == 0) {
+            s_same = (dm == 0ull && sig[N] == 1u) ? 1u : 0u;
+            sig[N] = 1u;
+        }
+    }
+    if (mode == 0 && t == 0) s_same = 0u;
+    __syncthreads();
+    if (s_same) return;
     if (t < N) {
         bool f = false;
         int x = 0, y = 0, z = 0;
@@ -1153,7 +1174,7 @@ void mg_setup(slosh_ctx *c)
     PLevel P = mode == 1 ? plevel_of(c, m - 1) : PLevel{};
     DenseLevel D = mode == 2 ? dlevel_of(c, m - 1) : DenseLevel{};
     launch_timed(c, K_COARSE_FACTOR, k_coarse_factor, dim3(1), dim3(256), 0, c->geo, c->ctype, c->diag, mode, P, D,
-                 c->coarse_idx, c->coarse_n, c->coarse_inv);
+                 c->coarse_idx, c->coarse_n, c->coarse_inv, c->coarse_sig);
 }
 
 // Captures the launches of `body` on the context's stream into c->mg_graph (updating the executable graph in
