@@ -108,6 +108,28 @@ def test_sort_is_stable_by_brick_and_drops_dead():
     assert np.array_equal(out.view(np.uint8), live[order].view(np.uint8))
 
 
+def test_sort_by_brick_helper_matches_the_sort():
+    """oracle_py.sort_by_brick (how the GPU tests order the oracle's records after a full step) gives exactly the
+    order of the oracle's sort: a 20 x 28 x 12 grid (partial bricks), dead records, positions clamped at the walls."""
+    grid = (20, 28, 12)
+    rng = np.random.default_rng(8)
+    h = 1.0 / 28
+    parts = np.zeros(6000, dtype=abi.PARTICLE_DTYPE)
+    for a in range(3):
+        parts["pos"][:, a] = rng.uniform(-0.5 * h, (grid[a] + 0.5) * h, size=6000).astype(np.float32)
+    parts["vel"] = rng.uniform(-1, 1, size=(6000, 3)).astype(np.float32)
+    parts["age"] = rng.uniform(0.0, 1.0, size=6000).astype(np.float32)
+    parts["age"][rng.random(6000) < 0.1] = -1.0
+    s = abi.make_settings(grid, h, 6000, solver=abi.SOLVER_NONE)
+    o = O.OracleSim(s)
+    o.write_particles(parts)
+    o.step(DT, G, stop_after=abi.STAGE_SORT)
+    out = o.read_particles()
+    ref = O.sort_by_brick(parts, grid, h)
+    assert len(ref) == int((parts["age"] >= 0).sum())
+    assert np.array_equal(out.view(np.uint8), ref.view(np.uint8))
+
+
 def test_brick_pool_is_dilated_touch_set():
     from scipy.ndimage import binary_dilation
 
