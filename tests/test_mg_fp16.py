@@ -9,8 +9,11 @@ from slosh_amd import abi, scenes
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("kind,n", [("dam", 32), ("dam", 64), ("slosh", 64)])
-def test_fp16_levels_meet_tolerance_and_fp64_error(kind, n):
+@pytest.mark.parametrize("kind,n,tuned", [("dam", 32, False), ("dam", 64, False), ("slosh", 64, False),
+                                          ("dam", 64, True), ("slosh", 64, True)])
+def test_fp16_levels_meet_tolerance_and_fp64_error(kind, n, tuned):
+    """tuned: the preconditioner options the 512^3 workload runs with FP16 levels (fluid-first coarse types, 3
+    red-black pairs on levels 4 and coarser), for both the FP32 and the FP16 run."""
     from slosh_amd import Sim
 
     if kind == "dam":
@@ -21,8 +24,10 @@ def test_fp16_levels_meet_tolerance_and_fp64_error(kind, n):
         dt, grav = 1.0 / 240.0, scenes.tank_gravity
     cap = sum(int(np.prod(r.cell_dims)) * 8 for r in reqs) + 1024
     runs = {}
+    opt_flags, sweeps = (abi.FLAG_MG_FLUID_COARSENING, 3) if tuned else (0, 0)
     for flags in (0, abi.FLAG_MG_FP16):
-        sim = Sim(abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6, flags=flags))
+        sim = Sim(abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6, flags=flags | opt_flags,
+                                    mg_coarse_sweeps=sweeps))
         for r in reqs:
             sim.emit(r)
         sim.step(dt, grav(0.0))
@@ -39,7 +44,7 @@ def test_fp16_levels_meet_tolerance_and_fp64_error(kind, n):
     assert r16.converged == 1 and r16.residual_rel <= 1e-6, r16.as_dict()
     err16 = np.linalg.norm(p16 - ref) / np.linalg.norm(ref)
     err32 = np.linalg.norm(p32 - ref) / np.linalg.norm(ref)
-    print(f"{kind} {n}^3: FP16 levels {r16.solver_iterations} iterations, error vs FP64 {err16:.2e}; "
+    print(f"{kind} {n}^3 (tuned {tuned}): FP16 levels {r16.solver_iterations} iterations, error vs FP64 {err16:.2e}; "
           f"FP32 levels {r32.solver_iterations} iterations, error {err32:.2e}")
     assert err16 <= 1e-5, err16
     assert r16.solver_iterations <= r32.solver_iterations + 4
