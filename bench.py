@@ -177,6 +177,7 @@ def bench_other(args):
                             "max_rel_residual": max(p[1] for p in per),
                             "max_true_rel_residual_fp64": max(p[2] for p in per),
                             "converged_steps": sum(p[3] for p in per),
+                            "warm_start": "reference",
                             "mg_preconditioner": {"fluid_first_coarse_types": True,
                                                   "coarse_sweeps": max(args.mg_coarse_sweeps, 1)},
                             "particles_per_s": round(args.steps / el * res.live_particles, 1)})
@@ -185,7 +186,8 @@ def bench_other(args):
     for grid, h, reqs, cols in specs:
         n = sum(int(r.cell_dims[0]) * int(r.cell_dims[1]) * int(r.cell_dims[2]) * 8 for r in reqs) + 1024
         sim = Sim(abi.make_settings(grid, h, n, solver=abi.SOLVER_MGPCG, tolerance=1e-6,
-                                    flags=abi.FLAG_MG_FLUID_COARSENING, mg_coarse_sweeps=args.mg_coarse_sweeps))
+                                    flags=abi.FLAG_MG_FLUID_COARSENING | abi.FLAG_WARM_START_EXTRAPOLATE,
+                                    mg_coarse_sweeps=args.mg_coarse_sweeps))
         sim.set_colliders(cols)
         for r in reqs:
             sim.emit(r)
@@ -212,6 +214,7 @@ def bench_other(args):
                 value=round(args.steps / el, 3), ms_per_step=round(1e3 * el / args.steps, 3),
                 config={"workload": WORKLOADS[args.workload], "tanks": len(specs), "particles": parts,
                         "autotuned": chosen, "mean_solver_iterations": round(its / (len(specs) * args.steps), 2),
+                        "warm_start": "extrapolated",
                         "mg_preconditioner": {"fluid_first_coarse_types": True,
                                               "coarse_sweeps": max(args.mg_coarse_sweeps, 1)},
                         "converged_tank_steps": conv, "tank_steps": len(specs) * args.steps,
@@ -229,8 +232,8 @@ def main():
     ap.add_argument("--kernel-report", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-FCG comparison run")
     ap.add_argument("--mg-reference-cycle", action="store_true",
-                    help="time the reference's V-cycle (air-first coarse types, one red-black pair per level) for "
-                         "`value` instead of the tuned preconditioner")
+                    help="time the reference's V-cycle (air-first coarse types, one red-black pair per level) and "
+                         "warm start for `value` instead of the tuned options")
     ap.add_argument("--mg-coarse-sweeps", type=int, default=3,
                     help="tuned preconditioner: red-black pairs per visit of levels 4 and coarser")
     ap.add_argument("--no-reference-cycle-run", action="store_true",
@@ -258,9 +261,11 @@ def main():
     grid, h, reqs, dt, grav = build_scene(args.workload)
     n = sum(int(r.cell_dims[0]) * int(r.cell_dims[1]) * int(r.cell_dims[2]) * 8 for r in reqs)
 
-    # MGPCG preconditioner: tuned (fluid-first coarse cell types, extra red-black pairs on levels 4 and coarser) or
-    # the reference's cycle; both solve to the same tolerance
-    tuned = {"flags": abi.FLAG_MG_FLUID_COARSENING, "mg_coarse_sweeps": args.mg_coarse_sweeps}
+    # MGPCG options: tuned (fluid-first coarse cell types, extra red-black pairs on levels 4 and coarser, the
+    # warm start extrapolated from the last two solutions) or the reference's cycle and warm start; both solve to the
+    # same tolerance
+    tuned = {"flags": abi.FLAG_MG_FLUID_COARSENING | abi.FLAG_WARM_START_EXTRAPOLATE,
+             "mg_coarse_sweeps": args.mg_coarse_sweeps}
     reference_cycle = {"flags": 0, "mg_coarse_sweeps": 0}
     mg = reference_cycle if args.mg_reference_cycle else tuned
 
@@ -359,8 +364,10 @@ def main():
     if args.solver == "mgpcg" and not args.mg_reference_cycle and not args.no_reference_cycle_run:
         rsps, rel_, rper, _, _, _, rsim = run(abi.SOLVER_MGPCG, None, args.steps, args.warmup, reference_cycle)
         rsim.close()
-        ref_cycle = {"steps_per_s": round(rsps, 3), "mean_solver_iterations": round(sum(p[0] for p in rper) / len(rper), 2),
-                     "max_rel_residual": max(p[1] for p in rper), "converged_steps": sum(p[2] for p in rper)}
+        ref_cycle = {"steps_per_s": round(rsps, 3),
+                     "mean_solver_iterations": round(sum(p[0] for p in rper) / len(rper), 2),
+                     "max_rel_residual": max(p[1] for p in rper), "converged_steps": sum(p[2] for p in rper),
+                     "warm_start": "reference"}
     line = {
         "metric": "steps/s @256^3,16M particles; learned-FCG vs plain MGPCG at 1e-6 residual",
         "value": round(sps, 3),
@@ -379,8 +386,9 @@ def main():
                    "fluid_cells": nf, "allocated_bricks": nb, "fluid_bricks": ns // 512,
                    "mean_solver_iterations": round(sum(iters) / len(iters), 2), "max_rel_residual": max(rels),
                    "converged_steps": conv, "particles_per_s": round(sps * res.live_particles, 1),
-                   "mg_preconditioner": {"fluid_first_coarse_types": mg["flags"] == abi.FLAG_MG_FLUID_COARSENING,
+                   "mg_preconditioner": {"fluid_first_coarse_types": bool(mg["flags"] & abi.FLAG_MG_FLUID_COARSENING),
                                          "coarse_sweeps": max(mg["mg_coarse_sweeps"], 1)},
+                   "warm_start": "extrapolated" if mg["flags"] & abi.FLAG_WARM_START_EXTRAPOLATE else "reference",
                    "mg_reference_cycle": ref_cycle,
                    "learned_fcg": learned},
         "roofline": roofline,

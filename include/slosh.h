@@ -82,6 +82,11 @@ typedef struct slosh_settings {
                                              SloshPressureMG.usf:256-278, makes it air when any child is air); the
                                              preconditioner keeps more of the free-surface band, the solution and
                                              its tolerance are unchanged */
+#define SLOSH_FLAG_WARM_START_EXTRAPOLATE 8u /* pressure solvers: start from the linear extrapolation in time of the
+                                                last two solutions (per unit dt) on cells that were fluid in both,
+                                                instead of the last solution scaled by dt/dt_prev (the reference's
+                                                warm start, SloshProxy.cpp:575 applied in Slosh.usf:569-598, kept
+                                                on every other cell); the solve still stops at the tolerance */
 
 /* Emission request: FSloshEmitRequest (SloshTypes.h:169-181). shape 0 = FillCells (8 jittered particles per cell). */
 enum { SLOSH_EMIT_FILL_CELLS = 0, SLOSH_EMIT_SPHERE = 1, SLOSH_EMIT_BOX = 2 };

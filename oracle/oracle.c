@@ -76,6 +76,8 @@ struct osim {
     uint32_t *ctype;
     int32_t *ccount;
     double *pres; /* FP64: the pressure solution is the CG alpha-update accumulator */
+    double *pres_prev; /* SLOSH_FLAG_WARM_START_EXTRAPOLATE: last solution per unit dt (NaN: none) */
+    uint8_t *had;      /* scratch of stage_p2g: the cell was fluid in the last step */
     float *div, *diag;
     /* solver */
     int nlevels;
@@ -239,7 +241,7 @@ void osim_destroy(osim *o)
     free(o->accum);
     for (int k = 0; k < 3; ++k) free(o->vel[k]);
     free(o->fflags); free(o->fweight); free(o->ctype); free(o->ccount);
-    free(o->pres); free(o->div); free(o->diag);
+    free(o->pres); free(o->pres_prev); free(o->had); free(o->div); free(o->diag);
     free(o->r); free(o->z); free(o->d); free(o->q);
     free(o->rd); free(o->zd); free(o->dd); free(o->qd); free(o->pd);
     free_levels(o);
@@ -286,6 +288,10 @@ osim *osim_create(const slosh_settings *s)
     o->ctype = (uint32_t *)xcalloc(pc, 4);
     o->ccount = (int32_t *)xcalloc(pc, 4);
     o->pres = (double *)xcalloc(pc, 8);
+    if (o->s.flags & SLOSH_FLAG_WARM_START_EXTRAPOLATE) {
+        o->pres_prev = (double *)xcalloc(pc, 8);
+        o->had = (uint8_t *)xcalloc(pc, 1);
+    }
     o->div = (float *)xcalloc(pc, 4);
     o->diag = (float *)xcalloc(pc, 4);
     o->r = (float *)xcalloc(pc, 4);
@@ -463,6 +469,7 @@ static void stage_pool(osim *o)
         for (uint32_t l = 0; l < BRICK_CELLS; ++l) {
             o->pres[base + l] = 0.0;
             o->ccount[base + l] = 0;
+            o->ctype[base + l] = CELL_AIR; /* a new slot has no fluid history (read by the warm start) */
         }
     }
     o->nactive = 0;
@@ -518,11 +525,15 @@ static void p2g_component(osim *o, int axis, const float xp[3], float up, const 
     }
 }
 
-/* Slosh.usf:365-395 (BuildSolidCS) + 493-563 (P2GCS) + 569-598 (MarkCellsCS). */
-static void stage_p2g(osim *o, float warm_scale)
+/* Slosh.usf:365-395 (BuildSolidCS) + 493-563 (P2GCS) + 569-598 (MarkCellsCS). The warm start of fluid cells is
+ * the last solution times warm_scale = dt/dt_prev (SloshProxy.cpp:575); with SLOSH_FLAG_WARM_START_EXTRAPOLATE (an
+ * option of this build) a cell fluid in the last two solves starts from P = dt (q1 + (q1 - q2) dt/dt_prev), the
+ * linear extrapolation of the pressure per unit dt (q1 = P_last/dt_prev, q2 the value before). */
+static void stage_p2g(osim *o, float warm_scale, float dt, float dt_prev)
 {
     FOR_ACTIVE_CELLS(o, idx, x, y, z)
     {
+        if (o->had) o->had[idx] = (o->ctype[idx] & CELL_TYPE_MASK) == CELL_FLUID;
         uint32_t packed = CELL_AIR;
         if (x == 0 || y == 0 || z == 0 || x == o->n[0] - 1 || y == o->n[1] - 1 || z == o->n[2] - 1) {
             packed = CELL_SOLID;
@@ -567,10 +578,22 @@ static void stage_p2g(osim *o, float warm_scale)
             o->ctype[idx] = packed;
         }
         if ((packed & CELL_TYPE_MASK) == CELL_FLUID) {
-            o->pres[idx] *= (double)warm_scale;
+            const double old = o->pres[idx];
+            double p0 = old * (double)warm_scale;
+            if (o->pres_prev) {
+                double q1 = NAN;
+                if (o->had[idx] && dt_prev > 0.0f) {
+                    q1 = old / (double)dt_prev;
+                    const double q2 = o->pres_prev[idx];
+                    if (!isnan(q2)) p0 = (double)dt * (q1 + (q1 - q2) * ((double)dt / (double)dt_prev));
+                }
+                o->pres_prev[idx] = q1;
+            }
+            o->pres[idx] = p0;
             fluid++;
         } else {
             o->pres[idx] = 0.0;
+            if (o->pres_prev) o->pres_prev[idx] = NAN;
         }
     }
     o->res.fluid_cells = fluid;
@@ -1268,13 +1291,14 @@ int osim_step_until(osim *o, float dt, const float g[3], int stop_after)
     o->frame++;
     o->res.sim_frame = o->frame;
     float warm = o->prev_dt > 0.0f ? dt / o->prev_dt : 1.0f;
+    const float dt_prev = o->prev_dt;
     o->prev_dt = dt;
     stage_sort(o);
     o->res.live_particles = o->np;
     if (stop_after <= SLOSH_STAGE_SORT) return SLOSH_OK;
     stage_pool(o);
     if (stop_after <= SLOSH_STAGE_POOL) return SLOSH_OK;
-    stage_p2g(o, warm);
+    stage_p2g(o, warm, dt, dt_prev);
     if (stop_after <= SLOSH_STAGE_P2G) return SLOSH_OK;
     stage_normalize(o, dt, g);
     if (stop_after <= SLOSH_STAGE_NORMALIZE) return SLOSH_OK;

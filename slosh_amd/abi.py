@@ -24,6 +24,7 @@ SOLVER_FCG_UNET = 3
 FLAG_CAPTURE_RESIDUALS = 1
 FLAG_MG_FP16 = 2
 FLAG_MG_FLUID_COARSENING = 4
+FLAG_WARM_START_EXTRAPOLATE = 8
 
 EMIT_FILL_CELLS = 0
 EMIT_SPHERE = 1

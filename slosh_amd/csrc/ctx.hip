@@ -76,7 +76,7 @@ void free_all(slosh_ctx *c)
         dfree(c->lv[l].x); dfree(c->lv[l].x1); dfree(c->lv[l].b);
     }
     dfree(c->dense_info); dfree(c->dense_coord);
-    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->coarse_sig); dfree(c->partials);
+    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->coarse_sig); dfree(c->pres_prev); dfree(c->partials);
     dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
     dfree(c->bp_clamp);
     dfree(c->state); dfree(c->stats); dfree(c->d_col);
@@ -112,7 +112,8 @@ bool settings_valid(const slosh_settings *s)
     if (s->pressure_solver == SLOSH_SOLVER_FCG_UNET)
         for (int a = 0; a < 3; ++a)
             if (s->grid_dim[a] % 8 != 0) return false; // the network's levels need whole bricks
-    if (s->flags & ~(uint32_t)(SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_MG_FP16 | SLOSH_FLAG_MG_FLUID_COARSENING))
+    if (s->flags & ~(uint32_t)(SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_MG_FP16 | SLOSH_FLAG_MG_FLUID_COARSENING |
+                               SLOSH_FLAG_WARM_START_EXTRAPOLATE))
         return false;
     if (!(s->tolerance > 0.0f) || s->max_iterations < 0) return false;
     if (!(s->sor_weight >= 0.5f && s->sor_weight <= 1.95f)) return false;
@@ -200,6 +201,7 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->fweight, 3 * pc);
     dalloc(&c->ctype, pc);
     dalloc(&c->pres, pc);
+    if (s.flags & SLOSH_FLAG_WARM_START_EXTRAPOLATE) dalloc(&c->pres_prev, pc);
     dalloc(&c->div, pc);
     dalloc(&c->diag, pc);
     dalloc(&c->fmask, pc);
@@ -349,6 +351,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
 {
     c->frame++;
     const float warm = c->prev_dt > 0.0f ? dt / c->prev_dt : 1.0f;
+    const float dt_prev = c->prev_dt;
     c->prev_dt = dt;
     c->res.sim_frame = c->frame;
     c->res.solver_iterations = 0;
@@ -373,7 +376,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
         c->res.pool_overflow_bricks = c->h_stats->overflow;
         build_brick_rows(c);
     }
-    if (stop > SLOSH_STAGE_POOL) launch_p2g(c, warm);
+    if (stop > SLOSH_STAGE_POOL) launch_p2g(c, warm, dt, dt_prev);
     if (stop > SLOSH_STAGE_P2G) launch_normalize(c, dt, gv);
     if (stop > SLOSH_STAGE_NORMALIZE) launch_divergence(c, dt);
     bool solved = false;

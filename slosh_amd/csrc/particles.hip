@@ -294,6 +294,16 @@ __global__ __launch_bounds__(256) void k_sort_scatter(Parts src, Parts dst, uint
 }
 
 // ------------------------------------------------------------------------------------------------------------
+// Warm start of the pressure solve on fluid cells: the last solution times WarmStartScale = dt/dt_prev
+// (SloshProxy.cpp:575, applied in MarkCellsCS, Slosh.usf:569-598). With `prev` set (SLOSH_FLAG_WARM_START_EXTRAPOLATE)
+// a cell that was fluid in the last two solves starts from the linear extrapolation in time of its pressure per unit
+// dt, P = dt (q1 + (q1 - q2) dt / dt_prev) with q1 = P_last / dt_prev and q2 the value before; any other fluid cell
+// keeps the reference's start. `prev` holds q1 for the next step (NaN where there is no history).
+struct WarmStart {
+    double *prev;
+    double dt, dt_prev;
+};
+
 // P2G (Slosh.usf:448-563) as a per-brick gather: one workgroup per allocated brick reads the brick's sorted
 // particles, accumulates the int32 fixed-point momentum and weight of every node they touch into an LDS tile
 // (nodes [brick_min - 1, brick_min + 9) per axis), and stores the tile; the normalise pass sums the up to 8 tiles
@@ -301,7 +311,8 @@ __global__ __launch_bounds__(256) void k_sort_scatter(Parts src, Parts dst, uint
 // workgroup counts particles per cell and classifies its 512 cells (BuildSolidCS :365-395, MarkCellsCS :569-598).
 __global__ __launch_bounds__(256) void k_p2g(Geo g, Parts p, const uint32_t *key_off, int32_t *tiles,
                                              uint8_t *tile_valid, uint32_t *ctype, double *pres, uint8_t *new_slot,
-                                             float warm, const slosh_collider *col, uint32_t ncol, uint32_t *bp_fluid)
+                                             float warm, WarmStart ws, const slosh_collider *col, uint32_t ncol,
+                                             uint32_t *bp_fluid)
 {
     __shared__ int32_t tile[kTileInts];
     __shared__ int32_t cnt[kBrickCells];
@@ -428,12 +439,25 @@ __global__ __launch_bounds__(256) void k_p2g(Geo g, Parts p, const uint32_t *key
             if ((float)count >= 0.5f * kTargetPpc) packed |= kCellDense;
         }
         uint32_t idx = slot * kBrickCells + l;
+        const uint32_t old_type = ws.prev ? ctype[idx] : 0u; // last step's type (a new slot has no history)
         ctype[idx] = packed;
         if ((packed & kCellTypeMask) == kCellFluid) {
-            pres[idx] = (fresh ? 0.0 : pres[idx]) * (double)warm;
+            const double old = fresh ? 0.0 : pres[idx];
+            double p0 = old * (double)warm;
+            if (ws.prev) {
+                double q1 = __longlong_as_double(0x7ff8000000000000ll);
+                if (!fresh && (old_type & kCellTypeMask) == kCellFluid && ws.dt_prev > 0.0) {
+                    q1 = old / ws.dt_prev;
+                    const double q2 = ws.prev[idx];
+                    if (!isnan(q2)) p0 = ws.dt * (q1 + (q1 - q2) * (ws.dt / ws.dt_prev));
+                }
+                ws.prev[idx] = q1;
+            }
+            pres[idx] = p0;
             fluid++;
         } else {
             pres[idx] = 0.0;
+            if (ws.prev) ws.prev[idx] = __longlong_as_double(0x7ff8000000000000ll);
         }
     }
     if (fluid) atomicAdd(&fluid_sum, fluid);
@@ -918,11 +942,12 @@ void run_sort(slosh_ctx *c)
     c->sorted = true;
 }
 
-void launch_p2g(slosh_ctx *c, float warm_scale)
+void launch_p2g(slosh_ctx *c, float warm_scale, float dt, float dt_prev)
 {
+    const WarmStart ws{c->pres_prev, (double)dt, (double)dt_prev};
     if (!c->nactive) return;
     launch_timed(c, K_P2G, k_p2g, dim3(c->nactive), dim3(256), 0, c->geo, c->parts[c->cur], c->key_off, c->tiles,
-                 c->tile_valid, c->ctype, c->pres, c->new_slot, warm_scale, c->d_col, c->ncol, c->bp_fluid);
+                 c->tile_valid, c->ctype, c->pres, c->new_slot, warm_scale, ws, c->d_col, c->ncol, c->bp_fluid);
     c->n_p2g_blocks = c->nactive;
     SLOSH_HIP(hipGetLastError());
 }

@@ -259,6 +259,7 @@ struct slosh_ctx {
     uint32_t *coarse_idx = nullptr; // [kCoarsestCells]
     uint32_t *coarse_n = nullptr;
     double *coarse_inv = nullptr;   // [kCoarsestCells^2]
+    double *pres_prev = nullptr;    // SLOSH_FLAG_WARM_START_EXTRAPOLATE: last solution per unit dt (NaN: none)
     uint32_t *coarse_sig = nullptr; // the coarsest level's cell types and indices at its last factorisation
     double *partials = nullptr;
     uint16_t *dense_info = nullptr;   // stencil words of all dense levels, level by level
@@ -316,7 +317,7 @@ void launch_emit_fill(slosh_ctx *c, const slosh_emit_request *r, uint32_t count)
 void launch_unpack_particles(slosh_ctx *c, const slosh_particle *d_src, uint32_t n);
 void launch_pack_particles(slosh_ctx *c, void *d_dst, int layout);
 void run_sort(slosh_ctx *c);
-void launch_p2g(slosh_ctx *c, float warm_scale);
+void launch_p2g(slosh_ctx *c, float warm_scale, float dt, float dt_prev);
 void launch_g2p(slosh_ctx *c, float dt, const float g[3]);
 
 // ---- grid.hip ----

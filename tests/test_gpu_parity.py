@@ -266,6 +266,49 @@ def test_mgpcg_grid_shapes(grid):
     orc.close()
 
 
+def test_warm_start_extrapolation_guess_gpu():
+    """SLOSH_FLAG_WARM_START_EXTRAPOLATE in the library: the solver's starting guess, bit for bit, from the two
+    previous solutions the library returned (tests/warm_start_check.py)."""
+    import warm_start_check as W
+
+    s, reqs = W.make_settings()
+    sim = Sim(s)
+    print("case warm start guess", flush=True)
+    W.run_and_check(sim, reqs)
+    sim.close()
+
+
+def test_mgpcg_extrapolated_warm_start():
+    """MGPCG with the extrapolated warm start and the tuned preconditioner against the oracle with the same options
+    on a 64^3 sloshing tank over six steps: every solve to 1e-6, the oracle's iteration count within 2, the last FP64
+    solution within 1e-5 and the true residual."""
+    grid, h, reqs = scenes.sloshing_tank(64, seed=1)
+    cap = sum(int(np.prod(r.cell_dims)) * 8 for r in reqs) + 1024
+    s = abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6,
+                          flags=abi.FLAG_WARM_START_EXTRAPOLATE | abi.FLAG_MG_FLUID_COARSENING, mg_coarse_sweeps=3)
+    print("case extrapolated warm start 64^3", flush=True)
+    gpu, orc = Sim(s), O.OracleSim(s)
+    for sim in (gpu, orc):
+        for r in reqs:
+            sim.emit(r)
+    t = 0.0
+    for k in range(6):
+        g = scenes.tank_gravity(t)
+        gpu.step(1.0 / 240.0, g)
+        orc.step(1.0 / 240.0, g)
+        t += 1.0 / 240.0
+        rg, ro = gpu.results(), orc.results()
+        assert rg.converged == 1 and rg.residual_rel <= 1e-6, rg.as_dict()
+        assert abs(rg.solver_iterations - ro.solver_iterations) <= 2, (k, rg.solver_iterations, ro.solver_iterations)
+    pg = gpu.read_grid(abi.FIELD_PRESSURE_F64)
+    ref, _, rel = orc.solve_fp64(1e-13, 600)
+    assert rel <= 1e-12
+    assert np.linalg.norm(pg - ref) <= 1e-5 * np.linalg.norm(ref)
+    assert orc.true_residual(pg) <= 1.2e-6
+    gpu.close()
+    orc.close()
+
+
 def test_rbsor_parity():
     """RB-SOR (omega 1.8) on the same operator reaches the same pressure as the FP64 solve."""
     grid, h, reqs = scenes.dam_break(24, seed=2)

@@ -545,3 +545,36 @@ def test_wall_clamp_removes_inward_velocity():
     hit = out["pos"][:, 0] == lo
     assert hit.sum() > 100
     assert np.all(out["vel"][hit, 0] == 0.0)  # inward normal velocity removed (max(v, 0))
+
+def test_warm_start_extrapolation_guess():
+    """SLOSH_FLAG_WARM_START_EXTRAPOLATE in the oracle: the solver's starting guess, bit for bit, from the two
+    previous solutions (tests/warm_start_check.py)."""
+    import warm_start_check as W
+
+    s, reqs = W.make_settings()
+    o = O.OracleSim(s)
+    W.run_and_check(o, reqs)
+
+
+def test_warm_start_extrapolation_saves_iterations():
+    """On a smoothly sloshing tank the extrapolated start takes fewer MGPCG iterations than the reference's scaled
+    last solution (the solves meet the same tolerance)."""
+    from slosh_amd import scenes
+
+    grid, h, reqs = scenes.sloshing_tank(48, seed=1)
+    cap = sum(int(np.prod(r.cell_dims)) * 8 for r in reqs) + 1024
+    total = {}
+    for flags in (0, abi.FLAG_WARM_START_EXTRAPOLATE):
+        o = O.OracleSim(abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6, flags=flags))
+        for r in reqs:
+            o.emit(r)
+        t, its = 0.0, []
+        for _ in range(8):
+            o.step(1.0 / 240.0, scenes.tank_gravity(t))
+            t += 1.0 / 240.0
+            r = o.results()
+            assert r.converged == 1 and r.residual_rel <= 1e-6
+            its.append(r.solver_iterations)
+        total[flags] = sum(its[2:])
+        o.close()
+    assert total[abi.FLAG_WARM_START_EXTRAPOLATE] < total[0], total
