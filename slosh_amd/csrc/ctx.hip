@@ -96,6 +96,8 @@ void free_all(slosh_ctx *c)
     if (c->ev_step1) (void)hipEventDestroy(c->ev_step1);
     if (c->ev_solve0) (void)hipEventDestroy(c->ev_solve0);
     if (c->ev_solve1) (void)hipEventDestroy(c->ev_solve1);
+    if (c->ev_counts) (void)hipEventDestroy(c->ev_counts);
+    if (c->ev_pool) (void)hipEventDestroy(c->ev_pool);
     if (c->mg_graph) (void)hipGraphExecDestroy(c->mg_graph);
     c->mg_graph = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -143,6 +145,8 @@ void create_impl(slosh_ctx *c)
     SLOSH_HIP(hipEventCreate(&c->ev_step1));
     SLOSH_HIP(hipEventCreate(&c->ev_solve0));
     SLOSH_HIP(hipEventCreate(&c->ev_solve1));
+    SLOSH_HIP(hipEventCreateWithFlags(&c->ev_counts, hipEventDisableTiming));
+    SLOSH_HIP(hipEventCreateWithFlags(&c->ev_pool, hipEventDisableTiming));
     Geo &g = c->geo;
     for (int a = 0; a < 3; ++a) {
         g.n[a] = s.grid_dim[a];
@@ -370,13 +374,32 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     c->res.live_particles = c->np;
     if (stop > SLOSH_STAGE_SORT) {
         run_pool(c);
-        read_stats(c);
+        // The brick rows and P2G are launched over a bound of the new active-brick count (the last count plus an
+        // eighth) and clip to the count on the device, so the host reads the count back while P2G runs; bricks
+        // past the bound follow in a second launch. With kernel timing on, the host waits first (event pairs are
+        // harvested on an idle stream).
+        const uint32_t npool_max = c->geo.npool > 0u ? c->geo.npool - 1u : 0u;
+        uint32_t bound = 0u;
+        if (c->timing_mask == 0u) {
+            const uint64_t b64 = (uint64_t)c->nactive + c->nactive / 8u + 64u;
+            bound = (uint32_t)(b64 < npool_max ? b64 : npool_max);
+            SLOSH_HIP(hipMemcpyAsync(c->h_stats, c->stats, sizeof(StepStats), hipMemcpyDeviceToHost, c->stream));
+            SLOSH_HIP(hipEventRecord(c->ev_pool, c->stream));
+            build_brick_rows(c, 0u, bound);
+            if (stop > SLOSH_STAGE_POOL) launch_p2g(c, warm, dt, dt_prev, 0u, bound);
+            SLOSH_HIP(hipEventSynchronize(c->ev_pool));
+        } else {
+            read_stats(c);
+        }
         c->nactive = c->h_stats->nactive;
         c->res.active_bricks = c->nactive;
         c->res.pool_overflow_bricks = c->h_stats->overflow;
-        build_brick_rows(c);
+        if (c->nactive > bound) {
+            build_brick_rows(c, bound, c->nactive - bound);
+            if (stop > SLOSH_STAGE_POOL) launch_p2g(c, warm, dt, dt_prev, bound, c->nactive - bound);
+        }
+        c->n_p2g_blocks = stop > SLOSH_STAGE_POOL ? c->nactive : 0u;
     }
-    if (stop > SLOSH_STAGE_POOL) launch_p2g(c, warm, dt, dt_prev);
     if (stop > SLOSH_STAGE_P2G) launch_normalize(c, dt, gv);
     if (stop > SLOSH_STAGE_NORMALIZE) launch_divergence(c, dt);
     bool solved = false;

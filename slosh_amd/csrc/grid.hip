@@ -226,11 +226,11 @@ __global__ void k_pool_active(Geo g, const uint32_t *alloc_flag, const uint32_t 
 // Per allocated brick (active order), once per step after the pool: the slots of its 3x3x3 brick neighbourhood
 // (word (dx+1) + 3(dy+1) + 9(dz+1); 0 outside the grid or unallocated, as the brick table gives) and, in word 27,
 // its brick index. One thread per word.
-__global__ void k_brick_rows(Geo g, uint32_t nactive, uint32_t *rows)
+__global__ void k_brick_rows(Geo g, uint32_t first, const uint32_t *nactive, uint32_t *rows)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t e = t / kBinfoWords, k = t % kBinfoWords;
-    if (e >= nactive) return;
+    const uint32_t e = first + t / kBinfoWords, k = t % kBinfoWords;
+    if (e >= *nactive) return;
     const uint32_t b = g.active[e];
     uint32_t v = 0u;
     if (k == 27u) {
@@ -242,7 +242,7 @@ __global__ void k_brick_rows(Geo g, uint32_t nactive, uint32_t *rows)
         if (x >= 0 && y >= 0 && z >= 0 && x < g.bd[0] && y < g.bd[1] && z < g.bd[2])
             v = g.table[(uint32_t)(x + g.bd[0] * (y + g.bd[1] * z))];
     }
-    rows[t] = v;
+    rows[(size_t)e * kBinfoWords + k] = v;
 }
 
 // ---- per-brick helpers -------------------------------------------------------------------------------------
@@ -709,11 +709,16 @@ void run_pool(slosh_ctx *c)
 }
 
 // The neighbourhood rows of the allocated bricks (once per step, after the pool; every per-brick pass uses them).
-void build_brick_rows(slosh_ctx *c)
+const uint32_t *device_nactive(const slosh_ctx *c)
 {
-    if (!c->nactive) return;
-    launch_timed(c, K_POOL, k_brick_rows, dim3(blocks_for((size_t)c->nactive * kBinfoWords, 256)), dim3(256), 0,
-                 c->geo, c->nactive, c->binfo);
+    return reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(c->stats) + offsetof(StepStats, nactive));
+}
+
+void build_brick_rows(slosh_ctx *c, uint32_t first, uint32_t count)
+{
+    if (!count) return;
+    launch_timed(c, K_POOL, k_brick_rows, dim3(blocks_for((size_t)count * kBinfoWords, 256)), dim3(256), 0, c->geo,
+                 first, device_nactive(c), c->binfo);
     SLOSH_HIP(hipGetLastError());
 }
 

@@ -312,12 +312,14 @@ struct WarmStart {
 __global__ __launch_bounds__(256) void k_p2g(Geo g, Parts p, const uint32_t *key_off, int32_t *tiles,
                                              uint8_t *tile_valid, uint32_t *ctype, double *pres, uint8_t *new_slot,
                                              float warm, WarmStart ws, const slosh_collider *col, uint32_t ncol,
-                                             uint32_t *bp_fluid)
+                                             uint32_t *bp_fluid, uint32_t first, const uint32_t *nactive)
 {
     __shared__ int32_t tile[kTileInts];
     __shared__ int32_t cnt[kBrickCells];
     __shared__ uint32_t fluid_sum;
-    const uint32_t b = g.active[blockIdx.x];
+    const uint32_t e = first + blockIdx.x;
+    if (e >= *nactive) return;
+    const uint32_t b = g.active[e];
     const uint32_t slot = g.table[b];
     const bool fresh = new_slot[slot] != 0u; // recycled slot: its old pressure is not a warm start (ClearNewSlotsCS)
     int bc[3];
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(256) void k_p2g(Geo g, Parts p, const uint32_t *key
     if (fluid) atomicAdd(&fluid_sum, fluid);
     __syncthreads();
     if (threadIdx.x == 0) {
-        bp_fluid[blockIdx.x] = fluid_sum;
+        bp_fluid[e] = fluid_sum;
         if (fresh) new_slot[slot] = 0u;
     }
 }
@@ -942,13 +944,13 @@ void run_sort(slosh_ctx *c)
     c->sorted = true;
 }
 
-void launch_p2g(slosh_ctx *c, float warm_scale, float dt, float dt_prev)
+void launch_p2g(slosh_ctx *c, float warm_scale, float dt, float dt_prev, uint32_t first, uint32_t count)
 {
     const WarmStart ws{c->pres_prev, (double)dt, (double)dt_prev};
-    if (!c->nactive) return;
-    launch_timed(c, K_P2G, k_p2g, dim3(c->nactive), dim3(256), 0, c->geo, c->parts[c->cur], c->key_off, c->tiles,
-                 c->tile_valid, c->ctype, c->pres, c->new_slot, warm_scale, ws, c->d_col, c->ncol, c->bp_fluid);
-    c->n_p2g_blocks = c->nactive;
+    if (!count) return;
+    launch_timed(c, K_P2G, k_p2g, dim3(count), dim3(256), 0, c->geo, c->parts[c->cur], c->key_off, c->tiles,
+                 c->tile_valid, c->ctype, c->pres, c->new_slot, warm_scale, ws, c->d_col, c->ncol, c->bp_fluid, first,
+                 device_nactive(c));
     SLOSH_HIP(hipGetLastError());
 }
 

@@ -301,6 +301,8 @@ struct slosh_ctx {
     uint64_t frame = 0;
     float prev_dt = 0.0f;
     hipEvent_t ev_step0 = nullptr, ev_step1 = nullptr, ev_solve0 = nullptr, ev_solve1 = nullptr;
+    hipEvent_t ev_pool = nullptr;   // after the pool pass's readback of the step statistics
+    hipEvent_t ev_counts = nullptr; // after the multigrid set-up's readback of the fluid brick and tile counts
     // per-kernel timing (event pairs around selected launches, accumulated after each step's final sync)
     uint64_t timing_mask = 0;
     hipEvent_t *tev = nullptr;
@@ -317,12 +319,16 @@ void launch_emit_fill(slosh_ctx *c, const slosh_emit_request *r, uint32_t count)
 void launch_unpack_particles(slosh_ctx *c, const slosh_particle *d_src, uint32_t n);
 void launch_pack_particles(slosh_ctx *c, void *d_dst, int layout);
 void run_sort(slosh_ctx *c);
-void launch_p2g(slosh_ctx *c, float warm_scale, float dt, float dt_prev);
+// Active bricks [first, first + count) of the pool, clipped on the device to the count the pool pass wrote (so that
+// they can be launched before the host has read that count back).
+void launch_p2g(slosh_ctx *c, float warm_scale, float dt, float dt_prev, uint32_t first, uint32_t count);
 void launch_g2p(slosh_ctx *c, float dt, const float g[3]);
 
 // ---- grid.hip ----
 void run_pool(slosh_ctx *c);
-void build_brick_rows(slosh_ctx *c);
+void build_brick_rows(slosh_ctx *c, uint32_t first, uint32_t count);
+// Device address of the active-brick count in the step statistics.
+const uint32_t *device_nactive(const slosh_ctx *c);
 void launch_normalize(slosh_ctx *c, float dt, const float g[3]);
 void launch_divergence(slosh_ctx *c, float dt);
 void launch_project(slosh_ctx *c);

@@ -1166,15 +1166,22 @@ void mg_setup(slosh_ctx *c)
         SLOSH_HIP(hipMemcpyAsync(c->h_nsolve + 1, c->tile_rank + c->ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost,
                                  c->stream));
     }
-    // one readback for the counts of bricks with fluid and of tiles
-    SLOSH_HIP(hipStreamSynchronize(c->stream));
-    c->nsolve = c->h_nsolve[0];
-    if (m >= 5) c->ntiles_f = c->h_nsolve[1];
+    // one readback for the counts of bricks with fluid and of tiles; the host waits for it (mg_counts) while the
+    // coarsest factor and the residual initialisation run
+    SLOSH_HIP(hipEventRecord(c->ev_counts, c->stream));
     const int mode = m == 1 ? 0 : (c->lv[m - 1].pooled ? 1 : 2);
     PLevel P = mode == 1 ? plevel_of(c, m - 1) : PLevel{};
     DenseLevel D = mode == 2 ? dlevel_of(c, m - 1) : DenseLevel{};
     launch_timed(c, K_COARSE_FACTOR, k_coarse_factor, dim3(1), dim3(256), 0, c->geo, c->ctype, c->diag, mode, P, D,
                  c->coarse_idx, c->coarse_n, c->coarse_inv, c->coarse_sig);
+}
+
+// The counts read back by mg_setup; every pass launched over the bricks with fluid or the level-3 tiles comes after.
+void mg_counts(slosh_ctx *c)
+{
+    SLOSH_HIP(hipEventSynchronize(c->ev_counts));
+    c->nsolve = c->h_nsolve[0];
+    c->ntiles_f = c->nlevels >= 5 ? c->h_nsolve[1] : 0u;
 }
 
 // Captures the launches of `body` on the context's stream into c->mg_graph (updating the executable graph in
@@ -1230,6 +1237,7 @@ void run_mgpcg(slosh_ctx *c)
     float *D[2] = {c->d, c->d2};
     l0_pcg_init(c, R[0]);
     finalize(c, kFinInit);
+    mg_counts(c);
     if (!c->nsolve) {
         // no fluid cell: b = 0, the solve is converged as it stands
         c->capture_n = 0;
@@ -1255,7 +1263,8 @@ void run_mgpcg(slosh_ctx *c)
     SLOSH_HIP(hipGetLastError());
     // Iterations are enqueued in chunks between reads of the device state (an even count, so the ping-pong
     // buffers return to the same roles). Without residual capture or kernel timing, a chunk is captured once per
-    // solve as a HIP graph and replayed: every chunk of a solve launches the same kernels on the same buffers.
+    // solve as a HIP graph and replayed from the second chunk on: every chunk of a solve launches the same kernels
+    // on the same buffers.
     const int chunk = c->solver_chunk >= 2 ? c->solver_chunk & ~1 : 4;
     const bool use_graph = c->use_graphs && !c->capture && c->timing_mask == 0;
     auto enqueue_chunk = [&](int it0) {
@@ -1274,11 +1283,13 @@ void run_mgpcg(slosh_ctx *c)
     // Before the first read of the state, as many chunks as the previous solve's iteration count fills (less one
     // iteration): consecutive solves of a simulation take similar counts, and a chunk enqueued past convergence
     // only costs its early-exit launches. Afterwards the state is read after every chunk.
-    int ahead = c->last_mg_iters > 1 ? (c->last_mg_iters - 1) / chunk : 1;
+    int ahead = c->last_mg_iters > 1 ? (c->last_mg_iters + chunk - 1) / chunk : 1;
     if (ahead < 1) ahead = 1;
     for (int it = 0; it <= maxit;) {
         for (int k = 0; k < ahead && it <= maxit; ++k, it += chunk) {
-            if (use_graph && c->use_graphs) {
+            // the solve's first chunk is enqueued directly, so that the GPU runs it while the host captures the
+            // graph and updates the executable one (which takes longer than the first iteration)
+            if (use_graph && c->use_graphs && it > 0) {
                 // on a capture failure the helper has enqueued the chunk directly and switched graphs off
                 if (!graph_ready) graph_ready = capture_chunk_graph(c, [&] { enqueue_chunk(it); });
                 if (graph_ready) SLOSH_HIP(hipGraphLaunch(c->mg_graph, c->stream));
