@@ -479,16 +479,15 @@ __global__ __launch_bounds__(512) void k_project(Geo g, const uint32_t *rows, fl
 __global__ __launch_bounds__(512) void k_div_stats(Geo g, const uint32_t *rows, const float *vel, const uint32_t *ctype,
                                                    uint32_t *bp_max, uint32_t *bp_cnt, double *bp_sum)
 {
-    __shared__ double ssum[512];
-    __shared__ uint32_t smax, scount;
+    constexpr uint32_t kWaves = 512u / 64u;
+    __shared__ double ssum[kWaves];
+    __shared__ uint32_t smax[kWaves], scount[kWaves];
     __shared__ uint32_t s_row[28];
-    if (threadIdx.x == 0) {
-        smax = 0u;
-        scount = 0u;
-    }
     CellPos c;
     uint32_t e;
     double rate_d = 0.0;
+    uint32_t rate_bits = 0u; // rates are non-negative, so their bits order like the values
+    bool fluid = false;
     if (cell_of_block(g, rows, s_row, c, e) && (ctype[c.idx] & kCellTypeMask) == kCellFluid) {
         float dv = 0.0f;
         for (int a = 0; a < 3; ++a) {
@@ -497,20 +496,34 @@ __global__ __launch_bounds__(512) void k_div_stats(Geo g, const uint32_t *rows, 
             dv += vel[face_index(g, c, a, p[0], p[1], p[2])] - vel[(uint32_t)a * g.pool_cells + c.idx];
         }
         float rate = fabsf(dv) * g.inv_h;
-        atomicMax(&smax, __float_as_uint(rate));
-        atomicAdd(&scount, 1u);
+        rate_bits = __float_as_uint(rate);
         rate_d = (double)rate;
+        fluid = true;
     }
-    ssum[threadIdx.x] = rate_d;
+    // per wavefront: the sum by a fixed shuffle tree, the maximum and the count in any order; then the eight
+    // wavefront values in wavefront order
+    for (int s = 32; s > 0; s >>= 1) {
+        rate_d += __shfl_down(rate_d, s);
+        rate_bits = max(rate_bits, (uint32_t)__shfl_down((int)rate_bits, s));
+    }
+    const uint32_t n = (uint32_t)__popcll(__ballot(fluid));
+    if ((threadIdx.x & 63u) == 0u) {
+        ssum[threadIdx.x >> 6] = rate_d;
+        smax[threadIdx.x >> 6] = rate_bits;
+        scount[threadIdx.x >> 6] = n;
+    }
     __syncthreads();
-    for (uint32_t s = 256; s > 0; s >>= 1) {
-        if (threadIdx.x < s) ssum[threadIdx.x] += ssum[threadIdx.x + s];
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        bp_sum[e] = ssum[0];
-        bp_max[e] = smax;
-        bp_cnt[e] = scount;
+        double sum = 0.0;
+        uint32_t mx = 0u, cnt = 0u;
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            sum += ssum[w];
+            mx = max(mx, smax[w]);
+            cnt += scount[w];
+        }
+        bp_sum[e] = sum;
+        bp_max[e] = mx;
+        bp_cnt[e] = cnt;
     }
 }
 

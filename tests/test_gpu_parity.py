@@ -132,7 +132,8 @@ def test_stage_bitexact(name, stage):
 
 @pytest.mark.parametrize("name", ["dam32_sphere", "random24"])
 def test_multistep_bitexact(name):
-    """Four full steps with a fresh seeded pressure each step: every particle bit-identical after every step."""
+    """Four full steps with a fresh seeded pressure each step: every particle bit-identical after every step, and
+    the step statistics (maximum speed, clamps, fluid cells, post-projection divergence) equal to the oracle's."""
     print(f"case {name}", flush=True)
     gpu, orc = make_pair(name)
     rng = np.random.default_rng(1)
@@ -148,6 +149,11 @@ def test_multistep_bitexact(name):
         rg, ro = gpu.results(), orc.results()
         assert rg.max_particle_speed == ro.max_particle_speed
         assert rg.clamped_particles == ro.clamped_particles
+        assert rg.fluid_cells == ro.fluid_cells
+        # post-projection divergence: the per-cell rates are bit-identical, so the maximum is; the mean sums the
+        # same FP32 rates in FP64 in another order
+        assert rg.max_divergence == ro.max_divergence and rg.max_divergence > 0.0
+        assert abs(rg.mean_divergence - ro.mean_divergence) <= 1e-6 * abs(ro.mean_divergence)
 
 
 def test_large_sparse_grid_bitexact():
