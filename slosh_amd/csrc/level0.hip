@@ -83,9 +83,9 @@ __device__ __forceinline__ uint32_t parent1(uint32_t l)
     return ((l & 7u) >> 1) + 4u * (((l >> 3) & 7u) >> 1) + 16u * ((l >> 6) >> 1);
 }
 
-// Workgroup sum of each thread's values into partials[blockIdx.x * NV + k]: a shuffle tree within each wavefront
-// (s = 32, 16, ..., 1), then the four wavefront sums added in wavefront order by thread 0 (one barrier). The order
-// is fixed, so repeated runs give identical partials.
+// Workgroup sum of each thread's values into partials[blockIdx.x * NV + k]: wave_sum within each wavefront, then
+// the four wavefront sums added in wavefront order by thread 0 (one barrier). The order is fixed, so repeated runs
+// give identical partials.
 template <int NV> __device__ void block_partial(const double (&v)[NV], double *partials)
 {
     static_assert(NV >= 1 && NV <= (int)kMaxBlockPartials, "partials buffer holds kMaxBlockPartials per brick");
@@ -93,9 +93,7 @@ template <int NV> __device__ void block_partial(const double (&v)[NV], double *p
     __shared__ double sh[NV][kThreads / 64u];
     const uint32_t t = threadIdx.x;
     double a[NV];
-    for (int k = 0; k < NV; ++k) a[k] = v[k];
-    for (int s = 32; s > 0; s >>= 1)
-        for (int k = 0; k < NV; ++k) a[k] += __shfl_down(a[k], s);
+    for (int k = 0; k < NV; ++k) a[k] = wave_sum(v[k]);
     if ((t & 63u) == 0u)
         for (int k = 0; k < NV; ++k) sh[k][t >> 6] = a[k];
     __syncthreads();

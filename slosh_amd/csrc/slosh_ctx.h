@@ -85,6 +85,29 @@ constexpr uint32_t kBinfoWords = 32u;
 // partials buffer holds this many per pool slot, one workgroup per allocated brick.
 constexpr uint32_t kMaxBlockPartials = 4u;
 
+// Wavefront sum of an FP64 value in a fixed order, without LDS round trips: inclusive sums within each row of 16
+// lanes by DPP row shifts of 1, 2, 4 and 8 (a lane shifted in from outside its row adds +0), then the four row
+// totals (lanes 15, 31, 47, 63) added in row order. Every lane returns the same value.
+template <int kCtrl> __device__ __forceinline__ double dpp_row_shr(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_value(double v, int lane)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_row_shr<0x111>(v); // row_shr:1
+    v += dpp_row_shr<0x112>(v); // row_shr:2
+    v += dpp_row_shr<0x114>(v); // row_shr:4
+    v += dpp_row_shr<0x118>(v); // row_shr:8
+    return ((lane_value(v, 15) + lane_value(v, 31)) + lane_value(v, 47)) + lane_value(v, 63);
+}
+
 // Brick handled by workgroup w of a one-brick-per-workgroup pass over n bricks. Workgroups are dispatched
 // round-robin over the MI355X's 8 XCDs, each with its own L2; giving XCD k a contiguous range of the ascending
 // brick list keeps most of a brick's neighbours (and their halo reads) in the same L2. A bijection on [0, n).

@@ -62,7 +62,7 @@ __host__ __device__ constexpr int fin_values(int kind)
 }
 
 // Sums the partials of the preceding reduction in a fixed order (per thread: its strided run in increasing order;
-// then a wavefront butterfly and the 16 wavefront sums in order) and applies them to the solver state.
+// then wave_sum and the 16 wavefront sums in order) and applies them to the solver state.
 __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
                                                    SolverState *st)
 {
@@ -92,8 +92,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
         for (int j = 0; j < kMaxV; ++j) a[j] += v[k][j];
     for (uint32_t b = t + 1024u * kFirst; b < nblocks; b += 1024u)
         for (int j = 0; j < nv; ++j) a[j] += partials[(size_t)b * nv + j];
-    for (int w = 32; w > 0; w >>= 1)
-        for (int j = 0; j < kMaxV; ++j) a[j] += __shfl_xor(a[j], w);
+    for (int j = 0; j < kMaxV; ++j) a[j] = wave_sum(a[j]);
     if ((t & 63u) == 0u)
         for (int j = 0; j < kMaxV; ++j) sh[j][t >> 6] = a[j];
     __syncthreads();
