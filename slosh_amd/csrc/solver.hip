@@ -62,23 +62,25 @@ __host__ __device__ constexpr int fin_values(int kind)
 }
 
 // Sums the partials of the preceding reduction in a fixed order (per thread: its strided run in increasing order;
-// then wave_sum and the 16 wavefront sums in order) and applies them to the solver state.
-__global__ __launch_bounds__(1024) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
-                                                   SolverState *st)
+// then wave_sum and the wavefront sums in order) and applies them to the solver state.
+constexpr uint32_t kFinThreads = 1024;
+__global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
+                                                          SolverState *st)
 {
     constexpr int kMaxV = 4;
+    constexpr uint32_t kWaves = kFinThreads / 64u;
     const int nv = fin_values(kind);
-    __shared__ double sh[kMaxV][16];
+    __shared__ double sh[kMaxV][kWaves];
     const uint32_t t = threadIdx.x;
-    // the first 8 partials of this thread (b = t + 1024 k) are read together with the done flag; a thread sums its
-    // partials in increasing b (an absent one adds +0, which leaves the sum unchanged: it starts at +0 and never
-    // becomes -0)
+    // the first 8 partials of this thread (b = t + kFinThreads k) are read together with the done flag; a thread
+    // sums its partials in increasing b (an absent one adds +0, which leaves the sum unchanged: it starts at +0 and
+    // never becomes -0)
     constexpr int kFirst = 8;
     const SolverState s0 = *st; // read with the partials; thread 0 updates the state from it
     const uint32_t done = (uint32_t)s0.done;
     double v[kFirst][kMaxV];
     for (int k = 0; k < kFirst; ++k) {
-        const uint32_t b = t + 1024u * (uint32_t)k;
+        const uint32_t b = t + kFinThreads * (uint32_t)k;
         for (int j = 0; j < kMaxV; ++j) v[k][j] = (b < nblocks && j < nv) ? partials[(size_t)b * nv + j] : 0.0;
     }
     issue_point();
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
     double a[kMaxV] = {0.0, 0.0, 0.0, 0.0};
     for (int k = 0; k < kFirst; ++k)
         for (int j = 0; j < kMaxV; ++j) a[j] += v[k][j];
-    for (uint32_t b = t + 1024u * kFirst; b < nblocks; b += 1024u)
+    for (uint32_t b = t + kFinThreads * kFirst; b < nblocks; b += kFinThreads)
         for (int j = 0; j < nv; ++j) a[j] += partials[(size_t)b * nv + j];
     for (int j = 0; j < kMaxV; ++j) a[j] = wave_sum(a[j]);
     if ((t & 63u) == 0u)
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, const double *parti
     __syncthreads();
     if (t != 0) return;
     double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-    for (int w = 0; w < 16; ++w) {
+    for (uint32_t w = 0; w < kWaves; ++w) {
         t0 += sh[0][w];
         t1 += sh[1][w];
         t2 += sh[2][w];
@@ -1040,7 +1042,8 @@ void finalize(slosh_ctx *c, int kind, int iters = 0)
 {
     SLOSH_TIMED(c, K_FINALIZE);
     const uint32_t nblocks = kind == kFinInit ? c->nactive : c->nsolve;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, kind, c->partials, nblocks, iters, c->state);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, c->stream, kind, c->partials, nblocks, iters,
+                       c->state);
     (void)fin_values;
 }
 
