@@ -224,7 +224,8 @@ def bench_other(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    # defaults: the sloshing tank's 200-step run, its first 5 steps untimed
+    ap.add_argument("--steps", type=int, default=195)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="slosh256_16m", choices=sorted(WORKLOADS))
     ap.add_argument("--solver", default="mgpcg", choices=["mgpcg", "rbsor"])
@@ -242,7 +243,8 @@ def main():
                     help="further steps over which the dominant kernel is timed with HIP events")
     ap.add_argument("--no-kernel-events", action="store_true",
                     help="no HIP events around the dominant kernel in the timed steps (no roofline figure)")
-    ap.add_argument("--fcg-steps", type=int, default=10, help="timed steps of the learned-FCG comparison run")
+    ap.add_argument("--fcg-steps", type=int, default=None,
+                    help="timed steps of the learned-FCG comparison run (default: the same window as `value`)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run 512 MiB device copies first (scale reference for rocprofv3 --pmc FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -341,8 +343,10 @@ def main():
     learned = None
     wpath = unet.default_weights_path()
     if args.solver == "mgpcg" and not args.no_learned and os.path.exists(wpath):
+        fcg_steps = args.fcg_steps if args.fcg_steps is not None else args.steps
+        fcg_warmup = args.warmup if args.fcg_steps is None else 2
         fsps, fel, fper, fdom, (flaunch, fms), fprof, fsim = run(abi.SOLVER_FCG_UNET, unet.load(wpath),
-                                                                 args.fcg_steps, 2, mg)
+                                                                 fcg_steps, fcg_warmup, mg)
         fsim.close()
         favg = fms / 1e3 / max(flaunch, 1) if fms > 0 else float("nan")
         # level-0 forward convolutions: E0a (4->16), E0b and D0b (16->16), 27 taps, 512 cells per brick
@@ -356,7 +360,7 @@ def main():
                        "frac": round(ach / 2500.0, 4), "algorithmic_flops": int(conv_flops[fdom])})
         learned = {"steps_per_s": round(fsps, 3), "speedup_vs_mgpcg": round(fsps / sps, 3),
                    "mean_iterations": round(sum(p[0] for p in fper) / len(fper), 2),
-                   "fallback_steps": sum(1 for p in fper if p[3] == 1), "steps": args.fcg_steps,
+                   "fallback_steps": sum(1 for p in fper if p[3] == 1), "steps": fcg_steps, "warmup": fcg_warmup,
                    "converged_steps": sum(p[2] for p in fper), "max_rel_residual": max(p[1] for p in fper),
                    "roofline": fl, "weights": os.path.relpath(wpath, ROOT)}
     # the same workload with the reference's V-cycle, for comparison
@@ -382,6 +386,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.workload], "solver": f"{args.solver} to ||r||/||b|| <= 1e-6",
+                   "timed_window": f"steps {args.warmup + 1}-{args.warmup + args.steps} of the simulation",
                    "value_solver": args.solver, "particles": res.live_particles, "grid": list(grid),
                    "fluid_cells": nf, "allocated_bricks": nb, "fluid_bricks": ns // 512,
                    "mean_solver_iterations": round(sum(iters) / len(iters), 2), "max_rel_residual": max(rels),
