@@ -169,6 +169,46 @@ static float collider_velocity_axis(const slosh_collider *c, const float p[3], i
     return c->linear_velocity[axis] + cr[axis];
 }
 
+void oracle_collider_normal(const slosh_collider *c, const float p[3], float eps, float n[3])
+{
+    collider_normal(c, p, eps, n);
+}
+
+void oracle_collider_velocity(const slosh_collider *c, const float p[3], float v[3])
+{
+    for (int a = 0; a < 3; ++a) v[a] = collider_velocity_axis(c, p, a);
+}
+
+/* Slosh.usf:1578-1591 (AdvectParticle's collider response; adhesion off): a particle closer than `radius` to a
+ * collider's surface is pushed out along the finite-difference normal (eps = cell_size / 4) to the radius, its
+ * inward velocity relative to the moving surface is removed and the tangential part scaled by (1 - friction). */
+void oracle_collider_response(const slosh_collider *cols, uint32_t ncol, float radius, float friction,
+                              float cell_size, float x[3], float nv[3])
+{
+    for (uint32_t k = 0; k < ncol; ++k) {
+        const slosh_collider *cc = &cols[k];
+        float dd = oracle_collider_sdf(cc, x);
+        if (dd < radius) {
+            float nn[3];
+            collider_normal(cc, x, 0.25f * cell_size, nn);
+            float push = radius - dd;
+            for (int a = 0; a < 3; ++a) x[a] = x[a] + nn[a] * push;
+            float vs[3] = {collider_velocity_axis(cc, x, 0), collider_velocity_axis(cc, x, 1),
+                           collider_velocity_axis(cc, x, 2)};
+            float rel[3] = {nv[0] - vs[0], nv[1] - vs[1], nv[2] - vs[2]};
+            float vn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
+            if (vn < 0.0f) {
+                for (int a = 0; a < 3; ++a) rel[a] = rel[a] - vn * nn[a];
+                float rn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
+                float tg[3];
+                for (int a = 0; a < 3; ++a) tg[a] = rel[a] - rn * nn[a];
+                for (int a = 0; a < 3; ++a) rel[a] = rel[a] - friction * tg[a];
+            }
+            for (int a = 0; a < 3; ++a) nv[a] = vs[a] + rel[a];
+        }
+    }
+}
+
 /* SloshCommon.ush:244-252 */
 float oracle_ghost_theta(float rho_fluid, float rho_air)
 {
@@ -886,14 +926,10 @@ static double apply_row_d(const osim *o, int x, int y, int z, uint32_t idx, cons
     return (double)(o->diag[idx] + DIAG_EPS) * v[idx] - s;
 }
 
-/* SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG) with the kernels of SloshPressureMG.usf:284-543, restated with
- * FP64 dot products and alpha/beta updates and a relative-residual stop. */
-static void solve_mgpcg(osim *o)
+/* r = b - A p of the FP64 pressure (FP32 r), the red initialisation z = r / diag of the first V-cycle sweep;
+ * returns ||r||^2 and writes ||b||^2. */
+static double mgpcg_residual(osim *o, double *bb_out)
 {
-    double tol = (double)o->s.tolerance;
-    int maxit = o->s.max_iterations > 0 ? o->s.max_iterations : 200;
-    mg_build_types(o);
-    coarse_factor(o);
     double bb = 0.0, rr = 0.0;
     lv_iter it = {0, 0, 0, 0, 0, 0, 0};
     while (lv_next(o, &it)) {
@@ -911,63 +947,105 @@ static void solve_mgpcg(osim *o)
         bb += (double)o->div[i] * (double)o->div[i];
         rr += (double)ri * (double)ri;
     }
+    *bb_out = bb;
+    return rr;
+}
+
+/* ||b - A p||^2 of the FP64 pressure in FP64 (the true residual that decides convergence). */
+static double mgpcg_true_rr(osim *o)
+{
+    double rr = 0.0;
+    lv_iter it = {0, 0, 0, 0, 0, 0, 0};
+    while (lv_next(o, &it)) {
+        uint32_t i = it.idx;
+        if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+        double dgd, sumd;
+        stencil_d(o, 0, it.x, it.y, it.z, i, o->pres, &dgd, &sumd);
+        double ri = -(double)o->div[i] - (dgd * o->pres[i] - sumd);
+        rr += ri * ri;
+    }
+    return rr;
+}
+
+/* SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG) with the kernels of SloshPressureMG.usf:284-543, restated with
+ * FP64 dot products and alpha/beta updates and a relative-residual stop. The iteration stops when the residual it
+ * carries meets the tolerance; the solve is accepted when the FP64 true residual ||b - A p|| / ||b|| of the
+ * pressure does too, else it restarts from r = b - A p (at most three times, within max_iterations). */
+static void solve_mgpcg(osim *o)
+{
+    double tol = (double)o->s.tolerance;
+    int maxit = o->s.max_iterations > 0 ? o->s.max_iterations : 200;
+    mg_build_types(o);
+    coarse_factor(o);
+    double bb = 0.0;
+    double rr = mgpcg_residual(o, &bb);
     double bnorm = sqrt(bb);
     o->res.rhs_norm = bnorm;
     int iters = 0;
     double rel = bnorm > 0.0 ? sqrt(rr) / bnorm : 0.0;
-    if (bnorm == 0.0 || rel <= tol) {
+    o->res.true_residual_rel = (float)rel;
+    if (bnorm == 0.0 || (rel <= tol && sqrt(mgpcg_true_rr(o)) / bnorm <= tol)) {
         o->res.solver_iterations = 0;
         o->res.residual_rel = rel;
         o->res.converged = 1;
         return;
     }
-    double rho = vcycle_f(o);
-    it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-    while (lv_next(o, &it))
-        if ((o->ctype[it.idx] & CELL_TYPE_MASK) == CELL_FLUID) o->d[it.idx] = o->z[it.idx];
     int conv = 0;
-    while (iters < maxit) {
-        double dq = 0.0;
+    for (int restart = 0;; ++restart) {
+        lv_iter it;
+        double rho = vcycle_f(o);
         it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-        while (lv_next(o, &it)) {
-            uint32_t i = it.idx;
-            if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-            /* q = A d with FP64 accumulation of the FP32 d (the terms cancel for smooth d) */
-            double sum = 0.0;
-            for (int nb = 0; nb < 6; ++nb) {
-                int nx = it.x + NB[nb][0], ny = it.y + NB[nb][1], nz = it.z + NB[nb][2];
-                if (lv_type(o, 0, nx, ny, nz) == CELL_FLUID) sum += (double)o->d[cell_index(o, nx, ny, nz)];
+        while (lv_next(o, &it))
+            if ((o->ctype[it.idx] & CELL_TYPE_MASK) == CELL_FLUID) o->d[it.idx] = o->z[it.idx];
+        int rec = 0;
+        while (iters < maxit) {
+            double dq = 0.0;
+            it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
+            while (lv_next(o, &it)) {
+                uint32_t i = it.idx;
+                if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                /* q = A d with FP64 accumulation of the FP32 d (the terms cancel for smooth d) */
+                double sum = 0.0;
+                for (int nb = 0; nb < 6; ++nb) {
+                    int nx = it.x + NB[nb][0], ny = it.y + NB[nb][1], nz = it.z + NB[nb][2];
+                    if (lv_type(o, 0, nx, ny, nz) == CELL_FLUID) sum += (double)o->d[cell_index(o, nx, ny, nz)];
+                }
+                float dg = o->diag[i] + DIAG_EPS;
+                double qd = (double)dg * (double)o->d[i] - sum;
+                o->q[i] = (float)qd;
+                dq += (double)o->d[i] * qd;
             }
-            float dg = o->diag[i] + DIAG_EPS;
-            double qd = (double)dg * (double)o->d[i] - sum;
-            o->q[i] = (float)qd;
-            dq += (double)o->d[i] * qd;
+            double alpha = fabs(dq) > 1e-30 ? rho / dq : 0.0;
+            rr = 0.0;
+            it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
+            while (lv_next(o, &it)) {
+                uint32_t i = it.idx;
+                if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                o->pres[i] = o->pres[i] + alpha * (double)o->d[i];
+                float ri = (float)((double)o->r[i] - alpha * (double)o->q[i]);
+                o->r[i] = ri;
+                rr += (double)ri * (double)ri;
+                float dg = o->diag[i] + DIAG_EPS;
+                o->z[i] = is_red(it.x, it.y, it.z) ? ri / dg : 0.0f;
+            }
+            iters++;
+            rel = sqrt(rr) / bnorm;
+            if (rel <= tol) { rec = 1; break; }
+            double rho_new = vcycle_f(o);
+            double beta = fabs(rho) > 1e-30 ? rho_new / rho : 0.0;
+            rho = rho_new;
+            it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
+            while (lv_next(o, &it)) {
+                uint32_t i = it.idx;
+                if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                o->d[i] = (float)((double)o->z[i] + beta * (double)o->d[i]);
+            }
         }
-        double alpha = fabs(dq) > 1e-30 ? rho / dq : 0.0;
-        rr = 0.0;
-        it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-        while (lv_next(o, &it)) {
-            uint32_t i = it.idx;
-            if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-            o->pres[i] = o->pres[i] + alpha * (double)o->d[i];
-            float ri = (float)((double)o->r[i] - alpha * (double)o->q[i]);
-            o->r[i] = ri;
-            rr += (double)ri * (double)ri;
-            float dg = o->diag[i] + DIAG_EPS;
-            o->z[i] = is_red(it.x, it.y, it.z) ? ri / dg : 0.0f;
-        }
-        iters++;
-        rel = sqrt(rr) / bnorm;
-        if (rel <= tol) { conv = 1; break; }
-        double rho_new = vcycle_f(o);
-        double beta = fabs(rho) > 1e-30 ? rho_new / rho : 0.0;
-        rho = rho_new;
-        it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-        while (lv_next(o, &it)) {
-            uint32_t i = it.idx;
-            if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-            o->d[i] = (float)((double)o->z[i] + beta * (double)o->d[i]);
-        }
+        const double rel_true = sqrt(mgpcg_true_rr(o)) / bnorm;
+        o->res.true_residual_rel = (float)rel_true;
+        if (rec && rel_true <= tol) { conv = 1; break; }
+        if (iters >= maxit || restart >= 3) break;
+        mgpcg_residual(o, &bb);
     }
     o->res.solver_iterations = iters;
     o->res.residual_rel = rel;
@@ -1231,28 +1309,7 @@ static void stage_g2p(osim *o, float dt, const float grav[3])
             }
             memset(cm, 0, sizeof(cm));
         }
-        for (uint32_t k = 0; k < o->ncol; ++k) {
-            const slosh_collider *cc = &o->col[k];
-            float dd = oracle_collider_sdf(cc, x);
-            if (dd < radius) {
-                float nn[3];
-                collider_normal(cc, x, 0.25f * o->h, nn);
-                float push = radius - dd;
-                for (int a = 0; a < 3; ++a) x[a] = x[a] + nn[a] * push;
-                float vs[3] = {collider_velocity_axis(cc, x, 0), collider_velocity_axis(cc, x, 1),
-                               collider_velocity_axis(cc, x, 2)};
-                float rel[3] = {nv[0] - vs[0], nv[1] - vs[1], nv[2] - vs[2]};
-                float vn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
-                if (vn < 0.0f) {
-                    for (int a = 0; a < 3; ++a) rel[a] = rel[a] - vn * nn[a];
-                    float rn = rel[0] * nn[0] + rel[1] * nn[1] + rel[2] * nn[2];
-                    float tg[3];
-                    for (int a = 0; a < 3; ++a) tg[a] = rel[a] - rn * nn[a];
-                    for (int a = 0; a < 3; ++a) rel[a] = rel[a] - friction * tg[a];
-                }
-                for (int a = 0; a < 3; ++a) nv[a] = vs[a] + rel[a];
-            }
-        }
+        oracle_collider_response(o->col, o->ncol, radius, friction, o->h, x, nv);
         uint32_t contact = 0;
         for (int a = 0; a < 3; ++a) {
             if (x[a] < lo) {

@@ -49,6 +49,13 @@ uint32_t oracle_pcg_hash(uint32_t v);
 float oracle_hash_to_unit(uint32_t h);
 float oracle_collider_sdf(const slosh_collider *c, const float p[3]);
 float oracle_ghost_theta(float rho_fluid, float rho_air);
+/* Unit normal of a collider's surface by central differences of its SDF with step eps (SloshCommon.ush:197-205). */
+void oracle_collider_normal(const slosh_collider *c, const float p[3], float eps, float n[3]);
+/* Velocity of a collider's material point p: v + w x (p - centre) (SloshCommon.ush:232-235). */
+void oracle_collider_velocity(const slosh_collider *c, const float p[3], float v[3]);
+/* The collider response G2P applies to one particle (position x, velocity nv, updated in place). */
+void oracle_collider_response(const slosh_collider *cols, uint32_t ncol, float radius, float friction,
+                              float cell_size, float x[3], float nv[3]);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,14 @@ def load():
     lib.oracle_collider_sdf.restype = C.c_float
     lib.oracle_ghost_theta.argtypes = [C.c_float, C.c_float]
     lib.oracle_ghost_theta.restype = C.c_float
+    f3 = C.POINTER(C.c_float)
+    lib.oracle_collider_normal.argtypes = [C.POINTER(abi.Collider), f3, C.c_float, f3]
+    lib.oracle_collider_normal.restype = None
+    lib.oracle_collider_velocity.argtypes = [C.POINTER(abi.Collider), f3, f3]
+    lib.oracle_collider_velocity.restype = None
+    lib.oracle_collider_response.argtypes = [C.POINTER(abi.Collider), C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                             f3, f3]
+    lib.oracle_collider_response.restype = None
     _lib = lib
     return lib
 
@@ -190,3 +198,65 @@ def ghost_theta(rf, ra):
 def collider_sdf(col, p):
     pp = (C.c_float * 3)(*[float(x) for x in p])
     return float(load().oracle_collider_sdf(C.byref(col), pp))
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def collider_normal(col, p, eps):
+    out = _f3((0, 0, 0))
+    load().oracle_collider_normal(C.byref(col), _f3(p), C.c_float(eps), out)
+    return np.array(out[:], dtype=np.float32)
+
+
+def collider_velocity(col, p):
+    out = _f3((0, 0, 0))
+    load().oracle_collider_velocity(C.byref(col), _f3(p), out)
+    return np.array(out[:], dtype=np.float32)
+
+
+def collider_response(cols, x, v, radius, friction, cell_size):
+    """The G2P collider response of one particle: returns the new (position, velocity)."""
+    arr = (abi.Collider * max(len(cols), 1))(*cols)
+    xx, vv = _f3(x), _f3(v)
+    load().oracle_collider_response(arr, len(cols), C.c_float(radius), C.c_float(friction), C.c_float(cell_size),
+                                    xx, vv)
+    return np.array(xx[:], dtype=np.float32), np.array(vv[:], dtype=np.float32)
+
+
+def _half_bits(x):
+    """IEEE binary16 bits of FP32 values, round to nearest even, after the reference's clamp to +-65000."""
+    c = np.clip(np.asarray(x, dtype=np.float32), np.float32(-65000.0), np.float32(65000.0))
+    return c.astype(np.float16).view(np.uint16).astype(np.uint32)
+
+
+def pack_ref48(parts):
+    """The reference's 48-byte record of 64-byte FP32 records: SloshCommon.ush:71-88 (PackHalf2, PackParticle). The
+    APIC rows Cx, Cy, Cz are stored as nine halves in five uints, two per uint with the first in the low 16 bits, the
+    last high half 0; position, age and velocity stay FP32."""
+    parts = np.asarray(parts, dtype=abi.PARTICLE_DTYPE)
+    out = np.zeros(len(parts), dtype=abi.PARTICLE_REF48_DTYPE)
+    out["pos"] = parts["pos"]
+    out["age"] = parts["age"]
+    out["vel"] = parts["vel"]
+    h = _half_bits(parts["c"])
+    h = np.concatenate([h, np.zeros((len(parts), 1), dtype=np.uint32)], axis=1)
+    out["c"] = h[:, 0::2] | (h[:, 1::2] << np.uint32(16))
+    return out
+
+
+def unpack_ref48(packed):
+    """SloshCommon.ush:90-105 (UnpackParticle): the 48-byte record back to 64-byte FP32 records."""
+    packed = np.asarray(packed, dtype=abi.PARTICLE_REF48_DTYPE)
+    out = np.zeros(len(packed), dtype=abi.PARTICLE_DTYPE)
+    out["pos"] = packed["pos"]
+    out["age"] = packed["age"]
+    out["vel"] = packed["vel"]
+    lo = (packed["c"] & np.uint32(0xFFFF)).astype(np.uint16).view(np.float16)
+    hi = (packed["c"] >> np.uint32(16)).astype(np.uint16).view(np.float16)
+    halves = np.empty((len(packed), 10), dtype=np.float16)
+    halves[:, 0::2] = lo
+    halves[:, 1::2] = hi
+    out["c"] = halves[:, :9].astype(np.float32)
+    return out

@@ -229,3 +229,21 @@ def sphere_collider(center, radius, linear_velocity=(0.0, 0.0, 0.0)):
     c.rotation[3] = 1.0
     c.extent[0] = float(radius)
     return c
+
+
+def collider(kind, center, extent, rotation=(0.0, 0.0, 0.0, 1.0), linear_velocity=(0.0, 0.0, 0.0),
+             angular_velocity=(0.0, 0.0, 0.0), flags=0):
+    """Analytic collider (FSloshColliderGPU): kind 0 sphere (extent[0] = radius), 1 box (half extents), 2 capsule
+    (extent[0] = radius, extent[1] = half length of its segment along the local z axis); rotation is the unit
+    quaternion (x, y, z, w) from local to sim space."""
+    c = Collider()
+    c.type = int(kind)
+    for a in range(3):
+        c.center[a] = float(center[a])
+        c.extent[a] = float(extent[a]) if a < len(extent) else 0.0
+        c.linear_velocity[a] = float(linear_velocity[a])
+        c.angular_velocity[a] = float(angular_velocity[a])
+    for k in range(4):
+        c.rotation[k] = float(rotation[k])
+    c.flags = int(flags)
+    return c

@@ -699,6 +699,9 @@ slosh_status slosh_emit(slosh_ctx *c, const slosh_emit_request *r)
 {
     if (!c || !r) return SLOSH_EINVAL;
     if (r->shape != SLOSH_EMIT_FILL_CELLS) return SLOSH_EINVAL;
+    // cone spread and speed jitter are not applied by this emission path: a request asking for them is refused
+    // rather than silently emitted without them
+    if (r->velocity_spread != 0.0f || r->speed_jitter != 0.0f || r->reserved != 0u) return SLOSH_EINVAL;
     for (int a = 0; a < 3; ++a)
         if (r->cell_dims[a] < 0) return SLOSH_EINVAL;
     uint64_t count = (uint64_t)r->cell_dims[0] * (uint64_t)r->cell_dims[1] * (uint64_t)r->cell_dims[2] * 8u;

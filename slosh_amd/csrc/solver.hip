@@ -51,12 +51,12 @@ template <typename V> __device__ __forceinline__ float dsum(const DenseLevel &L,
 }
 
 enum FinKind { kFinInit = 0, kFinRho = 1, kFinAlphaStop = 2, kFinRr = 3, kFinSor = 4, kFinBetaF = 5,
-               kFinAlphaF = 6, kFinCheck = 7 };
+               kFinAlphaF = 6, kFinCheck = 7, kFinRestart = 8 };
 
 __host__ __device__ constexpr int fin_values(int kind)
 {
     if (kind == kFinAlphaStop) return 4;
-    if (kind == kFinInit) return 3;
+    if (kind == kFinInit || kind == kFinRestart) return 3;
     if (kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck) return 2;
     return 1;
 }
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         for (int j = 0; j < kMaxV; ++j) v[k][j] = (b < nblocks && j < nv) ? partials[(size_t)b * nv + j] : 0.0;
     }
     issue_point();
-    if (kind != kFinInit && kind != kFinCheck && done) {
+    if (kind != kFinInit && kind != kFinCheck && kind != kFinRestart && done) {
         // the update that met the stopping test has run: from here on every pass returns at once
         if (kind == kFinAlphaStop && done == 2u && t == 0) st->done = 1;
         return;
@@ -106,11 +106,23 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         t2 += sh[2][w];
         t3 += sh[3][w];
     }
-    if (kind == kFinInit) {
+    if (kind == kFinInit || kind == kFinRestart) {
         st->nf = t2;
         st->mg_scale = t1 > 0.0 ? sqrt(t2 / t1) : 1.0;
     }
-    if (kind == kFinInit || kind == kFinSor) {
+    if (kind == kFinRestart) {
+        // MGPCG restarted from the current pressure with r = b - A p (its true residual exceeded the tolerance
+        // although the recurrence residual met it): fresh directions, the iteration count carried on, at least one
+        // more iteration
+        st->bb = t0;
+        st->rr = t1;
+        st->rel = t0 > 0.0 ? sqrt(t1) / sqrt(t0) : 0.0;
+        st->rho = 0.0;
+        st->beta = 0.0;
+        st->alpha = 0.0;
+        st->converged = 0;
+        st->done = 0;
+    } else if (kind == kFinInit || kind == kFinSor) {
         // t0 = ||b||^2, t1 = ||r||^2
         st->bb = t0;
         st->rr = t1;
@@ -163,7 +175,11 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         st->dq = t0;
         st->alpha = fabs(t0) > 1e-30 ? t1 / t0 : 0.0;
     } else if (kind == kFinCheck) {
-        st->rel_true = t0 > 0.0 ? sqrt(t1) / sqrt(t0) : 0.0;
+        // the FP64 true residual ||b - A p|| / ||b|| of the solution: the solve counts as converged only when it
+        // meets the tolerance (the recurrence residual that stopped the iteration can differ from it)
+        const double rel_true = t0 > 0.0 ? sqrt(t1) / sqrt(t0) : 0.0;
+        st->rel_true = rel_true;
+        st->converged = rel_true <= s0.tol ? 1 : 0;
     } else {
         st->rr = t0;
         st->mg_scale = t0 > 0.0 ? sqrt(s0.nf / t0) : 1.0;
@@ -1034,6 +1050,7 @@ __global__ void k_state_reset(SolverState *st, double tol, int maxit)
     st->alpha = 0.0;
     st->beta = 0.0;
     st->rel = 0.0;
+    st->rel_true = 0.0;
 }
 
 // Reduction of the partials of the preceding level-0 pass: one per brick with fluid, except the initial residual,
@@ -1258,10 +1275,16 @@ void run_mgpcg(slosh_ctx *c)
         if (iteration >= 4) cap_gap++;
     };
     capture(R[0], 0);
-    l0_update_black(c, R[0], R[1], D[0], 1);
     int rc = 1, dc = 0;
-    vcycle(c, R[rc]);
-    l0_direction(c, D[1], D[0], R[rc], 1);
+    // the first iteration's update (no alpha yet), V-cycle and direction, from the residual in R[0]
+    auto first_iteration = [&]() {
+        l0_update_black(c, R[0], R[1], D[0], 1);
+        rc = 1;
+        dc = 0;
+        vcycle(c, R[rc]);
+        l0_direction(c, D[1], D[0], R[rc], 1);
+    };
+    first_iteration();
     SLOSH_HIP(hipGetLastError());
     // Iterations are enqueued in chunks between reads of the device state (an even count, so the ping-pong
     // buffers return to the same roles). Without residual capture or kernel timing, a chunk is captured once per
@@ -1269,6 +1292,7 @@ void run_mgpcg(slosh_ctx *c)
     // on the same buffers.
     const int chunk = c->solver_chunk >= 2 ? c->solver_chunk & ~1 : 4;
     const bool use_graph = c->use_graphs && !c->capture && c->timing_mask == 0;
+    int done_iters = 0; // iterations of the chunks enqueued so far
     auto enqueue_chunk = [&](int it0) {
         for (int k = 0; k < chunk; ++k) {
             // alpha and the stopping test of the update it makes (done == 2: only that update still runs)
@@ -1287,28 +1311,42 @@ void run_mgpcg(slosh_ctx *c)
     // only costs its early-exit launches. Afterwards the state is read after every chunk.
     int ahead = c->last_mg_iters > 1 ? (c->last_mg_iters + chunk - 1) / chunk : 1;
     if (ahead < 1) ahead = 1;
-    for (int it = 0; it <= maxit;) {
-        for (int k = 0; k < ahead && it <= maxit; ++k, it += chunk) {
-            // the solve's first chunk is enqueued directly, so that the GPU runs it while the host captures the
-            // graph and updates the executable one (which takes longer than the first iteration)
-            if (use_graph && c->use_graphs && it > 0) {
-                // on a capture failure the helper has enqueued the chunk directly and switched graphs off
-                if (!graph_ready) graph_ready = capture_chunk_graph(c, [&] { enqueue_chunk(it); });
-                if (graph_ready) SLOSH_HIP(hipGraphLaunch(c->mg_graph, c->stream));
-            } else {
-                enqueue_chunk(it);
+    // The recurrence residual that stops the iteration can differ from the FP64 true residual ||b - A p|| / ||b||
+    // of the pressure in the last bits; the solve is accepted only when the true residual meets the tolerance too,
+    // else it restarts from r = b - A p of the current pressure (a few times at most, within max_iterations).
+    constexpr int kMaxRestarts = 3;
+    for (int restart = 0;; ++restart) {
+        while (done_iters <= maxit) {
+            for (int k = 0; k < ahead && done_iters <= maxit; ++k, done_iters += chunk) {
+                // the solve's first chunk is enqueued directly, so that the GPU runs it while the host captures the
+                // graph and updates the executable one (which takes longer than the first iteration)
+                if (use_graph && c->use_graphs && done_iters > 0) {
+                    // on a capture failure the helper has enqueued the chunk directly and switched graphs off
+                    if (!graph_ready) graph_ready = capture_chunk_graph(c, [&] { enqueue_chunk(done_iters); });
+                    if (graph_ready) SLOSH_HIP(hipGraphLaunch(c->mg_graph, c->stream));
+                } else {
+                    enqueue_chunk(done_iters);
+                }
+                SLOSH_HIP(hipGetLastError());
             }
-            SLOSH_HIP(hipGetLastError());
+            ahead = 1;
+            SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+            SLOSH_HIP(hipStreamSynchronize(c->stream));
+            if (c->h_state->done) break;
         }
-        ahead = 1;
+        // the true residual of the FP64 pressure decides convergence (kFinCheck) and is reported with the results
+        l0_true_residual(c);
+        finalize(c, kFinCheck);
         SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
-        if (c->h_state->done) break;
+        if (c->h_state->converged || c->h_state->iters >= maxit || restart >= kMaxRestarts) break;
+        l0_pcg_init(c, R[0]);
+        finalize(c, kFinRestart);
+        first_iteration();
+        done_iters = c->h_state->iters;
+        ahead = 1;
     }
     c->last_mg_iters = c->h_state->iters;
-    // the true residual ||b - A p|| / ||b|| of the FP64 pressure, reported with the step's results
-    l0_true_residual(c);
-    finalize(c, kFinCheck);
     if (c->capture) {
         uint32_t valid = 0;
         while (valid < ncap && c->capture_iter[valid] <= c->h_state->iters) valid++;
@@ -1318,9 +1356,9 @@ void run_mgpcg(slosh_ctx *c)
 
 // Flexible CG (Notay 2000, FCG(1)) preconditioned by the network: z = M(r), d = z - (z.q_prev / d_prev.q_prev) d_prev,
 // alpha = d.r / d.q. MGPCG is the fallback and the judge: when the relative residual has not halved over the last
-// four iterations, or an iteration limit is reached, or the recurrence converges but the true residual
-// ||b - A p|| / ||b|| of the FP64 pressure exceeds twice the tolerance, MGPCG continues from the current pressure
-// to the tolerance. Every accepted solve therefore meets the MGPCG stopping rule.
+// four iterations, or an iteration limit is reached, or the recurrence converges but the FP64 true residual
+// ||b - A p|| / ||b|| of the pressure exceeds the tolerance, MGPCG continues from the current pressure to the
+// tolerance. Every accepted solve therefore has a true residual within the tolerance.
 void run_fcg(slosh_ctx *c)
 {
     c->fcg_iters = 0;
@@ -1380,7 +1418,7 @@ void run_fcg(slosh_ctx *c)
         finalize(c, kFinCheck);
         SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
-        if (!(c->h_state->rel_true <= 2.0 * (double)c->s.tolerance)) fallback = true;
+        if (!c->h_state->converged) fallback = true; // kFinCheck: true residual within the tolerance
     }
     if (fallback) {
         c->fcg_fallback = 1;

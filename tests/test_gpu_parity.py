@@ -40,6 +40,15 @@ def scene(name):
         grid, h, reqs = scenes.dam_break(32, seed=5)
         col = [abi.sphere_collider((0.55, 0.25, 0.5), 0.12, linear_velocity=(0.3, 0.0, -0.2))]
         return abi.make_settings(grid, h, 300000, solver=abi.SOLVER_NONE, friction=0.2), reqs, col, None
+    if name == "dam32_box_capsule":
+        # a rotated box and a capsule, both moving and spinning: the box and capsule SDF branches, the
+        # finite-difference normals and the rigid-body surface velocity on faces and in the G2P contact response
+        grid, h, reqs = scenes.dam_break(32, seed=6)
+        col = [abi.collider(1, (0.62, 0.3, 0.45), (0.12, 0.07, 0.15), rotation=(0.1, 0.3, -0.2, 0.927),
+                            linear_velocity=(-0.2, 0.1, 0.3), angular_velocity=(0.5, -1.5, 2.0)),
+               abi.collider(2, (0.3, 0.18, 0.6), (0.06, 0.15, 0.0), rotation=(0.5, 0.5, 0.5, 0.5),
+                            linear_velocity=(0.4, 0.0, -0.1), angular_velocity=(0.0, 3.0, 0.0))]
+        return abi.make_settings(grid, h, 300000, solver=abi.SOLVER_NONE, friction=0.35), reqs, col, None
     if name == "random24":
         parts = rand_particles(24, 60000, seed=11)
         return abi.make_settings((24, 24, 24), 1.0 / 24, 80000, solver=abi.SOLVER_NONE), [], [], parts
@@ -88,6 +97,28 @@ def assert_bits(a, b, what):
                              f"{a[tuple(d[0] for d in diff)]} vs {b[tuple(d[0] for d in diff)]}")
 
 
+def system_residual(sim, p):
+    """||b - A p|| / ||b|| in FP64 numpy of a dense pressure p on the system of `sim`'s last step, assembled from its
+    own cell flags, divergence and ghost-fluid diagonal: (diag + 1e-4) p_c - sum_{fluid n} p_n = -div_c on fluid
+    cells (the library forms diag + 1e-4 in FP32)."""
+    flags = sim.read_grid(abi.FIELD_CELL_FLAGS)
+    fluid = (flags & 3) == 1
+    div = sim.read_grid(abi.FIELD_DIVERGENCE).astype(np.float64)
+    dg = (sim.read_grid(abi.FIELD_CELL_DIAG) + np.float32(1e-4)).astype(np.float64)
+    pf = np.where(fluid, np.asarray(p, dtype=np.float64), 0.0)
+    nb = np.zeros_like(pf)
+    for ax in range(3):
+        for sh in (1, -1):
+            q = np.roll(pf, sh, axis=ax)
+            sl = [slice(None)] * 3
+            sl[ax] = 0 if sh == 1 else -1
+            q[tuple(sl)] = 0.0
+            nb += q
+    r = np.where(fluid, -div - (dg * pf - nb), 0.0)
+    b = np.where(fluid, div, 0.0)
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
 STAGE_FIELDS = {
     abi.STAGE_POOL: [abi.FIELD_BRICK_TABLE],
     abi.STAGE_P2G: [abi.FIELD_ACCUM, abi.FIELD_CELL_FLAGS],
@@ -98,7 +129,7 @@ STAGE_FIELDS = {
 }
 
 
-@pytest.mark.parametrize("name", ["dam20", "dam32_sphere", "random24", "wave40x24x16"])
+@pytest.mark.parametrize("name", ["dam20", "dam32_sphere", "dam32_box_capsule", "random24", "wave40x24x16"])
 @pytest.mark.parametrize("stage", [abi.STAGE_SORT, abi.STAGE_POOL, abi.STAGE_P2G, abi.STAGE_NORMALIZE,
                                    abi.STAGE_DIVERGENCE, abi.STAGE_PROJECT, abi.STAGE_EXTRAPOLATE, abi.STAGE_ALL])
 def test_stage_bitexact(name, stage):
@@ -130,7 +161,7 @@ def test_stage_bitexact(name, stage):
         assert rg.fluid_cells == ro.fluid_cells
 
 
-@pytest.mark.parametrize("name", ["dam32_sphere", "random24"])
+@pytest.mark.parametrize("name", ["dam32_sphere", "dam32_box_capsule", "random24"])
 def test_multistep_bitexact(name):
     """Four full steps with a fresh seeded pressure each step: every particle bit-identical after every step, and
     the step statistics (maximum speed, clamps, fluid cells, post-projection divergence) equal to the oracle's."""
@@ -154,6 +185,25 @@ def test_multistep_bitexact(name):
         # same FP32 rates in FP64 in another order
         assert rg.max_divergence == ro.max_divergence and rg.max_divergence > 0.0
         assert abs(rg.mean_divergence - ro.mean_divergence) <= 1e-6 * abs(ro.mean_divergence)
+
+
+def test_ref48_layout_matches_the_reference_packing():
+    """SLOSH_LAYOUT_REF_48 (the renderer's record, FSloshParticleGPU) equals the reference's PackParticle
+    (SloshCommon.ush:71-88, restated in numpy: clamp to +-65000, binary16 round to nearest even) of the library's
+    own 64-byte records, through both readback entry points; APIC values beyond the half range exercise the clamp."""
+    gpu, _ = make_pair("dam32_sphere")
+    for k in range(2):
+        gpu.step(DT, G)
+    recs = gpu.read_particles()
+    assert np.any(recs["c"] != 0.0)
+    assert_bits(gpu.read_particles(abi.LAYOUT_REF_48), O.pack_ref48(recs), "48-byte records after two steps")
+    big = rand_particles(24, 5000, seed=4, dead_frac=0.0, cscale=1e5)
+    big["c"][:10] = np.array([1e9, -1e9, 65000.0, 65520.0, -65519.0, 1 + 2 ** -11, 2 ** -24, 3e-8, -0.0],
+                             np.float32)
+    s = abi.make_settings((24, 24, 24), 1.0 / 24, 8000, solver=abi.SOLVER_NONE)
+    sim = Sim(s)
+    sim.write_particles(big)
+    assert_bits(sim.read_particles(abi.LAYOUT_REF_48), O.pack_ref48(big), "48-byte records of written particles")
 
 
 def test_large_sparse_grid_bitexact():
@@ -233,7 +283,7 @@ def test_mgpcg_parity(n, steps):
             assert rel <= 1e-12
             err = np.linalg.norm(pg - ref) / np.linalg.norm(ref)
             assert err <= 1e-5, f"pressure rel L2 error {err:.3e}"
-            assert orc.true_residual(pg) <= 1.2e-6
+            assert orc.true_residual(pg) <= 1e-6
             vg, vo = gpu.read_grid(abi.FIELD_VEL), orc.read_grid(abi.FIELD_VEL)
             assert rel_linf(vg, vo) <= 1e-5
             pgp, pop = gpu.read_particles(), orc.read_particles_brick_order()
@@ -267,7 +317,7 @@ def test_mgpcg_grid_shapes(grid):
             ref, _, rel = orc.solve_fp64(1e-13, 600)
             assert rel <= 1e-12
             assert np.linalg.norm(pg - ref) <= 1e-5 * np.linalg.norm(ref)
-            assert orc.true_residual(pg) <= 1.2e-6
+            assert orc.true_residual(pg) <= 1e-6
     gpu.close()
     orc.close()
 
@@ -310,7 +360,9 @@ def test_mgpcg_extrapolated_warm_start():
     ref, _, rel = orc.solve_fp64(1e-13, 600)
     assert rel <= 1e-12
     assert np.linalg.norm(pg - ref) <= 1e-5 * np.linalg.norm(ref)
-    assert orc.true_residual(pg) <= 1.2e-6
+    # after six steps the two systems differ in the last bits (the pressures of earlier steps met the tolerance
+    # differently): the library's true residual is checked on its own system
+    assert rg.true_residual_rel <= 1e-6 and system_residual(gpu, pg) <= 1e-6
     gpu.close()
     orc.close()
 
@@ -331,7 +383,7 @@ def test_rbsor_parity():
     pg = gpu.read_grid(abi.FIELD_PRESSURE_F64)
     ref, _, _ = orc.solve_fp64(1e-13, 400)
     assert np.linalg.norm(pg - ref) / np.linalg.norm(ref) <= 1e-5
-    assert orc.true_residual(pg) <= 1.5e-6
+    assert orc.true_residual(pg) <= 1e-6
 
 
 def test_gpu_repeatable():
@@ -373,7 +425,7 @@ def test_mgpcg_mostly_allocated_tank():
     ref, _, rel = orc.solve_fp64(1e-13, 600)
     assert rel <= 1e-12
     assert np.linalg.norm(pg - ref) / np.linalg.norm(ref) <= 1e-5
-    assert orc.true_residual(pg) <= 1.2e-6
+    assert orc.true_residual(pg) <= 1e-6
     assert rel_linf(gpu.read_grid(abi.FIELD_VEL), orc.read_grid(abi.FIELD_VEL)) <= 1e-5
     gpu.close()
     orc.close()
@@ -407,14 +459,14 @@ def test_mgpcg_preconditioner_options(flags, sweeps):
             ref, _, rel = orc.solve_fp64(1e-13, 600)
             assert rel <= 1e-12
             assert np.linalg.norm(pg - ref) / np.linalg.norm(ref) <= 1e-5
-            assert orc.true_residual(pg) <= 1.2e-6
+            assert orc.true_residual(pg) <= 1e-6
     gpu.close()
     orc.close()
 
 
 def test_long_run_dam_break():
     """300 steps of a 64^3 dam break with the tuned options: every solve converges with a true FP64 residual within
-    1.2e-6, no particle is lost, and the allocated brick count both grows and shrinks (the pool pass's count is read
+    1e-6, no particle is lost, and the allocated brick count both grows and shrinks (the pool pass's count is read
     back while P2G runs over a bound of it, with a second launch for the bricks past the bound)."""
     print("case long dam break 64", flush=True)
     grid, h, reqs = scenes.dam_break(64, seed=0)
@@ -428,7 +480,7 @@ def test_long_run_dam_break():
     for k in range(300):
         sim.step(DT, G)
         r = sim.results()
-        assert r.converged and r.residual_rel <= 1e-6 and r.true_residual_rel <= 1.2e-6, (k, r.residual_rel)
+        assert r.converged and r.residual_rel <= 1e-6 and r.true_residual_rel <= 1e-6, (k, r.residual_rel)
         n0 = r.live_particles if n0 is None else n0
         assert r.live_particles == n0, k
         bricks.append(r.active_bricks)
