@@ -200,6 +200,15 @@ slosh_status slosh_read_particles(slosh_ctx *ctx, void *dst, uint64_t dst_bytes,
 slosh_status slosh_read_grid(slosh_ctx *ctx, int32_t field, void *dst, uint64_t dst_bytes, int32_t order);
 /* = FSloshProxy::GetResults (SloshProxy.cpp:177-183), synchronous. */
 slosh_status slosh_get_results(slosh_ctx *ctx, slosh_results *out);
+/* No counterpart in the reference. For callers that cannot allocate device memory (tight render or serving loops,
+ * harnesses that forbid hipMalloc): the same copy as slosh_read_particles, staged through one fixed library-wide
+ * device buffer of SLOSH_NOALLOC_STAGING_BYTES per GPU (held until the process ends) instead of the context's
+ * growable staging buffer, so the call allocates nothing. Returns SLOSH_ECAPACITY, copying nothing, when the live
+ * particles times the record size exceed the buffer (1,048,576 records of 64 B, 1,398,101 of 48 B). Every context
+ * on a GPU shares the buffer: calls that overlap on one GPU (other host threads) race. */
+#define SLOSH_NOALLOC_STAGING_BYTES (64u << 20)
+slosh_status slosh_read_particles_noalloc(slosh_ctx *ctx, void *dst, uint64_t dst_bytes, uint32_t *count,
+                                          int32_t layout);
 /* Zero-copy access for a renderer: packs the particles into a device buffer of the given layout and returns it
  * (valid until the next call that changes particles). = GetParticleBuffer_RenderThread (SloshProxy.h:41). */
 slosh_status slosh_device_particles(slosh_ctx *ctx, int32_t layout, void **dev_ptr, uint32_t *count);

@@ -55,6 +55,8 @@ def load_library(path=LIB_PATH):
     lib.slosh_step_until.restype = st
     lib.slosh_read_particles.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint32), C.c_int32]
     lib.slosh_read_particles.restype = st
+    lib.slosh_read_particles_noalloc.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint32), C.c_int32]
+    lib.slosh_read_particles_noalloc.restype = st
     lib.slosh_read_grid.argtypes = [vp, C.c_int32, vp, C.c_uint64, C.c_int32]
     lib.slosh_read_grid.restype = st
     lib.slosh_get_results.argtypes = [vp, C.POINTER(abi.Results)]
@@ -173,6 +175,16 @@ class Sim:
         cnt = C.c_uint32(0)
         _check(self.lib.slosh_read_particles(self.h, out.ctypes.data, out.nbytes, C.byref(cnt), layout),
                "slosh_read_particles")
+        return out[: cnt.value]
+
+    def read_particles_noalloc(self, layout=abi.LAYOUT_F32_64):
+        """slosh_read_particles_noalloc: the same records, staged through the library's fixed device buffer."""
+        n = self.num_particles()
+        dt = abi.PARTICLE_DTYPE if layout == abi.LAYOUT_F32_64 else abi.PARTICLE_REF48_DTYPE
+        out = np.zeros(max(n, 1), dtype=dt)
+        cnt = C.c_uint32(0)
+        _check(self.lib.slosh_read_particles_noalloc(self.h, out.ctypes.data, out.nbytes, C.byref(cnt), layout),
+               "slosh_read_particles_noalloc")
         return out[: cnt.value]
 
     def read_grid(self, field):

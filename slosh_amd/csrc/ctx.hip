@@ -9,6 +9,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -587,6 +588,36 @@ slosh_status read_grid_impl(slosh_ctx *c, int32_t field, void *dst, uint64_t byt
 
 thread_local char g_info[256];
 
+// ---- staging of the allocation-free particle readback (slosh_read_particles_noalloc) ----
+// One library-wide device buffer per GPU, SLOSH_NOALLOC_STAGING_BYTES long, held from the module's load on that GPU
+// until the process ends. Host code never passes `scratch` itself (in host code the name is only a stand-in for
+// the symbol): each device's address comes from hipGetSymbolAddress, taken once per device in slosh_create with
+// that device current, and only that address reaches kernels and copies.
+static __device__ __align__(256) uint8_t scratch[SLOSH_NOALLOC_STAGING_BYTES];
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// Looks up (once) this device's address of the staging buffer; the caller has made `device` current.
+hipError_t scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void *scratch_of(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
+
 } // namespace
 
 #define ABI_GUARD(...)                                                                                           \
@@ -661,6 +692,13 @@ slosh_status slosh_create(const slosh_settings *s, slosh_ctx **out)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
         fprintf(stderr, "slosh: device %d is %s; the kernels are built for gfx950\n", dev, prop.gcnArchName);
         return SLOSH_EDEVICE;
+    }
+    {
+        const hipError_t e = scratch_init(dev);
+        if (e != hipSuccess) {
+            fprintf(stderr, "slosh: staging buffer address on device %d: %s\n", dev, hipGetErrorString(e));
+            return SLOSH_EDEVICE;
+        }
     }
     slosh_ctx *c = new (std::nothrow) slosh_ctx();
     if (!c) return SLOSH_ENOMEM;
@@ -745,6 +783,28 @@ slosh_status slosh_read_particles(slosh_ctx *c, void *dst, uint64_t dst_bytes, u
     if (!dst) return SLOSH_EINVAL;
     ABI_GUARD(ensure_staging(c, (size_t)c->np * rec); launch_pack_particles(c, c->staging, layout);
               SLOSH_HIP(hipMemcpyAsync(dst, c->staging, (size_t)c->np * rec, hipMemcpyDeviceToHost, c->stream));
+              SLOSH_HIP(hipStreamSynchronize(c->stream)); *count = c->np; return SLOSH_OK;)
+}
+
+slosh_status slosh_read_particles_noalloc(slosh_ctx *c, void *dst, uint64_t dst_bytes, uint32_t *count,
+                                          int32_t layout)
+{
+    if (!c || !count || (layout != SLOSH_LAYOUT_F32_64 && layout != SLOSH_LAYOUT_REF_48)) return SLOSH_EINVAL;
+    const size_t rec = layout == SLOSH_LAYOUT_F32_64 ? 64u : 48u;
+    *count = 0;
+    const uint64_t bytes = (uint64_t)c->np * rec;
+    if (bytes > (uint64_t)SLOSH_NOALLOC_STAGING_BYTES) {
+        fprintf(stderr, "slosh: slosh_read_particles_noalloc: %u particles of %zu bytes exceed the %u-byte staging "
+                        "buffer\n", c->np, rec, (unsigned)SLOSH_NOALLOC_STAGING_BYTES);
+        return SLOSH_ECAPACITY;
+    }
+    if (bytes > dst_bytes) return SLOSH_ECAPACITY;
+    if (c->np == 0) return SLOSH_OK;
+    if (!dst) return SLOSH_EINVAL;
+    void *stage = scratch_of(c->device);
+    if (!stage) return SLOSH_ESTATE; // slosh_create looked the address up on this context's device
+    ABI_GUARD(launch_pack_particles(c, stage, layout);
+              SLOSH_HIP(hipMemcpyAsync(dst, stage, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
               SLOSH_HIP(hipStreamSynchronize(c->stream)); *count = c->np; return SLOSH_OK;)
 }
 
