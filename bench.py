@@ -121,25 +121,64 @@ def copy_calibration(mib=512, reps=3):
     torch.cuda.synchronize()
 
 
-def cpu_baseline(workload, seconds_budget=60.0):
-    """One step of the oracle on the same workload (single thread), as steps/s."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_py as O
-    from slosh_amd import abi
+def host_cpu():
+    """Logical CPUs this process may run on, and the CPU model."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, model
 
-    grid, h, reqs, dt, grav = build_scene(workload)
+
+def _oracle_steps(O, abi, grid, h, reqs, dt, grav, steps, threads):
+    """Seconds per step of the C oracle over the first `steps` steps after emission, on `threads` threads."""
     n = sum(int(r.cell_dims[0]) * int(r.cell_dims[1]) * int(r.cell_dims[2]) * 8 for r in reqs)
-    s = abi.make_settings(grid, h, n)
-    o = O.OracleSim(s)
+    O.set_threads(threads)
+    o = O.OracleSim(abi.make_settings(grid, h, n))
     for r in reqs:
         o.emit(r)
     t0 = time.perf_counter()
-    o.step(dt, grav(0.0))
+    for k in range(steps):
+        o.step(dt, grav(k * dt))
     el = time.perf_counter() - t0
     o.close()
-    return {"value": 1.0 / el, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 step (the first after emission) of the {workload} workload on the C oracle, "
-                      f"single thread, {el:.1f} s"}
+    return el / steps
+
+
+def cpu_baseline(workload):
+    """The C oracle (the reference's kernels restated; OpenMP) on the same workload over its first steps after
+    emission, on all host threads (`value`) and on one, plus the 64^3 dam break over 10 steps on both."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    from slosh_amd import abi, scenes
+
+    nthreads = O.threads()
+    ncpu, model = host_cpu()
+    grid, h, reqs, dt, grav = build_scene(workload)
+    all_s = _oracle_steps(O, abi, grid, h, reqs, dt, grav, 3, nthreads)
+    one_s = _oracle_steps(O, abi, grid, h, reqs, dt, grav, 1, 1)
+    dgrid, dh, dreqs = scenes.dam_break(64, seed=0)
+    dgrav = lambda t: (0.0, -9.8, 0.0)  # noqa: E731
+    dam_all = _oracle_steps(O, abi, dgrid, dh, dreqs, 1.0 / 120.0, dgrav, 10, nthreads)
+    dam_one = _oracle_steps(O, abi, dgrid, dh, dreqs, 1.0 / 120.0, dgrav, 10, 1)
+    O.set_threads(nthreads)
+    return {"value": round(1.0 / all_s, 4), "unit": "steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"steps 1-3 after emission of the {workload} workload on the C oracle (OpenMP, "
+                      f"{nthreads} threads, {all_s:.2f} s per step)",
+            "one_thread": {"value": round(1.0 / one_s, 4), "unit": "steps/s", "sample": "step 1, 1 thread",
+                           "s_per_step": round(one_s, 2)},
+            "dam_break_64": {"sample": "64^3 dam break, 471,200 particles, MGPCG to 1e-6, steps 1-10",
+                             "all_threads_steps_per_s": round(1.0 / dam_all, 3),
+                             "one_thread_steps_per_s": round(1.0 / dam_one, 3)},
+            "host": {"logical_cpus": ncpu, "model": model}}
 
 
 def bench_other(args):

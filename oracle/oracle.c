@@ -26,6 +26,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define BRICK_CELLS 512u
 #define NO_BRICK 0xFFFFFFFFu
@@ -78,6 +81,7 @@ struct osim {
     double *pres; /* FP64: the pressure solution is the CG alpha-update accumulator */
     double *pres_prev; /* SLOSH_FLAG_WARM_START_EXTRAPOLATE: last solution per unit dt (NaN: none) */
     uint8_t *had;      /* scratch of stage_p2g: the cell was fluid in the last step */
+    uint8_t *xmark;    /* scratch of stage_extrapolate: faces filled by the current layer */
     float *div, *diag;
     /* solver */
     int nlevels;
@@ -88,6 +92,8 @@ struct osim {
     uint32_t cn;        /* fluid unknowns */
     uint32_t *cidx;     /* level index of unknown i */
     double *chol;       /* cn x cn lower factor */
+    double *part;       /* per-block partial sums of the parallel reductions (fixed block order) */
+    size_t part_cap;
     slosh_collider col[SLOSH_MAX_COLLIDERS];
     uint32_t ncol;
     slosh_results res;
@@ -264,6 +270,10 @@ static const int NB[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0
 /* ------------------------------------------------------------------------------------------------------------ */
 static void *xcalloc(size_t n, size_t sz) { return n ? calloc(n, sz) : calloc(1, sz); }
 
+/* Atomic integer add and byte store of the parallel particle loops (integer sums are order-free). */
+static void add_i32(int32_t *p, int32_t v) { __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
+static void set_u8(uint8_t *p) { __atomic_store_n(p, (uint8_t)1u, __ATOMIC_RELAXED); }
+
 static void free_levels(osim *o)
 {
     for (int l = 1; l < o->nlevels; ++l) {
@@ -281,9 +291,10 @@ void osim_destroy(osim *o)
     free(o->accum);
     for (int k = 0; k < 3; ++k) free(o->vel[k]);
     free(o->fflags); free(o->fweight); free(o->ctype); free(o->ccount);
-    free(o->pres); free(o->pres_prev); free(o->had); free(o->div); free(o->diag);
+    free(o->pres); free(o->pres_prev); free(o->had); free(o->xmark); free(o->div); free(o->diag);
     free(o->r); free(o->z); free(o->d); free(o->q);
     free(o->rd); free(o->zd); free(o->dd); free(o->qd); free(o->pd);
+    free(o->part);
     free_levels(o);
     free(o);
 }
@@ -334,6 +345,7 @@ osim *osim_create(const slosh_settings *s)
     }
     o->div = (float *)xcalloc(pc, 4);
     o->diag = (float *)xcalloc(pc, 4);
+    o->xmark = (uint8_t *)xcalloc(3 * pc, 1);
     o->r = (float *)xcalloc(pc, 4);
     o->z = (float *)xcalloc(pc, 4);
     o->d = (float *)xcalloc(pc, 4);
@@ -343,7 +355,7 @@ osim *osim_create(const slosh_settings *s)
     o->dd = (double *)xcalloc(pc, 8);
     o->qd = (double *)xcalloc(pc, 8);
     o->pd = (double *)xcalloc(pc, 8);
-    if (!o->p || !o->tmp || !o->table || !o->slot_brick || !o->accum || !o->vel[2] || !o->pd) {
+    if (!o->p || !o->tmp || !o->table || !o->slot_brick || !o->accum || !o->vel[2] || !o->pd || !o->xmark) {
         osim_destroy(o);
         return NULL;
     }
@@ -364,8 +376,38 @@ osim *osim_create(const slosh_settings *s)
         lv->bd = (double *)xcalloc(lv->cells, 8);
         o->nlevels++;
     }
+    {
+        size_t blocks = o->nbricks;
+        if (o->nlevels > 1 && (size_t)(o->lv[1].cells + 511u) / 512u > blocks) blocks = (o->lv[1].cells + 511u) / 512u;
+        o->part_cap = blocks + 1u;
+        o->part = (double *)xcalloc(4u * o->part_cap, 8);
+        if (!o->part) {
+            osim_destroy(o);
+            return NULL;
+        }
+    }
     o->res.solver_iterations = 0;
     return o;
+}
+
+/* Threads of the parallel loops (OpenMP); results do not depend on it: floating-point reductions sum fixed blocks
+ * of cells in a fixed order, integer ones are order-free. */
+void oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int oracle_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 int osim_set_colliders(osim *o, const slosh_collider *c, uint32_t n)
@@ -380,7 +422,7 @@ static void touch_pos(osim *o, const float p[3])
 {
     int c[3];
     pos_cell(o, p, c);
-    o->touch[brick_of_cell(o, c[0], c[1], c[2])] = 1u;
+    set_u8(&o->touch[brick_of_cell(o, c[0], c[1], c[2])]);
 }
 
 /* Slosh.usf:280-358 (EmitParticlesCS), mode 0 FillCells: particle I goes to cell I/8, sub-cell I%8 of a 2x2x2
@@ -398,6 +440,7 @@ int osim_emit(osim *o, const slosh_emit_request *r)
     float hi[3];
     for (int a = 0; a < 3; ++a) hi[a] = ((float)o->n[a] - 1.01f) * o->h;
     uint32_t dx = (uint32_t)r->cell_dims[0], dy = (uint32_t)r->cell_dims[1];
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < (uint32_t)count; ++i) {
         uint32_t h0 = oracle_pcg_hash(i * 3u + r->seed * 7919u);
         uint32_t h1 = oracle_pcg_hash(h0);
@@ -449,6 +492,7 @@ static void stage_sort(osim *o)
     uint32_t nk = o->nbricks + 1u;
     uint32_t *cnt = (uint32_t *)xcalloc(nk + 1u, 4);
     uint32_t *key = (uint32_t *)xcalloc(o->np, 4);
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < o->np; ++i) {
         uint32_t k = o->nbricks;
         if (o->p[i].age >= 0.0f) {
@@ -457,7 +501,7 @@ static void stage_sort(osim *o)
             k = brick_of_cell(o, c[0], c[1], c[2]);
         }
         key[i] = k;
-        cnt[k + 1u]++;
+        __atomic_fetch_add(&cnt[k + 1u], 1u, __ATOMIC_RELAXED);
     }
     for (uint32_t k = 0; k < nk; ++k) cnt[k + 1u] += cnt[k];
     uint32_t live = cnt[o->nbricks];
@@ -473,8 +517,9 @@ static void stage_sort(osim *o)
 /* Slosh.usf:1719-1831 (BrickFreeCS, BrickAllocCS, PoolArgsCS, ClearNewSlotsCS) with deterministic slots. */
 static void stage_pool(osim *o)
 {
-    int c[3];
+#pragma omp parallel for schedule(static)
     for (uint32_t b = 0; b < o->nbricks; ++b) {
+        int c[3];
         brick_coord(o, b, c);
         uint8_t w = 0;
         for (int dz = -1; dz <= 1 && !w; ++dz)
@@ -520,14 +565,16 @@ static void stage_pool(osim *o)
     o->res.pool_overflow_bricks = overflow;
 }
 
-/* Iterates the allocated cells in active-list order (Slosh.usf:155-188 SparseCell). */
-#define FOR_ACTIVE_CELLS(o, IDX, X, Y, Z)                                                    \
-    for (uint32_t e_ = 0; e_ < (o)->nactive; ++e_)                                            \
-        for (uint32_t l_ = 0; l_ < BRICK_CELLS; ++l_)                                         \
-            for (int bc_[3], once_ = (brick_coord((o), (o)->active[e_], bc_), 1); once_; once_ = 0) \
-                for (int X = bc_[0] * 8 + (int)(l_ & 7u), Y = bc_[1] * 8 + (int)((l_ >> 3) & 7u),   \
-                         Z = bc_[2] * 8 + (int)(l_ >> 6), ok_ = in_grid((o), X, Y, Z); ok_; ok_ = 0) \
-                    for (uint32_t IDX = (o)->table[(o)->active[e_]] * BRICK_CELLS + l_, k_ = 1; k_; k_ = 0)
+/* Iterates the cells of active brick E (Slosh.usf:155-188 SparseCell); FOR_ACTIVE_CELLS walks every allocated cell
+ * in active-list order. The parallel loops run FOR_BRICK_CELLS over the active bricks. */
+#define FOR_BRICK_CELLS(o, E, IDX, X, Y, Z)                                                                     \
+    for (uint32_t l_ = 0; l_ < BRICK_CELLS; ++l_)                                                                 \
+        for (int bc_[3], once_ = (brick_coord((o), (o)->active[E], bc_), 1); once_; once_ = 0)                     \
+            for (int X = bc_[0] * 8 + (int)(l_ & 7u), Y = bc_[1] * 8 + (int)((l_ >> 3) & 7u),                     \
+                     Z = bc_[2] * 8 + (int)(l_ >> 6), ok_ = in_grid((o), X, Y, Z); ok_; ok_ = 0)                   \
+                for (uint32_t IDX = (o)->table[(o)->active[E]] * BRICK_CELLS + l_, k_ = 1; k_; k_ = 0)
+#define FOR_ACTIVE_CELLS(o, IDX, X, Y, Z)                                                                       \
+    for (uint32_t e_ = 0; e_ < (o)->nactive; ++e_) FOR_BRICK_CELLS(o, e_, IDX, X, Y, Z)
 
 /* Slosh.usf:448-491 (ScatterComponentTiled) for one particle and one axis: 8 staggered nodes, int32 fixed point. */
 static void p2g_component(osim *o, int axis, const float xp[3], float up, const float cp[3])
@@ -560,8 +607,8 @@ static void p2g_component(osim *o, int axis, const float xp[3], float up, const 
         uint32_t cell = cell_index(o, node[0], node[1], node[2]);
         if (cell < BRICK_CELLS) continue; /* null brick: dropped */
         size_t f = (size_t)axis * o->pool_cells + cell;
-        o->accum[f] += mom;
-        o->accum[3 * o->pool_cells + f] += wt;
+        add_i32(&o->accum[f], mom);
+        add_i32(&o->accum[3 * o->pool_cells + f], wt);
     }
 }
 
@@ -571,7 +618,8 @@ static void p2g_component(osim *o, int axis, const float xp[3], float up, const 
  * linear extrapolation of the pressure per unit dt (q1 = P_last/dt_prev, q2 the value before). */
 static void stage_p2g(osim *o, float warm_scale, float dt, float dt_prev)
 {
-    FOR_ACTIVE_CELLS(o, idx, x, y, z)
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
     {
         if (o->had) o->had[idx] = (o->ctype[idx] & CELL_TYPE_MASK) == CELL_FLUID;
         uint32_t packed = CELL_AIR;
@@ -594,6 +642,7 @@ static void stage_p2g(osim *o, float warm_scale, float dt, float dt_prev)
     /* Build semantics: a particle deposits only when its own brick has storage (the per-brick gather runs over
      * allocated bricks). The reference also deposits the stencil of a particle in an unallocated brick onto
      * allocated neighbours; the two differ only after a pool overflow. */
+#pragma omp parallel for schedule(static, 4096)
     for (uint32_t i = 0; i < o->np; ++i) {
         const slosh_particle *pp = &o->p[i];
         if (!(pp->age >= 0.0f)) continue;
@@ -604,10 +653,11 @@ static void stage_p2g(osim *o, float warm_scale, float dt, float dt_prev)
         p2g_component(o, 0, pp->pos, pp->vel[0], &pp->c[0]);
         p2g_component(o, 1, pp->pos, pp->vel[1], &pp->c[3]);
         p2g_component(o, 2, pp->pos, pp->vel[2], &pp->c[6]);
-        o->ccount[ci] += 1;
+        add_i32(&o->ccount[ci], 1);
     }
     uint32_t fluid = 0;
-    FOR_ACTIVE_CELLS(o, idx, x, y, z)
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : fluid)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
     {
         (void)x; (void)y; (void)z;
         uint32_t packed = o->ctype[idx];
@@ -652,7 +702,8 @@ static void stage_normalize(osim *o, float dt, const float g[3])
 {
     float *vel = o->vel[0];
     o->cur = 0;
-    FOR_ACTIVE_CELLS(o, idx, x, y, z)
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
     {
         for (int a = 0; a < 3; ++a) {
             size_t f = (size_t)a * o->pool_cells + idx;
@@ -712,7 +763,8 @@ static float air_face_weight(const osim *o, float rho_fluid, int x, int y, int z
 static void stage_divergence(osim *o, float dt)
 {
     const float *vel = o->vel[o->cur];
-    FOR_ACTIVE_CELLS(o, idx, x, y, z)
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
     {
         o->ccount[idx] = 0;
         if ((o->ctype[idx] & CELL_TYPE_MASK) != CELL_FLUID) {
@@ -775,6 +827,7 @@ static void mg_build_types(osim *o)
     const int fluid_first = (o->s.flags & SLOSH_FLAG_MG_FLUID_COARSENING) != 0;
     for (int l = 1; l < o->nlevels; ++l) {
         mglevel *lv = &o->lv[l];
+#pragma omp parallel for schedule(dynamic, 1)
         for (int z = 0; z < lv->dim[2]; ++z)
             for (int y = 0; y < lv->dim[1]; ++y)
                 for (int x = 0; x < lv->dim[0]; ++x) {
@@ -793,14 +846,15 @@ static void mg_build_types(osim *o)
     }
 }
 
-/* Cells of a level, in a fixed order: level 0 through the active list, dense levels x fastest. */
-typedef struct lv_iter { int l, x, y, z; uint32_t idx; uint32_t e, li; } lv_iter;
+/* Cells of a level, in a fixed order: level 0 through the active list, dense levels x fastest. `end` (when not 0)
+ * bounds the walk to one block: an active brick of level 0, or 512 consecutive cells of a dense level. */
+typedef struct lv_iter { int l, x, y, z; uint32_t idx; uint32_t e, li, end; } lv_iter;
 
 static int lv_next(const osim *o, lv_iter *it)
 {
     if (it->l == 0) {
         for (;;) {
-            if (it->e >= o->nactive) return 0;
+            if (it->e >= o->nactive || (it->end && it->e >= it->end)) return 0;
             uint32_t l = it->li++;
             uint32_t e = it->e;
             if (it->li >= BRICK_CELLS) { it->li = 0; it->e++; }
@@ -815,13 +869,32 @@ static int lv_next(const osim *o, lv_iter *it)
         }
     }
     const mglevel *lv = &o->lv[it->l];
-    if (it->e >= lv->cells) return 0;
+    if (it->e >= lv->cells || (it->end && it->e >= it->end)) return 0;
     uint32_t i = it->e++;
     it->idx = i;
     it->x = (int)(i % (uint32_t)lv->dim[0]);
     it->y = (int)((i / (uint32_t)lv->dim[0]) % (uint32_t)lv->dim[1]);
     it->z = (int)(i / (uint32_t)(lv->dim[0] * lv->dim[1]));
     return 1;
+}
+
+/* Blocks of level l for the parallel loops, and the iterator of block b. */
+static uint32_t lv_nblocks(const osim *o, int l) { return l == 0 ? o->nactive : (o->lv[l].cells + 511u) / 512u; }
+
+static lv_iter lv_block(int l, uint32_t b)
+{
+    lv_iter it = {l, 0, 0, 0, 0, 0, 0, 0};
+    it.e = l == 0 ? b : b * 512u;
+    it.end = l == 0 ? b + 1u : b * 512u + 512u;
+    return it;
+}
+
+/* Sum of the first n block partials of column k (stride 4), in block order. */
+static double part_sum(const osim *o, uint32_t n, int k)
+{
+    double s = 0.0;
+    for (uint32_t b = 0; b < n; ++b) s += o->part[4u * b + (uint32_t)k];
+    return s;
 }
 
 static int is_red(int x, int y, int z) { return (((unsigned)(x + y + z)) & 1u) == 0u; }
@@ -836,7 +909,7 @@ static void coarse_factor(osim *o)
     uint32_t *cx = (uint32_t *)xcalloc(cells, 4);
     int *pos = (int *)xcalloc((size_t)cells * 3u, sizeof(int));
     uint32_t n = 0;
-    lv_iter it = {l, 0, 0, 0, 0, 0, 0};
+    lv_iter it = {l, 0, 0, 0, 0, 0, 0, 0};
     while (lv_next(o, &it)) {
         if (lv_type(o, l, it.x, it.y, it.z) != CELL_FLUID) continue;
         cx[n] = it.idx;
@@ -930,41 +1003,52 @@ static double apply_row_d(const osim *o, int x, int y, int z, uint32_t idx, cons
  * returns ||r||^2 and writes ||b||^2. */
 static double mgpcg_residual(osim *o, double *bb_out)
 {
-    double bb = 0.0, rr = 0.0;
-    lv_iter it = {0, 0, 0, 0, 0, 0, 0};
-    while (lv_next(o, &it)) {
-        uint32_t i = it.idx;
-        if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) {
-            o->r[i] = 0.0f; o->z[i] = 0.0f; o->d[i] = 0.0f;
-            continue;
+    const uint32_t nb = lv_nblocks(o, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint32_t blk = 0; blk < nb; ++blk) {
+        double bb = 0.0, rr = 0.0;
+        lv_iter it = lv_block(0, blk);
+        while (lv_next(o, &it)) {
+            uint32_t i = it.idx;
+            if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) {
+                o->r[i] = 0.0f; o->z[i] = 0.0f; o->d[i] = 0.0f;
+                continue;
+            }
+            double dgd, sumd;
+            stencil_d(o, 0, it.x, it.y, it.z, i, o->pres, &dgd, &sumd);
+            float ri = (float)(-(double)o->div[i] - (dgd * o->pres[i] - sumd));
+            float dg = o->diag[i] + DIAG_EPS;
+            o->r[i] = ri;
+            o->z[i] = is_red(it.x, it.y, it.z) ? ri / dg : 0.0f;
+            bb += (double)o->div[i] * (double)o->div[i];
+            rr += (double)ri * (double)ri;
         }
-        double dgd, sumd;
-        stencil_d(o, 0, it.x, it.y, it.z, i, o->pres, &dgd, &sumd);
-        float ri = (float)(-(double)o->div[i] - (dgd * o->pres[i] - sumd));
-        float dg = o->diag[i] + DIAG_EPS;
-        o->r[i] = ri;
-        o->z[i] = is_red(it.x, it.y, it.z) ? ri / dg : 0.0f;
-        bb += (double)o->div[i] * (double)o->div[i];
-        rr += (double)ri * (double)ri;
+        o->part[4u * blk] = bb;
+        o->part[4u * blk + 1u] = rr;
     }
-    *bb_out = bb;
-    return rr;
+    *bb_out = part_sum(o, nb, 0);
+    return part_sum(o, nb, 1);
 }
 
 /* ||b - A p||^2 of the FP64 pressure in FP64 (the true residual that decides convergence). */
 static double mgpcg_true_rr(osim *o)
 {
-    double rr = 0.0;
-    lv_iter it = {0, 0, 0, 0, 0, 0, 0};
-    while (lv_next(o, &it)) {
-        uint32_t i = it.idx;
-        if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-        double dgd, sumd;
-        stencil_d(o, 0, it.x, it.y, it.z, i, o->pres, &dgd, &sumd);
-        double ri = -(double)o->div[i] - (dgd * o->pres[i] - sumd);
-        rr += ri * ri;
+    const uint32_t nb = lv_nblocks(o, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint32_t blk = 0; blk < nb; ++blk) {
+        double rr = 0.0;
+        lv_iter it = lv_block(0, blk);
+        while (lv_next(o, &it)) {
+            uint32_t i = it.idx;
+            if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+            double dgd, sumd;
+            stencil_d(o, 0, it.x, it.y, it.z, i, o->pres, &dgd, &sumd);
+            double ri = -(double)o->div[i] - (dgd * o->pres[i] - sumd);
+            rr += ri * ri;
+        }
+        o->part[4u * blk] = rr;
     }
-    return rr;
+    return part_sum(o, nb, 0);
 }
 
 /* SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG) with the kernels of SloshPressureMG.usf:284-543, restated with
@@ -991,54 +1075,70 @@ static void solve_mgpcg(osim *o)
         return;
     }
     int conv = 0;
+    const uint32_t nb = lv_nblocks(o, 0);
     for (int restart = 0;; ++restart) {
-        lv_iter it;
         double rho = vcycle_f(o);
-        it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-        while (lv_next(o, &it))
-            if ((o->ctype[it.idx] & CELL_TYPE_MASK) == CELL_FLUID) o->d[it.idx] = o->z[it.idx];
+#pragma omp parallel for schedule(dynamic, 16)
+        for (uint32_t blk = 0; blk < nb; ++blk) {
+            lv_iter it = lv_block(0, blk);
+            while (lv_next(o, &it))
+                if ((o->ctype[it.idx] & CELL_TYPE_MASK) == CELL_FLUID) o->d[it.idx] = o->z[it.idx];
+        }
         int rec = 0;
         while (iters < maxit) {
-            double dq = 0.0;
-            it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-            while (lv_next(o, &it)) {
-                uint32_t i = it.idx;
-                if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-                /* q = A d with FP64 accumulation of the FP32 d (the terms cancel for smooth d) */
-                double sum = 0.0;
-                for (int nb = 0; nb < 6; ++nb) {
-                    int nx = it.x + NB[nb][0], ny = it.y + NB[nb][1], nz = it.z + NB[nb][2];
-                    if (lv_type(o, 0, nx, ny, nz) == CELL_FLUID) sum += (double)o->d[cell_index(o, nx, ny, nz)];
+#pragma omp parallel for schedule(dynamic, 16)
+            for (uint32_t blk = 0; blk < nb; ++blk) {
+                double dq = 0.0;
+                lv_iter it = lv_block(0, blk);
+                while (lv_next(o, &it)) {
+                    uint32_t i = it.idx;
+                    if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                    /* q = A d with FP64 accumulation of the FP32 d (the terms cancel for smooth d) */
+                    double sum = 0.0;
+                    for (int k = 0; k < 6; ++k) {
+                        int nx = it.x + NB[k][0], ny = it.y + NB[k][1], nz = it.z + NB[k][2];
+                        if (lv_type(o, 0, nx, ny, nz) == CELL_FLUID) sum += (double)o->d[cell_index(o, nx, ny, nz)];
+                    }
+                    float dg = o->diag[i] + DIAG_EPS;
+                    double qd = (double)dg * (double)o->d[i] - sum;
+                    o->q[i] = (float)qd;
+                    dq += (double)o->d[i] * qd;
                 }
-                float dg = o->diag[i] + DIAG_EPS;
-                double qd = (double)dg * (double)o->d[i] - sum;
-                o->q[i] = (float)qd;
-                dq += (double)o->d[i] * qd;
+                o->part[4u * blk] = dq;
             }
+            const double dq = part_sum(o, nb, 0);
             double alpha = fabs(dq) > 1e-30 ? rho / dq : 0.0;
-            rr = 0.0;
-            it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-            while (lv_next(o, &it)) {
-                uint32_t i = it.idx;
-                if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-                o->pres[i] = o->pres[i] + alpha * (double)o->d[i];
-                float ri = (float)((double)o->r[i] - alpha * (double)o->q[i]);
-                o->r[i] = ri;
-                rr += (double)ri * (double)ri;
-                float dg = o->diag[i] + DIAG_EPS;
-                o->z[i] = is_red(it.x, it.y, it.z) ? ri / dg : 0.0f;
+#pragma omp parallel for schedule(dynamic, 16)
+            for (uint32_t blk = 0; blk < nb; ++blk) {
+                double rrb = 0.0;
+                lv_iter it = lv_block(0, blk);
+                while (lv_next(o, &it)) {
+                    uint32_t i = it.idx;
+                    if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                    o->pres[i] = o->pres[i] + alpha * (double)o->d[i];
+                    float ri = (float)((double)o->r[i] - alpha * (double)o->q[i]);
+                    o->r[i] = ri;
+                    rrb += (double)ri * (double)ri;
+                    float dg = o->diag[i] + DIAG_EPS;
+                    o->z[i] = is_red(it.x, it.y, it.z) ? ri / dg : 0.0f;
+                }
+                o->part[4u * blk] = rrb;
             }
+            rr = part_sum(o, nb, 0);
             iters++;
             rel = sqrt(rr) / bnorm;
             if (rel <= tol) { rec = 1; break; }
             double rho_new = vcycle_f(o);
             double beta = fabs(rho) > 1e-30 ? rho_new / rho : 0.0;
             rho = rho_new;
-            it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-            while (lv_next(o, &it)) {
-                uint32_t i = it.idx;
-                if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-                o->d[i] = (float)((double)o->z[i] + beta * (double)o->d[i]);
+#pragma omp parallel for schedule(dynamic, 16)
+            for (uint32_t blk = 0; blk < nb; ++blk) {
+                lv_iter it = lv_block(0, blk);
+                while (lv_next(o, &it)) {
+                    uint32_t i = it.idx;
+                    if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                    o->d[i] = (float)((double)o->z[i] + beta * (double)o->d[i]);
+                }
             }
         }
         const double rel_true = sqrt(mgpcg_true_rr(o)) / bnorm;
@@ -1058,44 +1158,57 @@ static void solve_rbsor(osim *o)
     double tol = (double)o->s.tolerance;
     int maxit = o->s.max_iterations > 0 ? o->s.max_iterations : 1000;
     float w = o->s.sor_weight;
-    double bb = 0.0;
-    lv_iter it = {0, 0, 0, 0, 0, 0, 0};
-    while (lv_next(o, &it))
-        if ((o->ctype[it.idx] & CELL_TYPE_MASK) == CELL_FLUID) bb += (double)o->div[it.idx] * (double)o->div[it.idx];
-    double bnorm = sqrt(bb);
+    const uint32_t nb = lv_nblocks(o, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint32_t blk = 0; blk < nb; ++blk) {
+        double bbb = 0.0;
+        lv_iter it = lv_block(0, blk);
+        while (lv_next(o, &it))
+            if ((o->ctype[it.idx] & CELL_TYPE_MASK) == CELL_FLUID) bbb += (double)o->div[it.idx] * (double)o->div[it.idx];
+        o->part[4u * blk] = bbb;
+    }
+    double bnorm = sqrt(part_sum(o, nb, 0));
     o->res.rhs_norm = bnorm;
     int iters = 0, conv = 0;
     double rel = 0.0;
     for (;;) {
-        double rr = 0.0;
-        it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-        while (lv_next(o, &it)) {
-            uint32_t i = it.idx;
-            if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-            double dg, sum;
-            stencil_d(o, 0, it.x, it.y, it.z, i, o->pres, &dg, &sum);
-            double ri = -(double)o->div[i] - (dg * o->pres[i] - sum);
-            rr += ri * ri;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (uint32_t blk = 0; blk < nb; ++blk) {
+            double rrb = 0.0;
+            lv_iter it = lv_block(0, blk);
+            while (lv_next(o, &it)) {
+                uint32_t i = it.idx;
+                if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                double dg, sum;
+                stencil_d(o, 0, it.x, it.y, it.z, i, o->pres, &dg, &sum);
+                double ri = -(double)o->div[i] - (dg * o->pres[i] - sum);
+                rrb += ri * ri;
+            }
+            o->part[4u * blk] = rrb;
         }
+        double rr = part_sum(o, nb, 0);
         rel = bnorm > 0.0 ? sqrt(rr) / bnorm : 0.0;
         if (rel <= tol) { conv = 1; break; }
         if (iters >= maxit) break;
         for (int k = 0; k < 8 && iters < maxit; ++k, ++iters) {
             for (unsigned colour = 0; colour < 2u; ++colour) {
-                it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-                while (lv_next(o, &it)) {
-                    uint32_t i = it.idx;
-                    if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-                    if ((((unsigned)(it.x + it.y + it.z)) & 1u) != colour) continue;
-                    double sum = 0.0;
-                    for (int nb = 0; nb < 6; ++nb) {
-                        int nx = it.x + NB[nb][0], ny = it.y + NB[nb][1], nz = it.z + NB[nb][2];
-                        if (lv_type(o, 0, nx, ny, nz) == CELL_FLUID) sum += o->pres[cell_index(o, nx, ny, nz)];
+#pragma omp parallel for schedule(dynamic, 16)
+                for (uint32_t blk = 0; blk < nb; ++blk) {
+                    lv_iter it = lv_block(0, blk);
+                    while (lv_next(o, &it)) {
+                        uint32_t i = it.idx;
+                        if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                        if ((((unsigned)(it.x + it.y + it.z)) & 1u) != colour) continue;
+                        double sum = 0.0;
+                        for (int q = 0; q < 6; ++q) {
+                            int nx = it.x + NB[q][0], ny = it.y + NB[q][1], nz = it.z + NB[q][2];
+                            if (lv_type(o, 0, nx, ny, nz) == CELL_FLUID) sum += o->pres[cell_index(o, nx, ny, nz)];
+                        }
+                        double dg = (double)(o->diag[i] + DIAG_EPS);
+                        double gs = (sum - (double)o->div[i]) / dg;
+                        double old = o->pres[i];
+                        o->pres[i] = old + (double)w * (gs - old);
                     }
-                    double dg = (double)(o->diag[i] + DIAG_EPS);
-                    double gs = (sum - (double)o->div[i]) / dg;
-                    double old = o->pres[i];
-                    o->pres[i] = old + (double)w * (gs - old);
                 }
             }
         }
@@ -1103,13 +1216,15 @@ static void solve_rbsor(osim *o)
     o->res.solver_iterations = iters;
     o->res.residual_rel = rel;
     o->res.converged = conv;
+    o->res.true_residual_rel = (float)rel;
 }
 
 /* Slosh.usf:995-1052 (ProjectFace) without the collider-coupling accumulations. */
 static void stage_project(osim *o)
 {
     float *vel = o->vel[o->cur];
-    FOR_ACTIVE_CELLS(o, idx, x, y, z)
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
     {
         for (int a = 0; a < 3; ++a) {
             size_t f = (size_t)a * o->pool_cells + idx;
@@ -1141,22 +1256,27 @@ static void stage_project(osim *o)
     }
     /* Slosh.usf:1222-1240 (ResidualCS): post-projection |div|/h over fluid cells. */
     float mx = 0.0f;
-    double sum = 0.0;
     uint32_t cnt = 0;
-    FOR_ACTIVE_CELLS(o, idx, x, y, z)
-    {
-        if ((o->ctype[idx] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-        float dv = 0.0f;
-        for (int a = 0; a < 3; ++a) {
-            int p[3] = {x, y, z};
-            p[a] += 1;
-            dv += vel[face_index(o, a, p[0], p[1], p[2])] - vel[face_index(o, a, x, y, z)];
+#pragma omp parallel for schedule(dynamic, 4) reduction(max : mx) reduction(+ : cnt)
+    for (uint32_t e = 0; e < o->nactive; ++e) {
+        double bsum = 0.0;
+        FOR_BRICK_CELLS(o, e, idx, x, y, z)
+        {
+            if ((o->ctype[idx] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+            float dv = 0.0f;
+            for (int a = 0; a < 3; ++a) {
+                int p[3] = {x, y, z};
+                p[a] += 1;
+                dv += vel[face_index(o, a, p[0], p[1], p[2])] - vel[face_index(o, a, x, y, z)];
+            }
+            float rate = fabsf(dv) * o->inv_h;
+            if (rate > mx) mx = rate;
+            bsum += (double)rate;
+            cnt++;
         }
-        float rate = fabsf(dv) * o->inv_h;
-        if (rate > mx) mx = rate;
-        sum += (double)rate;
-        cnt++;
+        o->part[4u * e] = bsum;
     }
+    const double sum = part_sum(o, o->nactive, 0);
     o->res.max_divergence = mx;
     o->res.mean_divergence = cnt ? (float)(sum / (double)cnt) : 0.0f;
 }
@@ -1169,7 +1289,10 @@ static void stage_extrapolate(osim *o, uint32_t layer)
     float *dst = o->vel[dsti];
     uint32_t known = FACE_VALID;
     for (uint32_t k = 0; k < layer; ++k) known |= 1u << (FACE_EXTRAP_SHIFT + k);
-    FOR_ACTIVE_CELLS(o, idx, x, y, z)
+    const uint32_t bit = 1u << (FACE_EXTRAP_SHIFT + layer);
+    /* faces filled by this layer get their bit after the pass: the pass reads only the bits of earlier layers */
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
     {
         for (int a = 0; a < 3; ++a) {
             size_t f = (size_t)a * o->pool_cells + idx;
@@ -1192,9 +1315,21 @@ static void stage_extrapolate(osim *o, uint32_t layer)
             }
             if (cnt > 0.0f) {
                 dst[f] = sum / cnt;
-                o->fflags[f] |= 1u << (FACE_EXTRAP_SHIFT + layer);
+                o->xmark[f] = 1u;
             } else {
                 dst[f] = 0.0f;
+            }
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
+    {
+        (void)x; (void)y; (void)z;
+        for (int a = 0; a < 3; ++a) {
+            size_t f = (size_t)a * o->pool_cells + idx;
+            if (o->xmark[f]) {
+                o->fflags[f] |= bit;
+                o->xmark[f] = 0u;
             }
         }
     }
@@ -1275,6 +1410,7 @@ static void stage_g2p(osim *o, float dt, const float grav[3])
     float radius = o->s.particle_radius_cells * o->h;
     float maxspeed = 0.0f;
     uint32_t clamped = 0;
+#pragma omp parallel for schedule(static, 4096) reduction(max : maxspeed) reduction(+ : clamped)
     for (uint32_t i = 0; i < o->np; ++i) {
         slosh_particle *pp = &o->p[i];
         if (!(pp->age >= 0.0f)) continue;
@@ -1459,64 +1595,86 @@ int osim_solve_fp64(const osim *oc, double rel_tol, int max_iter, double *dense_
     osim *o = (osim *)oc; /* scratch vectors only; no simulation state changes */
     mg_build_types(o);
     coarse_factor(o);
-    double bb = 0.0;
-    lv_iter it = {0, 0, 0, 0, 0, 0, 0};
-    while (lv_next(o, &it)) {
-        uint32_t i = it.idx;
-        o->pd[i] = 0.0;
-        o->dd[i] = 0.0;
-        o->qd[i] = 0.0;
-        if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) {
-            o->rd[i] = 0.0; o->zd[i] = 0.0;
-            continue;
+    const uint32_t nb = lv_nblocks(o, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint32_t blk = 0; blk < nb; ++blk) {
+        double bbb = 0.0;
+        lv_iter it = lv_block(0, blk);
+        while (lv_next(o, &it)) {
+            uint32_t i = it.idx;
+            o->pd[i] = 0.0;
+            o->dd[i] = 0.0;
+            o->qd[i] = 0.0;
+            if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) {
+                o->rd[i] = 0.0; o->zd[i] = 0.0;
+                continue;
+            }
+            double b = -(double)o->div[i];
+            o->rd[i] = b;
+            double dg = (double)(o->diag[i] + DIAG_EPS);
+            o->zd[i] = is_red(it.x, it.y, it.z) ? b / dg : 0.0;
+            bbb += b * b;
         }
-        double b = -(double)o->div[i];
-        o->rd[i] = b;
-        double dg = (double)(o->diag[i] + DIAG_EPS);
-        o->zd[i] = is_red(it.x, it.y, it.z) ? b / dg : 0.0;
-        bb += b * b;
+        o->part[4u * blk] = bbb;
     }
+    double bb = part_sum(o, nb, 0);
     double bnorm = sqrt(bb);
     int iters = 0;
     double rel = 0.0;
     if (bnorm > 0.0) {
         double rho = vcycle_d(o);
-        it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-        while (lv_next(o, &it)) o->dd[it.idx] = o->zd[it.idx];
+#pragma omp parallel for schedule(dynamic, 16)
+        for (uint32_t blk = 0; blk < nb; ++blk) {
+            lv_iter it = lv_block(0, blk);
+            while (lv_next(o, &it)) o->dd[it.idx] = o->zd[it.idx];
+        }
         for (iters = 1; iters <= max_iter; ++iters) {
-            double dq = 0.0;
-            it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-            while (lv_next(o, &it)) {
-                uint32_t i = it.idx;
-                if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-                o->qd[i] = apply_row_d(o, it.x, it.y, it.z, i, o->dd);
-                dq += o->dd[i] * o->qd[i];
+#pragma omp parallel for schedule(dynamic, 16)
+            for (uint32_t blk = 0; blk < nb; ++blk) {
+                double dqb = 0.0;
+                lv_iter it = lv_block(0, blk);
+                while (lv_next(o, &it)) {
+                    uint32_t i = it.idx;
+                    if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                    o->qd[i] = apply_row_d(o, it.x, it.y, it.z, i, o->dd);
+                    dqb += o->dd[i] * o->qd[i];
+                }
+                o->part[4u * blk] = dqb;
             }
+            double dq = part_sum(o, nb, 0);
             double alpha = rho / dq;
-            double rr = 0.0;
-            it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-            while (lv_next(o, &it)) {
-                uint32_t i = it.idx;
-                if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-                o->pd[i] += alpha * o->dd[i];
-                o->rd[i] -= alpha * o->qd[i];
-                rr += o->rd[i] * o->rd[i];
-                double dg = (double)(o->diag[i] + DIAG_EPS);
-                o->zd[i] = is_red(it.x, it.y, it.z) ? o->rd[i] / dg : 0.0;
+#pragma omp parallel for schedule(dynamic, 16)
+            for (uint32_t blk = 0; blk < nb; ++blk) {
+                double rrb = 0.0;
+                lv_iter it = lv_block(0, blk);
+                while (lv_next(o, &it)) {
+                    uint32_t i = it.idx;
+                    if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+                    o->pd[i] += alpha * o->dd[i];
+                    o->rd[i] -= alpha * o->qd[i];
+                    rrb += o->rd[i] * o->rd[i];
+                    double dg = (double)(o->diag[i] + DIAG_EPS);
+                    o->zd[i] = is_red(it.x, it.y, it.z) ? o->rd[i] / dg : 0.0;
+                }
+                o->part[4u * blk] = rrb;
             }
+            double rr = part_sum(o, nb, 0);
             rel = sqrt(rr) / bnorm;
             if (rel <= rel_tol) break;
             double rho_new = vcycle_d(o);
             double beta = rho_new / rho;
             rho = rho_new;
-            it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-            while (lv_next(o, &it)) o->dd[it.idx] = o->zd[it.idx] + beta * o->dd[it.idx];
+#pragma omp parallel for schedule(dynamic, 16)
+            for (uint32_t blk = 0; blk < nb; ++blk) {
+                lv_iter it = lv_block(0, blk);
+                while (lv_next(o, &it)) o->dd[it.idx] = o->zd[it.idx] + beta * o->dd[it.idx];
+            }
         }
     }
     if (dense_out) {
         size_t cells = (size_t)o->n[0] * o->n[1] * o->n[2];
         memset(dense_out, 0, cells * sizeof(double));
-        it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
+        lv_iter it = {0, 0, 0, 0, 0, 0, 0, 0};
         while (lv_next(o, &it)) {
             if ((o->ctype[it.idx] & CELL_TYPE_MASK) != CELL_FLUID) continue;
             dense_out[(size_t)it.x + (size_t)o->n[0] * ((size_t)it.y + (size_t)o->n[1] * (size_t)it.z)] = o->pd[it.idx];
@@ -1535,21 +1693,31 @@ double osim_true_residual_d(const osim *oc, const double *dense_p) { return true
 static double true_residual_impl(const osim *oc, const float *dense_p, const double *dense_pd)
 {
     osim *o = (osim *)oc;
-    lv_iter it = {0, 0, 0, 0, 0, 0, 0};
-    while (lv_next(o, &it)) {
-        size_t di = (size_t)it.x + (size_t)o->n[0] * ((size_t)it.y + (size_t)o->n[1] * (size_t)it.z);
-        double pv = dense_p ? (double)dense_p[di] : (dense_pd ? dense_pd[di] : o->pres[it.idx]);
-        o->pd[it.idx] = (o->ctype[it.idx] & CELL_TYPE_MASK) == CELL_FLUID ? pv : 0.0;
+    const uint32_t nb = lv_nblocks(o, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint32_t blk = 0; blk < nb; ++blk) {
+        lv_iter it = lv_block(0, blk);
+        while (lv_next(o, &it)) {
+            size_t di = (size_t)it.x + (size_t)o->n[0] * ((size_t)it.y + (size_t)o->n[1] * (size_t)it.z);
+            double pv = dense_p ? (double)dense_p[di] : (dense_pd ? dense_pd[di] : o->pres[it.idx]);
+            o->pd[it.idx] = (o->ctype[it.idx] & CELL_TYPE_MASK) == CELL_FLUID ? pv : 0.0;
+        }
     }
-    double bb = 0.0, rr = 0.0;
-    it = (lv_iter){0, 0, 0, 0, 0, 0, 0};
-    while (lv_next(o, &it)) {
-        uint32_t i = it.idx;
-        if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-        double b = -(double)o->div[i];
-        double r = b - apply_row_d(o, it.x, it.y, it.z, i, o->pd);
-        bb += b * b;
-        rr += r * r;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint32_t blk = 0; blk < nb; ++blk) {
+        double bbb = 0.0, rrb = 0.0;
+        lv_iter it = lv_block(0, blk);
+        while (lv_next(o, &it)) {
+            uint32_t i = it.idx;
+            if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+            double b = -(double)o->div[i];
+            double r = b - apply_row_d(o, it.x, it.y, it.z, i, o->pd);
+            bbb += b * b;
+            rrb += r * r;
+        }
+        o->part[4u * blk] = bbb;
+        o->part[4u * blk + 1u] = rrb;
     }
+    const double bb = part_sum(o, nb, 0), rr = part_sum(o, nb, 1);
     return bb > 0.0 ? sqrt(rr / bb) : 0.0;
 }

@@ -44,6 +44,10 @@ double osim_true_residual(const osim *o, const float *dense_p);
 /* Same for an FP64 dense field. */
 double osim_true_residual_d(const osim *o, const double *dense_p);
 
+/* Threads of the parallel (OpenMP) loops; results are the same for any count. */
+void oracle_set_threads(int n);
+int oracle_threads(void);
+
 /* Building blocks exposed for known-answer tests. */
 uint32_t oracle_pcg_hash(uint32_t v);
 float oracle_hash_to_unit(uint32_t h);

@@ -67,6 +67,10 @@ def load():
     lib.oracle_collider_response.argtypes = [C.POINTER(abi.Collider), C.c_uint32, C.c_float, C.c_float, C.c_float,
                                              f3, f3]
     lib.oracle_collider_response.restype = None
+    lib.oracle_set_threads.argtypes = [C.c_int]
+    lib.oracle_set_threads.restype = None
+    lib.oracle_threads.argtypes = []
+    lib.oracle_threads.restype = C.c_int
     _lib = lib
     return lib
 
@@ -185,6 +189,15 @@ def sort_by_brick(parts, grid, cell_size):
     live = parts["age"] >= 0.0
     order = np.argsort(np.where(live, key, bd[0] * bd[1] * bd[2]), kind="stable")
     return parts[order[: int(live.sum())]]
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (results do not depend on the count)."""
+    load().oracle_set_threads(int(n))
+
+
+def threads():
+    return int(load().oracle_threads())
 
 
 def pcg_hash(v):
