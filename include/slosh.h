@@ -87,6 +87,11 @@ typedef struct slosh_settings {
                                                 instead of the last solution scaled by dt/dt_prev (the reference's
                                                 warm start, SloshProxy.cpp:575 applied in Slosh.usf:569-598, kept
                                                 on every other cell); the solve still stops at the tolerance */
+#define SLOSH_FLAG_REPORT_FP64_ERROR 16u /* MGPCG: after each solve, solve the same system again from its solution to
+                                            an FP64 true residual of 1e-12 (restarts form b - A p in FP64, so the FP32
+                                            vectors and FP16 levels do not limit it) and report the first solution's
+                                            relative L2 error against it (slosh_results.fp64_error_rel); the first
+                                            solution is kept. A diagnostic: it costs a few solves per step */
 
 /* Emission request: FSloshEmitRequest (SloshTypes.h:169-181). shape 0 = FillCells (8 jittered particles per cell). */
 enum { SLOSH_EMIT_FILL_CELLS = 0, SLOSH_EMIT_SPHERE = 1, SLOSH_EMIT_BOX = 2 };
@@ -162,14 +167,16 @@ typedef struct slosh_results {
     float mean_divergence;         /* 1/s */
     uint32_t clamped_particles;    /* particles above max_speed */
     int32_t solver_iterations;
-    int32_t converged;             /* 1 when ||r||/||b|| <= tolerance */
+    int32_t converged;             /* 1 when the FP64 true residual ||b - A p||/||b|| <= tolerance */
     double residual_rel;           /* final ||r||_2 / ||b||_2 of the solver's recurrence residual */
     double rhs_norm;               /* ||b||_2 */
     float step_ms;                 /* device time of the step (HIP events) */
     float solve_ms;                /* device time of the pressure solve */
     int32_t fallback_used;         /* flexible CG: 1 when MGPCG finished the solve, 2 when no network was set */
     float true_residual_rel;       /* ||b - A p||_2 / ||b||_2 of the FP64 pressure after the solve */
-    int32_t reserved[4];
+    double fp64_error_rel;         /* SLOSH_FLAG_REPORT_FP64_ERROR: ||p - p64|| / ||p64|| over fluid cells against the
+                                      FP64-accurate solve of the same system; -1 when not computed */
+    int32_t reserved[2];
 } slosh_results;
 
 /* ---- lifecycle --------------------------------------------------------------------------------------------- */

@@ -25,6 +25,7 @@ FLAG_CAPTURE_RESIDUALS = 1
 FLAG_MG_FP16 = 2
 FLAG_MG_FLUID_COARSENING = 4
 FLAG_WARM_START_EXTRAPOLATE = 8
+FLAG_REPORT_FP64_ERROR = 16
 
 EMIT_FILL_CELLS = 0
 EMIT_SPHERE = 1
@@ -147,7 +148,8 @@ class Results(C.Structure):
         ("solve_ms", C.c_float),
         ("fallback_used", C.c_int32),
         ("true_residual_rel", C.c_float),
-        ("reserved", C.c_int32 * 4),
+        ("fp64_error_rel", C.c_double),
+        ("reserved", C.c_int32 * 2),
     ]
 
     def as_dict(self):

@@ -77,7 +77,7 @@ void free_all(slosh_ctx *c)
         dfree(c->lv[l].x); dfree(c->lv[l].x1); dfree(c->lv[l].b);
     }
     dfree(c->dense_info); dfree(c->dense_coord);
-    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->coarse_sig); dfree(c->pres_prev); dfree(c->partials);
+    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->coarse_sig); dfree(c->pres_prev); dfree(c->pres_lp); dfree(c->state_save); dfree(c->partials);
     dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
     dfree(c->bp_clamp);
     dfree(c->state); dfree(c->stats); dfree(c->d_col);
@@ -116,7 +116,7 @@ bool settings_valid(const slosh_settings *s)
         for (int a = 0; a < 3; ++a)
             if (s->grid_dim[a] % 8 != 0) return false; // the network's levels need whole bricks
     if (s->flags & ~(uint32_t)(SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_MG_FP16 | SLOSH_FLAG_MG_FLUID_COARSENING |
-                               SLOSH_FLAG_WARM_START_EXTRAPOLATE))
+                               SLOSH_FLAG_WARM_START_EXTRAPOLATE | SLOSH_FLAG_REPORT_FP64_ERROR))
         return false;
     if (!(s->tolerance > 0.0f) || s->max_iterations < 0) return false;
     if (!(s->sor_weight >= 0.5f && s->sor_weight <= 1.95f)) return false;
@@ -207,6 +207,10 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->ctype, pc);
     dalloc(&c->pres, pc);
     if (s.flags & SLOSH_FLAG_WARM_START_EXTRAPOLATE) dalloc(&c->pres_prev, pc);
+    if (s.flags & SLOSH_FLAG_REPORT_FP64_ERROR) {
+        dalloc(&c->pres_lp, pc);
+        dalloc(&c->state_save, 1);
+    }
     dalloc(&c->div, pc);
     dalloc(&c->diag, pc);
     dalloc(&c->fmask, pc);
@@ -362,6 +366,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     c->res.solver_iterations = 0;
     c->res.residual_rel = 0.0;
     c->res.true_residual_rel = 0.0f;
+    c->res.fp64_error_rel = -1.0;
     c->res.converged = 1;
     c->res.solve_ms = 0.0f;
     launch_fill_u32(c->stream, (uint32_t *)c->stats, 0u, sizeof(StepStats) / 4u);
@@ -436,6 +441,8 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
                                                                                   : (float)c->h_state->rel_true;
             c->res.rhs_norm = sqrt(c->h_state->bb);
             c->res.converged = c->h_state->converged;
+            if ((c->s.flags & SLOSH_FLAG_REPORT_FP64_ERROR) && c->s.pressure_solver != SLOSH_SOLVER_RBSOR)
+                c->res.fp64_error_rel = c->h_state->err_rel;
         }
         float ms = 0.0f;
         SLOSH_HIP(hipEventElapsedTime(&ms, c->ev_solve0, c->ev_solve1));

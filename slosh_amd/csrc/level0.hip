@@ -490,6 +490,23 @@ __global__ __launch_bounds__(kThreads) void k_fcg_update(const uint32_t *binfo, 
     block_partial<1>(v, partials);
 }
 
+// Partials sum (p_ref - p)^2 and sum p^2 over fluid cells (the error of a kept solution against the current one).
+__global__ __launch_bounds__(kThreads) void k_error_partials(const uint32_t *binfo, const uint8_t *fmask,
+                                                             const double *pres, const double *p_ref,
+                                                             double *partials)
+{
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    double v[2] = {0.0, 0.0};
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t i = B.base + colour_cell(threadIdx.x, c);
+        if (!(fmask[i] & kSelfFluid)) continue;
+        const double e = p_ref[i] - pres[i];
+        v[0] += e * e;
+        v[1] += pres[i] * pres[i];
+    }
+    block_partial<2>(v, partials);
+}
+
 } // namespace
 
 #define L0_LAUNCH(kid, kernel, ...)                                                                              \
@@ -562,6 +579,11 @@ void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w)
 void l0_sor_residual(slosh_ctx *c)
 {
     L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 0);
+}
+
+void l0_error_partials(slosh_ctx *c, const double *p_ref)
+{
+    L0_LAUNCH(K_SOR_RESIDUAL, k_error_partials, c->pres, p_ref, c->partials);
 }
 
 void l0_true_residual(slosh_ctx *c)

@@ -31,6 +31,7 @@ struct SolverState {
     double rel_true; // ||b - A p|| / ||b|| of the last true-residual check
     double nf;       // fluid cells
     double mg_scale; // FP16 levels: the V-cycle input scale sqrt(nf / ||r||^2), so the levels hold O(1) values
+    double err_rel;  // SLOSH_FLAG_REPORT_FP64_ERROR: relative L2 error of the solution against the FP64-accurate one
     int32_t done;  // 1 once converged or out of iterations: every solver kernel then returns at once; 2 when the
                    // MGPCG update being made is the last (only the update pass runs; the next finalize sets 1)
     int32_t iters;
@@ -283,6 +284,8 @@ struct slosh_ctx {
     uint32_t *coarse_n = nullptr;
     double *coarse_inv = nullptr;   // [kCoarsestCells^2]
     double *pres_prev = nullptr;    // SLOSH_FLAG_WARM_START_EXTRAPOLATE: last solution per unit dt (NaN: none)
+    double *pres_lp = nullptr;      // SLOSH_FLAG_REPORT_FP64_ERROR: the solution kept while the FP64-accurate one is formed
+    slosh::SolverState *state_save = nullptr;
     uint32_t *coarse_sig = nullptr; // the coarsest level's cell types and indices at its last factorisation
     double *partials = nullptr;
     uint16_t *dense_info = nullptr;   // stencil words of all dense levels, level by level
@@ -403,6 +406,8 @@ void l0_true_residual(slosh_ctx *c);
 void l0_fcg_zq(slosh_ctx *c, const float *z);
 void l0_fcg_direction(slosh_ctx *c, const float *z, const float *d_in, float *d_out, const float *r, int first);
 void l0_fcg_update(slosh_ctx *c, float *r, const float *d);
+// Partials sum (p_ref - pres)^2 and sum pres^2 over fluid cells, pres the current FP64 pressure.
+void l0_error_partials(slosh_ctx *c, const double *p_ref);
 
 // ---- unet.hip ----
 size_t unet_param_count();

@@ -51,13 +51,13 @@ template <typename V> __device__ __forceinline__ float dsum(const DenseLevel &L,
 }
 
 enum FinKind { kFinInit = 0, kFinRho = 1, kFinAlphaStop = 2, kFinRr = 3, kFinSor = 4, kFinBetaF = 5,
-               kFinAlphaF = 6, kFinCheck = 7, kFinRestart = 8 };
+               kFinAlphaF = 6, kFinCheck = 7, kFinRestart = 8, kFinErr = 9 };
 
 __host__ __device__ constexpr int fin_values(int kind)
 {
     if (kind == kFinAlphaStop) return 4;
     if (kind == kFinInit || kind == kFinRestart) return 3;
-    if (kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck) return 2;
+    if (kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck || kind == kFinErr) return 2;
     return 1;
 }
 
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         for (int j = 0; j < kMaxV; ++j) v[k][j] = (b < nblocks && j < nv) ? partials[(size_t)b * nv + j] : 0.0;
     }
     issue_point();
-    if (kind != kFinInit && kind != kFinCheck && kind != kFinRestart && done) {
+    if (kind != kFinInit && kind != kFinCheck && kind != kFinRestart && kind != kFinErr && done) {
         // the update that met the stopping test has run: from here on every pass returns at once
         if (kind == kFinAlphaStop && done == 2u && t == 0) st->done = 1;
         return;
@@ -174,6 +174,9 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         // flexible CG: alpha = (d . r) / (d . q)
         st->dq = t0;
         st->alpha = fabs(t0) > 1e-30 ? t1 / t0 : 0.0;
+    } else if (kind == kFinErr) {
+        // t0 = sum (p_ref - p)^2, t1 = sum p^2 with p the FP64-accurate solution
+        st->err_rel = t1 > 0.0 ? sqrt(t0 / t1) : 0.0;
     } else if (kind == kFinCheck) {
         // the FP64 true residual ||b - A p|| / ||b|| of the solution: the solve counts as converged only when it
         // meets the tolerance (the recurrence residual that stopped the iteration can differ from it)
@@ -1051,6 +1054,7 @@ __global__ void k_state_reset(SolverState *st, double tol, int maxit)
     st->beta = 0.0;
     st->rel = 0.0;
     st->rel_true = 0.0;
+    st->err_rel = -1.0;
 }
 
 // Reduction of the partials of the preceding level-0 pass: one per brick with fluid, except the initial residual,
@@ -1245,30 +1249,21 @@ template <typename F> bool capture_chunk_graph(slosh_ctx *c, F body)
 
 } // namespace
 
-// SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG), to ||r|| <= tol ||b||.
-void run_mgpcg(slosh_ctx *c)
+// The MGPCG iterations of a solve whose residual r = b - A p is in c->r and whose state holds the initial
+// reduction: chunks of iterations between reads of the state, then the true-residual check, restarting from
+// r = b - A p up to max_restarts times. Residuals are captured for training when capture_on and the settings ask.
+void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
 {
-    if (!c->nactive) return;
-    const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 200;
-    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
-    mg_setup(c);
     float *R[2] = {c->r, c->r2};
     float *D[2] = {c->d, c->d2};
-    l0_pcg_init(c, R[0]);
-    finalize(c, kFinInit);
-    mg_counts(c);
-    if (!c->nsolve) {
-        // no fluid cell: b = 0, the solve is converged as it stands
-        c->capture_n = 0;
-        return;
-    }
+    float *const cap_buf = capture_on ? c->capture : nullptr;
     // residuals kept for training the learned preconditioner: the initial one and those after iterations 1, 2,
     // 3, 4, 6, 8, 11, 15, 20, ... (slots past the solve's last iteration are dropped afterwards)
     uint32_t ncap = 0;
     int next_cap = 0, cap_gap = 1;
     auto capture = [&](const float *r, int iteration) {
-        if (!c->capture || ncap >= c->capture_cap || iteration != next_cap) return;
-        SLOSH_HIP(hipMemcpyAsync(c->capture + (size_t)ncap * c->geo.pool_cells, r, (size_t)c->geo.pool_cells * 4u,
+        if (!cap_buf || ncap >= c->capture_cap || iteration != next_cap) return;
+        SLOSH_HIP(hipMemcpyAsync(cap_buf + (size_t)ncap * c->geo.pool_cells, r, (size_t)c->geo.pool_cells * 4u,
                                  hipMemcpyDeviceToDevice, c->stream));
         c->capture_iter[ncap++] = iteration;
         next_cap += cap_gap;
@@ -1291,7 +1286,7 @@ void run_mgpcg(slosh_ctx *c)
     // solve as a HIP graph and replayed from the second chunk on: every chunk of a solve launches the same kernels
     // on the same buffers.
     const int chunk = c->solver_chunk >= 2 ? c->solver_chunk & ~1 : 4;
-    const bool use_graph = c->use_graphs && !c->capture && c->timing_mask == 0;
+    const bool use_graph = c->use_graphs && !cap_buf && c->timing_mask == 0;
     int done_iters = 0; // iterations of the chunks enqueued so far
     auto enqueue_chunk = [&](int it0) {
         for (int k = 0; k < chunk; ++k) {
@@ -1314,7 +1309,6 @@ void run_mgpcg(slosh_ctx *c)
     // The recurrence residual that stops the iteration can differ from the FP64 true residual ||b - A p|| / ||b||
     // of the pressure in the last bits; the solve is accepted only when the true residual meets the tolerance too,
     // else it restarts from r = b - A p of the current pressure (a few times at most, within max_iterations).
-    constexpr int kMaxRestarts = 3;
     for (int restart = 0;; ++restart) {
         while (done_iters <= maxit) {
             for (int k = 0; k < ahead && done_iters <= maxit; ++k, done_iters += chunk) {
@@ -1339,19 +1333,62 @@ void run_mgpcg(slosh_ctx *c)
         finalize(c, kFinCheck);
         SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
-        if (c->h_state->converged || c->h_state->iters >= maxit || restart >= kMaxRestarts) break;
+        if (c->h_state->converged || c->h_state->iters >= maxit || restart >= max_restarts) break;
         l0_pcg_init(c, R[0]);
         finalize(c, kFinRestart);
         first_iteration();
         done_iters = c->h_state->iters;
         ahead = 1;
     }
-    c->last_mg_iters = c->h_state->iters;
-    if (c->capture) {
+    if (cap_buf) {
         uint32_t valid = 0;
         while (valid < ncap && c->capture_iter[valid] <= c->h_state->iters) valid++;
         c->capture_n = valid;
     }
+}
+
+// SLOSH_FLAG_REPORT_FP64_ERROR: the error of the solution against an FP64-accurate solve of the same system. The
+// solution is kept, the system solved again from it to a true residual of 1e-12 (each restart forms r = b - A p
+// in FP64, so neither the FP32 vectors nor the FP16 levels limit the accuracy), the relative L2 difference reduced
+// over the fluid cells, and the first solution and its solver state restored.
+void reference_error(slosh_ctx *c, int maxit)
+{
+    constexpr double kReferenceTol = 1e-12;
+    const size_t pc = c->geo.pool_cells;
+    SLOSH_HIP(hipMemcpyAsync(c->pres_lp, c->pres, pc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    SLOSH_HIP(hipMemcpyAsync(c->state_save, c->state, sizeof(SolverState), hipMemcpyDeviceToDevice, c->stream));
+    const SolverState kept = *c->h_state;
+    const int last = c->last_mg_iters;
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, kReferenceTol, 4 * maxit);
+    l0_pcg_init(c, c->r);
+    finalize(c, kFinInit);
+    mgpcg_iterate(c, 4 * maxit, 12, false);
+    l0_error_partials(c, c->pres_lp);
+    SLOSH_HIP(hipMemcpyAsync(c->pres, c->pres_lp, pc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    SLOSH_HIP(hipMemcpyAsync(c->state, c->state_save, sizeof(SolverState), hipMemcpyDeviceToDevice, c->stream));
+    finalize(c, kFinErr);
+    *c->h_state = kept;
+    c->last_mg_iters = last;
+}
+
+// SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG), to ||r|| <= tol ||b||.
+void run_mgpcg(slosh_ctx *c)
+{
+    if (!c->nactive) return;
+    const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 200;
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
+    mg_setup(c);
+    l0_pcg_init(c, c->r);
+    finalize(c, kFinInit);
+    mg_counts(c);
+    if (!c->nsolve) {
+        // no fluid cell: b = 0, the solve is converged as it stands
+        c->capture_n = 0;
+        return;
+    }
+    mgpcg_iterate(c, maxit, 3, true);
+    c->last_mg_iters = c->h_state->iters;
+    if (c->s.flags & SLOSH_FLAG_REPORT_FP64_ERROR) reference_error(c, maxit);
 }
 
 // Flexible CG (Notay 2000, FCG(1)) preconditioned by the network: z = M(r), d = z - (z.q_prev / d_prev.q_prev) d_prev,

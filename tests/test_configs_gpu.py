@@ -1,0 +1,192 @@
+"""The benchmark configurations at their full sizes on the GPU, through the C ABI.
+
+- 256^3 sloshing tank (16.0M particles, seed 1): the first step against the oracle (sorted particles, brick table,
+  cell flags, divergence and diagonal bit for bit; MGPCG and RB-SOR (omega 1.8) to an FP64 true residual of 1e-6
+  with the solution within 1e-5 of the oracle's FP64 solve; face velocities after the step within 1e-5 relative
+  L-inf of the oracle's), then a property run: every solve converged, no particle lost.
+- 512^3 falling spheres and pool with FP16 multigrid levels (218M particles): every step converged, with the error
+  of each solution against an FP64-accurate solve of the same system reported and small; that reference agrees
+  with the oracle's FP64 solve where the oracle can run it (64^3).
+- 64 tanks of uneven size with sphere obstacles stepped as one batch: every tank converged, and the smallest,
+  median and largest tank bit-identical to their single-tank runs.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from slosh_amd import Batch, Sim, abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_linf(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def same_bits(a, b, what):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), what
+
+
+def sloshing_pair(solver, gpu_until=abi.STAGE_ALL, orc_until=abi.STAGE_ALL, **kw):
+    """The library and the oracle on the first step of the 256^3 sloshing tank, each stopped after a stage."""
+    grid, h, reqs = scenes.sloshing_tank(256, seed=1)
+    cap = scenes.particle_count(reqs)
+    s = abi.make_settings(grid, h, cap, solver=solver, tolerance=1e-6, **kw)
+    gpu, orc = Sim(s), O.OracleSim(s)
+    for sim in (gpu, orc):
+        for r in reqs:
+            sim.emit(r)
+    g = scenes.tank_gravity(0.0)
+    gpu.step(1.0 / 240.0, g, stop_after=gpu_until)
+    orc.step(1.0 / 240.0, g, stop_after=orc_until)
+    return gpu, orc
+
+
+def test_sloshing_256_first_step_classification_bitexact():
+    print("case 256^3 sloshing, stages up to the divergence", flush=True)
+    gpu, orc = sloshing_pair(abi.SOLVER_NONE, abi.STAGE_DIVERGENCE, abi.STAGE_DIVERGENCE)
+    assert gpu.num_particles() == orc.num_particles() == 15999968
+    same_bits(gpu.read_particles(), orc.read_particles(), "sorted particles")
+    for f in (abi.FIELD_BRICK_TABLE, abi.FIELD_CELL_FLAGS, abi.FIELD_DIVERGENCE, abi.FIELD_CELL_DIAG):
+        same_bits(gpu.read_grid(f), orc.read_grid(f), f"field {f}")
+    gpu.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("solver", [abi.SOLVER_MGPCG, abi.SOLVER_RBSOR])
+def test_sloshing_256_first_step_solve(solver):
+    print(f"case 256^3 sloshing, full step, solver {solver}", flush=True)
+    if solver == abi.SOLVER_MGPCG:
+        gpu, orc = sloshing_pair(solver)
+    else:
+        # the oracle's RB-SOR would take minutes at this size: it classifies the same system and solves it in FP64
+        gpu, orc = sloshing_pair(solver, abi.STAGE_ALL, abi.STAGE_DIVERGENCE, max_iterations=4000, sor_weight=1.8)
+    rg = gpu.results()
+    assert rg.converged == 1 and rg.true_residual_rel <= 1e-6, rg.as_dict()
+    pg = gpu.read_grid(abi.FIELD_PRESSURE_F64)
+    ref, _, rel = orc.solve_fp64(1e-13, 600)
+    assert rel <= 1e-12
+    assert orc.true_residual(pg) <= 1e-6
+    err = np.linalg.norm(pg - ref) / np.linalg.norm(ref)
+    print(f"solution error {err:.2e}, iterations {rg.solver_iterations}", flush=True)
+    assert err <= 1e-5
+    if solver == abi.SOLVER_MGPCG:
+        ro = orc.results()
+        assert ro.converged == 1
+        # both velocity fields follow from solutions within the tolerance of the same system
+        assert rel_linf(gpu.read_grid(abi.FIELD_VEL), orc.read_grid(abi.FIELD_VEL)) <= 1e-5
+    gpu.close()
+    orc.close()
+
+
+def test_sloshing_256_property_run():
+    """30 steps of the benchmark workload with the tuned options: every solve converged on its FP64 true residual,
+    the particle count constant."""
+    print("case 256^3 sloshing, 30 steps", flush=True)
+    grid, h, reqs = scenes.sloshing_tank(256, seed=1)
+    s = abi.make_settings(grid, h, scenes.particle_count(reqs), solver=abi.SOLVER_MGPCG, tolerance=1e-6,
+                          flags=abi.FLAG_MG_FLUID_COARSENING | abi.FLAG_WARM_START_EXTRAPOLATE, mg_coarse_sweeps=3)
+    sim = Sim(s)
+    for r in reqs:
+        sim.emit(r)
+    t = 0.0
+    for k in range(30):
+        sim.step(1.0 / 240.0, scenes.tank_gravity(t))
+        t += 1.0 / 240.0
+        r = sim.results()
+        assert r.converged == 1 and r.true_residual_rel <= 1e-6, (k, r.as_dict())
+        assert r.live_particles == 15999968 and r.pool_overflow_bricks == 0
+    sim.close()
+
+
+def test_fp64_reference_error_matches_the_oracle_solve():
+    """The library's error report (SLOSH_FLAG_REPORT_FP64_ERROR) measures the distance to the FP64 solution: on a
+    64^3 dam break with FP16 levels it equals the error computed against the oracle's FP64 solve of the same
+    system, and the reported solve is the kept one (not the reference)."""
+    print("case fp64 error report 64^3", flush=True)
+    grid, h, reqs = scenes.dam_break(64, seed=0)
+    flags = abi.FLAG_MG_FP16 | abi.FLAG_MG_FLUID_COARSENING
+    s = abi.make_settings(grid, h, scenes.particle_count(reqs), solver=abi.SOLVER_MGPCG, tolerance=1e-6,
+                          flags=flags | abi.FLAG_REPORT_FP64_ERROR)
+    s_plain = abi.make_settings(grid, h, scenes.particle_count(reqs), solver=abi.SOLVER_MGPCG, tolerance=1e-6,
+                                flags=flags)
+    gpu, plain, orc = Sim(s), Sim(s_plain), O.OracleSim(s_plain)
+    for sim in (gpu, plain, orc):
+        for r in reqs:
+            sim.emit(r)
+        sim.step(1.0 / 120.0, (0.0, -9.8, 0.0))
+    rg = gpu.results()
+    assert rg.converged == 1 and rg.fp64_error_rel >= 0.0, rg.as_dict()
+    pg = gpu.read_grid(abi.FIELD_PRESSURE_F64)
+    # the reported step kept its own solution: identical to a run without the report
+    same_bits(pg, plain.read_grid(abi.FIELD_PRESSURE_F64), "kept solution")
+    ref, _, rel = orc.solve_fp64(1e-13, 600)
+    assert rel <= 1e-12
+    err = np.linalg.norm(pg - ref) / np.linalg.norm(ref)
+    print(f"reported {rg.fp64_error_rel:.3e}, against the oracle {err:.3e}", flush=True)
+    assert abs(rg.fp64_error_rel - err) <= 1e-3 * err + 1e-11
+    assert plain.results().fp64_error_rel == -1.0
+    for sim in (gpu, plain, orc):
+        sim.close()
+
+
+def test_spheres_512_fp16_levels_property_run():
+    print("case 512^3 spheres + pool, FP16 levels, 3 steps", flush=True)
+    grid, h, reqs = scenes.spheres_pool(512, seed=2)
+    n = scenes.particle_count(reqs)
+    s = abi.make_settings(grid, h, n, solver=abi.SOLVER_MGPCG, tolerance=1e-6, budget=0.5,
+                          flags=abi.FLAG_MG_FP16 | abi.FLAG_MG_FLUID_COARSENING | abi.FLAG_REPORT_FP64_ERROR,
+                          mg_coarse_sweeps=3)
+    sim = Sim(s)
+    for r in reqs:
+        sim.emit(r)
+    for k in range(3):
+        sim.step(1.0 / 240.0, (0.0, -9.8, 0.0))
+        r = sim.results()
+        print(f"step {k}: {r.solver_iterations} iterations, true residual {r.true_residual_rel:.2e}, "
+              f"error against the FP64 solve {r.fp64_error_rel:.2e}", flush=True)
+        assert r.converged == 1 and r.true_residual_rel <= 1e-6, (k, r.as_dict())
+        assert 0.0 <= r.fp64_error_rel <= 1e-4, (k, r.as_dict())
+        assert r.live_particles == n and r.pool_overflow_bricks == 0
+    sim.close()
+
+
+def test_64_tank_batch():
+    print("case 64 tanks", flush=True)
+    specs = scenes.tank_batch(64, seed=4)
+    dt = 1.0 / 240.0
+
+    def make(spec):
+        grid, h, reqs, cols = spec
+        cap = scenes.particle_count(reqs) + 1024
+        sim = Sim(abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6,
+                                    flags=abi.FLAG_MG_FLUID_COARSENING | abi.FLAG_WARM_START_EXTRAPOLATE,
+                                    mg_coarse_sweeps=3))
+        sim.set_colliders(cols)
+        for r in reqs:
+            sim.emit(r)
+        return sim
+
+    sims = [make(sp) for sp in specs]
+    b = Batch(sims)
+    grav = np.tile(np.array((0.0, -9.8, 0.0), np.float32), (len(sims), 1))
+    for k in range(2):
+        b.step(dt, grav)
+        for i, sim in enumerate(sims):
+            r = sim.results()
+            assert r.converged == 1 and r.true_residual_rel <= 1e-6, (k, i, r.as_dict())
+    sizes = [int(np.prod(sp[0])) for sp in specs]
+    order = np.argsort(sizes, kind="stable")
+    for i in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        single = make(specs[i])
+        for _ in range(2):
+            single.step(dt, (0.0, -9.8, 0.0))
+        same_bits(sims[i].read_particles(), single.read_particles(), f"tank {i} particles")
+        same_bits(sims[i].read_grid(abi.FIELD_PRESSURE_F64), single.read_grid(abi.FIELD_PRESSURE_F64),
+                  f"tank {i} pressure")
+        single.close()
+    b.close()
+    for sim in sims:
+        sim.close()
