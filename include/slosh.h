@@ -46,8 +46,9 @@ enum {
     SLOSH_SOLVER_MGPCG = 0, /* multigrid-preconditioned CG, FP64 dots, to ||r||/||b|| <= tolerance */
     SLOSH_SOLVER_RBSOR = 1, /* red-black SOR with sor_weight, to tolerance or max_iterations */
     SLOSH_SOLVER_NONE = 2,  /* keep the (warm-started) pressure as is; used by parity tests of the other kernels */
-    SLOSH_SOLVER_FCG_UNET = 3 /* flexible CG preconditioned by the learned U-Net (slosh_unet_*), with MGPCG as the
-                                 fallback when it stalls and as the judge of its result; grid dims multiples of 8 */
+    SLOSH_SOLVER_FCG_UNET = 3 /* flexible CG preconditioned by M(r) = V(r) + N(r - A V(r)): the MGPCG V-cycle and the
+                                 learned U-Net's correction of the residual it leaves (slosh_unet_*); MGPCG is the
+                                 fallback when it stalls and the judge of its result; grid dims multiples of 8 */
 };
 
 /* Solver settings: FSloshSolverSettings (SloshTypes.h:113-160), restricted to what the per-step path reads. */
@@ -70,7 +71,11 @@ typedef struct slosh_settings {
     uint32_t flags;               /* SLOSH_FLAG_* */
     int32_t mg_coarse_sweeps;     /* MGPCG: red-black smoothing pairs per visit of the dense coarse levels (4 and
                                      coarser), 0..4; 0 and 1 give the reference's single pair (SloshProxy.cpp:904-959) */
-    int32_t reserved[5];          /* must be zero */
+    /* Two-sided density projection (SLOSH_FLAG_DENSITY_PROJECTION; SloshTypes.h:128-131): */
+    float density_relaxation;             /* fraction of the density error removed per projection, [0, 1] */
+    int32_t density_projection_iterations; /* RB-SOR sweeps of its solve (omega 1.7), 0..10000; MGPCG uses 2 iterations */
+    int32_t density_projection_interval;  /* project every N steps (1..8; 0 means 1), relaxation compounded */
+    int32_t reserved[2];          /* must be zero */
 } slosh_settings;
 
 #define SLOSH_FLAG_NONE 0u
@@ -87,6 +92,14 @@ typedef struct slosh_settings {
                                                 instead of the last solution scaled by dt/dt_prev (the reference's
                                                 warm start, SloshProxy.cpp:575 applied in Slosh.usf:569-598, kept
                                                 on every other cell); the solve still stops at the tolerance */
+#define SLOSH_FLAG_CAPTURE_VCYCLE_RESIDUALS 32u /* with SLOSH_FLAG_CAPTURE_RESIDUALS: capture r - A V(r), the
+                                                   residual each V-cycle leaves, instead of r (the training data of
+                                                   the hybrid preconditioner of SLOSH_SOLVER_FCG_UNET) */
+#define SLOSH_FLAG_DENSITY_PROJECTION 64u /* the reference's two-sided density projection (Slosh.usf:743-854,
+                                             SloshProxy.cpp:656-702): after the pressure projection, solve
+                                             A q = relaxation * clamp(rho - 1) with the pressure operator and shift
+                                             every particle by -grad q (at most half a cell); it replaces the
+                                             velocity density correction (density_correction is then not applied) */
 #define SLOSH_FLAG_REPORT_FP64_ERROR 16u /* MGPCG: after each solve, solve the same system again from its solution to
                                             an FP64 true residual of 1e-12 (restarts form b - A p in FP64, so the FP32
                                             vectors and FP16 levels do not limit it) and report the first solution's
@@ -151,7 +164,10 @@ enum {
     SLOSH_FIELD_ACCUM = 7,       /* int32[6 planes]: P2G momentum U,V,W then weight U,V,W (fixed point 2^20); defined
                                     between the P2G and normalise stages of a step */
     SLOSH_FIELD_BRICK_TABLE = 9, /* uint32 per 8^3 brick: pool slot, 0 = none (order argument ignored) */
-    SLOSH_FIELD_PRESSURE_F64 = 10 /* double: the solver's FP64 pressure */
+    SLOSH_FIELD_PRESSURE_F64 = 10, /* double: the solver's FP64 pressure */
+    SLOSH_FIELD_DENSITY_Q = 11,    /* float: density-projection potential q (cells), written as q by write_grid */
+    SLOSH_FIELD_DENSITY_RHS = 12,  /* float: its right-hand side as the solver's "divergence" (-relaxation * error) */
+    SLOSH_FIELD_DENSITY_DISP = 13  /* float[3 planes]: face displacements -(q_b - q_a) in cells */
 };
 enum { SLOSH_ORDER_DENSE = 0 /* x fastest over the whole grid, unallocated cells read 0 */ };
 
@@ -224,8 +240,9 @@ slosh_status slosh_device_particles(slosh_ctx *ctx, int32_t layout, void **dev_p
 /* Replaces all particles with n records of layout SLOSH_LAYOUT_F32_64 (no reference counterpart: the reference
  * only emits; this is the resume half of read_particles). The brick touch set is rebuilt from the positions. */
 slosh_status slosh_write_particles(slosh_ctx *ctx, const void *src, uint32_t n);
-/* Writes a per-cell float field (SLOSH_FIELD_PRESSURE only) in dense order; cells of unallocated bricks are
- * ignored. Used to resume a warm start. */
+/* Writes a per-cell float field (SLOSH_FIELD_PRESSURE or SLOSH_FIELD_DENSITY_Q) in dense order; cells of
+ * unallocated bricks are ignored. Used to resume a warm start; with SLOSH_SOLVER_NONE the written values are kept by
+ * the next step (parity tests of the passes that follow a solve). */
 slosh_status slosh_write_grid(slosh_ctx *ctx, int32_t field, const void *src, uint64_t src_bytes, int32_t order);
 
 /* ---- test hook ------------------------------------------------------------------------------------------------ */

@@ -82,6 +82,11 @@ struct osim {
     double *pres_prev; /* SLOSH_FLAG_WARM_START_EXTRAPOLATE: last solution per unit dt (NaN: none) */
     uint8_t *had;      /* scratch of stage_p2g: the cell was fluid in the last step */
     uint8_t *xmark;    /* scratch of stage_extrapolate: faces filled by the current layer */
+    /* SLOSH_FLAG_DENSITY_PROJECTION: right-hand side, potential q and face displacements */
+    float *drhs;
+    double *dq;
+    float *ddisp;
+    int dens_applied;
     float *div, *diag;
     /* solver */
     int nlevels;
@@ -292,6 +297,7 @@ void osim_destroy(osim *o)
     for (int k = 0; k < 3; ++k) free(o->vel[k]);
     free(o->fflags); free(o->fweight); free(o->ctype); free(o->ccount);
     free(o->pres); free(o->pres_prev); free(o->had); free(o->xmark); free(o->div); free(o->diag);
+    free(o->drhs); free(o->dq); free(o->ddisp);
     free(o->r); free(o->z); free(o->d); free(o->q);
     free(o->rd); free(o->zd); free(o->dd); free(o->qd); free(o->pd);
     free(o->part);
@@ -346,6 +352,11 @@ osim *osim_create(const slosh_settings *s)
     o->div = (float *)xcalloc(pc, 4);
     o->diag = (float *)xcalloc(pc, 4);
     o->xmark = (uint8_t *)xcalloc(3 * pc, 1);
+    if (o->s.flags & SLOSH_FLAG_DENSITY_PROJECTION) {
+        o->drhs = (float *)xcalloc(pc, 4);
+        o->dq = (double *)xcalloc(pc, 8);
+        o->ddisp = (float *)xcalloc(3 * pc, 4);
+    }
     o->r = (float *)xcalloc(pc, 4);
     o->z = (float *)xcalloc(pc, 4);
     o->d = (float *)xcalloc(pc, 4);
@@ -786,7 +797,9 @@ static void stage_divergence(osim *o, float dt)
         }
         float rho = ws / (6.0f * TARGET_PPC);
         float excess = fmaxf(rho - (1.0f + o->s.density_deadband), 0.0f);
-        float target = fminf(o->s.density_correction * excess, 0.5f) * o->h / dt;
+        /* the two-sided projection replaces the velocity correction (SloshProxy.cpp:421) */
+        const float dcorr = (o->s.flags & SLOSH_FLAG_DENSITY_PROJECTION) ? 0.0f : o->s.density_correction;
+        float target = fminf(dcorr * excess, 0.5f) * o->h / dt;
         o->div[idx] = dv - target;
         float dg = 0.0f;
         for (int nb = 0; nb < 6; ++nb) {
@@ -1055,10 +1068,8 @@ static double mgpcg_true_rr(osim *o)
  * FP64 dot products and alpha/beta updates and a relative-residual stop. The iteration stops when the residual it
  * carries meets the tolerance; the solve is accepted when the FP64 true residual ||b - A p|| / ||b|| of the
  * pressure does too, else it restarts from r = b - A p (at most three times, within max_iterations). */
-static void solve_mgpcg(osim *o)
+static void solve_mgpcg_opts(osim *o, double tol, int maxit, int max_restarts)
 {
-    double tol = (double)o->s.tolerance;
-    int maxit = o->s.max_iterations > 0 ? o->s.max_iterations : 200;
     mg_build_types(o);
     coarse_factor(o);
     double bb = 0.0;
@@ -1144,12 +1155,41 @@ static void solve_mgpcg(osim *o)
         const double rel_true = sqrt(mgpcg_true_rr(o)) / bnorm;
         o->res.true_residual_rel = (float)rel_true;
         if (rec && rel_true <= tol) { conv = 1; break; }
-        if (iters >= maxit || restart >= 3) break;
+        if (iters >= maxit || restart >= max_restarts) break;
         mgpcg_residual(o, &bb);
     }
     o->res.solver_iterations = iters;
     o->res.residual_rel = rel;
     o->res.converged = conv;
+}
+
+static void solve_mgpcg(osim *o)
+{
+    solve_mgpcg_opts(o, (double)o->s.tolerance, o->s.max_iterations > 0 ? o->s.max_iterations : 200, 3);
+}
+
+/* One colour sweep of Slosh.usf:856-892 (PressureRBGSCS) on the shared operator, relaxation w. */
+static void rbsor_sweep(osim *o, unsigned colour, float w)
+{
+    const uint32_t nb = lv_nblocks(o, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint32_t blk = 0; blk < nb; ++blk) {
+        lv_iter it = lv_block(0, blk);
+        while (lv_next(o, &it)) {
+            uint32_t i = it.idx;
+            if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
+            if ((((unsigned)(it.x + it.y + it.z)) & 1u) != colour) continue;
+            double sum = 0.0;
+            for (int q = 0; q < 6; ++q) {
+                int nx = it.x + NB[q][0], ny = it.y + NB[q][1], nz = it.z + NB[q][2];
+                if (lv_type(o, 0, nx, ny, nz) == CELL_FLUID) sum += o->pres[cell_index(o, nx, ny, nz)];
+            }
+            double dg = (double)(o->diag[i] + DIAG_EPS);
+            double gs = (sum - (double)o->div[i]) / dg;
+            double old = o->pres[i];
+            o->pres[i] = old + (double)w * (gs - old);
+        }
+    }
 }
 
 /* Slosh.usf:856-892 (PressureRBGSCS) on the shared operator; residual checked every 8 iterations in FP64. */
@@ -1190,28 +1230,8 @@ static void solve_rbsor(osim *o)
         rel = bnorm > 0.0 ? sqrt(rr) / bnorm : 0.0;
         if (rel <= tol) { conv = 1; break; }
         if (iters >= maxit) break;
-        for (int k = 0; k < 8 && iters < maxit; ++k, ++iters) {
-            for (unsigned colour = 0; colour < 2u; ++colour) {
-#pragma omp parallel for schedule(dynamic, 16)
-                for (uint32_t blk = 0; blk < nb; ++blk) {
-                    lv_iter it = lv_block(0, blk);
-                    while (lv_next(o, &it)) {
-                        uint32_t i = it.idx;
-                        if ((o->ctype[i] & CELL_TYPE_MASK) != CELL_FLUID) continue;
-                        if ((((unsigned)(it.x + it.y + it.z)) & 1u) != colour) continue;
-                        double sum = 0.0;
-                        for (int q = 0; q < 6; ++q) {
-                            int nx = it.x + NB[q][0], ny = it.y + NB[q][1], nz = it.z + NB[q][2];
-                            if (lv_type(o, 0, nx, ny, nz) == CELL_FLUID) sum += o->pres[cell_index(o, nx, ny, nz)];
-                        }
-                        double dg = (double)(o->diag[i] + DIAG_EPS);
-                        double gs = (sum - (double)o->div[i]) / dg;
-                        double old = o->pres[i];
-                        o->pres[i] = old + (double)w * (gs - old);
-                    }
-                }
-            }
-        }
+        for (int k = 0; k < 8 && iters < maxit; ++k, ++iters)
+            for (unsigned colour = 0; colour < 2u; ++colour) rbsor_sweep(o, colour, w);
     }
     o->res.solver_iterations = iters;
     o->res.residual_rel = rel;
@@ -1279,6 +1299,127 @@ static void stage_project(osim *o)
     const double sum = part_sum(o, o->nactive, 0);
     o->res.max_divergence = mx;
     o->res.mean_divergence = cnt ? (float)(sum / (double)cnt) : 0.0f;
+}
+
+/* ------------------------------------------------------------------------------------------------------------ */
+/* Two-sided density projection (Slosh.usf:743-854; driver SloshProxy.cpp:656-702).                             */
+/* Slosh.usf:753-773 (CellDensityInterior): P2G weight of the faces no solid shares, per rest count of those faces. */
+static float cell_density_interior(const osim *o, int x, int y, int z)
+{
+    float ws = 0.0f, faces = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        int m[3] = {x, y, z}, p[3] = {x, y, z};
+        m[a] -= 1;
+        p[a] += 1;
+        if ((cell_type_at(o, m[0], m[1], m[2]) & CELL_TYPE_MASK) != CELL_SOLID) {
+            ws += o->fweight[face_index(o, a, x, y, z)];
+            faces += 1.0f;
+        }
+        if ((cell_type_at(o, p[0], p[1], p[2]) & CELL_TYPE_MASK) != CELL_SOLID) {
+            ws += o->fweight[face_index(o, a, p[0], p[1], p[2])];
+            faces += 1.0f;
+        }
+    }
+    return faces > 0.0f ? ws / (faces * TARGET_PPC) : 1.0f;
+}
+
+/* Slosh.usf:775-805 (DensityProjectionRhsCS): the solver's "divergence" -relax * clamp(rho - 1, -0.5, 0.5) of fluid
+ * cells, a cell touching air only expanding; q from 0 (kept when the pressure solver is off: q as written). */
+static void stage_density_rhs(osim *o, float relax, int keep_q)
+{
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
+    {
+        if (!keep_q) o->dq[idx] = 0.0;
+        if ((o->ctype[idx] & CELL_TYPE_MASK) != CELL_FLUID) {
+            o->drhs[idx] = 0.0f;
+            continue;
+        }
+        float dev = cell_density_interior(o, x, y, z) - 1.0f;
+        int air = 0;
+        for (int k = 0; k < 6; ++k)
+            air = air || (cell_type_at(o, x + NB[k][0], y + NB[k][1], z + NB[k][2]) & CELL_TYPE_MASK) == CELL_AIR;
+        if (air) dev = fmaxf(dev, 0.0f);
+        o->drhs[idx] = -relax * clampf(dev, -0.5f, 0.5f);
+    }
+}
+
+/* Slosh.usf:808-828 (DensityFaceDisplacement): -(q_b - q_a) per face (q of a non-fluid cell 0), 0 at solids. */
+static void stage_density_disp(osim *o)
+{
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
+    {
+        uint32_t tb = o->ctype[idx] & CELL_TYPE_MASK;
+        for (int a = 0; a < 3; ++a) {
+            size_t f = (size_t)a * o->pool_cells + idx;
+            int ca[3] = {x, y, z};
+            ca[a] -= 1;
+            uint32_t ta = cell_type_at(o, ca[0], ca[1], ca[2]) & CELL_TYPE_MASK;
+            if (ta == CELL_SOLID || tb == CELL_SOLID) {
+                o->ddisp[f] = 0.0f;
+                continue;
+            }
+            double qa = ta == CELL_FLUID ? o->dq[cell_index(o, ca[0], ca[1], ca[2])] : 0.0;
+            double qb = tb == CELL_FLUID ? o->dq[idx] : 0.0;
+            o->ddisp[f] = (float)(-(qb - qa));
+        }
+    }
+}
+
+/* SloshProxy.cpp:656-683: the projection's solve with the pressure solver on the same operator, Divergence := rhs,
+ * Pressure := q: two MGPCG iterations (MGPCG and flexible CG), or density_projection_iterations RB-SOR sweeps at
+ * omega 1.7; no stopping test. The step's pressure-solve results are kept. */
+static void solve_density(osim *o)
+{
+    slosh_results keep = o->res;
+    float *div = o->div;
+    double *pres = o->pres;
+    o->div = o->drhs;
+    o->pres = o->dq;
+    if (o->s.pressure_solver == SLOSH_SOLVER_RBSOR) {
+        int iters = o->s.density_projection_iterations > 0 ? o->s.density_projection_iterations : 1;
+        for (int it = 0; it < iters; ++it)
+            for (unsigned colour = 0; colour < 2u; ++colour) rbsor_sweep(o, colour, 1.7f);
+    } else {
+        solve_mgpcg_opts(o, 0.0, 2, 0);
+    }
+    o->div = div;
+    o->pres = pres;
+    o->res = keep;
+}
+
+/* Slosh.usf:830-853 (SampleDensityDisplacement): face displacements interpolated per axis at xp (stencil nodes
+ * clamped into the grid), shortened to half a cell, in metres. */
+static void sample_displacement(const osim *o, const float xp[3], float out[3])
+{
+    float dv[3];
+    for (int axis = 0; axis < 3; ++axis) {
+        float off[3] = {0.5f, 0.5f, 0.5f};
+        off[axis] = 0.0f;
+        int base[3];
+        float fr[3];
+        for (int a = 0; a < 3; ++a) {
+            float g = xp[a] * o->inv_h - off[a];
+            base[a] = (int)floorf(g);
+            fr[a] = g - (float)base[a];
+        }
+        float value = 0.0f;
+        for (uint32_t oi = 0; oi < 8u; ++oi) {
+            int od[3] = {(int)(oi & 1u), (int)((oi >> 1) & 1u), (int)((oi >> 2) & 1u)};
+            int node[3];
+            float w3[3];
+            for (int a = 0; a < 3; ++a) {
+                node[a] = clampi(base[a] + od[a], 0, o->n[a] - 1);
+                w3[a] = od[a] ? fr[a] : 1.0f - fr[a];
+            }
+            value += w3[0] * w3[1] * w3[2] * o->ddisp[face_index(o, axis, node[0], node[1], node[2])];
+        }
+        dv[axis] = value;
+    }
+    float len = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    float s = len > 0.5f ? 0.5f / len : 1.0f;
+    for (int a = 0; a < 3; ++a) out[a] = (len > 0.5f ? dv[a] * s : dv[a]) * o->h;
 }
 
 /* Slosh.usf:1263-1310 (ExtrapolateFace), layer `layer`: src = vel[cur], dst = next buffer. */
@@ -1436,6 +1577,13 @@ static void stage_g2p(osim *o, float dt, const float grav[3])
         }
         float x[3];
         for (int a = 0; a < 3; ++a) x[a] = pp->pos[a] + dt * v2[a];
+        if (o->dens_applied) {
+            /* Slosh.usf:1506-1510: position-only shift, sampled where the density was measured */
+            float pc[3], dx[3];
+            for (int a = 0; a < 3; ++a) pc[a] = clampf(pp->pos[a], lo, hi[a]);
+            sample_displacement(o, pc, dx);
+            for (int a = 0; a < 3; ++a) x[a] = x[a] + dx[a];
+        }
         int c[3];
         pos_cell(o, pp->pos, c);
         if (cell_index(o, c[0], c[1], c[2]) < BRICK_CELLS) {
@@ -1504,6 +1652,18 @@ int osim_step_until(osim *o, float dt, const float g[3], int stop_after)
     else if (o->s.pressure_solver == SLOSH_SOLVER_RBSOR) solve_rbsor(o);
     if (stop_after <= SLOSH_STAGE_SOLVE) return SLOSH_OK;
     stage_project(o);
+    o->dens_applied = 0;
+    if ((o->s.flags & SLOSH_FLAG_DENSITY_PROJECTION) && o->nactive) {
+        int interval = o->s.density_projection_interval > 0 ? o->s.density_projection_interval : 1;
+        float relax = 1.0f - powf(1.0f - o->s.density_relaxation, (float)interval);
+        if (relax > 0.0f && o->frame % (uint64_t)interval == 0u) {
+            int keep = o->s.pressure_solver == SLOSH_SOLVER_NONE;
+            stage_density_rhs(o, relax, keep);
+            if (!keep) solve_density(o);
+            stage_density_disp(o);
+            o->dens_applied = 1;
+        }
+    }
     if (stop_after <= SLOSH_STAGE_PROJECT) return SLOSH_OK;
     for (int k = 0; k < o->s.extrapolation_layers; ++k) stage_extrapolate(o, (uint32_t)k);
     if (stop_after <= SLOSH_STAGE_EXTRAPOLATE) return SLOSH_OK;
@@ -1530,6 +1690,19 @@ int osim_write_pressure(osim *o, const float *dense)
     return SLOSH_OK;
 }
 
+int osim_write_density_q(osim *o, const float *dense)
+{
+    if (!o->dq) return SLOSH_ESTATE;
+    for (int z = 0; z < o->n[2]; ++z)
+        for (int y = 0; y < o->n[1]; ++y)
+            for (int x = 0; x < o->n[0]; ++x) {
+                uint32_t ci = cell_index(o, x, y, z);
+                if (ci < BRICK_CELLS) continue;
+                o->dq[ci] = (double)dense[(size_t)x + (size_t)o->n[0] * ((size_t)y + (size_t)o->n[1] * (size_t)z)];
+            }
+    return SLOSH_OK;
+}
+
 int osim_read_grid(const osim *o, int field, void *dst, uint64_t bytes)
 {
     size_t cells = (size_t)o->n[0] * o->n[1] * o->n[2];
@@ -1550,9 +1723,31 @@ int osim_read_grid(const osim *o, int field, void *dst, uint64_t bytes)
                 }
         return SLOSH_OK;
     }
+    if (field == SLOSH_FIELD_DENSITY_Q) {
+        if (!o->dq) return SLOSH_ESTATE;
+        if (bytes < (uint64_t)cells * 4u) return SLOSH_ECAPACITY;
+        float *outq = (float *)dst;
+        for (int z = 0; z < o->n[2]; ++z)
+            for (int y = 0; y < o->n[1]; ++y)
+                for (int x = 0; x < o->n[0]; ++x) {
+                    uint32_t ci = cell_index(o, x, y, z);
+                    size_t di = (size_t)x + (size_t)o->n[0] * ((size_t)y + (size_t)o->n[1] * (size_t)z);
+                    outq[di] = ci < BRICK_CELLS ? 0.0f : (float)o->dq[ci];
+                }
+        return SLOSH_OK;
+    }
     int planes = 1;
     const uint32_t *src = NULL;
     switch (field) {
+    case SLOSH_FIELD_DENSITY_RHS:
+        if (!o->drhs) return SLOSH_ESTATE;
+        src = (const uint32_t *)o->drhs;
+        break;
+    case SLOSH_FIELD_DENSITY_DISP:
+        if (!o->ddisp) return SLOSH_ESTATE;
+        src = (const uint32_t *)o->ddisp;
+        planes = 3;
+        break;
     case SLOSH_FIELD_CELL_FLAGS: src = o->ctype; break;
     case SLOSH_FIELD_PRESSURE: {
         if (bytes < (uint64_t)cells * 4u) return SLOSH_ECAPACITY;

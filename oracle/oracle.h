@@ -26,6 +26,8 @@ int osim_emit(osim *o, const slosh_emit_request *r);
 int osim_set_colliders(osim *o, const slosh_collider *c, uint32_t n);
 int osim_write_particles(osim *o, const slosh_particle *p, uint32_t n);
 int osim_write_pressure(osim *o, const float *dense);
+/* Writes the density-projection potential q (SLOSH_FLAG_DENSITY_PROJECTION), kept by a step with SOLVER_NONE. */
+int osim_write_density_q(osim *o, const float *dense);
 /* One step (stop_after = SLOSH_STAGE_*). */
 int osim_step_until(osim *o, float dt, const float g[3], int stop_after);
 uint32_t osim_num_particles(const osim *o);

@@ -40,6 +40,7 @@ def load():
     lib.osim_set_colliders.argtypes = [vp, C.POINTER(abi.Collider), C.c_uint32]
     lib.osim_write_particles.argtypes = [vp, vp, C.c_uint32]
     lib.osim_write_pressure.argtypes = [vp, vp]
+    lib.osim_write_density_q.argtypes = [vp, vp]
     lib.osim_step_until.argtypes = [vp, C.c_float, C.POINTER(C.c_float), C.c_int]
     lib.osim_num_particles.argtypes = [vp]
     lib.osim_num_particles.restype = C.c_uint32
@@ -116,6 +117,10 @@ class OracleSim:
     def write_pressure(self, dense):
         dense = np.ascontiguousarray(dense, dtype=np.float32)
         _check(self.lib.osim_write_pressure(self.h, dense.ctypes.data), "write_pressure")
+
+    def write_density_q(self, dense):
+        dense = np.ascontiguousarray(dense, dtype=np.float32)
+        _check(self.lib.osim_write_density_q(self.h, dense.ctypes.data), "write_density_q")
 
     def step(self, dt, gravity, stop_after=abi.STAGE_ALL):
         g = (C.c_float * 3)(*[float(x) for x in gravity])

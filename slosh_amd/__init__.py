@@ -153,6 +153,12 @@ class Sim:
         _check(self.lib.slosh_write_grid(self.h, abi.FIELD_PRESSURE, dense.ctypes.data, dense.nbytes, abi.ORDER_DENSE),
                "slosh_write_grid")
 
+    def write_density_q(self, dense):
+        """The density projection's potential q (dense (z, y, x)); a step with SOLVER_NONE keeps it."""
+        dense = np.ascontiguousarray(dense, dtype=np.float32)
+        _check(self.lib.slosh_write_grid(self.h, abi.FIELD_DENSITY_Q, dense.ctypes.data, dense.nbytes, abi.ORDER_DENSE),
+               "slosh_write_grid")
+
     def step(self, dt, gravity, stop_after=abi.STAGE_ALL):
         g = (C.c_float * 3)(*[float(x) for x in gravity])
         if stop_after == abi.STAGE_ALL:

@@ -26,6 +26,8 @@ FLAG_MG_FP16 = 2
 FLAG_MG_FLUID_COARSENING = 4
 FLAG_WARM_START_EXTRAPOLATE = 8
 FLAG_REPORT_FP64_ERROR = 16
+FLAG_CAPTURE_VCYCLE_RESIDUALS = 32
+FLAG_DENSITY_PROJECTION = 64
 
 EMIT_FILL_CELLS = 0
 EMIT_SPHERE = 1
@@ -44,6 +46,9 @@ FIELD_FACE_WEIGHT = 6
 FIELD_ACCUM = 7
 FIELD_BRICK_TABLE = 9
 FIELD_PRESSURE_F64 = 10
+FIELD_DENSITY_Q = 11
+FIELD_DENSITY_RHS = 12
+FIELD_DENSITY_DISP = 13
 
 # (numpy dtype, planes) of each dense grid field
 FIELD_INFO = {
@@ -56,6 +61,9 @@ FIELD_INFO = {
     FIELD_FACE_WEIGHT: (np.float32, 3),
     FIELD_ACCUM: (np.int32, 6),
     FIELD_PRESSURE_F64: (np.float64, 1),
+    FIELD_DENSITY_Q: (np.float32, 1),
+    FIELD_DENSITY_RHS: (np.float32, 1),
+    FIELD_DENSITY_DISP: (np.float32, 3),
 }
 
 ORDER_DENSE = 0
@@ -94,7 +102,10 @@ class Settings(C.Structure):
         ("density_deadband", C.c_float),
         ("flags", C.c_uint32),
         ("mg_coarse_sweeps", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("density_relaxation", C.c_float),
+        ("density_projection_iterations", C.c_int32),
+        ("density_projection_interval", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -186,7 +197,8 @@ assert PARTICLE_DTYPE.itemsize == 64 and PARTICLE_REF48_DTYPE.itemsize == 48
 
 def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6, max_iterations=200, sor_weight=1.8,
                   budget=1.0, extrapolation_layers=2, ghost_fluid=1, friction=0.0, max_speed=20.0, flags=0,
-                  mg_coarse_sweeps=0):
+                  mg_coarse_sweeps=0, density_correction=0.0, density_relaxation=0.5, density_iterations=30,
+                  density_interval=1):
     """Parity-profile settings (APIC, ghost fluid, no density correction) with the given grid and solver."""
     s = Settings()
     for a in range(3):
@@ -203,10 +215,13 @@ def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6
     s.boundary_friction = float(friction)
     s.max_speed = float(max_speed)
     s.particle_radius_cells = 0.25
-    s.density_correction = 0.0
+    s.density_correction = float(density_correction)
     s.density_deadband = 0.05
     s.flags = int(flags)
     s.mg_coarse_sweeps = int(mg_coarse_sweeps)
+    s.density_relaxation = float(density_relaxation)
+    s.density_projection_iterations = int(density_iterations)
+    s.density_projection_interval = int(density_interval)
     return s
 
 

@@ -25,7 +25,7 @@ const char *const kKernelNames[K_COUNT] = {
     "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d23down", "k_d23up",
     "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
     "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io",
-    "k_unet_train", "k_fcg", "k_g2p_move"};
+    "k_unet_train", "k_fcg", "k_g2p_move", "k_density"};
 } // namespace slosh
 
 static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");
@@ -69,7 +69,7 @@ void free_all(slosh_ctx *c)
     dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag); dfree(c->fmask);
     dfree(c->r); dfree(c->r2); dfree(c->z); dfree(c->z1); dfree(c->d); dfree(c->d2); dfree(c->q);
     dfree(c->binfo); dfree(c->binfo_f); dfree(c->fluid_flag); dfree(c->fluid_rank); dfree(c->tile_flag); dfree(c->tile_rank);
-    dfree(c->tile_list); dfree(c->fcg_z); dfree(c->capture);
+    dfree(c->tile_list); dfree(c->fcg_z); dfree(c->fcg_rp); dfree(c->capture);
     unet_free(c);
     for (int l = 1; l < kMaxLevels; ++l) {
         dfree(c->lv[l].type);
@@ -78,6 +78,7 @@ void free_all(slosh_ctx *c)
     }
     dfree(c->dense_info); dfree(c->dense_coord);
     dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->coarse_sig); dfree(c->pres_prev); dfree(c->pres_lp); dfree(c->state_save); dfree(c->partials);
+    dfree(c->dens_rhs); dfree(c->dens_q); dfree(c->dens_disp);
     dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
     dfree(c->bp_clamp);
     dfree(c->state); dfree(c->stats); dfree(c->d_col);
@@ -101,6 +102,8 @@ void free_all(slosh_ctx *c)
     if (c->ev_pool) (void)hipEventDestroy(c->ev_pool);
     if (c->mg_graph) (void)hipGraphExecDestroy(c->mg_graph);
     c->mg_graph = nullptr;
+    if (c->fcg_graph) (void)hipGraphExecDestroy(c->fcg_graph);
+    c->fcg_graph = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
 }
@@ -116,14 +119,19 @@ bool settings_valid(const slosh_settings *s)
         for (int a = 0; a < 3; ++a)
             if (s->grid_dim[a] % 8 != 0) return false; // the network's levels need whole bricks
     if (s->flags & ~(uint32_t)(SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_MG_FP16 | SLOSH_FLAG_MG_FLUID_COARSENING |
-                               SLOSH_FLAG_WARM_START_EXTRAPOLATE | SLOSH_FLAG_REPORT_FP64_ERROR))
+                               SLOSH_FLAG_WARM_START_EXTRAPOLATE | SLOSH_FLAG_REPORT_FP64_ERROR |
+                               SLOSH_FLAG_CAPTURE_VCYCLE_RESIDUALS | SLOSH_FLAG_DENSITY_PROJECTION))
         return false;
     if (!(s->tolerance > 0.0f) || s->max_iterations < 0) return false;
     if (!(s->sor_weight >= 0.5f && s->sor_weight <= 1.95f)) return false;
     if (s->extrapolation_layers < 0 || s->extrapolation_layers > 4) return false;
     if (!(s->boundary_friction >= 0.0f && s->boundary_friction <= 1.0f)) return false;
     if (s->mg_coarse_sweeps < 0 || s->mg_coarse_sweeps > 4) return false;
-    for (int k = 0; k < 5; ++k)
+    if (!(s->density_relaxation >= 0.0f && s->density_relaxation <= 1.0f)) return false;
+    if (s->density_projection_iterations < 0 || s->density_projection_iterations > 10000) return false;
+    if (s->density_projection_interval < 0 || s->density_projection_interval > 8) return false;
+    if (!(s->density_correction >= 0.0f) || !(s->density_deadband >= 0.0f)) return false;
+    for (int k = 0; k < 2; ++k)
         if (s->reserved[k] != 0) return false;
     return true;
 }
@@ -211,6 +219,11 @@ void create_impl(slosh_ctx *c)
         dalloc(&c->pres_lp, pc);
         dalloc(&c->state_save, 1);
     }
+    if (s.flags & SLOSH_FLAG_DENSITY_PROJECTION) {
+        dalloc(&c->dens_rhs, pc);
+        dalloc(&c->dens_q, pc);
+        dalloc(&c->dens_disp, 3 * pc);
+    }
     dalloc(&c->div, pc);
     dalloc(&c->diag, pc);
     dalloc(&c->fmask, pc);
@@ -232,7 +245,10 @@ void create_impl(slosh_ctx *c)
         dalloc(&c->tile_list, t3);
     }
     dalloc(&c->fluid_rank, (size_t)g.npool + 1u);
-    if (s.pressure_solver == SLOSH_SOLVER_FCG_UNET) dalloc(&c->fcg_z, pc);
+    if (s.pressure_solver == SLOSH_SOLVER_FCG_UNET) {
+        dalloc(&c->fcg_z, pc);
+        dalloc(&c->fcg_rp, pc);
+    }
     if (s.flags & SLOSH_FLAG_CAPTURE_RESIDUALS) {
         c->capture_cap = 12;
         dalloc(&c->capture, (size_t)c->capture_cap * pc);
@@ -424,6 +440,26 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
             SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
     }
     if (stop > SLOSH_STAGE_SOLVE) launch_project(c);
+    // SloshProxy.cpp:656-702: the two-sided density projection after the pressure projection, every
+    // density_projection_interval steps with the relaxation compounded to 1 - (1 - r)^N
+    c->dens_applied = false;
+    if (stop > SLOSH_STAGE_SOLVE && (c->s.flags & SLOSH_FLAG_DENSITY_PROJECTION) && c->nactive) {
+        const int interval = c->s.density_projection_interval > 0 ? c->s.density_projection_interval : 1;
+        const float relax = 1.0f - powf(1.0f - c->s.density_relaxation, (float)interval);
+        if (relax > 0.0f && c->frame % (uint64_t)interval == 0u) {
+            const int keep = c->s.pressure_solver == SLOSH_SOLVER_NONE; // q as written by the caller
+            launch_density_rhs(c, relax, keep);
+            if (!keep) {
+                // the solve reads its state back: keep the pressure solve's final state for the step's results
+                SLOSH_HIP(hipStreamSynchronize(c->stream));
+                const SolverState saved = *c->h_state;
+                run_density_solve(c);
+                *c->h_state = saved;
+            }
+            launch_density_disp(c);
+            c->dens_applied = true;
+        }
+    }
     if (stop > SLOSH_STAGE_PROJECT)
         for (int k = 0; k < c->s.extrapolation_layers; ++k) launch_extrapolate(c, (uint32_t)k);
     if (stop > SLOSH_STAGE_EXTRAPOLATE) launch_g2p(c, dt, gv);
@@ -512,6 +548,22 @@ slosh_status read_grid_impl(slosh_ctx *c, int32_t field, void *dst, uint64_t byt
         if (bytes < (uint64_t)cells * 8u) return SLOSH_ECAPACITY;
         auto h = fetch(c, c->pres, g.pool_cells);
         pool_to_dense(c, table, h.data(), (double *)dst, 1);
+        return SLOSH_OK;
+    }
+    if (field == SLOSH_FIELD_DENSITY_Q || field == SLOSH_FIELD_DENSITY_RHS || field == SLOSH_FIELD_DENSITY_DISP) {
+        if (!c->dens_q) return SLOSH_ESTATE; // the settings do not enable the density projection
+        const int pl = field == SLOSH_FIELD_DENSITY_DISP ? 3 : 1;
+        if (bytes < (uint64_t)cells * 4u * (uint64_t)pl) return SLOSH_ECAPACITY;
+        if (field == SLOSH_FIELD_DENSITY_Q) {
+            auto h = fetch(c, c->dens_q, g.pool_cells);
+            pool_to_dense(c, table, h.data(), (float *)dst, 1);
+        } else if (field == SLOSH_FIELD_DENSITY_RHS) {
+            auto h = fetch(c, c->dens_rhs, g.pool_cells);
+            pool_to_dense(c, table, h.data(), (float *)dst, 1);
+        } else {
+            auto h = fetch(c, c->dens_disp, 3 * (size_t)g.pool_cells);
+            pool_to_dense(c, table, h.data(), (float *)dst, 3);
+        }
         return SLOSH_OK;
     }
     if (field < 0 || field > SLOSH_FIELD_ACCUM) return SLOSH_EINVAL;
@@ -683,6 +735,9 @@ void slosh_default_settings(slosh_settings *s)
     s->particle_radius_cells = 0.25f;
     s->density_correction = 0.0f;
     s->density_deadband = 0.05f;
+    s->density_relaxation = 0.5f;
+    s->density_projection_iterations = 30;
+    s->density_projection_interval = 1;
 }
 
 slosh_status slosh_create(const slosh_settings *s, slosh_ctx **out)
@@ -847,13 +902,16 @@ slosh_status slosh_read_grid(slosh_ctx *c, int32_t field, void *dst, uint64_t ds
 
 slosh_status slosh_write_grid(slosh_ctx *c, int32_t field, const void *src, uint64_t src_bytes, int32_t order)
 {
-    if (!c || !src || order != SLOSH_ORDER_DENSE || field != SLOSH_FIELD_PRESSURE) return SLOSH_EINVAL;
+    if (!c || !src || order != SLOSH_ORDER_DENSE) return SLOSH_EINVAL;
+    if (field != SLOSH_FIELD_PRESSURE && field != SLOSH_FIELD_DENSITY_Q) return SLOSH_EINVAL;
+    if (field == SLOSH_FIELD_DENSITY_Q && !c->dens_q) return SLOSH_ESTATE;
+    double *const target = field == SLOSH_FIELD_PRESSURE ? c->pres : c->dens_q;
     const Geo &g = c->geo;
     const size_t cells = (size_t)g.n[0] * g.n[1] * g.n[2];
     if (src_bytes < cells * 4u) return SLOSH_ECAPACITY;
     ABI_GUARD({
         std::vector<uint32_t> table = fetch(c, c->table, g.nbricks);
-        std::vector<double> pool = fetch(c, c->pres, g.pool_cells);
+        std::vector<double> pool = fetch(c, target, g.pool_cells);
         const float *in = (const float *)src;
         for (int z = 0; z < g.n[2]; ++z)
             for (int y = 0; y < g.n[1]; ++y)
@@ -863,7 +921,7 @@ slosh_status slosh_write_grid(slosh_ctx *c, int32_t field, const void *src, uint
                     pool[(size_t)slot * kBrickCells + (size_t)((x & 7) + 8 * ((y & 7) + 8 * (z & 7)))] =
                         (double)in[(size_t)x + (size_t)g.n[0] * ((size_t)y + (size_t)g.n[1] * z)];
                 }
-        SLOSH_HIP(hipMemcpyAsync(c->pres, pool.data(), pool.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        SLOSH_HIP(hipMemcpyAsync(target, pool.data(), pool.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
         return SLOSH_OK;
     })

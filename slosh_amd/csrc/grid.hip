@@ -474,6 +474,84 @@ __global__ __launch_bounds__(512) void k_project(Geo g, const uint32_t *rows, fl
     }
 }
 
+// ---- two-sided density projection (Slosh.usf:743-854) --------------------------------------------------------
+// Slosh.usf:753-773 (CellDensityInterior): the P2G weight of the cell's faces that no solid shares, per rest
+// particle count of those faces; 1 when a solid shares every face.
+__device__ __forceinline__ float cell_density_interior(const Geo &g, const CellPos &c, const float *fw,
+                                                       const uint32_t *ctype)
+{
+    float ws = 0.0f, faces = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        int m[3] = {c.x, c.y, c.z}, p[3] = {c.x, c.y, c.z};
+        m[a] -= 1;
+        p[a] += 1;
+        if ((type_at(g, c, ctype, m[0], m[1], m[2]) & kCellTypeMask) != kCellSolid) {
+            ws += fw[(uint32_t)a * g.pool_cells + c.idx];
+            faces += 1.0f;
+        }
+        if ((type_at(g, c, ctype, p[0], p[1], p[2]) & kCellTypeMask) != kCellSolid) {
+            ws += fw[face_index(g, c, a, p[0], p[1], p[2])];
+            faces += 1.0f;
+        }
+    }
+    return faces > 0.0f ? ws / (faces * kTargetPpc) : 1.0f;
+}
+
+// Slosh.usf:775-805 (DensityProjectionRhsCS): the solver's "divergence" -relax * clamp(rho - 1, -0.5, 0.5) of fluid
+// cells, where a cell touching air may only expand (rho - 1 >= 0); q starts from 0 (kept when keep_q: the
+// pressure solver is off and q was written by the caller).
+__global__ __launch_bounds__(512) void k_density_rhs(Geo g, const uint32_t *rows, const float *fw, const uint32_t *ctype,
+                                                     float *rhs, double *q, float relax, int keep_q)
+{
+    __shared__ uint32_t s_row[28];
+    CellPos c;
+    uint32_t e;
+    const bool in = cell_of_block(g, rows, s_row, c, e);
+    if (!keep_q) q[c.idx] = 0.0;
+    if (!in || (ctype[c.idx] & kCellTypeMask) != kCellFluid) {
+        rhs[c.idx] = 0.0f;
+        return;
+    }
+    float dev = cell_density_interior(g, c, fw, ctype) - 1.0f;
+    bool air = false;
+    for (int nb = 0; nb < 6; ++nb) {
+        int d[3];
+        nb_offset(nb, d);
+        air = air || (type_at(g, c, ctype, c.x + d[0], c.y + d[1], c.z + d[2]) & kCellTypeMask) == kCellAir;
+    }
+    if (air) dev = fmaxf(dev, 0.0f);
+    rhs[c.idx] = -relax * clampf(dev, -0.5f, 0.5f);
+}
+
+// Slosh.usf:808-828 (DensityFaceDisplacement, DensityDisplacementFace): -(q_b - q_a) on each face, q of a non-fluid
+// cell 0, and 0 on faces that touch a solid.
+__global__ __launch_bounds__(512) void k_density_disp(Geo g, const uint32_t *rows, const uint32_t *ctype,
+                                                      const double *q, float *disp)
+{
+    __shared__ uint32_t s_row[28];
+    CellPos c;
+    uint32_t e;
+    const bool in = cell_of_block(g, rows, s_row, c, e);
+    const uint32_t tb = ctype[c.idx] & kCellTypeMask;
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t f = (uint32_t)a * g.pool_cells + c.idx;
+        if (!in) {
+            disp[f] = 0.0f;
+            continue;
+        }
+        int ca[3] = {c.x, c.y, c.z};
+        ca[a] -= 1;
+        const uint32_t ta = type_at(g, c, ctype, ca[0], ca[1], ca[2]) & kCellTypeMask;
+        if (ta == kCellSolid || tb == kCellSolid) {
+            disp[f] = 0.0f;
+            continue;
+        }
+        const double qa = ta == kCellFluid ? q[cell_index(g, c, ca[0], ca[1], ca[2])] : 0.0;
+        const double qb = tb == kCellFluid ? q[c.idx] : 0.0;
+        disp[f] = (float)(-(qb - qa));
+    }
+}
+
 // Slosh.usf:1222-1240 (ResidualCS): max and sum of |div|/h over fluid cells after projection, one partial per
 // brick (reduced with the other step statistics by k_step_stats).
 __global__ __launch_bounds__(512) void k_div_stats(Geo g, const uint32_t *rows, const float *vel, const uint32_t *ctype,
@@ -753,7 +831,8 @@ void launch_divergence(slosh_ctx *c, float dt)
     {
         launch_timed(c, K_DIVERGENCE, k_divergence, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo,
                      c->vel[c->vcur], c->fweight, c->ctype, c->div, c->diag, c->fmask, c->s.ghost_fluid,
-                     c->s.density_correction, c->s.density_deadband, c->geo.h, dt);
+                     (c->s.flags & SLOSH_FLAG_DENSITY_PROJECTION) ? 0.0f : c->s.density_correction,
+                     c->s.density_deadband, c->geo.h, dt);
     }
     SLOSH_HIP(hipGetLastError());
 }
@@ -770,6 +849,22 @@ void launch_project(slosh_ctx *c)
                      c->ctype, c->bp_div_max, c->bp_div_cnt, c->bp_div_sum);
     }
     c->n_div_blocks = c->nactive;
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_density_rhs(slosh_ctx *c, float relax, int keep_q)
+{
+    if (!c->nactive) return;
+    launch_timed(c, K_DENSITY, k_density_rhs, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo, c->fweight, c->ctype,
+                 c->dens_rhs, c->dens_q, relax, keep_q);
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_density_disp(slosh_ctx *c)
+{
+    if (!c->nactive) return;
+    launch_timed(c, K_DENSITY, k_density_disp, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo, c->ctype, c->dens_q,
+                 c->dens_disp);
     SLOSH_HIP(hipGetLastError());
 }
 

@@ -490,6 +490,26 @@ __global__ __launch_bounds__(kThreads) void k_fcg_update(const uint32_t *binfo, 
     block_partial<1>(v, partials);
 }
 
+// out = r - A z over fluid cells (FP64 stencil sum, FP32 result), 0 elsewhere: the residual the V-cycle's result
+// z leaves, which the network of the hybrid preconditioner corrects.
+__global__ __launch_bounds__(kThreads) void k_fcg_resid(const uint32_t *binfo, const uint8_t *fmask,
+                                                        const float *diag, const float *r, const float *z, float *out,
+                                                        const SolverState *st)
+{
+    if (st->done) return;
+    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
+        if (!(m & kSelfFluid)) {
+            out[i] = 0.0f;
+            continue;
+        }
+        const double s = nbsum<double>(B, l, m, z);
+        const double az = (double)(diag[i] + kDiagEps) * (double)z[i] - s;
+        out[i] = (float)((double)r[i] - az);
+    }
+}
+
 // Partials sum (p_ref - p)^2 and sum p^2 over fluid cells (the error of a kept solution against the current one).
 __global__ __launch_bounds__(kThreads) void k_error_partials(const uint32_t *binfo, const uint8_t *fmask,
                                                              const double *pres, const double *p_ref,
@@ -579,6 +599,11 @@ void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w)
 void l0_sor_residual(slosh_ctx *c)
 {
     L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 0);
+}
+
+void l0_fcg_resid(slosh_ctx *c, const float *r, const float *z, float *out)
+{
+    L0_LAUNCH(K_FCG, k_fcg_resid, c->diag, r, z, out, c->state);
 }
 
 void l0_error_partials(slosh_ctx *c, const double *p_ref)

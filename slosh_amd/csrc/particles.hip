@@ -657,12 +657,49 @@ struct G2PMove {
 // destination-brick flags and the speed statistics; the second recomputes the same values (the same operations on
 // the same operands) and writes the 64-byte record to its sorted place, so the records are written once and never
 // moved by a separate scatter.
+// Slosh.usf:830-853 (SampleDensityDisplacement): the face displacements of the density projection interpolated
+// trilinearly per axis at xp (stencil nodes clamped into the grid), shortened to half a cell, in metres.
+__device__ __forceinline__ void sample_displacement(const Geo &g, const float *disp, const float xp[3], float out[3])
+{
+    float dv[3];
+    for (int axis = 0; axis < 3; ++axis) {
+        float off[3] = {0.5f, 0.5f, 0.5f};
+        off[axis] = 0.0f;
+        int base[3];
+        float fr[3];
+        for (int a = 0; a < 3; ++a) {
+            const float gg = xp[a] * g.inv_h - off[a];
+            base[a] = (int)floorf(gg);
+            fr[a] = gg - (float)base[a];
+        }
+        float value = 0.0f;
+        for (uint32_t oi = 0; oi < 8u; ++oi) {
+            const int od[3] = {(int)(oi & 1u), (int)((oi >> 1) & 1u), (int)((oi >> 2) & 1u)};
+            int node[3];
+            float w3[3];
+            for (int a = 0; a < 3; ++a) {
+                const int v = base[a] + od[a];
+                node[a] = v < 0 ? 0 : (v > g.n[a] - 1 ? g.n[a] - 1 : v);
+                w3[a] = od[a] ? fr[a] : 1.0f - fr[a];
+            }
+            const uint32_t b = (uint32_t)((node[0] >> 3) + g.bd[0] * ((node[1] >> 3) + g.bd[1] * (node[2] >> 3)));
+            const uint32_t f = (uint32_t)axis * g.pool_cells + g.table[b] * kBrickCells +
+                               local_of_cell(node[0], node[1], node[2]);
+            value += w3[0] * w3[1] * w3[2] * disp[f];
+        }
+        dv[axis] = value;
+    }
+    const float len = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    const float s = len > 0.5f ? 0.5f / len : 1.0f;
+    for (int a = 0; a < 3; ++a) out[a] = (len > 0.5f ? dv[a] * s : dv[a]) * g.h;
+}
+
 template <bool kMove>
 __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off, const float *vel,
                                          const uint8_t *fflags, float dt, float gx, float gy, float gz, float friction,
                                          float radius, float max_speed, const slosh_collider *col, uint32_t ncol,
                                          uint8_t *touch, uint32_t *bp_speed, uint32_t *bp_clamp, uint32_t *keys,
-                                         uint32_t dead_key, const G2PMove &mv)
+                                         uint32_t dead_key, const G2PMove &mv, const float *disp)
 {
     __shared__ float s_box[3 * kG2PBoxNodes];
     __shared__ uint32_t s_nb[27]; // slots of the 3x3x3 brick neighbourhood, (dx+1) + 3(dy+1) + 9(dz+1)
@@ -768,6 +805,13 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
                 }
                 float x[3];
                 for (int a = 0; a < 3; ++a) x[a] = pos[a] + dt * v2[a];
+                if (disp) {
+                    // position-only shift of the density projection, sampled where the density was measured
+                    float pc[3], dx[3];
+                    for (int a = 0; a < 3; ++a) pc[a] = clampf(pos[a], lo, hi[a]);
+                    sample_displacement(g, disp, pc, dx);
+                    for (int a = 0; a < 3; ++a) x[a] = x[a] + dx[a];
+                }
                 int c[3];
                 // Every live particle of this key lies in brick `key` (its sort key is the brick of its clamped
                 // cell), so its cell is in an unallocated brick exactly when the brick is not staged.
@@ -859,19 +903,20 @@ __global__ __launch_bounds__(256) void k_g2p_keys(Geo g, Parts p, const uint32_t
                                                   float friction, float radius, float max_speed,
                                                   const slosh_collider *col, uint32_t ncol, uint8_t *touch,
                                                   uint32_t *bp_speed, uint32_t *bp_clamp, uint32_t *keys,
-                                                  uint32_t dead_key)
+                                                  uint32_t dead_key, const float *disp)
 {
     g2p_body<false>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, touch, bp_speed,
-                    bp_clamp, keys, dead_key, G2PMove{});
+                    bp_clamp, keys, dead_key, G2PMove{}, disp);
 }
 
 __global__ __launch_bounds__(256) void k_g2p_move(Geo g, Parts p, const uint32_t *key_off, const float *vel,
                                                   const uint8_t *fflags, float dt, float gx, float gy, float gz,
                                                   float friction, float radius, float max_speed,
-                                                  const slosh_collider *col, uint32_t ncol, uint32_t *keys, G2PMove mv)
+                                                  const slosh_collider *col, uint32_t ncol, uint32_t *keys, G2PMove mv,
+                                                  const float *disp)
 {
     g2p_body<true>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, nullptr, nullptr,
-                   nullptr, keys, 0u, mv);
+                   nullptr, keys, 0u, mv, disp);
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
@@ -964,7 +1009,8 @@ void launch_g2p(slosh_ctx *c, float dt, const float g[3])
     const float radius = c->s.particle_radius_cells * geo.h;
     launch_timed(c, K_G2P, k_g2p_keys, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
                  c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction, radius, c->s.max_speed,
-                 c->d_col, c->ncol, c->touch, c->bp_speed, c->bp_clamp, c->g2p_keys, c->nkeys - 1u);
+                 c->d_col, c->ncol, c->touch, c->bp_speed, c->bp_clamp, c->g2p_keys, c->nkeys - 1u,
+                 c->dens_applied ? (const float *)c->dens_disp : nullptr);
     c->n_g2p_blocks = geo.nbricks;
     SLOSH_HIP(hipGetLastError());
     // the new offsets go to the spare buffer: the second pass walks the current array by its own offsets
@@ -972,7 +1018,7 @@ void launch_g2p(slosh_ctx *c, float dt, const float g[3])
     G2PMove mv{c->parts[1 - c->cur], c->packed, c->run_base, &c->stats->misplaced};
     launch_timed(c, K_G2P_MOVE, k_g2p_move, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
                  c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction, radius, c->s.max_speed,
-                 c->d_col, c->ncol, c->g2p_keys, mv);
+                 c->d_col, c->ncol, c->g2p_keys, mv, c->dens_applied ? (const float *)c->dens_disp : nullptr);
     SLOSH_HIP(hipGetLastError());
     std::swap(c->key_off, c->key_off_next);
     c->cur = 1 - c->cur;

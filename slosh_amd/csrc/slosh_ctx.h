@@ -32,11 +32,15 @@ struct SolverState {
     double nf;       // fluid cells
     double mg_scale; // FP16 levels: the V-cycle input scale sqrt(nf / ||r||^2), so the levels hold O(1) values
     double err_rel;  // SLOSH_FLAG_REPORT_FP64_ERROR: relative L2 error of the solution against the FP64-accurate one
+    double hist[4];  // flexible CG: relative residuals of the last four iterations (stall test)
     int32_t done;  // 1 once converged or out of iterations: every solver kernel then returns at once; 2 when the
                    // MGPCG update being made is the last (only the update pass runs; the next finalize sets 1)
     int32_t iters;
     int32_t maxit;
     int32_t converged;
+    int32_t stalled; // flexible CG: stopped without converging (no halving over four iterations, the iteration
+                     // limit, or a non-finite residual): the MGPCG fallback takes over
+    int32_t pad;
 };
 
 // Multigrid level >= 1. Levels 1 and 2 are pooled (stored per allocated brick, (8 >> l)^3 cells per slot); level 3
@@ -170,7 +174,8 @@ enum KernelId {
     K_DIVERGENCE, K_PROJECT, K_DIV_STATS, K_EXTRAPOLATE, K_G2P, K_MG_SETUP, K_COARSE_FACTOR, K_PCG_INIT, K_RED_DOT,
     K_RESTRICT0, K_PROLONG_BLACK, K_DIRECTION, K_UPDATE_BLACK, K_PDOWN1, K_PDOWN2, K_PUP1, K_PUP2, K_D3DOWN, K_D3UP,
     K_DSMOOTH, K_DRESTRICT, K_DPROLONG, K_COARSE_SOLVE, K_COARSE_TAIL, K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL,
-    K_FINALIZE, K_STEP_STATS, K_UNET_CONV0, K_UNET_CONV1, K_UNET_CONV2, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_G2P_MOVE, K_COUNT
+    K_FINALIZE, K_STEP_STATS, K_UNET_CONV0, K_UNET_CONV1, K_UNET_CONV2, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_G2P_MOVE,
+    K_DENSITY, K_COUNT
 };
 // Names as the compiled kernels are called (what a kernel trace of the library shows).
 extern const char *const kKernelNames[K_COUNT];
@@ -199,6 +204,8 @@ struct UNet {
     uint16_t *p1 = nullptr, *a2 = nullptr, *b2 = nullptr, *c2 = nullptr;                                 // level 2
     float *zhat = nullptr;
     double *partials = nullptr;
+    int32_t *zero_flag = nullptr;     // a device 0: the forward kernels' skip flag outside a flexible-CG solve
+    const int32_t *skip = nullptr;    // forward kernels return at once while *skip != 0
     // training state (allocated by the first backward pass)
     float *grad = nullptr, *m = nullptr, *v = nullptr, *e = nullptr;
     uint16_t *g16a = nullptr, *g16b = nullptr, *g16c = nullptr, *g16_1a = nullptr, *g16_1b = nullptr;
@@ -285,6 +292,11 @@ struct slosh_ctx {
     double *coarse_inv = nullptr;   // [kCoarsestCells^2]
     double *pres_prev = nullptr;    // SLOSH_FLAG_WARM_START_EXTRAPOLATE: last solution per unit dt (NaN: none)
     double *pres_lp = nullptr;      // SLOSH_FLAG_REPORT_FP64_ERROR: the solution kept while the FP64-accurate one is formed
+    // SLOSH_FLAG_DENSITY_PROJECTION: the projection's right-hand side, potential and face displacements
+    float *dens_rhs = nullptr;
+    double *dens_q = nullptr;
+    float *dens_disp = nullptr;
+    bool dens_applied = false;      // this step's G2P applies the displacement
     slosh::SolverState *state_save = nullptr;
     uint32_t *coarse_sig = nullptr; // the coarsest level's cell types and indices at its last factorisation
     double *partials = nullptr;
@@ -308,6 +320,9 @@ struct slosh_ctx {
     // learned preconditioner and the residuals captured for its training
     slosh::UNet unet;
     float *fcg_z = nullptr;           // [pool_cells] preconditioned residual of the flexible CG
+    float *fcg_rp = nullptr;          // [pool_cells] the residual the V-cycle leaves (the network's input)
+    hipGraphExec_t fcg_graph = nullptr; // a flexible-CG iteration chunk, captured once per solve
+    int last_fcg_iters = 0;
     float *capture = nullptr;         // [capture_cap][pool_cells]
     uint32_t capture_cap = 0, capture_n = 0;
     int capture_iter[32] = {};        // MGPCG iteration of each captured residual
@@ -360,6 +375,8 @@ void launch_divergence(slosh_ctx *c, float dt);
 void launch_project(slosh_ctx *c);
 void launch_extrapolate(slosh_ctx *c, uint32_t layer);
 void launch_step_stats(slosh_ctx *c);
+void launch_density_rhs(slosh_ctx *c, float relax, int keep_q);
+void launch_density_disp(slosh_ctx *c);
 void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
 void launch_fill_u8(hipStream_t st, uint8_t *p, uint8_t v, size_t n);
@@ -368,6 +385,9 @@ void launch_fill_f64(hipStream_t st, double *p, double v, size_t n);
 // ---- solver.hip ----
 void run_mgpcg(slosh_ctx *c);
 void run_rbsor(slosh_ctx *c);
+// The density projection's solve A q = b, b = -dens_rhs, from q = 0, with the pressure solver's operator: two MGPCG
+// iterations (RB-SOR: density_projection_iterations sweeps at omega 1.7), as the reference's projection pass.
+void run_density_solve(slosh_ctx *c);
 
 // ---- pooled.hip: MG levels 1 and 2 in the brick pool ----
 PLevel plevel_of(const slosh_ctx *c, int l);
@@ -408,13 +428,15 @@ void l0_fcg_direction(slosh_ctx *c, const float *z, const float *d_in, float *d_
 void l0_fcg_update(slosh_ctx *c, float *r, const float *d);
 // Partials sum (p_ref - pres)^2 and sum pres^2 over fluid cells, pres the current FP64 pressure.
 void l0_error_partials(slosh_ctx *c, const double *p_ref);
+// out = r - A z over the fluid cells (the residual a preconditioner's result z leaves).
+void l0_fcg_resid(slosh_ctx *c, const float *r, const float *z, float *out);
 
 // ---- unet.hip ----
 size_t unet_param_count();
 void unet_init(slosh_ctx *c);
 void unet_free(slosh_ctx *c);
 void unet_upload_bf16(slosh_ctx *c);
-void unet_apply(slosh_ctx *c, const float *r, float *z);
+void unet_apply(slosh_ctx *c, const float *r, float *z, const float *zadd = nullptr);
 void unet_backward(slosh_ctx *c, const float *r);
 void unet_adam(slosh_ctx *c, float lr, float scale);
 

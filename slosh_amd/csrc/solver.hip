@@ -121,6 +121,8 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         st->beta = 0.0;
         st->alpha = 0.0;
         st->converged = 0;
+        st->stalled = 0;
+        for (int j = 0; j < 4; ++j) st->hist[j] = 0.0;
         st->done = 0;
     } else if (kind == kFinInit || kind == kFinSor) {
         // t0 = ||b||^2, t1 = ||r||^2
@@ -184,18 +186,26 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         st->rel_true = rel_true;
         st->converged = rel_true <= s0.tol ? 1 : 0;
     } else {
+        // kFinRr, the flexible CG's update: its stopping test, and the stall test that hands the solve to MGPCG
+        // (a residual that is not finite, the iteration limit, or no halving over the last four iterations)
         st->rr = t0;
         st->mg_scale = t0 > 0.0 ? sqrt(s0.nf / t0) : 1.0;
         const double rel = sqrt(t0) / sqrt(s0.bb);
         st->rel = rel;
         const int it = s0.iters + 1;
         st->iters = it;
-        if (rel <= s0.tol) {
+        const int k = (it - 1) & 3;
+        if (!(rel <= 1e300)) {
+            st->stalled = 1;
+            st->done = 1;
+        } else if (rel <= s0.tol) {
             st->converged = 1;
             st->done = 1;
-        } else if (it >= s0.maxit) {
+        } else if (it >= s0.maxit || (it - 1 >= 4 && rel > 0.5 * s0.hist[k])) {
+            st->stalled = 1;
             st->done = 1;
         }
+        st->hist[k] = rel;
     }
 }
 
@@ -1055,6 +1065,8 @@ __global__ void k_state_reset(SolverState *st, double tol, int maxit)
     st->rel = 0.0;
     st->rel_true = 0.0;
     st->err_rel = -1.0;
+    st->stalled = 0;
+    for (int j = 0; j < 4; ++j) st->hist[j] = 0.0;
 }
 
 // Reduction of the partials of the preceding level-0 pass: one per brick with fluid, except the initial residual,
@@ -1210,7 +1222,7 @@ void mg_counts(slosh_ctx *c)
 // Captures the launches of `body` on the context's stream into c->mg_graph (updating the executable graph in
 // place when the topology is unchanged, else instantiating it anew). On any capture failure the launches are
 // enqueued directly instead, graphs are switched off for the context, and false is returned.
-template <typename F> bool capture_chunk_graph(slosh_ctx *c, F body)
+template <typename F> bool capture_chunk_graph(slosh_ctx *c, hipGraphExec_t &exec, F body)
 {
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -1226,21 +1238,21 @@ template <typename F> bool capture_chunk_graph(slosh_ctx *c, F body)
         return false;
     }
     bool ok = false;
-    if (c->mg_graph) {
+    if (exec) {
         hipGraphNode_t err_node = nullptr;
         hipGraphExecUpdateResult res;
-        ok = hipGraphExecUpdate(c->mg_graph, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        ok = hipGraphExecUpdate(exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
         if (!ok) {
             (void)hipGetLastError();
-            (void)hipGraphExecDestroy(c->mg_graph);
-            c->mg_graph = nullptr;
+            (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
         }
     }
-    if (!ok) ok = hipGraphInstantiate(&c->mg_graph, g, nullptr, nullptr, 0) == hipSuccess;
+    if (!ok) ok = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess;
     (void)hipGraphDestroy(g);
     if (!ok) {
         (void)hipGetLastError();
-        c->mg_graph = nullptr;
+        exec = nullptr;
         c->use_graphs = false;
         body();
     }
@@ -1257,19 +1269,23 @@ void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
     float *R[2] = {c->r, c->r2};
     float *D[2] = {c->d, c->d2};
     float *const cap_buf = capture_on ? c->capture : nullptr;
+    // SLOSH_FLAG_CAPTURE_VCYCLE_RESIDUALS: capture r - A V(r), the residual the V-cycle leaves, after each V-cycle
+    // instead of the residual r before it
+    const bool after_vcycle = (c->s.flags & SLOSH_FLAG_CAPTURE_VCYCLE_RESIDUALS) != 0;
     // residuals kept for training the learned preconditioner: the initial one and those after iterations 1, 2,
     // 3, 4, 6, 8, 11, 15, 20, ... (slots past the solve's last iteration are dropped afterwards)
     uint32_t ncap = 0;
     int next_cap = 0, cap_gap = 1;
-    auto capture = [&](const float *r, int iteration) {
-        if (!cap_buf || ncap >= c->capture_cap || iteration != next_cap) return;
-        SLOSH_HIP(hipMemcpyAsync(cap_buf + (size_t)ncap * c->geo.pool_cells, r, (size_t)c->geo.pool_cells * 4u,
-                                 hipMemcpyDeviceToDevice, c->stream));
+    auto capture = [&](const float *r, int iteration, bool vcycled) {
+        if (!cap_buf || vcycled != after_vcycle || ncap >= c->capture_cap || iteration != next_cap) return;
+        float *slot = cap_buf + (size_t)ncap * c->geo.pool_cells;
+        if (vcycled) l0_fcg_resid(c, r, c->z1, slot);
+        else SLOSH_HIP(hipMemcpyAsync(slot, r, (size_t)c->geo.pool_cells * 4u, hipMemcpyDeviceToDevice, c->stream));
         c->capture_iter[ncap++] = iteration;
         next_cap += cap_gap;
         if (iteration >= 4) cap_gap++;
     };
-    capture(R[0], 0);
+    capture(R[0], 0, false);
     int rc = 1, dc = 0;
     // the first iteration's update (no alpha yet), V-cycle and direction, from the residual in R[0]
     auto first_iteration = [&]() {
@@ -1277,6 +1293,7 @@ void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
         rc = 1;
         dc = 0;
         vcycle(c, R[rc]);
+        capture(R[rc], 0, true);
         l0_direction(c, D[1], D[0], R[rc], 1);
     };
     first_iteration();
@@ -1294,8 +1311,9 @@ void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
             finalize(c, kFinAlphaStop);
             l0_update_black(c, R[rc], R[rc ^ 1], D[dc], 0);
             rc ^= 1;
-            capture(R[rc], it0 + k + 1);
+            capture(R[rc], it0 + k + 1, false);
             vcycle(c, R[rc]);
+            capture(R[rc], it0 + k + 1, true);
             l0_direction(c, D[dc], D[dc ^ 1], R[rc], 0);
             dc ^= 1;
         }
@@ -1316,7 +1334,7 @@ void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
                 // graph and updates the executable one (which takes longer than the first iteration)
                 if (use_graph && c->use_graphs && done_iters > 0) {
                     // on a capture failure the helper has enqueued the chunk directly and switched graphs off
-                    if (!graph_ready) graph_ready = capture_chunk_graph(c, [&] { enqueue_chunk(done_iters); });
+                    if (!graph_ready) graph_ready = capture_chunk_graph(c, c->mg_graph, [&] { enqueue_chunk(done_iters); });
                     if (graph_ready) SLOSH_HIP(hipGraphLaunch(c->mg_graph, c->stream));
                 } else {
                     enqueue_chunk(done_iters);
@@ -1341,8 +1359,11 @@ void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
         ahead = 1;
     }
     if (cap_buf) {
+        // the last update's residual gets no V-cycle (the solve stops): its capture after the V-cycle is dropped
         uint32_t valid = 0;
-        while (valid < ncap && c->capture_iter[valid] <= c->h_state->iters) valid++;
+        while (valid < ncap && (after_vcycle ? c->capture_iter[valid] < c->h_state->iters
+                                             : c->capture_iter[valid] <= c->h_state->iters))
+            valid++;
         c->capture_n = valid;
     }
 }
@@ -1391,11 +1412,14 @@ void run_mgpcg(slosh_ctx *c)
     if (c->s.flags & SLOSH_FLAG_REPORT_FP64_ERROR) reference_error(c, maxit);
 }
 
-// Flexible CG (Notay 2000, FCG(1)) preconditioned by the network: z = M(r), d = z - (z.q_prev / d_prev.q_prev) d_prev,
-// alpha = d.r / d.q. MGPCG is the fallback and the judge: when the relative residual has not halved over the last
-// four iterations, or an iteration limit is reached, or the recurrence converges but the FP64 true residual
-// ||b - A p|| / ||b|| of the pressure exceeds the tolerance, MGPCG continues from the current pressure to the
-// tolerance. Every accepted solve therefore has a true residual within the tolerance.
+// Flexible CG (Notay 2000, FCG(1)) with the hybrid learned preconditioner M(r) = V(r) + N(r - A V(r)): the MGPCG
+// V-cycle, and the network's correction of the residual the V-cycle leaves. z = M(r), d = z - (z.q_prev /
+// d_prev.q_prev) d_prev, alpha = d.r / d.q. Iterations are enqueued in chunks between reads of the device state
+// (every kernel, the network's included, returns at once once the solve is done), the chunk replayed as a captured
+// HIP graph. MGPCG is the fallback and the judge: when the residual stops halving over four iterations, is not
+// finite or the iteration limit is reached, MGPCG solves from the current pressure; a solve whose recurrence
+// converged restarts from r = b - A p until its FP64 true residual meets the tolerance too (a few times at most,
+// then MGPCG). Every accepted solve therefore has a true residual within the tolerance.
 void run_fcg(slosh_ctx *c)
 {
     c->fcg_iters = 0;
@@ -1408,59 +1432,128 @@ void run_fcg(slosh_ctx *c)
     }
     const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 200;
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
-    l0_brick_info(c);
-    float *D[2] = {c->d, c->d2};
-    float *r = c->r;
-    l0_pcg_init(c, r);
+    mg_setup(c);
+    l0_pcg_init(c, c->r);
     finalize(c, kFinInit);
+    mg_counts(c);
     if (!c->nsolve) return;
+    UNet &u = c->unet;
+    struct SkipGuard {
+        UNet &u;
+        ~SkipGuard() { u.skip = u.zero_flag; }
+    } guard{u};
+    u.skip = (const int32_t *)((const char *)c->state + offsetof(SolverState, done));
+    float *D[2] = {c->d, c->d2};
     int dc = 0;
-    double hist[4] = {0, 0, 0, 0};
-    bool fallback = false;
-    for (int it = 0; it < maxit; ++it) {
-        unet_apply(c, r, c->fcg_z);
-        if (it > 0) {
+    auto iteration = [&](bool first) {
+        l0_update_black(c, c->r, c->r2, D[dc], 1); // r2 = r; z: red r / diag, then the black sweep
+        vcycle(c, c->r2);                          // z1 = V(r)
+        l0_fcg_resid(c, c->r2, c->z1, c->fcg_rp);  // r' = r - A z1
+        unet_apply(c, c->fcg_rp, c->fcg_z, c->z1); // z = z1 + N(r')
+        if (!first) {
             l0_fcg_zq(c, c->fcg_z);
             finalize(c, kFinBetaF);
         }
-        l0_fcg_direction(c, c->fcg_z, D[dc], D[dc ^ 1], r, it == 0);
+        l0_fcg_direction(c, c->fcg_z, D[dc], D[dc ^ 1], c->r, first);
         dc ^= 1;
         finalize(c, kFinAlphaF);
-        l0_fcg_update(c, r, D[dc]);
+        l0_fcg_update(c, c->r, D[dc]);
         finalize(c, kFinRr);
-        SLOSH_HIP(hipGetLastError());
+    };
+    const int chunk = 2; // even: the direction buffers return to the same roles
+    const bool use_graph = c->use_graphs && c->timing_mask == 0;
+    bool graph_ready = false;
+    auto read_state = [&]() {
         SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
-        const SolverState &h = *c->h_state;
-        c->fcg_iters = h.iters;
-        if (!std::isfinite(h.rel)) {
-            launch_fill_f64(c->stream, c->pres, 0.0, c->geo.pool_cells);
+    };
+    bool fallback = false;
+    for (int restart = 0;; ++restart) {
+        dc = 0;
+        iteration(true);
+        int enqueued = 1;
+        int ahead = c->last_fcg_iters > 1 ? (c->last_fcg_iters + chunk - 2) / chunk : 1;
+        for (;;) {
+            for (int k = 0; k < ahead && enqueued < maxit; ++k, enqueued += chunk) {
+                if (use_graph && c->use_graphs) {
+                    if (!graph_ready)
+                        graph_ready = capture_chunk_graph(c, c->fcg_graph, [&] {
+                            for (int j = 0; j < chunk; ++j) iteration(false);
+                        });
+                    if (graph_ready) SLOSH_HIP(hipGraphLaunch(c->fcg_graph, c->stream));
+                } else {
+                    for (int j = 0; j < chunk; ++j) iteration(false);
+                }
+                SLOSH_HIP(hipGetLastError());
+            }
+            ahead = 1;
+            read_state();
+            if (c->h_state->done || enqueued >= maxit) break;
+        }
+        c->fcg_iters = c->h_state->iters;
+        if (c->h_state->stalled || !c->h_state->converged) {
             fallback = true;
             break;
         }
-        if (h.done) {
-            fallback = !h.converged;
-            break;
-        }
-        if (it >= 4 && h.rel > 0.5 * hist[it % 4]) {
-            fallback = true;
-            break;
-        }
-        hist[it % 4] = h.rel;
-        if (it + 1 == maxit) fallback = true;
-    }
-    if (!fallback) {
-        // MGPCG as the judge: the true residual of the accepted pressure
         l0_true_residual(c);
         finalize(c, kFinCheck);
-        SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
-        SLOSH_HIP(hipStreamSynchronize(c->stream));
-        if (!c->h_state->converged) fallback = true; // kFinCheck: true residual within the tolerance
+        read_state();
+        if (c->h_state->converged) break;
+        if (restart >= 3 || c->h_state->iters >= maxit) {
+            fallback = true;
+            break;
+        }
+        l0_pcg_init(c, c->r);
+        finalize(c, kFinRestart);
     }
+    c->last_fcg_iters = c->fcg_iters;
     if (fallback) {
+        if (!std::isfinite(c->h_state->rel)) launch_fill_f64(c->stream, c->pres, 0.0, c->geo.pool_cells);
         c->fcg_fallback = 1;
         run_mgpcg(c);
+    } else if (c->s.flags & SLOSH_FLAG_REPORT_FP64_ERROR) {
+        reference_error(c, maxit);
     }
+}
+
+// SloshProxy.cpp:656-683: the density projection's solve, run by the pressure solver on the same operator with
+// Divergence := the projection's right-hand side and Pressure := q (from 0): two MGPCG iterations, or
+// density_projection_iterations RB-SOR sweeps at omega 1.7. MGPCG (and the flexible CG, whose V-cycle hierarchy is
+// the same) reuse the multigrid set-up of the step's pressure solve. No stopping test (tolerance 0), as in the
+// reference.
+void run_density_solve(slosh_ctx *c)
+{
+    if (!c->nactive || !c->nsolve) return;
+    struct Swap {
+        slosh_ctx *c;
+        float *div;
+        double *pres;
+        int last;
+        ~Swap()
+        {
+            c->div = div;
+            c->pres = pres;
+            c->last_mg_iters = last;
+        }
+    } keep{c, c->div, c->pres, c->last_mg_iters};
+    c->div = c->dens_rhs;
+    c->pres = c->dens_q;
+    if (c->s.pressure_solver == SLOSH_SOLVER_RBSOR) {
+        const int iters = c->s.density_projection_iterations > 0 ? c->s.density_projection_iterations : 1;
+        hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, 0.0, iters);
+        for (int it = 0; it < iters; ++it) {
+            l0_sor_sweep(c, 0u, (double)1.7f);
+            l0_sor_sweep(c, 1u, (double)1.7f);
+        }
+        SLOSH_HIP(hipGetLastError());
+        return;
+    }
+    constexpr int kIters = 2;
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, 0.0, kIters);
+    l0_pcg_init(c, c->r);
+    finalize(c, kFinInit);
+    c->last_mg_iters = kIters;
+    mgpcg_iterate(c, kIters, 0, false);
 }
 
 // Slosh.usf:856-892 (PressureRBGSCS) iterated to the tolerance, residual checked every 8 iterations.

@@ -62,8 +62,10 @@ template <int S> __device__ __forceinline__ uint32_t nb_cell(const uint32_t *nb2
 // ---- convolution (forward, and backward w.r.t. the input with mirrored/transposed weights) -----------------
 template <int CIN, int COUT, int S, bool RELU, bool BIAS>
 __global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nbricks, const uint16_t *in,
-                                              const uint16_t *w, const float *bias, uint16_t *out)
+                                              const uint16_t *w, const float *bias, uint16_t *out,
+                                              const int32_t *skip)
 {
+    if (*skip) return;
     constexpr int BG = group_bricks(S), SB = S + 2, BOX = SB * SB * SB, CPB = S * S * S, M = BG * CPB;
     constexpr int MT = M / 16, NT = COUT / 16, KB = CIN / 16, MTW = MT / 4, CH8 = CIN / 8, LS = lds_stride(CIN);
     static_assert(MT % 4 == 0 && COUT % 16 == 0 && CIN % 16 == 0, "conv tiling");
@@ -150,8 +152,10 @@ __device__ __forceinline__ bf8 ld8(const uint16_t *p)
 __device__ __forceinline__ void st8(uint16_t *p, const bf8 &v) { *(uint4 *)p = v.v; }
 
 // Level l (S cells per axis) -> level l + 1: mean over the 2x2x2 children (same brick), child order x fastest.
-template <int C, int S> __global__ void k_pool(const uint32_t *binfo, uint32_t nbricks, const uint16_t *in, uint16_t *out)
+template <int C, int S>
+__global__ void k_pool(const uint32_t *binfo, uint32_t nbricks, const uint16_t *in, uint16_t *out, const int32_t *skip)
 {
+    if (*skip) return;
     constexpr int SC = S / 2, CPBC = SC * SC * SC, CPB = S * S * S;
     uint32_t slot, l, c8;
     if (!cell_chunk<C>(binfo, nbricks, CPBC, blockIdx.x * blockDim.x + threadIdx.x, slot, l, c8)) return;
@@ -176,8 +180,10 @@ __device__ __forceinline__ uint32_t parent_local(uint32_t l, int S)
 
 // t = ReLU(up(c) + s) at level l (S per axis); c lives at level l + 1.
 template <int C, int S>
-__global__ void k_upadd(const uint32_t *binfo, uint32_t nbricks, const uint16_t *c, const uint16_t *s, uint16_t *t)
+__global__ void k_upadd(const uint32_t *binfo, uint32_t nbricks, const uint16_t *c, const uint16_t *s, uint16_t *t,
+                        const int32_t *skip)
 {
+    if (*skip) return;
     constexpr int CPB = S * S * S, CPBC = CPB / 8;
     uint32_t slot, l, c8;
     if (!cell_chunk<C>(binfo, nbricks, CPB, blockIdx.x * blockDim.x + threadIdx.x, slot, l, c8)) return;
@@ -190,8 +196,9 @@ __global__ void k_upadd(const uint32_t *binfo, uint32_t nbricks, const uint16_t 
 
 // Scale of the input: per-brick partials of sum r^2 and fluid count over fluid cells.
 __global__ __launch_bounds__(256) void k_unet_norm(const uint32_t *binfo, const uint8_t *fmask, const float *r,
-                                                   double *partials)
+                                                   double *partials, const int32_t *skip)
 {
+    if (*skip) return;
     const uint32_t base = binfo[(size_t)xcd_remap(blockIdx.x, gridDim.x) * kBinfoWords + 13] * kBrickCells;
     double v0 = 0.0, v1 = 0.0;
     for (uint32_t l = threadIdx.x; l < kBrickCells; l += 256u) {
@@ -219,8 +226,10 @@ __global__ __launch_bounds__(256) void k_unet_norm(const uint32_t *binfo, const 
 
 // Sums n partial pairs in a fixed order; mode 0: scale (s, 1/s) of the input; mode 1: the loss terms
 // (sum e^2, sum rhat^2).
-__global__ __launch_bounds__(256) void k_unet_finalize(const double *partials, uint32_t n, int mode, UNetScalars *sc)
+__global__ __launch_bounds__(256) void k_unet_finalize(const double *partials, uint32_t n, int mode, UNetScalars *sc,
+                                                       const int32_t *skip)
 {
+    if (*skip) return;
     __shared__ double sh[2][256];
     double a = 0.0, b = 0.0;
     for (uint32_t k = threadIdx.x; k < n; k += 256u) {
@@ -252,8 +261,9 @@ __global__ __launch_bounds__(256) void k_unet_finalize(const double *partials, u
 
 // Network input at level 0: channels (r / s on fluid cells, fluid, air, solid, 0 x 12).
 __global__ __launch_bounds__(256) void k_unet_input(const uint32_t *binfo, const uint32_t *ctype, const float *r,
-                                                    const UNetScalars *sc, uint16_t *x0)
+                                                    const UNetScalars *sc, uint16_t *x0, const int32_t *skip)
 {
+    if (*skip) return;
     const uint32_t base = binfo[(size_t)xcd_remap(blockIdx.x, gridDim.x) * kBinfoWords + 13] * kBrickCells;
     const float s_inv = sc->s_inv;
     for (uint32_t l = threadIdx.x; l < kBrickCells; l += 256u) {
@@ -277,11 +287,13 @@ __global__ __launch_bounds__(256) void k_unet_input(const uint32_t *binfo, const
     }
 }
 
-// Head: zhat = sum_c W[c] d0[c] + b on fluid cells (0 elsewhere); z = s * zhat.
+// Head: zhat = sum_c W[c] d0[c] + b on fluid cells (0 elsewhere); z = s * zhat, plus zadd when given (the
+// V-cycle's part of the hybrid preconditioner).
 __global__ __launch_bounds__(256) void k_unet_head(const uint32_t *binfo, const uint8_t *fmask, const uint16_t *d0,
                                                    const uint16_t *wh, const float *bh, const UNetScalars *sc,
-                                                   float *zhat, float *z)
+                                                   float *zhat, float *z, const float *zadd, const int32_t *skip)
 {
+    if (*skip) return;
     const uint32_t base = binfo[(size_t)xcd_remap(blockIdx.x, gridDim.x) * kBinfoWords + 13] * kBrickCells;
     float w[16];
     for (int k = 0; k < 16; ++k) w[k] = bf2f(wh[k]);
@@ -297,7 +309,7 @@ __global__ __launch_bounds__(256) void k_unet_head(const uint32_t *binfo, const 
             zh = acc + b;
         }
         if (zhat) zhat[i] = zh;
-        if (z) z[i] = s * zh;
+        if (z) z[i] = zadd ? zadd[i] + s * zh : s * zh;
     }
 }
 
@@ -560,7 +572,7 @@ void conv(slosh_ctx *c, const uint16_t *in, const uint16_t *w, const float *bias
     const uint32_t groups = (c->nactive + BG - 1) / BG;
     KTimer timer(c, S == 8 ? K_UNET_CONV0 : (S == 4 ? K_UNET_CONV1 : K_UNET_CONV2));
     hipLaunchKernelGGL((k_conv<CIN, COUT, S, RELU, BIAS>), dim3(groups), dim3(256), 0, c->stream, c->binfo, c->nactive,
-                       in, w, bias, out);
+                       in, w, bias, out, c->unet.skip);
 }
 
 template <int CIN, int COUT, int S, int CINW = CIN>
@@ -615,6 +627,8 @@ void unet_init(slosh_ctx *c)
     ualloc(&u.c2, n2 * 32);
     ualloc(&u.zhat, n0);
     ualloc(&u.partials, 2 * (size_t)c->geo.npool + 2);
+    ualloc(&u.zero_flag, 1);
+    u.skip = u.zero_flag;
 }
 
 void unet_free(slosh_ctx *c)
@@ -625,7 +639,7 @@ void unet_free(slosh_ctx *c)
                     (void *)u.p1, (void *)u.a2, (void *)u.b2, (void *)u.c2, (void *)u.zhat, (void *)u.partials,
                     (void *)u.grad, (void *)u.m, (void *)u.v, (void *)u.e, (void *)u.g16a, (void *)u.g16b,
                     (void *)u.g16c, (void *)u.g16_1a, (void *)u.g16_1b, (void *)u.g32a, (void *)u.g32b, (void *)u.g32c,
-                    (void *)u.g32_2a, (void *)u.g32_2b, (void *)u.g64a, (void *)u.g64b})
+                    (void *)u.g32_2a, (void *)u.g32_2b, (void *)u.g64a, (void *)u.g64b, (void *)u.zero_flag})
         if (p) (void)hipFree(p);
     u = UNet{};
 }
@@ -645,9 +659,10 @@ void unet_upload_bf16(slosh_ctx *c)
     SLOSH_HIP(hipGetLastError());
 }
 
-// z = M(r): the network applied to the level-0 vector r (pool layout); z on fluid cells, 0 elsewhere. With
-// zhat != null the normalised head output is kept too (training).
-void unet_apply(slosh_ctx *c, const float *r, float *z)
+// z = N(r) (+ zadd): the network applied to the level-0 vector r (pool layout); z on fluid cells (zadd, or 0,
+// elsewhere). The normalised head output is kept in u.zhat (training). Every kernel returns at once while *u.skip
+// is set (the flexible-CG solve's done flag, so chunks enqueued past convergence cost only their launches).
+void unet_apply(slosh_ctx *c, const float *r, float *z, const float *zadd)
 {
     UNet &u = c->unet;
     const uint32_t nb = c->nactive;
@@ -656,36 +671,38 @@ void unet_apply(slosh_ctx *c, const float *r, float *z)
     auto bias = [&](int k) { return u.w32 + u.b_off[k]; };
     {
         SLOSH_TIMED(c, K_UNET_IO);
-        hipLaunchKernelGGL(k_unet_norm, dim3(nb), dim3(256), 0, c->stream, c->binfo, c->fmask, r, u.partials);
-        hipLaunchKernelGGL(k_unet_finalize, dim3(1), dim3(256), 0, c->stream, u.partials, nb, 0, u.sc);
-        hipLaunchKernelGGL(k_unet_input, dim3(nb), dim3(256), 0, c->stream, c->binfo, c->ctype, r, u.sc, u.x0);
+        hipLaunchKernelGGL(k_unet_norm, dim3(nb), dim3(256), 0, c->stream, c->binfo, c->fmask, r, u.partials, u.skip);
+        hipLaunchKernelGGL(k_unet_finalize, dim3(1), dim3(256), 0, c->stream, u.partials, nb, 0, u.sc, u.skip);
+        hipLaunchKernelGGL(k_unet_input, dim3(nb), dim3(256), 0, c->stream, c->binfo, c->ctype, r, u.sc, u.x0, u.skip);
     }
     conv<16, 16, 8, true, true>(c, u.x0, wf(0), bias(0), u.a0);
     conv<16, 16, 8, true, true>(c, u.a0, wf(1), bias(1), u.s0);
     {
-        launch_timed(c, K_UNET_IO, (k_pool<16, 8>), dim3(chunks(c, 1, 16)), dim3(256), 0, c->binfo, nb, u.s0, u.p0);
+        launch_timed(c, K_UNET_IO, (k_pool<16, 8>), dim3(chunks(c, 1, 16)), dim3(256), 0, c->binfo, nb, u.s0, u.p0,
+                     u.skip);
     }
     conv<16, 32, 4, true, true>(c, u.p0, wf(2), bias(2), u.a1);
     conv<32, 32, 4, true, true>(c, u.a1, wf(3), bias(3), u.s1);
     {
-        launch_timed(c, K_UNET_IO, (k_pool<32, 4>), dim3(chunks(c, 2, 32)), dim3(256), 0, c->binfo, nb, u.s1, u.p1);
+        launch_timed(c, K_UNET_IO, (k_pool<32, 4>), dim3(chunks(c, 2, 32)), dim3(256), 0, c->binfo, nb, u.s1, u.p1,
+                     u.skip);
     }
     conv<32, 64, 2, true, true>(c, u.p1, wf(4), bias(4), u.a2);
     conv<64, 64, 2, true, true>(c, u.a2, wf(5), bias(5), u.b2);
     conv<64, 32, 2, false, true>(c, u.b2, wf(6), bias(6), u.c2);
     {
         launch_timed(c, K_UNET_IO, (k_upadd<32, 4>), dim3(chunks(c, 1, 32)), dim3(256), 0, c->binfo, nb, u.c2, u.s1,
-                     u.t1);
+                     u.t1, u.skip);
     }
     conv<32, 32, 4, true, true>(c, u.t1, wf(7), bias(7), u.d1);
     conv<32, 16, 4, false, true>(c, u.d1, wf(8), bias(8), u.c1);
     {
         launch_timed(c, K_UNET_IO, (k_upadd<16, 8>), dim3(chunks(c, 0, 16)), dim3(256), 0, c->binfo, nb, u.c1, u.s0,
-                     u.t0);
+                     u.t0, u.skip);
     }
     conv<16, 16, 8, true, true>(c, u.t0, wf(9), bias(9), u.d0);
     launch_timed(c, K_UNET_IO, k_unet_head, dim3(nb), dim3(256), 0, c->binfo, c->fmask, u.d0, wf(10), bias(10), u.sc,
-                 u.zhat, z);
+                 u.zhat, z, zadd, u.skip);
 }
 
 // Accumulates dL/dparams for the system (fmask, diag) and residual r into u.grad; the loss terms go to u.sc.
@@ -718,7 +735,7 @@ void unet_backward(slosh_ctx *c, const float *r)
     SLOSH_TIMED(c, K_UNET_TRAIN);
     hipLaunchKernelGGL(k_unet_loss_e, dim3(nb), dim3(256), 0, c->stream, c->binfo, c->fmask, c->diag, u.zhat, u.x0, u.e,
                        u.partials);
-    hipLaunchKernelGGL(k_unet_finalize, dim3(1), dim3(256), 0, c->stream, u.partials, nb, 1, u.sc);
+    hipLaunchKernelGGL(k_unet_finalize, dim3(1), dim3(256), 0, c->stream, u.partials, nb, 1, u.sc, u.skip);
     hipLaunchKernelGGL(k_unet_head_bwd, dim3(nb), dim3(256), 0, c->stream, c->binfo, c->fmask, c->diag, u.e, u.d0,
                        W + u.wf_off[10], u.sc, u.g16a, u.grad + u.w_off[10]);
     // D0b (input t0): g16a = dL/d(pre-activation of d0)

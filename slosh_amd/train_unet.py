@@ -1,9 +1,12 @@
 """Trains the learned preconditioner on the GPU with the library's own forward/backward kernels.
 
-Training data are pressure systems that the plain solver generates from seeded scenes: each scene is stepped with
-MGPCG, the residuals of every solve are captured (SLOSH_FLAG_CAPTURE_RESIDUALS), and a few Adam steps are taken on
-them before the next step, so the network sees the systems of whole trajectories (sloshing tanks with random
-fill depth, wave amplitude and tank acceleration; dam breaks). Usage:
+The network is the second stage of the hybrid preconditioner M(r) = V(r) + N(r - A V(r)): it learns to correct what
+the multigrid V-cycle leaves. Training data are pressure systems that the plain solver generates from seeded
+scenes: each scene is stepped with MGPCG (with the V-cycle options the flexible CG uses), the residuals each V-cycle
+leaves are captured (SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_CAPTURE_VCYCLE_RESIDUALS), and a few Adam steps
+minimise ||A N(r') - r'|| / ||r'|| on them before the next step (= ||A M r - r|| / ||r - A V r||), so the network
+sees the systems of whole trajectories (sloshing tanks with random fill depth, wave amplitude and tank
+acceleration; dam breaks). Usage:
 
     python -m slosh_amd.train_unet --out slosh_amd/weights/unet_slosh.npz --seconds 300
 """
@@ -17,6 +20,11 @@ import time
 import numpy as np
 
 from . import Sim, abi, scenes, unet
+
+# V-cycle options of the training solves: those the flexible CG of the benchmark uses (fluid-first coarse types,
+# three red-black pairs on the coarse levels)
+TRAIN_MG_FLAGS = abi.FLAG_MG_FLUID_COARSENING
+TRAIN_MG_SWEEPS = 3
 
 
 def scene(kind, n, rng):
@@ -45,7 +53,9 @@ def train(out, seconds, sizes=(64, 128), steps_per_scene=24, iters_per_step=6, b
         n = int(sizes[nscene % len(sizes)])
         grid, h, reqs, dt, grav = scene(kind, n, rng)
         cap = sum(int(np.prod(r.cell_dims)) * 8 for r in reqs) + 1024
-        s = abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6, flags=abi.FLAG_CAPTURE_RESIDUALS)
+        s = abi.make_settings(grid, h, cap, solver=abi.SOLVER_MGPCG, tolerance=1e-6,
+                              flags=abi.FLAG_CAPTURE_RESIDUALS | abi.FLAG_CAPTURE_VCYCLE_RESIDUALS | TRAIN_MG_FLAGS,
+                              mg_coarse_sweeps=TRAIN_MG_SWEEPS)
         sim = Sim(s)
         for r in reqs:
             sim.emit(r)

@@ -102,11 +102,29 @@ def fcg_run(weights, steps=3):
 
 
 def test_fcg_stall_falls_back_to_mgpcg():
-    # a zero network gives z = 0: no progress, the stall test hands the solve to MGPCG, which meets the tolerance
-    sim, res, _ = fcg_run(np.zeros(unet.param_count(), np.float32))
+    # a network whose correction swamps the V-cycle's (its head scaled by 1e4): the residual stops halving, the
+    # stall test hands the solve to MGPCG from the current pressure, which meets the tolerance
+    w = unet.he_init(0)
+    w[-17:] *= 1e4
+    sim, res, _ = fcg_run(w)
     for r in res:
-        assert r.fallback_used == 1 and r.converged == 1 and r.residual_rel <= 1e-6, r.as_dict()
+        assert r.fallback_used == 1 and r.converged == 1 and r.true_residual_rel <= 1e-6, r.as_dict()
     sim.close()
+
+
+def test_fcg_with_a_zero_network_is_vcycle_pcg():
+    # N = 0 leaves M = V, the MGPCG preconditioner: flexible CG then takes MGPCG's iteration count (within 1, the
+    # flexible beta and MGPCG's stopping test differ in rounding) and needs no fallback
+    zero = np.zeros(unet.param_count(), np.float32)
+    sim, res, _ = fcg_run(zero, steps=2)
+    mg = make_sim(steps=0)
+    for r in res:
+        mg.step(1.0 / 120.0, (0.0, -9.8, 0.0))
+        rm = mg.results()
+        assert r.fallback_used == 0 and r.converged == 1 and r.true_residual_rel <= 1e-6, r.as_dict()
+        assert abs(r.solver_iterations - rm.solver_iterations) <= 1, (r.solver_iterations, rm.solver_iterations)
+    sim.close()
+    mg.close()
 
 
 def test_fcg_without_weights_is_mgpcg():
@@ -116,10 +134,10 @@ def test_fcg_without_weights_is_mgpcg():
 
 
 def test_fcg_accepted_solve_meets_tolerance_and_matches_fp64():
-    # a trained network: every accepted solve (with or without fallback) meets the MGPCG stopping rule, and the
-    # pressure matches the oracle's FP64 solve of the same system
-    trainer = make_sim(flags=abi.FLAG_CAPTURE_RESIDUALS, steps=2)
-    trainer.unet_set_weights(unet.he_init(0))
+    # a network trained on the residuals the V-cycle leaves: the flexible CG finishes the solve itself (no MGPCG
+    # fallback), its FP64 true residual meets the tolerance, and the pressure matches the oracle's FP64 solve
+    trainer = make_sim(flags=abi.FLAG_CAPTURE_RESIDUALS | abi.FLAG_CAPTURE_VCYCLE_RESIDUALS, steps=2)
+    trainer.unet_set_weights(unet.he_init(0, head_scale=0.1))
     trainer.unet_train(iterations=60, batch=4, learning_rate=2e-3)
     w = trainer.unet_get_weights()
     trainer.close()
@@ -132,10 +150,10 @@ def test_fcg_accepted_solve_meets_tolerance_and_matches_fp64():
     for sim in (gpu, orc):
         for r in reqs:
             sim.emit(r)
-    # the oracle runs the same step with SOLVER_NONE semantics up to the solve: compare the FP64 reference solve
     gpu.step(1.0 / 120.0, (0.0, -9.8, 0.0))
     res = gpu.results()
-    assert res.converged == 1 and res.residual_rel <= 1e-6, res.as_dict()
+    assert res.fallback_used == 0, res.as_dict()
+    assert res.converged == 1 and res.true_residual_rel <= 1e-6, res.as_dict()
     orc.step(1.0 / 120.0, (0.0, -9.8, 0.0))
     ref, _, _ = orc.solve_fp64(1e-13, 400)
     p = gpu.read_grid(abi.FIELD_PRESSURE_F64)
@@ -143,6 +161,34 @@ def test_fcg_accepted_solve_meets_tolerance_and_matches_fp64():
     assert err <= 1e-5, err
     gpu.close()
     orc.close()
+
+
+def test_vcycle_residual_capture_is_r_minus_a_v_of_r():
+    # SLOSH_FLAG_CAPTURE_VCYCLE_RESIDUALS: the first captured vector is b - A V(b) from rest, with V the V-cycle;
+    # checked against the numpy restatement of the V-cycle (tests/mg_numpy.py, FP64) on the step's own system
+    import mg_numpy as M
+
+    grid, h, reqs = scenes.dam_break(32, seed=0)
+    from slosh_amd import Sim
+
+    s = abi.make_settings(grid, h, 8 * 32 ** 3, solver=abi.SOLVER_MGPCG, tolerance=1e-6,
+                          flags=abi.FLAG_CAPTURE_RESIDUALS | abi.FLAG_CAPTURE_VCYCLE_RESIDUALS)
+    sim = Sim(s)
+    for r in reqs:
+        sim.emit(r)
+    sim.step(1.0 / 120.0, (0.0, -9.8, 0.0))
+    rp, n = sim.captured_residual(0)
+    assert n >= 4
+    t0 = (sim.read_grid(abi.FIELD_CELL_FLAGS) & 3).astype(np.int32)
+    diag = sim.read_grid(abi.FIELD_CELL_DIAG).astype(np.float64)
+    b = np.where(t0 == M.FLUID, -sim.read_grid(abi.FIELD_DIVERGENCE).astype(np.float64), 0.0)
+    A = M.Level(t0, diag)
+    V = M.VCycle(t0, diag)
+    ref = np.where(A.fluid, b - A.apply(V(b)), 0.0)
+    err = np.linalg.norm(rp - ref) / np.linalg.norm(ref)
+    assert err <= 1e-4, err
+    assert np.linalg.norm(ref) < np.linalg.norm(b)
+    sim.close()
 
 
 def test_training_data_are_the_solver_systems():
