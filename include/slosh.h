@@ -75,7 +75,10 @@ typedef struct slosh_settings {
     float density_relaxation;             /* fraction of the density error removed per projection, [0, 1] */
     int32_t density_projection_iterations; /* RB-SOR sweeps of its solve (omega 1.7), 0..10000; MGPCG uses 2 iterations */
     int32_t density_projection_interval;  /* project every N steps (1..8; 0 means 1), relaxation compounded */
-    int32_t reserved[2];          /* must be zero */
+    float kinematic_viscosity;    /* m^2/s; > 0 runs viscosity_iterations Jacobi sweeps of (I - nu dt Laplacian) u = u*
+                                     after the normalise pass (Slosh.usf:657-686, SloshProxy.cpp:603-617) */
+    int32_t viscosity_iterations; /* 0..1000 */
+    int32_t reserved[8];          /* must be zero */
 } slosh_settings;
 
 #define SLOSH_FLAG_NONE 0u
@@ -106,7 +109,10 @@ typedef struct slosh_settings {
                                             relative L2 error against it (slosh_results.fp64_error_rel); the first
                                             solution is kept. A diagnostic: it costs a few solves per step */
 
-/* Emission request: FSloshEmitRequest (SloshTypes.h:169-181). shape 0 = FillCells (8 jittered particles per cell). */
+/* Emission request: FSloshEmitRequest (SloshTypes.h:169-181). shape 0 = FillCells (8 jittered particles per cell),
+ * 1 = `count` random points in the sphere of radius extent[0] about center, 2 = `count` random points in the box of
+ * half extents `extent` rotated by `rotation` about center (Slosh.usf:280-358). A nonzero velocity_spread (cone half
+ * angle, radians) or speed_jitter (+- fraction of the speed) varies each particle's velocity in every mode. */
 enum { SLOSH_EMIT_FILL_CELLS = 0, SLOSH_EMIT_SPHERE = 1, SLOSH_EMIT_BOX = 2 };
 typedef struct slosh_emit_request {
     uint32_t shape;
@@ -136,6 +142,27 @@ typedef struct slosh_collider {
     float pad1;
 } slosh_collider;
 #define SLOSH_MAX_COLLIDERS 64u
+
+/* Particle manipulator: FSloshManipulatorGPU (SloshTypes.h:66-80), 80 bytes. kind 0 drain (removes the particles
+ * inside: all at once when param0 = 0, else at rate param0 per second), 1 directional (acceleration strength along
+ * axis), 2 radial (strength away from center), 3 swirl (strength around axis, param0 the inward pull ratio), 4 drag
+ * (velocity times exp(-strength dt)). shape 0 sphere, 1 box, 2 capsule (as colliders), 3 everywhere. The effect is
+ * full inside the shape and fades linearly to 0 over `falloff` metres beyond it (Slosh.usf:1523-1559, 1639-1654;
+ * SloshCommon.ush:148-230). */
+typedef struct slosh_manipulator {
+    float center[3];
+    uint32_t kind;
+    float rotation[4];
+    float extent[3];
+    uint32_t shape;
+    float axis[3];
+    float strength;
+    float falloff;
+    float param0;
+    float param1;
+    uint32_t pad;
+} slosh_manipulator;
+#define SLOSH_MAX_MANIPULATORS 32u
 
 /* Particle records returned by slosh_read_particles. */
 enum { SLOSH_LAYOUT_F32_64 = 0, SLOSH_LAYOUT_REF_48 = 1 };
@@ -192,7 +219,7 @@ typedef struct slosh_results {
     float true_residual_rel;       /* ||b - A p||_2 / ||b||_2 of the FP64 pressure after the solve */
     double fp64_error_rel;         /* SLOSH_FLAG_REPORT_FP64_ERROR: ||p - p64|| / ||p64|| over fluid cells against the
                                       FP64-accurate solve of the same system; -1 when not computed */
-    int32_t reserved[2];
+    uint64_t drained_particles;    /* particles removed by drain manipulators since the last reset */
 } slosh_results;
 
 /* ---- lifecycle --------------------------------------------------------------------------------------------- */
@@ -210,6 +237,8 @@ slosh_status slosh_reset(slosh_ctx *ctx);
 slosh_status slosh_emit(slosh_ctx *ctx, const slosh_emit_request *r);
 /* = FSloshFrameInput::Colliders (SloshTypes.h:187); n <= SLOSH_MAX_COLLIDERS; kept until replaced. */
 slosh_status slosh_set_colliders(slosh_ctx *ctx, const slosh_collider *c, uint32_t n);
+/* = FSloshFrameInput::Manipulators (SloshTypes.h:189); n <= SLOSH_MAX_MANIPULATORS; kept until replaced. */
+slosh_status slosh_set_manipulators(slosh_ctx *ctx, const slosh_manipulator *m, uint32_t n);
 
 /* ---- the hot path -------------------------------------------------------------------------------------------- */
 /* One substep of Step_RenderThread (SloshProxy.cpp:564-713) with a fixed dt and the given gravity (m/s^2):

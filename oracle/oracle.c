@@ -101,6 +101,9 @@ struct osim {
     size_t part_cap;
     slosh_collider col[SLOSH_MAX_COLLIDERS];
     uint32_t ncol;
+    slosh_manipulator man[SLOSH_MAX_MANIPULATORS];
+    uint32_t nman;
+    uint64_t drained_total;
     slosh_results res;
     uint64_t frame;
     float prev_dt;
@@ -429,6 +432,89 @@ int osim_set_colliders(osim *o, const slosh_collider *c, uint32_t n)
     return SLOSH_OK;
 }
 
+int osim_set_manipulators(osim *o, const slosh_manipulator *m, uint32_t n)
+{
+    if (n > SLOSH_MAX_MANIPULATORS) return SLOSH_ECAPACITY;
+    if (n && m) memcpy(o->man, m, n * sizeof(slosh_manipulator));
+    o->nman = n;
+    return SLOSH_OK;
+}
+
+static void cross3(const float a[3], const float b[3], float out[3])
+{
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* SloshCommon.ush:207-230 (ManipulatorSDF, ManipulatorWeight). */
+static float manipulator_sdf(const slosh_manipulator *m, const float p[3])
+{
+    if (m->shape == 3u) return -1.0f;
+    slosh_collider c;
+    memset(&c, 0, sizeof(c));
+    for (int a = 0; a < 3; ++a) {
+        c.center[a] = m->center[a];
+        c.extent[a] = m->extent[a];
+    }
+    for (int k = 0; k < 4; ++k) c.rotation[k] = m->rotation[k];
+    c.type = m->shape;
+    return oracle_collider_sdf(&c, p);
+}
+
+static float manipulator_weight(const slosh_manipulator *m, const float p[3])
+{
+    float d = manipulator_sdf(m, p);
+    if (d <= 0.0f) return 1.0f;
+    return m->falloff > 0.0f ? clampf(1.0f - d / m->falloff, 0.0f, 1.0f) : 0.0f;
+}
+
+/* Slosh.usf:1523-1559: force fields on the new velocity nv of a particle at x. */
+static void force_fields(const osim *o, const float x[3], float nv[3], float dt)
+{
+    for (uint32_t k = 0; k < o->nman; ++k) {
+        const slosh_manipulator *m = &o->man[k];
+        if (m->kind == 0u) continue;
+        float w = manipulator_weight(m, x);
+        if (w <= 0.0f) continue;
+        float r[3] = {x[0] - m->center[0], x[1] - m->center[1], x[2] - m->center[2]};
+        float f = m->strength * w * dt;
+        if (m->kind == 1u) {
+            for (int a = 0; a < 3; ++a) nv[a] = nv[a] + m->axis[a] * f;
+        } else if (m->kind == 2u) {
+            float rl = len3(r);
+            for (int a = 0; a < 3; ++a) nv[a] = nv[a] + (rl > 1e-5f ? r[a] / rl : 0.0f) * f;
+        } else if (m->kind == 3u) {
+            float ra = r[0] * m->axis[0] + r[1] * m->axis[1] + r[2] * m->axis[2];
+            float perp[3];
+            for (int a = 0; a < 3; ++a) perp[a] = r[a] - ra * m->axis[a];
+            float pl = len3(perp);
+            if (pl > 1e-5f) {
+                float pu[3], tg[3];
+                for (int a = 0; a < 3; ++a) pu[a] = perp[a] / pl;
+                cross3(m->axis, pu, tg);
+                for (int a = 0; a < 3; ++a) nv[a] = nv[a] + (tg[a] - pu[a] * m->param0) * f;
+            }
+        } else if (m->kind == 4u) {
+            float e = expf(-m->strength * w * dt);
+            for (int a = 0; a < 3; ++a) nv[a] = nv[a] * e;
+        }
+    }
+}
+
+/* Slosh.usf:1639-1654: whether a drain removes particle i at x (frame seed = frame * 16 + substep). */
+static int drained_at(const osim *o, const float x[3], float dt, uint32_t i, uint32_t frame_seed)
+{
+    for (uint32_t k = 0; k < o->nman; ++k) {
+        const slosh_manipulator *m = &o->man[k];
+        if (m->kind != 0u || manipulator_sdf(m, x) > 0.0f) continue;
+        float chance = m->param0 > 0.0f ? 1.0f - expf(-m->param0 * dt) : 1.0f;
+        if (chance >= 1.0f || oracle_hash_to_unit(oracle_pcg_hash(i * 2654435761u + frame_seed * 97u + k)) < chance)
+            return 1;
+    }
+    return 0;
+}
+
 static void touch_pos(osim *o, const float p[3])
 {
     int c[3];
@@ -436,15 +522,21 @@ static void touch_pos(osim *o, const float p[3])
     set_u8(&o->touch[brick_of_cell(o, c[0], c[1], c[2])]);
 }
 
-/* Slosh.usf:280-358 (EmitParticlesCS), mode 0 FillCells: particle I goes to cell I/8, sub-cell I%8 of a 2x2x2
- * stratification, jittered by 0.8 of the sub-cell; clamped to [1.01h, (N-1.01)h]; C = 0; age 0. */
+/* Slosh.usf:280-358 (EmitParticlesCS). Mode 0 FillCells: particle I goes to cell I/8, sub-cell I%8 of a 2x2x2
+ * stratification, jittered by 0.8 of the sub-cell; mode 1: a uniform point in the sphere of radius extent[0]
+ * (cube-root radius, uniform direction); mode 2: a uniform point in the rotated box. Then the optional cone spread
+ * and speed jitter of the velocity; clamped to [1.01h, (N-1.01)h]; C = 0; age 0. */
 int osim_emit(osim *o, const slosh_emit_request *r)
 {
-    if (!r) return SLOSH_EINVAL;
-    if (r->shape != SLOSH_EMIT_FILL_CELLS) return SLOSH_EINVAL;
-    for (int a = 0; a < 3; ++a)
-        if (r->cell_dims[a] < 0) return SLOSH_EINVAL;
-    uint64_t count = (uint64_t)r->cell_dims[0] * (uint64_t)r->cell_dims[1] * (uint64_t)r->cell_dims[2] * 8u;
+    if (!r || r->shape > 2u) return SLOSH_EINVAL;
+    uint64_t count = 0;
+    if (r->shape == SLOSH_EMIT_FILL_CELLS) {
+        for (int a = 0; a < 3; ++a)
+            if (r->cell_dims[a] < 0) return SLOSH_EINVAL;
+        count = (uint64_t)r->cell_dims[0] * (uint64_t)r->cell_dims[1] * (uint64_t)r->cell_dims[2] * 8u;
+    } else {
+        count = r->count;
+    }
     if (count == 0) return SLOSH_OK;
     if ((uint64_t)o->np + count > (uint64_t)o->cap) return SLOSH_ECAPACITY;
     float lo = o->h * 1.01f;
@@ -457,16 +549,52 @@ int osim_emit(osim *o, const slosh_emit_request *r)
         uint32_t h1 = oracle_pcg_hash(h0);
         uint32_t h2 = oracle_pcg_hash(h1);
         float rnd[3] = {oracle_hash_to_unit(h0), oracle_hash_to_unit(h1), oracle_hash_to_unit(h2)};
-        uint32_t cl = i / 8u, sub = i % 8u;
-        int local[3] = {(int)(cl % dx), (int)((cl / dx) % dy), (int)(cl / (dx * dy))};
-        float so[3] = {(float)(sub & 1u), (float)((sub >> 1u) & 1u), (float)((sub >> 2u) & 1u)};
+        float pos[3];
+        if (r->shape == SLOSH_EMIT_FILL_CELLS) {
+            uint32_t cl = i / 8u, sub = i % 8u;
+            int local[3] = {(int)(cl % dx), (int)((cl / dx) % dy), (int)(cl / (dx * dy))};
+            float so[3] = {(float)(sub & 1u), (float)((sub >> 1u) & 1u), (float)((sub >> 2u) & 1u)};
+            for (int a = 0; a < 3; ++a) {
+                float in_cell = ((so[a] + 0.5f) + 0.8f * (rnd[a] - 0.5f)) * 0.5f;
+                pos[a] = ((float)(r->cell_min[a] + local[a]) + in_cell) * o->h;
+            }
+        } else if (r->shape == SLOSH_EMIT_SPHERE) {
+            uint32_t h3 = oracle_pcg_hash(h2);
+            float cos_t = 2.0f * rnd[0] - 1.0f;
+            float sin_t = sqrtf(clampf(1.0f - cos_t * cos_t, 0.0f, 1.0f));
+            float phi = 6.28318530718f * rnd[1];
+            float rad = r->extent[0] * powf(fmaxf(oracle_hash_to_unit(h3), 1e-6f), 1.0f / 3.0f);
+            float dir[3] = {sin_t * cosf(phi), sin_t * sinf(phi), cos_t};
+            for (int a = 0; a < 3; ++a) pos[a] = r->center[a] + rad * dir[a];
+        } else {
+            float l[3], w[3];
+            for (int a = 0; a < 3; ++a) l[a] = (2.0f * rnd[a] - 1.0f) * r->extent[a];
+            quat_rotate(r->rotation, l, w);
+            for (int a = 0; a < 3; ++a) pos[a] = r->center[a] + w[a];
+        }
+        float vel[3] = {r->velocity[0], r->velocity[1], r->velocity[2]};
+        float speed = len3(vel);
+        if (speed > 1e-6f && (r->velocity_spread > 0.0f || r->speed_jitter > 0.0f)) {
+            uint32_t h4 = oracle_pcg_hash(h2 ^ 0x9E3779B9u), h5 = oracle_pcg_hash(h4), h6 = oracle_pcg_hash(h5);
+            float dir[3] = {vel[0] / speed, vel[1] / speed, vel[2] / speed};
+            float up[3] = {fabsf(dir[2]) < 0.9f ? 0.0f : 1.0f, 0.0f, fabsf(dir[2]) < 0.9f ? 1.0f : 0.0f};
+            float t0[3], t1[3];
+            cross3(dir, up, t0);
+            float tl = len3(t0);
+            for (int a = 0; a < 3; ++a) t0[a] = t0[a] / tl;
+            cross3(dir, t0, t1);
+            float cos_a = 1.0f + oracle_hash_to_unit(h4) * (cosf(r->velocity_spread) - 1.0f);
+            float sin_a = sqrtf(clampf(1.0f - cos_a * cos_a, 0.0f, 1.0f));
+            float phi = 6.28318530718f * oracle_hash_to_unit(h5);
+            float cp = cosf(phi), sp = sinf(phi);
+            float scale = fmaxf(1.0f + r->speed_jitter * (2.0f * oracle_hash_to_unit(h6) - 1.0f), 0.0f);
+            for (int a = 0; a < 3; ++a) vel[a] = (cos_a * dir[a] + sin_a * (cp * t0[a] + sp * t1[a])) * speed * scale;
+        }
         slosh_particle *pp = &o->p[o->np + i];
         memset(pp, 0, sizeof(*pp));
         for (int a = 0; a < 3; ++a) {
-            float in_cell = ((so[a] + 0.5f) + 0.8f * (rnd[a] - 0.5f)) * 0.5f;
-            float pos = ((float)(r->cell_min[a] + local[a]) + in_cell) * o->h;
-            pp->pos[a] = clampf(pos, lo, hi[a]);
-            pp->vel[a] = r->velocity[a];
+            pp->pos[a] = clampf(pos[a], lo, hi[a]);
+            pp->vel[a] = vel[a];
         }
         pp->age = 0.0f;
         touch_pos(o, pp->pos);
@@ -749,6 +877,45 @@ static void stage_normalize(osim *o, float dt, const float g[3])
             o->fflags[f] = flags;
         }
     }
+}
+
+/* Slosh.usf:657-686 (ViscosityJacobiFace) x viscosity_iterations, driver SloshProxy.cpp:603-617: Jacobi iterations of
+ * (I - nu dt Laplacian) u = u* on valid, non-solid faces; u* in vel[0], iterates alternating in vel[1] and vel[2]. */
+static void stage_viscosity(osim *o, float alpha, int iterations)
+{
+    memcpy(o->vel[1], o->vel[0], 3 * o->pool_cells * sizeof(float));
+    int src = 1;
+    for (int it = 0; it < iterations; ++it) {
+        int dst = src == 1 ? 2 : 1;
+        const float *us = o->vel[0], *vs = o->vel[src];
+        float *vd = o->vel[dst];
+#pragma omp parallel for schedule(dynamic, 4)
+        for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
+        {
+            for (int a = 0; a < 3; ++a) {
+                size_t f = (size_t)a * o->pool_cells + idx;
+                uint32_t flags = o->fflags[f];
+                float center = vs[f];
+                if ((flags & FACE_VALID) == 0u || (flags & FACE_SOLID) != 0u) {
+                    vd[f] = center;
+                    continue;
+                }
+                float sum = 0.0f;
+                for (int k = 0; k < 6; ++k) {
+                    int nx = x + NB[k][0], ny = y + NB[k][1], nz = z + NB[k][2];
+                    float value = center;
+                    if (in_grid(o, nx, ny, nz)) {
+                        uint32_t ni = face_index(o, a, nx, ny, nz);
+                        if ((o->fflags[ni] & FACE_VALID) != 0u) value = vs[ni];
+                    }
+                    sum += value;
+                }
+                vd[f] = (us[f] + alpha * sum) / (1.0f + 6.0f * alpha);
+            }
+        }
+        src = dst;
+    }
+    o->cur = src;
 }
 
 /* Slosh.usf:211-224 */
@@ -1550,8 +1717,9 @@ static void stage_g2p(osim *o, float dt, const float grav[3])
     float friction = o->s.boundary_friction;
     float radius = o->s.particle_radius_cells * o->h;
     float maxspeed = 0.0f;
-    uint32_t clamped = 0;
-#pragma omp parallel for schedule(static, 4096) reduction(max : maxspeed) reduction(+ : clamped)
+    uint32_t clamped = 0, drained = 0;
+    const uint32_t frame_seed = (uint32_t)(o->frame * 16u);
+#pragma omp parallel for schedule(static, 4096) reduction(max : maxspeed) reduction(+ : clamped, drained)
     for (uint32_t i = 0; i < o->np; ++i) {
         slosh_particle *pp = &o->p[i];
         if (!(pp->age >= 0.0f)) continue;
@@ -1593,6 +1761,7 @@ static void stage_g2p(osim *o, float dt, const float grav[3])
             }
             memset(cm, 0, sizeof(cm));
         }
+        if (o->nman) force_fields(o, x, nv, dt);
         oracle_collider_response(o->col, o->ncol, radius, friction, o->h, x, nv);
         uint32_t contact = 0;
         for (int a = 0; a < 3; ++a) {
@@ -1611,6 +1780,13 @@ static void stage_g2p(osim *o, float dt, const float grav[3])
                 if ((contact & (1u << a)) == 0u) nv[a] *= 1.0f - friction;
         float speed = len3(nv);
         if (speed > o->s.max_speed) clamped++;
+        if (o->nman && drained_at(o, x, dt, i, frame_seed)) {
+            /* Slosh.usf:1649-1652: the record becomes a dead slot (age -1), dropped by the next sort */
+            memset(pp, 0, sizeof(*pp));
+            pp->age = -1.0f;
+            drained++;
+            continue;
+        }
         for (int a = 0; a < 3; ++a) {
             pp->pos[a] = x[a];
             pp->vel[a] = nv[a];
@@ -1622,6 +1798,8 @@ static void stage_g2p(osim *o, float dt, const float grav[3])
     }
     o->res.max_particle_speed = maxspeed;
     o->res.clamped_particles = clamped;
+    o->drained_total += drained;
+    o->res.drained_particles = o->drained_total;
 }
 
 /* ------------------------------------------------------------------------------------------------------------ */
@@ -1642,6 +1820,10 @@ int osim_step_until(osim *o, float dt, const float g[3], int stop_after)
     stage_p2g(o, warm, dt, dt_prev);
     if (stop_after <= SLOSH_STAGE_P2G) return SLOSH_OK;
     stage_normalize(o, dt, g);
+    {
+        const float alpha = o->s.kinematic_viscosity * dt / (o->h * o->h);
+        if (alpha > 0.0f && o->s.viscosity_iterations > 0) stage_viscosity(o, alpha, o->s.viscosity_iterations);
+    }
     if (stop_after <= SLOSH_STAGE_NORMALIZE) return SLOSH_OK;
     stage_divergence(o, dt);
     if (stop_after <= SLOSH_STAGE_DIVERGENCE) return SLOSH_OK;

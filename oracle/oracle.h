@@ -24,6 +24,7 @@ osim *osim_create(const slosh_settings *s);
 void osim_destroy(osim *o);
 int osim_emit(osim *o, const slosh_emit_request *r);
 int osim_set_colliders(osim *o, const slosh_collider *c, uint32_t n);
+int osim_set_manipulators(osim *o, const slosh_manipulator *m, uint32_t n);
 int osim_write_particles(osim *o, const slosh_particle *p, uint32_t n);
 int osim_write_pressure(osim *o, const float *dense);
 /* Writes the density-projection potential q (SLOSH_FLAG_DENSITY_PROJECTION), kept by a step with SOLVER_NONE. */

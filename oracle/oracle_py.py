@@ -38,6 +38,7 @@ def load():
     lib.osim_destroy.restype = None
     lib.osim_emit.argtypes = [vp, C.POINTER(abi.EmitRequest)]
     lib.osim_set_colliders.argtypes = [vp, C.POINTER(abi.Collider), C.c_uint32]
+    lib.osim_set_manipulators.argtypes = [vp, C.POINTER(abi.Manipulator), C.c_uint32]
     lib.osim_write_particles.argtypes = [vp, vp, C.c_uint32]
     lib.osim_write_pressure.argtypes = [vp, vp]
     lib.osim_write_density_q.argtypes = [vp, vp]
@@ -109,6 +110,10 @@ class OracleSim:
     def set_colliders(self, colliders):
         arr = (abi.Collider * max(len(colliders), 1))(*colliders)
         _check(self.lib.osim_set_colliders(self.h, arr, len(colliders)), "set_colliders")
+
+    def set_manipulators(self, manips):
+        arr = (abi.Manipulator * max(len(manips), 1))(*manips)
+        _check(self.lib.osim_set_manipulators(self.h, arr, len(manips)), "set_manipulators")
 
     def write_particles(self, parts):
         parts = np.ascontiguousarray(parts, dtype=abi.PARTICLE_DTYPE)

@@ -49,6 +49,8 @@ def load_library(path=LIB_PATH):
     lib.slosh_emit.restype = st
     lib.slosh_set_colliders.argtypes = [vp, C.POINTER(abi.Collider), C.c_uint32]
     lib.slosh_set_colliders.restype = st
+    lib.slosh_set_manipulators.argtypes = [vp, C.POINTER(abi.Manipulator), C.c_uint32]
+    lib.slosh_set_manipulators.restype = st
     lib.slosh_step.argtypes = [vp, C.c_float, C.POINTER(C.c_float)]
     lib.slosh_step.restype = st
     lib.slosh_step_until.argtypes = [vp, C.c_float, C.POINTER(C.c_float), C.c_int32]
@@ -143,6 +145,10 @@ class Sim:
     def set_colliders(self, colliders):
         arr = (abi.Collider * max(len(colliders), 1))(*colliders)
         _check(self.lib.slosh_set_colliders(self.h, arr, len(colliders)), "slosh_set_colliders")
+
+    def set_manipulators(self, manips):
+        arr = (abi.Manipulator * max(len(manips), 1))(*manips)
+        _check(self.lib.slosh_set_manipulators(self.h, arr, len(manips)), "slosh_set_manipulators")
 
     def write_particles(self, parts):
         parts = np.ascontiguousarray(parts, dtype=abi.PARTICLE_DTYPE)

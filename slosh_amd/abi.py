@@ -105,7 +105,9 @@ class Settings(C.Structure):
         ("density_relaxation", C.c_float),
         ("density_projection_iterations", C.c_int32),
         ("density_projection_interval", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("kinematic_viscosity", C.c_float),
+        ("viscosity_iterations", C.c_int32),
+        ("reserved", C.c_int32 * 8),
     ]
 
 
@@ -160,11 +162,67 @@ class Results(C.Structure):
         ("fallback_used", C.c_int32),
         ("true_residual_rel", C.c_float),
         ("fp64_error_rel", C.c_double),
-        ("reserved", C.c_int32 * 2),
+        ("drained_particles", C.c_uint64),
     ]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class Manipulator(C.Structure):
+    _fields_ = [
+        ("center", C.c_float * 3),
+        ("kind", C.c_uint32),
+        ("rotation", C.c_float * 4),
+        ("extent", C.c_float * 3),
+        ("shape", C.c_uint32),
+        ("axis", C.c_float * 3),
+        ("strength", C.c_float),
+        ("falloff", C.c_float),
+        ("param0", C.c_float),
+        ("param1", C.c_float),
+        ("pad", C.c_uint32),
+    ]
+
+
+MANIP_DRAIN, MANIP_DIRECTIONAL, MANIP_RADIAL, MANIP_SWIRL, MANIP_DRAG = 0, 1, 2, 3, 4
+SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_EVERYWHERE = 0, 1, 2, 3
+MAX_MANIPULATORS = 32
+
+
+def manipulator(kind, shape, center=(0.0, 0.0, 0.0), extent=(0.0, 0.0, 0.0), strength=0.0, axis=(0.0, 0.0, 1.0),
+                falloff=0.0, param0=0.0, rotation=(0.0, 0.0, 0.0, 1.0)):
+    m = Manipulator()
+    m.kind = int(kind)
+    m.shape = int(shape)
+    for a in range(3):
+        m.center[a] = float(center[a])
+        m.extent[a] = float(extent[a])
+        m.axis[a] = float(axis[a])
+    for k in range(4):
+        m.rotation[k] = float(rotation[k])
+    m.strength = float(strength)
+    m.falloff = float(falloff)
+    m.param0 = float(param0)
+    return m
+
+
+def shape_request(shape, center, extent, count, seed, velocity=(0.0, 0.0, 0.0), rotation=(0.0, 0.0, 0.0, 1.0),
+                  spread=0.0, jitter=0.0):
+    """Sphere (shape 1, radius extent[0]) or box (shape 2, half extents) emission of `count` particles."""
+    r = EmitRequest()
+    r.shape = int(shape)
+    r.seed = int(seed) & 0xFFFFFFFF
+    for a in range(3):
+        r.center[a] = float(center[a])
+        r.extent[a] = float(extent[a])
+        r.velocity[a] = float(velocity[a])
+    for k in range(4):
+        r.rotation[k] = float(rotation[k])
+    r.velocity_spread = float(spread)
+    r.speed_jitter = float(jitter)
+    r.count = int(count)
+    return r
 
 
 class KernelTime(C.Structure):
@@ -189,7 +247,7 @@ PARTICLE_DTYPE = np.dtype([("pos", np.float32, 3), ("age", np.float32), ("vel", 
 PARTICLE_REF48_DTYPE = np.dtype([("pos", np.float32, 3), ("age", np.float32), ("vel", np.float32, 3), ("c", np.uint32, 5)])
 
 assert C.sizeof(Collider) == 80
-assert C.sizeof(Settings) == 96
+assert C.sizeof(Settings) == 128
 assert C.sizeof(EmitRequest) == 100
 assert C.sizeof(Results) == 96
 assert PARTICLE_DTYPE.itemsize == 64 and PARTICLE_REF48_DTYPE.itemsize == 48
@@ -198,7 +256,7 @@ assert PARTICLE_DTYPE.itemsize == 64 and PARTICLE_REF48_DTYPE.itemsize == 48
 def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6, max_iterations=200, sor_weight=1.8,
                   budget=1.0, extrapolation_layers=2, ghost_fluid=1, friction=0.0, max_speed=20.0, flags=0,
                   mg_coarse_sweeps=0, density_correction=0.0, density_relaxation=0.5, density_iterations=30,
-                  density_interval=1):
+                  density_interval=1, viscosity=0.0, viscosity_iterations=12):
     """Parity-profile settings (APIC, ghost fluid, no density correction) with the given grid and solver."""
     s = Settings()
     for a in range(3):
@@ -222,6 +280,8 @@ def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6
     s.density_relaxation = float(density_relaxation)
     s.density_projection_iterations = int(density_iterations)
     s.density_projection_interval = int(density_interval)
+    s.kinematic_viscosity = float(viscosity)
+    s.viscosity_iterations = int(viscosity_iterations)
     return s
 
 

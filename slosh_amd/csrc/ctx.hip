@@ -25,13 +25,14 @@ const char *const kKernelNames[K_COUNT] = {
     "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d23down", "k_d23up",
     "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
     "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io",
-    "k_unet_train", "k_fcg", "k_g2p_move", "k_density"};
+    "k_unet_train", "k_fcg", "k_g2p_move", "k_density", "k_viscosity"};
 } // namespace slosh
 
 static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");
 static_assert(sizeof(slosh_particle) == 64, "64-byte FP32 particle record");
 static_assert(sizeof(slosh_particle_ref48) == 48, "FSloshParticleGPU layout (SloshTypes.h:84-91)");
-static_assert(sizeof(slosh_settings) == 96, "settings ABI size");
+static_assert(sizeof(slosh_settings) == 128, "settings ABI size");
+static_assert(sizeof(slosh_manipulator) == 80, "manipulator layout (SloshTypes.h:66-80)");
 static_assert(sizeof(slosh_emit_request) == 100, "emit request ABI size");
 static_assert(sizeof(slosh_results) == 96, "results ABI size");
 
@@ -80,7 +81,7 @@ void free_all(slosh_ctx *c)
     dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->coarse_sig); dfree(c->pres_prev); dfree(c->pres_lp); dfree(c->state_save); dfree(c->partials);
     dfree(c->dens_rhs); dfree(c->dens_q); dfree(c->dens_disp);
     dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
-    dfree(c->bp_clamp);
+    dfree(c->bp_clamp); dfree(c->bp_drain); dfree(c->d_man);
     dfree(c->state); dfree(c->stats); dfree(c->d_col);
     dfree(c->staging);
     for (uint32_t k = 0; k < c->tev_cap; ++k) (void)hipEventDestroy(c->tev[k]);
@@ -131,7 +132,8 @@ bool settings_valid(const slosh_settings *s)
     if (s->density_projection_iterations < 0 || s->density_projection_iterations > 10000) return false;
     if (s->density_projection_interval < 0 || s->density_projection_interval > 8) return false;
     if (!(s->density_correction >= 0.0f) || !(s->density_deadband >= 0.0f)) return false;
-    for (int k = 0; k < 2; ++k)
+    if (!(s->kinematic_viscosity >= 0.0f) || s->viscosity_iterations < 0 || s->viscosity_iterations > 1000) return false;
+    for (int k = 0; k < 8; ++k)
         if (s->reserved[k] != 0) return false;
     return true;
 }
@@ -316,9 +318,11 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->bp_div_sum, g.npool);
     dalloc(&c->bp_speed, g.nbricks);
     dalloc(&c->bp_clamp, g.nbricks);
+    dalloc(&c->bp_drain, g.nbricks);
     dalloc(&c->state, 1);
     dalloc(&c->stats, 1);
     dalloc(&c->d_col, SLOSH_MAX_COLLIDERS);
+    dalloc(&c->d_man, SLOSH_MAX_MANIPULATORS);
     if (hipHostMalloc((void **)&c->h_stats, sizeof(StepStats), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
     if (hipHostMalloc((void **)&c->h_nsolve, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
     if (hipHostMalloc((void **)&c->h_state, sizeof(SolverState), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
@@ -341,6 +345,7 @@ void reset_impl(slosh_ctx *c)
     c->nactive = 0;
     c->frame = 0;
     c->prev_dt = 0.0f;
+    c->drained_total = 0;
     memset(&c->res, 0, sizeof(c->res));
     launch_fill_u32(c->stream, c->table, 0u, g.nbricks);
     launch_fill_u32(c->stream, c->slot_brick, kNoBrick, g.npool);
@@ -422,7 +427,12 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
         }
         c->n_p2g_blocks = stop > SLOSH_STAGE_POOL ? c->nactive : 0u;
     }
-    if (stop > SLOSH_STAGE_P2G) launch_normalize(c, dt, gv);
+    if (stop > SLOSH_STAGE_P2G) {
+        launch_normalize(c, dt, gv);
+        // SloshProxy.cpp:603-617: implicit viscosity, alpha = nu dt / h^2
+        const float alpha = c->s.kinematic_viscosity * dt / (c->geo.h * c->geo.h);
+        if (alpha > 0.0f && c->s.viscosity_iterations > 0) run_viscosity(c, alpha, c->s.viscosity_iterations);
+    }
     if (stop > SLOSH_STAGE_NORMALIZE) launch_divergence(c, dt);
     bool solved = false;
     if (stop > SLOSH_STAGE_DIVERGENCE) {
@@ -485,6 +495,13 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
         c->res.solve_ms = ms;
     }
     const StepStats &st = *c->h_stats;
+    if (stop > SLOSH_STAGE_EXTRAPOLATE && st.drained) {
+        // drained records took the dead key: the sorted array's live records are the first np - drained
+        c->np -= st.drained;
+        c->drained_total += st.drained;
+    }
+    c->res.drained_particles = c->drained_total;
+    c->res.live_particles = c->np;
     c->res.fluid_cells = st.fluid_cells;
     float f;
     memcpy(&f, &st.max_speed_bits, 4);
@@ -738,6 +755,8 @@ void slosh_default_settings(slosh_settings *s)
     s->density_relaxation = 0.5f;
     s->density_projection_iterations = 30;
     s->density_projection_interval = 1;
+    s->kinematic_viscosity = 0.0f;
+    s->viscosity_iterations = 12;
 }
 
 slosh_status slosh_create(const slosh_settings *s, slosh_ctx **out)
@@ -798,16 +817,21 @@ slosh_status slosh_reset(slosh_ctx *c)
 slosh_status slosh_emit(slosh_ctx *c, const slosh_emit_request *r)
 {
     if (!c || !r) return SLOSH_EINVAL;
-    if (r->shape != SLOSH_EMIT_FILL_CELLS) return SLOSH_EINVAL;
-    // cone spread and speed jitter are not applied by this emission path: a request asking for them is refused
-    // rather than silently emitted without them
-    if (r->velocity_spread != 0.0f || r->speed_jitter != 0.0f || r->reserved != 0u) return SLOSH_EINVAL;
-    for (int a = 0; a < 3; ++a)
-        if (r->cell_dims[a] < 0) return SLOSH_EINVAL;
-    uint64_t count = (uint64_t)r->cell_dims[0] * (uint64_t)r->cell_dims[1] * (uint64_t)r->cell_dims[2] * 8u;
+    if (r->shape > SLOSH_EMIT_BOX || r->reserved != 0u) return SLOSH_EINVAL;
+    if (!(r->velocity_spread >= 0.0f) || !(r->speed_jitter >= 0.0f)) return SLOSH_EINVAL;
+    uint64_t count = 0;
+    if (r->shape == SLOSH_EMIT_FILL_CELLS) {
+        for (int a = 0; a < 3; ++a)
+            if (r->cell_dims[a] < 0) return SLOSH_EINVAL;
+        count = (uint64_t)r->cell_dims[0] * (uint64_t)r->cell_dims[1] * (uint64_t)r->cell_dims[2] * 8u;
+    } else {
+        for (int a = 0; a < 3; ++a)
+            if (!(r->extent[a] >= 0.0f) || !std::isfinite(r->center[a])) return SLOSH_EINVAL;
+        count = r->count;
+    }
     if (count == 0) return SLOSH_OK;
     if ((uint64_t)c->np + count > (uint64_t)c->cap) return SLOSH_ECAPACITY;
-    ABI_GUARD(launch_emit_fill(c, r, (uint32_t)count); c->np += (uint32_t)count; c->sorted = false;
+    ABI_GUARD(launch_emit(c, r, (uint32_t)count); c->np += (uint32_t)count; c->sorted = false;
               SLOSH_HIP(hipStreamSynchronize(c->stream)); return SLOSH_OK;)
 }
 
@@ -820,6 +844,20 @@ slosh_status slosh_set_colliders(slosh_ctx *c, const slosh_collider *col, uint32
         SLOSH_HIP(hipMemcpyAsync(c->d_col, c->h_col, n * sizeof(slosh_collider), hipMemcpyHostToDevice, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
     } c->ncol = n;
+              return SLOSH_OK;)
+}
+
+slosh_status slosh_set_manipulators(slosh_ctx *c, const slosh_manipulator *m, uint32_t n)
+{
+    if (!c || (n && !m)) return SLOSH_EINVAL;
+    if (n > SLOSH_MAX_MANIPULATORS) return SLOSH_ECAPACITY;
+    for (uint32_t k = 0; k < n; ++k)
+        if (m[k].kind > 4u || m[k].shape > 3u || !(m[k].falloff >= 0.0f) || !(m[k].param0 >= 0.0f || m[k].kind == 3u))
+            return SLOSH_EINVAL;
+    ABI_GUARD(if (n) {
+        SLOSH_HIP(hipMemcpyAsync(c->d_man, m, n * sizeof(slosh_manipulator), hipMemcpyHostToDevice, c->stream));
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+    } c->nman = n;
               return SLOSH_OK;)
 }
 

@@ -474,6 +474,39 @@ __global__ __launch_bounds__(512) void k_project(Geo g, const uint32_t *rows, fl
     }
 }
 
+// Slosh.usf:657-686 (ViscosityJacobiFace): one Jacobi iteration of (I - nu dt Laplacian) u = u* on the valid, non-solid
+// faces (ustar = u*, src the current iterate); a neighbour face outside the grid or not valid reads as the centre.
+__global__ __launch_bounds__(512) void k_viscosity(Geo g, const uint32_t *rows, const float *ustar, const float *src,
+                                                   float *dst, const uint8_t *fflags, float alpha)
+{
+    __shared__ uint32_t s_row[28];
+    CellPos c;
+    uint32_t e;
+    const bool in = cell_of_block(g, rows, s_row, c, e);
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t f = (uint32_t)a * g.pool_cells + c.idx;
+        const float center = src[f];
+        const uint32_t flags = fflags[f];
+        if (!in || !(flags & kFaceValid) || (flags & kFaceSolid)) {
+            dst[f] = center;
+            continue;
+        }
+        float sum = 0.0f;
+        for (int nb = 0; nb < 6; ++nb) {
+            int d[3];
+            nb_offset(nb, d);
+            const int nx = c.x + d[0], ny = c.y + d[1], nz = c.z + d[2];
+            float value = center;
+            if (in_grid(g, nx, ny, nz)) {
+                const uint32_t ni = face_index(g, c, a, nx, ny, nz);
+                if (fflags[ni] & kFaceValid) value = src[ni];
+            }
+            sum += value;
+        }
+        dst[f] = (ustar[f] + alpha * sum) / (1.0f + 6.0f * alpha);
+    }
+}
+
 // ---- two-sided density projection (Slosh.usf:743-854) --------------------------------------------------------
 // Slosh.usf:753-773 (CellDensityInterior): the P2G weight of the cell's faces that no solid shares, per rest
 // particle count of those faces; 1 when a solid shares every face.
@@ -613,13 +646,13 @@ __global__ __launch_bounds__(kStatsThreads) void k_step_stats(const uint32_t *bp
                                                               const uint32_t *bp_div_max, const uint32_t *bp_div_cnt,
                                                               const double *bp_div_sum, uint32_t n_div,
                                                               const uint32_t *bp_speed, const uint32_t *bp_clamp,
-                                                              uint32_t n_g2p, StepStats *stats)
+                                                              const uint32_t *bp_drain, uint32_t n_g2p, StepStats *stats)
 {
     constexpr uint32_t T = kStatsThreads, U = 8;
-    __shared__ uint32_t su[5][T];
+    __shared__ uint32_t su[6][T];
     __shared__ double sd[T];
     const uint32_t t = threadIdx.x;
-    uint32_t fluid = 0, dmax = 0, dcnt = 0, smax = 0, clamp = 0;
+    uint32_t fluid = 0, dmax = 0, dcnt = 0, smax = 0, clamp = 0, drain = 0;
     double dsum = 0.0;
     for (uint32_t b0 = t; b0 < n_p2g; b0 += T * U) {
         uint32_t v[U];
@@ -643,16 +676,18 @@ __global__ __launch_bounds__(kStatsThreads) void k_step_stats(const uint32_t *bp
         }
     }
     for (uint32_t b0 = t; b0 < n_g2p; b0 += T * U) {
-        uint32_t sp[U], cl[U];
+        uint32_t sp[U], cl[U], dr[U];
         for (uint32_t u = 0; u < U; ++u) {
             const uint32_t b = b0 + T * u;
             const bool in = b < n_g2p;
             sp[u] = in ? bp_speed[b] : 0u;
             cl[u] = in ? bp_clamp[b] : 0u;
+            dr[u] = in ? bp_drain[b] : 0u;
         }
         for (uint32_t u = 0; u < U; ++u) {
             smax = max(smax, sp[u]);
             clamp += cl[u];
+            drain += dr[u];
         }
     }
     su[0][t] = fluid;
@@ -660,6 +695,7 @@ __global__ __launch_bounds__(kStatsThreads) void k_step_stats(const uint32_t *bp
     su[2][t] = dcnt;
     su[3][t] = smax;
     su[4][t] = clamp;
+    su[5][t] = drain;
     sd[t] = dsum;
     __syncthreads();
     for (uint32_t s = T / 2; s > 0; s >>= 1) {
@@ -669,6 +705,7 @@ __global__ __launch_bounds__(kStatsThreads) void k_step_stats(const uint32_t *bp
             su[2][t] += su[2][t + s];
             su[3][t] = max(su[3][t], su[3][t + s]);
             su[4][t] += su[4][t + s];
+            su[5][t] += su[5][t + s];
             sd[t] += sd[t + s];
         }
         __syncthreads();
@@ -679,6 +716,7 @@ __global__ __launch_bounds__(kStatsThreads) void k_step_stats(const uint32_t *bp
         stats->div_count = su[2][0];
         stats->max_speed_bits = su[3][0];
         stats->clamped = su[4][0];
+        stats->drained = su[5][0];
         stats->div_sum = sd[0];
     }
 }
@@ -852,6 +890,23 @@ void launch_project(slosh_ctx *c)
     SLOSH_HIP(hipGetLastError());
 }
 
+void run_viscosity(slosh_ctx *c, float alpha, int iterations)
+{
+    if (!c->nactive || iterations <= 0) return;
+    // vel[0] keeps u*; the iterates alternate between vel[1] and vel[2]
+    SLOSH_HIP(hipMemcpyAsync(c->vel[1], c->vel[0], (size_t)3 * c->geo.pool_cells * sizeof(float), hipMemcpyDeviceToDevice,
+                             c->stream));
+    int src = 1;
+    for (int it = 0; it < iterations; ++it) {
+        const int dst = src == 1 ? 2 : 1;
+        launch_timed(c, K_VISCOSITY, k_viscosity, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo, c->vel[0],
+                     c->vel[src], c->vel[dst], c->fflags, alpha);
+        src = dst;
+    }
+    SLOSH_HIP(hipGetLastError());
+    c->vcur = src;
+}
+
 void launch_density_rhs(slosh_ctx *c, float relax, int keep_q)
 {
     if (!c->nactive) return;
@@ -872,7 +927,7 @@ void launch_step_stats(slosh_ctx *c)
 {
     SLOSH_TIMED(c, K_STEP_STATS);
     hipLaunchKernelGGL(k_step_stats, dim3(1), dim3(kStatsThreads), 0, c->stream, c->bp_fluid, c->n_p2g_blocks, c->bp_div_max,
-                       c->bp_div_cnt, c->bp_div_sum, c->n_div_blocks, c->bp_speed, c->bp_clamp, c->n_g2p_blocks,
+                       c->bp_div_cnt, c->bp_div_sum, c->n_div_blocks, c->bp_speed, c->bp_clamp, c->bp_drain, c->n_g2p_blocks,
                        c->stats);
     SLOSH_HIP(hipGetLastError());
 }

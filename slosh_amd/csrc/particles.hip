@@ -26,8 +26,10 @@ __device__ __forceinline__ void touch_brick(const Geo &g, uint8_t *touch, float 
     touch[brick_of_cell(g, c[0], c[1], c[2])] = 1u;
 }
 
-// Slosh.usf:280-358 (EmitParticlesCS), mode 0 FillCells.
-__global__ void k_emit_fill(Geo g, Parts p, uint32_t base, uint32_t count, slosh_emit_request r, uint8_t *touch)
+// Slosh.usf:280-358 (EmitParticlesCS): mode 0 FillCells (particle I in cell I/8, sub-cell I%8, jittered), 1 a
+// uniform point in the sphere of radius extent[0] (cube-root radius, uniform direction), 2 a uniform point in the
+// rotated box; optional cone spread and speed jitter of the velocity; position clamped into [1.01h, (N-1.01)h].
+__global__ void k_emit(Geo g, Parts p, uint32_t base, uint32_t count, slosh_emit_request r, uint8_t *touch)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
@@ -35,21 +37,53 @@ __global__ void k_emit_fill(Geo g, Parts p, uint32_t base, uint32_t count, slosh
     uint32_t h1 = pcg_hash(h0);
     uint32_t h2 = pcg_hash(h1);
     float rnd[3] = {hash_to_unit(h0), hash_to_unit(h1), hash_to_unit(h2)};
-    uint32_t dx = (uint32_t)r.cell_dims[0], dy = (uint32_t)r.cell_dims[1];
-    uint32_t cl = i / 8u, sub = i % 8u;
-    int local[3] = {(int)(cl % dx), (int)((cl / dx) % dy), (int)(cl / (dx * dy))};
-    float so[3] = {(float)(sub & 1u), (float)((sub >> 1u) & 1u), (float)((sub >> 2u) & 1u)};
     float lo = g.h * 1.01f;
     float pos[3];
-    for (int a = 0; a < 3; ++a) {
-        float hi = ((float)g.n[a] - 1.01f) * g.h;
-        float in_cell = ((so[a] + 0.5f) + 0.8f * (rnd[a] - 0.5f)) * 0.5f;
-        float v = ((float)(r.cell_min[a] + local[a]) + in_cell) * g.h;
-        pos[a] = clampf(v, lo, hi);
+    if (r.shape == SLOSH_EMIT_FILL_CELLS) {
+        uint32_t dx = (uint32_t)r.cell_dims[0], dy = (uint32_t)r.cell_dims[1];
+        uint32_t cl = i / 8u, sub = i % 8u;
+        int local[3] = {(int)(cl % dx), (int)((cl / dx) % dy), (int)(cl / (dx * dy))};
+        float so[3] = {(float)(sub & 1u), (float)((sub >> 1u) & 1u), (float)((sub >> 2u) & 1u)};
+        for (int a = 0; a < 3; ++a) {
+            float in_cell = ((so[a] + 0.5f) + 0.8f * (rnd[a] - 0.5f)) * 0.5f;
+            pos[a] = ((float)(r.cell_min[a] + local[a]) + in_cell) * g.h;
+        }
+    } else if (r.shape == SLOSH_EMIT_SPHERE) {
+        uint32_t h3 = pcg_hash(h2);
+        float cos_t = 2.0f * rnd[0] - 1.0f;
+        float sin_t = sqrtf(clampf(1.0f - cos_t * cos_t, 0.0f, 1.0f));
+        float phi = 6.28318530718f * rnd[1];
+        float rad = r.extent[0] * powf(fmaxf(hash_to_unit(h3), 1e-6f), 1.0f / 3.0f);
+        float dir[3] = {sin_t * cosf(phi), sin_t * sinf(phi), cos_t};
+        for (int a = 0; a < 3; ++a) pos[a] = r.center[a] + rad * dir[a];
+    } else {
+        float l[3], w[3];
+        for (int a = 0; a < 3; ++a) l[a] = (2.0f * rnd[a] - 1.0f) * r.extent[a];
+        quat_rotate(r.rotation, l, w);
+        for (int a = 0; a < 3; ++a) pos[a] = r.center[a] + w[a];
+    }
+    for (int a = 0; a < 3; ++a) pos[a] = clampf(pos[a], lo, ((float)g.n[a] - 1.01f) * g.h);
+    float vel[3] = {r.velocity[0], r.velocity[1], r.velocity[2]};
+    const float speed = len3(vel[0], vel[1], vel[2]);
+    if (speed > 1e-6f && (r.velocity_spread > 0.0f || r.speed_jitter > 0.0f)) {
+        uint32_t h4 = pcg_hash(h2 ^ 0x9E3779B9u), h5 = pcg_hash(h4), h6 = pcg_hash(h5);
+        float dir[3] = {vel[0] / speed, vel[1] / speed, vel[2] / speed};
+        const float up[3] = {fabsf(dir[2]) < 0.9f ? 0.0f : 1.0f, 0.0f, fabsf(dir[2]) < 0.9f ? 1.0f : 0.0f};
+        float t0[3], t1[3];
+        cross3(dir, up, t0);
+        const float tl = len3(t0[0], t0[1], t0[2]);
+        for (int a = 0; a < 3; ++a) t0[a] = t0[a] / tl;
+        cross3(dir, t0, t1);
+        const float cos_a = 1.0f + hash_to_unit(h4) * (cosf(r.velocity_spread) - 1.0f);
+        const float sin_a = sqrtf(clampf(1.0f - cos_a * cos_a, 0.0f, 1.0f));
+        const float phi = 6.28318530718f * hash_to_unit(h5);
+        const float cp = cosf(phi), sp = sinf(phi);
+        const float scale = fmaxf(1.0f + r.speed_jitter * (2.0f * hash_to_unit(h6) - 1.0f), 0.0f);
+        for (int a = 0; a < 3; ++a) vel[a] = (cos_a * dir[a] + sin_a * (cp * t0[a] + sp * t1[a])) * speed * scale;
     }
     uint32_t j = base + i;
     p.pa[j] = make_float4(pos[0], pos[1], pos[2], 0.0f);
-    for (int a = 0; a < 3; ++a) p.v[a][j] = r.velocity[a];
+    for (int a = 0; a < 3; ++a) p.v[a][j] = vel[a];
     for (int k = 0; k < 9; ++k) p.c[k][j] = 0.0f;
     touch_brick(g, touch, pos[0], pos[1], pos[2]);
 }
@@ -699,21 +733,25 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
                                          const uint8_t *fflags, float dt, float gx, float gy, float gz, float friction,
                                          float radius, float max_speed, const slosh_collider *col, uint32_t ncol,
                                          uint8_t *touch, uint32_t *bp_speed, uint32_t *bp_clamp, uint32_t *keys,
-                                         uint32_t dead_key, const G2PMove &mv, const float *disp)
+                                         uint32_t dead_key, const G2PMove &mv, const float *disp,
+                                         const slosh_manipulator *man, uint32_t nman, uint32_t frame_seed,
+                                         uint32_t *bp_drain)
 {
     __shared__ float s_box[3 * kG2PBoxNodes];
     __shared__ uint32_t s_nb[27]; // slots of the 3x3x3 brick neighbourhood, (dx+1) + 3(dy+1) + 9(dz+1)
-    __shared__ uint32_t smax, sclamp;
+    __shared__ uint32_t smax, sclamp, sdrain;
     const uint32_t key = blockIdx.x;
     const uint32_t beg = key_off[key], end = key_off[key + 1];
     if (threadIdx.x == 0) {
         smax = 0u;
         sclamp = 0u;
+        sdrain = 0u;
     }
     if (beg == end) {
         if (!kMove && threadIdx.x == 0) {
             bp_speed[key] = 0u;
             bp_clamp[key] = 0u;
+            bp_drain[key] = 0u;
         }
         return;
     }
@@ -762,7 +800,8 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
         }
     }
     __syncthreads();
-    uint32_t vmax = 0u, nclamp = 0u; // per thread; speeds are non-negative, so their bits order like the values
+    uint32_t vmax = 0u, nclamp = 0u, ndrain = 0u; // per thread; speeds are non-negative, so their bits order like
+                                                  // the values
     for (uint32_t base_i = beg; base_i < end; base_i += 256u) {
         const uint32_t i = base_i + threadIdx.x;
         uint32_t brick = kNoBrick;
@@ -822,6 +861,7 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
                     }
                     for (int k = 0; k < 9; ++k) cm[k] = 0.0f;
                 }
+                if (nman) apply_force_fields(man, nman, x, nv, dt);
                 for (uint32_t k = 0; k < ncol; ++k) {
                     const slosh_collider cc = col[k];
                     float dd = collider_sdf(cc, x);
@@ -860,19 +900,29 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
                     for (int a = 0; a < 3; ++a)
                         if ((contact & (1u << a)) == 0u) nv[a] *= 1.0f - friction;
                 float speed = len3(nv[0], nv[1], nv[2]);
-                pos_cell(g, x[0], x[1], x[2], c);
-                brick = brick_of_cell(g, c[0], c[1], c[2]);
-                if (kMove) {
-                    const uint32_t pk = mv.packed[i];
-                    const uint32_t j = mv.run_base[(i & ~(kTile - 1u)) + (pk >> 8)] + (pk & 255u);
-                    mv.dst.pa[j] = make_float4(x[0], x[1], x[2], pa.w + dt);
-                    for (int a = 0; a < 3; ++a) mv.dst.v[a][j] = nv[a];
-                    for (int k = 0; k < 9; ++k) mv.dst.c[k][j] = cm[k];
-                    if (brick != keys[i]) atomicAdd(mv.misplaced, 1u);
+                if (nman && drained_at(man, nman, x, dt, i, frame_seed)) {
+                    // removed: the sort drops the record (its key is the dead key), its brick is not touched (the
+                    // lane keeps brick = kNoBrick and still takes part in the wavefront's touch marking below)
+                    if (!kMove) {
+                        if (speed > max_speed) nclamp++;
+                        ndrain++;
+                        keys[i] = dead_key;
+                    }
                 } else {
-                    if (speed > max_speed) nclamp++;
-                    vmax = max(vmax, __float_as_uint(speed));
-                    keys[i] = brick;
+                    pos_cell(g, x[0], x[1], x[2], c);
+                    brick = brick_of_cell(g, c[0], c[1], c[2]);
+                    if (kMove) {
+                        const uint32_t pk = mv.packed[i];
+                        const uint32_t j = mv.run_base[(i & ~(kTile - 1u)) + (pk >> 8)] + (pk & 255u);
+                        mv.dst.pa[j] = make_float4(x[0], x[1], x[2], pa.w + dt);
+                        for (int a = 0; a < 3; ++a) mv.dst.v[a][j] = nv[a];
+                        for (int k = 0; k < 9; ++k) mv.dst.c[k][j] = cm[k];
+                        if (brick != keys[i]) atomicAdd(mv.misplaced, 1u);
+                    } else {
+                        if (speed > max_speed) nclamp++;
+                        vmax = max(vmax, __float_as_uint(speed));
+                        keys[i] = brick;
+                    }
                 }
             }
         }
@@ -886,15 +936,18 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
     for (int w = 32; w > 0; w >>= 1) {
         vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, w));
         nclamp += (uint32_t)__shfl_xor((int)nclamp, w);
+        ndrain += (uint32_t)__shfl_xor((int)ndrain, w);
     }
     if ((threadIdx.x & 63u) == 0u) {
         atomicMax(&smax, vmax);
         atomicAdd(&sclamp, nclamp);
+        atomicAdd(&sdrain, ndrain);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         bp_speed[key] = smax;
         bp_clamp[key] = sclamp;
+        bp_drain[key] = sdrain;
     }
 }
 
@@ -903,31 +956,33 @@ __global__ __launch_bounds__(256) void k_g2p_keys(Geo g, Parts p, const uint32_t
                                                   float friction, float radius, float max_speed,
                                                   const slosh_collider *col, uint32_t ncol, uint8_t *touch,
                                                   uint32_t *bp_speed, uint32_t *bp_clamp, uint32_t *keys,
-                                                  uint32_t dead_key, const float *disp)
+                                                  uint32_t dead_key, const float *disp, const slosh_manipulator *man,
+                                                  uint32_t nman, uint32_t frame_seed, uint32_t *bp_drain)
 {
     g2p_body<false>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, touch, bp_speed,
-                    bp_clamp, keys, dead_key, G2PMove{}, disp);
+                    bp_clamp, keys, dead_key, G2PMove{}, disp, man, nman, frame_seed, bp_drain);
 }
 
 __global__ __launch_bounds__(256) void k_g2p_move(Geo g, Parts p, const uint32_t *key_off, const float *vel,
                                                   const uint8_t *fflags, float dt, float gx, float gy, float gz,
                                                   float friction, float radius, float max_speed,
                                                   const slosh_collider *col, uint32_t ncol, uint32_t *keys, G2PMove mv,
-                                                  const float *disp)
+                                                  const float *disp, const slosh_manipulator *man, uint32_t nman,
+                                                  uint32_t frame_seed)
 {
     g2p_body<true>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, nullptr, nullptr,
-                   nullptr, keys, 0u, mv, disp);
+                   nullptr, keys, 0u, mv, disp, man, nman, frame_seed, nullptr);
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
 } // namespace
 
-void launch_emit_fill(slosh_ctx *c, const slosh_emit_request *r, uint32_t count)
+void launch_emit(slosh_ctx *c, const slosh_emit_request *r, uint32_t count)
 {
     if (!count) return;
     {
-        launch_timed(c, K_EMIT, k_emit_fill, dim3(blocks_for(count, 256)), dim3(256), 0, c->geo, c->parts[c->cur],
+        launch_timed(c, K_EMIT, k_emit, dim3(blocks_for(count, 256)), dim3(256), 0, c->geo, c->parts[c->cur],
                      c->np, count, *r, c->touch);
     }
     SLOSH_HIP(hipGetLastError());
@@ -1010,7 +1065,8 @@ void launch_g2p(slosh_ctx *c, float dt, const float g[3])
     launch_timed(c, K_G2P, k_g2p_keys, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
                  c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction, radius, c->s.max_speed,
                  c->d_col, c->ncol, c->touch, c->bp_speed, c->bp_clamp, c->g2p_keys, c->nkeys - 1u,
-                 c->dens_applied ? (const float *)c->dens_disp : nullptr);
+                 c->dens_applied ? (const float *)c->dens_disp : nullptr, c->d_man, c->nman,
+                 (uint32_t)(c->frame * 16u), c->bp_drain);
     c->n_g2p_blocks = geo.nbricks;
     SLOSH_HIP(hipGetLastError());
     // the new offsets go to the spare buffer: the second pass walks the current array by its own offsets
@@ -1018,7 +1074,8 @@ void launch_g2p(slosh_ctx *c, float dt, const float g[3])
     G2PMove mv{c->parts[1 - c->cur], c->packed, c->run_base, &c->stats->misplaced};
     launch_timed(c, K_G2P_MOVE, k_g2p_move, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
                  c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction, radius, c->s.max_speed,
-                 c->d_col, c->ncol, c->g2p_keys, mv, c->dens_applied ? (const float *)c->dens_disp : nullptr);
+                 c->d_col, c->ncol, c->g2p_keys, mv, c->dens_applied ? (const float *)c->dens_disp : nullptr,
+                 c->d_man, c->nman, (uint32_t)(c->frame * 16u));
     SLOSH_HIP(hipGetLastError());
     std::swap(c->key_off, c->key_off_next);
     c->cur = 1 - c->cur;

@@ -175,7 +175,7 @@ enum KernelId {
     K_RESTRICT0, K_PROLONG_BLACK, K_DIRECTION, K_UPDATE_BLACK, K_PDOWN1, K_PDOWN2, K_PUP1, K_PUP2, K_D3DOWN, K_D3UP,
     K_DSMOOTH, K_DRESTRICT, K_DPROLONG, K_COARSE_SOLVE, K_COARSE_TAIL, K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL,
     K_FINALIZE, K_STEP_STATS, K_UNET_CONV0, K_UNET_CONV1, K_UNET_CONV2, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_G2P_MOVE,
-    K_DENSITY, K_COUNT
+    K_DENSITY, K_VISCOSITY, K_COUNT
 };
 // Names as the compiled kernels are called (what a kernel trace of the library shows).
 extern const char *const kKernelNames[K_COUNT];
@@ -224,6 +224,7 @@ struct StepStats {
     uint32_t nactive;
     uint32_t nfree;
     uint32_t misplaced; // particles whose two G2P passes disagree on the new brick (never expected; fails the step)
+    uint32_t drained;   // particles removed by drain manipulators this step
     double div_sum;
     uint32_t div_count;
     uint32_t pad2;
@@ -311,6 +312,7 @@ struct slosh_ctx {
     double *bp_div_sum = nullptr;   // [npool]
     uint32_t *bp_speed = nullptr;   // [nbricks] G2P: max speed bits per brick of the sorted particles
     uint32_t *bp_clamp = nullptr;   // [nbricks]
+    uint32_t *bp_drain = nullptr;   // [nbricks] G2P: particles drained per brick
     uint32_t n_p2g_blocks = 0, n_div_blocks = 0, n_g2p_blocks = 0;
     // MGPCG iterations per host check of the solver state, and their replay as a captured HIP graph
     int solver_chunk = 4;
@@ -327,10 +329,13 @@ struct slosh_ctx {
     uint32_t capture_cap = 0, capture_n = 0;
     int capture_iter[32] = {};        // MGPCG iteration of each captured residual
     int fcg_fallback = 0, fcg_iters = 0;
-    // colliders
+    // colliders and manipulators
     slosh_collider *d_col = nullptr;
     slosh_collider h_col[SLOSH_MAX_COLLIDERS];
     uint32_t ncol = 0;
+    slosh_manipulator *d_man = nullptr;
+    uint32_t nman = 0;
+    uint64_t drained_total = 0;
     // host pinned mirrors
     slosh::StepStats *h_stats = nullptr;
     slosh::SolverState *h_state = nullptr;
@@ -356,7 +361,7 @@ struct slosh_ctx {
 namespace slosh {
 
 // ---- particles.hip ----
-void launch_emit_fill(slosh_ctx *c, const slosh_emit_request *r, uint32_t count);
+void launch_emit(slosh_ctx *c, const slosh_emit_request *r, uint32_t count);
 void launch_unpack_particles(slosh_ctx *c, const slosh_particle *d_src, uint32_t n);
 void launch_pack_particles(slosh_ctx *c, void *d_dst, int layout);
 void run_sort(slosh_ctx *c);
@@ -376,6 +381,9 @@ void launch_project(slosh_ctx *c);
 void launch_extrapolate(slosh_ctx *c, uint32_t layer);
 void launch_step_stats(slosh_ctx *c);
 void launch_density_rhs(slosh_ctx *c, float relax, int keep_q);
+// Slosh.usf:657-686 / SloshProxy.cpp:603-617: `iterations` Jacobi sweeps of the implicit viscosity on the
+// normalised velocities (alpha = nu dt / h^2); the result becomes the current velocity buffer.
+void run_viscosity(slosh_ctx *c, float alpha, int iterations);
 void launch_density_disp(slosh_ctx *c);
 void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);

@@ -97,6 +97,12 @@ __device__ __forceinline__ void brick_coord(const Geo &g, uint32_t b, int c[3])
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 __device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+__device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
 
 // Slosh.usf:357 / :518: clamped cell of a position.
 __device__ __forceinline__ void pos_cell(const Geo &g, float px, float py, float pz, int c[3])
@@ -177,6 +183,76 @@ __device__ __forceinline__ float collider_velocity_axis(const slosh_collider &c,
     float cr2 = w[0] * r[1] - w[1] * r[0];
     float cr = axis == 0 ? cr0 : (axis == 1 ? cr1 : cr2);
     return c.linear_velocity[axis] + cr;
+}
+
+// SloshCommon.ush:207-230 (ManipulatorSDF, ManipulatorWeight): the shape's signed distance (shape 3 everywhere: -1),
+// and the effect's weight, 1 inside the shape fading linearly to 0 over `falloff` beyond it.
+__device__ inline float manipulator_sdf(const slosh_manipulator &m, const float p[3])
+{
+    if (m.shape == 3u) return -1.0f;
+    slosh_collider c{};
+    for (int a = 0; a < 3; ++a) {
+        c.center[a] = m.center[a];
+        c.extent[a] = m.extent[a];
+    }
+    for (int k = 0; k < 4; ++k) c.rotation[k] = m.rotation[k];
+    c.type = m.shape;
+    return collider_sdf(c, p);
+}
+
+__device__ inline float manipulator_weight(const slosh_manipulator &m, const float p[3])
+{
+    const float d = manipulator_sdf(m, p);
+    if (d <= 0.0f) return 1.0f;
+    return m.falloff > 0.0f ? clampf(1.0f - d / m.falloff, 0.0f, 1.0f) : 0.0f;
+}
+
+// Slosh.usf:1523-1559: the force fields (every kind but drains) acting on the particle's new velocity nv at x.
+__device__ inline void apply_force_fields(const slosh_manipulator *man, uint32_t nman, const float x[3], float nv[3],
+                                          float dt)
+{
+    for (uint32_t k = 0; k < nman; ++k) {
+        const slosh_manipulator m = man[k];
+        if (m.kind == 0u) continue;
+        const float w = manipulator_weight(m, x);
+        if (w <= 0.0f) continue;
+        const float r[3] = {x[0] - m.center[0], x[1] - m.center[1], x[2] - m.center[2]};
+        const float f = m.strength * w * dt;
+        if (m.kind == 1u) {
+            for (int a = 0; a < 3; ++a) nv[a] = nv[a] + m.axis[a] * f;
+        } else if (m.kind == 2u) {
+            const float rl = len3(r[0], r[1], r[2]);
+            for (int a = 0; a < 3; ++a) nv[a] = nv[a] + (rl > 1e-5f ? r[a] / rl : 0.0f) * f;
+        } else if (m.kind == 3u) {
+            const float ra = r[0] * m.axis[0] + r[1] * m.axis[1] + r[2] * m.axis[2];
+            float perp[3];
+            for (int a = 0; a < 3; ++a) perp[a] = r[a] - ra * m.axis[a];
+            const float pl = len3(perp[0], perp[1], perp[2]);
+            if (pl > 1e-5f) {
+                float pu[3], tg[3];
+                for (int a = 0; a < 3; ++a) pu[a] = perp[a] / pl;
+                cross3(m.axis, pu, tg);
+                for (int a = 0; a < 3; ++a) nv[a] = nv[a] + (tg[a] - pu[a] * m.param0) * f;
+            }
+        } else if (m.kind == 4u) {
+            const float e = expf(-m.strength * w * dt);
+            for (int a = 0; a < 3; ++a) nv[a] = nv[a] * e;
+        }
+    }
+}
+
+// Slosh.usf:1639-1654: whether a drain removes the particle (index i) at x: at once (param0 = 0), else with
+// probability 1 - exp(-param0 dt) from the PCG hash of the particle, the frame and the drain.
+__device__ inline bool drained_at(const slosh_manipulator *man, uint32_t nman, const float x[3], float dt, uint32_t i,
+                                  uint32_t frame_seed)
+{
+    for (uint32_t k = 0; k < nman; ++k) {
+        const slosh_manipulator m = man[k];
+        if (m.kind != 0u || manipulator_sdf(m, x) > 0.0f) continue;
+        const float chance = m.param0 > 0.0f ? 1.0f - expf(-m.param0 * dt) : 1.0f;
+        if (chance >= 1.0f || hash_to_unit(pcg_hash(i * 2654435761u + frame_seed * 97u + k)) < chance) return true;
+    }
+    return false;
 }
 
 // SloshCommon.ush:244-252
