@@ -78,7 +78,10 @@ typedef struct slosh_settings {
     float kinematic_viscosity;    /* m^2/s; > 0 runs viscosity_iterations Jacobi sweeps of (I - nu dt Laplacian) u = u*
                                      after the normalise pass (Slosh.usf:657-686, SloshProxy.cpp:603-617) */
     int32_t viscosity_iterations; /* 0..1000 */
-    int32_t reserved[8];          /* must be zero */
+    float fluid_density;          /* kg/m^3 (SloshTypes.h:121); converts the solver's pressure to pascals for the
+                                     collider feedback, p_Pa = p * density * h / dt (SloshProxy.cpp:562); 0 means
+                                     1000 */
+    int32_t reserved[7];          /* must be zero */
 } slosh_settings;
 
 #define SLOSH_FLAG_NONE 0u
@@ -103,6 +106,9 @@ typedef struct slosh_settings {
                                              A q = relaxation * clamp(rho - 1) with the pressure operator and shift
                                              every particle by -grad q (at most half a cell); it replaces the
                                              velocity density correction (density_correction is then not applied) */
+#define SLOSH_FLAG_BINARY_COUPLING 128u /* collider feedback: the pressure force from the binary fluid-solid cell faces
+                                           (bSmoothCouplingForce = false, Slosh.usf:930-946) instead of the smoothed
+                                           SDF-weighted estimator (Slosh.usf:1138-1216) */
 #define SLOSH_FLAG_REPORT_FP64_ERROR 16u /* MGPCG: after each solve, solve the same system again from its solution to
                                             an FP64 true residual of 1e-12 (restarts form b - A p in FP64, so the FP32
                                             vectors and FP16 levels do not limit it) and report the first solution's
@@ -142,6 +148,22 @@ typedef struct slosh_collider {
     float pad1;
 } slosh_collider;
 #define SLOSH_MAX_COLLIDERS 64u
+#define SLOSH_COLLIDER_TWO_WAY 1u /* collider flags: the body receives pressure force feedback (SloshCommon.ush:42) */
+
+/* Per-collider feedback of the last step: FSloshColliderFeedback (SloshTypes.h:197-206), decoded as
+ * PollReadbacks_RenderThread does for one substep (SloshProxy.cpp:755-768). Force, torque, wetted faces and fluid
+ * velocity are zero for one-way colliders; the cell counts and the centroid are kept for every collider. */
+typedef struct slosh_collider_feedback {
+    float pressure_force[3];           /* N */
+    float pressure_torque[3];          /* N m about the collider's centre */
+    float mean_fluid_velocity[3];      /* m/s, fluid cell-centre velocity after the projection over wetted faces */
+    float submerged_centroid_offset[3];/* m, mean offset of the submerged cells from the centre */
+    int32_t wetted_faces;              /* fluid-solid faces of the body */
+    int32_t submerged_cells;           /* body cells whose horizontal walk out of the body first meets dense,
+                                          supported liquid */
+    int32_t solid_cells;               /* body cells in allocated bricks */
+    int32_t reserved;
+} slosh_collider_feedback;
 
 /* Particle manipulator: FSloshManipulatorGPU (SloshTypes.h:66-80), 80 bytes. kind 0 drain (removes the particles
  * inside: all at once when param0 = 0, else at rate param0 per second), 1 directional (acceleration strength along
@@ -252,6 +274,9 @@ slosh_status slosh_read_particles(slosh_ctx *ctx, void *dst, uint64_t dst_bytes,
 slosh_status slosh_read_grid(slosh_ctx *ctx, int32_t field, void *dst, uint64_t dst_bytes, int32_t order);
 /* = FSloshProxy::GetResults (SloshProxy.cpp:177-183), synchronous. */
 slosh_status slosh_get_results(slosh_ctx *ctx, slosh_results *out);
+/* = FSloshResults::Colliders (SloshTypes.h:210): one entry per collider of the last step, in the order given to
+ * slosh_set_colliders; *n receives the collider count and at most `cap` entries are written. Synchronous. */
+slosh_status slosh_get_collider_feedback(slosh_ctx *ctx, slosh_collider_feedback *out, uint32_t cap, uint32_t *n);
 /* No counterpart in the reference. For callers that cannot allocate device memory (tight render or serving loops,
  * harnesses that forbid hipMalloc): the same copy as slosh_read_particles, staged through one fixed library-wide
  * device buffer of SLOSH_NOALLOC_STAGING_BYTES per GPU (held until the process ends) instead of the context's

@@ -104,6 +104,8 @@ struct osim {
     slosh_manipulator man[SLOSH_MAX_MANIPULATORS];
     uint32_t nman;
     uint64_t drained_total;
+    int32_t coupling[SLOSH_MAX_COLLIDERS * 16]; /* fixed-point collider coupling sums of the last step */
+    uint32_t coupling_ncol;
     slosh_results res;
     uint64_t frame;
     float prev_dt;
@@ -1469,6 +1471,263 @@ static void stage_project(osim *o)
 }
 
 /* ------------------------------------------------------------------------------------------------------------ */
+/* Collider coupling (Slosh.usf:909-1216, 1316-1370; driver SloshProxy.cpp:397, 562, 645-649, 715-718). Sums per
+ * collider in 1/1024 units, integer adds, so the parallel order does not matter. Slots: 0-2 force, 3-5 torque,
+ * 6 wetted faces, 7 submerged cells, 8-10 fluid velocity, 11 solid cells, 12-14 submerged offset.
+ * The wetted faces' fluid velocity is read after the whole projection (the reference reads it while other faces
+ * are still being projected). */
+#define COUPLING_SCALE 1024.0f
+static uint32_t collider_id(uint32_t packed) { return (packed >> CELL_COLLIDER_SHIFT) & 0xFFu; }
+static void add_fixed(int32_t *p, float v) { add_i32(p, (int32_t)rintf(v * COUPLING_SCALE)); }
+
+/* Slosh.usf:913-926 */
+static void pressure_force(osim *o, uint32_t k, const float fp[3], const float into[3], float p_hat, float p2pa)
+{
+    const float pa = p_hat * p2pa;
+    float f[3], r[3], t[3];
+    for (int a = 0; a < 3; ++a) {
+        f[a] = pa * o->h * o->h * into[a];
+        r[a] = fp[a] - o->col[k].center[a];
+    }
+    cross3(r, f, t);
+    for (int a = 0; a < 3; ++a) add_fixed(&o->coupling[k * 16u + (uint32_t)a], f[a]);
+    for (int a = 0; a < 3; ++a) add_fixed(&o->coupling[k * 16u + 3u + (uint32_t)a], t[a]);
+}
+
+/* Slosh.usf:951-983: walks of up to 16 cells along +x, -x, +y, -y out of collider id's solid region. */
+static int seepage_pressure(const osim *o, int x, int y, int z, uint32_t id, float *p_out)
+{
+    static const int dirs[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+    float sum = 0.0f, cnt = 0.0f;
+    for (int d = 0; d < 4; ++d) {
+        int px = x, py = y;
+        for (int s = 0; s < 16; ++s) {
+            px += dirs[d][0];
+            py += dirs[d][1];
+            const uint32_t t = cell_type_at(o, px, py, z);
+            if ((t & CELL_TYPE_MASK) != CELL_SOLID || collider_id(t) != id) {
+                if ((t & CELL_TYPE_MASK) == CELL_FLUID) {
+                    sum += (float)o->pres[cell_index(o, px, py, z)];
+                    cnt += 1.0f;
+                }
+                break;
+            }
+        }
+    }
+    *p_out = cnt > 0.0f ? sum / cnt : 0.0f;
+    return cnt > 0.0f;
+}
+
+/* Slosh.usf:985-998 */
+static void contact(osim *o, uint32_t packed, int x, int y, int z, const float fp[3], const float into[3], float p2pa)
+{
+    const uint32_t id = collider_id(packed);
+    float p;
+    if (id == 0u || !(o->col[id - 1u].flags & SLOSH_COLLIDER_TWO_WAY)) return;
+    if (seepage_pressure(o, x, y, z, id, &p)) pressure_force(o, id - 1u, fp, into, p, p2pa);
+}
+
+/* Slosh.usf:1003-1028 (the coupling terms of ProjectFace) */
+static void stage_coupling_faces(osim *o, float p2pa, int binary)
+{
+    const float *vel = o->vel[o->cur];
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
+    {
+        const uint32_t B = o->ctype[idx], tb = B & CELL_TYPE_MASK;
+        for (int a = 0; a < 3; ++a) {
+            int ca[3] = {x, y, z};
+            ca[a] -= 1;
+            const uint32_t A = cell_type_at(o, ca[0], ca[1], ca[2]), ta = A & CELL_TYPE_MASK;
+            /* FacePosition (Slosh.usf:204-208) */
+            float fp[3] = {((float)x + 0.5f) * o->h, ((float)y + 0.5f) * o->h, ((float)z + 0.5f) * o->h};
+            const int cc[3] = {x, y, z};
+            fp[a] = (float)cc[a] * o->h;
+            float axis[3] = {0.0f, 0.0f, 0.0f}, naxis[3] = {0.0f, 0.0f, 0.0f};
+            axis[a] = 1.0f;
+            naxis[a] = -1.0f;
+            if (ta == CELL_SOLID && tb == CELL_SOLID) {
+                if (collider_id(A) != collider_id(B)) {
+                    contact(o, B, x, y, z, fp, axis, p2pa);
+                    contact(o, A, ca[0], ca[1], ca[2], fp, naxis, p2pa);
+                }
+            } else if (ta == CELL_SOLID || tb == CELL_SOLID) {
+                uint32_t id = 0u;
+                int fc[3] = {x, y, z};
+                const float *into = axis;
+                if (ta == CELL_FLUID && collider_id(B) != 0u) {
+                    id = collider_id(B);
+                    fc[0] = ca[0];
+                    fc[1] = ca[1];
+                    fc[2] = ca[2];
+                } else if (tb == CELL_FLUID && collider_id(A) != 0u) {
+                    id = collider_id(A);
+                    into = naxis;
+                }
+                if (id == 0u || !(o->col[id - 1u].flags & SLOSH_COLLIDER_TWO_WAY)) continue;
+                const uint32_t k = id - 1u, fi = cell_index(o, fc[0], fc[1], fc[2]);
+                if (binary) pressure_force(o, k, fp, into, (float)o->pres[fi], p2pa);
+                add_i32(&o->coupling[k * 16u + 6u], 1);
+                /* Slosh.usf:898-907 (CellCenterVelocity) */
+                for (int b = 0; b < 3; ++b) {
+                    int up[3] = {fc[0], fc[1], fc[2]};
+                    up[b] += 1;
+                    const float v = 0.5f * (vel[face_index(o, b, fc[0], fc[1], fc[2])] +
+                                            vel[face_index(o, b, up[0], up[1], up[2])]);
+                    add_fixed(&o->coupling[k * 16u + 8u + (uint32_t)b], v);
+                }
+            }
+        }
+    }
+}
+
+static uint32_t type_of(const osim *o, int x, int y, int z) { return cell_type_at(o, x, y, z) & CELL_TYPE_MASK; }
+
+/* Slosh.usf:1138-1216 (CouplingForceCS with FluidPressureGradient 1094-1111 and BodySurfacePressure 1115-1136) */
+static void stage_coupling_force(osim *o, float p2pa)
+{
+    const float h = o->h;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
+    {
+        const uint32_t packed = o->ctype[idx], type = packed & CELL_TYPE_MASK;
+        if (type == CELL_AIR) continue;
+        const float X[3] = {((float)x + 0.5f) * h, ((float)y + 0.5f) * h, ((float)z + 0.5f) * h};
+        const int C[3] = {x, y, z};
+        float ph = 0.0f, gr[3] = {0.0f, 0.0f, 0.0f};
+        int have = 0;
+        if (type == CELL_FLUID) {
+            ph = (float)o->pres[idx];
+            for (int a = 0; a < 3; ++a) {
+                int p[3] = {x, y, z}, m[3] = {x, y, z};
+                p[a] += 1;
+                m[a] -= 1;
+                const int bp = type_of(o, p[0], p[1], p[2]) == CELL_FLUID, bm = type_of(o, m[0], m[1], m[2]) == CELL_FLUID;
+                const float pp = bp ? (float)o->pres[cell_index(o, p[0], p[1], p[2])] : 0.0f;
+                const float pm = bm ? (float)o->pres[cell_index(o, m[0], m[1], m[2])] : 0.0f;
+                gr[a] = bp && bm ? (pp - pm) * 0.5f : (bp ? pp - ph : (bm ? ph - pm : 0.0f));
+            }
+            for (int a = 0; a < 3; ++a) gr[a] = gr[a] * o->inv_h;
+            have = 1;
+        } else if (collider_id(packed) != 0u) {
+            float sum = 0.0f, cnt = 0.0f;
+            for (int nb = 0; nb < 6; ++nb) {
+                int d[3] = {0, 0, 0};
+                d[nb >> 1] = (nb & 1) ? 1 : -1;
+                const int n1[3] = {C[0] + d[0], C[1] + d[1], C[2] + d[2]};
+                if (type_of(o, n1[0], n1[1], n1[2]) != CELL_FLUID) continue;
+                const float pn = (float)o->pres[cell_index(o, n1[0], n1[1], n1[2])];
+                const int n2[3] = {n1[0] + d[0], n1[1] + d[1], n1[2] + d[2]};
+                const float slope =
+                    type_of(o, n2[0], n2[1], n2[2]) == CELL_FLUID ? (float)o->pres[cell_index(o, n2[0], n2[1], n2[2])] - pn : 0.0f;
+                sum += pn - 0.5f * slope;
+                cnt += 1.0f;
+            }
+            ph = cnt > 0.0f ? sum / cnt : 0.0f;
+            have = cnt > 0.0f;
+        }
+        if (!have) continue;
+        for (uint32_t k = 0; k < o->ncol; ++k) {
+            const slosh_collider *cc = &o->col[k];
+            if (!(cc->flags & SLOSH_COLLIDER_TWO_WAY) || fabsf(oracle_collider_sdf(cc, X)) > h) continue;
+            if (type == CELL_SOLID && collider_id(packed) != k + 1u) continue;
+            float sp = ph;
+            if (type == CELL_FLUID) {
+                const float phi = oracle_collider_sdf(cc, X);
+                float n[3];
+                collider_normal(cc, X, 0.25f * h, n);
+                sp += gr[0] * (-phi * n[0]) + gr[1] * (-phi * n[1]) + gr[2] * (-phi * n[2]);
+            }
+            const float scale = -sp * p2pa * h * h;
+            float F[3] = {0.0f, 0.0f, 0.0f}, T[3] = {0.0f, 0.0f, 0.0f};
+            for (int nb = 0; nb < 6; ++nb) {
+                float nrm[3] = {0.0f, 0.0f, 0.0f};
+                nrm[nb >> 1] = (nb & 1) ? 1.0f : -1.0f;
+                if (type == CELL_SOLID) {
+                    int d[3] = {0, 0, 0};
+                    d[nb >> 1] = (nb & 1) ? 1 : -1;
+                    const uint32_t nbp = cell_type_at(o, x + d[0], y + d[1], z + d[2]);
+                    if ((nbp & CELL_TYPE_MASK) == CELL_SOLID && collider_id(nbp) != k + 1u) continue;
+                }
+                float fpos[3], df[3], r[3], t[3];
+                for (int a = 0; a < 3; ++a) fpos[a] = X[a] + 0.5f * h * nrm[a];
+                const float w = clampf(0.5f + oracle_collider_sdf(cc, fpos) * o->inv_h, 0.0f, 1.0f);
+                for (int a = 0; a < 3; ++a) {
+                    df[a] = scale * w * nrm[a];
+                    r[a] = fpos[a] - cc->center[a];
+                }
+                cross3(r, df, t);
+                for (int a = 0; a < 3; ++a) {
+                    F[a] += df[a];
+                    T[a] += t[a];
+                }
+            }
+            for (int a = 0; a < 3; ++a) add_fixed(&o->coupling[k * 16u + (uint32_t)a], F[a]);
+            for (int a = 0; a < 3; ++a) add_fixed(&o->coupling[k * 16u + 3u + (uint32_t)a], T[a]);
+        }
+    }
+}
+
+/* Slosh.usf:1316-1370 (SubmergedCS): walks of up to 12 cells along +x, -x, +y, -y; the exit cell must be dense fluid
+ * with no air in the three cells below it along -z. */
+static void stage_submerged(osim *o)
+{
+    static const int dirs[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t e = 0; e < o->nactive; ++e) FOR_BRICK_CELLS(o, e, idx, x, y, z)
+    {
+        const uint32_t packed = o->ctype[idx], id = collider_id(packed);
+        if ((packed & CELL_TYPE_MASK) != CELL_SOLID || id == 0u) continue;
+        int32_t *acc = &o->coupling[(id - 1u) * 16u];
+        add_i32(&acc[11], 1);
+        int sub = 0;
+        for (int d = 0; d < 4; ++d) {
+            int px = x, py = y;
+            for (int s = 0; s < 12; ++s) {
+                px += dirs[d][0];
+                py += dirs[d][1];
+                const uint32_t t = cell_type_at(o, px, py, z);
+                if ((t & CELL_TYPE_MASK) != CELL_SOLID || collider_id(t) != id) {
+                    int liquid = (t & CELL_TYPE_MASK) == CELL_FLUID && (t & CELL_DENSE_FLAG) != 0u;
+                    for (int below = 1; below <= 3; ++below)
+                        liquid = liquid && type_of(o, px, py, z - below) != CELL_AIR;
+                    sub = sub || liquid;
+                    break;
+                }
+            }
+        }
+        if (!sub) continue;
+        add_i32(&acc[7], 1);
+        const slosh_collider *cc = &o->col[id - 1u];
+        const float off[3] = {((float)x + 0.5f) * o->h - cc->center[0], ((float)y + 0.5f) * o->h - cc->center[1],
+                              ((float)z + 0.5f) * o->h - cc->center[2]};
+        for (int a = 0; a < 3; ++a) add_fixed(&acc[12 + a], off[a]);
+    }
+}
+
+static void stage_coupling(osim *o, float dt)
+{
+    memset(o->coupling, 0, sizeof(o->coupling));
+    o->coupling_ncol = 0;
+    if (!o->ncol) return;
+    /* SloshProxy.cpp:562: PressureToPascal = density h / dt */
+    const float rho = o->s.fluid_density > 0.0f ? o->s.fluid_density : 1000.0f;
+    const float p2pa = rho * o->h / dt;
+    const int binary = (o->s.flags & SLOSH_FLAG_BINARY_COUPLING) != 0u;
+    stage_coupling_faces(o, p2pa, binary);
+    if (!binary) stage_coupling_force(o, p2pa);
+    stage_submerged(o);
+    o->coupling_ncol = o->ncol;
+}
+
+int osim_coupling_raw(const osim *o, int32_t *dst, uint32_t cap)
+{
+    const uint32_t m = o->coupling_ncol < cap ? o->coupling_ncol : cap;
+    memcpy(dst, o->coupling, (size_t)m * 16u * sizeof(int32_t));
+    return (int)o->coupling_ncol;
+}
+
+/* ------------------------------------------------------------------------------------------------------------ */
 /* Two-sided density projection (Slosh.usf:743-854; driver SloshProxy.cpp:656-702).                             */
 /* Slosh.usf:753-773 (CellDensityInterior): P2G weight of the faces no solid shares, per rest count of those faces. */
 static float cell_density_interior(const osim *o, int x, int y, int z)
@@ -1834,6 +2093,7 @@ int osim_step_until(osim *o, float dt, const float g[3], int stop_after)
     else if (o->s.pressure_solver == SLOSH_SOLVER_RBSOR) solve_rbsor(o);
     if (stop_after <= SLOSH_STAGE_SOLVE) return SLOSH_OK;
     stage_project(o);
+    stage_coupling(o, dt);
     o->dens_applied = 0;
     if ((o->s.flags & SLOSH_FLAG_DENSITY_PROJECTION) && o->nactive) {
         int interval = o->s.density_projection_interval > 0 ? o->s.density_projection_interval : 1;

@@ -47,6 +47,10 @@ double osim_true_residual(const osim *o, const float *dense_p);
 /* Same for an FP64 dense field. */
 double osim_true_residual_d(const osim *o, const double *dense_p);
 
+/* The last step's collider coupling sums, 16 int32 per collider in 1/1024 units (the layout of
+ * slosh_get_collider_feedback before decoding); copies min(count, cap) colliders, returns the count. */
+int osim_coupling_raw(const osim *o, int32_t *dst, uint32_t cap);
+
 /* Threads of the parallel (OpenMP) loops; results are the same for any count. */
 void oracle_set_threads(int n);
 int oracle_threads(void);

@@ -69,6 +69,8 @@ def load():
     lib.oracle_collider_response.argtypes = [C.POINTER(abi.Collider), C.c_uint32, C.c_float, C.c_float, C.c_float,
                                              f3, f3]
     lib.oracle_collider_response.restype = None
+    lib.osim_coupling_raw.restype = C.c_int
+    lib.osim_coupling_raw.argtypes = [vp, vp, C.c_uint32]
     lib.oracle_set_threads.argtypes = [C.c_int]
     lib.oracle_set_threads.restype = None
     lib.oracle_threads.argtypes = []
@@ -164,6 +166,15 @@ class OracleSim:
         r = abi.Results()
         self.lib.osim_get_results(self.h, C.byref(r))
         return r
+
+    def coupling_raw(self):
+        """The last step's collider coupling sums, int32 [colliders][16] in 1/1024 units."""
+        out = np.zeros((64, 16), dtype=np.int32)
+        n = self.lib.osim_coupling_raw(self.h, out.ctypes.data, 64)
+        return out[:n].copy()
+
+    def collider_feedback(self):
+        return abi.decode_coupling(self.coupling_raw())
 
     def solve_fp64(self, rel_tol=1e-13, max_iter=500):
         nx, ny, nz = self.dims

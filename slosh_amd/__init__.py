@@ -62,6 +62,9 @@ def load_library(path=LIB_PATH):
     lib.slosh_read_grid.argtypes = [vp, C.c_int32, vp, C.c_uint64, C.c_int32]
     lib.slosh_read_grid.restype = st
     lib.slosh_get_results.argtypes = [vp, C.POINTER(abi.Results)]
+    lib.slosh_get_collider_feedback.restype = st
+    lib.slosh_get_collider_feedback.argtypes = [vp, C.POINTER(abi.ColliderFeedback), C.c_uint32,
+                                                C.POINTER(C.c_uint32)]
     lib.slosh_get_results.restype = st
     lib.slosh_device_particles.argtypes = [vp, C.c_int32, C.POINTER(vp), C.POINTER(C.c_uint32)]
     lib.slosh_device_particles.restype = st
@@ -176,6 +179,13 @@ class Sim:
         r = abi.Results()
         _check(self.lib.slosh_get_results(self.h, C.byref(r)), "slosh_get_results")
         return r
+
+    def collider_feedback(self):
+        """Per-collider feedback of the last step (slosh_get_collider_feedback), a list of abi.ColliderFeedback."""
+        buf = (abi.ColliderFeedback * 64)()
+        n = C.c_uint32()
+        _check(self.lib.slosh_get_collider_feedback(self.h, buf, 64, C.byref(n)), "slosh_get_collider_feedback")
+        return [buf[k] for k in range(n.value)]
 
     def num_particles(self):
         return int(self.results().live_particles)

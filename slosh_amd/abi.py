@@ -28,6 +28,8 @@ FLAG_WARM_START_EXTRAPOLATE = 8
 FLAG_REPORT_FP64_ERROR = 16
 FLAG_CAPTURE_VCYCLE_RESIDUALS = 32
 FLAG_DENSITY_PROJECTION = 64
+FLAG_BINARY_COUPLING = 128
+COLLIDER_TWO_WAY = 1
 
 EMIT_FILL_CELLS = 0
 EMIT_SPHERE = 1
@@ -107,7 +109,8 @@ class Settings(C.Structure):
         ("density_projection_interval", C.c_int32),
         ("kinematic_viscosity", C.c_float),
         ("viscosity_iterations", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("fluid_density", C.c_float),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -167,6 +170,48 @@ class Results(C.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class ColliderFeedback(C.Structure):
+    """slosh_collider_feedback (FSloshColliderFeedback, SloshTypes.h:197-206), 64 bytes."""
+    _fields_ = [
+        ("pressure_force", C.c_float * 3),
+        ("pressure_torque", C.c_float * 3),
+        ("mean_fluid_velocity", C.c_float * 3),
+        ("submerged_centroid_offset", C.c_float * 3),
+        ("wetted_faces", C.c_int32),
+        ("submerged_cells", C.c_int32),
+        ("solid_cells", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            if name != "reserved":
+                out[name] = list(v) if isinstance(v, C.Array) else v
+        return out
+
+
+def decode_coupling(raw):
+    """The decode of slosh_get_collider_feedback (SloshProxy.cpp:755-768, one substep) applied to raw int32 sums
+    [n][16] in 1/1024 units, with the library's FP32 operations."""
+    out = []
+    inv = np.float32(1.0 / 1024.0)
+    for row in np.asarray(raw, dtype=np.int32).reshape(-1, 16):
+        f = ColliderFeedback()
+        c = row.astype(np.float32)
+        for a in range(3):
+            f.pressure_force[a] = float(c[a] * inv)
+            f.pressure_torque[a] = float(c[3 + a] * inv)
+            f.mean_fluid_velocity[a] = float(c[8 + a] * (inv / c[6])) if row[6] > 0 else 0.0
+            f.submerged_centroid_offset[a] = float(c[12 + a] * (inv / c[7])) if row[7] > 0 else 0.0
+        f.wetted_faces = int(row[6])
+        f.submerged_cells = int(row[7])
+        f.solid_cells = int(row[11])
+        out.append(f)
+    return out
 
 
 class Manipulator(C.Structure):
@@ -256,7 +301,7 @@ assert PARTICLE_DTYPE.itemsize == 64 and PARTICLE_REF48_DTYPE.itemsize == 48
 def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6, max_iterations=200, sor_weight=1.8,
                   budget=1.0, extrapolation_layers=2, ghost_fluid=1, friction=0.0, max_speed=20.0, flags=0,
                   mg_coarse_sweeps=0, density_correction=0.0, density_relaxation=0.5, density_iterations=30,
-                  density_interval=1, viscosity=0.0, viscosity_iterations=12):
+                  density_interval=1, viscosity=0.0, viscosity_iterations=12, fluid_density=1000.0):
     """Parity-profile settings (APIC, ghost fluid, no density correction) with the given grid and solver."""
     s = Settings()
     for a in range(3):
@@ -282,6 +327,7 @@ def make_settings(grid, cell_size, capacity, solver=SOLVER_MGPCG, tolerance=1e-6
     s.density_projection_interval = int(density_interval)
     s.kinematic_viscosity = float(viscosity)
     s.viscosity_iterations = int(viscosity_iterations)
+    s.fluid_density = float(fluid_density)
     return s
 
 

@@ -25,7 +25,7 @@ const char *const kKernelNames[K_COUNT] = {
     "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d23down", "k_d23up",
     "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
     "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io",
-    "k_unet_train", "k_fcg", "k_g2p_move", "k_density", "k_viscosity"};
+    "k_unet_train", "k_fcg", "k_g2p_move", "k_density", "k_viscosity", "k_coupling"};
 } // namespace slosh
 
 static_assert(sizeof(slosh_collider) == 80, "collider layout (SloshTypes.h:32-44)");
@@ -35,6 +35,7 @@ static_assert(sizeof(slosh_settings) == 128, "settings ABI size");
 static_assert(sizeof(slosh_manipulator) == 80, "manipulator layout (SloshTypes.h:66-80)");
 static_assert(sizeof(slosh_emit_request) == 100, "emit request ABI size");
 static_assert(sizeof(slosh_results) == 96, "results ABI size");
+static_assert(sizeof(slosh_collider_feedback) == 64, "collider feedback ABI size");
 
 namespace {
 
@@ -82,7 +83,7 @@ void free_all(slosh_ctx *c)
     dfree(c->dens_rhs); dfree(c->dens_q); dfree(c->dens_disp);
     dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
     dfree(c->bp_clamp); dfree(c->bp_drain); dfree(c->d_man);
-    dfree(c->state); dfree(c->stats); dfree(c->d_col);
+    dfree(c->state); dfree(c->stats); dfree(c->d_col); dfree(c->coupling);
     dfree(c->staging);
     for (uint32_t k = 0; k < c->tev_cap; ++k) (void)hipEventDestroy(c->tev[k]);
     delete[] c->tev;
@@ -121,7 +122,8 @@ bool settings_valid(const slosh_settings *s)
             if (s->grid_dim[a] % 8 != 0) return false; // the network's levels need whole bricks
     if (s->flags & ~(uint32_t)(SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_MG_FP16 | SLOSH_FLAG_MG_FLUID_COARSENING |
                                SLOSH_FLAG_WARM_START_EXTRAPOLATE | SLOSH_FLAG_REPORT_FP64_ERROR |
-                               SLOSH_FLAG_CAPTURE_VCYCLE_RESIDUALS | SLOSH_FLAG_DENSITY_PROJECTION))
+                               SLOSH_FLAG_CAPTURE_VCYCLE_RESIDUALS | SLOSH_FLAG_DENSITY_PROJECTION |
+                               SLOSH_FLAG_BINARY_COUPLING))
         return false;
     if (!(s->tolerance > 0.0f) || s->max_iterations < 0) return false;
     if (!(s->sor_weight >= 0.5f && s->sor_weight <= 1.95f)) return false;
@@ -133,7 +135,8 @@ bool settings_valid(const slosh_settings *s)
     if (s->density_projection_interval < 0 || s->density_projection_interval > 8) return false;
     if (!(s->density_correction >= 0.0f) || !(s->density_deadband >= 0.0f)) return false;
     if (!(s->kinematic_viscosity >= 0.0f) || s->viscosity_iterations < 0 || s->viscosity_iterations > 1000) return false;
-    for (int k = 0; k < 8; ++k)
+    if (!(s->fluid_density >= 0.0f && s->fluid_density <= 1e6f)) return false;
+    for (int k = 0; k < 7; ++k)
         if (s->reserved[k] != 0) return false;
     return true;
 }
@@ -322,6 +325,7 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->state, 1);
     dalloc(&c->stats, 1);
     dalloc(&c->d_col, SLOSH_MAX_COLLIDERS);
+    dalloc(&c->coupling, (size_t)SLOSH_MAX_COLLIDERS * 16u);
     dalloc(&c->d_man, SLOSH_MAX_MANIPULATORS);
     if (hipHostMalloc((void **)&c->h_stats, sizeof(StepStats), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
     if (hipHostMalloc((void **)&c->h_nsolve, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
@@ -346,6 +350,7 @@ void reset_impl(slosh_ctx *c)
     c->frame = 0;
     c->prev_dt = 0.0f;
     c->drained_total = 0;
+    c->coupling_ncol = 0;
     memset(&c->res, 0, sizeof(c->res));
     launch_fill_u32(c->stream, c->table, 0u, g.nbricks);
     launch_fill_u32(c->stream, c->slot_brick, kNoBrick, g.npool);
@@ -450,6 +455,16 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
             SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
     }
     if (stop > SLOSH_STAGE_SOLVE) launch_project(c);
+    // collider coupling of this step (SloshProxy.cpp:397 clears it per frame; one step is one substep here): the
+    // contact and wetted-face terms of the projection, the smoothed force (Slosh.usf:1138-1216) and the submerged
+    // cells (SloshProxy.cpp:715-718), all on this step's cell types and solved pressure
+    c->coupling_ncol = 0;
+    if (stop > SLOSH_STAGE_SOLVE && c->ncol) {
+        launch_fill_u32(c->stream, (uint32_t *)c->coupling, 0u, (size_t)c->ncol * 16u);
+        launch_coupling(c, dt);
+        launch_submerged(c);
+        c->coupling_ncol = c->ncol;
+    }
     // SloshProxy.cpp:656-702: the two-sided density projection after the pressure projection, every
     // density_projection_interval steps with the relaxation compounded to 1 - (1 - r)^N
     c->dens_applied = false;
@@ -757,6 +772,7 @@ void slosh_default_settings(slosh_settings *s)
     s->density_projection_interval = 1;
     s->kinematic_viscosity = 0.0f;
     s->viscosity_iterations = 12;
+    s->fluid_density = 1000.0f;
 }
 
 slosh_status slosh_create(const slosh_settings *s, slosh_ctx **out)
@@ -1154,6 +1170,39 @@ slosh_status slosh_unet_train(slosh_ctx *c, const slosh_unet_train_config *cfg, 
             out->last_loss = last;
             out->systems = c->capture_n;
             out->adam_steps = (uint32_t)u.adam_t;
+        }
+        return SLOSH_OK;
+    })
+}
+
+// SloshProxy.cpp:755-768 for one substep: sums in 1/1024 units, means over the wetted faces / submerged cells.
+slosh_status slosh_get_collider_feedback(slosh_ctx *c, slosh_collider_feedback *out, uint32_t cap, uint32_t *n)
+{
+    if (!c || !n || (cap && !out)) return SLOSH_EINVAL;
+    ABI_GUARD({
+        const uint32_t nc = c->coupling_ncol;
+        *n = nc;
+        const uint32_t m = nc < cap ? nc : cap;
+        if (m) {
+            int32_t raw[SLOSH_MAX_COLLIDERS * 16];
+            SLOSH_HIP(hipMemcpyAsync(raw, c->coupling, (size_t)m * 16u * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                     c->stream));
+            SLOSH_HIP(hipStreamSynchronize(c->stream));
+            const float inv = 1.0f / 1024.0f;
+            for (uint32_t k = 0; k < m; ++k) {
+                const int32_t *C = raw + k * 16u;
+                slosh_collider_feedback &o = out[k];
+                memset(&o, 0, sizeof(o));
+                for (int a = 0; a < 3; ++a) {
+                    o.pressure_force[a] = (float)C[a] * inv;
+                    o.pressure_torque[a] = (float)C[3 + a] * inv;
+                    o.mean_fluid_velocity[a] = C[6] > 0 ? (float)C[8 + a] * (inv / (float)C[6]) : 0.0f;
+                    o.submerged_centroid_offset[a] = C[7] > 0 ? (float)C[12 + a] * (inv / (float)C[7]) : 0.0f;
+                }
+                o.wetted_faces = C[6];
+                o.submerged_cells = C[7];
+                o.solid_cells = C[11];
+            }
         }
         return SLOSH_OK;
     })

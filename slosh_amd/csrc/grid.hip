@@ -507,6 +507,281 @@ __global__ __launch_bounds__(512) void k_viscosity(Geo g, const uint32_t *rows, 
     }
 }
 
+// ---- collider coupling (Slosh.usf:909-1216, 1316-1370) ----------------------------------------------------------
+// Per two-way collider k, 16 int32 sums in fixed point (1024 per unit; integer adds, so any order gives the same
+// sums): 0-2 pressure force (N), 3-5 torque (N m), 6 wetted faces, 7 submerged cells, 8-10 fluid velocity over the
+// wetted faces, 11 solid cells, 12-14 submerged-cell offset from the centre.
+constexpr float kCouplingScale = 1024.0f;
+
+__device__ __forceinline__ uint32_t collider_of(uint32_t packed) { return (packed >> kColliderShift) & 0xFFu; }
+
+// Cell type of any cell of the grid (walks leave the brick neighbourhood): through the brick table, SOLID outside.
+__device__ __forceinline__ uint32_t type_global(const Geo &g, const uint32_t *ctype, int x, int y, int z)
+{
+    if (!in_grid(g, x, y, z)) return kCellSolid;
+    const uint32_t b = (uint32_t)((x >> 3) + g.bd[0] * ((y >> 3) + g.bd[1] * (z >> 3)));
+    return ctype[g.table[b] * kBrickCells + local_of_cell(x, y, z)];
+}
+__device__ __forceinline__ uint32_t index_global(const Geo &g, int x, int y, int z)
+{
+    const uint32_t b = (uint32_t)((x >> 3) + g.bd[0] * ((y >> 3) + g.bd[1] * (z >> 3)));
+    return g.table[b] * kBrickCells + local_of_cell(x, y, z);
+}
+
+__device__ __forceinline__ void add_fixed(int32_t *p, float v) { atomicAdd(p, (int32_t)rintf(v * kCouplingScale)); }
+
+// Slosh.usf:913-926 (AccumulatePressureForce): force p_hat * PressureToPascal * h^2 along `into`, torque about the
+// collider's centre.
+__device__ void pressure_force(const Geo &g, const slosh_collider &col, int32_t *acc, const float fp[3],
+                               const float into[3], float p_hat, float p2pa)
+{
+    const float pa = p_hat * p2pa;
+    float f[3], r[3], t[3];
+    for (int a = 0; a < 3; ++a) {
+        f[a] = pa * g.h * g.h * into[a];
+        r[a] = fp[a] - col.center[a];
+    }
+    cross3(r, f, t);
+    for (int a = 0; a < 3; ++a) add_fixed(&acc[a], f[a]);
+    for (int a = 0; a < 3; ++a) add_fixed(&acc[3 + a], t[a]);
+}
+
+// Slosh.usf:951-983 (EstimateSeepagePressure): the mean pressure of the liquid met first by walks of up to 16 cells
+// along +x, -x, +y, -y out of collider `id`'s solid region (the reference's horizontal axes); false when dry.
+__device__ bool seepage_pressure(const Geo &g, const uint32_t *ctype, const double *pres, int x, int y, int z,
+                                 uint32_t id, float &p_out)
+{
+    const int dirs[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+    float sum = 0.0f, cnt = 0.0f;
+    for (int d = 0; d < 4; ++d) {
+        int px = x, py = y;
+        for (int s = 0; s < 16; ++s) {
+            px += dirs[d][0];
+            py += dirs[d][1];
+            const uint32_t t = type_global(g, ctype, px, py, z);
+            if ((t & kCellTypeMask) != kCellSolid || collider_of(t) != id) {
+                if ((t & kCellTypeMask) == kCellFluid) {
+                    sum += (float)pres[index_global(g, px, py, z)];
+                    cnt += 1.0f;
+                }
+                break;
+            }
+        }
+    }
+    p_out = cnt > 0.0f ? sum / cnt : 0.0f;
+    return cnt > 0.0f;
+}
+
+// Slosh.usf:985-998 (AccumulateContact): the seepage pressure on a contact face of a two-way body with another solid.
+__device__ void contact(const Geo &g, const uint32_t *ctype, const double *pres, const slosh_collider *col,
+                        int32_t *coupling, uint32_t packed, int x, int y, int z, const float fp[3], const float into[3],
+                        float p2pa)
+{
+    const uint32_t id = collider_of(packed);
+    if (id == 0u || !(col[id - 1u].flags & 1u)) return;
+    float p;
+    if (seepage_pressure(g, ctype, pres, x, y, z, id, p))
+        pressure_force(g, col[id - 1u], coupling + (id - 1u) * 16u, fp, into, p, p2pa);
+}
+
+// ProjectFace's coupling terms (Slosh.usf:1003-1028) after the projection: contact faces between different solids
+// get the seepage pressure; a fluid face of a two-way body adds to its wetted faces and fluid velocity (the fluid
+// cell's centre velocity after the projection) and, with the binary estimator, its face pressure force.
+__global__ __launch_bounds__(512) void k_coupling_faces(Geo g, const uint32_t *rows, const uint32_t *ctype,
+                                                        const double *pres, const float *vel, const slosh_collider *col,
+                                                        int32_t *coupling, float p2pa, int binary)
+{
+    __shared__ uint32_t s_row[28];
+    CellPos c;
+    uint32_t e;
+    if (!cell_of_block(g, rows, s_row, c, e)) return;
+    const uint32_t B = ctype[c.idx], tb = B & kCellTypeMask;
+    for (int a = 0; a < 3; ++a) {
+        int ca[3] = {c.x, c.y, c.z};
+        ca[a] -= 1;
+        const uint32_t A = type_at(g, c, ctype, ca[0], ca[1], ca[2]), ta = A & kCellTypeMask;
+        float fp[3] = {((float)c.x + 0.5f) * g.h, ((float)c.y + 0.5f) * g.h, ((float)c.z + 0.5f) * g.h};
+        fp[a] = (float)(a == 0 ? c.x : (a == 1 ? c.y : c.z)) * g.h;
+        float axis[3] = {0.0f, 0.0f, 0.0f}, naxis[3] = {0.0f, 0.0f, 0.0f};
+        axis[a] = 1.0f;
+        naxis[a] = -1.0f;
+        if (ta == kCellSolid && tb == kCellSolid) {
+            if (collider_of(A) != collider_of(B)) {
+                contact(g, ctype, pres, col, coupling, B, c.x, c.y, c.z, fp, axis, p2pa);
+                contact(g, ctype, pres, col, coupling, A, ca[0], ca[1], ca[2], fp, naxis, p2pa);
+            }
+        } else if (ta == kCellSolid || tb == kCellSolid) {
+            uint32_t id = 0u;
+            int fc[3];
+            const float *into = axis;
+            if (ta == kCellFluid && collider_of(B) != 0u) {
+                id = collider_of(B);
+                fc[0] = ca[0]; fc[1] = ca[1]; fc[2] = ca[2];
+            } else if (tb == kCellFluid && collider_of(A) != 0u) {
+                id = collider_of(A);
+                fc[0] = c.x; fc[1] = c.y; fc[2] = c.z;
+                into = naxis;
+            }
+            if (id == 0u || !(col[id - 1u].flags & 1u)) continue;
+            int32_t *acc = coupling + (id - 1u) * 16u;
+            const uint32_t fi = cell_index(g, c, fc[0], fc[1], fc[2]);
+            if (binary) pressure_force(g, col[id - 1u], acc, fp, into, (float)pres[fi], p2pa);
+            atomicAdd(&acc[6], 1);
+            for (int b = 0; b < 3; ++b) {
+                int up[3] = {fc[0], fc[1], fc[2]};
+                up[b] += 1;
+                const float v = 0.5f * (vel[(uint32_t)b * g.pool_cells + fi] + vel[face_index(g, c, b, up[0], up[1], up[2])]);
+                add_fixed(&acc[8 + b], v);
+            }
+        }
+    }
+}
+
+// Slosh.usf:1094-1111 (FluidPressureGradient): per metre, central where both neighbours are fluid, else one-sided.
+__device__ void fluid_pressure_gradient(const Geo &g, const CellPos &c, const uint32_t *ctype, const double *pres,
+                                        float pc, float gr[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        int p[3] = {c.x, c.y, c.z}, m[3] = {c.x, c.y, c.z};
+        p[a] += 1;
+        m[a] -= 1;
+        const bool bp = (type_at(g, c, ctype, p[0], p[1], p[2]) & kCellTypeMask) == kCellFluid;
+        const bool bm = (type_at(g, c, ctype, m[0], m[1], m[2]) & kCellTypeMask) == kCellFluid;
+        const float pp = bp ? (float)pres[cell_index(g, c, p[0], p[1], p[2])] : 0.0f;
+        const float pm = bm ? (float)pres[cell_index(g, c, m[0], m[1], m[2])] : 0.0f;
+        gr[a] = bp && bm ? (pp - pm) * 0.5f : (bp ? pp - pc : (bm ? pc - pm : 0.0f));
+    }
+    for (int a = 0; a < 3; ++a) gr[a] = gr[a] * g.inv_h;
+}
+
+// Slosh.usf:1115-1136 (BodySurfacePressure): from each fluid neighbour, extrapolated linearly to the shared face.
+__device__ bool body_surface_pressure(const Geo &g, const CellPos &c, const uint32_t *ctype, const double *pres,
+                                      float &p_out)
+{
+    float sum = 0.0f, cnt = 0.0f;
+    for (int nb = 0; nb < 6; ++nb) {
+        int d[3];
+        nb_offset(nb, d);
+        const int nx = c.x + d[0], ny = c.y + d[1], nz = c.z + d[2];
+        if ((type_at(g, c, ctype, nx, ny, nz) & kCellTypeMask) != kCellFluid) continue;
+        const float pn = (float)pres[cell_index(g, c, nx, ny, nz)];
+        const int fx = nx + d[0], fy = ny + d[1], fz = nz + d[2];
+        const float slope = (type_at(g, c, ctype, fx, fy, fz) & kCellTypeMask) == kCellFluid
+                                ? (float)pres[cell_index(g, c, fx, fy, fz)] - pn
+                                : 0.0f;
+        sum += pn - 0.5f * slope;
+        cnt += 1.0f;
+    }
+    p_out = cnt > 0.0f ? sum / cnt : 0.0f;
+    return cnt > 0.0f;
+}
+
+// Slosh.usf:1138-1216 (CouplingForceCS): the smoothed pressure force on two-way colliders, -p h^2 sum_faces w_f n_f with
+// w_f = saturate(0.5 + phi(x_f) / h), from fluid cells within a cell of the surface (their pressure moved to the
+// surface along the gradient) and the body's own surface cells; faces a body cell shares with another solid are
+// left to the contact rule.
+__global__ __launch_bounds__(512) void k_coupling_force(Geo g, const uint32_t *rows, const uint32_t *ctype,
+                                                        const double *pres, const slosh_collider *col, uint32_t ncol,
+                                                        int32_t *coupling, float p2pa)
+{
+    __shared__ uint32_t s_row[28];
+    CellPos c;
+    uint32_t e;
+    if (!cell_of_block(g, rows, s_row, c, e)) return;
+    const uint32_t packed = ctype[c.idx], type = packed & kCellTypeMask;
+    if (type == kCellAir) return;
+    const float X[3] = {((float)c.x + 0.5f) * g.h, ((float)c.y + 0.5f) * g.h, ((float)c.z + 0.5f) * g.h};
+    float ph = 0.0f, gr[3] = {0.0f, 0.0f, 0.0f};
+    bool have = false;
+    if (type == kCellFluid) {
+        ph = (float)pres[c.idx];
+        fluid_pressure_gradient(g, c, ctype, pres, ph, gr);
+        have = true;
+    } else if (collider_of(packed) != 0u) {
+        have = body_surface_pressure(g, c, ctype, pres, ph);
+    }
+    if (!have) return;
+    for (uint32_t k = 0; k < ncol; ++k) {
+        const slosh_collider cc = col[k];
+        if (!(cc.flags & 1u) || fabsf(collider_sdf(cc, X)) > g.h) continue;
+        if (type == kCellSolid && collider_of(packed) != k + 1u) continue;
+        float sp = ph;
+        if (type == kCellFluid) {
+            const float phi = collider_sdf(cc, X);
+            float n[3];
+            collider_normal(cc, X, 0.25f * g.h, n);
+            sp += gr[0] * (-phi * n[0]) + gr[1] * (-phi * n[1]) + gr[2] * (-phi * n[2]);
+        }
+        const float scale = -sp * p2pa * g.h * g.h;
+        float F[3] = {0.0f, 0.0f, 0.0f}, T[3] = {0.0f, 0.0f, 0.0f};
+        for (int nb = 0; nb < 6; ++nb) {
+            float nrm[3] = {0.0f, 0.0f, 0.0f};
+            nrm[nb >> 1] = (nb & 1) ? 1.0f : -1.0f;
+            float fpos[3];
+            for (int a = 0; a < 3; ++a) fpos[a] = X[a] + 0.5f * g.h * nrm[a];
+            if (type == kCellSolid) {
+                int d[3];
+                nb_offset(nb, d);
+                const uint32_t nbp = type_at(g, c, ctype, c.x + d[0], c.y + d[1], c.z + d[2]);
+                if ((nbp & kCellTypeMask) == kCellSolid && collider_of(nbp) != k + 1u) continue;
+            }
+            const float w = clampf(0.5f + collider_sdf(cc, fpos) * g.inv_h, 0.0f, 1.0f);
+            float df[3], r[3], t[3];
+            for (int a = 0; a < 3; ++a) {
+                df[a] = scale * w * nrm[a];
+                r[a] = fpos[a] - cc.center[a];
+            }
+            cross3(r, df, t);
+            for (int a = 0; a < 3; ++a) {
+                F[a] += df[a];
+                T[a] += t[a];
+            }
+        }
+        int32_t *acc = coupling + k * 16u;
+        for (int a = 0; a < 3; ++a) add_fixed(&acc[a], F[a]);
+        for (int a = 0; a < 3; ++a) add_fixed(&acc[3 + a], T[a]);
+    }
+}
+
+// Slosh.usf:1316-1370 (SubmergedCS): a solid cell of a collider counts as submerged when a walk of up to 12 cells along
+// +x, -x, +y, -y out of the collider first meets dense fluid with no air in the three cells below (-z, the
+// reference's down axis).
+__global__ __launch_bounds__(512) void k_submerged(Geo g, const uint32_t *rows, const uint32_t *ctype,
+                                                   const slosh_collider *col, int32_t *coupling)
+{
+    __shared__ uint32_t s_row[28];
+    CellPos c;
+    uint32_t e;
+    if (!cell_of_block(g, rows, s_row, c, e)) return;
+    const uint32_t packed = ctype[c.idx], id = collider_of(packed);
+    if ((packed & kCellTypeMask) != kCellSolid || id == 0u) return;
+    int32_t *acc = coupling + (id - 1u) * 16u;
+    atomicAdd(&acc[11], 1);
+    const int dirs[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+    bool sub = false;
+    for (int d = 0; d < 4; ++d) {
+        int px = c.x, py = c.y;
+        for (int s = 0; s < 12; ++s) {
+            px += dirs[d][0];
+            py += dirs[d][1];
+            const uint32_t t = type_global(g, ctype, px, py, c.z);
+            if ((t & kCellTypeMask) != kCellSolid || collider_of(t) != id) {
+                bool liquid = (t & kCellTypeMask) == kCellFluid && (t & kCellDense) != 0u;
+                for (int below = 1; below <= 3; ++below)
+                    liquid = liquid && (type_global(g, ctype, px, py, c.z - below) & kCellTypeMask) != kCellAir;
+                sub = sub || liquid;
+                break;
+            }
+        }
+    }
+    if (!sub) return;
+    atomicAdd(&acc[7], 1);
+    const slosh_collider cc = col[id - 1u];
+    const float off[3] = {((float)c.x + 0.5f) * g.h - cc.center[0], ((float)c.y + 0.5f) * g.h - cc.center[1],
+                          ((float)c.z + 0.5f) * g.h - cc.center[2]};
+    for (int a = 0; a < 3; ++a) add_fixed(&acc[12 + a], off[a]);
+}
+
 // ---- two-sided density projection (Slosh.usf:743-854) --------------------------------------------------------
 // Slosh.usf:753-773 (CellDensityInterior): the P2G weight of the cell's faces that no solid shares, per rest
 // particle count of those faces; 1 when a solid shares every face.
@@ -887,6 +1162,29 @@ void launch_project(slosh_ctx *c)
                      c->ctype, c->bp_div_max, c->bp_div_cnt, c->bp_div_sum);
     }
     c->n_div_blocks = c->nactive;
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_coupling(slosh_ctx *c, float dt)
+{
+    if (!c->nactive || !c->ncol) return;
+    // SloshProxy.cpp:562: PressureToPascal = density h / dt
+    const float rho = c->s.fluid_density > 0.0f ? c->s.fluid_density : 1000.0f;
+    const float p2pa = rho * c->geo.h / dt;
+    const int binary = (c->s.flags & SLOSH_FLAG_BINARY_COUPLING) != 0;
+    launch_timed(c, K_COUPLING, k_coupling_faces, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo, c->ctype, c->pres,
+                 c->vel[c->vcur], c->d_col, c->coupling, p2pa, binary);
+    if (!binary)
+        launch_timed(c, K_COUPLING, k_coupling_force, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo, c->ctype,
+                     c->pres, c->d_col, c->ncol, c->coupling, p2pa);
+    SLOSH_HIP(hipGetLastError());
+}
+
+void launch_submerged(slosh_ctx *c)
+{
+    if (!c->nactive || !c->ncol) return;
+    launch_timed(c, K_COUPLING, k_submerged, dim3(c->nactive), dim3(512), 0, c->geo, c->binfo, c->ctype, c->d_col,
+                 c->coupling);
     SLOSH_HIP(hipGetLastError());
 }
 

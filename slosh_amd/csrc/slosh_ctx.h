@@ -175,7 +175,7 @@ enum KernelId {
     K_RESTRICT0, K_PROLONG_BLACK, K_DIRECTION, K_UPDATE_BLACK, K_PDOWN1, K_PDOWN2, K_PUP1, K_PUP2, K_D3DOWN, K_D3UP,
     K_DSMOOTH, K_DRESTRICT, K_DPROLONG, K_COARSE_SOLVE, K_COARSE_TAIL, K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL,
     K_FINALIZE, K_STEP_STATS, K_UNET_CONV0, K_UNET_CONV1, K_UNET_CONV2, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_G2P_MOVE,
-    K_DENSITY, K_VISCOSITY, K_COUNT
+    K_DENSITY, K_VISCOSITY, K_COUPLING, K_COUNT
 };
 // Names as the compiled kernels are called (what a kernel trace of the library shows).
 extern const char *const kKernelNames[K_COUNT];
@@ -335,6 +335,8 @@ struct slosh_ctx {
     uint32_t ncol = 0;
     slosh_manipulator *d_man = nullptr;
     uint32_t nman = 0;
+    int32_t *coupling = nullptr;      // [SLOSH_MAX_COLLIDERS][16] fixed-point coupling sums of the last step
+    uint32_t coupling_ncol = 0;       // colliders the last step's sums cover
     uint64_t drained_total = 0;
     // host pinned mirrors
     slosh::StepStats *h_stats = nullptr;
@@ -383,6 +385,8 @@ void launch_step_stats(slosh_ctx *c);
 void launch_density_rhs(slosh_ctx *c, float relax, int keep_q);
 // Slosh.usf:657-686 / SloshProxy.cpp:603-617: `iterations` Jacobi sweeps of the implicit viscosity on the
 // normalised velocities (alpha = nu dt / h^2); the result becomes the current velocity buffer.
+void launch_coupling(slosh_ctx *c, float dt);
+void launch_submerged(slosh_ctx *c);
 void run_viscosity(slosh_ctx *c, float alpha, int iterations);
 void launch_density_disp(slosh_ctx *c);
 void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n);
