@@ -530,15 +530,15 @@ __device__ __forceinline__ uint32_t index_global(const Geo &g, int x, int y, int
 
 __device__ __forceinline__ void add_fixed(int32_t *p, float v) { atomicAdd(p, (int32_t)rintf(v * kCouplingScale)); }
 
-// Slosh.usf:913-926 (AccumulatePressureForce): force p_hat * PressureToPascal * h^2 along `into`, torque about the
-// collider's centre.
-__device__ void pressure_force(const Geo &g, const slosh_collider &col, int32_t *acc, const float fp[3],
-                               const float into[3], float p_hat, float p2pa)
+// Slosh.usf:913-926 (AccumulatePressureForce): force p_hat * PressureToPascal * h^2 along the direction into the
+// solid (`sign` times axis `axis`), torque about the collider's centre.
+__device__ __forceinline__ void pressure_force(const Geo &g, const slosh_collider &col, int32_t *acc, const float fp[3],
+                                               int axis, float sign, float p_hat, float p2pa)
 {
     const float pa = p_hat * p2pa;
     float f[3], r[3], t[3];
     for (int a = 0; a < 3; ++a) {
-        f[a] = pa * g.h * g.h * into[a];
+        f[a] = pa * g.h * g.h * (a == axis ? sign : 0.0f);
         r[a] = fp[a] - col.center[a];
     }
     cross3(r, f, t);
@@ -548,16 +548,16 @@ __device__ void pressure_force(const Geo &g, const slosh_collider &col, int32_t 
 
 // Slosh.usf:951-983 (EstimateSeepagePressure): the mean pressure of the liquid met first by walks of up to 16 cells
 // along +x, -x, +y, -y out of collider `id`'s solid region (the reference's horizontal axes); false when dry.
-__device__ bool seepage_pressure(const Geo &g, const uint32_t *ctype, const double *pres, int x, int y, int z,
+__device__ __forceinline__ bool seepage_pressure(const Geo &g, const uint32_t *ctype, const double *pres, int x, int y, int z,
                                  uint32_t id, float &p_out)
 {
-    const int dirs[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+    // directions +x, -x, +y, -y
     float sum = 0.0f, cnt = 0.0f;
     for (int d = 0; d < 4; ++d) {
         int px = x, py = y;
         for (int s = 0; s < 16; ++s) {
-            px += dirs[d][0];
-            py += dirs[d][1];
+            px += d == 0 ? 1 : (d == 1 ? -1 : 0);
+            py += d == 2 ? 1 : (d == 3 ? -1 : 0);
             const uint32_t t = type_global(g, ctype, px, py, z);
             if ((t & kCellTypeMask) != kCellSolid || collider_of(t) != id) {
                 if ((t & kCellTypeMask) == kCellFluid) {
@@ -573,15 +573,15 @@ __device__ bool seepage_pressure(const Geo &g, const uint32_t *ctype, const doub
 }
 
 // Slosh.usf:985-998 (AccumulateContact): the seepage pressure on a contact face of a two-way body with another solid.
-__device__ void contact(const Geo &g, const uint32_t *ctype, const double *pres, const slosh_collider *col,
-                        int32_t *coupling, uint32_t packed, int x, int y, int z, const float fp[3], const float into[3],
+__device__ __forceinline__ void contact(const Geo &g, const uint32_t *ctype, const double *pres, const slosh_collider *col,
+                        int32_t *coupling, uint32_t packed, int x, int y, int z, const float fp[3], int axis, float sign,
                         float p2pa)
 {
     const uint32_t id = collider_of(packed);
     if (id == 0u || !(col[id - 1u].flags & 1u)) return;
     float p;
     if (seepage_pressure(g, ctype, pres, x, y, z, id, p))
-        pressure_force(g, col[id - 1u], coupling + (id - 1u) * 16u, fp, into, p, p2pa);
+        pressure_force(g, col[id - 1u], coupling + (id - 1u) * 16u, fp, axis, sign, p, p2pa);
 }
 
 // ProjectFace's coupling terms (Slosh.usf:1003-1028) after the projection: contact faces between different solids
@@ -596,37 +596,35 @@ __global__ __launch_bounds__(512) void k_coupling_faces(Geo g, const uint32_t *r
     uint32_t e;
     if (!cell_of_block(g, rows, s_row, c, e)) return;
     const uint32_t B = ctype[c.idx], tb = B & kCellTypeMask;
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
         int ca[3] = {c.x, c.y, c.z};
         ca[a] -= 1;
         const uint32_t A = type_at(g, c, ctype, ca[0], ca[1], ca[2]), ta = A & kCellTypeMask;
         float fp[3] = {((float)c.x + 0.5f) * g.h, ((float)c.y + 0.5f) * g.h, ((float)c.z + 0.5f) * g.h};
         fp[a] = (float)(a == 0 ? c.x : (a == 1 ? c.y : c.z)) * g.h;
-        float axis[3] = {0.0f, 0.0f, 0.0f}, naxis[3] = {0.0f, 0.0f, 0.0f};
-        axis[a] = 1.0f;
-        naxis[a] = -1.0f;
         if (ta == kCellSolid && tb == kCellSolid) {
             if (collider_of(A) != collider_of(B)) {
-                contact(g, ctype, pres, col, coupling, B, c.x, c.y, c.z, fp, axis, p2pa);
-                contact(g, ctype, pres, col, coupling, A, ca[0], ca[1], ca[2], fp, naxis, p2pa);
+                contact(g, ctype, pres, col, coupling, B, c.x, c.y, c.z, fp, a, 1.0f, p2pa);
+                contact(g, ctype, pres, col, coupling, A, ca[0], ca[1], ca[2], fp, a, -1.0f, p2pa);
             }
         } else if (ta == kCellSolid || tb == kCellSolid) {
             uint32_t id = 0u;
-            int fc[3];
-            const float *into = axis;
+            int fc[3] = {c.x, c.y, c.z};
+            bool body_above = true; // the body is cell B, on the face's +axis side
             if (ta == kCellFluid && collider_of(B) != 0u) {
                 id = collider_of(B);
-                fc[0] = ca[0]; fc[1] = ca[1]; fc[2] = ca[2];
+                fc[a] -= 1;
             } else if (tb == kCellFluid && collider_of(A) != 0u) {
                 id = collider_of(A);
-                fc[0] = c.x; fc[1] = c.y; fc[2] = c.z;
-                into = naxis;
+                body_above = false;
             }
             if (id == 0u || !(col[id - 1u].flags & 1u)) continue;
             int32_t *acc = coupling + (id - 1u) * 16u;
             const uint32_t fi = cell_index(g, c, fc[0], fc[1], fc[2]);
-            if (binary) pressure_force(g, col[id - 1u], acc, fp, into, (float)pres[fi], p2pa);
+            if (binary) pressure_force(g, col[id - 1u], acc, fp, a, body_above ? 1.0f : -1.0f, (float)pres[fi], p2pa);
             atomicAdd(&acc[6], 1);
+#pragma unroll
             for (int b = 0; b < 3; ++b) {
                 int up[3] = {fc[0], fc[1], fc[2]};
                 up[b] += 1;
@@ -757,13 +755,13 @@ __global__ __launch_bounds__(512) void k_submerged(Geo g, const uint32_t *rows, 
     if ((packed & kCellTypeMask) != kCellSolid || id == 0u) return;
     int32_t *acc = coupling + (id - 1u) * 16u;
     atomicAdd(&acc[11], 1);
-    const int dirs[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+    // directions +x, -x, +y, -y
     bool sub = false;
     for (int d = 0; d < 4; ++d) {
         int px = c.x, py = c.y;
         for (int s = 0; s < 12; ++s) {
-            px += dirs[d][0];
-            py += dirs[d][1];
+            px += d == 0 ? 1 : (d == 1 ? -1 : 0);
+            py += d == 2 ? 1 : (d == 3 ? -1 : 0);
             const uint32_t t = type_global(g, ctype, px, py, c.z);
             if ((t & kCellTypeMask) != kCellSolid || collider_of(t) != id) {
                 bool liquid = (t & kCellTypeMask) == kCellFluid && (t & kCellDense) != 0u;

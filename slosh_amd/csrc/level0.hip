@@ -405,12 +405,14 @@ __global__ __launch_bounds__(kThreads) void k_sor_sweep(const uint32_t *binfo, c
     pres[i] = old + w * (gs - old);
 }
 
-// Partials ||b||^2 and ||b - A p||^2 (the SOR convergence check; with `always` also after the solver is done).
+// Partials ||b||^2 and ||b - A p||^2: mode 0 the SOR convergence check (skipped once the solver is done), 1 always,
+// 2 only once the solver is done (the true-residual check of a finished MGPCG solve).
 __global__ __launch_bounds__(kThreads) void k_sor_residual(const uint32_t *binfo, const uint8_t *fmask,
                                                            const float *div, const float *diag, const double *pres,
-                                                           double *partials, const SolverState *st, int always)
+                                                           double *partials, const SolverState *st, int mode)
 {
-    if (!always && st->done) return;
+    const int32_t done = st->done;
+    if ((mode == 0 && done) || (mode == 2 && !done)) return;
     const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
     double v[2] = {0.0, 0.0};
     for (uint32_t c = 0; c < 2u; ++c) {
@@ -614,6 +616,11 @@ void l0_error_partials(slosh_ctx *c, const double *p_ref)
 void l0_true_residual(slosh_ctx *c)
 {
     L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 1);
+}
+
+void l0_true_residual_if_done(slosh_ctx *c)
+{
+    L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 2);
 }
 
 } // namespace slosh

@@ -728,7 +728,8 @@ __device__ __forceinline__ void sample_displacement(const Geo &g, const float *d
     for (int a = 0; a < 3; ++a) out[a] = (len > 0.5f ? dv[a] * s : dv[a]) * g.h;
 }
 
-template <bool kMove>
+// kExtras: the density projection's position shift, force fields and drains (compiled out of the common step)
+template <bool kMove, bool kExtras>
 __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off, const float *vel,
                                          const uint8_t *fflags, float dt, float gx, float gy, float gz, float friction,
                                          float radius, float max_speed, const slosh_collider *col, uint32_t ncol,
@@ -844,7 +845,7 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
                 }
                 float x[3];
                 for (int a = 0; a < 3; ++a) x[a] = pos[a] + dt * v2[a];
-                if (disp) {
+                if (kExtras && disp) {
                     // position-only shift of the density projection, sampled where the density was measured
                     float pc[3], dx[3];
                     for (int a = 0; a < 3; ++a) pc[a] = clampf(pos[a], lo, hi[a]);
@@ -861,7 +862,7 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
                     }
                     for (int k = 0; k < 9; ++k) cm[k] = 0.0f;
                 }
-                if (nman) apply_force_fields(man, nman, x, nv, dt);
+                if (kExtras && nman) apply_force_fields(man, nman, x, nv, dt);
                 for (uint32_t k = 0; k < ncol; ++k) {
                     const slosh_collider cc = col[k];
                     float dd = collider_sdf(cc, x);
@@ -900,7 +901,7 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
                     for (int a = 0; a < 3; ++a)
                         if ((contact & (1u << a)) == 0u) nv[a] *= 1.0f - friction;
                 float speed = len3(nv[0], nv[1], nv[2]);
-                if (nman && drained_at(man, nman, x, dt, i, frame_seed)) {
+                if (kExtras && nman && drained_at(man, nman, x, dt, i, frame_seed)) {
                     // removed: the sort drops the record (its key is the dead key), its brick is not touched (the
                     // lane keeps brick = kNoBrick and still takes part in the wavefront's touch marking below)
                     if (!kMove) {
@@ -951,6 +952,7 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
     }
 }
 
+template <bool kExtras>
 __global__ __launch_bounds__(256) void k_g2p_keys(Geo g, Parts p, const uint32_t *key_off, const float *vel,
                                                   const uint8_t *fflags, float dt, float gx, float gy, float gz,
                                                   float friction, float radius, float max_speed,
@@ -959,10 +961,12 @@ __global__ __launch_bounds__(256) void k_g2p_keys(Geo g, Parts p, const uint32_t
                                                   uint32_t dead_key, const float *disp, const slosh_manipulator *man,
                                                   uint32_t nman, uint32_t frame_seed, uint32_t *bp_drain)
 {
-    g2p_body<false>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, touch, bp_speed,
+    g2p_body<false, kExtras>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, touch,
+                             bp_speed,
                     bp_clamp, keys, dead_key, G2PMove{}, disp, man, nman, frame_seed, bp_drain);
 }
 
+template <bool kExtras>
 __global__ __launch_bounds__(256) void k_g2p_move(Geo g, Parts p, const uint32_t *key_off, const float *vel,
                                                   const uint8_t *fflags, float dt, float gx, float gy, float gz,
                                                   float friction, float radius, float max_speed,
@@ -970,7 +974,8 @@ __global__ __launch_bounds__(256) void k_g2p_move(Geo g, Parts p, const uint32_t
                                                   const float *disp, const slosh_manipulator *man, uint32_t nman,
                                                   uint32_t frame_seed)
 {
-    g2p_body<true>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, nullptr, nullptr,
+    g2p_body<true, kExtras>(g, p, key_off, vel, fflags, dt, gx, gy, gz, friction, radius, max_speed, col, ncol, nullptr,
+                            nullptr,
                    nullptr, keys, 0u, mv, disp, man, nman, frame_seed, nullptr);
 }
 
@@ -1062,7 +1067,8 @@ void launch_g2p(slosh_ctx *c, float dt, const float g[3])
     if (!c->np) return;
     const Geo &geo = c->geo;
     const float radius = c->s.particle_radius_cells * geo.h;
-    launch_timed(c, K_G2P, k_g2p_keys, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
+    const bool extras = c->dens_applied || c->nman > 0u;
+    launch_timed(c, K_G2P, extras ? k_g2p_keys<true> : k_g2p_keys<false>, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
                  c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction, radius, c->s.max_speed,
                  c->d_col, c->ncol, c->touch, c->bp_speed, c->bp_clamp, c->g2p_keys, c->nkeys - 1u,
                  c->dens_applied ? (const float *)c->dens_disp : nullptr, c->d_man, c->nman,
@@ -1072,7 +1078,7 @@ void launch_g2p(slosh_ctx *c, float dt, const float g[3])
     // the new offsets go to the spare buffer: the second pass walks the current array by its own offsets
     sort_keys(c, c->g2p_keys);
     G2PMove mv{c->parts[1 - c->cur], c->packed, c->run_base, &c->stats->misplaced};
-    launch_timed(c, K_G2P_MOVE, k_g2p_move, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
+    launch_timed(c, K_G2P_MOVE, extras ? k_g2p_move<true> : k_g2p_move<false>, dim3(geo.nbricks), dim3(256), 0, geo, c->parts[c->cur], c->key_off,
                  c->vel[c->vcur], c->fflags, dt, g[0], g[1], g[2], c->s.boundary_friction, radius, c->s.max_speed,
                  c->d_col, c->ncol, c->g2p_keys, mv, c->dens_applied ? (const float *)c->dens_disp : nullptr,
                  c->d_man, c->nman, (uint32_t)(c->frame * 16u));

@@ -434,6 +434,7 @@ void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w);
 // Partials ||b||^2 and ||b - A p||^2 of the FP64 pressure (the true residual).
 void l0_sor_residual(slosh_ctx *c);
 // The same partials whatever the solver state (the true residual of a finished solve).
+void l0_true_residual_if_done(slosh_ctx *c);
 void l0_true_residual(slosh_ctx *c);
 void l0_fcg_zq(slosh_ctx *c, const float *z);
 void l0_fcg_direction(slosh_ctx *c, const float *z, const float *d_in, float *d_out, const float *r, int first);

@@ -51,13 +51,13 @@ template <typename V> __device__ __forceinline__ float dsum(const DenseLevel &L,
 }
 
 enum FinKind { kFinInit = 0, kFinRho = 1, kFinAlphaStop = 2, kFinRr = 3, kFinSor = 4, kFinBetaF = 5,
-               kFinAlphaF = 6, kFinCheck = 7, kFinRestart = 8, kFinErr = 9 };
+               kFinAlphaF = 6, kFinCheck = 7, kFinRestart = 8, kFinErr = 9, kFinCheckDone = 10 };
 
 __host__ __device__ constexpr int fin_values(int kind)
 {
     if (kind == kFinAlphaStop) return 4;
     if (kind == kFinInit || kind == kFinRestart) return 3;
-    if (kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck || kind == kFinErr) return 2;
+    if (kind == kFinSor || kind == kFinAlphaF || kind == kFinCheck || kind == kFinErr || kind == kFinCheckDone) return 2;
     return 1;
 }
 
@@ -84,7 +84,10 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         for (int j = 0; j < kMaxV; ++j) v[k][j] = (b < nblocks && j < nv) ? partials[(size_t)b * nv + j] : 0.0;
     }
     issue_point();
-    if (kind != kFinInit && kind != kFinCheck && kind != kFinRestart && kind != kFinErr && done) {
+    // kFinCheckDone: the true-residual check enqueued with every read of the state; it applies once the solve is done
+    if (kind == kFinCheckDone && !done) return;
+    if (kind != kFinInit && kind != kFinCheck && kind != kFinCheckDone && kind != kFinRestart && kind != kFinErr &&
+        done) {
         // the update that met the stopping test has run: from here on every pass returns at once
         if (kind == kFinAlphaStop && done == 2u && t == 0) st->done = 1;
         return;
@@ -179,7 +182,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
     } else if (kind == kFinErr) {
         // t0 = sum (p_ref - p)^2, t1 = sum p^2 with p the FP64-accurate solution
         st->err_rel = t1 > 0.0 ? sqrt(t0 / t1) : 0.0;
-    } else if (kind == kFinCheck) {
+    } else if (kind == kFinCheck || kind == kFinCheckDone) {
         // the FP64 true residual ||b - A p|| / ||b|| of the solution: the solve counts as converged only when it
         // meets the tolerance (the recurrence residual that stopped the iteration can differ from it)
         const double rel_true = t0 > 0.0 ? sqrt(t1) / sqrt(t0) : 0.0;
@@ -194,6 +197,8 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         st->rel = rel;
         const int it = s0.iters + 1;
         st->iters = it;
+        // hist is read through the pointer: an indexed copy in the thread's own state would put the whole state
+        // copy of every thread in scratch memory
         const int k = (it - 1) & 3;
         if (!(rel <= 1e300)) {
             st->stalled = 1;
@@ -201,7 +206,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         } else if (rel <= s0.tol) {
             st->converged = 1;
             st->done = 1;
-        } else if (it >= s0.maxit || (it - 1 >= 4 && rel > 0.5 * s0.hist[k])) {
+        } else if (it >= s0.maxit || (it - 1 >= 4 && rel > 0.5 * st->hist[k])) {
             st->stalled = 1;
             st->done = 1;
         }
@@ -1342,15 +1347,22 @@ void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
                 SLOSH_HIP(hipGetLastError());
             }
             ahead = 1;
+            // the true residual of the FP64 pressure decides convergence and is reported with the results; its
+            // check rides along with every read of the state (both passes return at once while the solve
+            // iterates), so a finished solve costs no second host round trip
+            l0_true_residual_if_done(c);
+            finalize(c, kFinCheckDone);
             SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
             SLOSH_HIP(hipStreamSynchronize(c->stream));
             if (c->h_state->done) break;
         }
-        // the true residual of the FP64 pressure decides convergence (kFinCheck) and is reported with the results
-        l0_true_residual(c);
-        finalize(c, kFinCheck);
-        SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
-        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        if (!c->h_state->done) {
+            // out of enqueued iterations without the device's stop (not expected: the limit stops it first)
+            l0_true_residual(c);
+            finalize(c, kFinCheck);
+            SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+            SLOSH_HIP(hipStreamSynchronize(c->stream));
+        }
         if (c->h_state->converged || c->h_state->iters >= maxit || restart >= max_restarts) break;
         l0_pcg_init(c, R[0]);
         finalize(c, kFinRestart);
