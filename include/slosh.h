@@ -267,6 +267,22 @@ slosh_status slosh_set_manipulators(slosh_ctx *ctx, const slosh_manipulator *m, 
  * sort -> brick pool -> classify -> P2G -> normalise -> divergence -> pressure -> project -> extrapolate -> G2P. */
 slosh_status slosh_step(slosh_ctx *ctx, float dt, const float gravity[3]);
 
+/* ---- pressure systems of their own -------------------------------------------------------------------------- */
+/* No counterpart in the reference. For the training-set generator (64^3 windows of seeded scenes) and for callers
+ * that solve pressure systems of their own with the library's solvers. Loads the system
+ *     (diag_c + 1e-4) p_c - sum_{fluid neighbours n} p_n = -divergence_c   over the fluid cells c
+ * given in dense order (x fastest) over the context's grid: types 0 air, 1 fluid, 2 solid; diag (>= 0: one per
+ * fluid neighbour plus the ghost-fluid weights of air neighbours) and the finite divergence are read at fluid cells
+ * only (SLOSH_EINVAL otherwise). Cells outside the grid count as solid, so a window cut from a larger
+ * grid keeps the diagonal of its original neighbourhood: its matrix is a principal submatrix of the original one.
+ * Drops the particles and the grid state (as slosh_reset), allocates the bricks holding fluid with their
+ * neighbourhood (SLOSH_ECAPACITY, loading nothing, when the pool is too small) and sets the pressure to 0. */
+slosh_status slosh_load_system(slosh_ctx *ctx, const uint8_t *types, const float *diag, const float *divergence);
+/* No counterpart in the reference. Solves the current system (the last step's, or a loaded one) from the current
+ * pressure with the configured solver, as the step's pressure stage does (capture flags included); the solver's
+ * figures go to slosh_results, the pressure to SLOSH_FIELD_PRESSURE / SLOSH_FIELD_PRESSURE_F64. */
+slosh_status slosh_solve(slosh_ctx *ctx);
+
 /* ---- readback ------------------------------------------------------------------------------------------------ */
 /* Copies the live particles (sorted order of the last step) into dst; *count receives the number written. */
 slosh_status slosh_read_particles(slosh_ctx *ctx, void *dst, uint64_t dst_bytes, uint32_t *count, int32_t layout);

@@ -62,6 +62,10 @@ def load_library(path=LIB_PATH):
     lib.slosh_read_grid.argtypes = [vp, C.c_int32, vp, C.c_uint64, C.c_int32]
     lib.slosh_read_grid.restype = st
     lib.slosh_get_results.argtypes = [vp, C.POINTER(abi.Results)]
+    lib.slosh_load_system.restype = st
+    lib.slosh_load_system.argtypes = [vp, vp, vp, vp]
+    lib.slosh_solve.restype = st
+    lib.slosh_solve.argtypes = [vp]
     lib.slosh_get_collider_feedback.restype = st
     lib.slosh_get_collider_feedback.argtypes = [vp, C.POINTER(abi.ColliderFeedback), C.c_uint32,
                                                 C.POINTER(C.c_uint32)]
@@ -179,6 +183,20 @@ class Sim:
         r = abi.Results()
         _check(self.lib.slosh_get_results(self.h, C.byref(r)), "slosh_get_results")
         return r
+
+    def load_system(self, types, diag, divergence):
+        """Loads a pressure system in dense (z, y, x) arrays over the grid: types 0 air / 1 fluid / 2 solid, the
+        operator diagonal and the divergence (slosh_load_system)."""
+        shape = tuple(int(self.settings.grid_dim[a]) for a in (2, 1, 0))
+        t = np.ascontiguousarray(types, dtype=np.uint8).reshape(shape)
+        d = np.ascontiguousarray(diag, dtype=np.float32).reshape(shape)
+        v = np.ascontiguousarray(divergence, dtype=np.float32).reshape(shape)
+        _check(self.lib.slosh_load_system(self.h, t.ctypes.data, d.ctypes.data, v.ctypes.data), "slosh_load_system")
+
+    def solve(self):
+        """Solves the current system with the configured solver (slosh_solve); returns the results."""
+        _check(self.lib.slosh_solve(self.h), "slosh_solve")
+        return self.results()
 
     def collider_feedback(self):
         """Per-collider feedback of the last step (slosh_get_collider_feedback), a list of abi.ColliderFeedback."""

@@ -381,6 +381,122 @@ void read_stats(slosh_ctx *c)
     harvest_timing(c);
 }
 
+template <typename T> std::vector<T> fetch(const slosh_ctx *c, const T *d, size_t n)
+{
+    std::vector<T> h(n);
+    SLOSH_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    SLOSH_HIP(hipStreamSynchronize(c->stream));
+    return h;
+}
+
+// The pressure solve of the current system with the configured solver (SloshProxy.cpp:618-641); the solver's final
+// state is copied back asynchronously. Returns whether a solver ran.
+bool run_solver(slosh_ctx *c)
+{
+    SLOSH_HIP(hipEventRecord(c->ev_solve0, c->stream));
+    c->fcg_iters = 0;
+    c->fcg_fallback = 0;
+    if (c->s.pressure_solver == SLOSH_SOLVER_MGPCG) run_mgpcg(c);
+    else if (c->s.pressure_solver == SLOSH_SOLVER_RBSOR) run_rbsor(c);
+    else if (c->s.pressure_solver == SLOSH_SOLVER_FCG_UNET) run_fcg(c);
+    SLOSH_HIP(hipEventRecord(c->ev_solve1, c->stream));
+    c->res.fallback_used = c->fcg_fallback;
+    // the solver's final state comes back with the step's statistics (no host wait between solve and G2P)
+    const bool solved = c->s.pressure_solver != SLOSH_SOLVER_NONE;
+    if (solved) SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+    return solved;
+}
+
+// The solver's figures in the results, once the stream has passed run_solver's copy of the state.
+void solver_results(slosh_ctx *c)
+{
+    if (c->nactive) {
+        // flexible CG: its own iterations, plus those of the MGPCG fallback when it ran
+        const bool fcg = c->s.pressure_solver == SLOSH_SOLVER_FCG_UNET && c->fcg_fallback != 2;
+        c->res.solver_iterations = fcg ? (c->fcg_fallback ? c->fcg_iters + c->h_state->iters : c->fcg_iters)
+                                       : c->h_state->iters;
+        c->res.residual_rel = c->h_state->rel;
+        c->res.true_residual_rel = c->s.pressure_solver == SLOSH_SOLVER_RBSOR ? (float)c->h_state->rel
+                                                                              : (float)c->h_state->rel_true;
+        c->res.rhs_norm = sqrt(c->h_state->bb);
+        c->res.converged = c->h_state->converged;
+        if ((c->s.flags & SLOSH_FLAG_REPORT_FP64_ERROR) && c->s.pressure_solver != SLOSH_SOLVER_RBSOR)
+            c->res.fp64_error_rel = c->h_state->err_rel;
+    }
+    float ms = 0.0f;
+    SLOSH_HIP(hipEventElapsedTime(&ms, c->ev_solve0, c->ev_solve1));
+    c->res.solve_ms = ms;
+}
+
+// Puts a pressure system given per cell in dense order into the context as P2G and the divergence pass would leave
+// it: the bricks holding fluid are touched and the pool allocates them with their neighbourhood, then cell types,
+// the operator diagonal, the divergence, the fluid-neighbour masks and the per-brick fluid counts are written.
+// Returns false, with the context reset, when the pool cannot hold the bricks.
+bool load_system_impl(slosh_ctx *c, const uint8_t *types, const float *diag, const float *div)
+{
+    reset_impl(c);
+    const Geo &g = c->geo;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    auto dense = [&](int x, int y, int z) { return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z); };
+    auto fluid_at = [&](int x, int y, int z) {
+        return x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz && types[dense(x, y, z)] == kCellFluid;
+    };
+    std::vector<uint8_t> touch(g.nbricks, 0u);
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x)
+                if (types[dense(x, y, z)] == kCellFluid)
+                    touch[(size_t)((x >> 3) + g.bd[0] * ((y >> 3) + g.bd[1] * (z >> 3)))] = 1u;
+    SLOSH_HIP(hipMemcpyAsync(c->touch, touch.data(), touch.size(), hipMemcpyHostToDevice, c->stream));
+    run_pool(c);
+    read_stats(c);
+    if (c->h_stats->overflow) {
+        reset_impl(c);
+        return false;
+    }
+    c->nactive = c->h_stats->nactive;
+    c->res.active_bricks = c->nactive;
+    c->res.pool_overflow_bricks = 0;
+    build_brick_rows(c, 0u, c->nactive);
+    const std::vector<uint32_t> table = fetch(c, c->table, g.nbricks);
+    const std::vector<uint32_t> active = fetch(c, c->active, c->nactive);
+    std::vector<uint32_t> ctype(g.pool_cells, 0u), bp_fluid(c->nactive, 0u);
+    std::vector<float> pdiag(g.pool_cells, 0.0f), pdiv(g.pool_cells, 0.0f);
+    std::vector<uint8_t> fmask(g.pool_cells, 0u);
+    for (uint32_t e = 0; e < c->nactive; ++e) {
+        const uint32_t b = active[e], slot = table[b];
+        const int bx = (int)(b % (uint32_t)g.bd[0]), by = (int)(b / (uint32_t)g.bd[0] % (uint32_t)g.bd[1]),
+                  bz = (int)(b / (uint32_t)(g.bd[0] * g.bd[1]));
+        for (uint32_t l = 0; l < kBrickCells; ++l) {
+            const int x = bx * 8 + (int)(l & 7u), y = by * 8 + (int)((l >> 3) & 7u), z = bz * 8 + (int)(l >> 6);
+            if (x >= nx || y >= ny || z >= nz) continue;
+            const size_t i = (size_t)slot * kBrickCells + l, d = dense(x, y, z);
+            ctype[i] = types[d];
+            if (types[d] != kCellFluid) continue;
+            // fluid-neighbour bits in the order -x +x -y +y -z +z, bit 6 the cell itself (the divergence pass's mask)
+            uint32_t m = 1u << 6;
+            const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+            for (int n = 0; n < 6; ++n)
+                if (fluid_at(x + off[n][0], y + off[n][1], z + off[n][2])) m |= 1u << n;
+            fmask[i] = (uint8_t)m;
+            pdiag[i] = diag[d];
+            pdiv[i] = div[d];
+            bp_fluid[e]++;
+        }
+    }
+    SLOSH_HIP(hipMemcpyAsync(c->ctype, ctype.data(), ctype.size() * 4u, hipMemcpyHostToDevice, c->stream));
+    SLOSH_HIP(hipMemcpyAsync(c->diag, pdiag.data(), pdiag.size() * 4u, hipMemcpyHostToDevice, c->stream));
+    SLOSH_HIP(hipMemcpyAsync(c->div, pdiv.data(), pdiv.size() * 4u, hipMemcpyHostToDevice, c->stream));
+    SLOSH_HIP(hipMemcpyAsync(c->fmask, fmask.data(), fmask.size(), hipMemcpyHostToDevice, c->stream));
+    if (c->nactive)
+        SLOSH_HIP(hipMemcpyAsync(c->bp_fluid, bp_fluid.data(), bp_fluid.size() * 4u, hipMemcpyHostToDevice, c->stream));
+    SLOSH_HIP(hipStreamSynchronize(c->stream));
+    uint32_t nf = 0;
+    for (uint32_t v : bp_fluid) nf += v;
+    c->res.fluid_cells = nf;
+    return true;
+}
+
 // One substep in the order of SloshProxy.cpp:544-713, stopping after `stop` (SLOSH_STAGE_*).
 void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
 {
@@ -440,20 +556,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     }
     if (stop > SLOSH_STAGE_NORMALIZE) launch_divergence(c, dt);
     bool solved = false;
-    if (stop > SLOSH_STAGE_DIVERGENCE) {
-        SLOSH_HIP(hipEventRecord(c->ev_solve0, c->stream));
-        c->fcg_iters = 0;
-        c->fcg_fallback = 0;
-        if (c->s.pressure_solver == SLOSH_SOLVER_MGPCG) run_mgpcg(c);
-        else if (c->s.pressure_solver == SLOSH_SOLVER_RBSOR) run_rbsor(c);
-        else if (c->s.pressure_solver == SLOSH_SOLVER_FCG_UNET) run_fcg(c);
-        SLOSH_HIP(hipEventRecord(c->ev_solve1, c->stream));
-        c->res.fallback_used = c->fcg_fallback;
-        // the solver's final state comes back with the step's statistics (no host wait between solve and G2P)
-        solved = c->s.pressure_solver != SLOSH_SOLVER_NONE;
-        if (solved)
-            SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
-    }
+    if (stop > SLOSH_STAGE_DIVERGENCE) solved = run_solver(c);
     if (stop > SLOSH_STAGE_SOLVE) launch_project(c);
     // collider coupling of this step (SloshProxy.cpp:397 clears it per frame; one step is one substep here): the
     // contact and wetted-face terms of the projection, the smoothed force (Slosh.usf:1138-1216) and the submerged
@@ -491,24 +594,7 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     launch_step_stats(c);
     SLOSH_HIP(hipEventRecord(c->ev_step1, c->stream));
     read_stats(c);
-    if (solved) {
-        if (c->nactive) {
-            // flexible CG: its own iterations, plus those of the MGPCG fallback when it ran
-            const bool fcg = c->s.pressure_solver == SLOSH_SOLVER_FCG_UNET && c->fcg_fallback != 2;
-            c->res.solver_iterations = fcg ? (c->fcg_fallback ? c->fcg_iters + c->h_state->iters : c->fcg_iters)
-                                           : c->h_state->iters;
-            c->res.residual_rel = c->h_state->rel;
-            c->res.true_residual_rel = c->s.pressure_solver == SLOSH_SOLVER_RBSOR ? (float)c->h_state->rel
-                                                                                  : (float)c->h_state->rel_true;
-            c->res.rhs_norm = sqrt(c->h_state->bb);
-            c->res.converged = c->h_state->converged;
-            if ((c->s.flags & SLOSH_FLAG_REPORT_FP64_ERROR) && c->s.pressure_solver != SLOSH_SOLVER_RBSOR)
-                c->res.fp64_error_rel = c->h_state->err_rel;
-        }
-        float ms = 0.0f;
-        SLOSH_HIP(hipEventElapsedTime(&ms, c->ev_solve0, c->ev_solve1));
-        c->res.solve_ms = ms;
-    }
+    if (solved) solver_results(c);
     const StepStats &st = *c->h_stats;
     if (stop > SLOSH_STAGE_EXTRAPOLATE && st.drained) {
         // drained records took the dead key: the sorted array's live records are the first np - drained
@@ -553,14 +639,6 @@ void pool_to_dense(const slosh_ctx *c, const std::vector<uint32_t> &table, const
                     uint32_t l = (uint32_t)((x & 7) + 8 * ((y & 7) + 8 * (z & 7)));
                     dense[di] = (Dst)pool[(size_t)pl * g.pool_cells + (size_t)slot * kBrickCells + l];
                 }
-}
-
-template <typename T> std::vector<T> fetch(const slosh_ctx *c, const T *d, size_t n)
-{
-    std::vector<T> h(n);
-    SLOSH_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
-    SLOSH_HIP(hipStreamSynchronize(c->stream));
-    return h;
 }
 
 slosh_status read_grid_impl(slosh_ctx *c, int32_t field, void *dst, uint64_t bytes)
@@ -977,6 +1055,37 @@ slosh_status slosh_write_grid(slosh_ctx *c, int32_t field, const void *src, uint
                 }
         SLOSH_HIP(hipMemcpyAsync(target, pool.data(), pool.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
+        return SLOSH_OK;
+    })
+}
+
+slosh_status slosh_load_system(slosh_ctx *c, const uint8_t *types, const float *diag, const float *divergence)
+{
+    if (!c || !types || !diag || !divergence) return SLOSH_EINVAL;
+    const Geo &g = c->geo;
+    const size_t cells = (size_t)g.n[0] * g.n[1] * g.n[2];
+    for (size_t i = 0; i < cells; ++i)
+        if (types[i] > kCellSolid || (types[i] == kCellFluid && !(diag[i] >= 0.0f && diag[i] <= 1e6f)) ||
+            (types[i] == kCellFluid && !(fabsf(divergence[i]) <= 3.0e38f)))
+            return SLOSH_EINVAL;
+    ABI_GUARD(return load_system_impl(c, types, diag, divergence) ? SLOSH_OK : SLOSH_ECAPACITY;)
+}
+
+slosh_status slosh_solve(slosh_ctx *c)
+{
+    if (!c) return SLOSH_EINVAL;
+    ABI_GUARD({
+        c->res.solver_iterations = 0;
+        c->res.residual_rel = 0.0;
+        c->res.true_residual_rel = 0.0f;
+        c->res.fp64_error_rel = -1.0;
+        c->res.converged = 1;
+        if (run_solver(c)) {
+            SLOSH_HIP(hipStreamSynchronize(c->stream));
+            solver_results(c);
+        }
+        SLOSH_HIP(hipStreamSynchronize(c->stream));
+        harvest_timing(c);
         return SLOSH_OK;
     })
 }

@@ -9,6 +9,10 @@ sees the systems of whole trajectories (sloshing tanks with random fill depth, w
 acceleration; dam breaks). Usage:
 
     python -m slosh_amd.train_unet --out slosh_amd/weights/unet_slosh.npz --seconds 300
+
+With --dataset DIR the systems come from a training set written by slosh_amd.training_set (64^3 windows of seeded
+dam breaks and sloshing tanks): each is loaded into one 64^3 context (slosh_load_system), solved with MGPCG and the
+capture flags (slosh_solve), and trained on like a step's solve.
 """
 import argparse
 import json
@@ -85,6 +89,52 @@ def train(out, seconds, sizes=(64, 128), steps_per_scene=24, iters_per_step=6, b
     return w, history
 
 
+def train_dataset(out, dataset, seconds, iters_per_system=6, batch=4, lr=1e-3, seed=0, init=None, log=None,
+                  epochs=1000):
+    """Trains on the systems of a training set, in a seeded random order of its shards, until `seconds` pass."""
+    from . import training_set
+
+    rng = np.random.default_rng(seed)
+    w = unet.load(init) if init else unet.he_init(seed, head_scale=0.1)
+    s = training_set.window_settings(flags=abi.FLAG_CAPTURE_RESIDUALS | abi.FLAG_CAPTURE_VCYCLE_RESIDUALS |
+                                     TRAIN_MG_FLAGS, mg_coarse_sweeps=TRAIN_MG_SWEEPS)
+    sim = Sim(s)
+    sim.unet_set_weights(w)
+    shards = sorted(n for n in os.listdir(dataset) if n.startswith("systems_") and n.endswith(".npz"))
+    t0 = time.time()
+    history = []
+    done = False
+    for epoch in range(epochs):
+        for name in [shards[k] for k in rng.permutation(len(shards))]:
+            with np.load(os.path.join(dataset, name), allow_pickle=False) as z:
+                types, diag, div = z["types"], z["diag"], z["divergence"]
+                for k in rng.permutation(len(types)):
+                    sim.load_system(types[k], diag[k], div[k])
+                    sim.solve()
+                    step_lr = lr * (0.3 if time.time() - t0 > 0.7 * seconds else 1.0)
+                    rep = sim.unet_train(iters_per_system, batch=batch, learning_rate=step_lr,
+                                         offset=int(rng.integers(0, 64)))
+                    history.append({"shard": name, "system": int(k), "first_loss": rep["first_loss"],
+                                    "last_loss": rep["last_loss"], "systems": rep["systems"]})
+                    if time.time() - t0 >= seconds:
+                        done = True
+                        break
+            msg = f"epoch {epoch} {name}: loss {history[-1]['last_loss']:.4f} ({time.time() - t0:.0f} s, {len(history)} systems)"
+            print(msg, flush=True)
+            if log:
+                with open(log, "a") as f:
+                    f.write(msg + "\n")
+            if done:
+                break
+        if done:
+            break
+    w = sim.unet_get_weights()
+    sim.close()
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    unet.save(out, w, seconds=seconds, systems=len(history), final_loss=history[-1]["last_loss"] if history else -1.0)
+    return w, history
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=unet.default_weights_path())
@@ -98,14 +148,17 @@ def main():
     ap.add_argument("--init", default=None)
     ap.add_argument("--log", default=None)
     ap.add_argument("--history", default=None, help="write the per-step loss history (JSON) here")
+    ap.add_argument("--dataset", default=None, help="train on a training set written by slosh_amd.training_set")
     a = ap.parse_args()
-    _, hist = train(a.out, a.seconds, tuple(int(x) for x in a.sizes.split(",")), a.steps_per_scene, a.iters_per_step,
-                    a.batch, a.lr, a.seed, a.init, a.log)
+    if a.dataset:
+        _, hist = train_dataset(a.out, a.dataset, a.seconds, a.iters_per_step, a.batch, a.lr, a.seed, a.init, a.log)
+    else:
+        _, hist = train(a.out, a.seconds, tuple(int(x) for x in a.sizes.split(",")), a.steps_per_scene,
+                        a.iters_per_step, a.batch, a.lr, a.seed, a.init, a.log)
     if a.history:
         with open(a.history, "w") as f:
             json.dump(hist, f)
-    print(json.dumps({"weights": a.out, "scenes": len({h["scene"] for h in hist}),
-                      "final_loss": hist[-1]["last_loss"] if hist else None}))
+    print(json.dumps({"weights": a.out, "items": len(hist), "final_loss": hist[-1]["last_loss"] if hist else None}))
     return 0
 
 
