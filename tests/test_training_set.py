@@ -163,18 +163,29 @@ def tank_window_gpu():
     return max(wins, key=lambda w: int((w["types"] == 0).sum()) * int((w["types"] == 1).sum()))
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("solver", ["mgpcg", "rbsor", "fcg"])
-def test_gpu_loaded_windows_solve_to_tolerance(solver):
+@pytest.fixture(scope="module")
+def loaded_windows():
+    """The two windows of the solve tests (a dam break's whole grid, a sloshing tank's free-surface window), each
+    with scipy's sparse direct solve of its matrix (computed once for the three solvers)."""
     import scipy.sparse.linalg as spla
 
+    from slosh_amd import Sim
+
+    dam = next(T.dam_break_systems(Sim, seed=4, steps=2, every=2))
+    out = []
+    for sysm in (dam, tank_window_gpu()):
+        A, b = T.matrix(sysm)
+        out.append((sysm, spla.spsolve(A.tocsc(), b)))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", ["mgpcg", "rbsor", "fcg"])
+def test_gpu_loaded_windows_solve_to_tolerance(solver, loaded_windows):
     from slosh_amd import Sim, unet
 
     print(f"case loaded windows, {solver}", flush=True)
-    dam = next(T.dam_break_systems(Sim, seed=4, steps=2, every=2))
-    for sysm in (dam, tank_window_gpu()):
-        A, b = T.matrix(sysm)
-        ref = spla.spsolve(A.tocsc(), b)
+    for sysm, ref in loaded_windows:
         # at 1e-6 every accepted solve meets the true residual; a solve from p = 0 to 1e-6 can still be 2e-5 from
         # the exact solution on these systems (condition number), so the solution is checked at 1e-7
         for tol in (1e-6, 1e-7):
