@@ -172,7 +172,10 @@ void create_impl(slosh_ctx *c)
     if (slots > g.nbricks) slots = g.nbricks;
     g.npool = slots + 1u;
     const uint64_t pc64 = (uint64_t)g.npool * kBrickCells;
-    if (pc64 * 3u > 0xFFFFFFFFull || g.nbricks >= (1u << 23)) throw std::bad_alloc();
+    // 32-bit indices over three face planes, and 32-bit byte offsets into the FP64 pool arrays (the solver's
+    // column passes address them so): at most 2^20 - 1 pool slots
+    if (pc64 * 3u > 0xFFFFFFFFull || pc64 * sizeof(double) > 0xFFFFFFFFull || g.nbricks >= (1u << 23))
+        throw std::bad_alloc();
     g.pool_cells = (uint32_t)pc64;
     g.h = s.cell_size;
     g.inv_h = 1.0f / s.cell_size;

@@ -143,250 +143,395 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(const uint32_t *binfo, co
     block_partial<3>(v, partials);
 }
 
-// The residual update of the iteration fused with the V-cycle's first level-0 sweeps (PCGUpdatePRCS with its
-// fused red initialisation, then MGSmoothCS black): red z = r / diag, black z = (r + sum_n z_n) / diag where
-// the red neighbours' z is recomputed from r_in and q.
-__global__ __launch_bounds__(kThreads) void k_update_black(const uint32_t *binfo, const uint8_t *fmask,
-                                                           const float *diag, double *pres, const float *r_in,
-                                                           float *r_out, float *z, const float *d, const float *q,
-                                                           int first, const SolverState *st)
+// ---- column form of the per-iteration passes ----
+// One wavefront per brick: lane = x + 8y owns the brick's column (x, y, 0..7) and walks its eight planes z = k.
+// Every plane is one coalesced 256-byte load per array; a cell's neighbours in z are the planes above and below in
+// registers (the planes of the bricks below and above for k = 0 and 7), in x and y the values of the lanes beside
+// it (lane shuffles). The 32 cells beside the brick's x and y faces in a plane are loaded by lanes 0-31, one each,
+// and handed to the lanes on the faces by two more shuffles. Compared with two cells per thread and every
+// neighbour loaded, this moves each array's bytes through the caches once and issues a quarter of the load
+// instructions, and the wavefront's FP64 sums need no barrier: each brick's partials come from one wave_sum.
+constexpr uint32_t kColThreads = 64;
+
+// Element i of a pool array as the array's (wavefront-uniform) base plus a 32-bit byte offset: the loads and stores
+// of a column then share one offset register with the planes as immediate offsets, instead of a 64-bit address
+// per access. Pool arrays stay below 4 GiB (the context refuses larger pools).
+template <typename T> __device__ __forceinline__ T pld(const T *p, uint32_t i)
 {
-    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    return *(const T *)((const char *)p + i * (uint32_t)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ void pst(T *p, uint32_t i, T v)
+{
+    *(T *)((char *)p + i * (uint32_t)sizeof(T)) = v;
+}
+
+
+struct Col {
+    uint32_t lane, x, y;
+    uint32_t at;      // pool index of the lane's cell in plane 0 (plane k: + 64 k)
+    uint32_t below;   // the cell under plane 0 (plane 7 of the -z brick) and
+    uint32_t above;   // the cell over plane 7 (plane 0 of the +z brick)
+    // The face cell this lane loads in plane 0 (+ 64 k for plane k): lanes 0-7 the -x brick's (7, lane), 8-15 the
+    // +x brick's (0, lane - 8), 16-23 the -y brick's (lane - 16, 7), 24-31 the +y brick's (lane - 24, 0); lanes
+    // 32-63 their own cell (an unused value: every lane issues the same loads, so none waits behind a branch).
+    uint32_t h;
+    uint32_t hbase, hx, hy; // pool base of that cell's brick and its local x and y
+    int sx, sy;             // lanes holding this lane's x-face (x == 0: -x, x == 7: +x) and y-face neighbour values
+};
+
+__device__ __forceinline__ Col make_col(const Brick &B, uint32_t lane)
+{
+    Col C;
+    C.lane = lane;
+    C.x = lane & 7u;
+    C.y = lane >> 3;
+    C.at = B.base + lane;
+    C.below = B.nb[4] + lane + 448u;
+    C.above = B.nb[5] + lane;
+    const uint32_t j = lane & 7u;
+    switch (lane >> 3) {
+    case 0: C.hbase = B.nb[0]; C.hx = 7u; C.hy = j; break;
+    case 1: C.hbase = B.nb[1]; C.hx = 0u; C.hy = j; break;
+    case 2: C.hbase = B.nb[2]; C.hx = j; C.hy = 7u; break;
+    case 3: C.hbase = B.nb[3]; C.hx = j; C.hy = 0u; break;
+    default: C.hbase = B.base; C.hx = C.x; C.hy = C.y; break;
+    }
+    C.h = C.hbase + C.hx + 8u * C.hy;
+    C.sx = C.x == 0u ? (int)C.y : (C.x == 7u ? 8 + (int)C.y : (int)lane);
+    C.sy = C.y == 0u ? 16 + (int)C.x : (C.y == 7u ? 24 + (int)C.x : (int)lane);
+    return C;
+}
+
+// The six neighbour values (-x +x -y +y -z +z) of the lane's cell in one plane, whose own value is v; hv is the
+// value of the face cell the lane loaded in that plane, lo / hi the values of the planes under and over it. Every
+// lane of the wavefront must call it (the shuffles read other lanes).
+__device__ __forceinline__ void col_nbrs(const Col &C, float v, float hv, float lo, float hi, float (&nb)[6])
+{
+    const float xm = __shfl_up(v, 1u, 64), xp = __shfl_down(v, 1u, 64);
+    const float ym = __shfl_up(v, 8u, 64), yp = __shfl_down(v, 8u, 64);
+    const float hxv = __shfl(hv, C.sx, 64), hyv = __shfl(hv, C.sy, 64);
+    nb[0] = C.x == 0u ? hxv : xm;
+    nb[1] = C.x == 7u ? hxv : xp;
+    nb[2] = C.y == 0u ? hyv : ym;
+    nb[3] = C.y == 7u ? hyv : yp;
+    nb[4] = lo;
+    nb[5] = hi;
+}
+
+// Writes the brick's partials (the wavefront sums of v) to partials[e * NV + k].
+template <int NV> __device__ __forceinline__ void col_partial(const double (&v)[NV], double *partials, uint32_t e,
+                                                              uint32_t lane)
+{
+    static_assert(NV >= 1 && NV <= (int)kMaxBlockPartials, "partials buffer holds kMaxBlockPartials per brick");
+    double a[NV];
+    for (int k = 0; k < NV; ++k) a[k] = wave_sum(v[k]);
+    if (lane == 0u)
+        for (int k = 0; k < NV; ++k) partials[(size_t)e * NV + k] = a[k];
+}
+
+// The residual update of the iteration fused with the V-cycle's first level-0 sweeps (PCGUpdatePRCS with its
+// fused red initialisation, then MGSmoothCS black): p += alpha d, r_out = r_in - alpha q, red z = r_out / diag,
+// black z = (r_out + sum_n z_n) / diag where the red neighbours' z is recomputed from r_in, q and diag.
+// kFirst: the solve's first update (no alpha yet: r_out = r_in, p unchanged); a template parameter so that the
+// loads of q, d and p are not split into branches of their own.
+template <bool kFirst>
+__global__ __launch_bounds__(kColThreads) void k_update_black(const uint32_t *binfo, const uint8_t *fmask,
+                                                              const float *diag, double *pres, const float *r_in,
+                                                              float *r_out, float *z, const float *d, const float *q,
+                                                              const SolverState *st)
+{
+    constexpr bool first = kFirst;
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const Brick B = load_brick(binfo, e);
     const uint32_t done = st->done;
     const double alpha = first ? 0.0 : st->alpha;
     issue_point();
     if (done == 1u) return; // done == 2: the solve's last update, made before every other pass stops
-    const uint32_t l0 = colour_cell(threadIdx.x, 0u), l1 = colour_cell(threadIdx.x, 1u);
-    const uint32_t i0 = B.base + l0, i1 = B.base + l1;
-    const uint32_t m0 = fmask[i0], m1 = fmask[i1];
-    const float g0 = diag[i0], g1 = diag[i1], ra0 = r_in[i0], ra1 = r_in[i1];
-    float q0 = 0.0f, q1 = 0.0f, d0 = 0.0f, d1 = 0.0f;
-    double p0 = 0.0, p1 = 0.0;
-    float rj[6], qj[6], gj[6];
-    for (int n = 0; n < 6; ++n) {
-        const uint32_t j = nbr(B, l1, n);
-        rj[n] = r_in[j];
-        gj[n] = diag[j];
+    const Col C = make_col(B, threadIdx.x);
+    // the red value r / diag of a neighbour cell after the update (0 when not needed)
+    auto red_of = [&](uint32_t i) {
+        const float ra = pld(r_in, i);
+        const float g = pld(diag, i);
+        const float qv = first ? 0.0f : pld(q, i);
+        const float r = first ? ra : (float)((double)ra - alpha * (double)qv);
+        return r / (g + kDiagEps);
+    };
+    float zr[10], hr[8], ra[8], qv[8], g[8];
+    uint32_t m[8];
+    zr[0] = red_of(C.below);
+    zr[9] = red_of(C.above);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t i = C.at + 64u * k;
+        m[k] = pld(fmask, i);
+        ra[k] = pld(r_in, i);
+        g[k] = pld(diag, i);
+        qv[k] = first ? 0.0f : pld(q, i);
+        hr[k] = red_of(C.h + 64u * k);
     }
-    if (!first) {
-        q0 = q[i0];
-        q1 = q[i1];
-        d0 = d[i0];
-        d1 = d[i1];
-        p0 = pres[i0];
-        p1 = pres[i1];
-        for (int n = 0; n < 6; ++n) qj[n] = q[nbr(B, l1, n)];
+    float rn[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        rn[k] = first ? ra[k] : (float)((double)ra[k] - alpha * (double)qv[k]);
+        zr[k + 1] = rn[k] / (g[k] + kDiagEps);
     }
-    if (m0 & kSelfFluid) {
-        float ri = ra0;
-        if (!first) {
-            pres[i0] = p0 + alpha * (double)d0;
-            ri = (float)((double)ra0 - alpha * (double)q0);
-        }
-        r_out[i0] = ri;
-        z[i0] = ri / (g0 + kDiagEps);
-    }
-    if (m1 & kSelfFluid) {
-        float ri = ra1;
-        if (!first) {
-            pres[i1] = p1 + alpha * (double)d1;
-            ri = (float)((double)ra1 - alpha * (double)q1);
-        }
-        r_out[i1] = ri;
-        float s = 0.0f;
-        for (int n = 0; n < 6; ++n) {
-            if (!(m1 >> n & 1u)) continue;
-            const float r = first ? rj[n] : (float)((double)rj[n] - alpha * (double)qj[n]);
-            s += r / (gj[n] + kDiagEps);
-        }
-        z[i1] = (ri + s) / (g1 + kDiagEps);
-    }
-}
-
-// Restriction into pooled level 1: the residuals of the brick's 512 cells in LDS, then b_c = 1/4 of the fluid
-// children's sum for the brick's 4^3 level-1 cells, and x_c = b_c / diag_c on red cells (fused red sweep).
-__global__ __launch_bounds__(kThreads) void k_restrict0(const uint32_t *binfo, const uint8_t *fmask,
-                                                        const float *diag, const float *xf, const float *bf,
-                                                        PLevel L1, const SolverState *st)
-{
-    __shared__ float res[kBrickCells];
-    __shared__ uint8_t fl[kBrickCells];
-    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
-    const uint32_t done = st->done;
-    const double mg_scale = st->mg_scale;
+    // the pressure update's loads are issued once the red values are formed (not hoisted above them: all of them
+    // in flight at once would cost occupancy)
     issue_point();
-    if (done) return;
-    const uint32_t t = threadIdx.x;
-    uint32_t l[2], m[2];
-    float xv[2], bv[2], gv[2], xn[2][6];
-    for (uint32_t c = 0; c < 2u; ++c) {
-        l[c] = colour_cell(t, c);
-        const uint32_t i = B.base + l[c];
-        m[c] = fmask[i];
-        xv[c] = xf[i];
-        bv[c] = bf[i];
-        gv[c] = diag[i];
-        for (int n = 0; n < 6; ++n) xn[c][n] = xf[nbr(B, l[c], n)];
+    double pv[8];
+    float dv[8];
+    if (!first) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            pv[k] = pld(pres, C.at + 64u * k);
+            dv[k] = pld(d, C.at + 64u * k);
+        }
     }
-    const uint32_t ci = (B.base >> 3) + (t & 63u);
-    const uint32_t info = L1.info[ci];
-    for (uint32_t c = 0; c < 2u; ++c) {
-        fl[l[c]] = (uint8_t)((m[c] & kSelfFluid) ? 1u : 0u);
-        if (m[c] & kSelfFluid) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float nb[6];
+        col_nbrs(C, zr[k + 1], hr[k], zr[k], zr[k + 2], nb);
+        if (!(m[k] & kSelfFluid)) continue;
+        const uint32_t i = C.at + 64u * k;
+        if (!first) pst(pres, i, pv[k] + alpha * (double)dv[k]);
+        pst(r_out, i, rn[k]);
+        if (((C.x + C.y + (uint32_t)k) & 1u) == 0u) {
+            pst(z, i, zr[k + 1]);
+        } else {
             float s = 0.0f;
             for (int n = 0; n < 6; ++n)
-                if (m[c] >> n & 1u) s += xn[c][n];
-            float dg = gv[c] + kDiagEps;
-            res[l[c]] = bv[c] - (dg * xv[c] - s);
+                if (m[k] >> n & 1u) s += nb[n];
+            pst(z, i, (rn[k] + s) / (g[k] + kDiagEps));
         }
     }
-    __syncthreads();
-    if (t >= 64u) return;
-    const uint32_t lx = t & 3u, ly = (t >> 2) & 3u, lz = t >> 4;
-    if (!(info & kInfoSelf)) {
-        L1.b[ci] = 0.0f;
-        L1.x[ci] = 0.0f;
-        return;
-    }
-    float sum = 0.0f;
-    for (uint32_t oi = 0; oi < 8u; ++oi) {
-        uint32_t fx = 2u * lx + (oi & 1u), fy = 2u * ly + ((oi >> 1) & 1u), fz = 2u * lz + (oi >> 2);
-        uint32_t f = fx + 8u * fy + 64u * fz;
-        if (fl[f]) sum += res[f];
-    }
-    float b = 0.25f * sum;
-    if (L1.b.half) b *= (float)mg_scale; // FP16 levels hold the scaled cycle (the V-cycle is linear)
-    L1.b[ci] = b;
-    L1.x[ci] = (info & kInfoRed) ? b / info_diag(info) : 0.0f;
 }
 
-// Prolongation from pooled level 1 (x1) fused with the black sweep: red z1 = z + x1(parent), black
-// z1 = (r + sum_n (z_n + x1(parent_n))) / diag; a parent contributes only when it is fluid.
-__global__ __launch_bounds__(kThreads) void k_prolong_black(const uint32_t *binfo, const uint8_t *fmask,
-                                                            const float *diag, const float *r, const float *z,
-                                                            float *z1, PLevel L1, const SolverState *st)
+// Restriction into pooled level 1: the residual b - A x of every fluid cell (neighbours' x through the column
+// form), then b_c = 1/4 of the fluid children's sum for the brick's 4^3 level-1 cells, in the children's order
+// (x, then y, then z fastest to slowest), and x_c = b_c / diag_c on red cells (fused red sweep). The lane with even
+// x and y sums its 2x2 group of lanes over the plane pair.
+__global__ __launch_bounds__(kColThreads) void k_restrict0(const uint32_t *binfo, const uint8_t *fmask,
+                                                           const float *diag, const float *xf, const float *bf,
+                                                           PLevel L1, const SolverState *st)
 {
-    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const Brick B = load_brick(binfo, e);
     const uint32_t done = st->done;
     const double mg_scale = st->mg_scale;
     issue_point();
     if (done) return;
-    const uint32_t l0 = colour_cell(threadIdx.x, 0u), l1 = colour_cell(threadIdx.x, 1u);
-    const uint32_t i0 = B.base + l0, i1 = B.base + l1;
-    const uint32_t m0 = fmask[i0], m1 = fmask[i1];
-    const uint32_t pi0 = (B.base >> 3) + parent1(l0);
-    const float z0 = z[i0], r1 = r[i1], g1 = diag[i1];
-    const uint32_t pinfo0 = L1.info[pi0];
-    const float px0 = L1.x1[pi0];
-    float zn[6], pxn[6];
-    uint32_t pinfon[6];
-    for (int n = 0; n < 6; ++n) {
-        uint32_t nl;
-        const uint32_t j = nbr(B, l1, n, nl);
-        const uint32_t pj = ((j - nl) >> 3) + parent1(nl);
-        zn[n] = z[j];
-        pinfon[n] = L1.info[pj];
-        pxn[n] = L1.x1[pj];
+    const Col C = make_col(B, threadIdx.x);
+    float xv[10], hv[8], bv[8], gv[8];
+    uint32_t m[8];
+    xv[0] = pld(xf, C.below);
+    xv[9] = pld(xf, C.above);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t i = C.at + 64u * k;
+        m[k] = pld(fmask, i);
+        xv[k + 1] = pld(xf, i);
+        bv[k] = pld(bf, i);
+        gv[k] = pld(diag, i);
+        hv[k] = pld(xf, C.h + 64u * k);
     }
-    const float unscale = L1.x1.half ? (float)(1.0 / mg_scale) : 1.0f;
-    if (m0 & kSelfFluid) {
-        float v = z0;
-        if (pinfo0 & kInfoSelf) v += px0 * unscale;
-        z1[i0] = v;
-    }
-    if (m1 & kSelfFluid) {
-        float s = 0.0f;
-        for (int n = 0; n < 6; ++n) {
-            if (!(m1 >> n & 1u)) continue;
-            float v = zn[n];
-            if (pinfon[n] & kInfoSelf) v += pxn[n] * unscale;
-            s += v;
+    const bool writer = ((C.x | C.y) & 1u) == 0u;
+    const uint32_t c0 = (B.base >> 3) + (C.x >> 1) + 4u * (C.y >> 1);
+    uint32_t info[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) info[p] = writer ? L1.info[c0 + 16u * p] : 0u;
+    float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float nb[6];
+        col_nbrs(C, xv[k + 1], hv[k], xv[k], xv[k + 2], nb);
+        const bool fluid = (m[k] & kSelfFluid) != 0u;
+        float res = 0.0f;
+        if (fluid) {
+            float s = 0.0f;
+            for (int n = 0; n < 6; ++n)
+                if (m[k] >> n & 1u) s += nb[n];
+            const float dg = gv[k] + kDiagEps;
+            res = bv[k] - (dg * xv[k + 1] - s);
         }
-        z1[i1] = (r1 + s) / (g1 + kDiagEps);
+        // children of this plane in order (x, y), (x+1, y), (x, y+1), (x+1, y+1); only fluid children count
+        const uint64_t fl = __ballot(fluid);
+        const float r1 = __shfl_down(res, 1u, 64), r8 = __shfl_down(res, 8u, 64), r9 = __shfl_down(res, 9u, 64);
+        const uint32_t L = C.lane;
+        float &acc = sum[k >> 1];
+        if (fl >> L & 1u) acc += res;
+        if (writer && (fl >> (L + 1u) & 1u)) acc += r1;
+        if (writer && (fl >> (L + 8u) & 1u)) acc += r8;
+        if (writer && (fl >> (L + 9u) & 1u)) acc += r9;
+    }
+    if (!writer) return;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const uint32_t ci = c0 + 16u * p;
+        if (!(info[p] & kInfoSelf)) {
+            L1.b[ci] = 0.0f;
+            L1.x[ci] = 0.0f;
+            continue;
+        }
+        float b = 0.25f * sum[p];
+        if (L1.b.half) b *= (float)mg_scale; // FP16 levels hold the scaled cycle (the V-cycle is linear)
+        L1.b[ci] = b;
+        L1.x[ci] = (info[p] & kInfoRed) ? b / info_diag(info[p]) : 0.0f;
+    }
+}
+
+// Prolongation from pooled level 1 (x1) fused with the black sweep: every cell's w = z + x1(parent) (the parent
+// contributes only when it is fluid); red z1 = w, black z1 = (r + sum_n w_n) / diag.
+__global__ __launch_bounds__(kColThreads) void k_prolong_black(const uint32_t *binfo, const uint8_t *fmask,
+                                                               const float *diag, const float *r, const float *z,
+                                                               float *z1, PLevel L1, const SolverState *st)
+{
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const Brick B = load_brick(binfo, e);
+    const uint32_t done = st->done;
+    const double mg_scale = st->mg_scale;
+    issue_point();
+    if (done) return;
+    const Col C = make_col(B, threadIdx.x);
+    const float unscale = L1.x1.half ? (float)(1.0 / mg_scale) : 1.0f;
+    // w of the cell at pool index i whose level-1 parent is at pooled index pi
+    auto w_of = [&](uint32_t i, uint32_t pi) {
+        const float v = pld(z, i), px = L1.x1[pi];
+        const uint32_t info = L1.info[pi];
+        return (info & kInfoSelf) ? v + px * unscale : v;
+    };
+    // level-1 parent of local cell (x, y, z) of the brick whose pool base is base
+    auto parent = [](uint32_t base, uint32_t x, uint32_t y, uint32_t zz) {
+        return (base >> 3) + (x >> 1) + 4u * (y >> 1) + 16u * (zz >> 1);
+    };
+    float w[10], hw[8], rv[8], gv[8];
+    uint32_t m[8];
+    w[0] = w_of(C.below, parent(B.nb[4], C.x, C.y, 7u));
+    w[9] = w_of(C.above, parent(B.nb[5], C.x, C.y, 0u));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t i = C.at + 64u * k;
+        m[k] = pld(fmask, i);
+        rv[k] = pld(r, i);
+        gv[k] = pld(diag, i);
+        w[k + 1] = w_of(i, parent(B.base, C.x, C.y, (uint32_t)k));
+        hw[k] = w_of(C.h + 64u * k, parent(C.hbase, C.hx, C.hy, (uint32_t)k));
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float nb[6];
+        col_nbrs(C, w[k + 1], hw[k], w[k], w[k + 2], nb);
+        if (!(m[k] & kSelfFluid)) continue;
+        const uint32_t i = C.at + 64u * k;
+        if (((C.x + C.y + (uint32_t)k) & 1u) == 0u) {
+            pst(z1, i, w[k + 1]);
+        } else {
+            float s = 0.0f;
+            for (int n = 0; n < 6; ++n)
+                if (m[k] >> n & 1u) s += nb[n];
+            pst(z1, i, (rv[k] + s) / (gv[k] + kDiagEps));
+        }
     }
 }
 
 // Red sweep z1 = (r + sum_n z1_n) / diag and the partials of r.z1 over all fluid cells.
-__global__ __launch_bounds__(kThreads) void k_red_dot(const uint32_t *binfo, const uint8_t *fmask, const float *diag,
-                                                      const float *r, float *z1, double *partials,
-                                                      const SolverState *st)
+__global__ __launch_bounds__(kColThreads) void k_red_dot(const uint32_t *binfo, const uint8_t *fmask,
+                                                         const float *diag, const float *r, float *z1,
+                                                         double *partials, const SolverState *st)
 {
-    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const Brick B = load_brick(binfo, e);
     const uint32_t done = st->done;
     issue_point();
     if (done) return;
-    const uint32_t l0 = colour_cell(threadIdx.x, 0u), i0 = B.base + l0, i1 = B.base + colour_cell(threadIdx.x, 1u);
-    const uint32_t m0 = fmask[i0], m1 = fmask[i1];
-    const float r0 = r[i0], g0 = diag[i0], r1 = r[i1], zb1 = z1[i1];
-    float zn[6];
-    for (int n = 0; n < 6; ++n) zn[n] = z1[nbr(B, l0, n)];
-    double v[1] = {0.0};
-    if (m0 & kSelfFluid) {
-        float s = 0.0f;
-        for (int n = 0; n < 6; ++n)
-            if (m0 >> n & 1u) s += zn[n];
-        float xv = (r0 + s) / (g0 + kDiagEps);
-        z1[i0] = xv;
-        v[0] += (double)r0 * (double)xv;
+    const Col C = make_col(B, threadIdx.x);
+    float zv[10], hv[8], rv[8], gv[8];
+    uint32_t m[8];
+    zv[0] = pld(z1, C.below);
+    zv[9] = pld(z1, C.above);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t i = C.at + 64u * k;
+        m[k] = pld(fmask, i);
+        zv[k + 1] = pld(z1, i);
+        rv[k] = pld(r, i);
+        gv[k] = pld(diag, i);
+        hv[k] = pld(z1, C.h + 64u * k);
     }
-    if (m1 & kSelfFluid) v[0] += (double)r1 * (double)zb1;
-    block_partial<1>(v, partials);
+    double v[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float nb[6];
+        col_nbrs(C, zv[k + 1], hv[k], zv[k], zv[k + 2], nb);
+        if (!(m[k] & kSelfFluid)) continue;
+        float xv = zv[k + 1];
+        if (((C.x + C.y + (uint32_t)k) & 1u) == 0u) {
+            float s = 0.0f;
+            for (int n = 0; n < 6; ++n)
+                if (m[k] >> n & 1u) s += nb[n];
+            xv = (rv[k] + s) / (gv[k] + kDiagEps);
+            pst(z1, C.at + 64u * k, xv);
+        }
+        v[0] += (double)rv[k] * (double)xv;
+    }
+    col_partial<1>(v, partials, e, C.lane);
 }
 
 // d_out = z1 + beta d_in (beta = 0 for the first direction) fused with q = A d_out: the neighbours' new
 // direction is recomputed from z1 and d_in. The stencil sum is FP64 (for smooth d the diagonal and neighbour
 // terms cancel, and an FP32 sum would put a large relative error into q that the recurrence residual never
-// sees); partials d.q.
-__global__ __launch_bounds__(kThreads) void k_direction(const uint32_t *binfo, const uint8_t *fmask,
-                                                        const float *diag, const float *z1, const float *d_in,
-                                                        float *d_out, float *q, const float *r, int first,
-                                                        double *partials, const SolverState *st)
+// sees); partials d.q, and for the stopping test of the coming update r - alpha q: r.q, q.q and r.r (q as stored,
+// in FP32). kFirst: the first direction d = z1 (a template parameter so that the loads of d_in are not split into
+// branches of their own).
+template <bool kFirst>
+__global__ __launch_bounds__(kColThreads) void k_direction(const uint32_t *binfo, const uint8_t *fmask,
+                                                           const float *diag, const float *z1, const float *d_in,
+                                                           float *d_out, float *q, const float *r, double *partials,
+                                                           const SolverState *st)
 {
-    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    constexpr bool first = kFirst;
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const Brick B = load_brick(binfo, e);
     const uint32_t done = st->done;
     const double beta = first ? 0.0 : st->beta;
     issue_point();
     if (done) return;
-    uint32_t i[2], m[2];
-    float zc[2], dc[2] = {0.0f, 0.0f}, gc[2], rc[2], zn[2][6], dn[2][6];
-    for (uint32_t c = 0; c < 2u; ++c) {
-        const uint32_t l = colour_cell(threadIdx.x, c);
-        i[c] = B.base + l;
-        m[c] = fmask[i[c]];
-        zc[c] = z1[i[c]];
-        gc[c] = diag[i[c]];
-        rc[c] = r[i[c]];
-        for (int n = 0; n < 6; ++n) zn[c][n] = z1[nbr(B, l, n)];
+    const Col C = make_col(B, threadIdx.x);
+    auto dir = [&](float zv, float dv) { return first ? zv : (float)((double)zv + beta * (double)dv); };
+    float dd[10], hv[8], gv[8], rv[8];
+    uint32_t m[8];
+    dd[0] = dir(pld(z1, C.below), first ? 0.0f : pld(d_in, C.below));
+    dd[9] = dir(pld(z1, C.above), first ? 0.0f : pld(d_in, C.above));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t i = C.at + 64u * k;
+        m[k] = pld(fmask, i);
+        gv[k] = pld(diag, i);
+        rv[k] = pld(r, i);
+        dd[k + 1] = dir(pld(z1, i), first ? 0.0f : pld(d_in, i));
+        hv[k] = dir(pld(z1, C.h + 64u * k), first ? 0.0f : pld(d_in, C.h + 64u * k));
     }
-    if (!first) {
-        for (uint32_t c = 0; c < 2u; ++c) {
-            const uint32_t l = colour_cell(threadIdx.x, c);
-            dc[c] = d_in[i[c]];
-            for (int n = 0; n < 6; ++n) dn[c][n] = d_in[nbr(B, l, n)];
-        }
-    }
-    // d.q, and for the stopping test of the coming update r - alpha q: r.q, q.q and r.r (q as stored, in FP32)
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t c = 0; c < 2u; ++c) {
-        if (!(m[c] & kSelfFluid)) continue;
-        const float dv = first ? zc[c] : (float)((double)zc[c] + beta * (double)dc[c]);
-        d_out[i[c]] = dv;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float nb[6];
+        col_nbrs(C, dd[k + 1], hv[k], dd[k], dd[k + 2], nb);
+        if (!(m[k] & kSelfFluid)) continue;
+        const uint32_t i = C.at + 64u * k;
+        const float dv = dd[k + 1];
+        pst(d_out, i, dv);
         double s = 0.0;
-        for (int n = 0; n < 6; ++n) {
-            if (!(m[c] >> n & 1u)) continue;
-            const float dj = first ? zn[c][n] : (float)((double)zn[c][n] + beta * (double)dn[c][n]);
-            s += (double)dj;
-        }
-        const float dg = gc[c] + kDiagEps;
+        for (int n = 0; n < 6; ++n)
+            if (m[k] >> n & 1u) s += (double)nb[n];
+        const float dg = gv[k] + kDiagEps;
         const double qd = (double)dg * (double)dv - s;
         const float qf = (float)qd;
-        q[i[c]] = qf;
+        pst(q, i, qf);
         v[0] += (double)dv * qd;
-        v[1] += (double)rc[c] * (double)qf;
+        v[1] += (double)rv[k] * (double)qf;
         v[2] += (double)qf * (double)qf;
-        v[3] += (double)rc[c] * (double)rc[c];
+        v[3] += (double)rv[k] * (double)rv[k];
     }
-    block_partial<4>(v, partials);
+    col_partial<4>(v, partials, e, C.lane);
 }
 
 // Red-black SOR sweep on the FP64 pressure: p += w ((sum p_n - div) / diag - p).
@@ -533,6 +678,9 @@ __global__ __launch_bounds__(kThreads) void k_error_partials(const uint32_t *bin
 
 #define L0_LAUNCH(kid, kernel, ...)                                                                              \
     launch_timed(c, slosh::kid, kernel, dim3(c->nsolve), dim3(kThreads), 0u, c->binfo_f, c->fmask, __VA_ARGS__)
+// the column-form passes: one wavefront per brick holding fluid
+#define L0C_LAUNCH(kid, kernel, ...)                                                                             \
+    launch_timed(c, slosh::kid, kernel, dim3(c->nsolve), dim3(kColThreads), 0u, c->binfo_f, c->fmask, __VA_ARGS__)
 
 void l0_brick_info(slosh_ctx *c, bool readback)
 {
@@ -561,24 +709,26 @@ void l0_pcg_init(slosh_ctx *c, float *r_out)
 
 void l0_update_black(slosh_ctx *c, const float *r_in, float *r_out, const float *d, int first)
 {
-    L0_LAUNCH(K_UPDATE_BLACK, k_update_black, c->diag, c->pres, r_in, r_out, c->z, d, c->q, first, c->state);
+    if (first) L0C_LAUNCH(K_UPDATE_BLACK, k_update_black<true>, c->diag, c->pres, r_in, r_out, c->z, d, c->q, c->state);
+    else L0C_LAUNCH(K_UPDATE_BLACK, k_update_black<false>, c->diag, c->pres, r_in, r_out, c->z, d, c->q, c->state);
 }
 
 void l0_restrict(slosh_ctx *c, const float *r)
 {
-    L0_LAUNCH(K_RESTRICT0, k_restrict0, c->diag, c->z, r, plevel_of(c, 1), c->state);
+    L0C_LAUNCH(K_RESTRICT0, k_restrict0, c->diag, c->z, r, plevel_of(c, 1), c->state);
 }
 
 void l0_prolong_black(slosh_ctx *c, const float *r)
 {
-    L0_LAUNCH(K_PROLONG_BLACK, k_prolong_black, c->diag, r, c->z, c->z1, plevel_of(c, 1), c->state);
+    L0C_LAUNCH(K_PROLONG_BLACK, k_prolong_black, c->diag, r, c->z, c->z1, plevel_of(c, 1), c->state);
 }
 
-void l0_red_dot(slosh_ctx *c, const float *r) { L0_LAUNCH(K_RED_DOT, k_red_dot, c->diag, r, c->z1, c->partials, c->state); }
+void l0_red_dot(slosh_ctx *c, const float *r) { L0C_LAUNCH(K_RED_DOT, k_red_dot, c->diag, r, c->z1, c->partials, c->state); }
 
 void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, const float *r, int first)
 {
-    L0_LAUNCH(K_DIRECTION, k_direction, c->diag, c->z1, d_in, d_out, c->q, r, first, c->partials, c->state);
+    if (first) L0C_LAUNCH(K_DIRECTION, k_direction<true>, c->diag, c->z1, d_in, d_out, c->q, r, c->partials, c->state);
+    else L0C_LAUNCH(K_DIRECTION, k_direction<false>, c->diag, c->z1, d_in, d_out, c->q, r, c->partials, c->state);
 }
 
 void l0_fcg_zq(slosh_ctx *c, const float *z) { L0_LAUNCH(K_FCG, k_fcg_zq, z, c->q, c->partials, c->state); }
