@@ -119,30 +119,6 @@ __global__ void k_fluid_rows(uint32_t nactive, const uint32_t *binfo, const uint
     binfo_f[(size_t)rank[e] * kBinfoWords + k] = binfo[t];
 }
 
-// r = b - A p (FP64 p); partials ||b||^2, ||r||^2 and the fluid cell count.
-__global__ __launch_bounds__(kThreads) void k_pcg_init(const uint32_t *binfo, const uint8_t *fmask, const float *div,
-                                                       const float *diag, const double *pres, float *r,
-                                                       double *partials)
-{
-    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
-    double v[3] = {0.0, 0.0, 0.0};
-    for (uint32_t c = 0; c < 2u; ++c) {
-        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
-        if (!(m & kSelfFluid)) {
-            r[i] = 0.0f;
-            continue;
-        }
-        v[2] += 1.0;
-        double s = nbsum<double>(B, l, m, pres);
-        float dg = diag[i] + kDiagEps;
-        float ri = (float)(-(double)div[i] - ((double)dg * pres[i] - s));
-        r[i] = ri;
-        v[0] += (double)div[i] * (double)div[i];
-        v[1] += (double)ri * (double)ri;
-    }
-    block_partial<3>(v, partials);
-}
-
 // ---- column form of the per-iteration passes ----
 // One wavefront per brick: lane = x + 8y owns the brick's column (x, y, 0..7) and walks its eight planes z = k.
 // Every plane is one coalesced 256-byte load per array; a cell's neighbours in z are the planes above and below in
@@ -534,42 +510,133 @@ __global__ __launch_bounds__(kColThreads) void k_direction(const uint32_t *binfo
     col_partial<4>(v, partials, e, C.lane);
 }
 
-// Red-black SOR sweep on the FP64 pressure: p += w ((sum p_n - div) / diag - p).
-__global__ __launch_bounds__(kThreads) void k_sor_sweep(const uint32_t *binfo, const uint8_t *fmask, const float *div,
-                                                        const float *diag, double *pres, uint32_t colour, double w,
-                                                        const SolverState *st)
+// ---- passes on the FP64 pressure, in column form ----
+// The FP64 counterpart of col_nbrs.
+__device__ __forceinline__ void col_nbrs64(const Col &C, double v, double hv, double lo, double hi, double (&nb)[6])
 {
-    if (st->done) return;
-    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
-    const uint32_t l = colour_cell(threadIdx.x, colour), i = B.base + l, m = fmask[i];
-    if (!(m & kSelfFluid)) return;
-    double s = nbsum<double>(B, l, m, pres);
-    double dg = (double)(diag[i] + kDiagEps);
-    double gs = (s - (double)div[i]) / dg;
-    double old = pres[i];
-    pres[i] = old + w * (gs - old);
+    const double xm = __shfl_up(v, 1u, 64), xp = __shfl_down(v, 1u, 64);
+    const double ym = __shfl_up(v, 8u, 64), yp = __shfl_down(v, 8u, 64);
+    const double hxv = __shfl(hv, C.sx, 64), hyv = __shfl(hv, C.sy, 64);
+    nb[0] = C.x == 0u ? hxv : xm;
+    nb[1] = C.x == 7u ? hxv : xp;
+    nb[2] = C.y == 0u ? hyv : ym;
+    nb[3] = C.y == 7u ? hyv : yp;
+    nb[4] = lo;
+    nb[5] = hi;
+}
+
+// A column's pressure planes (with the planes under and over it), its face cells' pressure, and its divergence,
+// diagonal and stencil masks.
+struct PresCol {
+    double p[10], h[8];
+    float div[8], g[8];
+    uint32_t m[8];
+};
+__device__ __forceinline__ void load_pres_col(const Col &C, const uint8_t *fmask, const float *div, const float *diag,
+                                              const double *pres, PresCol &P)
+{
+    P.p[0] = pld(pres, C.below);
+    P.p[9] = pld(pres, C.above);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t i = C.at + 64u * k;
+        P.m[k] = pld(fmask, i);
+        P.div[k] = pld(div, i);
+        P.g[k] = pld(diag, i);
+        P.p[k + 1] = pld(pres, i);
+        P.h[k] = pld(pres, C.h + 64u * k);
+    }
+}
+
+// r = b - A p (FP64 p) over every cell of an allocated brick (0 off the fluid cells); partials ||b||^2, ||r||^2 and
+// the fluid cell count.
+__global__ __launch_bounds__(kColThreads) void k_pcg_init(const uint32_t *binfo, const uint8_t *fmask, const float *div,
+                                                          const float *diag, const double *pres, float *r,
+                                                          double *partials)
+{
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const Col C = make_col(load_brick(binfo, e), threadIdx.x);
+    PresCol P;
+    load_pres_col(C, fmask, div, diag, pres, P);
+    double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double nb[6];
+        col_nbrs64(C, P.p[k + 1], P.h[k], P.p[k], P.p[k + 2], nb);
+        const uint32_t i = C.at + 64u * k;
+        if (!(P.m[k] & kSelfFluid)) {
+            pst(r, i, 0.0f);
+            continue;
+        }
+        v[2] += 1.0;
+        double s = 0.0;
+        for (int n = 0; n < 6; ++n)
+            if (P.m[k] >> n & 1u) s += nb[n];
+        const float dg = P.g[k] + kDiagEps;
+        const float ri = (float)(-(double)P.div[k] - ((double)dg * P.p[k + 1] - s));
+        pst(r, i, ri);
+        v[0] += (double)P.div[k] * (double)P.div[k];
+        v[1] += (double)ri * (double)ri;
+    }
+    col_partial<3>(v, partials, e, C.lane);
+}
+
+// Red-black SOR sweep of one colour on the FP64 pressure: p += w ((sum p_n - div) / diag - p). The neighbours are of
+// the other colour, which the sweep does not write.
+__global__ __launch_bounds__(kColThreads) void k_sor_sweep(const uint32_t *binfo, const uint8_t *fmask, const float *div,
+                                                           const float *diag, double *pres, uint32_t colour, double w,
+                                                           const SolverState *st)
+{
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const Brick B = load_brick(binfo, e);
+    const uint32_t done = st->done;
+    issue_point();
+    if (done) return;
+    const Col C = make_col(B, threadIdx.x);
+    PresCol P;
+    load_pres_col(C, fmask, div, diag, pres, P);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double nb[6];
+        col_nbrs64(C, P.p[k + 1], P.h[k], P.p[k], P.p[k + 2], nb);
+        if (!(P.m[k] & kSelfFluid) || ((C.x + C.y + (uint32_t)k) & 1u) != colour) continue;
+        double s = 0.0;
+        for (int n = 0; n < 6; ++n)
+            if (P.m[k] >> n & 1u) s += nb[n];
+        const double dg = (double)(P.g[k] + kDiagEps);
+        const double gs = (s - (double)P.div[k]) / dg;
+        const double old = P.p[k + 1];
+        pst(pres, C.at + 64u * k, old + w * (gs - old));
+    }
 }
 
 // Partials ||b||^2 and ||b - A p||^2: mode 0 the SOR convergence check (skipped once the solver is done), 1 always,
 // 2 only once the solver is done (the true-residual check of a finished MGPCG solve).
-__global__ __launch_bounds__(kThreads) void k_sor_residual(const uint32_t *binfo, const uint8_t *fmask,
-                                                           const float *div, const float *diag, const double *pres,
-                                                           double *partials, const SolverState *st, int mode)
+__global__ __launch_bounds__(kColThreads) void k_sor_residual(const uint32_t *binfo, const uint8_t *fmask,
+                                                              const float *div, const float *diag, const double *pres,
+                                                              double *partials, const SolverState *st, int mode)
 {
     const int32_t done = st->done;
     if ((mode == 0 && done) || (mode == 2 && !done)) return;
-    const Brick B = load_brick(binfo, xcd_remap(blockIdx.x, gridDim.x));
+    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const Col C = make_col(load_brick(binfo, e), threadIdx.x);
+    PresCol P;
+    load_pres_col(C, fmask, div, diag, pres, P);
     double v[2] = {0.0, 0.0};
-    for (uint32_t c = 0; c < 2u; ++c) {
-        const uint32_t l = colour_cell(threadIdx.x, c), i = B.base + l, m = fmask[i];
-        if (!(m & kSelfFluid)) continue;
-        double s = nbsum<double>(B, l, m, pres);
-        double dg = (double)(diag[i] + kDiagEps);
-        double ri = -(double)div[i] - (dg * pres[i] - s);
-        v[0] += (double)div[i] * (double)div[i];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double nb[6];
+        col_nbrs64(C, P.p[k + 1], P.h[k], P.p[k], P.p[k + 2], nb);
+        if (!(P.m[k] & kSelfFluid)) continue;
+        double s = 0.0;
+        for (int n = 0; n < 6; ++n)
+            if (P.m[k] >> n & 1u) s += nb[n];
+        const double dg = (double)(P.g[k] + kDiagEps);
+        const double ri = -(double)P.div[k] - (dg * P.p[k + 1] - s);
+        v[0] += (double)P.div[k] * (double)P.div[k];
         v[1] += ri * ri;
     }
-    block_partial<2>(v, partials);
+    col_partial<2>(v, partials, e, C.lane);
 }
 
 // ---- flexible CG (Notay's FCG(1)) with a nonlinear preconditioner z = M(r) ----
@@ -703,7 +770,7 @@ void l0_brick_info(slosh_ctx *c, bool readback)
 // Over every allocated brick (non-fluid cells get r = 0, also in bricks without fluid).
 void l0_pcg_init(slosh_ctx *c, float *r_out)
 {
-    launch_timed(c, K_PCG_INIT, k_pcg_init, dim3(c->nactive), dim3(kThreads), 0, c->binfo, c->fmask, c->div, c->diag,
+    launch_timed(c, K_PCG_INIT, k_pcg_init, dim3(c->nactive), dim3(kColThreads), 0, c->binfo, c->fmask, c->div, c->diag,
                  c->pres, r_out, c->partials);
 }
 
@@ -745,12 +812,12 @@ void l0_fcg_update(slosh_ctx *c, float *r, const float *d)
 
 void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w)
 {
-    L0_LAUNCH(K_SOR_SWEEP, k_sor_sweep, c->div, c->diag, c->pres, colour, w, c->state);
+    L0C_LAUNCH(K_SOR_SWEEP, k_sor_sweep, c->div, c->diag, c->pres, colour, w, c->state);
 }
 
 void l0_sor_residual(slosh_ctx *c)
 {
-    L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 0);
+    L0C_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 0);
 }
 
 void l0_fcg_resid(slosh_ctx *c, const float *r, const float *z, float *out)
@@ -765,12 +832,12 @@ void l0_error_partials(slosh_ctx *c, const double *p_ref)
 
 void l0_true_residual(slosh_ctx *c)
 {
-    L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 1);
+    L0C_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 1);
 }
 
 void l0_true_residual_if_done(slosh_ctx *c)
 {
-    L0_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 2);
+    L0C_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 2);
 }
 
 } // namespace slosh
