@@ -301,6 +301,7 @@ struct slosh_ctx {
     slosh::SolverState *state_save = nullptr;
     uint32_t *coarse_sig = nullptr; // the coarsest level's cell types and indices at its last factorisation
     double *partials = nullptr;
+    uint32_t nparts = 0;              // partial sets the last reduction pass wrote (what the next finalize sums)
     uint16_t *dense_info = nullptr;   // stencil words of all dense levels, level by level
     uint32_t *dense_coord = nullptr;  // their packed cell coordinates
     slosh::SolverState *state = nullptr;

@@ -1079,7 +1079,7 @@ __global__ void k_state_reset(SolverState *st, double tol, int maxit)
 void finalize(slosh_ctx *c, int kind, int iters = 0)
 {
     SLOSH_TIMED(c, K_FINALIZE);
-    const uint32_t nblocks = kind == kFinInit ? c->nactive : c->nsolve;
+    const uint32_t nblocks = c->nparts;
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, c->stream, kind, c->partials, nblocks, iters,
                        c->state);
     (void)fin_values;
