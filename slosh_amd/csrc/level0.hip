@@ -127,19 +127,20 @@ __global__ void k_fluid_rows(uint32_t nactive, const uint32_t *binfo, const uint
 // and handed to the lanes on the faces by two more shuffles. Compared with two cells per thread and every
 // neighbour loaded, this moves each array's bytes through the caches once and issues a quarter of the load
 // instructions, and the wavefront's FP64 sums need no barrier: each brick's partials come from one wave_sum.
-constexpr uint32_t kColBricks = 4; // bricks (wavefronts) per workgroup
-constexpr uint32_t kColThreads = 64u * kColBricks;
+constexpr uint32_t kColMaxBricks = 4; // most bricks (wavefronts) per workgroup (the launch picks 1, 2 or 4)
+constexpr uint32_t kColMaxThreads = 64u * kColMaxBricks;
 
-// Brick of this wavefront in a column pass over n bricks: kColBricks consecutive bricks per workgroup, workgroups
-// spread over the XCDs in contiguous ranges. `live` is false for the last workgroup's waves past n.
+// Brick of this wavefront in a column pass over n bricks: blockDim.x / 64 consecutive bricks per workgroup,
+// workgroups spread over the XCDs in contiguous ranges. `live` is false for the last workgroup's waves past n. The
+// result does not depend on how many bricks a workgroup takes (each brick writes its own partials).
 struct ColBrick {
     uint32_t e;
     bool live;
 };
-__device__ __forceinline__ ColBrick col_brick(uint32_t n)
+__device__ __forceinline__ ColBrick col_brick(const Blk &G, uint32_t n)
 {
     ColBrick b;
-    b.e = xcd_remap(blockIdx.x, gridDim.x) * kColBricks + (threadIdx.x >> 6);
+    b.e = xcd_remap(G.b, G.nb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     b.live = b.e < n;
     return b;
 }
@@ -210,23 +211,15 @@ __device__ __forceinline__ void col_nbrs(const Col &C, float v, float hv, float 
     nb[5] = hi;
 }
 
-// Writes the workgroup's partials to partials[blockIdx.x * NV + k]: the wavefront sums of v (0 for waves past the
-// last brick), added in wavefront order. Every thread of the workgroup must call it.
-template <int NV> __device__ __forceinline__ void col_partial(const double (&v)[NV], double *partials)
+// Writes the brick's partials (the wavefront sums of v) to partials[e * NV + k].
+template <int NV> __device__ __forceinline__ void col_partial(const double (&v)[NV], double *partials, uint32_t e,
+                                                              uint32_t lane)
 {
     static_assert(NV >= 1 && NV <= (int)kMaxBlockPartials, "partials buffer holds kMaxBlockPartials per brick");
-    __shared__ double sh[NV][kColBricks];
     double a[NV];
     for (int k = 0; k < NV; ++k) a[k] = wave_sum(v[k]);
-    if ((threadIdx.x & 63u) == 0u)
-        for (int k = 0; k < NV; ++k) sh[k][threadIdx.x >> 6] = a[k];
-    __syncthreads();
-    if (threadIdx.x == 0u)
-        for (int k = 0; k < NV; ++k) {
-            double t = sh[k][0];
-            for (uint32_t w = 1; w < kColBricks; ++w) t += sh[k][w];
-            partials[(size_t)blockIdx.x * NV + k] = t;
-        }
+    if (lane == 0u)
+        for (int k = 0; k < NV; ++k) partials[(size_t)e * NV + k] = a[k];
 }
 
 // The residual update of the iteration fused with the V-cycle's first level-0 sweeps (PCGUpdatePRCS with its
@@ -235,13 +228,13 @@ template <int NV> __device__ __forceinline__ void col_partial(const double (&v)[
 // kFirst: the solve's first update (no alpha yet: r_out = r_in, p unchanged); a template parameter so that the
 // loads of q, d and p are not split into branches of their own.
 template <bool kFirst>
-__global__ __launch_bounds__(kColThreads) void k_update_black(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+__device__ __forceinline__ void update_black_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
                                                               const float *diag, double *pres, const float *r_in,
                                                               float *r_out, float *z, const float *d, const float *q,
                                                               const SolverState *st)
 {
     constexpr bool first = kFirst;
-    const ColBrick cb = col_brick(n);
+    const ColBrick cb = col_brick(G, n);
     const Brick B = load_brick(binfo, cb.live ? cb.e : 0u);
     const uint32_t done = st->done;
     const double alpha = first ? 0.0 : st->alpha;
@@ -311,11 +304,11 @@ __global__ __launch_bounds__(kColThreads) void k_update_black(uint32_t n, const 
 // form), then b_c = 1/4 of the fluid children's sum for the brick's 4^3 level-1 cells, in the children's order
 // (x, then y, then z fastest to slowest), and x_c = b_c / diag_c on red cells (fused red sweep). The lane with even
 // x and y sums its 2x2 group of lanes over the plane pair.
-__global__ __launch_bounds__(kColThreads) void k_restrict0(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+__device__ __forceinline__ void restrict0_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
                                                            const float *diag, const float *xf, const float *bf,
                                                            PLevel L1, const SolverState *st)
 {
-    const ColBrick cb = col_brick(n);
+    const ColBrick cb = col_brick(G, n);
     const Brick B = load_brick(binfo, cb.live ? cb.e : 0u);
     const uint32_t done = st->done;
     const double mg_scale = st->mg_scale;
@@ -383,11 +376,11 @@ __global__ __launch_bounds__(kColThreads) void k_restrict0(uint32_t n, const uin
 
 // Prolongation from pooled level 1 (x1) fused with the black sweep: every cell's w = z + x1(parent) (the parent
 // contributes only when it is fluid); red z1 = w, black z1 = (r + sum_n w_n) / diag.
-__global__ __launch_bounds__(kColThreads) void k_prolong_black(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+__device__ __forceinline__ void prolong_black_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
                                                                const float *diag, const float *r, const float *z,
                                                                float *z1, PLevel L1, const SolverState *st)
 {
-    const ColBrick cb = col_brick(n);
+    const ColBrick cb = col_brick(G, n);
     const Brick B = load_brick(binfo, cb.live ? cb.e : 0u);
     const uint32_t done = st->done;
     const double mg_scale = st->mg_scale;
@@ -437,20 +430,16 @@ __global__ __launch_bounds__(kColThreads) void k_prolong_black(uint32_t n, const
 }
 
 // Red sweep z1 = (r + sum_n z1_n) / diag and the partials of r.z1 over all fluid cells.
-__global__ __launch_bounds__(kColThreads) void k_red_dot(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+__device__ __forceinline__ void red_dot_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
                                                          const float *diag, const float *r, float *z1,
                                                          double *partials, const SolverState *st)
 {
-    const ColBrick cb = col_brick(n);
+    const ColBrick cb = col_brick(G, n);
     const Brick B = load_brick(binfo, cb.live ? cb.e : 0u);
     const uint32_t done = st->done;
     issue_point();
     if (done) return;
-    if (!cb.live) {
-        double none[1] = {};
-        col_partial<1>(none, partials);
-        return;
-    }
+    if (!cb.live) return;
     const Col C = make_col(B, threadIdx.x);
     float zv[10], hv[8], rv[8], gv[8];
     uint32_t m[8];
@@ -481,7 +470,7 @@ __global__ __launch_bounds__(kColThreads) void k_red_dot(uint32_t n, const uint3
         }
         v[0] += (double)rv[k] * (double)xv;
     }
-    col_partial<1>(v, partials);
+    col_partial<1>(v, partials, cb.e, C.lane);
 }
 
 // d_out = z1 + beta d_in (beta = 0 for the first direction) fused with q = A d_out: the neighbours' new
@@ -491,23 +480,19 @@ __global__ __launch_bounds__(kColThreads) void k_red_dot(uint32_t n, const uint3
 // in FP32). kFirst: the first direction d = z1 (a template parameter so that the loads of d_in are not split into
 // branches of their own).
 template <bool kFirst>
-__global__ __launch_bounds__(kColThreads) void k_direction(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+__device__ __forceinline__ void direction_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
                                                            const float *diag, const float *z1, const float *d_in,
                                                            float *d_out, float *q, const float *r, double *partials,
                                                            const SolverState *st)
 {
     constexpr bool first = kFirst;
-    const ColBrick cb = col_brick(n);
+    const ColBrick cb = col_brick(G, n);
     const Brick B = load_brick(binfo, cb.live ? cb.e : 0u);
     const uint32_t done = st->done;
     const double beta = first ? 0.0 : st->beta;
     issue_point();
     if (done) return;
-    if (!cb.live) {
-        double none[4] = {};
-        col_partial<4>(none, partials);
-        return;
-    }
+    if (!cb.live) return;
     const Col C = make_col(B, threadIdx.x);
     auto dir = [&](float zv, float dv) { return first ? zv : (float)((double)zv + beta * (double)dv); };
     float dd[10], hv[8], gv[8], rv[8];
@@ -544,7 +529,7 @@ __global__ __launch_bounds__(kColThreads) void k_direction(uint32_t n, const uin
         v[2] += (double)qf * (double)qf;
         v[3] += (double)rv[k] * (double)rv[k];
     }
-    col_partial<4>(v, partials);
+    col_partial<4>(v, partials, cb.e, C.lane);
 }
 
 // ---- passes on the FP64 pressure, in column form ----
@@ -587,16 +572,12 @@ __device__ __forceinline__ void load_pres_col(const Col &C, const uint8_t *fmask
 
 // r = b - A p (FP64 p) over every cell of an allocated brick (0 off the fluid cells); partials ||b||^2, ||r||^2 and
 // the fluid cell count.
-__global__ __launch_bounds__(kColThreads) void k_pcg_init(uint32_t n, const uint32_t *binfo, const uint8_t *fmask, const float *div,
+__device__ __forceinline__ void pcg_init_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask, const float *div,
                                                           const float *diag, const double *pres, float *r,
                                                           double *partials)
 {
-    const ColBrick cb = col_brick(n);
-    if (!cb.live) {
-        double none[3] = {};
-        col_partial<3>(none, partials);
-        return;
-    }
+    const ColBrick cb = col_brick(G, n);
+    if (!cb.live) return;
     const Col C = make_col(load_brick(binfo, cb.e), threadIdx.x);
     PresCol P;
     load_pres_col(C, fmask, div, diag, pres, P);
@@ -620,16 +601,16 @@ __global__ __launch_bounds__(kColThreads) void k_pcg_init(uint32_t n, const uint
         v[0] += (double)P.div[k] * (double)P.div[k];
         v[1] += (double)ri * (double)ri;
     }
-    col_partial<3>(v, partials);
+    col_partial<3>(v, partials, cb.e, C.lane);
 }
 
 // Red-black SOR sweep of one colour on the FP64 pressure: p += w ((sum p_n - div) / diag - p). The neighbours are of
 // the other colour, which the sweep does not write.
-__global__ __launch_bounds__(kColThreads) void k_sor_sweep(uint32_t n, const uint32_t *binfo, const uint8_t *fmask, const float *div,
+__device__ __forceinline__ void sor_sweep_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask, const float *div,
                                                            const float *diag, double *pres, uint32_t colour, double w,
                                                            const SolverState *st)
 {
-    const ColBrick cb = col_brick(n);
+    const ColBrick cb = col_brick(G, n);
     const Brick B = load_brick(binfo, cb.live ? cb.e : 0u);
     const uint32_t done = st->done;
     issue_point();
@@ -655,18 +636,14 @@ __global__ __launch_bounds__(kColThreads) void k_sor_sweep(uint32_t n, const uin
 
 // Partials ||b||^2 and ||b - A p||^2: mode 0 the SOR convergence check (skipped once the solver is done), 1 always,
 // 2 only once the solver is done (the true-residual check of a finished MGPCG solve).
-__global__ __launch_bounds__(kColThreads) void k_sor_residual(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+__device__ __forceinline__ void sor_residual_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
                                                               const float *div, const float *diag, const double *pres,
                                                               double *partials, const SolverState *st, int mode)
 {
     const int32_t done = st->done;
     if ((mode == 0 && done) || (mode == 2 && !done)) return;
-    const ColBrick cb = col_brick(n);
-    if (!cb.live) {
-        double none[2] = {};
-        col_partial<2>(none, partials);
-        return;
-    }
+    const ColBrick cb = col_brick(G, n);
+    if (!cb.live) return;
     const Col C = make_col(load_brick(binfo, cb.e), threadIdx.x);
     PresCol P;
     load_pres_col(C, fmask, div, diag, pres, P);
@@ -684,7 +661,68 @@ __global__ __launch_bounds__(kColThreads) void k_sor_residual(uint32_t n, const 
         v[0] += (double)P.div[k] * (double)P.div[k];
         v[1] += ri * ri;
     }
-    col_partial<2>(v, partials);
+    col_partial<2>(v, partials, cb.e, C.lane);
+}
+
+// ---- single-tank launches of the column passes ----
+template <bool kFirst>
+__global__ __launch_bounds__(kColMaxThreads) void k_update_black(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+                                                              const float *diag, double *pres, const float *r_in,
+                                                              float *r_out, float *z, const float *d, const float *q,
+                                                              const SolverState *st)
+{
+    update_black_body<kFirst>(this_blk(), n, binfo, fmask, diag, pres, r_in, r_out, z, d, q, st);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void k_restrict0(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+                                                           const float *diag, const float *xf, const float *bf,
+                                                           PLevel L1, const SolverState *st)
+{
+    restrict0_body(this_blk(), n, binfo, fmask, diag, xf, bf, L1, st);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void k_prolong_black(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+                                                               const float *diag, const float *r, const float *z,
+                                                               float *z1, PLevel L1, const SolverState *st)
+{
+    prolong_black_body(this_blk(), n, binfo, fmask, diag, r, z, z1, L1, st);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void k_red_dot(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+                                                         const float *diag, const float *r, float *z1,
+                                                         double *partials, const SolverState *st)
+{
+    red_dot_body(this_blk(), n, binfo, fmask, diag, r, z1, partials, st);
+}
+
+template <bool kFirst>
+__global__ __launch_bounds__(kColMaxThreads) void k_direction(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+                                                           const float *diag, const float *z1, const float *d_in,
+                                                           float *d_out, float *q, const float *r, double *partials,
+                                                           const SolverState *st)
+{
+    direction_body<kFirst>(this_blk(), n, binfo, fmask, diag, z1, d_in, d_out, q, r, partials, st);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void k_pcg_init(uint32_t n, const uint32_t *binfo, const uint8_t *fmask, const float *div,
+                                                          const float *diag, const double *pres, float *r,
+                                                          double *partials)
+{
+    pcg_init_body(this_blk(), n, binfo, fmask, div, diag, pres, r, partials);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void k_sor_sweep(uint32_t n, const uint32_t *binfo, const uint8_t *fmask, const float *div,
+                                                           const float *diag, double *pres, uint32_t colour, double w,
+                                                           const SolverState *st)
+{
+    sor_sweep_body(this_blk(), n, binfo, fmask, div, diag, pres, colour, w, st);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void k_sor_residual(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+                                                              const float *div, const float *diag, const double *pres,
+                                                              double *partials, const SolverState *st, int mode)
+{
+    sor_residual_body(this_blk(), n, binfo, fmask, div, diag, pres, partials, st, mode);
 }
 
 // ---- flexible CG (Notay's FCG(1)) with a nonlinear preconditioner z = M(r) ----
@@ -794,10 +832,10 @@ __global__ __launch_bounds__(kThreads) void k_error_partials(const uint32_t *bin
 #define L0_LAUNCH(kid, kernel, ...)                                                                              \
     launch_timed(c, slosh::kid, kernel, dim3(c->nsolve), dim3(kThreads), 0u, c->binfo_f, c->fmask, __VA_ARGS__)
 // the column-form passes: one wavefront per brick holding fluid
-inline uint32_t col_blocks(uint32_t n) { return (n + kColBricks - 1u) / kColBricks; }
+inline uint32_t col_blocks(const slosh_ctx *c, uint32_t n) { return (n + c->col_bricks - 1u) / c->col_bricks; }
 #define L0C_LAUNCH(kid, kernel, ...)                                                                             \
-    launch_timed(c, slosh::kid, kernel, dim3(col_blocks(c->nsolve)), dim3(kColThreads), 0u, c->nsolve, c->binfo_f,   \
-                 c->fmask, __VA_ARGS__)
+    launch_timed(c, slosh::kid, kernel, dim3(col_blocks(c, c->nsolve)), dim3(64u * c->col_bricks), 0u, c->nsolve,    \
+                 c->binfo_f, c->fmask, __VA_ARGS__)
 
 void l0_brick_info(slosh_ctx *c, bool readback)
 {
@@ -820,9 +858,9 @@ void l0_brick_info(slosh_ctx *c, bool readback)
 // Over every allocated brick (non-fluid cells get r = 0, also in bricks without fluid).
 void l0_pcg_init(slosh_ctx *c, float *r_out)
 {
-    c->nparts = col_blocks(c->nactive);
-    launch_timed(c, K_PCG_INIT, k_pcg_init, dim3(c->nparts), dim3(kColThreads), 0, c->nactive, c->binfo, c->fmask, c->div,
-                 c->diag, c->pres, r_out, c->partials);
+    c->nparts = c->nactive;
+    launch_timed(c, K_PCG_INIT, k_pcg_init, dim3(col_blocks(c, c->nactive)), dim3(64u * c->col_bricks), 0, c->nactive,
+                 c->binfo, c->fmask, c->div, c->diag, c->pres, r_out, c->partials);
 }
 
 void l0_update_black(slosh_ctx *c, const float *r_in, float *r_out, const float *d, int first)
@@ -843,13 +881,13 @@ void l0_prolong_black(slosh_ctx *c, const float *r)
 
 void l0_red_dot(slosh_ctx *c, const float *r)
 {
-    c->nparts = col_blocks(c->nsolve);
+    c->nparts = c->nsolve;
     L0C_LAUNCH(K_RED_DOT, k_red_dot, c->diag, r, c->z1, c->partials, c->state);
 }
 
 void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, const float *r, int first)
 {
-    c->nparts = col_blocks(c->nsolve);
+    c->nparts = c->nsolve;
     if (first) L0C_LAUNCH(K_DIRECTION, k_direction<true>, c->diag, c->z1, d_in, d_out, c->q, r, c->partials, c->state);
     else L0C_LAUNCH(K_DIRECTION, k_direction<false>, c->diag, c->z1, d_in, d_out, c->q, r, c->partials, c->state);
 }
@@ -879,7 +917,7 @@ void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w)
 
 void l0_sor_residual(slosh_ctx *c)
 {
-    c->nparts = col_blocks(c->nsolve);
+    c->nparts = c->nsolve;
     L0C_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 0);
 }
 
@@ -896,13 +934,13 @@ void l0_error_partials(slosh_ctx *c, const double *p_ref)
 
 void l0_true_residual(slosh_ctx *c)
 {
-    c->nparts = col_blocks(c->nsolve);
+    c->nparts = c->nsolve;
     L0C_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 1);
 }
 
 void l0_true_residual_if_done(slosh_ctx *c)
 {
-    c->nparts = col_blocks(c->nsolve);
+    c->nparts = c->nsolve;
     L0C_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 2);
 }
 

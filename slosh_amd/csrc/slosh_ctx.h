@@ -122,6 +122,13 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t w, uint32_t n)
     return k * q + (k < r ? k : r) + j;
 }
 
+// A workgroup's index among the workgroups of its pass and their count: blockIdx.x and gridDim.x of a single-tank
+// launch; a launch over a batch of tanks gives each tank's workgroups their own range.
+struct Blk {
+    uint32_t b, nb;
+};
+__device__ __forceinline__ Blk this_blk() { return Blk{blockIdx.x, gridDim.x}; }
+
 // Stencil word of a coarse cell: bits 0-5 fluid neighbours (-x +x -y +y -z +z), bit 6 the cell is fluid, bit 7 red
 // ((x + y + z) even), bits 8-11 the number of non-solid neighbours (the coarse diagonal without the 1e-4).
 constexpr uint32_t kInfoSelf = 1u << 6;
@@ -302,6 +309,7 @@ struct slosh_ctx {
     uint32_t *coarse_sig = nullptr; // the coarsest level's cell types and indices at its last factorisation
     double *partials = nullptr;
     uint32_t nparts = 0;              // partial sets the last reduction pass wrote (what the next finalize sums)
+    uint32_t col_bricks = 4;          // bricks (wavefronts) per workgroup of the level-0 column passes: 1, 2 or 4
     uint16_t *dense_info = nullptr;   // stencil words of all dense levels, level by level
     uint32_t *dense_coord = nullptr;  // their packed cell coordinates
     slosh::SolverState *state = nullptr;
