@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SLOSH_ABI_VERSION 1u
+#define SLOSH_ABI_VERSION 2u
 
 typedef struct slosh_ctx slosh_ctx; /* opaque; owns all device memory of one simulation domain */
 
@@ -382,15 +382,19 @@ slosh_status slosh_unet_read_capture(slosh_ctx *ctx, uint32_t k, float *dst, uin
 slosh_status slosh_unet_train(slosh_ctx *ctx, const slosh_unet_train_config *cfg, slosh_unet_train_report *out);
 
 /* ---- batches of independent tanks (no reference counterpart: the reference runs one volume per proxy) ----------
- * A batch steps existing contexts together: tanks run on several host threads at once, each on its own stream,
- * longest first; solver iteration chunks replay as HIP graphs. The configuration changes timing only: every tank's
- * results equal its single-tank run bit for bit. */
+ * A batch steps existing contexts together. Mode 0: every tank's whole step runs on its own stream, tanks on several
+ * host threads at once, longest first. Mode 1 (lockstep): the passes before and after the pressure solve run so,
+ * while the MGPCG iterations of all tanks run in lockstep on one stream, each pass of an iteration one launch over
+ * every tank's bricks (tanks whose solve cannot join, e.g. another solver, solve on their own stream). Solver
+ * iteration chunks replay as HIP graphs. The configuration changes timing only: every tank's results equal its
+ * single-tank run bit for bit. */
 typedef struct slosh_batch slosh_batch;
 typedef struct slosh_batch_config {
-    uint32_t workers;      /* tanks stepped at once (1..64) */
+    uint32_t workers;      /* tanks stepped at once on host threads (1..64) */
     uint32_t solver_chunk; /* MGPCG iterations per read of the solver state (even, 2..32) */
     uint32_t graphs;       /* 1: replay iteration chunks as captured HIP graphs */
-    uint32_t reserved;
+    uint32_t mode;         /* 0: each tank on its own stream; 1: lockstep MGPCG iterations over all tanks */
+    uint32_t col_bricks;   /* bricks (wavefronts) per workgroup of the level-0 solver passes: 1, 2 or 4 */
 } slosh_batch_config;
 slosh_status slosh_batch_create(slosh_ctx *const *tanks, uint32_t n, slosh_batch **out);
 void slosh_batch_destroy(slosh_batch *batch);

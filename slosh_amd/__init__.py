@@ -339,18 +339,20 @@ class Batch:
         g, ptr = self._grav(gravity)
         _check(self.lib.slosh_batch_step(self.h, float(dt), ptr), "slosh_batch_step")
 
-    def set_config(self, workers, solver_chunk=4, graphs=1):
-        cfg = abi.BatchConfig(int(workers), int(solver_chunk), int(graphs), 0)
+    def set_config(self, workers, solver_chunk=4, graphs=1, mode=0, col_bricks=4):
+        cfg = abi.BatchConfig(int(workers), int(solver_chunk), int(graphs), int(mode), int(col_bricks))
         _check(self.lib.slosh_batch_set_config(self.h, C.byref(cfg)), "slosh_batch_set_config")
 
     def config(self):
         cfg = abi.BatchConfig()
         _check(self.lib.slosh_batch_get_config(self.h, C.byref(cfg)), "slosh_batch_get_config")
-        return {"workers": cfg.workers, "solver_chunk": cfg.solver_chunk, "graphs": cfg.graphs}
+        return {"workers": cfg.workers, "solver_chunk": cfg.solver_chunk, "graphs": cfg.graphs, "mode": cfg.mode,
+                "col_bricks": cfg.col_bricks}
 
     def autotune(self, dt, gravity, steps_per_candidate=1):
         g, ptr = self._grav(gravity)
         cfg = abi.BatchConfig()
         _check(self.lib.slosh_batch_autotune(self.h, float(dt), ptr, int(steps_per_candidate), C.byref(cfg)),
                "slosh_batch_autotune")
-        return {"workers": cfg.workers, "solver_chunk": cfg.solver_chunk, "graphs": cfg.graphs}
+        return {"workers": cfg.workers, "solver_chunk": cfg.solver_chunk, "graphs": cfg.graphs, "mode": cfg.mode,
+                "col_bricks": cfg.col_bricks}

@@ -276,7 +276,7 @@ class KernelTime(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("workers", C.c_uint32), ("solver_chunk", C.c_uint32), ("graphs", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("mode", C.c_uint32), ("col_bricks", C.c_uint32)]
 
 
 class UNetTrainConfig(C.Structure):

@@ -101,6 +101,7 @@ void free_all(slosh_ctx *c)
     if (c->ev_solve0) (void)hipEventDestroy(c->ev_solve0);
     if (c->ev_solve1) (void)hipEventDestroy(c->ev_solve1);
     if (c->ev_counts) (void)hipEventDestroy(c->ev_counts);
+    if (c->ev_batch) (void)hipEventDestroy(c->ev_batch);
     if (c->ev_pool) (void)hipEventDestroy(c->ev_pool);
     if (c->mg_graph) (void)hipGraphExecDestroy(c->mg_graph);
     c->mg_graph = nullptr;
@@ -160,6 +161,7 @@ void create_impl(slosh_ctx *c)
     SLOSH_HIP(hipEventCreate(&c->ev_solve0));
     SLOSH_HIP(hipEventCreate(&c->ev_solve1));
     SLOSH_HIP(hipEventCreateWithFlags(&c->ev_counts, hipEventDisableTiming));
+    SLOSH_HIP(hipEventCreateWithFlags(&c->ev_batch, hipEventDisableTiming));
     SLOSH_HIP(hipEventCreateWithFlags(&c->ev_pool, hipEventDisableTiming));
     Geo &g = c->geo;
     for (int a = 0; a < 3; ++a) {
@@ -410,6 +412,26 @@ bool run_solver(slosh_ctx *c)
     return solved;
 }
 
+// A lockstep batch's solve of this tank, before the batched iterations: the solver's set-up (run_solver's first
+// part). Returns whether the solve needs iterations (false: no fluid).
+bool batch_solve_begin(slosh_ctx *c)
+{
+    SLOSH_HIP(hipEventRecord(c->ev_solve0, c->stream));
+    c->fcg_iters = 0;
+    c->fcg_fallback = 0;
+    return mgpcg_begin(c);
+}
+
+// After the batched iterations (the tank's stream waits for them): a restart when the FP64 true residual missed the
+// tolerance, then run_solver's last part.
+void batch_solve_end(slosh_ctx *c, bool iterated)
+{
+    if (iterated) mgpcg_resume(c);
+    SLOSH_HIP(hipEventRecord(c->ev_solve1, c->stream));
+    c->res.fallback_used = c->fcg_fallback;
+    SLOSH_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+}
+
 // The solver's figures in the results, once the stream has passed run_solver's copy of the state.
 void solver_results(slosh_ctx *c)
 {
@@ -500,9 +522,14 @@ bool load_system_impl(slosh_ctx *c, const uint8_t *types, const float *diag, con
     return true;
 }
 
-// One substep in the order of SloshProxy.cpp:544-713, stopping after `stop` (SLOSH_STAGE_*).
-void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
+// One substep in the order of SloshProxy.cpp:544-713, stopping after `stop` (SLOSH_STAGE_*): the passes before the
+// pressure solve (step_front), the solve, the passes after it (step_back). A batch in lockstep mode runs the solves
+// of its tanks between the two halves.
+void step_front(slosh_ctx *c, float dt, const float gv[3], int stop)
 {
+    c->st_dt = dt;
+    c->st_stop = stop;
+    for (int a = 0; a < 3; ++a) c->st_g[a] = gv[a];
     c->frame++;
     const float warm = c->prev_dt > 0.0f ? dt / c->prev_dt : 1.0f;
     const float dt_prev = c->prev_dt;
@@ -558,8 +585,13 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
         if (alpha > 0.0f && c->s.viscosity_iterations > 0) run_viscosity(c, alpha, c->s.viscosity_iterations);
     }
     if (stop > SLOSH_STAGE_NORMALIZE) launch_divergence(c, dt);
-    bool solved = false;
-    if (stop > SLOSH_STAGE_DIVERGENCE) solved = run_solver(c);
+}
+
+void step_back(slosh_ctx *c, bool solved)
+{
+    const float dt = c->st_dt;
+    const float *gv = c->st_g;
+    const int stop = c->st_stop;
     if (stop > SLOSH_STAGE_SOLVE) launch_project(c);
     // collider coupling of this step (SloshProxy.cpp:397 clears it per frame; one step is one substep here): the
     // contact and wetted-face terms of the projection, the smoothed force (Slosh.usf:1138-1216) and the submerged
@@ -620,6 +652,14 @@ void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
     float ms = 0.0f;
     SLOSH_HIP(hipEventElapsedTime(&ms, c->ev_step0, c->ev_step1));
     c->res.step_ms = ms;
+}
+
+void step_impl(slosh_ctx *c, float dt, const float gv[3], int stop)
+{
+    step_front(c, dt, gv, stop);
+    bool solved = false;
+    if (stop > SLOSH_STAGE_DIVERGENCE) solved = run_solver(c);
+    step_back(c, solved);
 }
 
 // Pool -> dense conversion on the host (x fastest; unallocated cells read 0).
@@ -791,6 +831,14 @@ void *scratch_of(int device)
 }
 
 } // namespace
+
+namespace slosh {
+void tank_step_front(slosh_ctx *c, float dt, const float gv[3]) { ::step_front(c, dt, gv, SLOSH_STAGE_ALL); }
+void tank_step_back(slosh_ctx *c, bool solved) { ::step_back(c, solved); }
+bool tank_solve(slosh_ctx *c) { return ::run_solver(c); }
+bool tank_solve_begin(slosh_ctx *c) { return ::batch_solve_begin(c); }
+void tank_solve_end(slosh_ctx *c, bool iterated) { ::batch_solve_end(c, iterated); }
+} // namespace slosh
 
 #define ABI_GUARD(...)                                                                                           \
     try {                                                                                                        \

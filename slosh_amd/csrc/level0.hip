@@ -725,6 +725,59 @@ __global__ __launch_bounds__(kColMaxThreads) void k_sor_residual(uint32_t n, con
     sor_residual_body(this_blk(), n, binfo, fmask, div, diag, pres, partials, st, mode);
 }
 
+// ---- the column passes over a batch of tanks (one launch for all of them) ----
+template <bool kFirst>
+__global__ __launch_bounds__(kColMaxThreads) void kb_update_black(const BatchTank *T, const uint32_t *range,
+                                                                  uint32_t nt, int ri, int ro, int di)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    update_black_body<kFirst>(G, B.nsolve, B.binfo_f, B.fmask, B.diag, B.pres, B.r[ri], B.r[ro], B.z, B.d[di], B.q,
+                              B.state);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void kb_restrict0(const BatchTank *T, const uint32_t *range,
+                                                               uint32_t nt, int ri)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    restrict0_body(G, B.nsolve, B.binfo_f, B.fmask, B.diag, B.z, B.r[ri], B.L1, B.state);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void kb_prolong_black(const BatchTank *T, const uint32_t *range,
+                                                                   uint32_t nt, int ri)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    prolong_black_body(G, B.nsolve, B.binfo_f, B.fmask, B.diag, B.r[ri], B.z, B.z1, B.L1, B.state);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void kb_red_dot(const BatchTank *T, const uint32_t *range, uint32_t nt,
+                                                             int ri)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    red_dot_body(G, B.nsolve, B.binfo_f, B.fmask, B.diag, B.r[ri], B.z1, B.partials, B.state);
+}
+
+template <bool kFirst>
+__global__ __launch_bounds__(kColMaxThreads) void kb_direction(const BatchTank *T, const uint32_t *range,
+                                                               uint32_t nt, int din, int dout, int ri)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    direction_body<kFirst>(G, B.nsolve, B.binfo_f, B.fmask, B.diag, B.z1, B.d[din], B.d[dout], B.q, B.r[ri],
+                           B.partials, B.state);
+}
+
+__global__ __launch_bounds__(kColMaxThreads) void kb_true_residual_if_done(const BatchTank *T, const uint32_t *range,
+                                                                           uint32_t nt)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    sor_residual_body(G, B.nsolve, B.binfo_f, B.fmask, B.div, B.diag, B.pres, B.partials, B.state, 2);
+}
+
 // ---- flexible CG (Notay's FCG(1)) with a nonlinear preconditioner z = M(r) ----
 // Partials of z . q_prev (the orthogonalisation coefficient of the next direction).
 __global__ __launch_bounds__(kThreads) void k_fcg_zq(const uint32_t *binfo, const uint8_t *fmask, const float *z,
@@ -942,6 +995,35 @@ void l0_true_residual_if_done(slosh_ctx *c)
 {
     c->nparts = c->nsolve;
     L0C_LAUNCH(K_SOR_RESIDUAL, k_sor_residual, c->div, c->diag, c->pres, c->partials, c->state, 2);
+}
+
+// The column passes over a batch: the workgroup count of the column range, 64 threads per brick.
+#define L0B_LAUNCH(kernel, ...)                                                                                  \
+    hipLaunchKernelGGL(kernel, dim3(B.total[kRangeCol]), dim3(64u * B.col_bricks), 0u, B.stream, B.d_tanks,          \
+                       B.range(kRangeCol), B.ntanks, __VA_ARGS__)
+
+void b_update_black(const BatchRun &B, int ri, int ro, int di, bool first)
+{
+    if (first) L0B_LAUNCH(kb_update_black<true>, ri, ro, di);
+    else L0B_LAUNCH(kb_update_black<false>, ri, ro, di);
+}
+
+void b_restrict(const BatchRun &B, int ri) { L0B_LAUNCH(kb_restrict0, ri); }
+
+void b_prolong_black(const BatchRun &B, int ri) { L0B_LAUNCH(kb_prolong_black, ri); }
+
+void b_red_dot(const BatchRun &B, int ri) { L0B_LAUNCH(kb_red_dot, ri); }
+
+void b_direction(const BatchRun &B, int din, int dout, int ri, bool first)
+{
+    if (first) L0B_LAUNCH(kb_direction<true>, din, dout, ri);
+    else L0B_LAUNCH(kb_direction<false>, din, dout, ri);
+}
+
+void b_true_residual_if_done(const BatchRun &B)
+{
+    hipLaunchKernelGGL(kb_true_residual_if_done, dim3(B.total[kRangeCol]), dim3(64u * B.col_bricks), 0u, B.stream,
+                       B.d_tanks, B.range(kRangeCol), B.ntanks);
 }
 
 } // namespace slosh

@@ -172,8 +172,8 @@ template <int S> constexpr uint32_t pool_threads() { return S == 4 ? 256u : 64u;
 
 // Down pass of pooled level L (S = 8 >> l): black sweep, residual, restriction into pooled level C.
 template <int S>
-__global__ __launch_bounds__(pool_threads<S>()) void k_pdown_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
-                                                   const SolverState *st)
+__device__ __forceinline__ void pdown_body(const Blk &G, const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+                                           const SolverState *st)
 {
     using Bx = Box<S>;
     constexpr int S2 = S / 2, CPB2 = S2 * S2 * S2;
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(pool_threads<S>()) void k_pdown_lds(const uint32_t 
     __shared__ uint16_t s_info[Bx::NB];
     __shared__ float s_res[Bx::CPB];
     const uint32_t t = threadIdx.x;
-    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t e = xcd_remap(G.b, G.nb);
     if (t < 28u) s_nb[t] = binfo[(size_t)e * kBinfoWords + t];
     const uint32_t done = st->done;
     // the level-2 pass restricts into level 3 (dense by brick index): its target word is known from the brick index
@@ -257,8 +257,8 @@ __global__ __launch_bounds__(pool_threads<S>()) void k_pdown_lds(const uint32_t 
 
 // Up pass of pooled level L (S = 8 >> l) from pooled parent level C: prolongation, black sweep, red sweep.
 template <int S>
-__global__ __launch_bounds__(pool_threads<S>()) void k_pup_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
-                                                 const SolverState *st)
+__device__ __forceinline__ void pup_body(const Blk &G, const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+                                         const SolverState *st)
 {
     using Bx = Box<S>;
     constexpr int S2 = S / 2, CPB2 = S2 * S2 * S2;
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(pool_threads<S>()) void k_pup_lds(const uint32_t *b
     __shared__ float s_v[Bx::NB], s_b[Bx::NB];
     __shared__ uint16_t s_info[Bx::NB];
     const uint32_t t = threadIdx.x;
-    const uint32_t e = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t e = xcd_remap(G.b, G.nb);
     if (t < 28u) s_nb[t] = binfo[(size_t)e * kBinfoWords + t];
     const uint32_t done = st->done;
     __syncthreads();
@@ -332,6 +332,35 @@ __global__ __launch_bounds__(pool_threads<S>()) void k_pup_lds(const uint32_t *b
         L.x1[slot * Bx::CPB + k] =
             (info & kInfoRed) ? (s_b[kb] + box_sum<S>(s_v, kb, info)) / info_diag(info) : s_v[kb];
     }
+}
+
+template <int S>
+__global__ __launch_bounds__(pool_threads<S>()) void k_pdown_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+                                                                 const SolverState *st)
+{
+    pdown_body<S>(this_blk(), binfo, L, C, D, st);
+}
+
+template <int S>
+__global__ __launch_bounds__(pool_threads<S>()) void k_pup_lds(const uint32_t *binfo, PLevel L, PLevel C, DenseLevel D,
+                                                               const SolverState *st)
+{
+    pup_body<S>(this_blk(), binfo, L, C, D, st);
+}
+
+// The level-1 passes over a batch of tanks (one brick per workgroup, every tank in one launch).
+__global__ __launch_bounds__(pool_threads<4>()) void kb_pdown1(const BatchTank *T, const uint32_t *range, uint32_t nt)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    pdown_body<4>(G, B.binfo_f, B.L1, B.L2, DenseLevel{}, B.state);
+}
+
+__global__ __launch_bounds__(pool_threads<4>()) void kb_pup1(const BatchTank *T, const uint32_t *range, uint32_t nt)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    pup_body<4>(G, B.binfo_f, B.L1, B.L2, DenseLevel{}, B.state);
 }
 
 uint32_t pass_blocks(const slosh_ctx *c, const PLevel &L) { return blocks_for((uint64_t)c->nactive * L.cpb, 256); }
@@ -422,6 +451,18 @@ void pooled_up(slosh_ctx *c, int l)
     else
         hipLaunchKernelGGL(k_pup_lds<2>, dim3(c->nsolve), dim3(pool_threads<2>()), 0, c->stream, c->binfo_f, L, C, D,
                            c->state);
+}
+
+void b_pooled_down1(const BatchRun &B)
+{
+    hipLaunchKernelGGL(kb_pdown1, dim3(B.total[kRangeBrick]), dim3(pool_threads<4>()), 0u, B.stream, B.d_tanks,
+                       B.range(kRangeBrick), B.ntanks);
+}
+
+void b_pooled_up1(const BatchRun &B)
+{
+    hipLaunchKernelGGL(kb_pup1, dim3(B.total[kRangeBrick]), dim3(pool_threads<4>()), 0u, B.stream, B.d_tanks,
+                       B.range(kRangeBrick), B.ntanks);
 }
 
 } // namespace slosh

@@ -161,6 +161,57 @@ struct DenseLevel {
     LVec x, x1, b;
 };
 
+// The dense levels the coarse tail runs in LDS (one workgroup).
+struct TailArgs {
+    int n;                   // dense levels in the tail, the last one the coarsest
+    int sweeps;              // red-black smoothing pairs per visit of a level
+    uint32_t total;          // cells of all tail levels
+    const uint16_t *info;    // stencil words of the tail's levels, contiguous from the first one
+    const uint32_t *coord;   // their packed cell coordinates
+    uint32_t off[kMaxLevels];  // first cell of each tail level in the two arrays above and in LDS
+    DenseLevel L[kMaxLevels];  // per level: dimensions and cell count; b and the red-initialised x of the first
+                               // level are in memory
+};
+
+// One tank of a batched MGPCG solve: what its solver passes read, in device memory. The batched passes launch the
+// workgroups of every tank at once, each workgroup finding its tank from per-tank prefix sums of workgroup counts.
+struct BatchTank {
+    const uint32_t *binfo_f;
+    const uint8_t *fmask;
+    const float *diag, *div;
+    double *pres;
+    float *r[2], *z, *z1, *d[2], *q;
+    double *partials;
+    SolverState *state;
+    uint32_t nsolve, ntiles_f;
+    Geo geo;
+    const uint32_t *tile_list;
+    PLevel L1, L2;
+    DenseLevel L3, L4;
+    TailArgs tail;
+    const uint32_t *coarse_idx, *coarse_n;
+    const double *coarse_inv;
+};
+constexpr uint32_t kMaxBatchTanks = 256;
+// Workgroup ranges of the batched passes: column passes (bricks per workgroup as configured), one brick per
+// workgroup, one level-3 tile per workgroup, one workgroup per tank.
+enum BatchRange { kRangeCol = 0, kRangeBrick = 1, kRangeTile = 2, kRangeTank = 3, kBatchRanges = 4 };
+
+// The tank of workgroup b of a batched pass (range: the pass's prefix sums over the tanks, range[0] = 0) and the
+// workgroup's place among that tank's workgroups.
+__device__ __forceinline__ uint32_t batch_tank(const uint32_t *range, uint32_t ntanks, uint32_t b, Blk &G)
+{
+    uint32_t lo = 0, hi = ntanks; // range[lo] <= b < range[hi]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (range[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    G.b = b - range[lo];
+    G.nb = range[lo + 1] - range[lo];
+    return lo;
+}
+
 __device__ __forceinline__ bool pinside(const PLevel &L, int x, int y, int z)
 {
     return x >= 0 && y >= 0 && z >= 0 && x < L.dim[0] && y < L.dim[1] && z < L.dim[2];
@@ -310,6 +361,11 @@ struct slosh_ctx {
     double *partials = nullptr;
     uint32_t nparts = 0;              // partial sets the last reduction pass wrote (what the next finalize sums)
     uint32_t col_bricks = 4;          // bricks (wavefronts) per workgroup of the level-0 column passes: 1, 2 or 4
+    // the step between its two halves (step_front / step_back): dt, gravity and the last stage
+    float st_dt = 0.0f;
+    float st_g[3]{};
+    int st_stop = 0;
+    hipEvent_t ev_batch = nullptr;    // a lockstep batch waits on the tank's stream at this event
     uint16_t *dense_info = nullptr;   // stencil words of all dense levels, level by level
     uint32_t *dense_coord = nullptr;  // their packed cell coordinates
     slosh::SolverState *state = nullptr;
@@ -452,6 +508,49 @@ void l0_fcg_update(slosh_ctx *c, float *r, const float *d);
 void l0_error_partials(slosh_ctx *c, const double *p_ref);
 // out = r - A z over the fluid cells (the residual a preconditioner's result z leaves).
 void l0_fcg_resid(slosh_ctx *c, const float *r, const float *z, float *out);
+
+// ---- batched MGPCG iterations over many tanks (batch.hip drives them; each file launches its own passes) ----
+struct BatchRun {
+    hipStream_t stream = nullptr;
+    uint32_t ntanks = 0;
+    BatchTank *d_tanks = nullptr;
+    uint32_t *d_range = nullptr;     // [kBatchRanges][kMaxBatchTanks + 1] prefix sums of workgroups per tank
+    uint32_t total[kBatchRanges]{};  // workgroups of each range over the whole batch
+    uint32_t col_bricks = 4;         // bricks per workgroup of the column passes
+    const uint32_t *range(int k) const { return d_range + (size_t)k * (kMaxBatchTanks + 1u); }
+};
+// level0.hip: the column passes over every tank (buffer selectors: r[ri] in, r[ro] out, d[di], d[dout])
+void b_update_black(const BatchRun &B, int ri, int ro, int di, bool first);
+void b_restrict(const BatchRun &B, int ri);
+void b_prolong_black(const BatchRun &B, int ri);
+void b_red_dot(const BatchRun &B, int ri);
+void b_direction(const BatchRun &B, int din, int dout, int ri, bool first);
+void b_true_residual_if_done(const BatchRun &B);
+// pooled.hip: the level-1 down and up passes
+void b_pooled_down1(const BatchRun &B);
+void b_pooled_up1(const BatchRun &B);
+// solver.hip: the fused level-2/3 passes, the coarse tail, the finalize (one workgroup per tank)
+void b_d23down(const BatchRun &B);
+void b_d23up(const BatchRun &B);
+void b_coarse_tail(const BatchRun &B);
+void b_finalize(const BatchRun &B, int kind);
+
+// solver.hip: the MGPCG set-up of a solve (returns whether it needs iterations), whether a tank's solve can join a
+// lockstep batch, its descriptor, the batched iterations, and the single-tank continuation after them (restarts on
+// a missed true residual)
+bool mgpcg_begin(slosh_ctx *c);
+bool mgpcg_batchable(const slosh_ctx *c);
+void batch_tank_of(const slosh_ctx *c, BatchTank &T);
+void batch_iterate(BatchRun &B, slosh_ctx *const *tanks, uint32_t n, int chunk, bool graphs, hipGraphExec_t &graph,
+                   SolverState *d_states, SolverState *h_states);
+void mgpcg_resume(slosh_ctx *c);
+// ctx.hip: the halves of a full step around its pressure solve, the single-tank solve, and a batched solve's
+// per-tank ends
+void tank_step_front(slosh_ctx *c, float dt, const float gv[3]);
+void tank_step_back(slosh_ctx *c, bool solved);
+bool tank_solve(slosh_ctx *c);
+bool tank_solve_begin(slosh_ctx *c);
+void tank_solve_end(slosh_ctx *c, bool iterated);
 
 // ---- unet.hip ----
 size_t unet_param_count();

@@ -14,6 +14,7 @@
 // partial sum, and a one-workgroup finalize kernel sums the partials in a fixed order and updates the state. (A
 // grid-wide "last workgroup" ticket needs a device-scope fence per workgroup, which on gfx950's separate per-XCD
 // L2 caches costs more than the whole pass.)
+#include <algorithm>
 #include <cmath>
 
 #include "slosh_ctx.h"
@@ -64,8 +65,8 @@ __host__ __device__ constexpr int fin_values(int kind)
 // Sums the partials of the preceding reduction in a fixed order (per thread: its strided run in increasing order;
 // then wave_sum and the wavefront sums in order) and applies them to the solver state.
 constexpr uint32_t kFinThreads = 1024;
-__global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
-                                                          SolverState *st)
+__device__ __forceinline__ void finalize_body(int kind, const double *partials, uint32_t nblocks, int iters,
+                                              SolverState *st)
 {
     constexpr int kMaxV = 4;
     constexpr uint32_t kWaves = kFinThreads / 64u;
@@ -212,6 +213,19 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
         }
         st->hist[k] = rel;
     }
+}
+
+__global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double *partials, uint32_t nblocks, int iters,
+                                                          SolverState *st)
+{
+    finalize_body(kind, partials, nblocks, iters, st);
+}
+
+// The finalize of every tank of a batch (workgroup t: tank t, whose partials are per brick with fluid).
+__global__ __launch_bounds__(kFinThreads) void kb_finalize(int kind, const BatchTank *T)
+{
+    const BatchTank &B = T[blockIdx.x];
+    finalize_body(kind, B.partials, B.nsolve, 0, B.state);
 }
 
 // ---- dense levels (>= 4) ----------------------------------------------------------------------------------
@@ -381,8 +395,8 @@ __device__ __forceinline__ uint32_t l2_index(const PLevel &L2, const uint32_t *s
 // tile's level-3 cells and their 2-cell halo, the restriction into level 3 with its fused red sweep (MGRestrictCS),
 // the level-3 black sweep and residual, and the restriction into the tile's 2^3 level-4 cells. Stored: the level-2
 // black values of the tile's cells, level 3's b and x on the tile, level 4's b and x.
-__global__ __launch_bounds__(kD23DownThreads) void k_d23down(Geo g, const uint32_t *tiles, PLevel L2, DenseLevel L3,
-                                                             DenseLevel L4, const SolverState *st)
+__device__ __forceinline__ void d23down_body(const Blk &G, Geo g, const uint32_t *tiles, PLevel L2, DenseLevel L3,
+                                             DenseLevel L4, const SolverState *st)
 {
     constexpr uint32_t NT = kD23DownThreads;
     constexpr int kPer = (kDNB2 + (int)NT - 1) / (int)NT;
@@ -394,7 +408,7 @@ __global__ __launch_bounds__(kD23DownThreads) void k_d23down(Geo g, const uint32
     __shared__ uint16_t s_i3[kDR3 * kDR3 * kDR3];
     __shared__ float s_r3[kT3 * kT3 * kT3];
     const uint32_t t = threadIdx.x;
-    const uint32_t tile = tiles[blockIdx.x];
+    const uint32_t tile = tiles[G.b];
     const uint32_t done = st->done;
     issue_point();
     if (done) return;
@@ -530,8 +544,8 @@ __global__ __launch_bounds__(kD23DownThreads) void k_d23down(Geo g, const uint32
 // prolonged iterate x + x4(parent) on the tile and a 3-cell halo, its black sweep and red sweep, which give level 3's
 // result on the tile and a 1-cell halo; level 2's prolonged iterate from it, its black sweep and red sweep; level 2's
 // result x1 on the tile's cells goes back to memory.
-__global__ __launch_bounds__(kD23UpThreads) void k_d23up(Geo g, const uint32_t *tiles, PLevel L2, DenseLevel L3,
-                                                         DenseLevel L4, const SolverState *st)
+__device__ __forceinline__ void d23up_body(const Blk &G, Geo g, const uint32_t *tiles, PLevel L2, DenseLevel L3,
+                                           DenseLevel L4, const SolverState *st)
 {
     constexpr uint32_t NT = kD23UpThreads;
     constexpr int kN3 = kUB3 * kUB3 * kUB3, kN2 = kUB2 * kUB2 * kUB2;
@@ -542,7 +556,7 @@ __global__ __launch_bounds__(kD23UpThreads) void k_d23up(Geo g, const uint32_t *
     __shared__ float s_v2[kN2], s_b2[kN2];
     __shared__ uint16_t s_i2[kN2];
     const uint32_t t = threadIdx.x;
-    const uint32_t tile = tiles[blockIdx.x];
+    const uint32_t tile = tiles[G.b];
     const uint32_t done = st->done;
     issue_point();
     if (done) return;
@@ -651,6 +665,33 @@ __global__ __launch_bounds__(kD23UpThreads) void k_d23up(Geo g, const uint32_t *
     }
 }
 
+__global__ __launch_bounds__(kD23DownThreads) void k_d23down(Geo g, const uint32_t *tiles, PLevel L2, DenseLevel L3,
+                                                             DenseLevel L4, const SolverState *st)
+{
+    d23down_body(this_blk(), g, tiles, L2, L3, L4, st);
+}
+
+__global__ __launch_bounds__(kD23UpThreads) void k_d23up(Geo g, const uint32_t *tiles, PLevel L2, DenseLevel L3,
+                                                         DenseLevel L4, const SolverState *st)
+{
+    d23up_body(this_blk(), g, tiles, L2, L3, L4, st);
+}
+
+// The fused level-2/3 passes over a batch of tanks (one level-3 tile per workgroup).
+__global__ __launch_bounds__(kD23DownThreads) void kb_d23down(const BatchTank *T, const uint32_t *range, uint32_t nt)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    d23down_body(G, B.geo, B.tile_list, B.L2, B.L3, B.L4, B.state);
+}
+
+__global__ __launch_bounds__(kD23UpThreads) void kb_d23up(const BatchTank *T, const uint32_t *range, uint32_t nt)
+{
+    Blk G;
+    const BatchTank &B = T[batch_tank(range, nt, blockIdx.x, G)];
+    d23up_body(G, B.geo, B.tile_list, B.L2, B.L3, B.L4, B.state);
+}
+
 // Tiles of level 3 whose bricks hold fluid cells (fluid on levels 2 and 3 needs a fluid cell below it in the same
 // brick under either coarsening rule): flag per tile, set by the bricks (the flags start at 0).
 __global__ void k_tile_flags(Geo g, uint32_t nactive, const uint32_t *fluid_flag, DenseLevel L3, uint32_t *flag)
@@ -677,16 +718,6 @@ __global__ void k_dprolong(DenseLevel F, DenseLevel C, const SolverState *st)
     if (i < F.cells) prolong_cell(F, C, i);
 }
 
-struct TailArgs {
-    int n;                   // dense levels in the tail, the last one the coarsest
-    int sweeps;              // red-black smoothing pairs per visit of a level
-    uint32_t total;          // cells of all tail levels
-    const uint16_t *info;    // stencil words of the tail's levels, contiguous from the first one
-    const uint32_t *coord;   // their packed cell coordinates
-    uint32_t off[kMaxLevels];  // first cell of each tail level in the two arrays above and in LDS
-    DenseLevel L[kMaxLevels];  // per level: dimensions and cell count; b and the red-initialised x of the first
-                               // level are in memory
-};
 
 // Cells per thread of any tail pass (1024 threads).
 constexpr int kTailPer = (int)((kTailCells + 1023u) / 1024u);
@@ -815,8 +846,8 @@ __device__ __forceinline__ void tail_prolong(const TailLds &S, const TailLevel &
 // is issued at the start (solver flag, stencil words, coordinates, the coarsest inverse, the first level's inputs);
 // the passes then visit only fluid cells, from per-level lists built once; only the first level's result goes back
 // to memory.
-__global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t *cidx,
-                                                      const uint32_t *cn, const double *inv, const SolverState *st)
+__device__ __forceinline__ void coarse_tail_body(const TailArgs &T, const uint32_t *cidx, const uint32_t *cn,
+                                                 const double *inv, const SolverState *st)
 {
     constexpr int N = (int)kCoarsestCells;
     __shared__ uint16_t s_info[kTailCells];
@@ -921,6 +952,19 @@ __global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t
         const uint32_t i = tid + 1024u * (uint32_t)u;
         if (i < V0.cells) T.L[0].x[i] = s_x[i];
     }
+}
+
+__global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs T, const uint32_t *cidx, const uint32_t *cn,
+                                                      const double *inv, const SolverState *st)
+{
+    coarse_tail_body(T, cidx, cn, inv, st);
+}
+
+// The coarse tail of every tank of a batch (workgroup t: tank t).
+__global__ __launch_bounds__(1024) void kb_coarse_tail(const BatchTank *T)
+{
+    const BatchTank &B = T[blockIdx.x];
+    coarse_tail_body(B.tail, B.coarse_idx, B.coarse_n, B.coarse_inv, B.state);
 }
 
 // ---- coarsest level: FP64 explicit inverse of the <= 64 fluid unknowns' operator (once per solve) ----
@@ -1091,6 +1135,33 @@ void coarse_solve(slosh_ctx *c, LVec b, LVec x)
                  x, c->state);
 }
 
+// The coarse tail's view of dense levels t .. m-1.
+TailArgs tail_args(const slosh_ctx *c, int t)
+{
+    const int m = c->nlevels;
+    TailArgs T{};
+    T.n = m - t;
+    T.sweeps = coarse_sweeps(c);
+    T.info = c->dense_info + c->lv[t].dense_off;
+    T.coord = c->dense_coord + c->lv[t].dense_off;
+    for (int k = 0; k < T.n; ++k) {
+        T.L[k] = dlevel_of(c, t + k);
+        T.off[k] = c->lv[t + k].dense_off - c->lv[t].dense_off; // levels are contiguous in the blocks
+        T.total = T.off[k] + T.L[k].cells;
+    }
+    return T;
+}
+
+// The first dense level the coarse tail takes (the coarser ones that fit its LDS together), for m >= 5 levels.
+int tail_start(const slosh_ctx *c)
+{
+    const int m = c->nlevels;
+    int t = m - 1;
+    uint32_t tail_cells = c->lv[m - 1].cells;
+    while (t - 1 >= kFirstDense && tail_cells + c->lv[t - 1].cells <= kTailCells) tail_cells += c->lv[--t].cells;
+    return t;
+}
+
 // Levels 2 .. m-1 (m >= 5): the fused level-2/3 down pass with the restriction into level 4, grid-wide passes for
 // dense levels kFirstDense .. t-1 (only when the dense levels do not all fit the tail's LDS), the tail on levels
 // t .. m-1, the grid-wide up passes, the fused level-3/2 up pass.
@@ -1103,9 +1174,7 @@ void dense_part(slosh_ctx *c)
         launch_timed(c, K_D3DOWN, k_d23down, dim3(c->ntiles_f), dim3(kD23DownThreads), 0, c->geo, c->tile_list, L2,
                      L3, L4, c->state);
     }
-    int t = m - 1;
-    uint32_t tail_cells = c->lv[m - 1].cells;
-    while (t - 1 >= kFirstDense && tail_cells + c->lv[t - 1].cells <= kTailCells) tail_cells += c->lv[--t].cells;
+    const int t = tail_start(c);
     const int sweeps = coarse_sweeps(c);
     for (int l = kFirstDense; l < t; ++l) {
         DenseLevel F = dlevel_of(c, l), C = dlevel_of(c, l + 1);
@@ -1117,16 +1186,7 @@ void dense_part(slosh_ctx *c)
             }
         launch_timed(c, K_DRESTRICT, k_drestrict, dim3(blocks_for(C.cells, 256)), dim3(256), 0, F, C, c->state);
     }
-    TailArgs T{};
-    T.n = m - t;
-    T.sweeps = sweeps;
-    T.info = c->dense_info + c->lv[t].dense_off;
-    T.coord = c->dense_coord + c->lv[t].dense_off;
-    for (int k = 0; k < T.n; ++k) {
-        T.L[k] = dlevel_of(c, t + k);
-        T.off[k] = c->lv[t + k].dense_off - c->lv[t].dense_off; // levels are contiguous in the blocks
-        T.total = T.off[k] + T.L[k].cells;
-    }
+    const TailArgs T = tail_args(c, t);
     {
         launch_timed(c, K_COARSE_TAIL, k_coarse_tail, dim3(1), dim3(1024), 0, T, c->coarse_idx, c->coarse_n,
                      c->coarse_inv, c->state);
@@ -1269,7 +1329,7 @@ template <typename F> bool capture_chunk_graph(slosh_ctx *c, hipGraphExec_t &exe
 // The MGPCG iterations of a solve whose residual r = b - A p is in c->r and whose state holds the initial
 // reduction: chunks of iterations between reads of the state, then the true-residual check, restarting from
 // r = b - A p up to max_restarts times. Residuals are captured for training when capture_on and the settings ask.
-void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
+void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on, bool resumed = false)
 {
     float *R[2] = {c->r, c->r2};
     float *D[2] = {c->d, c->d2};
@@ -1290,7 +1350,7 @@ void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
         next_cap += cap_gap;
         if (iteration >= 4) cap_gap++;
     };
-    capture(R[0], 0, false);
+    if (!resumed) capture(R[0], 0, false);
     int rc = 1, dc = 0;
     // the first iteration's update (no alpha yet), V-cycle and direction, from the residual in R[0]
     auto first_iteration = [&]() {
@@ -1301,7 +1361,7 @@ void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
         capture(R[rc], 0, true);
         l0_direction(c, D[1], D[0], R[rc], 1);
     };
-    first_iteration();
+    if (!resumed) first_iteration();
     SLOSH_HIP(hipGetLastError());
     // Iterations are enqueued in chunks between reads of the device state (an even count, so the ping-pong
     // buffers return to the same roles). Without residual capture or kernel timing, a chunk is captured once per
@@ -1332,6 +1392,8 @@ void mgpcg_iterate(slosh_ctx *c, int maxit, int max_restarts, bool capture_on)
     // The recurrence residual that stops the iteration can differ from the FP64 true residual ||b - A p|| / ||b||
     // of the pressure in the last bits; the solve is accepted only when the true residual meets the tolerance too,
     // else it restarts from r = b - A p of the current pressure (a few times at most, within max_iterations).
+    // resumed: the iterations ran in a lockstep batch, whose last read of the state is in h_state
+    if (resumed) done_iters = maxit + 1;
     for (int restart = 0;; ++restart) {
         while (done_iters <= maxit) {
             for (int k = 0; k < ahead && done_iters <= maxit; ++k, done_iters += chunk) {
@@ -1404,10 +1466,12 @@ void reference_error(slosh_ctx *c, int maxit)
     c->last_mg_iters = last;
 }
 
-// SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG), to ||r|| <= tol ||b||.
-void run_mgpcg(slosh_ctx *c)
+// The set-up of an MGPCG solve (state reset, multigrid set-up, r = b - A p, its reduction, the counts of bricks and
+// tiles with fluid). Returns whether the solve needs iterations: false without fluid (b = 0, converged as it
+// stands).
+bool mgpcg_begin(slosh_ctx *c)
 {
-    if (!c->nactive) return;
+    if (!c->nactive) return false;
     const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 200;
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, (double)c->s.tolerance, maxit);
     mg_setup(c);
@@ -1415,10 +1479,17 @@ void run_mgpcg(slosh_ctx *c)
     finalize(c, kFinInit);
     mg_counts(c);
     if (!c->nsolve) {
-        // no fluid cell: b = 0, the solve is converged as it stands
         c->capture_n = 0;
-        return;
+        return false;
     }
+    return true;
+}
+
+// SloshProxy.cpp:807-1020 (AddPressureSolveMGPCG), to ||r|| <= tol ||b||.
+void run_mgpcg(slosh_ctx *c)
+{
+    if (!mgpcg_begin(c)) return;
+    const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 200;
     mgpcg_iterate(c, maxit, 3, true);
     c->last_mg_iters = c->h_state->iters;
     if (c->s.flags & SLOSH_FLAG_REPORT_FP64_ERROR) reference_error(c, maxit);
@@ -1601,6 +1672,195 @@ void run_rbsor(slosh_ctx *c)
         SLOSH_HIP(hipStreamSynchronize(c->stream));
         if (c->h_state->done) break;
     }
+}
+
+// ---- batched MGPCG: the iterations of many tanks in lockstep, each pass one launch over all of them ----------
+// A tank's solve joins when it is MGPCG without residual capture, FP64 error report or kernel timing, on a hierarchy
+// of at least five levels whose dense levels all fit the coarse tail (the V-cycle is then the same pass sequence for
+// every tank: level 0, level 1, the fused level-2/3 passes, the tail).
+bool mgpcg_batchable(const slosh_ctx *c)
+{
+    if (c->s.pressure_solver != SLOSH_SOLVER_MGPCG || c->timing_mask) return false;
+    if (c->s.flags & (SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_REPORT_FP64_ERROR)) return false;
+    return c->nlevels >= 5 && tail_start(c) == kFirstDense;
+}
+
+void batch_tank_of(const slosh_ctx *c, BatchTank &T)
+{
+    T = BatchTank{};
+    T.binfo_f = c->binfo_f;
+    T.fmask = c->fmask;
+    T.diag = c->diag;
+    T.div = c->div;
+    T.pres = c->pres;
+    T.r[0] = c->r;
+    T.r[1] = c->r2;
+    T.z = c->z;
+    T.z1 = c->z1;
+    T.d[0] = c->d;
+    T.d[1] = c->d2;
+    T.q = c->q;
+    T.partials = c->partials;
+    T.state = c->state;
+    T.nsolve = c->nsolve;
+    T.ntiles_f = c->ntiles_f;
+    T.geo = c->geo;
+    T.tile_list = c->tile_list;
+    T.L1 = plevel_of(c, 1);
+    T.L2 = plevel_of(c, 2);
+    T.L3 = dlevel_of(c, 3);
+    T.L4 = dlevel_of(c, kFirstDense);
+    T.tail = tail_args(c, kFirstDense);
+    T.coarse_idx = c->coarse_idx;
+    T.coarse_n = c->coarse_n;
+    T.coarse_inv = c->coarse_inv;
+}
+
+void b_d23down(const BatchRun &B)
+{
+    hipLaunchKernelGGL(kb_d23down, dim3(B.total[kRangeTile]), dim3(kD23DownThreads), 0u, B.stream, B.d_tanks,
+                       B.range(kRangeTile), B.ntanks);
+}
+
+void b_d23up(const BatchRun &B)
+{
+    hipLaunchKernelGGL(kb_d23up, dim3(B.total[kRangeTile]), dim3(kD23UpThreads), 0u, B.stream, B.d_tanks,
+                       B.range(kRangeTile), B.ntanks);
+}
+
+void b_coarse_tail(const BatchRun &B)
+{
+    hipLaunchKernelGGL(kb_coarse_tail, dim3(B.ntanks), dim3(1024), 0u, B.stream, B.d_tanks);
+}
+
+void b_finalize(const BatchRun &B, int kind)
+{
+    hipLaunchKernelGGL(kb_finalize, dim3(B.ntanks), dim3(kFinThreads), 0u, B.stream, kind, B.d_tanks);
+}
+
+namespace {
+// Copies every tank's solver state into one array (one read back for the batch).
+__global__ void kb_gather_states(const BatchTank *T, SolverState *out)
+{
+    static_assert(sizeof(SolverState) % 4u == 0u, "word copy");
+    const uint32_t *src = (const uint32_t *)T[blockIdx.x].state;
+    uint32_t *dst = (uint32_t *)(out + blockIdx.x);
+    for (uint32_t i = threadIdx.x; i < sizeof(SolverState) / 4u; i += blockDim.x) dst[i] = src[i];
+}
+
+// mgpcg_iterate's graph capture for the batch's stream.
+template <typename F> bool capture_batch_graph(const BatchRun &B, hipGraphExec_t &exec, bool &graphs, F body)
+{
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        graphs = false;
+        body();
+        return false;
+    }
+    body();
+    if (hipStreamEndCapture(B.stream, &g) != hipSuccess || !g) {
+        graphs = false;
+        (void)hipGetLastError();
+        body();
+        return false;
+    }
+    bool ok = false;
+    if (exec) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res;
+        ok = hipGraphExecUpdate(exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+        }
+    }
+    if (!ok) ok = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess;
+    (void)hipGraphDestroy(g);
+    if (!ok) {
+        (void)hipGetLastError();
+        exec = nullptr;
+        graphs = false;
+        body();
+    }
+    return ok;
+}
+} // namespace
+
+// mgpcg_iterate's iterations for every tank of the batch at once, on the batch's stream (which the caller has made
+// wait for every tank's set-up, with the descriptors and ranges uploaded): the first iteration, then chunks of
+// `chunk` iterations (replayed as a captured graph) between reads of all tanks' states, until every tank's solve is
+// done. Each pass is the single-tank pass applied to every tank, so each tank computes what its own solve would.
+// The last read of each tank's state is left in its h_state.
+void batch_iterate(BatchRun &B, slosh_ctx *const *tanks, uint32_t n, int chunk, bool graphs, hipGraphExec_t &graph,
+                   SolverState *d_states, SolverState *h_states)
+{
+    chunk = chunk >= 2 ? chunk & ~1 : 4;
+    int maxit = 0, ahead = 1;
+    for (uint32_t k = 0; k < n; ++k) {
+        const slosh_ctx *c = tanks[k];
+        maxit = std::max(maxit, c->s.max_iterations > 0 ? c->s.max_iterations : 200);
+        if (c->last_mg_iters > 1) ahead = std::max(ahead, (c->last_mg_iters + chunk - 1) / chunk);
+    }
+    int rc = 1, dc = 0;
+    auto vcycle_b = [&](int ri) {
+        b_restrict(B, ri);
+        b_pooled_down1(B);
+        b_d23down(B);
+        b_coarse_tail(B);
+        b_d23up(B);
+        b_pooled_up1(B);
+        b_prolong_black(B, ri);
+        b_red_dot(B, ri);
+        b_finalize(B, kFinRho);
+    };
+    // the first iteration's update (no alpha yet), V-cycle and direction, from the residual in r[0]
+    b_update_black(B, 0, 1, 0, true);
+    vcycle_b(rc);
+    b_direction(B, 1, 0, rc, true);
+    auto chunk_body = [&]() {
+        for (int k = 0; k < chunk; ++k) {
+            b_finalize(B, kFinAlphaStop);
+            b_update_black(B, rc, rc ^ 1, dc, false);
+            rc ^= 1;
+            vcycle_b(rc);
+            b_direction(B, dc, dc ^ 1, rc, false);
+            dc ^= 1;
+        }
+    };
+    SLOSH_HIP(hipGetLastError());
+    bool graph_ready = false;
+    int done_iters = 0;
+    for (;;) {
+        for (int k = 0; k < ahead && done_iters <= maxit; ++k, done_iters += chunk) {
+            if (graphs && done_iters > 0) {
+                if (!graph_ready) graph_ready = capture_batch_graph(B, graph, graphs, chunk_body);
+                if (graph_ready) SLOSH_HIP(hipGraphLaunch(graph, B.stream));
+            } else {
+                chunk_body();
+            }
+            SLOSH_HIP(hipGetLastError());
+        }
+        ahead = 1;
+        b_true_residual_if_done(B);
+        b_finalize(B, kFinCheckDone);
+        hipLaunchKernelGGL(kb_gather_states, dim3(n), dim3(64), 0u, B.stream, B.d_tanks, d_states);
+        SLOSH_HIP(hipMemcpyAsync(h_states, d_states, n * sizeof(SolverState), hipMemcpyDeviceToHost, B.stream));
+        SLOSH_HIP(hipStreamSynchronize(B.stream));
+        bool all = true;
+        for (uint32_t k = 0; k < n; ++k) all = all && h_states[k].done != 0;
+        if (all || done_iters > maxit) break;
+    }
+    for (uint32_t k = 0; k < n; ++k) *tanks[k]->h_state = h_states[k];
+}
+
+// After batched iterations: restarts from r = b - A p while the FP64 true residual misses the tolerance (on the
+// tank's own stream), as mgpcg_iterate does after its own iterations.
+void mgpcg_resume(slosh_ctx *c)
+{
+    const int maxit = c->s.max_iterations > 0 ? c->s.max_iterations : 200;
+    mgpcg_iterate(c, maxit, 3, false, true);
+    c->last_mg_iters = c->h_state->iters;
 }
 
 } // namespace slosh

@@ -59,7 +59,7 @@ def test_default_settings_are_the_parity_profile():
     assert s.pressure_solver == abi.SOLVER_MGPCG and abs(s.tolerance - 1e-6) < 1e-12
     assert s.ghost_fluid == 1 and s.extrapolation_layers == 2 and s.density_correction == 0.0
     assert list(s.grid_dim) == [64, 64, 64] and s.sparse_storage_budget == 1.0
-    assert lib.slosh_abi_version() == 1
+    assert lib.slosh_abi_version() == 2
     assert b"gfx950" in lib.slosh_build_info()
 
 

@@ -7,8 +7,9 @@
 - 512^3 falling spheres and pool with FP16 multigrid levels (218M particles): every step converged, with the error
   of each solution against an FP64-accurate solve of the same system reported and small; that reference agrees
   with the oracle's FP64 solve where the oracle can run it (64^3).
-- 64 tanks of uneven size with sphere obstacles stepped as one batch: every tank converged, and the smallest,
-  median and largest tank bit-identical to their single-tank runs.
+- 64 tanks of uneven size with sphere obstacles stepped as one lockstep batch (the MGPCG iterations of all tanks
+  together, each pass one launch): every tank converged, and the smallest, median and largest tank bit-identical to
+  their single-tank runs.
 """
 import numpy as np
 import pytest
@@ -171,6 +172,7 @@ def test_64_tank_batch():
 
     sims = [make(sp) for sp in specs]
     b = Batch(sims)
+    b.set_config(4, 4, 1, 1, 4)  # lockstep: the MGPCG iterations of all 64 tanks, each pass one launch
     grav = np.tile(np.array((0.0, -9.8, 0.0), np.float32), (len(sims), 1))
     for k in range(2):
         b.step(dt, grav)
