@@ -741,43 +741,69 @@ __global__ __launch_bounds__(512) void k_coupling_force(Geo g, const uint32_t *r
     }
 }
 
-// Slosh.usf:1316-1370 (SubmergedCS): a solid cell of a collider counts as submerged when a walk of up to 12 cells along
-// +x, -x, +y, -y out of the collider first meets dense fluid with no air in the three cells below (-z, the
-// reference's down axis).
+// Adds each lane's values v[k] to coupling[(id - 1) * 16 + slot[k]] for the lanes with `on`: one atomic per value
+// and collider present in the wavefront instead of one per lane (integer sums, whose grouping changes nothing; the
+// additions wrap like the atomics). Every lane of the wavefront must call it.
+template <int N>
+__device__ __forceinline__ void wave_coupling_add(int32_t *coupling, uint32_t id, bool on, const int (&slot)[N],
+                                                  const int32_t (&v)[N])
+{
+    uint64_t pending = __ballot(on);
+    const int lane = (int)(threadIdx.x & 63u);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lid = (uint32_t)__shfl((int)id, leader, 64);
+        const bool mine = on && id == lid;
+        for (int k = 0; k < N; ++k) {
+            uint32_t sum = mine ? (uint32_t)v[k] : 0u;
+            for (int o = 32; o > 0; o >>= 1) sum += (uint32_t)__shfl_xor((int)sum, o, 64);
+            if (lane == leader) atomicAdd(&coupling[(lid - 1u) * 16u + (uint32_t)slot[k]], (int32_t)sum);
+        }
+        pending &= ~(uint64_t)__ballot(mine);
+    }
+}
+
+// Slosh.usf:1316-1370 (SubmergedCS): per collider the count of its solid cells and of the submerged ones, with their
+// offsets from the collider's centre. A solid cell counts as submerged when a walk of up to 12 cells along +x, -x,
+// +y, -y out of the collider first meets dense fluid with no air in the three cells below (-z, the reference's down
+// axis).
 __global__ __launch_bounds__(512) void k_submerged(Geo g, const uint32_t *rows, const uint32_t *ctype,
                                                    const slosh_collider *col, int32_t *coupling)
 {
     __shared__ uint32_t s_row[28];
     CellPos c;
     uint32_t e;
-    if (!cell_of_block(g, rows, s_row, c, e)) return;
-    const uint32_t packed = ctype[c.idx], id = collider_of(packed);
-    if ((packed & kCellTypeMask) != kCellSolid || id == 0u) return;
-    int32_t *acc = coupling + (id - 1u) * 16u;
-    atomicAdd(&acc[11], 1);
+    const bool inside = cell_of_block(g, rows, s_row, c, e);
+    const uint32_t packed = inside ? ctype[c.idx] : 0u, id = collider_of(packed);
+    const bool body = inside && (packed & kCellTypeMask) == kCellSolid && id != 0u;
     // directions +x, -x, +y, -y
     bool sub = false;
-    for (int d = 0; d < 4; ++d) {
-        int px = c.x, py = c.y;
-        for (int s = 0; s < 12; ++s) {
-            px += d == 0 ? 1 : (d == 1 ? -1 : 0);
-            py += d == 2 ? 1 : (d == 3 ? -1 : 0);
-            const uint32_t t = type_global(g, ctype, px, py, c.z);
-            if ((t & kCellTypeMask) != kCellSolid || collider_of(t) != id) {
-                bool liquid = (t & kCellTypeMask) == kCellFluid && (t & kCellDense) != 0u;
-                for (int below = 1; below <= 3; ++below)
-                    liquid = liquid && (type_global(g, ctype, px, py, c.z - below) & kCellTypeMask) != kCellAir;
-                sub = sub || liquid;
-                break;
+    if (body) {
+        for (int d = 0; d < 4; ++d) {
+            int px = c.x, py = c.y;
+            for (int s = 0; s < 12; ++s) {
+                px += d == 0 ? 1 : (d == 1 ? -1 : 0);
+                py += d == 2 ? 1 : (d == 3 ? -1 : 0);
+                const uint32_t t = type_global(g, ctype, px, py, c.z);
+                if ((t & kCellTypeMask) != kCellSolid || collider_of(t) != id) {
+                    bool liquid = (t & kCellTypeMask) == kCellFluid && (t & kCellDense) != 0u;
+                    for (int below = 1; below <= 3; ++below)
+                        liquid = liquid && (type_global(g, ctype, px, py, c.z - below) & kCellTypeMask) != kCellAir;
+                    sub = sub || liquid;
+                    break;
+                }
             }
         }
     }
-    if (!sub) return;
-    atomicAdd(&acc[7], 1);
-    const slosh_collider cc = col[id - 1u];
-    const float off[3] = {((float)c.x + 0.5f) * g.h - cc.center[0], ((float)c.y + 0.5f) * g.h - cc.center[1],
-                          ((float)c.z + 0.5f) * g.h - cc.center[2]};
-    for (int a = 0; a < 3; ++a) add_fixed(&acc[12 + a], off[a]);
+    int32_t fx[3] = {0, 0, 0};
+    if (sub) {
+        const slosh_collider cc = col[id - 1u];
+        const float off[3] = {((float)c.x + 0.5f) * g.h - cc.center[0], ((float)c.y + 0.5f) * g.h - cc.center[1],
+                              ((float)c.z + 0.5f) * g.h - cc.center[2]};
+        for (int a = 0; a < 3; ++a) fx[a] = (int32_t)rintf(off[a] * kCouplingScale);
+    }
+    wave_coupling_add<1>(coupling, id, body, {11}, {1});
+    wave_coupling_add<4>(coupling, id, sub, {7, 12, 13, 14}, {1, fx[0], fx[1], fx[2]});
 }
 
 // ---- two-sided density projection (Slosh.usf:743-854) --------------------------------------------------------
@@ -1166,6 +1192,10 @@ void launch_project(slosh_ctx *c)
 void launch_coupling(slosh_ctx *c, float dt)
 {
     if (!c->nactive || !c->ncol) return;
+    // the face and force terms accumulate only for two-way bodies (every other collider returns at once in them)
+    bool two_way = false;
+    for (uint32_t k = 0; k < c->ncol; ++k) two_way = two_way || (c->h_col[k].flags & 1u) != 0u;
+    if (!two_way) return;
     // SloshProxy.cpp:562: PressureToPascal = density h / dt
     const float rho = c->s.fluid_density > 0.0f ? c->s.fluid_density : 1000.0f;
     const float p2pa = rho * c->geo.h / dt;
