@@ -233,7 +233,13 @@ def bench_other(args):
         sims.append(sim)
     b = Batch(sims)
     grav = np.tile(np.array(g, np.float32), (len(sims), 1))
-    chosen = b.autotune(dt, grav, steps_per_candidate=1)
+    if args.no_autotune:
+        # lockstep MGPCG iterations, 4 host threads for the passes around the solve, chunks of 4, graphs, one brick
+        # per workgroup of the level-0 passes
+        b.set_config(4, 4, 1, 1, 1)
+        chosen = b.config()
+    else:
+        chosen = b.autotune(dt, grav, steps_per_candidate=1)
     for _ in range(args.warmup):
         b.step(dt, grav)
     t0 = time.perf_counter()
@@ -284,6 +290,8 @@ def main():
                     help="no HIP events around the dominant kernel in the timed steps (no roofline figure)")
     ap.add_argument("--fcg-steps", type=int, default=None,
                     help="timed steps of the learned-FCG comparison run (default: the same window as `value`)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="tanks64: the lockstep batch configuration instead of the autotuner's choice")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run 512 MiB device copies first (scale reference for rocprofv3 --pmc FETCH/WRITE_SIZE)")
     args = ap.parse_args()
