@@ -318,8 +318,10 @@ def main():
     reference_cycle = {"flags": 0, "mg_coarse_sweeps": 0}
     mg = reference_cycle if args.mg_reference_cycle else tuned
 
-    def run(solver, weights, steps, warmup, mg_opts):
-        """Times `steps` steps after `warmup`; returns (steps/s, per-step data, dominant kernel, its timing, sim)."""
+    def run(solver, weights, steps, warmup, mg_opts, among=None):
+        """Times `steps` steps after `warmup`; returns (steps/s, per-step data, dominant kernel, its timing, sim).
+        The dominant kernel is the one with the largest total time among `among` (default: the kernels with a byte
+        model)."""
         s = abi.make_settings(grid, h, n, solver=solver, tolerance=1e-6,
                               max_iterations=20000 if solver == abi.SOLVER_RBSOR else 200, sor_weight=1.8,
                               flags=mg_opts["flags"], mg_coarse_sweeps=mg_opts["mg_coarse_sweeps"])
@@ -339,7 +341,10 @@ def main():
         sim.step(dt, grav(t))
         t += dt
         prof = sim.kernel_times(reset=True)
-        modelled = {k: v for k, v in prof.items() if algorithmic_bytes(k, 1, 1, 1, 1) is not None}
+        if among is None:
+            modelled = {k: v for k, v in prof.items() if algorithmic_bytes(k, 1, 1, 1, 1) is not None}
+        else:
+            modelled = {k: v for k, v in prof.items() if k in among}
         dominant = max((modelled or prof).items(), key=lambda kv: kv[1][1])[0]
         sim.set_kernel_timing(0)
         # the timed steps: no instrumentation (slosh_step returns with the library's stream idle)
@@ -394,8 +399,11 @@ def main():
     if args.solver == "mgpcg" and not args.no_learned and os.path.exists(wpath):
         fcg_steps = args.fcg_steps if args.fcg_steps is not None else args.steps
         fcg_warmup = args.warmup if args.fcg_steps is None else 2
+        # its roofline: the network's dominant convolution (MFMA-bound)
         fsps, fel, fper, fdom, (flaunch, fms), fprof, fsim = run(abi.SOLVER_FCG_UNET, unet.load(wpath),
-                                                                 fcg_steps, fcg_warmup, mg)
+                                                                 fcg_steps, fcg_warmup, mg,
+                                                                 among=("k_unet_conv_l0", "k_unet_conv_l1",
+                                                                        "k_unet_conv_l2"))
         fsim.close()
         favg = fms / 1e3 / max(flaunch, 1) if fms > 0 else float("nan")
         # level-0 forward convolutions: E0a (4->16), E0b and D0b (16->16), 27 taps, 512 cells per brick
