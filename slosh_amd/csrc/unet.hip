@@ -78,14 +78,29 @@ __global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nb
         s_nb[bg][k % 28u] = e < nbricks ? binfo[(size_t)e * kBinfoWords + k % 28u] : 0u;
     }
     __syncthreads();
-    for (uint32_t k = tid; k < (uint32_t)(BG * BOX * CH8); k += 256u) {
-        const uint32_t c8 = k % CH8, q = k / CH8, bg = q / BOX, bi = q % BOX;
-        const int u = (int)(bi % SB) - 1, v = (int)((bi / SB) % SB) - 1, ww = (int)(bi / (SB * SB)) - 1;
-        const uint32_t cell = nb_cell<S>(s_nb[bg], u, v, ww);
-        const uint4 val = *(const uint4 *)(in + (size_t)cell * CIN + c8 * 8);
-        uint2 *dst = (uint2 *)(s_in + ((size_t)(bg * BOX + bi) * LS + c8 * 8));
-        dst[0] = make_uint2(val.x, val.y);
-        dst[1] = make_uint2(val.z, val.w);
+    // the box in rounds of up to 8 16-byte chunks per thread: every chunk of a round is loaded before any is stored
+    constexpr uint32_t kChunks = (uint32_t)(BG * BOX * CH8), kRound = 8u * 256u;
+    for (uint32_t k0 = 0; k0 < kChunks; k0 += kRound) {
+        uint4 sv[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8u; ++q) {
+            const uint32_t k = k0 + tid + 256u * q;
+            sv[q] = make_uint4(0u, 0u, 0u, 0u);
+            if (k >= kChunks) continue;
+            const uint32_t c8 = k % CH8, qq = k / CH8, bg = qq / BOX, bi = qq % BOX;
+            const int u = (int)(bi % SB) - 1, v = (int)((bi / SB) % SB) - 1, ww = (int)(bi / (SB * SB)) - 1;
+            const uint32_t cell = nb_cell<S>(s_nb[bg], u, v, ww);
+            sv[q] = *(const uint4 *)(in + (size_t)cell * CIN + c8 * 8);
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 8u; ++q) {
+            const uint32_t k = k0 + tid + 256u * q;
+            if (k >= kChunks) continue;
+            const uint32_t c8 = k % CH8, qq = k / CH8;
+            uint2 *dst = (uint2 *)(s_in + ((size_t)qq * LS + c8 * 8));
+            dst[0] = make_uint2(sv[q].x, sv[q].y);
+            dst[1] = make_uint2(sv[q].z, sv[q].w);
+        }
     }
     __syncthreads();
     const uint32_t row = lane & 15u, quad = lane >> 4;
@@ -111,18 +126,27 @@ __global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nb
             }
         }
     }
+    // the output tile goes through LDS (the staged input is no longer read), then to memory in 16-byte chunks
+    static_assert(M * COUT <= BG * BOX * LS, "output tile fits the input's LDS");
+    __syncthreads();
     for (int mt = 0; mt < MTW; ++mt)
         for (int nt = 0; nt < NT; ++nt) {
             const uint32_t co = (uint32_t)nt * 16u + row;
             const float bv = BIAS ? bias[co] : 0.0f;
             for (int j = 0; j < 4; ++j) {
-                const uint32_t vox = (wave * MTW + mt) * 16u + quad * 4u + (uint32_t)j, bg = vox / CPB, l = vox % CPB;
-                if (grp * BG + bg >= nbricks) continue;
+                const uint32_t vox = (wave * MTW + mt) * 16u + quad * 4u + (uint32_t)j;
                 float val = acc[mt][nt][j] + bv;
                 if (RELU) val = fmaxf(val, 0.0f);
-                out[((size_t)s_nb[bg][13] * CPB + l) * COUT + co] = f2bf(val);
+                s_in[vox * COUT + co] = f2bf(val);
             }
         }
+    __syncthreads();
+    constexpr int CO8 = COUT / 8;
+    for (uint32_t k = tid; k < (uint32_t)(M * CO8); k += 256u) {
+        const uint32_t vox = k / CO8, c8 = k % CO8, bg = vox / CPB, l = vox % CPB;
+        if (grp * BG + bg >= nbricks) continue;
+        *(uint4 *)(out + ((size_t)s_nb[bg][13] * CPB + l) * COUT + c8 * 8) = *(const uint4 *)(s_in + vox * COUT + c8 * 8);
+    }
 }
 
 // ---- elementwise passes ---------------------------------------------------------------------------------------
