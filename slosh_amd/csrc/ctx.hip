@@ -343,8 +343,17 @@ void create_impl(slosh_ctx *c)
     SLOSH_HIP(hipStreamSynchronize(c->stream));
 }
 
+// The submerged-cell pass of the last step, when it has not run yet (see step_back).
+void flush_submerged(slosh_ctx *c)
+{
+    if (!c->submerged_pending) return;
+    launch_submerged(c);
+    c->submerged_pending = false;
+}
+
 void reset_impl(slosh_ctx *c)
 {
+    c->submerged_pending = false;
     const Geo &g = c->geo;
     c->np = 0;
     c->ndead = 0;
@@ -594,13 +603,16 @@ void step_back(slosh_ctx *c, bool solved)
     const int stop = c->st_stop;
     if (stop > SLOSH_STAGE_SOLVE) launch_project(c);
     // collider coupling of this step (SloshProxy.cpp:397 clears it per frame; one step is one substep here): the
-    // contact and wetted-face terms of the projection, the smoothed force (Slosh.usf:1138-1216) and the submerged
-    // cells (SloshProxy.cpp:715-718), all on this step's cell types and solved pressure
+    // contact and wetted-face terms of the projection and the smoothed force (Slosh.usf:1138-1216) on this step's
+    // cell types and solved pressure. The submerged cells (SloshProxy.cpp:715-718) are a once-per-frame pass over
+    // the last substep's cell types in the reference: here it runs when the feedback is read (or before the
+    // colliders change), on the same cell types, so steps whose feedback nobody reads do not pay for it.
     c->coupling_ncol = 0;
+    c->submerged_pending = false;
     if (stop > SLOSH_STAGE_SOLVE && c->ncol) {
         launch_fill_u32(c->stream, (uint32_t *)c->coupling, 0u, (size_t)c->ncol * 16u);
         launch_coupling(c, dt);
-        launch_submerged(c);
+        c->submerged_pending = true;
         c->coupling_ncol = c->ncol;
     }
     // SloshProxy.cpp:656-702: the two-sided density projection after the pressure projection, every
@@ -984,7 +996,7 @@ slosh_status slosh_set_colliders(slosh_ctx *c, const slosh_collider *col, uint32
 {
     if (!c || (n && !col)) return SLOSH_EINVAL;
     if (n > SLOSH_MAX_COLLIDERS) return SLOSH_ECAPACITY;
-    ABI_GUARD(if (n) {
+    ABI_GUARD(flush_submerged(c); if (n) {
         memcpy(c->h_col, col, n * sizeof(slosh_collider));
         SLOSH_HIP(hipMemcpyAsync(c->d_col, c->h_col, n * sizeof(slosh_collider), hipMemcpyHostToDevice, c->stream));
         SLOSH_HIP(hipStreamSynchronize(c->stream));
@@ -1340,6 +1352,7 @@ slosh_status slosh_get_collider_feedback(slosh_ctx *c, slosh_collider_feedback *
 {
     if (!c || !n || (cap && !out)) return SLOSH_EINVAL;
     ABI_GUARD({
+        flush_submerged(c);
         const uint32_t nc = c->coupling_ncol;
         *n = nc;
         const uint32_t m = nc < cap ? nc : cap;

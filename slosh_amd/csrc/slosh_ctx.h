@@ -366,6 +366,7 @@ struct slosh_ctx {
     float st_g[3]{};
     int st_stop = 0;
     hipEvent_t ev_batch = nullptr;    // a lockstep batch waits on the tank's stream at this event
+    bool submerged_pending = false;   // the last step's submerged-cell pass has not run yet (it runs on demand)
     uint16_t *dense_info = nullptr;   // stencil words of all dense levels, level by level
     uint32_t *dense_coord = nullptr;  // their packed cell coordinates
     slosh::SolverState *state = nullptr;
