@@ -808,6 +808,14 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
         uint32_t brick = kNoBrick;
         const float4 pa = pa_next;
         if (i + 256u < end) pa_next = p.pa[i + 256u];
+        // second pass: the record's destination and its first-pass key, loaded before the gathers so that their
+        // latency overlaps the arithmetic
+        uint32_t dst_j = 0u, key_first = 0u;
+        if (kMove && i < end) {
+            const uint32_t pk = mv.packed[i];
+            dst_j = mv.run_base[(i & ~(kTile - 1u)) + (pk >> 8)] + (pk & 255u);
+            key_first = keys[i];
+        }
         if (i < end) {
             if (!kMove && !(pa.w >= 0.0f)) keys[i] = dead_key; // a dead record is dropped by the sort
             if (pa.w >= 0.0f) {
@@ -913,12 +921,11 @@ __device__ __forceinline__ void g2p_body(Geo g, Parts p, const uint32_t *key_off
                     pos_cell(g, x[0], x[1], x[2], c);
                     brick = brick_of_cell(g, c[0], c[1], c[2]);
                     if (kMove) {
-                        const uint32_t pk = mv.packed[i];
-                        const uint32_t j = mv.run_base[(i & ~(kTile - 1u)) + (pk >> 8)] + (pk & 255u);
+                        const uint32_t j = dst_j;
                         mv.dst.pa[j] = make_float4(x[0], x[1], x[2], pa.w + dt);
                         for (int a = 0; a < 3; ++a) mv.dst.v[a][j] = nv[a];
                         for (int k = 0; k < 9; ++k) mv.dst.c[k][j] = cm[k];
-                        if (brick != keys[i]) atomicAdd(mv.misplaced, 1u);
+                        if (brick != key_first) atomicAdd(mv.misplaced, 1u);
                     } else {
                         if (speed > max_speed) nclamp++;
                         vmax = max(vmax, __float_as_uint(speed));
