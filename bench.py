@@ -266,6 +266,41 @@ def bench_other(args):
                         "particles_per_s": round(args.steps / el * parts, 1)})
 
 
+def learned_on_dam_break_256(weights, mg, warmup=5, steps=20):
+    """The dam break of the 64^3 configuration at 256^3 (31.7M particles; dt 1/480 keeps its CFL number), MGPCG and
+    the learned flexible CG with the same V-cycle options: steps/s over steps warmup+1..warmup+steps, iterations,
+    fallbacks."""
+    from slosh_amd import Sim, abi, scenes
+
+    grid, h, reqs = scenes.dam_break(256, seed=0)
+    n = scenes.particle_count(reqs)
+    out = {"workload": "dam break 256^3, block x < 0.4, y < 0.6, 8 jittered particles per cell (seed 0), dt 1/480",
+           "particles": n, "timed_window": f"steps {warmup + 1}-{warmup + steps}"}
+    for name, solver in (("mgpcg", abi.SOLVER_MGPCG), ("learned_fcg", abi.SOLVER_FCG_UNET)):
+        sim = Sim(abi.make_settings(grid, h, n + 16, solver=solver, tolerance=1e-6, flags=mg["flags"],
+                                    mg_coarse_sweeps=mg["mg_coarse_sweeps"]))
+        for r in reqs:
+            sim.emit(r)
+        if solver == abi.SOLVER_FCG_UNET:
+            sim.unet_set_weights(weights)
+        its, fb, conv = [], 0, 0
+        for k in range(warmup + steps):
+            if k == warmup:
+                t0 = time.perf_counter()
+            sim.step(1.0 / 480.0, (0.0, -9.8, 0.0))
+            if k >= warmup:
+                r = sim.results()
+                its.append(r.solver_iterations)
+                fb += r.fallback_used == 1
+                conv += r.converged
+        el = time.perf_counter() - t0
+        sim.close()
+        out[name] = {"steps_per_s": round(steps / el, 3), "mean_iterations": round(sum(its) / len(its), 2),
+                     "fallback_steps": fb, "converged_steps": conv}
+    out["speedup_vs_mgpcg"] = round(out["learned_fcg"]["steps_per_s"] / out["mgpcg"]["steps_per_s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -415,11 +450,13 @@ def main():
             ach = conv_flops[fdom] / favg / 1e12
             fl.update({"bound": "mfma", "achieved": round(ach, 1), "peak": 2500.0, "unit": "TFLOP/s",
                        "frac": round(ach / 2500.0, 4), "algorithmic_flops": int(conv_flops[fdom])})
+        # the learned preconditioner's evaluation scene: the dam break at 256^3
+        c4 = learned_on_dam_break_256(unet.load(wpath), mg)
         learned = {"steps_per_s": round(fsps, 3), "speedup_vs_mgpcg": round(fsps / sps, 3),
                    "mean_iterations": round(sum(p[0] for p in fper) / len(fper), 2),
                    "fallback_steps": sum(1 for p in fper if p[3] == 1), "steps": fcg_steps, "warmup": fcg_warmup,
                    "converged_steps": sum(p[2] for p in fper), "max_rel_residual": max(p[1] for p in fper),
-                   "roofline": fl, "weights": os.path.relpath(wpath, ROOT)}
+                   "roofline": fl, "weights": os.path.relpath(wpath, ROOT), "dam_break_256": c4}
     # the same workload with the reference's V-cycle, for comparison
     ref_cycle = None
     if args.solver == "mgpcg" and not args.mg_reference_cycle and not args.no_reference_cycle_run:

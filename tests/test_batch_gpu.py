@@ -123,3 +123,20 @@ def test_lockstep_batch_with_a_tank_outside_the_lockstep():
     b.close()
     for s in sims:
         s.close()
+
+
+def test_batch_config_is_validated():
+    from slosh_amd import Batch, SloshError
+
+    print("case batch configuration checks", flush=True)
+    specs = tanks()[:2]
+    sims = [make(s) for s in specs]
+    b = Batch(sims)
+    for bad in [(2, 4, 1, 2, 4), (2, 4, 1, 1, 3), (2, 3, 1, 1, 4), (0, 4, 1, 1, 4)]:
+        with pytest.raises(SloshError):
+            b.set_config(*bad)
+    b.set_config(2, 4, 1, 1, 2)
+    assert b.config() == {"workers": 2, "solver_chunk": 4, "graphs": 1, "mode": 1, "col_bricks": 2}
+    b.close()
+    for s in sims:
+        s.close()

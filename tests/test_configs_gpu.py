@@ -7,6 +7,8 @@
 - 512^3 falling spheres and pool with FP16 multigrid levels (218M particles): every step converged, with the error
   of each solution against an FP64-accurate solve of the same system reported and small; that reference agrees
   with the oracle's FP64 solve where the oracle can run it (64^3).
+- the learned flexible CG on the C1 dam-break scene at 256^3 (31.7M particles, dt 1/480, the 64^3 scene's CFL
+  number): every solve accepted without the MGPCG fallback, at an FP64 true residual within 1e-6.
 - 64 tanks of uneven size with sphere obstacles stepped as one lockstep batch (the MGPCG iterations of all tanks
   together, each pass one launch): every tank converged, and the smallest, median and largest tank bit-identical to
   their single-tank runs.
@@ -192,3 +194,31 @@ def test_64_tank_batch():
     b.close()
     for sim in sims:
         sim.close()
+
+
+def test_learned_fcg_dam_break_256():
+    from slosh_amd import unet
+
+    print("case learned FCG, dam break 256^3", flush=True)
+    grid, h, reqs = scenes.dam_break(256, seed=0)
+    n = scenes.particle_count(reqs)
+    its = {}
+    for solver in (abi.SOLVER_MGPCG, abi.SOLVER_FCG_UNET):
+        sim = Sim(abi.make_settings(grid, h, n + 16, solver=solver, tolerance=1e-6,
+                                    flags=abi.FLAG_MG_FLUID_COARSENING, mg_coarse_sweeps=3))
+        for r in reqs:
+            sim.emit(r)
+        if solver == abi.SOLVER_FCG_UNET:
+            sim.unet_set_weights(unet.load(unet.default_weights_path()))
+        its[solver] = []
+        for k in range(3):
+            sim.step(1.0 / 480.0, (0.0, -9.8, 0.0))
+            r = sim.results()
+            assert r.converged == 1 and r.true_residual_rel <= 1e-6, (solver, k, r.as_dict())
+            if solver == abi.SOLVER_FCG_UNET:
+                assert r.fallback_used == 0, (k, r.as_dict())
+            its[solver].append(r.solver_iterations)
+        assert sim.results().live_particles == n
+        sim.close()
+    print("iterations: MGPCG", its[abi.SOLVER_MGPCG], "learned FCG", its[abi.SOLVER_FCG_UNET], flush=True)
+
