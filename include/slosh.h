@@ -398,7 +398,8 @@ typedef struct slosh_batch_config {
 } slosh_batch_config;
 slosh_status slosh_batch_create(slosh_ctx *const *tanks, uint32_t n, slosh_batch **out);
 void slosh_batch_destroy(slosh_batch *batch);
-/* One substep of every tank; gravity holds 3 floats per tank. */
+/* One substep of every tank; gravity holds 3 floats per tank. On an error status some tanks may have stopped
+ * part-way through the step: reset them (slosh_reset) before stepping again. */
 slosh_status slosh_batch_step(slosh_batch *batch, float dt, const float *gravity);
 slosh_status slosh_batch_set_config(slosh_batch *batch, const slosh_batch_config *cfg);
 slosh_status slosh_batch_get_config(const slosh_batch *batch, slosh_batch_config *cfg);
