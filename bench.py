@@ -82,9 +82,19 @@ def algorithmic_bytes(kernel, np_, nc, nf, nb, ns=None):
         # diag, r, z1 read; z1 written on red cells
         "k_red_dot": 14 * nf + ns,
         "k_pcg_init": 8 * nf + 16 * nf + 12 * nc + 4 * nc,
-        "k_sor_sweep": 16 * nf + 8 * nf // 2 + nc,
+        # a red and a black sweep: p, diag, div read once, p written on every fluid cell
+        "k_sor_pair": 16 * nf + 8 * nf + nc,
     }
     return table.get(kernel)
+
+
+def step_bytes(np_, nc, nf, iters):
+    """Bytes one plain-MGPCG step must move by the reference pass structure's byte model: 328 per particle (sort
+    pass 1 12, sort scatter 128, P2G particle side 60, G2P particle side 128), 258 per allocated cell (P2G grid
+    side 28, classify 13, normalise 52, divergence 33, project 47, two extrapolations 60, G2P grid side 15, solver
+    set-up 10), 108 per fluid cell of solver set-up, and per iteration 136 per fluid cell plus 8 per allocated
+    cell."""
+    return 328.0 * np_ + 258.0 * nc + 108.0 * nf + iters * (136.0 * nf + 8.0 * nc)
 
 
 def fluid_brick_cells(sim):
@@ -492,6 +502,15 @@ def main():
                    "learned_fcg": learned},
         "roofline": roofline,
     }
+    if args.solver == "mgpcg":
+        # the whole step against HBM: the per-step byte model at the timed steps' mean iteration count, over the
+        # measured time per step
+        sb = step_bytes(res.live_particles, nc, nf, sum(iters) / len(iters))
+        ach = sb / (el / args.steps) / 1e9
+        line["step_roofline"] = {"bound": "hbm", "bytes_per_step": int(sb), "achieved": round(ach, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                 "model": "328 Np + 258 Nc + 108 Nf + k (136 Nf + 8 Nc): Np particles, Nc "
+                                          "allocated cells, Nf fluid cells, k mean iterations per step"}
     if args.kernel_report:
         for k, (l, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
             if l:

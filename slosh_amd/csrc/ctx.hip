@@ -23,7 +23,7 @@ const char *const kKernelNames[K_COUNT] = {
     "k_pool", "k_p2g", "k_normalize", "k_divergence", "k_project", "k_div_stats", "k_extrapolate", "k_g2p_keys",
     "k_mg_setup", "k_coarse_factor", "k_pcg_init", "k_red_dot", "k_restrict0", "k_prolong_black", "k_direction",
     "k_update_black", "k_pdown_lds<4>", "k_pdown_lds<2>", "k_pup_lds<4>", "k_pup_lds<2>", "k_d23down", "k_d23up",
-    "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_sweep", "k_sor_residual",
+    "k_dsmooth", "k_drestrict", "k_dprolong", "k_coarse_solve", "k_coarse_tail", "k_sor_pair", "k_sor_residual",
     "k_fill", "k_finalize", "k_step_stats", "k_unet_conv_l0", "k_unet_conv_l1", "k_unet_conv_l2", "k_unet_io",
     "k_unet_train", "k_fcg", "k_g2p_move", "k_density", "k_viscosity", "k_coupling"};
 } // namespace slosh
@@ -68,7 +68,7 @@ void free_all(slosh_ctx *c)
     dfree(c->new_slot); dfree(c->touch);
     dfree(c->tiles); dfree(c->tile_valid);
     for (int k = 0; k < 3; ++k) dfree(c->vel[k]);
-    dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->div); dfree(c->diag); dfree(c->fmask);
+    dfree(c->fflags); dfree(c->fweight); dfree(c->ctype); dfree(c->pres); dfree(c->sor_tmp); dfree(c->div); dfree(c->diag); dfree(c->fmask);
     dfree(c->r); dfree(c->r2); dfree(c->z); dfree(c->z1); dfree(c->d); dfree(c->d2); dfree(c->q);
     dfree(c->binfo); dfree(c->binfo_f); dfree(c->fluid_flag); dfree(c->fluid_rank); dfree(c->tile_flag); dfree(c->tile_rank);
     dfree(c->tile_list); dfree(c->fcg_z); dfree(c->fcg_rp); dfree(c->capture);
@@ -224,6 +224,7 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->fweight, 3 * pc);
     dalloc(&c->ctype, pc);
     dalloc(&c->pres, pc);
+    if (s.pressure_solver == SLOSH_SOLVER_RBSOR) dalloc(&c->sor_tmp, pc);
     if (s.flags & SLOSH_FLAG_WARM_START_EXTRAPOLATE) dalloc(&c->pres_prev, pc);
     if (s.flags & SLOSH_FLAG_REPORT_FP64_ERROR) {
         dalloc(&c->pres_lp, pc);
@@ -371,6 +372,7 @@ void reset_impl(slosh_ctx *c)
     launch_fill_u8(c->stream, c->touch, 0u, g.nbricks);
     launch_fill_u8(c->stream, c->new_slot, 0u, g.npool);
     launch_fill_f64(c->stream, c->pres, 0.0, g.pool_cells);
+    if (c->sor_tmp) launch_fill_f64(c->stream, c->sor_tmp, 0.0, g.pool_cells);
     SLOSH_HIP(hipStreamSynchronize(c->stream));
 }
 

@@ -604,34 +604,213 @@ __device__ __forceinline__ void pcg_init_body(const Blk &G, uint32_t n, const ui
     col_partial<3>(v, partials, cb.e, C.lane);
 }
 
-// Red-black SOR sweep of one colour on the FP64 pressure: p += w ((sum p_n - div) / diag - p). The neighbours are of
-// the other colour, which the sweep does not write.
-__device__ __forceinline__ void sor_sweep_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask, const float *div,
-                                                           const float *diag, double *pres, uint32_t colour, double w,
-                                                           const SolverState *st)
+// c ? a : b, with a and b first passed through an empty asm: a select between two elements of a register array
+// would otherwise be folded into one load at a computed index (i + c), which sends the array to scratch memory.
+template <typename T> __device__ __forceinline__ T pick(bool c, T a, T b)
 {
+    __asm__("" : "+v"(a), "+v"(b));
+    return c ? a : b;
+}
+
+// One SOR update of a cell's FP64 pressure, p + w ((sum of the fluid neighbours' p - div) / diag - p): bit n of
+// the stencil mask m says neighbour n (-x +x -y +y -z +z, summed in that order) is fluid.
+__device__ __forceinline__ double sor_update(uint32_t m, const double (&nb)[6], float div, float diag, double old,
+                                             double w)
+{
+    double s = 0.0;
+    for (int n = 0; n < 6; ++n)
+        if (m >> n & 1u) s += nb[n];
+    const double dg = (double)(diag + kDiagEps);
+    const double gs = (s - (double)div) / dg;
+    return old + w * (gs - old);
+}
+
+// A red sweep and then a black sweep of red-black SOR in one pass, bit for bit the two sweeps. A red cell's
+// neighbours are black and a black cell's red, so each wavefront, from the old values alone, updates its brick's red
+// cells and the red cells just outside the brick that its black cells read (the cells beside its x and y faces and
+// the planes under and over it, each from its own old neighbours exactly as its own brick's wavefront updates it),
+// then its black cells; only the brick's own fluid cells are stored. The updates are made in place in the
+// registers: every red update reads black values only, which the red half does not change, and every black update
+// reads red values only. The pass reads one array and writes another (the neighbouring bricks' old values must stay
+// readable while they are updated); the cells it does not write (not fluid) are never used.
+//
+// The planes are taken in pairs (2j, 2j + 1): a lane's column has exactly one red and one black cell in each pair
+// (red in plane 2j + par, par the parity of x + y), so a lane makes one update per pair and colour, and the values
+// its neighbours need from it are all its cell of the other colour in that pair (the lanes beside it have the other
+// parity): one shuffle per direction serves both planes of a pair.
+//
+// Lane L < 32 holds face cell L (its plane k at C.h + 64 k; red planes k = 2 m + q, q the parity of the cell's
+// x + y) and loads, for those red planes, the cell's mask, divergence and diagonal and the two neighbours that no
+// other lane holds: the one beyond it (`deep`) and, at the ends of a face (j = 0, 7), the corner cell beside it in
+// an edge brick (`end`). Its neighbours along the face are lanes L -+ 1, the brick's own cell beside it is lane
+// `own`. Of the planes under and over the brick a lane updates the one whose cell is red (under: x + y odd, over:
+// even), and the face cell under or over face cell L that this needs is the one that cell's own red update needs at
+// the end of its column (`zf`). The per-lane addresses are formed with selects (a switch over the face would be
+// compiled to a table in scratch memory).
+__device__ __forceinline__ void sor_pair_body(const Blk &G, uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+                                              const float *div, const float *diag, const double *pres, double *pout,
+                                              double w, const SolverState *st)
+{
+    constexpr int NP = 8;
     const ColBrick cb = col_brick(G, n);
+    const uint32_t *row = binfo + (size_t)(cb.live ? cb.e : 0u) * kBinfoWords;
     const Brick B = load_brick(binfo, cb.live ? cb.e : 0u);
+    // the corner bricks beside the z edges (word (dx+1) + 3(dy+1) + 9(dz+1)) and the edge bricks under and over the
+    // faces (-x +x -y +y)
+    const uint32_t e_mm = row[9] * kBrickCells, e_pm = row[11] * kBrickCells, e_mp = row[15] * kBrickCells,
+                   e_pp = row[17] * kBrickCells;
+    const uint32_t zl0 = row[3] * kBrickCells, zl1 = row[5] * kBrickCells, zl2 = row[1] * kBrickCells,
+                   zl3 = row[7] * kBrickCells;
+    const uint32_t zh0 = row[21] * kBrickCells, zh1 = row[23] * kBrickCells, zh2 = row[19] * kBrickCells,
+                   zh3 = row[25] * kBrickCells;
     const uint32_t done = st->done;
     issue_point();
     if (done) return;
     if (!cb.live) return;
     const Col C = make_col(B, threadIdx.x);
-    PresCol P;
-    load_pres_col(C, fmask, div, diag, pres, P);
+    const uint32_t L = C.lane, f = (L >> 3) & 3u, j = L & 7u;
+    const bool par = ((C.x + C.y) & 1u) != 0u, q = ((C.hx + C.hy) & 1u) != 0u;
+    const uint32_t at = C.at, below = C.below, above = C.above, hat = C.h;
+    // face f of lane L < 32: 0 -x, 1 +x (f & 1: the + side), 2 -y, 3 +y (f & 2: a y face); j runs along the face
+    const bool fp = (f & 1u) != 0u, fy = (f & 2u) != 0u;
+    const bool face = L < 32u;
+    const uint32_t hl = C.hx + 8u * C.hy; // the face cell's index in its own brick
+    // the cell beyond it, one further out from the brick
+    const uint32_t deep = face ? (uint32_t)((int)hat + (fy ? (fp ? 8 : -8) : (fp ? 1 : -1))) : at;
+    // the brick's own cell beside it
+    const int own = face ? (int)(fy ? j + (fp ? 56u : 0u) : 8u * j + (fp ? 7u : 0u)) : (int)L;
+    // the corner cell at the end of the face: x side -1 or 8, y side -1 or 8
+    const bool cx8 = fy ? j == 7u : fp, cy8 = fy ? fp : j == 7u;
+    const uint32_t corner = cy8 ? (cx8 ? e_pp : e_mp + 7u) : (cx8 ? e_pm + 56u : e_mm + 63u);
+    const uint32_t end = face && (j == 0u || j == 7u) ? corner : at;
+    // the face cell's column continues under plane 0 in the edge brick under its brick, over plane 7 over it
+    const uint32_t zlb = fy ? (fp ? zl3 : zl2) : (fp ? zl1 : zl0), zhb = fy ? (fp ? zh3 : zh2) : (fp ? zh1 : zh0);
+    const uint32_t zlo = face ? zlb + hl + 448u : at, zhi = face ? zhb + hl : at;
+    double p[NP + 2], h[NP];
+    float dv[NP], gv[NP];
+    uint32_t mv[NP];
+    p[0] = pld(pres, below);
+    p[NP + 1] = pld(pres, above);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        double nb[6];
-        col_nbrs64(C, P.p[k + 1], P.h[k], P.p[k], P.p[k + 2], nb);
-        if (!(P.m[k] & kSelfFluid) || ((C.x + C.y + (uint32_t)k) & 1u) != colour) continue;
-        double s = 0.0;
-        for (int n = 0; n < 6; ++n)
-            if (P.m[k] >> n & 1u) s += nb[n];
-        const double dg = (double)(P.g[k] + kDiagEps);
-        const double gs = (s - (double)P.div[k]) / dg;
-        const double old = P.p[k + 1];
-        pst(pres, C.at + 64u * k, old + w * (gs - old));
+    for (int k = 0; k < NP; ++k) {
+        const uint32_t i = at + 64u * k;
+        mv[k] = pld(fmask, i);
+        dv[k] = pld(div, i);
+        gv[k] = pld(diag, i);
+        p[k + 1] = pld(pres, i);
+        h[k] = pld(pres, hat + 64u * k);
     }
+    // the red cell under or over the brick: mask, divergence, diagonal, the cell beyond it; the face cell's end
+    const uint32_t zc = par ? below : above;
+    const uint32_t zm_ = pld(fmask, zc);
+    const float zd = pld(div, zc), zg = pld(diag, zc);
+    const double zbeyond = pld(pres, par ? below - 64u : above + 64u);
+    const double zf = pld(pres, q ? zhi : zlo);
+    // the face cell's red planes
+    constexpr int NJ = NP / 2;
+    uint32_t hm[NJ];
+    float hd[NJ], hg[NJ];
+    double hdeep[NJ], hend[NJ];
+#pragma unroll
+    for (int m = 0; m < NJ; ++m) {
+        const uint32_t o = 64u * (2u * (uint32_t)m + q);
+        hm[m] = pld(fmask, hat + o);
+        hd[m] = pld(div, hat + o);
+        hg[m] = pld(diag, hat + o);
+        hdeep[m] = pld(pres, deep + o);
+        hend[m] = pld(pres, end + o);
+    }
+    issue_point(); // every load is in flight before the first update waits for one
+    // the four in-plane neighbours (-x +x -y +y) of the lane's cell from the values v its neighbour lanes offer and
+    // the value hv its face lanes offer
+    auto plane_nbrs = [&](double v, double hv, double (&nb)[6]) {
+        const double xm = __shfl_up(v, 1u, 64), xp = __shfl_down(v, 1u, 64);
+        const double ym = __shfl_up(v, 8u, 64), yp = __shfl_down(v, 8u, 64);
+        const double hxv = __shfl(hv, C.sx, 64), hyv = __shfl(hv, C.sy, 64);
+        nb[0] = C.x == 0u ? hxv : xm;
+        nb[1] = C.x == 7u ? hxv : xp;
+        nb[2] = C.y == 0u ? hyv : ym;
+        nb[3] = C.y == 7u ? hyv : yp;
+    };
+    // red: the brick's own cells, one per pair (plane 2j + par); the neighbours are the black cells of the pair
+    double bv[NJ]; // the lane's black value of each pair (the own-lane neighbour of a red face cell)
+#pragma unroll
+    for (int jj = 0; jj < NJ; ++jj) {
+        const int k0 = 2 * jj;
+        bv[jj] = pick(par, p[k0 + 1], p[k0 + 2]);
+        double nb[6];
+        plane_nbrs(bv[jj], pick(q, h[k0], h[k0 + 1]), nb);
+        nb[4] = pick(par, p[k0 + 1], p[k0]);
+        nb[5] = pick(par, p[k0 + 3], p[k0 + 2]);
+        const uint32_t m = pick(par, mv[k0 + 1], mv[k0]);
+        const double old = pick(par, p[k0 + 2], p[k0 + 1]);
+        const double v = sor_update(m, nb, pick(par, dv[k0 + 1], dv[k0]), pick(par, gv[k0 + 1], gv[k0]), old, w);
+        const bool upd = (m & kSelfFluid) != 0u;
+        p[k0 + 1] = upd && !par ? v : p[k0 + 1];
+        p[k0 + 2] = upd && par ? v : p[k0 + 2];
+    }
+    // red: the face cells (lanes 0-31), plane k = 2m + q; the lanes beside one along its face offer their black
+    // value of the pair, which is in plane k
+#pragma unroll
+    for (int m = 0; m < NJ; ++m) {
+        const double hb = pick(q, h[2 * m], h[2 * m + 1]);
+        const double a = __shfl_up(hb, 1u, 64), b = __shfl_down(hb, 1u, 64);
+        const double ov = __shfl(bv[m], own, 64);
+        const double lm = j == 0u ? hend[m] : a; // the face neighbour at j - 1
+        const double lp = j == 7u ? hend[m] : b; // and at j + 1
+        const double hc = pick(q, h[2 * m + 1], h[2 * m]);
+        const double zm = pick(q, h[2 * m], m == 0 ? zf : h[2 * m - 1]);
+        const double zp = pick(q, m == NJ - 1 ? zf : h[2 * m + 2], h[2 * m + 1]);
+        double nb[6];
+        if (f < 2u) {
+            nb[0] = f == 0u ? hdeep[m] : ov;
+            nb[1] = f == 0u ? ov : hdeep[m];
+            nb[2] = lm;
+            nb[3] = lp;
+        } else {
+            nb[0] = lm;
+            nb[1] = lp;
+            nb[2] = f == 2u ? hdeep[m] : ov;
+            nb[3] = f == 2u ? ov : hdeep[m];
+        }
+        nb[4] = zm;
+        nb[5] = zp;
+        const bool upd = L < 32u && (hm[m] & kSelfFluid);
+        const double v = sor_update(hm[m], nb, hd[m], hg[m], hc, w);
+        h[2 * m] = upd && !q ? v : h[2 * m];
+        h[2 * m + 1] = upd && q ? v : h[2 * m + 1];
+    }
+    // red: the cell under the brick (x + y odd) or over it (even); its in-plane neighbours are black, offered by the
+    // lanes beside it (the plane under by even lanes, the plane over by odd ones) and by the face lanes (their `zf`)
+    {
+        double nb[6];
+        plane_nbrs(pick(par, p[NP + 1], p[0]), zf, nb);
+        nb[4] = pick(par, zbeyond, p[NP]);
+        nb[5] = pick(par, p[1], zbeyond);
+        const double old = pick(par, p[0], p[NP + 1]);
+        const double v = sor_update(zm_, nb, zd, zg, old, w);
+        const bool upd = (zm_ & kSelfFluid) != 0u;
+        p[0] = upd && par ? v : p[0];
+        p[NP + 1] = upd && !par ? v : p[NP + 1];
+    }
+    // black: the brick's own cells, one per pair (plane 2j + 1 - par); the neighbours are the red cells of the pair
+#pragma unroll
+    for (int jj = 0; jj < NJ; ++jj) {
+        const int k0 = 2 * jj;
+        double nb[6];
+        plane_nbrs(pick(par, p[k0 + 2], p[k0 + 1]), pick(q, h[k0 + 1], h[k0]), nb);
+        nb[4] = pick(par, p[k0], p[k0 + 1]);
+        nb[5] = pick(par, p[k0 + 2], p[k0 + 3]);
+        const uint32_t m = pick(par, mv[k0], mv[k0 + 1]);
+        const double old = pick(par, p[k0 + 1], p[k0 + 2]);
+        const double v = sor_update(m, nb, pick(par, dv[k0], dv[k0 + 1]), pick(par, gv[k0], gv[k0 + 1]), old, w);
+        const bool upd = (m & kSelfFluid) != 0u;
+        p[k0 + 1] = upd && par ? v : p[k0 + 1];
+        p[k0 + 2] = upd && !par ? v : p[k0 + 2];
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k)
+        if (mv[k] & kSelfFluid) pst(pout, at + 64u * k, p[k + 1]);
 }
 
 // Partials ||b||^2 and ||b - A p||^2: mode 0 the SOR convergence check (skipped once the solver is done), 1 always,
@@ -711,11 +890,11 @@ __global__ __launch_bounds__(kColMaxThreads) void k_pcg_init(uint32_t n, const u
     pcg_init_body(this_blk(), n, binfo, fmask, div, diag, pres, r, partials);
 }
 
-__global__ __launch_bounds__(kColMaxThreads) void k_sor_sweep(uint32_t n, const uint32_t *binfo, const uint8_t *fmask, const float *div,
-                                                           const float *diag, double *pres, uint32_t colour, double w,
-                                                           const SolverState *st)
+__global__ __launch_bounds__(kColMaxThreads) void k_sor_pair(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
+                                                          const float *div, const float *diag, const double *pres,
+                                                          double *pout, double w, const SolverState *st)
 {
-    sor_sweep_body(this_blk(), n, binfo, fmask, div, diag, pres, colour, w, st);
+    sor_pair_body(this_blk(), n, binfo, fmask, div, diag, pres, pout, w, st);
 }
 
 __global__ __launch_bounds__(kColMaxThreads) void k_sor_residual(uint32_t n, const uint32_t *binfo, const uint8_t *fmask,
@@ -963,9 +1142,9 @@ void l0_fcg_update(slosh_ctx *c, float *r, const float *d)
     L0_LAUNCH(K_FCG, k_fcg_update, c->pres, r, d, c->q, c->partials, c->state);
 }
 
-void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w)
+void l0_sor_pair(slosh_ctx *c, const double *pin, double *pout, double w)
 {
-    L0C_LAUNCH(K_SOR_SWEEP, k_sor_sweep, c->div, c->diag, c->pres, colour, w, c->state);
+    L0C_LAUNCH(K_SOR_PAIR, k_sor_pair, c->div, c->diag, pin, pout, w, c->state);
 }
 
 void l0_sor_residual(slosh_ctx *c)

@@ -231,7 +231,7 @@ enum KernelId {
     K_EMIT = 0, K_SORT_TILES, K_SORT_RUNS, K_SORT_KEY_RUNS, K_SORT_SCATTER, K_SCAN, K_POOL, K_P2G, K_NORMALIZE,
     K_DIVERGENCE, K_PROJECT, K_DIV_STATS, K_EXTRAPOLATE, K_G2P, K_MG_SETUP, K_COARSE_FACTOR, K_PCG_INIT, K_RED_DOT,
     K_RESTRICT0, K_PROLONG_BLACK, K_DIRECTION, K_UPDATE_BLACK, K_PDOWN1, K_PDOWN2, K_PUP1, K_PUP2, K_D3DOWN, K_D3UP,
-    K_DSMOOTH, K_DRESTRICT, K_DPROLONG, K_COARSE_SOLVE, K_COARSE_TAIL, K_SOR_SWEEP, K_SOR_RESIDUAL, K_FILL,
+    K_DSMOOTH, K_DRESTRICT, K_DPROLONG, K_COARSE_SOLVE, K_COARSE_TAIL, K_SOR_PAIR, K_SOR_RESIDUAL, K_FILL,
     K_FINALIZE, K_STEP_STATS, K_UNET_CONV0, K_UNET_CONV1, K_UNET_CONV2, K_UNET_IO, K_UNET_TRAIN, K_FCG, K_G2P_MOVE,
     K_DENSITY, K_VISCOSITY, K_COUPLING, K_COUNT
 };
@@ -329,6 +329,7 @@ struct slosh_ctx {
     float *fweight = nullptr;
     uint32_t *ctype = nullptr;
     double *pres = nullptr;
+    double *sor_tmp = nullptr; // RB-SOR: the second pressure array of the two-colour passes
     float *div = nullptr, *diag = nullptr;
     uint8_t *fmask = nullptr; // solver stencil mask per cell (fluid neighbours, self fluid)
     // solver: residual and search direction ping-pong between iterations, z is the down-sweep iterate and z1 the
@@ -463,6 +464,8 @@ void launch_fill_f64(hipStream_t st, double *p, double v, size_t n);
 // ---- solver.hip ----
 void run_mgpcg(slosh_ctx *c);
 void run_rbsor(slosh_ctx *c);
+// `iters` red-black SOR iterations on c->pres (two-colour passes alternating between c->pres and c->sor_tmp).
+void sor_iterate(slosh_ctx *c, int iters, double w);
 // The density projection's solve A q = b, b = -dens_rhs, from q = 0, with the pressure solver's operator: two MGPCG
 // iterations (RB-SOR: density_projection_iterations sweeps at omega 1.7), as the reference's projection pass.
 void run_density_solve(slosh_ctx *c);
@@ -497,6 +500,8 @@ void l0_red_dot(slosh_ctx *c, const float *r);
 // d_out = z1 + beta d_in (first: d_out = z1), q = A d_out; partials d.q.
 void l0_direction(slosh_ctx *c, const float *d_in, float *d_out, const float *r, int first);
 void l0_sor_sweep(slosh_ctx *c, uint32_t colour, double w);
+// A red and then a black sweep in one pass, from pin into pout (distinct arrays whose non-fluid cells agree).
+void l0_sor_pair(slosh_ctx *c, const double *pin, double *pout, double w);
 // Partials ||b||^2 and ||b - A p||^2 of the FP64 pressure (the true residual).
 void l0_sor_residual(slosh_ctx *c);
 // The same partials whatever the solver state (the true residual of a finished solve).

@@ -1167,7 +1167,6 @@ int tail_start(const slosh_ctx *c)
 // t .. m-1, the grid-wide up passes, the fused level-3/2 up pass.
 void dense_part(slosh_ctx *c)
 {
-    const int m = c->nlevels;
     const DenseLevel L3 = dlevel_of(c, 3), L4 = dlevel_of(c, kFirstDense);
     const PLevel L2 = plevel_of(c, 2);
     {
@@ -1624,10 +1623,7 @@ void run_density_solve(slosh_ctx *c)
     if (c->s.pressure_solver == SLOSH_SOLVER_RBSOR) {
         const int iters = c->s.density_projection_iterations > 0 ? c->s.density_projection_iterations : 1;
         hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(1), 0, c->stream, c->state, 0.0, iters);
-        for (int it = 0; it < iters; ++it) {
-            l0_sor_sweep(c, 0u, (double)1.7f);
-            l0_sor_sweep(c, 1u, (double)1.7f);
-        }
+        sor_iterate(c, iters, (double)1.7f);
         SLOSH_HIP(hipGetLastError());
         return;
     }
@@ -1640,6 +1636,25 @@ void run_density_solve(slosh_ctx *c)
 }
 
 // Slosh.usf:856-892 (PressureRBGSCS) iterated to the tolerance, residual checked every 8 iterations.
+// `iters` red-black SOR iterations on c->pres, each one two-colour pass from one of c->pres and c->sor_tmp into the
+// other, the last one into c->pres (an odd count starts from a copy in c->sor_tmp). A pass writes every fluid cell
+// and reads the other cells' values only through masks that exclude them, so what c->sor_tmp holds there is never
+// used. The passes of a solve that is already done return at once, leaving c->pres as it is.
+void sor_iterate(slosh_ctx *c, int iters, double w)
+{
+    if (iters <= 0) return;
+    double *a = c->pres, *b = c->sor_tmp;
+    if (iters & 1) {
+        SLOSH_HIP(hipMemcpyAsync(b, a, (size_t)c->geo.pool_cells * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        std::swap(a, b);
+    }
+    for (int k = 0; k < iters; ++k) {
+        l0_sor_pair(c, a, b, w);
+        std::swap(a, b);
+    }
+}
+
 void run_rbsor(slosh_ctx *c)
 {
     if (!c->nactive) return;
@@ -1661,10 +1676,9 @@ void run_rbsor(slosh_ctx *c)
     residual(0);
     while (it < maxit) {
         for (int chunk = 0; chunk < 16 && it < maxit; ++chunk) {
-            for (int k = 0; k < 8 && it < maxit; ++k, ++it) {
-                l0_sor_sweep(c, 0u, w);
-                l0_sor_sweep(c, 1u, w);
-            }
+            const int k = maxit - it < 8 ? maxit - it : 8;
+            sor_iterate(c, k, w);
+            it += k;
             residual(it);
         }
         SLOSH_HIP(hipGetLastError());
