@@ -211,6 +211,33 @@ __device__ __forceinline__ void col_nbrs(const Col &C, float v, float hv, float 
     nb[5] = hi;
 }
 
+// c ? a : b, with a and b first passed through an empty asm: a select between two elements of a register array
+// would otherwise be folded into one load at a computed index (i + c), which sends the array to scratch memory.
+template <typename T> __device__ __forceinline__ T pick(bool c, T a, T b)
+{
+    __asm__("" : "+v"(a), "+v"(b));
+    return c ? a : b;
+}
+
+// The four in-plane neighbours (-x +x -y +y) of the lane's cell when every lane offers v and lanes 0-31 offer hv
+// (their face cell's value); nb[4] and nb[5] are left to the caller.
+template <typename T> __device__ __forceinline__ void plane_nbrs(const Col &C, T v, T hv, T (&nb)[6])
+{
+    const T xm = __shfl_up(v, 1u, 64), xp = __shfl_down(v, 1u, 64);
+    const T ym = __shfl_up(v, 8u, 64), yp = __shfl_down(v, 8u, 64);
+    const T hxv = __shfl(hv, C.sx, 64), hyv = __shfl(hv, C.sy, 64);
+    nb[0] = C.x == 0u ? hxv : xm;
+    nb[1] = C.x == 7u ? hxv : xp;
+    nb[2] = C.y == 0u ? hyv : ym;
+    nb[3] = C.y == 7u ? hyv : yp;
+}
+
+// Red-black passes take a column's planes in pairs (2j, 2j + 1): the lane's red cell of pair j is in plane 2j + par
+// and its black cell in plane 2j + 1 - par (par: parity of x + y), so a lane makes one update per pair and colour.
+// The lanes beside it have the other parity: the value a red cell needs from each of them is that lane's black cell
+// of the same pair, and the other way round, so one shuffle per direction serves both planes of a pair. A face cell
+// (lanes 0-31) has parity q = parity of its own x + y: its red cell of pair j is in plane 2j + q.
+
 // Writes the brick's partials (the wavefront sums of v) to partials[e * NV + k].
 template <int NV> __device__ __forceinline__ void col_partial(const double (&v)[NV], double *partials, uint32_t e,
                                                               uint32_t lane)
@@ -242,7 +269,7 @@ __device__ __forceinline__ void update_black_body(const Blk &G, uint32_t n, cons
     if (done == 1u) return; // done == 2: the solve's last update, made before every other pass stops
     if (!cb.live) return;
     const Col C = make_col(B, threadIdx.x);
-    // the red value r / diag of a neighbour cell after the update (0 when not needed)
+    // the red value r / diag of a neighbour cell after the update
     auto red_of = [&](uint32_t i) {
         const float ra = pld(r_in, i);
         const float g = pld(diag, i);
@@ -250,53 +277,60 @@ __device__ __forceinline__ void update_black_body(const Blk &G, uint32_t n, cons
         const float r = first ? ra : (float)((double)ra - alpha * (double)qv);
         return r / (g + kDiagEps);
     };
-    float zr[10], hr[8], ra[8], qv[8], g[8];
+    // pairs of planes: the red values of the face cells and of the cell under (par) or over the column
+    const bool par = ((C.x + C.y) & 1u) != 0u, hq = ((C.hx + C.hy) & 1u) != 0u;
+    const uint32_t red0 = C.at + (par ? 64u : 0u), blk0 = C.at + (par ? 0u : 64u), hred0 = C.h + (hq ? 64u : 0u);
     uint32_t m[8];
-    zr[0] = red_of(C.below);
-    zr[9] = red_of(C.above);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint32_t i = C.at + 64u * k;
-        m[k] = pld(fmask, i);
-        ra[k] = pld(r_in, i);
-        g[k] = pld(diag, i);
-        qv[k] = first ? 0.0f : pld(q, i);
-        hr[k] = red_of(C.h + 64u * k);
-    }
-    float rn[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        rn[k] = first ? ra[k] : (float)((double)ra[k] - alpha * (double)qv[k]);
-        zr[k + 1] = rn[k] / (g[k] + kDiagEps);
-    }
-    // the pressure update's loads are issued once the red values are formed (not hoisted above them: all of them
-    // in flight at once would cost occupancy)
-    issue_point();
-    double pv[8];
-    float dv[8];
+    for (int k = 0; k < 8; ++k) m[k] = pld(fmask, C.at + 64u * k);
+    // the pressure update first: its values are dead once stored, so they do not add to the registers the
+    // multigrid part holds
     if (!first) {
+        double pv[8];
+        float dv[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             pv[k] = pld(pres, C.at + 64u * k);
             dv[k] = pld(d, C.at + 64u * k);
         }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (m[k] & kSelfFluid) pst(pres, C.at + 64u * k, pv[k] + alpha * (double)dv[k]);
     }
+    issue_point();
+    const float zext = red_of(par ? C.below : C.above);
+    float hr[4], ra[8], qv[8], g[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        float nb[6];
-        col_nbrs(C, zr[k + 1], hr[k], zr[k], zr[k + 2], nb);
-        if (!(m[k] & kSelfFluid)) continue;
         const uint32_t i = C.at + 64u * k;
-        if (!first) pst(pres, i, pv[k] + alpha * (double)dv[k]);
-        pst(r_out, i, rn[k]);
-        if (((C.x + C.y + (uint32_t)k) & 1u) == 0u) {
-            pst(z, i, zr[k + 1]);
-        } else {
-            float s = 0.0f;
-            for (int n = 0; n < 6; ++n)
-                if (m[k] >> n & 1u) s += nb[n];
-            pst(z, i, (rn[k] + s) / (g[k] + kDiagEps));
-        }
+        ra[k] = pld(r_in, i);
+        g[k] = pld(diag, i);
+        qv[k] = first ? 0.0f : pld(q, i);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) hr[j] = red_of(hred0 + 128u * j);
+    float rn[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) rn[k] = first ? ra[k] : (float)((double)ra[k] - alpha * (double)qv[k]);
+    float zr[4]; // the red cells' z
+#pragma unroll
+    for (int j = 0; j < 4; ++j) zr[j] = pick(par, rn[2 * j + 1], rn[2 * j]) / (pick(par, g[2 * j + 1], g[2 * j]) + kDiagEps);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (m[k] & kSelfFluid) pst(r_out, C.at + 64u * k, rn[k]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t mr = pick(par, m[2 * j + 1], m[2 * j]), mb = pick(par, m[2 * j], m[2 * j + 1]);
+        if (mr & kSelfFluid) pst(z, red0 + 128u * j, zr[j]);
+        float nb[6];
+        plane_nbrs(C, zr[j], hr[j], nb);
+        nb[4] = pick(par, j == 0 ? zext : zr[j - 1], zr[j]);
+        nb[5] = pick(par, zr[j], j == 3 ? zext : zr[j + 1]);
+        float s = 0.0f;
+        for (int t = 0; t < 6; ++t)
+            if (mb >> t & 1u) s += nb[t];
+        if (mb & kSelfFluid)
+            pst(z, blk0 + 128u * j, (pick(par, rn[2 * j], rn[2 * j + 1]) + s) / (pick(par, g[2 * j], g[2 * j + 1]) + kDiagEps));
     }
 }
 
@@ -399,33 +433,35 @@ __device__ __forceinline__ void prolong_black_body(const Blk &G, uint32_t n, con
     auto parent = [](uint32_t base, uint32_t x, uint32_t y, uint32_t zz) {
         return (base >> 3) + (x >> 1) + 4u * (y >> 1) + 16u * (zz >> 1);
     };
-    float w[10], hw[8], rv[8], gv[8];
+    // pairs of planes (the two planes of a pair share their level-1 parents): w is needed on the red cells only,
+    // r and the diagonal on the black ones
+    const bool par = ((C.x + C.y) & 1u) != 0u, q = ((C.hx + C.hy) & 1u) != 0u;
+    const uint32_t red0 = C.at + (par ? 64u : 0u), blk0 = C.at + (par ? 0u : 64u), hred0 = C.h + (q ? 64u : 0u);
+    // the red cell under the column (par) or over it
+    const float wext = par ? w_of(C.below, parent(B.nb[4], C.x, C.y, 7u)) : w_of(C.above, parent(B.nb[5], C.x, C.y, 0u));
+    float wr[4], hw[4], rb[4], gb[4];
     uint32_t m[8];
-    w[0] = w_of(C.below, parent(B.nb[4], C.x, C.y, 7u));
-    w[9] = w_of(C.above, parent(B.nb[5], C.x, C.y, 0u));
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint32_t i = C.at + 64u * k;
-        m[k] = pld(fmask, i);
-        rv[k] = pld(r, i);
-        gv[k] = pld(diag, i);
-        w[k + 1] = w_of(i, parent(B.base, C.x, C.y, (uint32_t)k));
-        hw[k] = w_of(C.h + 64u * k, parent(C.hbase, C.hx, C.hy, (uint32_t)k));
+    for (int k = 0; k < 8; ++k) m[k] = pld(fmask, C.at + 64u * k);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        wr[j] = w_of(red0 + 128u * j, parent(B.base, C.x, C.y, 2u * j));
+        hw[j] = w_of(hred0 + 128u * j, parent(C.hbase, C.hx, C.hy, 2u * j));
+        rb[j] = pld(r, blk0 + 128u * j);
+        gb[j] = pld(diag, blk0 + 128u * j);
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t mr = pick(par, m[2 * j + 1], m[2 * j]), mb = pick(par, m[2 * j], m[2 * j + 1]);
+        if (mr & kSelfFluid) pst(z1, red0 + 128u * j, wr[j]);
         float nb[6];
-        col_nbrs(C, w[k + 1], hw[k], w[k], w[k + 2], nb);
-        if (!(m[k] & kSelfFluid)) continue;
-        const uint32_t i = C.at + 64u * k;
-        if (((C.x + C.y + (uint32_t)k) & 1u) == 0u) {
-            pst(z1, i, w[k + 1]);
-        } else {
-            float s = 0.0f;
-            for (int n = 0; n < 6; ++n)
-                if (m[k] >> n & 1u) s += nb[n];
-            pst(z1, i, (rv[k] + s) / (gv[k] + kDiagEps));
-        }
+        plane_nbrs(C, wr[j], hw[j], nb);
+        nb[4] = pick(par, j == 0 ? wext : wr[j - 1], wr[j]);
+        nb[5] = pick(par, wr[j], j == 3 ? wext : wr[j + 1]);
+        float s = 0.0f;
+        for (int t = 0; t < 6; ++t)
+            if (mb >> t & 1u) s += nb[t];
+        if (mb & kSelfFluid) pst(z1, blk0 + 128u * j, (rb[j] + s) / (gb[j] + kDiagEps));
     }
 }
 
@@ -441,34 +477,43 @@ __device__ __forceinline__ void red_dot_body(const Blk &G, uint32_t n, const uin
     if (done) return;
     if (!cb.live) return;
     const Col C = make_col(B, threadIdx.x);
-    float zv[10], hv[8], rv[8], gv[8];
+    // pairs of planes: the red cell of pair j at red0 + 128 j, the black one at blk0 + 128 j, the face cell's black
+    // one at hblk0 + 128 j
+    const bool par = ((C.x + C.y) & 1u) != 0u, q = ((C.hx + C.hy) & 1u) != 0u;
+    const uint32_t red0 = C.at + (par ? 64u : 0u), blk0 = C.at + (par ? 0u : 64u), hblk0 = C.h + (q ? 0u : 64u);
+    float rv[8], zb[4], gr[4], hb[4];
     uint32_t m[8];
-    zv[0] = pld(z1, C.below);
-    zv[9] = pld(z1, C.above);
+    const float zext = pld(z1, par ? C.above : C.below); // the black cell over the column (par) or under it
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const uint32_t i = C.at + 64u * k;
-        m[k] = pld(fmask, i);
-        zv[k + 1] = pld(z1, i);
-        rv[k] = pld(r, i);
-        gv[k] = pld(diag, i);
-        hv[k] = pld(z1, C.h + 64u * k);
+        m[k] = pld(fmask, C.at + 64u * k);
+        rv[k] = pld(r, C.at + 64u * k);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        zb[j] = pld(z1, blk0 + 128u * j);
+        gr[j] = pld(diag, red0 + 128u * j);
+        hb[j] = pld(z1, hblk0 + 128u * j);
     }
     double v[1] = {0.0};
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int j = 0; j < 4; ++j) {
         float nb[6];
-        col_nbrs(C, zv[k + 1], hv[k], zv[k], zv[k + 2], nb);
-        if (!(m[k] & kSelfFluid)) continue;
-        float xv = zv[k + 1];
-        if (((C.x + C.y + (uint32_t)k) & 1u) == 0u) {
-            float s = 0.0f;
-            for (int n = 0; n < 6; ++n)
-                if (m[k] >> n & 1u) s += nb[n];
-            xv = (rv[k] + s) / (gv[k] + kDiagEps);
-            pst(z1, C.at + 64u * k, xv);
-        }
-        v[0] += (double)rv[k] * (double)xv;
+        plane_nbrs(C, zb[j], hb[j], nb);
+        nb[4] = pick(par, zb[j], j == 0 ? zext : zb[j - 1]);
+        nb[5] = pick(par, j == 3 ? zext : zb[j + 1], zb[j]);
+        const uint32_t mr = pick(par, m[2 * j + 1], m[2 * j]), mb = pick(par, m[2 * j], m[2 * j + 1]);
+        const float rr = pick(par, rv[2 * j + 1], rv[2 * j]), rb = pick(par, rv[2 * j], rv[2 * j + 1]);
+        float s = 0.0f;
+        for (int t = 0; t < 6; ++t)
+            if (mr >> t & 1u) s += nb[t];
+        const float xr = (rr + s) / (gr[j] + kDiagEps);
+        if (mr & kSelfFluid) pst(z1, red0 + 128u * j, xr);
+        // the dot over the column in plane order (a skipped cell adds +0.0, which leaves the sum's bits unchanged)
+        const double c_r = (mr & kSelfFluid) ? (double)rr * (double)xr : 0.0;
+        const double c_b = (mb & kSelfFluid) ? (double)rb * (double)zb[j] : 0.0;
+        v[0] += pick(par, c_b, c_r);
+        v[0] += pick(par, c_r, c_b);
     }
     col_partial<1>(v, partials, cb.e, C.lane);
 }
@@ -604,14 +649,6 @@ __device__ __forceinline__ void pcg_init_body(const Blk &G, uint32_t n, const ui
     col_partial<3>(v, partials, cb.e, C.lane);
 }
 
-// c ? a : b, with a and b first passed through an empty asm: a select between two elements of a register array
-// would otherwise be folded into one load at a computed index (i + c), which sends the array to scratch memory.
-template <typename T> __device__ __forceinline__ T pick(bool c, T a, T b)
-{
-    __asm__("" : "+v"(a), "+v"(b));
-    return c ? a : b;
-}
-
 // One SOR update of a cell's FP64 pressure, p + w ((sum of the fluid neighbours' p - div) / diag - p): bit n of
 // the stencil mask m says neighbour n (-x +x -y +y -z +z, summed in that order) is fluid.
 __device__ __forceinline__ double sor_update(uint32_t m, const double (&nb)[6], float div, float diag, double old,
@@ -721,17 +758,6 @@ __device__ __forceinline__ void sor_pair_body(const Blk &G, uint32_t n, const ui
         hend[m] = pld(pres, end + o);
     }
     issue_point(); // every load is in flight before the first update waits for one
-    // the four in-plane neighbours (-x +x -y +y) of the lane's cell from the values v its neighbour lanes offer and
-    // the value hv its face lanes offer
-    auto plane_nbrs = [&](double v, double hv, double (&nb)[6]) {
-        const double xm = __shfl_up(v, 1u, 64), xp = __shfl_down(v, 1u, 64);
-        const double ym = __shfl_up(v, 8u, 64), yp = __shfl_down(v, 8u, 64);
-        const double hxv = __shfl(hv, C.sx, 64), hyv = __shfl(hv, C.sy, 64);
-        nb[0] = C.x == 0u ? hxv : xm;
-        nb[1] = C.x == 7u ? hxv : xp;
-        nb[2] = C.y == 0u ? hyv : ym;
-        nb[3] = C.y == 7u ? hyv : yp;
-    };
     // red: the brick's own cells, one per pair (plane 2j + par); the neighbours are the black cells of the pair
     double bv[NJ]; // the lane's black value of each pair (the own-lane neighbour of a red face cell)
 #pragma unroll
@@ -739,7 +765,7 @@ __device__ __forceinline__ void sor_pair_body(const Blk &G, uint32_t n, const ui
         const int k0 = 2 * jj;
         bv[jj] = pick(par, p[k0 + 1], p[k0 + 2]);
         double nb[6];
-        plane_nbrs(bv[jj], pick(q, h[k0], h[k0 + 1]), nb);
+        plane_nbrs(C, bv[jj], pick(q, h[k0], h[k0 + 1]), nb);
         nb[4] = pick(par, p[k0 + 1], p[k0]);
         nb[5] = pick(par, p[k0 + 3], p[k0 + 2]);
         const uint32_t m = pick(par, mv[k0 + 1], mv[k0]);
@@ -784,7 +810,7 @@ __device__ __forceinline__ void sor_pair_body(const Blk &G, uint32_t n, const ui
     // lanes beside it (the plane under by even lanes, the plane over by odd ones) and by the face lanes (their `zf`)
     {
         double nb[6];
-        plane_nbrs(pick(par, p[NP + 1], p[0]), zf, nb);
+        plane_nbrs(C, pick(par, p[NP + 1], p[0]), zf, nb);
         nb[4] = pick(par, zbeyond, p[NP]);
         nb[5] = pick(par, p[1], zbeyond);
         const double old = pick(par, p[0], p[NP + 1]);
@@ -798,7 +824,7 @@ __device__ __forceinline__ void sor_pair_body(const Blk &G, uint32_t n, const ui
     for (int jj = 0; jj < NJ; ++jj) {
         const int k0 = 2 * jj;
         double nb[6];
-        plane_nbrs(pick(par, p[k0 + 2], p[k0 + 1]), pick(q, h[k0 + 1], h[k0]), nb);
+        plane_nbrs(C, pick(par, p[k0 + 2], p[k0 + 1]), pick(q, h[k0 + 1], h[k0]), nb);
         nb[4] = pick(par, p[k0], p[k0 + 1]);
         nb[5] = pick(par, p[k0 + 2], p[k0 + 3]);
         const uint32_t m = pick(par, mv[k0], mv[k0 + 1]);
