@@ -7,8 +7,9 @@ A step is one full substep of the solver: sort, brick pool, P2G, normalise, dive
 projection, extrapolation and G2P, with inputs resident in HBM.
 
 Prints ONE JSON line with steps/s, the roofline of the dominant kernel (HIP events on the library's stream over
-the timed steps, against the algorithmic bytes of that kernel) and a CPU baseline: the oracle (a single-threaded C
-restatement of the reference kernels) timed on one step of the same workload.
+further steps, against the algorithmic bytes of that kernel; its measured HBM traffic and VALU issue share from the
+committed rocprofv3 summaries), the whole step against the per-step byte model, and a CPU baseline: the oracle (a C
+restatement of the reference kernels, OpenMP) on all host threads over the workload's first steps, and on one.
 """
 import argparse
 import json
@@ -117,6 +118,19 @@ def measured_traffic(workload, kernel):
         f = line.split()
         if len(f) >= 7 and not line.startswith("#") and f[0] == workload and f[1] == kernel:
             return int(float(f[6]))
+    return None
+
+
+def measured_valu(workload, kernel):
+    """Share of the kernel's time the GPU's SIMDs spend issuing VALU instructions, from the committed rocprofv3 PMC
+    summary (4 cycles per wave64 instruction per SIMD over the dispatch's elapsed cycles), or None."""
+    path = os.path.join(ROOT, "profiles", "valu_issue.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 9 and not line.startswith("#") and f[0] == workload and f[1] == kernel:
+            return float(f[8])
     return None
 
 
@@ -425,7 +439,8 @@ def main():
     ns = fluid_brick_cells(sim)
     abytes = algorithmic_bytes(dominant, res.live_particles, nc, nf, nb, ns)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": None, "traffic": measured_traffic(args.workload, dominant), "launches": launches,
+                "frac": None, "traffic": measured_traffic(args.workload, dominant),
+                "valu_issue_frac": measured_valu(args.workload, dominant), "launches": launches,
                 "avg_us": avg_s * 1e6,
                 "timing": f"a HIP event pair carried by each launch's dispatch on the library's stream (the kernel's "
                           f"own begin and end), {args.roofline_steps} steps after the timed ones"}
