@@ -242,14 +242,18 @@ def test_gpu_density_projection_bitexact_with_a_written_potential():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("solver", [abi.SOLVER_MGPCG, abi.SOLVER_RBSOR])
-def test_gpu_density_potential_matches_the_oracle(solver):
+@pytest.mark.parametrize("solver,iters", [(abi.SOLVER_MGPCG, 30), (abi.SOLVER_RBSOR, 30), (abi.SOLVER_RBSOR, 7)])
+def test_gpu_density_potential_matches_the_oracle(solver, iters):
+    """The density projection's potential: two MGPCG iterations within 1e-5 of the oracle's (FP64 dots summed in
+    another order), or `iters` RB-SOR iterations at omega 1.7 from q = 0 bit for bit (the same FP64 updates in the
+    same order; an odd count ends in the library's second pressure array and is copied back)."""
     n = 32
-    print(f"case density projection solve {solver}", flush=True)
+    print(f"case density projection solve {solver}, {iters} iterations", flush=True)
     grid, h, reqs = scenes.dam_break(n, seed=4)
     from slosh_amd import Sim
 
-    s = abi.make_settings(grid, h, 8 * n ** 3, solver=solver, flags=FLAG, max_iterations=3000)
+    s = abi.make_settings(grid, h, 8 * n ** 3, solver=solver, flags=FLAG, max_iterations=3000,
+                          density_iterations=iters)
     gpu, orc = Sim(s), O.OracleSim(s)
     for sim in (gpu, orc):
         for r in reqs:
@@ -262,7 +266,10 @@ def test_gpu_density_potential_matches_the_oracle(solver):
     assert np.abs(qo).max() > 1e-3
     err = np.linalg.norm(qg - qo) / np.linalg.norm(qo)
     print(f"potential difference {err:.2e}", flush=True)
-    assert err <= 1e-5
+    if solver == abi.SOLVER_RBSOR:
+        same_bits(qg, qo, "potential")
+    else:
+        assert err <= 1e-5
 
 
 @pytest.mark.gpu
