@@ -79,7 +79,7 @@ void free_all(slosh_ctx *c)
         dfree(c->lv[l].x); dfree(c->lv[l].x1); dfree(c->lv[l].b);
     }
     dfree(c->dense_info); dfree(c->dense_coord);
-    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->coarse_sig); dfree(c->pres_prev); dfree(c->pres_lp); dfree(c->state_save); dfree(c->partials);
+    dfree(c->coarse_idx); dfree(c->coarse_n); dfree(c->coarse_inv); dfree(c->coarse_sig); dfree(c->pres_prev); dfree(c->pres_lp); dfree(c->state_save); dfree(c->partials); dfree(c->fold);
     dfree(c->dens_rhs); dfree(c->dens_q); dfree(c->dens_disp);
     dfree(c->bp_fluid); dfree(c->bp_div_max); dfree(c->bp_div_cnt); dfree(c->bp_div_sum); dfree(c->bp_speed);
     dfree(c->bp_clamp); dfree(c->bp_drain); dfree(c->d_man);
@@ -321,6 +321,7 @@ void create_impl(slosh_ctx *c)
     dalloc(&c->coarse_inv, (size_t)kCoarsestCells * kCoarsestCells);
     dalloc(&c->coarse_sig, (size_t)kCoarsestCells + 1u);
     dalloc(&c->partials, (size_t)kMaxBlockPartials * ((size_t)g.npool + 1u));
+    dalloc(&c->fold, (size_t)kMaxBlockPartials * kFoldGroups);
     dalloc(&c->bp_fluid, g.npool);
     dalloc(&c->bp_div_max, g.npool);
     dalloc(&c->bp_div_cnt, g.npool);

@@ -89,6 +89,12 @@ constexpr uint32_t kBinfoWords = 32u;
 // Most values a level-0 reduction writes per workgroup (k_direction: d.q, r.q, q.q, r.r; k_pcg_init three); the
 // partials buffer holds this many per pool slot, one workgroup per allocated brick.
 constexpr uint32_t kMaxBlockPartials = 4u;
+// Reductions over more partial sets than kFinOneMax (one set per brick: grids of about 400^3 and more) are first
+// folded by up to kFoldGroups workgroups into the context's `fold` array, so that no single workgroup reads them
+// all. A tank above the bound solves on its own stream rather than in a lockstep batch, whose finalize reads each
+// tank's partials in one workgroup.
+constexpr uint32_t kFinOneMax = 16384u;
+constexpr uint32_t kFoldGroups = 512u;
 
 // Wavefront sum of an FP64 value in a fixed order, without LDS round trips: inclusive sums within each row of 16
 // lanes by DPP row shifts of 1, 2, 4 and 8 (a lane shifted in from outside its row adds +0), then the four row
@@ -360,6 +366,7 @@ struct slosh_ctx {
     slosh::SolverState *state_save = nullptr;
     uint32_t *coarse_sig = nullptr; // the coarsest level's cell types and indices at its last factorisation
     double *partials = nullptr;
+    double *fold = nullptr; // [kFoldGroups][kMaxBlockPartials]: the partials of a large reduction, folded
     uint32_t nparts = 0;              // partial sets the last reduction pass wrote (what the next finalize sums)
     uint32_t col_bricks = 4;          // bricks (wavefronts) per workgroup of the level-0 column passes: 1, 2 or 4
     // the step between its two halves (step_front / step_back): dt, gravity and the last stage

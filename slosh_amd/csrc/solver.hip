@@ -221,6 +221,33 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(int kind, const double
     finalize_body(kind, partials, nblocks, iters, st);
 }
 
+// First stage of a large reduction: workgroup g sums partial sets [g chunk, (g + 1) chunk) in a fixed order (per
+// thread its strided run in increasing order, then wave_sum and the four wavefront sums in order) into fold[g].
+// It returns at once where the finalize it feeds would (a solve already done), like every pass of an iteration.
+__global__ __launch_bounds__(256) void k_fold_partials(int kind, const double *partials, uint32_t nblocks,
+                                                       uint32_t chunk, double *fold, const SolverState *st)
+{
+    constexpr int kMaxV = 4;
+    __shared__ double sh[kMaxV][4];
+    const uint32_t done = st->done;
+    issue_point();
+    if (kind == kFinCheckDone && !done) return;
+    if (kind != kFinInit && kind != kFinCheck && kind != kFinCheckDone && kind != kFinRestart && kind != kFinErr &&
+        done)
+        return;
+    const int nv = fin_values(kind);
+    const uint32_t b0 = blockIdx.x * chunk, b1 = min(b0 + chunk, nblocks);
+    double a[kMaxV] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = b0 + threadIdx.x; b < b1; b += 256u)
+        for (int j = 0; j < kMaxV; ++j) a[j] += j < nv ? partials[(size_t)b * nv + j] : 0.0;
+    for (int j = 0; j < kMaxV; ++j) a[j] = wave_sum(a[j]);
+    if ((threadIdx.x & 63u) == 0u)
+        for (int j = 0; j < kMaxV; ++j) sh[j][threadIdx.x >> 6] = a[j];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int j = 0; j < nv; ++j) fold[(size_t)blockIdx.x * nv + j] = ((sh[j][0] + sh[j][1]) + sh[j][2]) + sh[j][3];
+}
+
 // The finalize of every tank of a batch (workgroup t: tank t, whose partials are per brick with fluid).
 __global__ __launch_bounds__(kFinThreads) void kb_finalize(int kind, const BatchTank *T)
 {
@@ -1123,10 +1150,16 @@ __global__ void k_state_reset(SolverState *st, double tol, int maxit)
 void finalize(slosh_ctx *c, int kind, int iters = 0)
 {
     SLOSH_TIMED(c, K_FINALIZE);
-    const uint32_t nblocks = c->nparts;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, c->stream, kind, c->partials, nblocks, iters,
-                       c->state);
-    (void)fin_values;
+    uint32_t nblocks = c->nparts;
+    const double *parts = c->partials;
+    if (nblocks > kFinOneMax) {
+        const uint32_t chunk = (nblocks + kFoldGroups - 1u) / kFoldGroups, groups = (nblocks + chunk - 1u) / chunk;
+        hipLaunchKernelGGL(k_fold_partials, dim3(groups), dim3(256), 0, c->stream, kind, c->partials, nblocks, chunk,
+                           c->fold, c->state);
+        parts = c->fold;
+        nblocks = groups;
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, c->stream, kind, parts, nblocks, iters, c->state);
 }
 
 void coarse_solve(slosh_ctx *c, LVec b, LVec x)
@@ -1696,6 +1729,8 @@ bool mgpcg_batchable(const slosh_ctx *c)
 {
     if (c->s.pressure_solver != SLOSH_SOLVER_MGPCG || c->timing_mask) return false;
     if (c->s.flags & (SLOSH_FLAG_CAPTURE_RESIDUALS | SLOSH_FLAG_REPORT_FP64_ERROR)) return false;
+    // the batch's finalize reads a tank's partials in one workgroup (at most one set per allocated brick)
+    if (c->nactive > kFinOneMax) return false;
     return c->nlevels >= 5 && tail_start(c) == kFirstDense;
 }
 
