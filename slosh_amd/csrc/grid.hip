@@ -16,6 +16,15 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+__global__ void k_zero3_u32(uint32_t *a, uint32_t *b, uint32_t *c, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        a[i] = 0u;
+        b[i] = 0u;
+        c[i] = 0u;
+    }
+}
 __global__ void k_fill_u8(uint8_t *p, uint8_t v, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -56,8 +65,13 @@ constexpr uint32_t kScanOneLds = kScanOneMax + kScanOneMax / 32u;
 
 __device__ __forceinline__ uint32_t scan_pad(uint32_t i) { return i + (i >> 5); }
 
-__global__ __launch_bounds__(kScanOneThreads) void k_scan_one(const uint32_t *in, uint32_t n, uint32_t *out)
+// Two independent scans can share a launch: workgroup 1 scans (in1, n1) into out1.
+__global__ __launch_bounds__(kScanOneThreads) void k_scan_one(const uint32_t *in0, uint32_t n0, uint32_t *out0,
+                                                              const uint32_t *in1, uint32_t n1, uint32_t *out1)
 {
+    const uint32_t *const in = blockIdx.x ? in1 : in0;
+    const uint32_t n = blockIdx.x ? n1 : n0;
+    uint32_t *const out = blockIdx.x ? out1 : out0;
     constexpr uint32_t T = kScanOneThreads, U = kScanOneMax / kScanOneThreads, W = 64u;
     __shared__ uint32_t v[kScanOneLds];
     __shared__ uint32_t wsum[T / W];
@@ -1068,6 +1082,12 @@ void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n)
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
     SLOSH_HIP(hipGetLastError());
 }
+void launch_zero3_u32(hipStream_t st, uint32_t *a, uint32_t *b, uint32_t *c, size_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_zero3_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, a, b, c, n);
+    SLOSH_HIP(hipGetLastError());
+}
 void launch_fill_u8(hipStream_t st, uint8_t *p, uint8_t v, size_t n)
 {
     if (!n) return;
@@ -1085,7 +1105,7 @@ void launch_fill_f64(hipStream_t st, double *p, double v, size_t n)
 void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n)
 {
     if (n <= kScanOneMax) {
-        launch_timed(c, K_SCAN, k_scan_one, dim3(1), dim3(kScanOneThreads), 0, in, n, out);
+        launch_timed(c, K_SCAN, k_scan_one, dim3(1), dim3(kScanOneThreads), 0, in, n, out, in, n, out);
         SLOSH_HIP(hipGetLastError());
         return;
     }
@@ -1102,6 +1122,18 @@ void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n)
     SLOSH_HIP(hipGetLastError());
 }
 
+void scan_exclusive2(slosh_ctx *c, const uint32_t *in0, uint32_t *out0, uint32_t n0, const uint32_t *in1, uint32_t *out1,
+                     uint32_t n1)
+{
+    if (n0 <= kScanOneMax && n1 <= kScanOneMax) {
+        launch_timed(c, K_SCAN, k_scan_one, dim3(2), dim3(kScanOneThreads), 0, in0, n0, out0, in1, n1, out1);
+        SLOSH_HIP(hipGetLastError());
+        return;
+    }
+    scan_exclusive(c, in0, out0, n0);
+    scan_exclusive(c, in1, out1, n1);
+}
+
 void run_pool(slosh_ctx *c)
 {
     const Geo &g = c->geo;
@@ -1115,8 +1147,7 @@ void run_pool(slosh_ctx *c)
                      c->freeflag);
     }
     SLOSH_HIP(hipGetLastError());
-    scan_exclusive(c, c->need, c->need_rank, g.nbricks);
-    scan_exclusive(c, c->freeflag, c->free_rank, g.npool);
+    scan_exclusive2(c, c->need, c->need_rank, g.nbricks, c->freeflag, c->free_rank, g.npool);
     {
         launch_timed(c, K_POOL, k_pool_free_list, dim3(blocks_for(g.npool, 256)), dim3(256), 0, c->freeflag,
                      c->free_rank, g.npool, c->free_list);

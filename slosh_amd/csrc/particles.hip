@@ -1023,9 +1023,7 @@ static void sort_keys(slosh_ctx *c, const uint32_t *keys)
 {
     const uint32_t np = c->np;
     const uint32_t dead = c->nkeys - 1u;
-    launch_fill_u32(c->stream, c->key_count, 0u, c->nkeys);
-    launch_fill_u32(c->stream, c->key_runs, 0u, c->nkeys);
-    launch_fill_u32(c->stream, c->run_cursor, 0u, c->nkeys);
+    launch_zero3_u32(c->stream, c->key_count, c->key_runs, c->run_cursor, c->nkeys);
     if (np == 0) {
         scan_exclusive(c, c->key_count, c->key_off_next, c->nkeys);
         return;
@@ -1034,8 +1032,7 @@ static void sort_keys(slosh_ctx *c, const uint32_t *keys)
     launch_timed(c, K_SORT_TILES, k_sort_tiles, dim3(ntiles), dim3(kTile), 0, c->geo, c->parts[c->cur].pa, keys, np,
                  dead, c->packed, c->tile_nruns, c->run_key, c->run_len, c->key_count, c->key_runs);
     SLOSH_HIP(hipGetLastError());
-    scan_exclusive(c, c->key_count, c->key_off_next, c->nkeys);
-    scan_exclusive(c, c->key_runs, c->run_off, c->nkeys);
+    scan_exclusive2(c, c->key_count, c->key_off_next, c->nkeys, c->key_runs, c->run_off, c->nkeys);
     launch_timed(c, K_SORT_RUNS, k_sort_runs_scatter, dim3(ntiles), dim3(kTile), 0, c->tile_nruns, c->run_key,
                  c->run_off, c->run_cursor, c->run_sorted);
     SLOSH_HIP(hipGetLastError());

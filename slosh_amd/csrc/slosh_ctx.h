@@ -464,7 +464,11 @@ void launch_submerged(slosh_ctx *c);
 void run_viscosity(slosh_ctx *c, float alpha, int iterations);
 void launch_density_disp(slosh_ctx *c);
 void scan_exclusive(slosh_ctx *c, const uint32_t *in, uint32_t *out, uint32_t n);
+// Two independent exclusive scans (one launch when both fit one workgroup).
+void scan_exclusive2(slosh_ctx *c, const uint32_t *in0, uint32_t *out0, uint32_t n0, const uint32_t *in1, uint32_t *out1,
+                     uint32_t n1);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
+void launch_zero3_u32(hipStream_t st, uint32_t *a, uint32_t *b, uint32_t *c, size_t n); // three arrays of n, one launch
 void launch_fill_u8(hipStream_t st, uint8_t *p, uint8_t v, size_t n);
 void launch_fill_f64(hipStream_t st, double *p, double v, size_t n);
 
