@@ -475,6 +475,18 @@ def main():
             ach = conv_flops[fdom] / favg / 1e12
             fl.update({"bound": "mfma", "achieved": round(ach, 1), "peak": 2500.0, "unit": "TFLOP/s",
                        "frac": round(ach / 2500.0, 4), "algorithmic_flops": int(conv_flops[fdom])})
+        # every level's convolution from the instrumented step; level 0 also against HBM (each launch reads 16 BF16
+        # channels and writes 16 per cell, 512 cells per brick)
+        fl["convs"] = {}
+        for k, flops in conv_flops.items():
+            cl, cms = fprof.get(k, (0, 0.0))
+            if cl and cms > 0:
+                cavg = cms / 1e3 / cl
+                fl["convs"][k] = {"launches": cl, "avg_us": round(cavg * 1e6, 2),
+                                  "mfma_frac": round(flops / cavg / 1e12 / 2500.0, 4)}
+                if k == "k_unet_conv_l0":
+                    cbytes = nb * 512 * (16 + 16) * 2
+                    fl["convs"][k]["hbm_frac"] = round(cbytes / cavg / 1e9 / HBM_PEAK_GBS, 4)
         # the learned preconditioner's evaluation scene: the dam break at 256^3
         c4 = learned_on_dam_break_256(unet.load(wpath), mg)
         learned = {"steps_per_s": round(fsps, 3), "speedup_vs_mgpcg": round(fsps / sps, 3),
@@ -527,9 +539,12 @@ def main():
                                  "model": "328 Np + 258 Nc + 108 Nf + k (136 Nf + 8 Nc): Np particles, Nc "
                                           "allocated cells, Nf fluid cells, k mean iterations per step"}
     if args.kernel_report:
-        for k, (l, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
-            if l:
-                print(f"{k:22s} {l:6d} launches {ms:9.3f} ms", file=sys.stderr)
+        for name, table in (("mgpcg", prof), ("learned fcg", fprof if learned is not None else {})):
+            if table:
+                print(f"-- {name}: one instrumented step", file=sys.stderr)
+            for k, (l, ms) in sorted(table.items(), key=lambda kv: -kv[1][1]):
+                if l:
+                    print(f"{k:22s} {l:6d} launches {ms:9.3f} ms", file=sys.stderr)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(line), flush=True)

@@ -17,7 +17,8 @@
 // of 8 bricks (level 2: 64), N = output channels, K = 27 taps x input channels. The input block of the workgroup's
 // bricks plus a one-cell halo is staged in LDS from the brick's 3x3x3 neighbourhood; A fragments (16 cells x 16
 // channels) come from LDS, B fragments (16 input x 16 output channels of one tap) from the L2-resident weights,
-// and v_mfma_f32_16x16x16_bf16 accumulates in FP32. The backward pass w.r.t. the input is the same kernel with the
+// and v_mfma_f32_16x16x16_bf16 accumulates in FP32 (8^3 bricks: v_mfma_f32_16x16x32_bf16 over pairs of taps, B
+// fragments held in registers, A fragments reused across the three z taps). The backward pass w.r.t. the input is the same kernel with the
 // weights mirrored in the tap and transposed in the channels; weight gradients are FP32 reductions.
 #include <math.h>
 
@@ -31,6 +32,7 @@ namespace {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -51,6 +53,25 @@ __host__ __device__ constexpr int group_bricks(int S) { return S == 2 ? 16 : (S 
 // fragment fall into distinct LDS banks.
 __host__ __device__ constexpr int lds_stride(int C) { return C + 4; }
 
+// Group-relative output cell of row r (0..15) of M tile t. In 8^3 bricks a tile holds x = 0..7 of rows y and y + 4
+// of one z plane: the halves lie 40 staged cells apart, so that for every tap the rows' 8-byte A reads fall into
+// distinct LDS bank pairs (with rows y and y + 1, 10 cells apart, cells c and c + 16 of a fragment share a pair;
+// measured 8% slower).
+template <int S> __device__ __forceinline__ uint32_t tile_vox(uint32_t t, uint32_t r)
+{
+    if constexpr (S == 8) return (r & 7u) + 8u * ((t & 3u) + 4u * (r >> 3)) + 64u * (t >> 2);
+    return t * 16u + r;
+}
+
+// M tile of a wavefront's mt-th fragment row. In 8^3 bricks wavefront w takes rows y = w and w + 4 of every z plane
+// (tile 4 mt + w holds plane mt), so that one A fragment read at box plane b serves the three output planes b - 2,
+// b - 1 and b of the taps dz = 2, 1, 0: 10 reads per (dx, dy) instead of 24.
+template <int S, int MTW> __device__ __forceinline__ uint32_t wave_tile(uint32_t wave, int mt)
+{
+    if constexpr (S == 8) return (uint32_t)mt * 4u + wave;
+    return wave * MTW + (uint32_t)mt;
+}
+
 // Pool index (cell) of block-relative coordinates (u, v, w) in [-1, S + 1)^3 through the 27-neighbourhood slots.
 template <int S> __device__ __forceinline__ uint32_t nb_cell(const uint32_t *nb27, int u, int v, int w)
 {
@@ -60,10 +81,11 @@ template <int S> __device__ __forceinline__ uint32_t nb_cell(const uint32_t *nb2
 }
 
 // ---- convolution (forward, and backward w.r.t. the input with mirrored/transposed weights) -----------------
+// 8^3 bricks: at most 128 VGPRs, so that the 4 workgroups per CU that 40 KB of LDS allows are resident together.
 template <int CIN, int COUT, int S, bool RELU, bool BIAS>
-__global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nbricks, const uint16_t *in,
-                                              const uint16_t *w, const float *bias, uint16_t *out,
-                                              const int32_t *skip)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 : 1))) void k_conv(
+    const uint32_t *binfo, uint32_t nbricks, const uint16_t *in, const uint16_t *w, const float *bias, uint16_t *out,
+    const int32_t *skip)
 {
     if (*skip) return;
     constexpr int BG = group_bricks(S), SB = S + 2, BOX = SB * SB * SB, CPB = S * S * S, M = BG * CPB;
@@ -73,6 +95,25 @@ __global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nb
     __shared__ uint32_t s_nb[BG][28];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t grp = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t row = lane & 15u, quad = lane >> 4;
+    // 8^3 bricks (CIN = COUT = 16): K = 32 blocks pair the taps (dx, dy) = 2p and 2p + 1 of one dz (quads 0-1 the
+    // first, 2-3 the second; the second of p = 4 is zero). A lane's 15 B fragments fit in 60 VGPRs, loaded once,
+    // before the box, so that their latency overlaps the staging instead of stalling each tap.
+    constexpr bool kWReg = S == 8;
+    static_assert(!kWReg || (KB == 1 && NT == 1 && MTW == 8 && BG == 1), "8^3 bricks: 16 channels in and out");
+    v8bf wreg[kWReg ? 15 : 1];
+    if constexpr (kWReg) {
+        const uint32_t hi = quad >> 1, c0 = (quad & 1u) * 8u;
+#pragma unroll
+        for (int p = 0; p < 5; ++p)
+#pragma unroll
+            for (int dz = 0; dz < 3; ++dz) {
+                const uint32_t dxy = 2u * (uint32_t)p + hi;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (dxy < 9u) v = *(const uint4 *)(w + ((size_t)((dxy + 9u * (uint32_t)dz) * COUT + row) * CIN + c0));
+                wreg[p * 3 + dz] = __builtin_bit_cast(v8bf, v);
+            }
+    }
     for (uint32_t k = tid; k < (uint32_t)(BG * 28); k += 256u) {
         const uint32_t bg = k / 28u, e = grp * BG + bg;
         s_nb[bg][k % 28u] = e < nbricks ? binfo[(size_t)e * kBinfoWords + k % 28u] : 0u;
@@ -103,26 +144,49 @@ __global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nb
         }
     }
     __syncthreads();
-    const uint32_t row = lane & 15u, quad = lane >> 4;
     v4f acc[MTW][NT];
     for (int mt = 0; mt < MTW; ++mt)
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t abase[MTW]; // LDS element offset of this lane's A row at tap (0,0,0) of the box
     for (int mt = 0; mt < MTW; ++mt) {
-        const uint32_t vox = (wave * MTW + mt) * 16u + row, bg = vox / CPB, l = vox % CPB;
+        const uint32_t vox = tile_vox<S>(wave_tile<S, MTW>(wave, mt), row), bg = vox / CPB, l = vox % CPB;
         const uint32_t lx = l % S, ly = (l / S) % S, lz = l / (S * S);
         abase[mt] = (bg * BOX + lx + SB * (ly + SB * lz)) * LS + quad * 4u;
     }
-    for (int tap = 0; tap < 27; ++tap) {
-        const uint32_t toff = (uint32_t)((tap % 3) + SB * (((tap / 3) % 3) + SB * (tap / 9))) * LS;
-        for (int kb = 0; kb < KB; ++kb) {
-            v4s bfr[NT];
-            for (int nt = 0; nt < NT; ++nt)
-                bfr[nt] = *(const v4s *)(w + ((size_t)(tap * COUT + nt * 16 + (int)row) * CIN + kb * 16 + quad * 4));
-            for (int mt = 0; mt < MTW; ++mt) {
-                const v4s a = *(const v4s *)(s_in + abase[mt] + toff + kb * 16);
+    if constexpr (kWReg) {
+        // A rows: x = row & 7, y = wave + 4 (row >> 3); fragment mt is output plane z = mt, and the box plane b (cells
+        // at z = b - 1) feeds plane b - dz through tap dz. Lanes of the zero tap read the last tap's cells.
+        const uint32_t hi = quad >> 1;
+        const uint32_t rowbase = ((row & 7u) + SB * (wave + 4u * (row >> 3))) * LS + (quad & 1u) * 8u;
+#pragma unroll
+        for (int p = 0; p < 5; ++p) {
+            const uint32_t dxy = 2u * (uint32_t)p + hi < 8u ? 2u * (uint32_t)p + hi : 8u;
+            const uint32_t poff = rowbase + ((dxy % 3u) + SB * (dxy / 3u)) * LS;
+#pragma unroll
+            for (int b = 0; b < SB; ++b) {
+                const uint2 lo = *(const uint2 *)(s_in + poff + (uint32_t)(SB * SB * LS * b));
+                const uint2 up = *(const uint2 *)(s_in + poff + (uint32_t)(SB * SB * LS * b) + 4u);
+                const v8bf a = __builtin_bit_cast(v8bf, make_uint4(lo.x, lo.y, up.x, up.y));
+#pragma unroll
+                for (int dz = 0; dz < 3; ++dz)
+                    if (b - dz >= 0 && b - dz < MTW)
+                        acc[b - dz][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wreg[p * 3 + dz], acc[b - dz][0],
+                                                                                 0, 0, 0);
+            }
+        }
+    } else {
+        for (int tap = 0; tap < 27; ++tap) {
+            const uint32_t toff = (uint32_t)((tap % 3) + SB * (((tap / 3) % 3) + SB * (tap / 9))) * LS;
+            for (int kb = 0; kb < KB; ++kb) {
+                v4s bfr[NT];
                 for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfr[nt], acc[mt][nt], 0, 0, 0);
+                    bfr[nt] =
+                        *(const v4s *)(w + ((size_t)(tap * COUT + nt * 16 + (int)row) * CIN + kb * 16 + quad * 4));
+                for (int mt = 0; mt < MTW; ++mt) {
+                    const v4s a = *(const v4s *)(s_in + abase[mt] + toff + kb * 16);
+                    for (int nt = 0; nt < NT; ++nt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfr[nt], acc[mt][nt], 0, 0, 0);
+                }
             }
         }
     }
@@ -134,7 +198,7 @@ __global__ __launch_bounds__(256) void k_conv(const uint32_t *binfo, uint32_t nb
             const uint32_t co = (uint32_t)nt * 16u + row;
             const float bv = BIAS ? bias[co] : 0.0f;
             for (int j = 0; j < 4; ++j) {
-                const uint32_t vox = (wave * MTW + mt) * 16u + quad * 4u + (uint32_t)j;
+                const uint32_t vox = tile_vox<S>(wave_tile<S, MTW>(wave, mt), quad * 4u + (uint32_t)j);
                 float val = acc[mt][nt][j] + bv;
                 if (RELU) val = fmaxf(val, 0.0f);
                 s_in[vox * COUT + co] = f2bf(val);
