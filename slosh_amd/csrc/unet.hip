@@ -63,13 +63,13 @@ template <int S> __device__ __forceinline__ uint32_t tile_vox(uint32_t t, uint32
     return t * 16u + r;
 }
 
-// M tile of a wavefront's mt-th fragment row. In 8^3 bricks wavefront w takes rows y = w and w + 4 of every z plane
-// (tile 4 mt + w holds plane mt), so that one A fragment read at box plane b serves the three output planes b - 2,
-// b - 1 and b of the taps dz = 2, 1, 0: 10 reads per (dx, dy) instead of 24.
-template <int S, int MTW> __device__ __forceinline__ uint32_t wave_tile(uint32_t wave, int mt)
+// M tile of the i-th fragment of M group mw (of MTG tiles). In 8^3 bricks group w takes rows y = w and w + 4 of
+// every z plane (tile 4 i + w holds plane i), so that one A fragment read at box plane b serves the three output
+// planes b - 2, b - 1 and b of the taps dz = 2, 1, 0: 10 reads per (dx, dy) instead of 24.
+template <int S, int MTG> __device__ __forceinline__ uint32_t wave_tile(uint32_t mw, int i)
 {
-    if constexpr (S == 8) return (uint32_t)mt * 4u + wave;
-    return wave * MTW + (uint32_t)mt;
+    if constexpr (S == 8) return (uint32_t)i * 4u + mw;
+    return mw * MTG + (uint32_t)i;
 }
 
 // Pool index (cell) of block-relative coordinates (u, v, w) in [-1, S + 1)^3 through the 27-neighbourhood slots.
@@ -81,7 +81,9 @@ template <int S> __device__ __forceinline__ uint32_t nb_cell(const uint32_t *nb2
 }
 
 // ---- convolution (forward, and backward w.r.t. the input with mirrored/transposed weights) -----------------
-// 8^3 bricks: at most 128 VGPRs, so that the 4 workgroups per CU that 40 KB of LDS allows are resident together.
+// 8^3 bricks: at most 128 VGPRs, so that the 4 workgroups per CU that 40 KB of LDS allows are resident together
+// (the coarser levels' VGPRs already fit the workgroups their LDS allows, except 16 -> 32 channels at 4^3, which
+// would spill under the cap).
 template <int CIN, int COUT, int S, bool RELU, bool BIAS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 : 1))) void k_conv(
     const uint32_t *binfo, uint32_t nbricks, const uint16_t *in, const uint16_t *w, const float *bias, uint16_t *out,
@@ -89,20 +91,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
 {
     if (*skip) return;
     constexpr int BG = group_bricks(S), SB = S + 2, BOX = SB * SB * SB, CPB = S * S * S, M = BG * CPB;
-    constexpr int MT = M / 16, NT = COUT / 16, KB = CIN / 16, MTW = MT / 4, CH8 = CIN / 8, LS = lds_stride(CIN);
+    constexpr int MT = M / 16, NT = COUT / 16, KB = CIN / 16, CH8 = CIN / 8, LS = lds_stride(CIN);
     static_assert(MT % 4 == 0 && COUT % 16 == 0 && CIN % 16 == 0, "conv tiling");
     __shared__ __attribute__((aligned(16))) uint16_t s_in[BG * BOX * LS];
     __shared__ uint32_t s_nb[BG][28];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t grp = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t row = lane & 15u, quad = lane >> 4;
-    // 8^3 bricks (CIN = COUT = 16): K = 32 blocks pair the taps (dx, dy) = 2p and 2p + 1 of one dz (quads 0-1 the
-    // first, 2-3 the second; the second of p = 4 is zero). A lane's 15 B fragments fit in 60 VGPRs, loaded once,
-    // before the box, so that their latency overlaps the staging instead of stalling each tap.
-    constexpr bool kWReg = S == 8;
-    static_assert(!kWReg || (KB == 1 && NT == 1 && MTW == 8 && BG == 1), "8^3 bricks: 16 channels in and out");
-    v8bf wreg[kWReg ? 15 : 1];
-    if constexpr (kWReg) {
+    // Wavefront w takes N tile w % NT and every (NT / 4)-th group of M tiles, so that its B fragments of every tap
+    // stay in registers, loaded once before the box: their latency overlaps the staging instead of stalling each
+    // tap. 8^3 bricks (CIN = COUT = 16): K = 32 blocks pair the taps (dx, dy) = 2p and 2p + 1 of one dz (quads 0-1
+    // the first, 2-3 the second; the second of p = 4 is zero), 15 fragments in 60 VGPRs. Other levels: one K = 16
+    // fragment per tap and 16 input channels (at most 108 in 216 VGPRs).
+    constexpr bool kS8 = S == 8;
+    constexpr int WM = 4 / NT, MTG = MT / WM;
+    static_assert(NT == 1 || NT == 2 || NT == 4, "one N tile per wavefront");
+    static_assert(!kS8 || (KB == 1 && NT == 1 && MTG == 8 && BG == 1), "8^3 bricks: 16 channels in and out");
+    const uint32_t nw = wave % (uint32_t)NT, mw = wave / (uint32_t)NT;
+    v8bf wpair[kS8 ? 15 : 1];
+    v4s wtap[kS8 ? 1 : 27 * KB];
+    if constexpr (kS8) {
         const uint32_t hi = quad >> 1, c0 = (quad & 1u) * 8u;
 #pragma unroll
         for (int p = 0; p < 5; ++p)
@@ -111,8 +119,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
                 const uint32_t dxy = 2u * (uint32_t)p + hi;
                 uint4 v = make_uint4(0u, 0u, 0u, 0u);
                 if (dxy < 9u) v = *(const uint4 *)(w + ((size_t)((dxy + 9u * (uint32_t)dz) * COUT + row) * CIN + c0));
-                wreg[p * 3 + dz] = __builtin_bit_cast(v8bf, v);
+                wpair[p * 3 + dz] = __builtin_bit_cast(v8bf, v);
             }
+    } else {
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap)
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+                wtap[tap * KB + kb] =
+                    *(const v4s *)(w + ((size_t)(tap * COUT + nw * 16u + row) * CIN + kb * 16 + quad * 4u));
     }
     for (uint32_t k = tid; k < (uint32_t)(BG * 28); k += 256u) {
         const uint32_t bg = k / 28u, e = grp * BG + bg;
@@ -144,20 +159,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
         }
     }
     __syncthreads();
-    v4f acc[MTW][NT];
-    for (int mt = 0; mt < MTW; ++mt)
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    uint32_t abase[MTW]; // LDS element offset of this lane's A row at tap (0,0,0) of the box
-    for (int mt = 0; mt < MTW; ++mt) {
-        const uint32_t vox = tile_vox<S>(wave_tile<S, MTW>(wave, mt), row), bg = vox / CPB, l = vox % CPB;
-        const uint32_t lx = l % S, ly = (l / S) % S, lz = l / (S * S);
-        abase[mt] = (bg * BOX + lx + SB * (ly + SB * lz)) * LS + quad * 4u;
-    }
-    if constexpr (kWReg) {
-        // A rows: x = row & 7, y = wave + 4 (row >> 3); fragment mt is output plane z = mt, and the box plane b (cells
+    v4f acc[MTG];
+    for (int i = 0; i < MTG; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (kS8) {
+        // A rows: x = row & 7, y = mw + 4 (row >> 3); fragment i is output plane z = i, and the box plane b (cells
         // at z = b - 1) feeds plane b - dz through tap dz. Lanes of the zero tap read the last tap's cells.
         const uint32_t hi = quad >> 1;
-        const uint32_t rowbase = ((row & 7u) + SB * (wave + 4u * (row >> 3))) * LS + (quad & 1u) * 8u;
+        const uint32_t rowbase = ((row & 7u) + SB * (mw + 4u * (row >> 3))) * LS + (quad & 1u) * 8u;
 #pragma unroll
         for (int p = 0; p < 5; ++p) {
             const uint32_t dxy = 2u * (uint32_t)p + hi < 8u ? 2u * (uint32_t)p + hi : 8u;
@@ -169,41 +177,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
                 const v8bf a = __builtin_bit_cast(v8bf, make_uint4(lo.x, lo.y, up.x, up.y));
 #pragma unroll
                 for (int dz = 0; dz < 3; ++dz)
-                    if (b - dz >= 0 && b - dz < MTW)
-                        acc[b - dz][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wreg[p * 3 + dz], acc[b - dz][0],
-                                                                                 0, 0, 0);
+                    if (b - dz >= 0 && b - dz < MTG)
+                        acc[b - dz] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wpair[p * 3 + dz], acc[b - dz], 0, 0, 0);
             }
         }
     } else {
+        uint32_t abase[MTG]; // LDS element offset of this lane's A row at tap (0,0,0) of the box
+        for (int i = 0; i < MTG; ++i) {
+            const uint32_t vox = tile_vox<S>(wave_tile<S, MTG>(mw, i), row), bg = vox / CPB, l = vox % CPB;
+            const uint32_t lx = l % S, ly = (l / S) % S, lz = l / (S * S);
+            abase[i] = (bg * BOX + lx + SB * (ly + SB * lz)) * LS + quad * 4u;
+        }
+#pragma unroll
         for (int tap = 0; tap < 27; ++tap) {
             const uint32_t toff = (uint32_t)((tap % 3) + SB * (((tap / 3) % 3) + SB * (tap / 9))) * LS;
-            for (int kb = 0; kb < KB; ++kb) {
-                v4s bfr[NT];
-                for (int nt = 0; nt < NT; ++nt)
-                    bfr[nt] =
-                        *(const v4s *)(w + ((size_t)(tap * COUT + nt * 16 + (int)row) * CIN + kb * 16 + quad * 4));
-                for (int mt = 0; mt < MTW; ++mt) {
-                    const v4s a = *(const v4s *)(s_in + abase[mt] + toff + kb * 16);
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfr[nt], acc[mt][nt], 0, 0, 0);
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int i = 0; i < MTG; ++i) {
+                    const v4s a = *(const v4s *)(s_in + abase[i] + toff + kb * 16);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wtap[tap * KB + kb], acc[i], 0, 0, 0);
                 }
-            }
         }
     }
     // the output tile goes through LDS (the staged input is no longer read), then to memory in 16-byte chunks
     static_assert(M * COUT <= BG * BOX * LS, "output tile fits the input's LDS");
     __syncthreads();
-    for (int mt = 0; mt < MTW; ++mt)
-        for (int nt = 0; nt < NT; ++nt) {
-            const uint32_t co = (uint32_t)nt * 16u + row;
-            const float bv = BIAS ? bias[co] : 0.0f;
+    {
+        const uint32_t co = nw * 16u + row;
+        const float bv = BIAS ? bias[co] : 0.0f;
+        for (int i = 0; i < MTG; ++i)
             for (int j = 0; j < 4; ++j) {
-                const uint32_t vox = tile_vox<S>(wave_tile<S, MTW>(wave, mt), quad * 4u + (uint32_t)j);
-                float val = acc[mt][nt][j] + bv;
+                const uint32_t vox = tile_vox<S>(wave_tile<S, MTG>(mw, i), quad * 4u + (uint32_t)j);
+                float val = acc[i][j] + bv;
                 if (RELU) val = fmaxf(val, 0.0f);
                 s_in[vox * COUT + co] = f2bf(val);
             }
-        }
+    }
     __syncthreads();
     constexpr int CO8 = COUT / 8;
     for (uint32_t k = tid; k < (uint32_t)(M * CO8); k += 256u) {
