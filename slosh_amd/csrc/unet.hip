@@ -48,6 +48,9 @@ constexpr uint32_t kSelfFluid = 1u << 6;
 // Bricks per workgroup of a pass at a level with S cells per brick axis: 512, 256 and 128 output cells, so that a
 // workgroup's weight fragments feed at least 2 M-tiles per wavefront at the coarse levels.
 __host__ __device__ constexpr int group_bricks(int S) { return S == 2 ? 16 : (S == 4 ? 4 : 1); }
+// The forward/input-gradient convolution groups 8 bricks at 2^3 (64 output cells): 64 channels then stage 71 KB of
+// LDS, two workgroups per CU instead of one (16 bricks: 141 KB).
+__host__ __device__ constexpr int conv_group_bricks(int S) { return S == 2 ? 8 : group_bricks(S); }
 
 // LDS row stride (in BF16 elements) of one staged cell: the channels plus 4 of padding, so that the 16 rows of an A
 // fragment fall into distinct LDS banks.
@@ -81,16 +84,17 @@ template <int S> __device__ __forceinline__ uint32_t nb_cell(const uint32_t *nb2
 }
 
 // ---- convolution (forward, and backward w.r.t. the input with mirrored/transposed weights) -----------------
-// 8^3 bricks: at most 128 VGPRs, so that the 4 workgroups per CU that 40 KB of LDS allows are resident together
-// (the coarser levels' VGPRs already fit the workgroups their LDS allows, except 16 -> 32 channels at 4^3, which
-// would spill under the cap).
+// 8^3 bricks: at most 128 VGPRs, so that the 4 workgroups per CU that 40 KB of LDS allows are resident together;
+// 2^3 bricks: at most 256, for the 2 that 71 KB allow (64 input channels then spill about 20 registers: measured
+// 48.7 us per launch against 52.1 for one workgroup of 16 bricks per CU). Level 1 is left uncapped (16 -> 32
+// channels would spill under a cap of 128).
 template <int CIN, int COUT, int S, bool RELU, bool BIAS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 : 1))) void k_conv(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 : (S == 2 ? 2 : 1)))) void k_conv(
     const uint32_t *binfo, uint32_t nbricks, const uint16_t *in, const uint16_t *w, const float *bias, uint16_t *out,
     const int32_t *skip)
 {
     if (*skip) return;
-    constexpr int BG = group_bricks(S), SB = S + 2, BOX = SB * SB * SB, CPB = S * S * S, M = BG * CPB;
+    constexpr int BG = conv_group_bricks(S), SB = S + 2, BOX = SB * SB * SB, CPB = S * S * S, M = BG * CPB;
     constexpr int MT = M / 16, NT = COUT / 16, KB = CIN / 16, CH8 = CIN / 8, LS = lds_stride(CIN);
     static_assert(MT % 4 == 0 && COUT % 16 == 0 && CIN % 16 == 0, "conv tiling");
     __shared__ __attribute__((aligned(16))) uint16_t s_in[BG * BOX * LS];
@@ -666,7 +670,7 @@ size_t level_cells(const slosh_ctx *c, int level) { return (size_t)c->geo.npool 
 template <int CIN, int COUT, int S, bool RELU, bool BIAS>
 void conv(slosh_ctx *c, const uint16_t *in, const uint16_t *w, const float *bias, uint16_t *out)
 {
-    constexpr int BG = group_bricks(S);
+    constexpr int BG = conv_group_bricks(S);
     const uint32_t groups = (c->nactive + BG - 1) / BG;
     KTimer timer(c, S == 8 ? K_UNET_CONV0 : (S == 4 ? K_UNET_CONV1 : K_UNET_CONV2));
     hipLaunchKernelGGL((k_conv<CIN, COUT, S, RELU, BIAS>), dim3(groups), dim3(256), 0, c->stream, c->binfo, c->nactive,
