@@ -13,8 +13,8 @@
 // the null slot all zero), weights as BF16 [tap][cout][cin]; the FP32 master weights live in the flat parameter
 // vector (per layer W[27][cout][cin] then bias[cout]).
 //
-// A convolution is an implicit GEMM per workgroup: M = the output cells of one brick (level 0: 512; level 1: 64) or
-// of 8 bricks (level 2: 64), N = output channels, K = 27 taps x input channels. The input block of the workgroup's
+// A convolution is an implicit GEMM per workgroup: M = the output cells of its bricks (level 0: one brick, 512;
+// level 1: four, 256; level 2: eight, 64), N = output channels, K = 27 taps x input channels. The input block of the workgroup's
 // bricks plus a one-cell halo is staged in LDS from the brick's 3x3x3 neighbourhood; A fragments (16 cells x 16
 // channels) come from LDS, B fragments (16 input x 16 output channels of one tap) from the L2-resident weights,
 // and v_mfma_f32_16x16x16_bf16 accumulates in FP32 (8^3 bricks: v_mfma_f32_16x16x32_bf16 over pairs of taps, B
@@ -102,8 +102,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t grp = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t row = lane & 15u, quad = lane >> 4;
-    // Wavefront w takes N tile w % NT and every (NT / 4)-th group of M tiles, so that its B fragments of every tap
-    // stay in registers, loaded once before the box: their latency overlaps the staging instead of stalling each
+    // Wavefront w takes N tile w % NT and M group w / NT (the M tiles split into 4 / NT groups), so that its B
+    // fragments of every tap stay in registers, loaded once before the box: their latency overlaps the staging instead of stalling each
     // tap. 8^3 bricks (CIN = COUT = 16): K = 32 blocks pair the taps (dx, dy) = 2p and 2p + 1 of one dz (quads 0-1
     // the first, 2-3 the second; the second of p = 4 is zero), 15 fragments in 60 VGPRs. Other levels: one K = 16
     // fragment per tap and 16 input channels (at most 108 in 216 VGPRs).
