@@ -185,6 +185,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
                         acc[b - dz] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wpair[p * 3 + dz], acc[b - dz], 0, 0, 0);
             }
         }
+    } else if constexpr (S == 4) {
+        // 4^3 bricks: M tile 4 j + z of a wavefront is plane z of its j-th brick (rows x = row & 3, y = row >> 2), so
+        // one A read at box plane b serves planes b, b - 1 and b - 2 through taps dz = 0, 1, 2 (6 reads per (dx, dy)
+        // and K block instead of 12)
+        static_assert(MTG % 4 == 0, "whole bricks per wavefront");
+#pragma unroll
+        for (int j = 0; j < MTG / 4; ++j) {
+            const uint32_t vox = tile_vox<S>(wave_tile<S, MTG>(mw, 4 * j), row), bg = vox / CPB, l = vox % CPB;
+            const uint32_t rowbase = (bg * BOX + l % S + SB * ((l / S) % S)) * LS + quad * 4u;
+#pragma unroll
+            for (int dxy = 0; dxy < 9; ++dxy)
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                    for (int b = 0; b < SB; ++b) {
+                        const v4s a = *(const v4s *)(s_in + rowbase +
+                                                     (uint32_t)(((dxy % 3) + SB * ((dxy / 3) + SB * b)) * LS + kb * 16));
+#pragma unroll
+                        for (int dz = 0; dz < 3; ++dz)
+                            if (b - dz >= 0 && b - dz < 4)
+                                acc[4 * j + b - dz] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                                    a, wtap[(dxy + 9 * dz) * KB + kb], acc[4 * j + b - dz], 0, 0, 0);
+                    }
+        }
     } else {
         uint32_t abase[MTG]; // LDS element offset of this lane's A row at tap (0,0,0) of the box
         for (int i = 0; i < MTG; ++i) {
