@@ -60,9 +60,14 @@ __host__ __device__ constexpr int lds_stride(int C) { return C + 4; }
 // of one z plane: the halves lie 40 staged cells apart, so that for every tap the rows' 8-byte A reads fall into
 // distinct LDS bank pairs (with rows y and y + 1, 10 cells apart, cells c and c + 16 of a fragment share a pair;
 // measured 8% slower).
+// In 2^3 bricks (8 per workgroup) tile t holds plane z = t & 1 of bricks 2 (t >> 1) + {0, 1, 4, 5}, 2x2 cells each:
+// with the staged boxes 4 dwords apart modulo 32 banks, those four bricks' rows fall into distinct bank pairs, and
+// the tiles of one brick set come in z pairs for the reuse of A reads across the z taps. In 4^3 bricks tile t is
+// plane t & 3 of brick t >> 2.
 template <int S> __device__ __forceinline__ uint32_t tile_vox(uint32_t t, uint32_t r)
 {
     if constexpr (S == 8) return (r & 7u) + 8u * ((t & 3u) + 4u * (r >> 3)) + 64u * (t >> 2);
+    if constexpr (S == 2) return (2u * (t >> 1) + ((r >> 2) & 1u) + 4u * (r >> 3)) * 8u + (r & 3u) + 4u * (t & 1u);
     return t * 16u + r;
 }
 
@@ -96,8 +101,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
     if (*skip) return;
     constexpr int BG = conv_group_bricks(S), SB = S + 2, BOX = SB * SB * SB, CPB = S * S * S, M = BG * CPB;
     constexpr int MT = M / 16, NT = COUT / 16, KB = CIN / 16, CH8 = CIN / 8, LS = lds_stride(CIN);
+    // LDS elements per staged box: 2^3 bricks pad 8 (4 dwords), so that consecutive boxes start 4 banks apart
+    constexpr int BSTR = BOX * LS + (S == 2 ? 8 : 0);
     static_assert(MT % 4 == 0 && COUT % 16 == 0 && CIN % 16 == 0, "conv tiling");
-    __shared__ __attribute__((aligned(16))) uint16_t s_in[BG * BOX * LS];
+    static_assert(S != 2 || (BG == 8 && (BOX * LS / 2) % 32 == 0), "2^3 tiles: 8 bricks, boxes 4 banks apart");
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[BG * BSTR];
     __shared__ uint32_t s_nb[BG][28];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t grp = xcd_remap(blockIdx.x, gridDim.x);
@@ -156,8 +164,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
         for (uint32_t q = 0; q < 8u; ++q) {
             const uint32_t k = k0 + tid + 256u * q;
             if (k >= kChunks) continue;
-            const uint32_t c8 = k % CH8, qq = k / CH8;
-            uint2 *dst = (uint2 *)(s_in + ((size_t)qq * LS + c8 * 8));
+            const uint32_t c8 = k % CH8, qq = k / CH8, bg = qq / BOX, bi = qq % BOX;
+            uint2 *dst = (uint2 *)(s_in + ((size_t)bg * BSTR + bi * LS + c8 * 8));
             dst[0] = make_uint2(sv[q].x, sv[q].y);
             dst[1] = make_uint2(sv[q].z, sv[q].w);
         }
@@ -185,15 +193,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
                         acc[b - dz] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wpair[p * 3 + dz], acc[b - dz], 0, 0, 0);
             }
         }
-    } else if constexpr (S == 4) {
-        // 4^3 bricks: M tile 4 j + z of a wavefront is plane z of its j-th brick (rows x = row & 3, y = row >> 2), so
-        // one A read at box plane b serves planes b, b - 1 and b - 2 through taps dz = 0, 1, 2 (6 reads per (dx, dy)
-        // and K block instead of 12)
-        static_assert(MTG % 4 == 0, "whole bricks per wavefront");
+    } else {
+        // 4^3 and 2^3 bricks: a wavefront's M tiles come in sets of S planes of the same bricks (tile S j + z: plane
+        // z), so one A read at box plane b serves planes b, b - 1 and b - 2 through taps dz = 0, 1, 2 (S + 2 reads per
+        // (dx, dy) and K block instead of 3 S)
+        static_assert((S == 4 || S == 2) && MTG % S == 0, "whole plane sets per wavefront");
 #pragma unroll
-        for (int j = 0; j < MTG / 4; ++j) {
-            const uint32_t vox = tile_vox<S>(wave_tile<S, MTG>(mw, 4 * j), row), bg = vox / CPB, l = vox % CPB;
-            const uint32_t rowbase = (bg * BOX + l % S + SB * ((l / S) % S)) * LS + quad * 4u;
+        for (int j = 0; j < MTG / S; ++j) {
+            const uint32_t vox = tile_vox<S>(wave_tile<S, MTG>(mw, S * j), row), bg = vox / CPB, l = vox % CPB;
+            const uint32_t rowbase = bg * BSTR + (l % S + SB * ((l / S) % S)) * LS + quad * 4u;
 #pragma unroll
             for (int dxy = 0; dxy < 9; ++dxy)
 #pragma unroll
@@ -204,32 +212,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
                                                      (uint32_t)(((dxy % 3) + SB * ((dxy / 3) + SB * b)) * LS + kb * 16));
 #pragma unroll
                         for (int dz = 0; dz < 3; ++dz)
-                            if (b - dz >= 0 && b - dz < 4)
-                                acc[4 * j + b - dz] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                                    a, wtap[(dxy + 9 * dz) * KB + kb], acc[4 * j + b - dz], 0, 0, 0);
+                            if (b - dz >= 0 && b - dz < S)
+                                acc[S * j + b - dz] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                                    a, wtap[(dxy + 9 * dz) * KB + kb], acc[S * j + b - dz], 0, 0, 0);
                     }
-        }
-    } else {
-        uint32_t abase[MTG]; // LDS element offset of this lane's A row at tap (0,0,0) of the box
-        for (int i = 0; i < MTG; ++i) {
-            const uint32_t vox = tile_vox<S>(wave_tile<S, MTG>(mw, i), row), bg = vox / CPB, l = vox % CPB;
-            const uint32_t lx = l % S, ly = (l / S) % S, lz = l / (S * S);
-            abase[i] = (bg * BOX + lx + SB * (ly + SB * lz)) * LS + quad * 4u;
-        }
-#pragma unroll
-        for (int tap = 0; tap < 27; ++tap) {
-            const uint32_t toff = (uint32_t)((tap % 3) + SB * (((tap / 3) % 3) + SB * (tap / 9))) * LS;
-#pragma unroll
-            for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-                for (int i = 0; i < MTG; ++i) {
-                    const v4s a = *(const v4s *)(s_in + abase[i] + toff + kb * 16);
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wtap[tap * KB + kb], acc[i], 0, 0, 0);
-                }
         }
     }
     // the output tile goes through LDS (the staged input is no longer read), then to memory in 16-byte chunks
-    static_assert(M * COUT <= BG * BOX * LS, "output tile fits the input's LDS");
+    static_assert(M * COUT <= BG * BSTR, "output tile fits the input's LDS");
     __syncthreads();
     {
         const uint32_t co = nw * 16u + row;
