@@ -63,12 +63,13 @@ __host__ __device__ constexpr int lds_stride(int C) { return C + 4; }
 // In 2^3 bricks (8 per workgroup) tile t holds plane z = t & 1 of bricks 2 (t >> 1) + {0, 1, 4, 5}, 2x2 cells each:
 // with the staged boxes 4 dwords apart modulo 32 banks, those four bricks' rows fall into distinct bank pairs, and
 // the tiles of one brick set come in z pairs for the reuse of A reads across the z taps. In 4^3 bricks tile t is
-// plane t & 3 of brick t >> 2.
+// the x-z plane y = t & 3 of brick t >> 2 (rows x = r & 3, z = r >> 2): its rows lie 0..3 and 36..39, ... staged
+// cells apart, distinct bank pairs at 16 and 32 channels (x-y planes would pair rows 2 and 12, 18 cells apart).
 template <int S> __device__ __forceinline__ uint32_t tile_vox(uint32_t t, uint32_t r)
 {
     if constexpr (S == 8) return (r & 7u) + 8u * ((t & 3u) + 4u * (r >> 3)) + 64u * (t >> 2);
     if constexpr (S == 2) return (2u * (t >> 1) + ((r >> 2) & 1u) + 4u * (r >> 3)) * 8u + (r & 3u) + 4u * (t & 1u);
-    return t * 16u + r;
+    return (t >> 2) * 64u + (r & 3u) + 4u * (t & 3u) + 16u * (r >> 2);
 }
 
 // M tile of the i-th fragment of M group mw (of MTG tiles). In 8^3 bricks group w takes rows y = w and w + 4 of
@@ -194,28 +195,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 8 ? 4 
             }
         }
     } else {
-        // 4^3 and 2^3 bricks: a wavefront's M tiles come in sets of S planes of the same bricks (tile S j + z: plane
-        // z), so one A read at box plane b serves planes b, b - 1 and b - 2 through taps dz = 0, 1, 2 (S + 2 reads per
-        // (dx, dy) and K block instead of 3 S)
+        // 4^3 and 2^3 bricks: a wavefront's M tiles come in sets of S planes of the same bricks (tile S j + k: plane
+        // k across axis Q, y at 4^3 and z at 2^3), so one A read at box plane b serves planes b, b - 1 and b - 2
+        // through the taps 0, 1, 2 along Q (S + 2 reads per two other tap offsets and K block instead of 3 S)
         static_assert((S == 4 || S == 2) && MTG % S == 0, "whole plane sets per wavefront");
+        constexpr int Q = S == 4 ? 1 : 2;                          // the reuse axis
+        constexpr int QSTR = Q == 1 ? SB : SB * SB;                // its box stride
 #pragma unroll
         for (int j = 0; j < MTG / S; ++j) {
             const uint32_t vox = tile_vox<S>(wave_tile<S, MTG>(mw, S * j), row), bg = vox / CPB, l = vox % CPB;
-            const uint32_t rowbase = bg * BSTR + (l % S + SB * ((l / S) % S)) * LS + quad * 4u;
+            // the lane's cell with the reuse coordinate at 0
+            const uint32_t cx = l % S, cy = Q == 1 ? 0u : (l / S) % S, cz = Q == 1 ? l / (S * S) : 0u;
+            const uint32_t rowbase = bg * BSTR + (cx + SB * (cy + SB * cz)) * LS + quad * 4u;
 #pragma unroll
-            for (int dxy = 0; dxy < 9; ++dxy)
+            for (int o = 0; o < 9; ++o) {
+                // the two other tap offsets: (dx, dz) at 4^3, (dx, dy) at 2^3
+                const int dx = o % 3, d2 = o / 3;
+                const int ooff = Q == 1 ? dx + SB * SB * d2 : dx + SB * d2;
 #pragma unroll
                 for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
                     for (int b = 0; b < SB; ++b) {
-                        const v4s a = *(const v4s *)(s_in + rowbase +
-                                                     (uint32_t)(((dxy % 3) + SB * ((dxy / 3) + SB * b)) * LS + kb * 16));
+                        const v4s a = *(const v4s *)(s_in + rowbase + (uint32_t)((ooff + QSTR * b) * LS + kb * 16));
 #pragma unroll
-                        for (int dz = 0; dz < 3; ++dz)
-                            if (b - dz >= 0 && b - dz < S)
-                                acc[S * j + b - dz] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                                    a, wtap[(dxy + 9 * dz) * KB + kb], acc[S * j + b - dz], 0, 0, 0);
+                        for (int dq = 0; dq < 3; ++dq)
+                            if (b - dq >= 0 && b - dq < S) {
+                                const int tap = Q == 1 ? dx + 3 * dq + 9 * d2 : dx + 3 * d2 + 9 * dq;
+                                acc[S * j + b - dq] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                                    a, wtap[tap * KB + kb], acc[S * j + b - dq], 0, 0, 0);
+                            }
                     }
+            }
         }
     }
     // the output tile goes through LDS (the staged input is no longer read), then to memory in 16-byte chunks
